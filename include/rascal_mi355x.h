@@ -1,0 +1,143 @@
+/*
+ * C ABI of the MI355X (gfx950) SOAP + sparse-GAP library librascal_mi355x.so.
+ *
+ * Drop-in boundary for librascal's SOAP prediction path.  One context owns
+ * the calculator (hypers, radial tables), the current batch of structures
+ * (neighbour list, expansion, power spectrum) and the sparse GAP model.
+ *
+ * Conventions
+ *   - All functions return RGPU_OK (0) or an RGPU_ERR_* code; the message of
+ *     the last failure on the calling thread is rgpu_last_error().
+ *   - Array arguments are plain pointers; output pointers may be host or
+ *     device memory (copies use hipMemcpyDefault).  Nothing is retained from
+ *     caller memory after a call returns.
+ *   - positions [n_atoms][3]; cells [n_struct][9] with cell[d*3+k] the
+ *     component d of lattice vector k (lattice vectors are columns, as the
+ *     reference AtomicStructure); pbc [n_struct][3].
+ *   - Dense feature layout over the context's sorted species list S:
+ *       expansion  [n_centres][S][N][(L+1)^2]   (n slowest, lm = l*l+l+m)
+ *       invariants [n_centres][K][N*N][L+1]     (K = S(S+1)/2 keys a<=b)
+ *     which is the reference's dense export (property_block_sparse.hh
+ *     :911-932) before dropping keys absent from the whole batch.
+ *   - Every atom is a centre (no centre masks).
+ *   - Calls on one context are not re-entrant (as the reference calculators).
+ */
+#ifndef RASCAL_MI355X_H_
+#define RASCAL_MI355X_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RGPU_OK 0
+#define RGPU_ERR_INVALID 1 /* bad argument or hypers (reference: std::logic_error) */
+#define RGPU_ERR_RUNTIME 2 /* bad structure etc. (reference: std::runtime_error) */
+#define RGPU_ERR_HIP 3     /* HIP runtime failure */
+#define RGPU_ERR_STATE 4   /* call out of order (e.g. no structures set) */
+
+#define RGPU_FP64 0  /* FP64 everywhere (parity path) */
+#define RGPU_MIXED 1 /* FP32 expansion/invariants, BF16 hi+lo GEMM, FP64 reductions */
+
+#define RGPU_INFO_MAX_RADIAL 0
+#define RGPU_INFO_MAX_ANGULAR 1
+#define RGPU_INFO_N_SPECIES 2
+#define RGPU_INFO_FEATURE_SIZE 3 /* D = K N^2 (L+1) (PowerSpectrum) or S N (L+1)^2 */
+#define RGPU_INFO_N_CENTRES 4
+#define RGPU_INFO_N_PAIRS 5      /* strict pairs including the self pairs */
+#define RGPU_INFO_N_STRUCT 6
+#define RGPU_INFO_SPLINE_GRID 7  /* 0 when the radial integral is not splined */
+#define RGPU_INFO_N_SPARSE 8
+#define RGPU_INFO_IS_INVARIANTS 9 /* 1 PowerSpectrum, 0 SphericalExpansion */
+#define RGPU_INFO_PRECISION 10
+
+typedef struct rgpu_context rgpu_context;
+
+const char* rgpu_last_error(void);
+
+/* Calculator from the reference's JSON hypers: a SphericalInvariants
+ * (soap_type "PowerSpectrum") or SphericalExpansion hypers dict.
+ * Replaces CalculatorSphericalInvariants(const json&)
+ * (representations/calculator_spherical_invariants.hh:171-176) and
+ * CalculatorSphericalExpansion(const json&) (calculator_spherical_expansion.hh
+ * :1779-1783), including the radial-spline set-up (:1107-1135). */
+int rgpu_context_create(const char* hypers_json, rgpu_context** ctx);
+void rgpu_context_destroy(rgpu_context* ctx);
+
+/* HIP stream used by every later call (default: the null stream). */
+int rgpu_set_stream(rgpu_context* ctx, void* hip_stream);
+/* RGPU_FP64 (default) or RGPU_MIXED; invalidates computed results. */
+int rgpu_set_precision(rgpu_context* ctx, int precision);
+int rgpu_get_info(rgpu_context* ctx, int key, int64_t* value);
+
+/* Batch of structures (host arrays).  Builds the neighbour list on the GPU.
+ * species[n_species] sorted ascending must contain every atom type.
+ * Replaces AtomsList / ManagerCollection::add_structures with the stack
+ * centers -> neighbourlist -> centercontribution -> strict
+ * (structure_manager_collection.hh:177-236; adaptor_neighbour_list.hh
+ * :904-1118; adaptor_center_contribution.hh:334-403; adaptor_strict.hh
+ * :373-455). */
+int rgpu_set_structures(rgpu_context* ctx, int n_struct, const int64_t* atom_offsets,
+                        const double* positions, const int32_t* types, const double* cells,
+                        const int32_t* pbc, int n_species, const int32_t* species);
+/* New positions (and cells, may be NULL) for the same atoms and types, e.g.
+ * one MD step; rebuilds the neighbour list. */
+int rgpu_update_positions(rgpu_context* ctx, const double* positions, const double* cells);
+
+/* Strict pair list, self pair first per centre: centre_offsets[n_centres+1],
+ * pair_j[n_pairs] (original atom index within the batch), pair_vec[n_pairs][3]
+ * = r_j(image) - r_i.  pair_vec may be NULL. */
+int rgpu_get_neighbours(rgpu_context* ctx, int64_t* centre_offsets, int32_t* pair_j,
+                        double* pair_vec);
+
+/* Expansion (and power spectrum for invariants hypers) of the batch.
+ * Replaces compute() (calculator_spherical_invariants.hh:506-531,
+ * 693-931; calculator_spherical_expansion.hh:2139-2411). */
+int rgpu_compute(rgpu_context* ctx);
+/* expansion coefficients (dense layout) and per-centre key masks (bit s set
+ * when species s is among the centre's expansion keys); either may be NULL */
+int rgpu_get_expansion(rgpu_context* ctx, double* coeffs, uint64_t* key_mask);
+/* power spectrum [n_centres][D], normalised when the hypers say so, and the
+ * per-centre inverse norms (1 when not normalising); either may be NULL */
+int rgpu_get_features(rgpu_context* ctx, double* features, double* inv_norms);
+
+/* Adjoint of the features: given dL/dX [n_centres][D] (X as returned by
+ * rgpu_get_features), writes dL/dr [n_atoms][3] and the negative virial
+ * [n_struct][6] (Voigt xx,yy,zz,yz,xz,xy, divided by the cell volume); either
+ * output may be NULL.  Uses the same pair-slot conventions as the reference's
+ * gradient properties (calculator_spherical_expansion.hh:2306-2401;
+ * sparse_kernel_predict.hh:206-377). */
+int rgpu_features_backward(rgpu_context* ctx, const double* dL_dX, double* dL_dr,
+                           double* neg_stress);
+
+/* Sparse points of a GAP model: values [n_sparse][D] in the context's dense
+ * layout, species[n_sparse] centre types, weights[n_sparse] (may be NULL for
+ * kernel-only use).  Replaces SparsePointsBlockSparse::push_back / JSON load
+ * (models/sparse_points.hh:372-500) and SparseKernel{"name":"GAP","zeta":z}
+ * (models/sparse_kernels.hh:517-742). */
+int rgpu_set_sparse_points(rgpu_context* ctx, int zeta, int64_t n_sparse, const int32_t* species,
+                           const double* values, const double* weights);
+/* K_NM: target 0 = Structure [n_struct][n_sparse] (sum over centres),
+ * 1 = Atom [n_centres][n_sparse]; columns in the order the sparse points were
+ * given (SparseKernelImpl<GAP>::compute, sparse_kernels.hh:104-190). */
+int rgpu_gap_knm(rgpu_context* ctx, int target, double* knm);
+/* Prediction without K_NM rows: energy[n_struct] = K_NM w, gradient
+ * [n_atoms][3] = dE/dr, neg_stress [n_struct][6] (/ volume).  Any output may
+ * be NULL; gradient/neg_stress trigger the adjoint
+ * (compute_sparse_kernel_gradients / _neg_stress,
+ * sparse_kernel_predict.hh:206-377). */
+int rgpu_gap_predict(rgpu_context* ctx, double* energy, double* gradient, double* neg_stress);
+
+/* Diagnostics (unit-level parity): radial integrals O^T R(r) (or raw R for
+ * the direct basis, see the library docs) at n points: R[n][N][L+1],
+ * dR[n][N][L+1] (may be NULL), evaluated by the GPU path of the current
+ * precision; Y_lm and gradients at n unit vectors: Y[n][(L+1)^2],
+ * dY[n][3][(L+1)^2] (may be NULL). */
+int rgpu_eval_radial(rgpu_context* ctx, int64_t n, const double* r, double* R, double* dR);
+int rgpu_eval_ylm(rgpu_context* ctx, int64_t n, const double* dirs, double* Y, double* dY);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* RASCAL_MI355X_H_ */

@@ -1,0 +1,213 @@
+"""ctypes binding of the C ABI in include/rascal_mi355x.h.
+
+The shared library must have been built (``python -m librascal_amd.build``);
+there is no CPU fallback: loading fails loudly when it is missing, and every
+call raises when the library reports an error.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from .build import LIB_PATH
+
+RGPU_OK = 0
+RGPU_ERR_INVALID = 1
+RGPU_ERR_RUNTIME = 2
+RGPU_ERR_HIP = 3
+RGPU_ERR_STATE = 4
+
+FP64 = 0
+MIXED = 1
+
+INFO = dict(max_radial=0, max_angular=1, n_species=2, feature_size=3, n_centres=4,
+            n_pairs=5, n_struct=6, spline_grid=7, n_sparse=8, is_invariants=9, precision=10)
+
+_lib = None
+
+
+class RascalGPUError(RuntimeError):
+    """Error raised by the MI355X library (carries the reference's message)."""
+
+    def __init__(self, code, msg):
+        super().__init__(msg)
+        self.code = code
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            "librascal_mi355x.so is not built (%s); run `python -m librascal_amd.build`" % LIB_PATH)
+    L = C.CDLL(LIB_PATH)
+    p, i, i64 = C.c_void_p, C.c_int, C.c_int64
+    L.rgpu_last_error.restype = C.c_char_p
+    L.rgpu_context_create.argtypes = [C.c_char_p, C.POINTER(C.c_void_p)]
+    L.rgpu_context_destroy.argtypes = [p]
+    L.rgpu_context_destroy.restype = None
+    L.rgpu_set_stream.argtypes = [p, p]
+    L.rgpu_set_precision.argtypes = [p, i]
+    L.rgpu_get_info.argtypes = [p, i, C.POINTER(C.c_int64)]
+    L.rgpu_set_structures.argtypes = [p, i, p, p, p, p, p, i, p]
+    L.rgpu_update_positions.argtypes = [p, p, p]
+    L.rgpu_get_neighbours.argtypes = [p, p, p, p]
+    L.rgpu_compute.argtypes = [p]
+    L.rgpu_get_expansion.argtypes = [p, p, p]
+    L.rgpu_get_features.argtypes = [p, p, p]
+    L.rgpu_features_backward.argtypes = [p, p, p, p]
+    L.rgpu_set_sparse_points.argtypes = [p, i, i64, p, p, p]
+    L.rgpu_gap_knm.argtypes = [p, i, p]
+    L.rgpu_gap_predict.argtypes = [p, p, p, p]
+    L.rgpu_eval_radial.argtypes = [p, i64, p, p, p]
+    L.rgpu_eval_ylm.argtypes = [p, i64, p, p, p]
+    for name in ("rgpu_context_create", "rgpu_set_stream", "rgpu_set_precision", "rgpu_get_info",
+                 "rgpu_set_structures", "rgpu_update_positions", "rgpu_get_neighbours",
+                 "rgpu_compute", "rgpu_get_expansion", "rgpu_get_features",
+                 "rgpu_features_backward", "rgpu_set_sparse_points", "rgpu_gap_knm",
+                 "rgpu_gap_predict", "rgpu_eval_radial", "rgpu_eval_ylm"):
+        getattr(L, name).restype = C.c_int
+    _lib = L
+    return L
+
+
+def check(rc):
+    if rc != RGPU_OK:
+        msg = lib().rgpu_last_error().decode()
+        if rc == RGPU_ERR_INVALID:
+            raise RascalGPUError(rc, msg)
+        raise RascalGPUError(rc, msg)
+
+
+def ptr(a):
+    """Host numpy array or torch tensor (host or device) -> void pointer."""
+    if a is None:
+        return None
+    if isinstance(a, np.ndarray):
+        assert a.flags["C_CONTIGUOUS"], "arrays passed to the library must be C-contiguous"
+        return a.ctypes.data_as(C.c_void_p)
+    # torch tensor
+    assert a.is_contiguous(), "tensors passed to the library must be contiguous"
+    return C.c_void_p(a.data_ptr())
+
+
+class Context:
+    """Thin owner of an rgpu_context (one calculator + batch + GAP model)."""
+
+    def __init__(self, hypers_json):
+        self._p = C.c_void_p()
+        check(lib().rgpu_context_create(hypers_json.encode(), C.byref(self._p)))
+
+    def __del__(self):
+        if getattr(self, "_p", None) and self._p.value and _lib is not None:
+            _lib.rgpu_context_destroy(self._p)
+            self._p = None
+
+    @property
+    def handle(self):
+        return self._p
+
+    def info(self, key):
+        v = C.c_int64(0)
+        check(lib().rgpu_get_info(self._p, INFO[key], C.byref(v)))
+        return v.value
+
+    def set_stream(self, stream_handle):
+        check(lib().rgpu_set_stream(self._p, C.c_void_p(stream_handle)))
+
+    def set_precision(self, precision):
+        check(lib().rgpu_set_precision(self._p, int(precision)))
+
+    def set_structures(self, atom_offsets, positions, types, cells, pbc, species):
+        ao = np.ascontiguousarray(atom_offsets, np.int64)
+        pos = np.ascontiguousarray(positions, np.float64).reshape(-1, 3)
+        ty = np.ascontiguousarray(types, np.int32)
+        ce = np.ascontiguousarray(cells, np.float64).reshape(-1, 9)
+        pb = np.ascontiguousarray(pbc, np.int32).reshape(-1, 3)
+        sp = np.ascontiguousarray(sorted(species), np.int32)
+        check(lib().rgpu_set_structures(self._p, len(ao) - 1, ptr(ao), ptr(pos), ptr(ty), ptr(ce),
+                                        ptr(pb), len(sp), ptr(sp)))
+
+    def update_positions(self, positions, cells=None):
+        pos = np.ascontiguousarray(positions, np.float64).reshape(-1, 3)
+        ce = None if cells is None else np.ascontiguousarray(cells, np.float64).reshape(-1, 9)
+        check(lib().rgpu_update_positions(self._p, ptr(pos), ptr(ce)))
+
+    def neighbours(self, with_vectors=True):
+        n = self.info("n_centres")
+        npairs = self.info("n_pairs")
+        off = np.zeros(n + 1, np.int64)
+        pj = np.zeros(npairs, np.int32)
+        vec = np.zeros((npairs, 3)) if with_vectors else None
+        check(lib().rgpu_get_neighbours(self._p, ptr(off), ptr(pj), ptr(vec)))
+        return off, pj, vec
+
+    def compute(self):
+        check(lib().rgpu_compute(self._p))
+
+    def expansion(self):
+        n = self.info("n_centres")
+        S = self.info("n_species")
+        N = self.info("max_radial")
+        L1 = self.info("max_angular") + 1
+        c = np.zeros((n, S, N, L1 * L1))
+        km = np.zeros(n, np.uint64)
+        check(lib().rgpu_get_expansion(self._p, ptr(c), ptr(km)))
+        return c, km
+
+    def features(self):
+        n = self.info("n_centres")
+        D = self.info("feature_size")
+        X = np.zeros((n, D))
+        inv = np.zeros(n)
+        check(lib().rgpu_get_features(self._p, ptr(X), ptr(inv)))
+        return X, inv
+
+    def features_backward(self, dL_dX, want_stress=True):
+        n = self.info("n_centres")
+        ns = self.info("n_struct")
+        g = np.ascontiguousarray(dL_dX, np.float64)
+        out = np.zeros((n, 3))
+        st = np.zeros((ns, 6)) if want_stress else None
+        check(lib().rgpu_features_backward(self._p, ptr(g), ptr(out), ptr(st)))
+        return out, st
+
+    def set_sparse_points(self, zeta, species, values, weights=None):
+        sp = np.ascontiguousarray(species, np.int32)
+        v = np.ascontiguousarray(values, np.float64)
+        w = None if weights is None else np.ascontiguousarray(weights, np.float64)
+        check(lib().rgpu_set_sparse_points(self._p, int(zeta), len(sp), ptr(sp), ptr(v), ptr(w)))
+
+    def gap_knm(self, target="Structure"):
+        t = 0 if target == "Structure" else 1
+        rows = self.info("n_struct") if t == 0 else self.info("n_centres")
+        k = np.zeros((rows, self.info("n_sparse")))
+        check(lib().rgpu_gap_knm(self._p, t, ptr(k)))
+        return k
+
+    def gap_predict(self, energy=True, gradient=True, neg_stress=True):
+        ns = self.info("n_struct")
+        n = self.info("n_centres")
+        e = np.zeros(ns) if energy else None
+        g = np.zeros((n, 3)) if gradient else None
+        s = np.zeros((ns, 6)) if neg_stress else None
+        check(lib().rgpu_gap_predict(self._p, ptr(e), ptr(g), ptr(s)))
+        return e, g, s
+
+    def eval_radial(self, r, derivative=False):
+        r = np.ascontiguousarray(r, np.float64)
+        N = self.info("max_radial")
+        L1 = self.info("max_angular") + 1
+        R = np.zeros((len(r), N, L1))
+        dR = np.zeros_like(R) if derivative else None
+        check(lib().rgpu_eval_radial(self._p, len(r), ptr(r), ptr(R), ptr(dR)))
+        return (R, dR) if derivative else R
+
+    def eval_ylm(self, dirs, derivative=False):
+        d = np.ascontiguousarray(dirs, np.float64).reshape(-1, 3)
+        L1 = self.info("max_angular") + 1
+        Y = np.zeros((len(d), L1 * L1))
+        dY = np.zeros((len(d), 3, L1 * L1)) if derivative else None
+        check(lib().rgpu_eval_ylm(self._p, len(d), ptr(d), ptr(Y), ptr(dY)))
+        return (Y, dY) if derivative else Y

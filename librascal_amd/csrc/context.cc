@@ -1,0 +1,837 @@
+// Context object and C ABI (include/rascal_mi355x.h).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <type_traits>
+#include <vector>
+
+#include "../../include/rascal_mi355x.h"
+#include "common.hh"
+#include "device_buffer.hh"
+#include "gap_kernels.hh"
+#include "gap_mfma.hh"
+#include "hypers.hh"
+#include "neighbours.hh"
+#include "radial.hh"
+#include "soap_kernels.hh"
+
+namespace rgpu {
+
+namespace {
+thread_local std::string g_last_error;
+}
+
+struct Context {
+  SoapHypers hp;
+  std::unique_ptr<RadialHost> rh;
+  RadialDev rd{};
+  hipStream_t stream = nullptr;
+  int precision = RGPU_FP64;
+
+  // radial tables on the device
+  DeviceBuffer<double> d_fac_b, d_O, d_center, d_hc, d_ha, d_grid, d_spl64;
+  DeviceBuffer<float> d_spl32;
+  DeviceBuffer<HypCoef> d_hyp;
+
+  // batch
+  bool have_batch = false;
+  int n_struct = 0;
+  int64_t n_atoms = 0;
+  std::vector<int64_t> atom_off;
+  std::vector<double> cells;
+  std::vector<int> pbc, types;
+  std::vector<int> species;      // sorted
+  std::vector<int> species_of;   // [n_atoms] index into species
+  DeviceBuffer<int> d_species_of;
+  DeviceBuffer<int64_t> d_atom_off;
+  DeviceBuffer<int> d_centre_struct;
+  PairList pl;
+
+  // representation
+  bool have_rep = false;
+  DeviceBuffer<double> c64, p64;
+  DeviceBuffer<float> c32, p32;
+  DeviceBuffer<double> norm2;
+  DeviceBuffer<unsigned long long> keymask;
+
+  // sparse GAP model
+  bool have_sparse = false;
+  int zeta = 1;
+  int64_t M = 0;
+  std::vector<int> sp_species;    // species-sorted order
+  std::vector<int64_t> sp_order;  // sorted position -> given position
+  std::vector<int64_t> sp_off;    // per species index s: [sp_off[s], sp_off[s+1])
+  bool have_weights = false;
+  DeviceBuffer<double> T64, alpha;
+  DeviceBuffer<float> T32;
+  MixedSparse mixed_sparse;
+
+  // per-species centre rows
+  std::vector<int64_t> row_off;   // [S+1]
+  DeviceBuffer<int> d_rows;
+
+  // scratch
+  DeviceBuffer<double> e_centre, grad, cvir, tmp64, out64;
+  DeviceBuffer<double> wt64, G64, W64;
+  DeviceBuffer<float> wt32, G32, W32;
+  DeviceBuffer<int64_t> knm_perm;
+
+  int64_t D() const {
+    const int S = int(species.size());
+    const int N = hp.max_radial, L1 = hp.max_angular + 1;
+    if (hp.kind == SoapHypers::kPowerSpectrum) return int64_t(S) * (S + 1) / 2 * N * N * L1;
+    return int64_t(S) * N * L1 * L1;
+  }
+};
+
+namespace {
+
+void build_tables(Context& c) {
+  c.rh.reset(new RadialHost(c.hp));
+  RadialHost& h = *c.rh;
+  const int N = h.N, L1 = h.L + 1;
+  auto up = [&](auto& buf, const auto& vec) {
+    using V = typename std::decay<decltype(vec)>::type::value_type;
+    auto* p = buf.reserve(vec.size());
+    RGPU_HIP(hipMemcpy(p, vec.data(), sizeof(V) * vec.size(), hipMemcpyHostToDevice));
+    return p;
+  };
+  RadialDev& r = c.rd;
+  r.N = N;
+  r.L = h.L;
+  r.L1 = L1;
+  r.LL = L1 * L1;
+  r.recursion = h.L >= 3 ? 1 : 0;  // hyp1f1.cc:383-390
+  r.cutoff_type = c.hp.cutoff_type;
+  r.rc = c.hp.cutoff;
+  r.sw = c.hp.smooth_width;
+  r.rs_rate = c.hp.rs_rate;
+  r.rs_scale = c.hp.rs_scale;
+  r.rs_exponent = c.hp.rs_exponent;
+  r.fac_a = h.fac_a;
+  r.fac_b = up(c.d_fac_b, h.fac_b);
+  r.O = up(c.d_O, h.O);
+  r.hyp = up(c.d_hyp, h.hyp);
+  r.hc = up(c.d_hc, h.hc);
+  r.ha = up(c.d_ha, h.ha);
+  r.use_spline = 0;
+  r.center = up(c.d_center, h.center_raw);
+  if (c.hp.optimization != 1) return;
+
+  // Spline of O^T R(r) on [1e-5, rc] (calculator_spherical_expansion.hh
+  // :1107-1135): grids of 2<<k points, k = 5, 6, ..., refined until the mean
+  // absolute error at the cell midpoints is below the accuracy
+  // (math/interpolator.hh:705-712, 1229-1248).  Function values come from
+  // the FP64 Hyp1f1 kernel; the tridiagonal solve runs on the host.
+  const int F = N * L1;
+  const double x1 = 1e-5, x2 = c.hp.cutoff;
+  DeviceBuffer<double> d_r, d_raw, d_fin;
+  std::vector<double> grid, y, d2, test, ref, inter(F);
+  auto eval_direct = [&](const std::vector<double>& xs, std::vector<double>& out) {
+    int64_t n = int64_t(xs.size());
+    double* dr = d_r.reserve(n);
+    RGPU_HIP(hipMemcpy(dr, xs.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+    double* raw = d_raw.reserve(size_t(n) * F);
+    double* fin = d_fin.reserve(size_t(n) * F);
+    launch_radial_direct(r, dr, n, raw, nullptr, nullptr);
+    launch_radial_finalise(r, raw, n, fin, nullptr);
+    out.resize(size_t(n) * F);
+    RGPU_HIP(hipMemcpy(out.data(), fin, sizeof(double) * n * F, hipMemcpyDeviceToHost));
+  };
+  int dof = 5;
+  double h_step = 0, per_unit = 0;
+  while (true) {
+    const int npts = 2 << dof;
+    grid = linspaced(npts, x1, x2);
+    eval_direct(grid, y);
+    h_step = grid[1] - grid[0];
+    spline_second_derivatives(npts, F, h_step, y, d2);
+    per_unit = double(npts) / (grid[npts - 1] - grid[0]);
+    const int tdof = std::max(dof, 5);
+    const int tn = 2 << tdof;
+    const double off = (x2 - x1) / (2 * (tn - 1));
+    test = linspaced(tn - 1, x1 + off, x2 - off);
+    eval_direct(test, ref);
+    double err = 0;
+    for (size_t t = 0; t < test.size(); ++t) {
+      const double x = test[t];
+      int k = std::max(0, std::min(npts - 2, int((x - x1) * per_unit) - 1));
+      const double a = (grid[k + 1] - x) / h_step, b = 1 - a;
+      for (int f = 0; f < F; ++f) {
+        const double v = a * (y[size_t(k) * F + f] + (a * a - 1) * d2[size_t(k) * F + f]) +
+                         b * (y[size_t(k + 1) * F + f] + (b * b - 1) * d2[size_t(k + 1) * F + f]);
+        err += std::fabs(v - ref[t * F + f]);
+      }
+    }
+    err /= double(test.size()) * F;
+    if (err < c.hp.spline_accuracy || npts >= 100000) break;
+    ++dof;
+  }
+  const int npts = int(grid.size());
+  std::vector<double> tab(size_t(npts) * 2 * F);
+  std::vector<float> tab32(tab.size());
+  for (int k = 0; k < npts; ++k)
+    for (int f = 0; f < F; ++f) {
+      tab[size_t(k) * 2 * F + f] = y[size_t(k) * F + f];
+      tab[size_t(k) * 2 * F + F + f] = d2[size_t(k) * F + f];
+    }
+  for (size_t k = 0; k < tab.size(); ++k) tab32[k] = float(tab[k]);
+  h.grid_n = npts;
+  h.x1 = x1;
+  h.x2 = x2;
+  h.h = h_step;
+  h.per_unit = per_unit;
+  h.spl64 = tab;
+  r.use_spline = 1;
+  r.grid_n = npts;
+  r.search_size = npts - 2;
+  r.x1 = x1;
+  r.h = h_step;
+  r.per_unit = per_unit;
+  r.grid = up(c.d_grid, grid);
+  r.spl64 = up(c.d_spl64, tab);
+  r.spl32 = up(c.d_spl32, tab32);
+  r.center = up(c.d_center, h.center_fin);
+}
+
+void require_batch(const Context& c) {
+  if (!c.have_batch) throw InvalidArgument("no structures set (call rgpu_set_structures first)");
+}
+
+void build_batch_nl(Context& c, const double* positions) {
+  build_pair_list(c.pl, c.n_struct, c.atom_off.data(), positions, c.cells.data(), c.pbc.data(),
+                  c.hp.cutoff, c.stream);
+  c.have_rep = false;
+}
+
+PairView pair_view(Context& c) {
+  PairView pv;
+  pv.n_centres = c.n_atoms;
+  pv.centre_off = c.pl.centre_off.get();
+  pv.pair_j = c.pl.pair_j.get();
+  pv.pair_vec = c.pl.pair_vec.get();
+  pv.pair_dist = c.pl.pair_dist.get();
+  pv.species_of = c.d_species_of.get();
+  return pv;
+}
+
+template <class T>
+DeviceBuffer<T>& cbuf(Context& c);
+template <>
+DeviceBuffer<double>& cbuf<double>(Context& c) {
+  return c.c64;
+}
+template <>
+DeviceBuffer<float>& cbuf<float>(Context& c) {
+  return c.c32;
+}
+template <class T>
+DeviceBuffer<T>& pbuf(Context& c);
+template <>
+DeviceBuffer<double>& pbuf<double>(Context& c) {
+  return c.p64;
+}
+template <>
+DeviceBuffer<float>& pbuf<float>(Context& c) {
+  return c.p32;
+}
+
+template <class T>
+void compute_rep(Context& c) {
+  const int S = int(c.species.size());
+  const int64_t n = c.n_atoms;
+  const int64_t E = int64_t(S) * c.rd.N * c.rd.LL;
+  T* cc = cbuf<T>(c).reserve(size_t(std::max<int64_t>(n * E, 1)));
+  unsigned long long* km = c.keymask.reserve(size_t(std::max<int64_t>(n, 1)));
+  launch_expansion<T>(c.rd, S, pair_view(c), cc, km, c.stream);
+  if (c.hp.kind == SoapHypers::kPowerSpectrum) {
+    const int64_t D = c.D();
+    T* pp = pbuf<T>(c).reserve(size_t(std::max<int64_t>(n * D, 1)));
+    double* n2 = c.norm2.reserve(size_t(std::max<int64_t>(n, 1)));
+    launch_power_spectrum<T>(c.rd, S, n, cc, pp, n2, c.stream);
+  }
+  c.have_rep = true;
+}
+
+void compute(Context& c) {
+  require_batch(c);
+  if (c.precision == RGPU_FP64)
+    compute_rep<double>(c);
+  else
+    compute_rep<float>(c);
+}
+
+void ensure_rep(Context& c) {
+  if (!c.have_rep) compute(c);
+}
+
+void require_invariants(const Context& c) {
+  if (c.hp.kind != SoapHypers::kPowerSpectrum)
+    throw InvalidArgument("this operation needs SphericalInvariants (PowerSpectrum) hypers");
+}
+
+// GAP species blocks of the current batch
+void build_rows(Context& c) {
+  const int S = int(c.species.size());
+  c.row_off.assign(S + 1, 0);
+  std::vector<int> rows;
+  rows.reserve(size_t(c.n_atoms));
+  for (int s = 0; s < S; ++s) {
+    for (int64_t i = 0; i < c.n_atoms; ++i)
+      if (c.species_of[i] == s) rows.push_back(int(i));
+    c.row_off[s + 1] = int64_t(rows.size());
+  }
+  int* d = c.d_rows.reserve(std::max<size_t>(rows.size(), 1));
+  if (!rows.empty())
+    RGPU_HIP(hipMemcpyAsync(d, rows.data(), sizeof(int) * rows.size(), hipMemcpyHostToDevice, c.stream));
+}
+
+template <class T>
+GapBlock<T> gap_block(Context& c, int s, const T* X, const T* Tm) {
+  GapBlock<T> b;
+  b.n_rows = c.row_off[s + 1] - c.row_off[s];
+  b.row_idx = c.d_rows.get() + c.row_off[s];
+  b.n_cols = c.sp_off[s + 1] - c.sp_off[s];
+  b.col0 = c.sp_off[s];
+  b.X = X;
+  b.norm2 = c.norm2.get();
+  b.normalize = c.hp.normalize ? 1 : 0;
+  b.Tm = Tm;
+  b.alpha = c.have_weights ? c.alpha.get() : nullptr;
+  b.zeta = c.zeta;
+  b.D = c.D();
+  return b;
+}
+
+template <class T>
+const T* sparse_values(Context& c);
+template <>
+const double* sparse_values<double>(Context& c) {
+  return c.T64.get();
+}
+template <>
+const float* sparse_values<float>(Context& c) {
+  return c.T32.get();
+}
+template <class T>
+DeviceBuffer<T>& wtbuf(Context& c);
+template <>
+DeviceBuffer<double>& wtbuf<double>(Context& c) {
+  return c.wt64;
+}
+template <>
+DeviceBuffer<float>& wtbuf<float>(Context& c) {
+  return c.wt32;
+}
+template <class T>
+DeviceBuffer<T>& Gbuf(Context& c);
+template <>
+DeviceBuffer<double>& Gbuf<double>(Context& c) {
+  return c.G64;
+}
+template <>
+DeviceBuffer<float>& Gbuf<float>(Context& c) {
+  return c.G32;
+}
+template <class T>
+DeviceBuffer<T>& Wbuf(Context& c);
+template <>
+DeviceBuffer<double>& Wbuf<double>(Context& c) {
+  return c.W64;
+}
+template <>
+DeviceBuffer<float>& Wbuf<float>(Context& c) {
+  return c.W32;
+}
+
+// Backward from G (device, [n][D] in precision T) to grad/virial (device).
+template <class T>
+void backward_device(Context& c, const T* G, double* grad, double* cvir) {
+  const int S = int(c.species.size());
+  const int64_t E = int64_t(S) * c.rd.N * c.rd.LL;
+  T* W = Wbuf<T>(c).reserve(size_t(std::max<int64_t>(c.n_atoms * E, 1)));
+  launch_backward<T>(c.rd, S, pair_view(c), nullptr, cbuf<T>(c).get(), pbuf<T>(c).get(), c.norm2.get(),
+                     c.hp.normalize, G, W, grad, cvir, c.stream);
+}
+
+double cell_volume(const double* C) {
+  return std::fabs(C[0] * (C[4] * C[8] - C[5] * C[7]) - C[1] * (C[3] * C[8] - C[5] * C[6]) +
+                   C[2] * (C[3] * C[7] - C[4] * C[6]));
+}
+
+// gradient/neg_stress outputs from device grad [n_atoms][3] and per-centre
+// virials
+void emit_forces(Context& c, double* grad_d, double* cvir_d, double* gradient, double* neg_stress) {
+  if (gradient)
+    RGPU_HIP(hipMemcpyAsync(gradient, grad_d, sizeof(double) * 3 * c.n_atoms, hipMemcpyDefault, c.stream));
+  if (neg_stress) {
+    double* ns = c.out64.reserve(size_t(6 * std::max(c.n_struct, 1)));
+    RGPU_HIP(hipMemsetAsync(ns, 0, sizeof(double) * 6 * c.n_struct, c.stream));
+    launch_segment_sum(cvir_d, 6, c.d_atom_off.get(), c.n_struct, ns, c.stream);
+    std::vector<double> h(static_cast<size_t>(6 * c.n_struct));
+    RGPU_HIP(hipMemcpyAsync(h.data(), ns, sizeof(double) * h.size(), hipMemcpyDeviceToHost, c.stream));
+    RGPU_HIP(hipStreamSynchronize(c.stream));
+    for (int s = 0; s < c.n_struct; ++s) {
+      const double vol = cell_volume(&c.cells[9 * s]);
+      for (int k = 0; k < 6; ++k) h[6 * s + k] /= vol;
+    }
+    RGPU_HIP(hipMemcpyAsync(neg_stress, h.data(), sizeof(double) * h.size(), hipMemcpyDefault, c.stream));
+  }
+  RGPU_HIP(hipStreamSynchronize(c.stream));
+}
+
+template <class T>
+void gap_predict_t(Context& c, double* energy, double* gradient, double* neg_stress) {
+  const int S = int(c.species.size());
+  const int64_t n = c.n_atoms, D = c.D();
+  const bool want_grad = gradient || neg_stress;
+  double* ec = c.e_centre.reserve(size_t(std::max<int64_t>(n, 1)));
+  RGPU_HIP(hipMemsetAsync(ec, 0, sizeof(double) * n, c.stream));
+  T* G = nullptr;
+  if (want_grad) {
+    G = Gbuf<T>(c).reserve(size_t(std::max<int64_t>(n * D, 1)));
+    RGPU_HIP(hipMemsetAsync(G, 0, sizeof(T) * n * D, c.stream));
+  }
+  const T* X = pbuf<T>(c).get();
+  for (int s = 0; s < S; ++s) {
+    GapBlock<T> blk = gap_block<T>(c, s, X, sparse_values<T>(c));
+    if (blk.n_rows == 0 || blk.n_cols == 0) continue;
+    T* wt = want_grad ? wtbuf<T>(c).reserve(size_t(blk.n_rows * blk.n_cols)) : nullptr;
+    bool done = false;
+    if constexpr (std::is_same<T, float>::value) {
+      if (mfma_available()) {
+        run_gap_mixed(blk, c.mixed_sparse, ec, wt, G, c.stream);
+        done = true;
+      }
+    }
+    if (!done) {
+      launch_gap_kernel<T>(blk, c.M, ec, wt, nullptr, nullptr, nullptr, c.stream);
+      if (want_grad) launch_gap_force_contraction<T>(blk, wt, G, c.stream);
+    }
+  }
+  if (energy) {
+    double* es = c.out64.reserve(size_t(std::max(c.n_struct, 1)));
+    RGPU_HIP(hipMemsetAsync(es, 0, sizeof(double) * c.n_struct, c.stream));
+    launch_segment_sum(ec, 1, c.d_atom_off.get(), c.n_struct, es, c.stream);
+    RGPU_HIP(hipMemcpyAsync(energy, es, sizeof(double) * c.n_struct, hipMemcpyDefault, c.stream));
+    RGPU_HIP(hipStreamSynchronize(c.stream));
+  }
+  if (!want_grad) return;
+  double* gd = c.grad.reserve(size_t(std::max<int64_t>(3 * n, 1)));
+  double* cv = c.cvir.reserve(size_t(std::max<int64_t>(6 * n, 1)));
+  RGPU_HIP(hipMemsetAsync(gd, 0, sizeof(double) * 3 * n, c.stream));
+  backward_device<T>(c, G, gd, cv);
+  emit_forces(c, gd, cv, gradient, neg_stress);
+}
+
+template <class T>
+void gap_knm_t(Context& c, int target, double* knm) {
+  const int S = int(c.species.size());
+  const int64_t n = c.n_atoms;
+  const int64_t rows = target == 0 ? c.n_struct : n;
+  double* k = c.tmp64.reserve(size_t(std::max<int64_t>(rows * c.M, 1)));
+  RGPU_HIP(hipMemsetAsync(k, 0, sizeof(double) * rows * c.M, c.stream));
+  const T* X = pbuf<T>(c).get();
+  for (int s = 0; s < S; ++s) {
+    GapBlock<T> blk = gap_block<T>(c, s, X, sparse_values<T>(c));
+    blk.alpha = nullptr;
+    if (blk.n_rows == 0 || blk.n_cols == 0) continue;
+    launch_gap_kernel<T>(blk, c.M, nullptr, nullptr, target == 1 ? k : nullptr,
+                         target == 0 ? k : nullptr, c.d_centre_struct.get(), c.stream);
+  }
+  // columns back to the order the sparse points were given
+  std::vector<double> h(static_cast<size_t>(rows * c.M)), o(h.size());
+  RGPU_HIP(hipMemcpyAsync(h.data(), k, sizeof(double) * h.size(), hipMemcpyDeviceToHost, c.stream));
+  RGPU_HIP(hipStreamSynchronize(c.stream));
+  for (int64_t r = 0; r < rows; ++r)
+    for (int64_t m = 0; m < c.M; ++m) o[size_t(r * c.M + c.sp_order[m])] = h[size_t(r * c.M + m)];
+  RGPU_HIP(hipMemcpy(knm, o.data(), sizeof(double) * o.size(), hipMemcpyDefault));
+}
+
+int fail(int code, const std::string& msg) {
+  g_last_error = msg;
+  return code;
+}
+
+template <class F>
+int guarded(F&& f) {
+  try {
+    g_last_error.clear();
+    f();
+    return RGPU_OK;
+  } catch (const InvalidArgument& e) {
+    return fail(RGPU_ERR_INVALID, e.what());
+  } catch (const RuntimeError& e) {
+    return fail(RGPU_ERR_RUNTIME, e.what());
+  } catch (const HipError& e) {
+    return fail(RGPU_ERR_HIP, e.what());
+  } catch (const std::exception& e) {
+    return fail(RGPU_ERR_RUNTIME, e.what());
+  }
+}
+
+}  // namespace
+}  // namespace rgpu
+
+using namespace rgpu;
+
+struct rgpu_context {
+  Context c;
+};
+
+extern "C" {
+
+const char* rgpu_last_error(void) { return g_last_error.c_str(); }
+
+int rgpu_context_create(const char* hypers_json, rgpu_context** out) {
+  return guarded([&] {
+    if (!hypers_json || !out) throw InvalidArgument("null argument");
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+      throw HipError("no HIP device available: the MI355X library needs a GPU");
+    std::unique_ptr<rgpu_context> ctx(new rgpu_context());
+    ctx->c.hp = parse_soap_hypers(hypers_json);
+    build_tables(ctx->c);
+    RGPU_HIP(hipDeviceSynchronize());
+    *out = ctx.release();
+  });
+}
+
+void rgpu_context_destroy(rgpu_context* ctx) { delete ctx; }
+
+int rgpu_set_stream(rgpu_context* ctx, void* s) {
+  return guarded([&] {
+    if (!ctx) throw InvalidArgument("null context");
+    ctx->c.stream = static_cast<hipStream_t>(s);
+  });
+}
+
+int rgpu_set_precision(rgpu_context* ctx, int precision) {
+  return guarded([&] {
+    if (!ctx) throw InvalidArgument("null context");
+    if (precision != RGPU_FP64 && precision != RGPU_MIXED) throw InvalidArgument("unknown precision");
+    Context& c = ctx->c;
+    if (c.precision != precision) c.have_rep = false;
+    c.precision = precision;
+    if (c.have_sparse && precision == RGPU_MIXED && c.T32.get() == nullptr)
+      throw InvalidArgument("internal: sparse points not prepared for mixed precision");
+  });
+}
+
+int rgpu_get_info(rgpu_context* ctx, int key, int64_t* value) {
+  return guarded([&] {
+    if (!ctx || !value) throw InvalidArgument("null argument");
+    Context& c = ctx->c;
+    switch (key) {
+      case RGPU_INFO_MAX_RADIAL: *value = c.hp.max_radial; break;
+      case RGPU_INFO_MAX_ANGULAR: *value = c.hp.max_angular; break;
+      case RGPU_INFO_N_SPECIES: *value = int64_t(c.species.size()); break;
+      case RGPU_INFO_FEATURE_SIZE: *value = c.D(); break;
+      case RGPU_INFO_N_CENTRES: *value = c.n_atoms; break;
+      case RGPU_INFO_N_PAIRS: *value = c.have_batch ? c.pl.n_pairs : 0; break;
+      case RGPU_INFO_N_STRUCT: *value = c.n_struct; break;
+      case RGPU_INFO_SPLINE_GRID: *value = c.rd.use_spline ? c.rd.grid_n : 0; break;
+      case RGPU_INFO_N_SPARSE: *value = c.M; break;
+      case RGPU_INFO_IS_INVARIANTS: *value = c.hp.kind == SoapHypers::kPowerSpectrum; break;
+      case RGPU_INFO_PRECISION: *value = c.precision; break;
+      default: throw InvalidArgument("unknown info key");
+    }
+  });
+}
+
+int rgpu_set_structures(rgpu_context* ctx, int n_struct, const int64_t* atom_offsets,
+                        const double* positions, const int32_t* types, const double* cells,
+                        const int32_t* pbc, int n_species, const int32_t* species) {
+  return guarded([&] {
+    if (!ctx || !atom_offsets || n_struct < 0 || n_species < 0) throw InvalidArgument("bad arguments");
+    Context& c = ctx->c;
+    c.have_batch = c.have_rep = false;
+    const int64_t n_atoms = atom_offsets[n_struct];
+    if (atom_offsets[0] != 0) throw InvalidArgument("atom_offsets[0] must be 0");
+    if (n_atoms > 0 && (!positions || !types)) throw InvalidArgument("null positions/types");
+    if (n_struct > 0 && (!cells || !pbc)) throw InvalidArgument("null cells/pbc");
+    if (n_species > kMaxSpecies) throw InvalidArgument("at most 64 species");
+    std::vector<int> sp(species, species + n_species);
+    for (int k = 1; k < n_species; ++k)
+      if (sp[k] <= sp[k - 1]) throw InvalidArgument("species must be sorted and unique");
+    std::vector<int> sof(static_cast<size_t>(n_atoms));
+    for (int64_t a = 0; a < n_atoms; ++a) {
+      const int t = types[a];
+      if (t < 0 || t >= kMaxChemElements)
+        throw RuntimeError("atom types (or atomic numbers) should be in the range [0, 130].");
+      auto it = std::lower_bound(sp.begin(), sp.end(), t);
+      if (it == sp.end() || *it != t) throw InvalidArgument("atom type missing from the species list");
+      sof[a] = int(it - sp.begin());
+    }
+    c.n_struct = n_struct;
+    c.n_atoms = n_atoms;
+    c.atom_off.assign(atom_offsets, atom_offsets + n_struct + 1);
+    c.cells.assign(cells, cells + 9 * n_struct);
+    c.pbc.assign(pbc, pbc + 3 * n_struct);
+    c.types.assign(types, types + n_atoms);
+    c.species = sp;
+    c.species_of = sof;
+    int* d = c.d_species_of.reserve(std::max<size_t>(sof.size(), 1));
+    if (n_atoms) RGPU_HIP(hipMemcpyAsync(d, sof.data(), sizeof(int) * n_atoms, hipMemcpyHostToDevice, c.stream));
+    int64_t* dof = c.d_atom_off.reserve(size_t(n_struct + 1));
+    RGPU_HIP(hipMemcpyAsync(dof, c.atom_off.data(), sizeof(int64_t) * (n_struct + 1), hipMemcpyHostToDevice,
+                            c.stream));
+    std::vector<int> cs(size_t(std::max<int64_t>(n_atoms, 1)));
+    for (int s = 0; s < n_struct; ++s)
+      for (int64_t a = atom_offsets[s]; a < atom_offsets[s + 1]; ++a) cs[a] = s;
+    int* dcs = c.d_centre_struct.reserve(cs.size());
+    RGPU_HIP(hipMemcpyAsync(dcs, cs.data(), sizeof(int) * cs.size(), hipMemcpyHostToDevice, c.stream));
+    build_batch_nl(c, positions);
+    build_rows(c);
+    RGPU_HIP(hipStreamSynchronize(c.stream));
+    c.have_batch = true;
+    if (c.have_sparse && (int64_t(c.sp_off.size()) != int64_t(c.species.size()) + 1))
+      c.have_sparse = false;  // species layout changed: sparse points must be set again
+  });
+}
+
+int rgpu_update_positions(rgpu_context* ctx, const double* positions, const double* cells) {
+  return guarded([&] {
+    if (!ctx) throw InvalidArgument("null context");
+    Context& c = ctx->c;
+    require_batch(c);
+    if (cells) c.cells.assign(cells, cells + 9 * c.n_struct);
+    build_batch_nl(c, positions);
+    RGPU_HIP(hipStreamSynchronize(c.stream));
+  });
+}
+
+int rgpu_get_neighbours(rgpu_context* ctx, int64_t* centre_offsets, int32_t* pair_j, double* pair_vec) {
+  return guarded([&] {
+    if (!ctx) throw InvalidArgument("null context");
+    Context& c = ctx->c;
+    require_batch(c);
+    if (centre_offsets)
+      RGPU_HIP(hipMemcpyAsync(centre_offsets, c.pl.centre_off.get(), sizeof(int64_t) * (c.n_atoms + 1),
+                              hipMemcpyDefault, c.stream));
+    if (pair_j && c.pl.n_pairs)
+      RGPU_HIP(hipMemcpyAsync(pair_j, c.pl.pair_j.get(), sizeof(int) * c.pl.n_pairs, hipMemcpyDefault,
+                              c.stream));
+    if (pair_vec && c.pl.n_pairs)
+      RGPU_HIP(hipMemcpyAsync(pair_vec, c.pl.pair_vec.get(), sizeof(double) * 3 * c.pl.n_pairs,
+                              hipMemcpyDefault, c.stream));
+    RGPU_HIP(hipStreamSynchronize(c.stream));
+  });
+}
+
+int rgpu_compute(rgpu_context* ctx) {
+  return guarded([&] {
+    if (!ctx) throw InvalidArgument("null context");
+    compute(ctx->c);
+    RGPU_HIP(hipStreamSynchronize(ctx->c.stream));
+  });
+}
+
+int rgpu_get_expansion(rgpu_context* ctx, double* coeffs, uint64_t* key_mask) {
+  return guarded([&] {
+    if (!ctx) throw InvalidArgument("null context");
+    Context& c = ctx->c;
+    require_batch(c);
+    ensure_rep(c);
+    const int64_t E = int64_t(c.species.size()) * c.rd.N * c.rd.LL;
+    if (coeffs && c.n_atoms) {
+      if (c.precision == RGPU_FP64) {
+        RGPU_HIP(hipMemcpyAsync(coeffs, c.c64.get(), sizeof(double) * c.n_atoms * E, hipMemcpyDefault,
+                                c.stream));
+      } else {
+        double* o = c.out64.reserve(size_t(c.n_atoms * E));
+        launch_normalise<float>(c.n_atoms, E, c.c32.get(), nullptr, false, o, c.stream);
+        RGPU_HIP(hipMemcpyAsync(coeffs, o, sizeof(double) * c.n_atoms * E, hipMemcpyDefault, c.stream));
+      }
+    }
+    if (key_mask && c.n_atoms)
+      RGPU_HIP(hipMemcpyAsync(key_mask, c.keymask.get(), sizeof(uint64_t) * c.n_atoms, hipMemcpyDefault,
+                              c.stream));
+    RGPU_HIP(hipStreamSynchronize(c.stream));
+  });
+}
+
+int rgpu_get_features(rgpu_context* ctx, double* features, double* inv_norms) {
+  return guarded([&] {
+    if (!ctx) throw InvalidArgument("null context");
+    Context& c = ctx->c;
+    require_batch(c);
+    require_invariants(c);
+    ensure_rep(c);
+    const int64_t D = c.D(), n = c.n_atoms;
+    if (features && n) {
+      double* o = c.out64.reserve(size_t(n * D));
+      if (c.precision == RGPU_FP64)
+        launch_normalise<double>(n, D, c.p64.get(), c.norm2.get(), c.hp.normalize, o, c.stream);
+      else
+        launch_normalise<float>(n, D, c.p32.get(), c.norm2.get(), c.hp.normalize, o, c.stream);
+      RGPU_HIP(hipMemcpyAsync(features, o, sizeof(double) * n * D, hipMemcpyDefault, c.stream));
+    }
+    if (inv_norms && n) {
+      std::vector<double> h(static_cast<size_t>(n));
+      RGPU_HIP(hipMemcpyAsync(h.data(), c.norm2.get(), sizeof(double) * n, hipMemcpyDeviceToHost, c.stream));
+      RGPU_HIP(hipStreamSynchronize(c.stream));
+      for (auto& v : h) v = (c.hp.normalize && v > 0) ? 1.0 / std::sqrt(v) : 1.0;
+      RGPU_HIP(hipMemcpyAsync(inv_norms, h.data(), sizeof(double) * n, hipMemcpyDefault, c.stream));
+    }
+    RGPU_HIP(hipStreamSynchronize(c.stream));
+  });
+}
+
+int rgpu_features_backward(rgpu_context* ctx, const double* dL_dX, double* dL_dr, double* neg_stress) {
+  return guarded([&] {
+    if (!ctx || !dL_dX) throw InvalidArgument("null argument");
+    Context& c = ctx->c;
+    require_batch(c);
+    require_invariants(c);
+    ensure_rep(c);
+    const int64_t D = c.D(), n = c.n_atoms;
+    double* gd = c.grad.reserve(size_t(std::max<int64_t>(3 * n, 1)));
+    double* cv = c.cvir.reserve(size_t(std::max<int64_t>(6 * n, 1)));
+    RGPU_HIP(hipMemsetAsync(gd, 0, sizeof(double) * 3 * n, c.stream));
+    double* g64 = c.tmp64.reserve(size_t(std::max<int64_t>(n * D, 1)));
+    RGPU_HIP(hipMemcpyAsync(g64, dL_dX, sizeof(double) * n * D, hipMemcpyDefault, c.stream));
+    if (c.precision == RGPU_FP64) {
+      backward_device<double>(c, g64, gd, cv);
+    } else {
+      std::vector<double> h(static_cast<size_t>(n * D));
+      RGPU_HIP(hipMemcpyAsync(h.data(), g64, sizeof(double) * h.size(), hipMemcpyDeviceToHost, c.stream));
+      RGPU_HIP(hipStreamSynchronize(c.stream));
+      std::vector<float> hf(h.begin(), h.end());
+      float* g32 = c.G32.reserve(size_t(std::max<int64_t>(n * D, 1)));
+      RGPU_HIP(hipMemcpyAsync(g32, hf.data(), sizeof(float) * hf.size(), hipMemcpyHostToDevice, c.stream));
+      RGPU_HIP(hipStreamSynchronize(c.stream));
+      backward_device<float>(c, g32, gd, cv);
+    }
+    emit_forces(c, gd, cv, dL_dr, neg_stress);
+  });
+}
+
+int rgpu_set_sparse_points(rgpu_context* ctx, int zeta, int64_t n_sparse, const int32_t* species,
+                           const double* values, const double* weights) {
+  return guarded([&] {
+    if (!ctx) throw InvalidArgument("null context");
+    Context& c = ctx->c;
+    require_batch(c);
+    require_invariants(c);
+    if (zeta < 1) throw InvalidArgument("zeta should be specified for the GAP kernel (>= 1)");
+    if (n_sparse < 0 || (n_sparse > 0 && (!species || !values))) throw InvalidArgument("bad sparse points");
+    const int S = int(c.species.size());
+    const int64_t D = c.D();
+    // species-sorted (stable) order
+    std::vector<int64_t> order(static_cast<size_t>(n_sparse));
+    for (int64_t m = 0; m < n_sparse; ++m) order[m] = m;
+    std::vector<int> sidx(static_cast<size_t>(n_sparse));
+    for (int64_t m = 0; m < n_sparse; ++m) {
+      auto it = std::lower_bound(c.species.begin(), c.species.end(), species[m]);
+      sidx[m] = (it == c.species.end() || *it != species[m]) ? S : int(it - c.species.begin());
+    }
+    std::stable_sort(order.begin(), order.end(), [&](int64_t a, int64_t b) { return sidx[a] < sidx[b]; });
+    c.sp_order = order;
+    c.sp_species.resize(size_t(n_sparse));
+    c.sp_off.assign(S + 1, 0);
+    for (int64_t m = 0; m < n_sparse; ++m) {
+      c.sp_species[m] = species[order[m]];
+      if (sidx[order[m]] < S) c.sp_off[sidx[order[m]] + 1]++;
+    }
+    for (int s = 0; s < S; ++s) c.sp_off[s + 1] += c.sp_off[s];
+    std::vector<double> T(size_t(n_sparse * D)), a(size_t(std::max<int64_t>(n_sparse, 1)), 0.0);
+    for (int64_t m = 0; m < n_sparse; ++m) {
+      std::copy(values + order[m] * D, values + (order[m] + 1) * D, T.begin() + m * D);
+      if (weights) a[m] = weights[order[m]];
+    }
+    double* dT = c.T64.reserve(std::max<size_t>(T.size(), 1));
+    if (!T.empty()) RGPU_HIP(hipMemcpy(dT, T.data(), sizeof(double) * T.size(), hipMemcpyHostToDevice));
+    std::vector<float> Tf(T.begin(), T.end());
+    float* dTf = c.T32.reserve(std::max<size_t>(Tf.size(), 1));
+    if (!Tf.empty()) RGPU_HIP(hipMemcpy(dTf, Tf.data(), sizeof(float) * Tf.size(), hipMemcpyHostToDevice));
+    double* da = c.alpha.reserve(a.size());
+    RGPU_HIP(hipMemcpy(da, a.data(), sizeof(double) * a.size(), hipMemcpyHostToDevice));
+    c.zeta = zeta;
+    c.M = n_sparse;
+    c.have_weights = weights != nullptr;
+    prepare_mixed_sparse(c.mixed_sparse, T, n_sparse, D, c.stream);
+    c.have_sparse = true;
+  });
+}
+
+int rgpu_gap_knm(rgpu_context* ctx, int target, double* knm) {
+  return guarded([&] {
+    if (!ctx || !knm) throw InvalidArgument("null argument");
+    Context& c = ctx->c;
+    require_batch(c);
+    if (!c.have_sparse) throw InvalidArgument("no sparse points set");
+    if (target != 0 && target != 1) throw InvalidArgument("target must be 0 (Structure) or 1 (Atom)");
+    ensure_rep(c);
+    if (c.precision == RGPU_FP64)
+      gap_knm_t<double>(c, target, knm);
+    else
+      gap_knm_t<float>(c, target, knm);
+  });
+}
+
+int rgpu_gap_predict(rgpu_context* ctx, double* energy, double* gradient, double* neg_stress) {
+  return guarded([&] {
+    if (!ctx) throw InvalidArgument("null context");
+    Context& c = ctx->c;
+    require_batch(c);
+    if (!c.have_sparse) throw InvalidArgument("no sparse points set");
+    if (!c.have_weights) throw InvalidArgument("sparse points were set without weights");
+    ensure_rep(c);
+    if (c.precision == RGPU_FP64)
+      gap_predict_t<double>(c, energy, gradient, neg_stress);
+    else
+      gap_predict_t<float>(c, energy, gradient, neg_stress);
+  });
+}
+
+int rgpu_eval_radial(rgpu_context* ctx, int64_t n, const double* r, double* R, double* dR) {
+  return guarded([&] {
+    if (!ctx || !r || !R) throw InvalidArgument("null argument");
+    Context& c = ctx->c;
+    const int64_t F = int64_t(c.rd.N) * c.rd.L1;
+    DeviceBuffer<double> dr, dv, dd;
+    double* pr = dr.reserve(size_t(std::max<int64_t>(n, 1)));
+    RGPU_HIP(hipMemcpy(pr, r, sizeof(double) * n, hipMemcpyDefault));
+    double* pv = dv.reserve(size_t(std::max<int64_t>(n * F, 1)));
+    double* pd = dR ? dd.reserve(size_t(std::max<int64_t>(n * F, 1))) : nullptr;
+    if (c.rd.use_spline) {
+      if (c.precision == RGPU_FP64)
+        launch_radial_spline_eval<double>(c.rd, pr, n, pv, pd, c.stream);
+      else
+        launch_radial_spline_eval<float>(c.rd, pr, n, pv, pd, c.stream);
+    } else {
+      launch_radial_direct(c.rd, pr, n, pv, pd, c.stream);
+    }
+    RGPU_HIP(hipStreamSynchronize(c.stream));
+    RGPU_HIP(hipMemcpy(R, pv, sizeof(double) * n * F, hipMemcpyDefault));
+    if (dR) RGPU_HIP(hipMemcpy(dR, pd, sizeof(double) * n * F, hipMemcpyDefault));
+  });
+}
+
+int rgpu_eval_ylm(rgpu_context* ctx, int64_t n, const double* dirs, double* Y, double* dY) {
+  return guarded([&] {
+    if (!ctx || !dirs || !Y) throw InvalidArgument("null argument");
+    Context& c = ctx->c;
+    const int64_t LL = c.rd.LL;
+    DeviceBuffer<double> dd, dy, ddy;
+    double* pd = dd.reserve(size_t(std::max<int64_t>(3 * n, 1)));
+    RGPU_HIP(hipMemcpy(pd, dirs, sizeof(double) * 3 * n, hipMemcpyDefault));
+    double* py = dy.reserve(size_t(std::max<int64_t>(n * LL, 1)));
+    double* pdy = dY ? ddy.reserve(size_t(std::max<int64_t>(3 * n * LL, 1))) : nullptr;
+    if (c.precision == RGPU_FP64)
+      launch_ylm_eval<double>(c.rd.L, pd, n, py, pdy, c.stream);
+    else
+      launch_ylm_eval<float>(c.rd.L, pd, n, py, pdy, c.stream);
+    RGPU_HIP(hipStreamSynchronize(c.stream));
+    RGPU_HIP(hipMemcpy(Y, py, sizeof(double) * n * LL, hipMemcpyDefault));
+    if (dY) RGPU_HIP(hipMemcpy(dY, pdy, sizeof(double) * 3 * n * LL, hipMemcpyDefault));
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,45 @@
+// Growable device buffer owned by a context (hipMalloc / hipFree).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+
+#include "common.hh"
+
+namespace rgpu {
+
+template <class T>
+class DeviceBuffer {
+ public:
+  DeviceBuffer() = default;
+  DeviceBuffer(const DeviceBuffer&) = delete;
+  DeviceBuffer& operator=(const DeviceBuffer&) = delete;
+  ~DeviceBuffer() { release(); }
+
+  // Ensure capacity for n elements (contents are not preserved on growth).
+  T* reserve(size_t n) {
+    if (n > cap_) {
+      release();
+      size_t bytes = (n == 0 ? 1 : n) * sizeof(T);
+      RGPU_HIP(hipMalloc(reinterpret_cast<void**>(&ptr_), bytes));
+      cap_ = n;
+    }
+    size_ = n;
+    return ptr_;
+  }
+  void release() {
+    if (ptr_) (void)hipFree(ptr_);
+    ptr_ = nullptr;
+    cap_ = size_ = 0;
+  }
+  T* get() const { return ptr_; }
+  size_t size() const { return size_; }
+  size_t bytes() const { return size_ * sizeof(T); }
+
+ private:
+  T* ptr_ = nullptr;
+  size_t cap_ = 0, size_ = 0;
+};
+
+}  // namespace rgpu

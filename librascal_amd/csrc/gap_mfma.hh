@@ -1,0 +1,31 @@
+// Mixed-precision sparse GAP kernel on the gfx950 matrix cores: BF16 hi+lo
+// split operands (x = hi + lo, three products hi.hi + hi.lo + lo.hi) with
+// FP32 accumulation, fused zeta-power / energy / weight epilogue.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "device_buffer.hh"
+#include "gap_kernels.hh"
+
+namespace rgpu {
+
+// Sparse points pre-split into BF16 hi/lo planes, in the two layouts the
+// two GEMMs read: [M][D] (K-contiguous rows for K = X T^T) and [D][M]
+// (K-contiguous columns for T~ = W~ T).
+struct MixedSparse {
+  int64_t M = 0, D = 0;
+  DeviceBuffer<unsigned short> t_hi, t_lo;    // [M][D]
+  DeviceBuffer<unsigned short> tt_hi, tt_lo;  // [D][M]
+};
+
+bool mfma_available();
+void prepare_mixed_sparse(MixedSparse& ms, const std::vector<double>& T, int64_t M, int64_t D,
+                          hipStream_t stream);
+// energy (per centre, +=), wt [n_rows][n_cols] and G rows (when non-null)
+void run_gap_mixed(const GapBlock<float>& blk, const MixedSparse& ms, double* energy, float* wt,
+                   float* G, hipStream_t stream);
+
+}  // namespace rgpu

@@ -1,0 +1,117 @@
+#include "hypers.hh"
+
+#include <algorithm>
+#include <set>
+
+#include "common.hh"
+
+namespace rgpu {
+
+namespace {
+double unit_value(const json::Value& v) {
+  // {"value": x, "unit": "AA"} or a bare number
+  if (v.is_object()) return v.at("value").as_number();
+  return v.as_number();
+}
+}  // namespace
+
+SoapHypers parse_soap_hypers(const std::string& text) {
+  json::Value h;
+  try {
+    h = json::parse(text);
+  } catch (const std::invalid_argument& e) {
+    throw InvalidArgument(e.what());
+  }
+  SoapHypers o;
+  o.json = text;
+  try {
+    o.max_radial = static_cast<int>(h.at("max_radial").as_number());
+    o.max_angular = static_cast<int>(h.at("max_angular").as_number());
+    if (h.has("soap_type")) {
+      const std::string& t = h.at("soap_type").as_string();
+      if (t == "PowerSpectrum") {
+        o.kind = SoapHypers::kPowerSpectrum;
+      } else if (t == "RadialSpectrum" || t == "BiSpectrum") {
+        throw InvalidArgument("soap_type '" + t +
+                              "' is not provided by this library (PowerSpectrum only)");
+      } else {
+        throw InvalidArgument("Requested SphericalInvariants type '" + t +
+                              "' has not been implemented.  Must be one of: "
+                              "'PowerSpectrum', 'RadialSpectrum', or 'BiSpectrum'.");
+      }
+      o.normalize = h.at("normalize").as_bool();
+      if (h.has("coefficient_subselection"))
+        throw InvalidArgument("coefficient_subselection is not supported by this library");
+    } else {
+      o.kind = SoapHypers::kExpansion;
+    }
+    if (h.has("compute_gradients")) o.compute_gradients = h.at("compute_gradients").as_bool();
+    if (h.has("expansion_by_species_method")) {
+      const std::string& m = h.at("expansion_by_species_method").as_string();
+      if (m != "environment wise" && m != "user defined" && m != "structure wise")
+        throw InvalidArgument("expansion_by_species_method provided:'" + m +
+                              "' is not part of the implemented methods");
+      o.expansion_by_species = m;
+    }
+    if (h.has("global_species")) {
+      for (auto& v : h.at("global_species").arr)
+        o.global_species.push_back(static_cast<int>(v.as_number()));
+    } else if (o.expansion_by_species == "user defined") {
+      throw InvalidArgument(
+          "expansion_by_species_method is 'user defined' but global_species is not defined.");
+    }
+    const json::Value& gd = h.at("gaussian_density");
+    const std::string& gt = gd.at("type").as_string();
+    if (gt != "Constant")
+      throw InvalidArgument("Requested Smearing type '" + gt +
+                            "' has not been implemented.  Must be one of: 'Constant'.");
+    o.gaussian_sigma = unit_value(gd.at("gaussian_sigma"));
+    const json::Value& rcn = h.at("radial_contribution");
+    const std::string& rt = rcn.at("type").as_string();
+    if (rt != "GTO")
+      throw InvalidArgument("Requested Radial contribution type '" + rt +
+                            "' is not provided by this library (GTO only)");
+    o.optimization = 0;
+    if (rcn.has("optimization")) {
+      const json::Value& op = rcn.at("optimization");
+      bool spline = op.has("Spline"), rdr = op.has("RadialDimReduction");
+      if (rdr && !spline) throw InvalidArgument("RadialDimReduction requires' Spline to be set.");
+      if (rdr) throw InvalidArgument("RadialDimReduction is not provided by this library");
+      if (spline) {
+        o.optimization = 1;
+        const json::Value& sp = op.at("Spline");
+        if (!sp.has("accuracy")) throw InvalidArgument("No Spline accuracy was given.");
+        o.spline_accuracy = sp.at("accuracy").as_number();
+      }
+    }
+    const json::Value& fc = h.at("cutoff_function");
+    const std::string& ft = fc.at("type").as_string();
+    o.cutoff = unit_value(fc.at("cutoff"));
+    o.smooth_width = unit_value(fc.at("smooth_width"));
+    if (ft == "ShiftedCosine") {
+      o.cutoff_type = 0;
+    } else if (ft == "RadialScaling") {
+      o.cutoff_type = 1;
+      o.rs_rate = unit_value(fc.at("rate"));
+      if (o.rs_rate < 0) throw RuntimeError("RadialScaling's rate should be positive");
+      o.rs_exponent = unit_value(fc.at("exponent"));
+      o.rs_scale = unit_value(fc.at("scale"));
+    } else {
+      throw InvalidArgument("Requested cutoff function type '" + ft +
+                            "' has not been implemented.  Must be one of: 'ShiftedCosine' or "
+                            "'RadialScaling'.");
+    }
+  } catch (const std::invalid_argument& e) {
+    throw InvalidArgument(e.what());
+  }
+  if (o.max_radial < 1 || o.max_radial > 24) throw InvalidArgument("max_radial must be in [1, 24]");
+  if (o.max_angular < 0 || o.max_angular > 16)
+    throw InvalidArgument("max_angular must be in [0, 16]");
+  if (!(o.cutoff > 0)) throw InvalidArgument("cutoff must be positive");
+  if (!(o.gaussian_sigma > 0)) throw InvalidArgument("gaussian_sigma must be positive");
+  if (o.optimization == 1 && !(o.spline_accuracy > 0))
+    throw InvalidArgument("Error bound must be > 0");
+  return o;
+}
+
+}  // namespace rgpu

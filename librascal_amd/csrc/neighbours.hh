@@ -1,0 +1,62 @@
+// Linked-cell full neighbour list on the GPU with the self pair first and a
+// strict cutoff, for a ragged batch of structures.
+//
+// Reproduces the pair sets and pair order of the reference stack
+// AdaptorNeighbourList -> AdaptorCenterContribution -> AdaptorStrict
+// (structure_managers/adaptor_neighbour_list.hh:904-1118,
+//  adaptor_center_contribution.hh:334-403, adaptor_strict.hh:373-455).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "common.hh"
+#include "device_buffer.hh"
+
+namespace rgpu {
+
+struct PairList {
+  int64_t n_atoms = 0;
+  int64_t n_pairs = 0;
+  int64_t n_ext = 0;      // atoms + ghosts
+  int64_t n_boxes = 0;
+  int n_struct = 0;
+  std::vector<NLGeom> geom_host;
+  DeviceBuffer<NLGeom> geom;
+  DeviceBuffer<int> atom_struct;     // [n_atoms]
+  DeviceBuffer<double> pos;          // [n_atoms][3]
+  DeviceBuffer<int> ghost_count;     // [n_atoms]
+  DeviceBuffer<int64_t> ghost_off;   // [n_atoms+1]
+  DeviceBuffer<double> ext_pos;      // [n_ext][3]
+  DeviceBuffer<int> ext_orig;        // [n_ext] original atom (global index)
+  DeviceBuffer<int64_t> ext_box;     // [n_ext] global box index
+  DeviceBuffer<int> box_count;       // [n_boxes]
+  DeviceBuffer<int64_t> box_start;   // [n_boxes+1]
+  DeviceBuffer<int> box_fill;        // [n_boxes]
+  DeviceBuffer<int> box_items;       // [n_ext]
+  DeviceBuffer<int> pair_count;      // [n_atoms]
+  DeviceBuffer<int64_t> centre_off;  // [n_atoms+1]
+  DeviceBuffer<int> pair_j;          // [n_pairs] original atom, global index
+  DeviceBuffer<double> pair_vec;     // [n_pairs][3]  r_j(image) - r_i
+  DeviceBuffer<double> pair_dist;    // [n_pairs]
+  DeviceBuffer<int> err;             // [1]
+  DeviceBuffer<int64_t> scratch;
+};
+
+// Host inputs: positions [n_atoms][3], cells [n_struct][9] (lattice vectors
+// as columns), pbc [n_struct][3], atom_offsets [n_struct+1].  Validates the
+// structures (cell not zero, atoms inside the cell within 1e-10) and builds
+// the list on `stream` (synchronises for sizes).
+void build_pair_list(PairList& pl, int n_struct, const int64_t* atom_offsets,
+                     const double* positions, const double* cells, const int* pbc,
+                     double cutoff, hipStream_t stream);
+
+// Per-structure linked-cell geometry (host, FP64).
+NLGeom make_geometry(const double* cell, const int* pbc, double cutoff);
+
+// Structure checks of structure_manager_centers.cc:67-99 and
+// atomic_structure.hh:249-255 (throws RuntimeError).
+void check_structure(int n_atoms, const double* pos, const double* cell);
+
+}  // namespace rgpu

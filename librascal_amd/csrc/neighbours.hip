@@ -1,0 +1,392 @@
+// Linked-cell neighbour list kernels.  Built with -ffp-contract=off: every
+// geometric quantity (ghost image positions, box indices, squared distances)
+// is rounded exactly as the reference's separate multiply/add sequence so the
+// inclusive cutoff test takes the same decisions.
+#include <algorithm>
+#include <cmath>
+
+#include "neighbours.hh"
+#include "scan.hh"
+
+namespace rgpu {
+
+namespace {
+
+void inverse3(const double* C, double* Ci) {
+  double a = C[0], b = C[1], c = C[2], d = C[3], e = C[4], f = C[5], g = C[6], h = C[7],
+         i = C[8];
+  double A = e * i - f * h, B = -(d * i - f * g), Cc = d * h - e * g;
+  double id = 1.0 / (a * A + b * B + c * Cc);
+  Ci[0] = A * id;
+  Ci[3] = B * id;
+  Ci[6] = Cc * id;
+  Ci[1] = -(b * i - c * h) * id;
+  Ci[4] = (a * i - c * g) * id;
+  Ci[7] = -(a * h - b * g) * id;
+  Ci[2] = (b * f - c * e) * id;
+  Ci[5] = -(a * f - c * d) * id;
+  Ci[8] = (a * e - b * d) * id;
+}
+
+__device__ __forceinline__ void box_of(const NLGeom& g, const double* x, double rc, int* b) {
+  for (int d = 0; d < 3; ++d) {
+    double v = x[d] - g.mesh_min[d];
+    int k = int(floor(v / rc));
+    b[d] = k < 1 ? 1 : k;
+  }
+}
+
+__device__ __forceinline__ int64_t box_lin(const NLGeom& g, const int* b) {
+  return g.box0 + (int64_t(b[0]) * g.nboxes[1] + b[1]) * g.nboxes[2] + b[2];
+}
+
+// ghost images of atom a that fall inside the ghost bounds
+// (adaptor_neighbour_list.hh:1044-1072); count or write
+template <bool WRITE>
+__global__ void ghosts_kernel(const NLGeom* __restrict__ geom, const int* __restrict__ atom_struct,
+                              const double* __restrict__ pos, int64_t n_atoms,
+                              int* __restrict__ count, const int64_t* __restrict__ off,
+                              double* __restrict__ ext_pos, int* __restrict__ ext_orig) {
+  int64_t a = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (a >= n_atoms) return;
+  const NLGeom& g = geom[atom_struct[a]];
+  const double x0 = pos[3 * a], x1 = pos[3 * a + 1], x2 = pos[3 * a + 2];
+  int c = 0;
+  int64_t w = WRITE ? off[a] + n_atoms : 0;
+  for (int idx = 0; idx < g.n_images; ++idx) {
+    int p[3];
+    int f = 1;
+    for (int i = 2; i >= 0; --i) {
+      p[i] = idx / f % g.rep[i] + g.pmin[i];
+      f *= g.rep[i];
+    }
+    if (p[0] == 0 && p[1] == 0 && p[2] == 0) continue;
+    double xg[3];
+    const double xs[3] = {x0, x1, x2};
+    bool inside = true;
+    for (int d = 0; d < 3; ++d) {
+      double t = g.cell[d * 3 + 0] * double(p[0]);
+      t = t + g.cell[d * 3 + 1] * double(p[1]);
+      t = t + g.cell[d * 3 + 2] * double(p[2]);
+      xg[d] = xs[d] + t;
+      if (!((xg[d] - g.ghost_min[d]) > g.tol && (xg[d] - g.ghost_max[d]) < -g.tol)) inside = false;
+    }
+    if (!inside) continue;
+    if (WRITE) {
+      ext_pos[3 * (w + c)] = xg[0];
+      ext_pos[3 * (w + c) + 1] = xg[1];
+      ext_pos[3 * (w + c) + 2] = xg[2];
+      ext_orig[w + c] = int(a);
+    }
+    ++c;
+  }
+  if (!WRITE) count[a] = c;
+}
+
+__global__ void originals_kernel(const double* __restrict__ pos, int64_t n_atoms,
+                                 double* __restrict__ ext_pos, int* __restrict__ ext_orig) {
+  int64_t a = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (a >= n_atoms) return;
+  ext_pos[3 * a] = pos[3 * a];
+  ext_pos[3 * a + 1] = pos[3 * a + 1];
+  ext_pos[3 * a + 2] = pos[3 * a + 2];
+  ext_orig[a] = int(a);
+}
+
+// box of every extended slot (adaptor_neighbour_list.hh:1076-1083,
+// :395-422 range check)
+__global__ void bin_count_kernel(const NLGeom* __restrict__ geom, const int* __restrict__ atom_struct,
+                                 const double* __restrict__ ext_pos, const int* __restrict__ ext_orig,
+                                 int64_t n_ext, double rc, int64_t* __restrict__ ext_box,
+                                 int* __restrict__ box_count, int* __restrict__ err) {
+  int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t >= n_ext) return;
+  const NLGeom& g = geom[atom_struct[ext_orig[t]]];
+  int b[3];
+  box_of(g, ext_pos + 3 * t, rc, b);
+  for (int d = 0; d < 3; ++d)
+    if (b[d] >= g.nboxes[d] - 1 || b[d] <= 0) {
+      atomicOr(err, 1);
+      ext_box[t] = -1;
+      return;
+    }
+  int64_t bl = box_lin(g, b);
+  ext_box[t] = bl;
+  atomicAdd(box_count + bl, 1);
+}
+
+__global__ void bin_fill_kernel(const int64_t* __restrict__ ext_box, int64_t n_ext,
+                                const int64_t* __restrict__ box_start, int* __restrict__ box_fill,
+                                int* __restrict__ box_items) {
+  int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t >= n_ext) return;
+  int64_t bl = ext_box[t];
+  if (bl < 0) return;
+  int k = atomicAdd(box_fill + bl, 1);
+  box_items[box_start[bl] + k] = int(t);
+}
+
+// ascending slot order within each box = the reference's tag order
+__global__ void bin_sort_kernel(const int64_t* __restrict__ box_start, int64_t n_boxes,
+                                int* __restrict__ box_items) {
+  int64_t bx = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (bx >= n_boxes) return;
+  int64_t s = box_start[bx], e = box_start[bx + 1];
+  for (int64_t i = s + 1; i < e; ++i) {
+    int v = box_items[i];
+    int64_t j = i - 1;
+    while (j >= s && box_items[j] > v) {
+      box_items[j + 1] = box_items[j];
+      --j;
+    }
+    box_items[j + 1] = v;
+  }
+}
+
+// One wavefront per centre: 27-box stencil (last dimension fastest,
+// adaptor_neighbour_list.hh:92-158), strict inclusive cutoff on the squared
+// distance (adaptor_strict.hh:414-427).  Pass 1 counts, pass 2 writes in
+// stencil/box order, preceded by the self pair.
+template <bool WRITE>
+__global__ __launch_bounds__(256) void pairs_kernel(
+    const NLGeom* __restrict__ geom, const int* __restrict__ atom_struct,
+    const double* __restrict__ ext_pos, const int* __restrict__ ext_orig,
+    const int64_t* __restrict__ box_start, const int* __restrict__ box_items, int64_t n_atoms,
+    double rc, int* __restrict__ pair_count, const int64_t* __restrict__ centre_off,
+    int* __restrict__ pair_j, double* __restrict__ pair_vec, double* __restrict__ pair_dist) {
+  const int lane = threadIdx.x & 63;
+  const int64_t i = int64_t(blockIdx.x) * (blockDim.x / 64) + (threadIdx.x >> 6);
+  if (i >= n_atoms) return;
+  const NLGeom& g = geom[atom_struct[i]];
+  const double xi0 = ext_pos[3 * i], xi1 = ext_pos[3 * i + 1], xi2 = ext_pos[3 * i + 2];
+  const double xi[3] = {xi0, xi1, xi2};
+  int b[3];
+  box_of(g, xi, rc, b);
+  const double rc2 = rc * rc;
+  int64_t w = WRITE ? centre_off[i] : 0;
+  if (WRITE && lane == 0) {
+    pair_j[w] = int(i);
+    pair_vec[3 * w] = 0.0;
+    pair_vec[3 * w + 1] = 0.0;
+    pair_vec[3 * w + 2] = 0.0;
+    pair_dist[w] = 0.0;
+  }
+  int64_t n = 1;
+  for (int s = 0; s < 27; ++s) {
+    int c[3];
+    int f = 1;
+    for (int d = 2; d >= 0; --d) {
+      c[d] = s / f % 3 + b[d] - 1;
+      f *= 3;
+    }
+    int64_t bl = box_lin(g, c);
+    int64_t s0 = box_start[bl], s1 = box_start[bl + 1];
+    for (int64_t q0 = s0; q0 < s1; q0 += 64) {
+      int64_t q = q0 + lane;
+      bool keep = false;
+      int t = -1;
+      double v0 = 0, v1 = 0, v2 = 0, d2 = 0;
+      if (q < s1) {
+        t = box_items[q];
+        if (t != int(i)) {
+          v0 = ext_pos[3 * t] - xi0;
+          v1 = ext_pos[3 * t + 1] - xi1;
+          v2 = ext_pos[3 * t + 2] - xi2;
+          double a0 = v0 * v0;
+          double a1 = v1 * v1;
+          double a2 = v2 * v2;
+          d2 = (a0 + a1) + a2;
+          keep = d2 <= rc2;
+        }
+      }
+      unsigned long long mask = __ballot(keep);
+      if (WRITE && keep) {
+        unsigned long long below = mask & ((1ull << lane) - 1ull);
+        int64_t o = w + n + __popcll(below);
+        pair_j[o] = ext_orig[t];
+        pair_vec[3 * o] = v0;
+        pair_vec[3 * o + 1] = v1;
+        pair_vec[3 * o + 2] = v2;
+        pair_dist[o] = sqrt(d2);
+      }
+      n += __popcll(mask);
+    }
+  }
+  if (!WRITE && lane == 0) pair_count[i] = int(n);
+}
+
+}  // namespace
+
+NLGeom make_geometry(const double* cell, const int* pbc, double rc) {
+  NLGeom g{};
+  for (int k = 0; k < 9; ++k) g.cell[k] = cell[k];
+  for (int i = 0; i < 3; ++i) {
+    double mn = 0.0, mx = 0.0;
+    for (int col = 0; col < 3; ++col) {
+      double v = cell[i * 3 + col];
+      mn += v * static_cast<double>(v < 0.);
+      mx += v * static_cast<double>(v > 0.);
+    }
+    g.mesh_min[i] = mn - 2. * rc;
+    double lmesh = mx - mn + 4. * rc;
+    g.nboxes[i] = static_cast<int>(std::ceil(lmesh / rc));
+    g.ghost_min[i] = mn - rc;
+    double lghost = mx - mn + 2 * rc;
+    g.ghost_max[i] = std::ceil(lghost / rc) * rc + g.ghost_min[i];
+  }
+  double maxlen = std::max({g.ghost_max[0] - g.ghost_min[0], g.ghost_max[1] - g.ghost_min[1],
+                            g.ghost_max[2] - g.ghost_min[2]});
+  g.tol = maxlen * 1e-8;
+  double lo[3], hi[3], Ci[9];
+  for (int i = 0; i < 3; ++i) {
+    lo[i] = g.ghost_min[i] - g.tol;
+    hi[i] = g.ghost_max[i] + g.tol;
+  }
+  inverse3(cell, Ci);
+  double xmin[3] = {INFINITY, INFINITY, INFINITY}, xmax[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (int c = 0; c < 8; ++c) {
+    double x[3];
+    for (int i = 0; i < 3; ++i) x[i] = ((c >> i) & 1) ? hi[i] : lo[i];
+    for (int d = 0; d < 3; ++d) {
+      double m = Ci[d * 3 + 0] * x[0] + Ci[d * 3 + 1] * x[1] + Ci[d * 3 + 2] * x[2];
+      xmin[d] = std::min(xmin[d], m);
+      xmax[d] = std::max(xmax[d], m);
+    }
+  }
+  g.n_images = 1;
+  for (int i = 0; i < 3; ++i) {
+    int mmin = static_cast<int>(std::floor(xmin[i]));
+    int mmax = static_cast<int>(std::ceil(xmax[i])) - 1;
+    if (pbc[i]) {
+      g.pmin[i] = mmin;
+      g.rep[i] = mmax - mmin + 1;
+    } else {
+      g.pmin[i] = 0;
+      g.rep[i] = 1;
+    }
+    g.n_images *= g.rep[i];
+  }
+  return g;
+}
+
+void check_structure(int n_atoms, const double* pos, const double* cell) {
+  bool all_zero = true;
+  for (int k = 0; k < 9; ++k)
+    if (std::fabs(cell[k]) >= 1e-10) all_zero = false;
+  if (all_zero)
+    throw RuntimeError(
+        "The unit cell should not be filled with zeros but it should contain all the atoms "
+        "even if the structure is not periodic.");
+  double Ci[9];
+  inverse3(cell, Ci);
+  for (int a = 0; a < n_atoms; ++a)
+    for (int d = 0; d < 3; ++d) {
+      double s = Ci[d * 3] * pos[3 * a] + Ci[d * 3 + 1] * pos[3 * a + 1] + Ci[d * 3 + 2] * pos[3 * a + 2];
+      if (s < -1e-10 || s > 1. + 1e-10)
+        throw RuntimeError(
+            "Some of the atoms in the structure are not inside the unit cell. Please wrap them "
+            "inside it with at least tolerance of 1e-10.");
+    }
+}
+
+void build_pair_list(PairList& pl, int n_struct, const int64_t* atom_offsets,
+                     const double* positions, const double* cells, const int* pbc, double rc,
+                     hipStream_t stream) {
+  const int64_t n_atoms = atom_offsets[n_struct];
+  pl.n_struct = n_struct;
+  pl.n_atoms = n_atoms;
+  pl.geom_host.resize(n_struct);
+  std::vector<int> astruct(size_t(n_atoms > 0 ? n_atoms : 1));
+  int64_t boxes = 0;
+  for (int s = 0; s < n_struct; ++s) {
+    int64_t a0 = atom_offsets[s], a1 = atom_offsets[s + 1];
+    if (a1 < a0) throw InvalidArgument("atom_offsets must be non-decreasing");
+    check_structure(int(a1 - a0), positions + 3 * a0, cells + 9 * s);
+    NLGeom g = make_geometry(cells + 9 * s, pbc + 3 * s, rc);
+    g.atom0 = int(a0);
+    g.n_atoms = int(a1 - a0);
+    g.box0 = boxes;
+    boxes += int64_t(g.nboxes[0]) * g.nboxes[1] * g.nboxes[2];
+    pl.geom_host[s] = g;
+    for (int64_t a = a0; a < a1; ++a) astruct[a] = s;
+  }
+  pl.n_boxes = boxes;
+  if (n_atoms == 0) {
+    pl.n_pairs = 0;
+    pl.n_ext = 0;
+    pl.centre_off.reserve(1);
+    RGPU_HIP(hipMemsetAsync(pl.centre_off.get(), 0, sizeof(int64_t), stream));
+    return;
+  }
+  NLGeom* dgeom = pl.geom.reserve(n_struct);
+  RGPU_HIP(hipMemcpyAsync(dgeom, pl.geom_host.data(), sizeof(NLGeom) * n_struct,
+                          hipMemcpyHostToDevice, stream));
+  int* dstruct = pl.atom_struct.reserve(n_atoms);
+  RGPU_HIP(hipMemcpyAsync(dstruct, astruct.data(), sizeof(int) * n_atoms, hipMemcpyHostToDevice,
+                          stream));
+  double* dpos = pl.pos.reserve(3 * n_atoms);
+  RGPU_HIP(hipMemcpyAsync(dpos, positions, sizeof(double) * 3 * n_atoms, hipMemcpyHostToDevice,
+                          stream));
+  int* err = pl.err.reserve(1);
+  RGPU_HIP(hipMemsetAsync(err, 0, sizeof(int), stream));
+
+  const int TB = 256;
+  auto blocks = [](int64_t n, int tb) { return dim3(unsigned((n + tb - 1) / tb)); };
+  // ghosts
+  int* gcount = pl.ghost_count.reserve(n_atoms);
+  int64_t* goff = pl.ghost_off.reserve(n_atoms + 1);
+  hipLaunchKernelGGL(ghosts_kernel<false>, blocks(n_atoms, TB), dim3(TB), 0, stream, dgeom, dstruct,
+                     dpos, n_atoms, gcount, nullptr, nullptr, nullptr);
+  RGPU_LAUNCH_CHECK();
+  int64_t n_ghosts = exclusive_scan<int>(gcount, goff, n_atoms, stream, pl.scratch);
+  pl.n_ext = n_atoms + n_ghosts;
+  double* epos = pl.ext_pos.reserve(3 * pl.n_ext);
+  int* eorig = pl.ext_orig.reserve(pl.n_ext);
+  hipLaunchKernelGGL(originals_kernel, blocks(n_atoms, TB), dim3(TB), 0, stream, dpos, n_atoms, epos,
+                     eorig);
+  RGPU_LAUNCH_CHECK();
+  hipLaunchKernelGGL(ghosts_kernel<true>, blocks(n_atoms, TB), dim3(TB), 0, stream, dgeom, dstruct,
+                     dpos, n_atoms, nullptr, goff, epos, eorig);
+  RGPU_LAUNCH_CHECK();
+  // binning
+  int* bcount = pl.box_count.reserve(boxes);
+  int64_t* bstart = pl.box_start.reserve(boxes + 1);
+  int* bfill = pl.box_fill.reserve(boxes);
+  int* bitems = pl.box_items.reserve(pl.n_ext);
+  int64_t* ebox = pl.ext_box.reserve(pl.n_ext);
+  RGPU_HIP(hipMemsetAsync(bcount, 0, sizeof(int) * boxes, stream));
+  RGPU_HIP(hipMemsetAsync(bfill, 0, sizeof(int) * boxes, stream));
+  hipLaunchKernelGGL(bin_count_kernel, blocks(pl.n_ext, TB), dim3(TB), 0, stream, dgeom, dstruct, epos,
+                     eorig, pl.n_ext, rc, ebox, bcount, err);
+  RGPU_LAUNCH_CHECK();
+  int herr = 0;
+  RGPU_HIP(hipMemcpyAsync(&herr, err, sizeof(int), hipMemcpyDeviceToHost, stream));
+  RGPU_HIP(hipStreamSynchronize(stream));
+  if (herr)
+    throw RuntimeError(
+        "Error: this atom does not fall in one of the linked cell bin. It might have not been "
+        "wrapped properly.");
+  exclusive_scan<int>(bcount, bstart, boxes, stream, pl.scratch);
+  hipLaunchKernelGGL(bin_fill_kernel, blocks(pl.n_ext, TB), dim3(TB), 0, stream, ebox, pl.n_ext, bstart,
+                     bfill, bitems);
+  RGPU_LAUNCH_CHECK();
+  hipLaunchKernelGGL(bin_sort_kernel, blocks(boxes, TB), dim3(TB), 0, stream, bstart, boxes, bitems);
+  RGPU_LAUNCH_CHECK();
+  // pairs
+  int* pcount = pl.pair_count.reserve(n_atoms);
+  int64_t* coff = pl.centre_off.reserve(n_atoms + 1);
+  const int WPB = TB / 64;
+  dim3 pgrid(unsigned((n_atoms + WPB - 1) / WPB));
+  hipLaunchKernelGGL(pairs_kernel<false>, pgrid, dim3(TB), 0, stream, dgeom, dstruct, epos, eorig,
+                     bstart, bitems, n_atoms, rc, pcount, nullptr, nullptr, nullptr, nullptr);
+  RGPU_LAUNCH_CHECK();
+  pl.n_pairs = exclusive_scan<int>(pcount, coff, n_atoms, stream, pl.scratch);
+  int* pj = pl.pair_j.reserve(pl.n_pairs);
+  double* pv = pl.pair_vec.reserve(3 * pl.n_pairs);
+  double* pd = pl.pair_dist.reserve(pl.n_pairs);
+  hipLaunchKernelGGL(pairs_kernel<true>, pgrid, dim3(TB), 0, stream, dgeom, dstruct, epos, eorig, bstart,
+                     bitems, n_atoms, rc, nullptr, coff, pj, pv, pd);
+  RGPU_LAUNCH_CHECK();
+}
+
+}  // namespace rgpu

@@ -1,0 +1,78 @@
+// GTO radial basis tables shared by the host set-up and the HIP kernels.
+//
+// Host side (radial_host.cc): Gaussian-type-orbital constants, the overlap
+// orthonormalisation O = diag(norm) S^{-1/2}, the per-(n,l) Hyp1f1 series /
+// asymptotic coefficients and switching points, and the uniform cubic-spline
+// tables of O^T R(r).  Device side (soap_device.hh): evaluation of R(r) by
+// Hyp1f1 (FP64) or by the spline (FP64 or FP32), Y_lm and cutoff functions.
+#pragma once
+
+#include <vector>
+
+#include "common.hh"
+#include "hypers.hh"
+
+namespace rgpu {
+
+constexpr int kMaxN = 24;
+constexpr int kMaxL1 = 17;
+
+// One confluent hypergeometric function 1F1(a, b, z), a = (n+l+3)/2,
+// b = l + 3/2 (math/hyp1f1.hh:49-229).
+struct HypCoef {
+  double a, b;
+  double prefac_s;  // Gamma(a)/Gamma(b)
+  double prefac_a;  // Gamma(b)/Gamma(a)
+  double z_switch;  // series for z <= z_switch, asymptotic above
+  int is_exp;       // a == b: 1F1 = exp(z)
+  int is_n_and_l;   // 2(a-b) integer
+};
+
+// Everything a kernel needs to evaluate the radial part, passed by value.
+struct RadialDev {
+  int N, L, L1, LL;       // max_radial, max_angular, L+1, (L+1)^2
+  int use_spline;         // 1: spline table, 0: direct Hyp1f1
+  int recursion;          // downward l-recursion (L >= 3)
+  int cutoff_type;        // 0 ShiftedCosine, 1 RadialScaling
+  double rc, sw, rs_rate, rs_scale, rs_exponent;
+  double fac_a;           // 1 / (2 sigma^2)
+  // device arrays
+  const double* fac_b;    // [N]
+  const double* O;        // [N][N] = diag(norm) S^{-1/2}
+  const double* center;   // [N] centre term (finalised when use_spline)
+  const HypCoef* hyp;     // [N][L+1]
+  const double* hc;       // [N][L+1][2][kHypTerms] series coeff, derivative coeff
+  const double* ha;       // [N][L+1][2][kHypTerms] asymptotic coeff, derivative coeff
+  // spline on [x1, x2], grid_n points; row k = {y_k[F], d_k[F]} with
+  // d = y'' h^2/6, function index f = n*L1 + l
+  int grid_n, search_size;
+  double x1, h, per_unit;
+  const double* grid;     // [grid_n] grid points
+  const double* spl64;    // [grid_n][2F]
+  const float* spl32;     // [grid_n][2F]
+};
+
+// Host-side constants (all FP64).
+struct RadialHost {
+  int N = 0, L = 0;
+  double fac_a = 0;
+  std::vector<double> sigma, fac_b, norm, nfac, O, center_raw, center_fin;
+  std::vector<HypCoef> hyp;
+  std::vector<double> hc, ha;
+  // spline
+  int grid_n = 0;
+  double x1 = 0, x2 = 0, h = 0, per_unit = 0;
+  std::vector<double> spl64;
+
+  explicit RadialHost(const SoapHypers& hp);
+};
+
+// Natural cubic spline second derivatives (times h^2/6) of the columns of
+// y[n][F] on a uniform grid with spacing h (math/interpolator.cc:148-175).
+void spline_second_derivatives(int n, int F, double h, const std::vector<double>& y,
+                               std::vector<double>& d2);
+
+// Eigen 3.3 LinSpaced for floating point (used for the spline grids).
+std::vector<double> linspaced(int num, double lo, double hi);
+
+}  // namespace rgpu

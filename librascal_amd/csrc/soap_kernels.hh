@@ -1,0 +1,64 @@
+// Launch interface of the SOAP kernels (expansion, power spectrum, adjoint).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "radial.hh"
+
+namespace rgpu {
+
+// Pair records produced by the neighbour list, grouped per centre (self pair
+// first).
+struct PairView {
+  int64_t n_centres;
+  const int64_t* centre_off;  // [n_centres+1]
+  const int* pair_j;          // original atom (global index)
+  const double* pair_vec;     // [n_pairs][3]
+  const double* pair_dist;    // [n_pairs]
+  const int* species_of;      // [n_atoms] species index in the dense layout
+};
+
+// c[i][s][n][l*l+l+m] (finalised) and the per-centre key mask.
+template <class T>
+void launch_expansion(const RadialDev& rad, int S, const PairView& pv, T* c,
+                      unsigned long long* keymask, hipStream_t stream);
+
+// p[i][key(a<=b)][n1*N+n2][l] UN-normalised, and norm2[i] = |p_i|^2 (FP64).
+template <class T>
+void launch_power_spectrum(const RadialDev& rad, int S, int64_t n_centres, const T* c, T* p,
+                           double* norm2, hipStream_t stream);
+
+// Normalised copy X = p / |p| (or X = p), FP64 output for export.
+template <class T>
+void launch_normalise(int64_t n_centres, int64_t D, const T* p, const double* norm2, bool normalize,
+                      double* out, hipStream_t stream);
+
+// Adjoint: given G = dL/dX per centre (X the exported features), accumulate
+// dL/dr on atoms (grad[n_atoms][3], += ) and the structure virial
+// neg_virial[n_struct][6] (+= sum over pairs r_ji (x) dL/dr_j, Voigt
+// xx,yy,zz,yz,xz,xy, not divided by the volume).
+template <class T>
+void launch_backward(const RadialDev& rad, int S, const PairView& pv, const int* atom_struct,
+                     const T* c, const T* p, const double* norm2, bool normalize, const T* G,
+                     T* dLdc_scratch, double* grad, double* neg_virial, hipStream_t stream);
+
+// out[s][:w] += sum of in[a][:w] over a in [seg_off[s], seg_off[s+1])
+void launch_segment_sum(const double* in, int w, const int64_t* seg_off, int n_seg, double* out,
+                        hipStream_t stream);
+// raw radial integrals (FP64 Hyp1f1) raw[pt][n][l] (+ derivatives, may be null)
+void launch_radial_direct(const RadialDev& rad, const double* r, int64_t n_pts, double* raw,
+                          double* draw, hipStream_t stream);
+// out[pt][n][l] = sum_m O[m][n] raw[pt][m][l]
+void launch_radial_finalise(const RadialDev& rad, const double* raw, int64_t n_pts, double* out,
+                            hipStream_t stream);
+// spline lookup in precision T: R[pt][n][l], dR (may be null)
+template <class T>
+void launch_radial_spline_eval(const RadialDev& rad, const double* r, int64_t n_pts, double* R,
+                               double* dR, hipStream_t stream);
+// Y_lm (+ gradients) at unit vectors in precision T
+template <class T>
+void launch_ylm_eval(int L, const double* dirs, int64_t n, double* Y, double* dY, hipStream_t stream);
+
+}  // namespace rgpu

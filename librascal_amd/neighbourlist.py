@@ -1,0 +1,163 @@
+"""AtomsList: a batch of structures with its computed representation.
+
+Mirrors bindings/rascal/neighbourlist/structure_manager.py:35-330 for the
+SOAP path: structure conversion (ase.Atoms-like objects or dicts), the
+non-periodic bounding-box cell (sanitize_non_periodic_structure, :260-286),
+get_features / get_features_gradient / get_gradients_info with the
+reference's dense layouts.
+"""
+import numpy as np
+
+from .synthetic import pack
+
+
+def _as_structure(frame):
+    """dict(positions, numbers, cell (columns), pbc) from an ase.Atoms-like
+    object or a dict (ASE cell rows are transposed to columns)."""
+    if isinstance(frame, dict):
+        st = dict(positions=np.asarray(frame["positions"], np.float64).reshape(-1, 3),
+                  numbers=np.asarray(frame.get("numbers", frame.get("atom_types")),
+                                     np.int32).ravel(),
+                  cell=np.asarray(frame["cell"], np.float64).reshape(3, 3),
+                  pbc=np.asarray(frame["pbc"], np.int32).ravel())
+    elif all(hasattr(frame, a) for a in ("get_cell", "get_positions", "get_atomic_numbers",
+                                        "get_pbc")):
+        st = dict(positions=np.asarray(frame.get_positions(), np.float64),
+                  numbers=np.asarray(frame.get_atomic_numbers(), np.int32),
+                  cell=np.asarray(frame.get_cell(), np.float64).T.copy(),
+                  pbc=np.asarray(frame.get_pbc(), np.int32))
+    else:
+        raise RuntimeError("Cannot convert structure of type {}".format(type(frame)))
+    return sanitize_non_periodic_structure(st)
+
+
+def sanitize_non_periodic_structure(structure):
+    """Zero cell + no periodicity: bounding box x1.05 and recentred atoms
+    (structure_manager.py:260-286)."""
+    if np.all(np.asarray(structure["pbc"]) == 0):
+        cell = np.asarray(structure["cell"])
+        if np.allclose(cell, np.zeros((3, 3))):
+            pos = np.asarray(structure["positions"], np.float64)
+            lo, hi = pos.min(axis=0), pos.max(axis=0)
+            lengths = (hi - lo) * 1.05
+            com = pos.mean(axis=0)
+            structure = dict(structure)
+            structure["positions"] = pos + (0.5 * lengths - com)[None, :]
+            structure["cell"] = np.diag(lengths)
+    return structure
+
+
+class AtomsList:
+    """A collection of structures and their representation on the GPU."""
+
+    def __init__(self, frames, nl_options=None, start=None, length=None):
+        if hasattr(frames, "get_positions") or isinstance(frames, dict):
+            frames = [frames]
+        frames = list(frames)
+        if start is not None:
+            frames = frames[start:start + length if length is not None else None]
+        self.structures = [_as_structure(f) for f in frames]
+        self.nl_options = nl_options
+        (self.atom_offsets, self.positions, self.numbers, self.cells, self.pbc) = \
+            pack(self.structures)
+        self.calculator = None
+        self.species = None
+
+    def __len__(self):
+        return len(self.structures)
+
+    @property
+    def n_centres(self):
+        return int(self.atom_offsets[-1])
+
+    def compute(self, calculator, species=None):
+        sp = sorted(set(self.numbers.tolist()) | set(species or []))
+        ctx = calculator._context
+        ctx.set_structures(self.atom_offsets, self.positions, self.numbers, self.cells, self.pbc, sp)
+        ctx.compute()
+        self.calculator = calculator
+        self.species = sp
+        return self
+
+    def _require(self, calculator):
+        if calculator is None:
+            calculator = self.calculator
+        if calculator is None or calculator is not self.calculator:
+            raise ValueError("the representation of this AtomsList was computed with another "
+                             "calculator")
+        return calculator
+
+    def _present_keys(self, calculator):
+        """Keys present in at least one centre (dense export drops the rest,
+        property_block_sparse.hh:911-932 with the keys of the collection)."""
+        _, mask = calculator._context.expansion()
+        S = len(self.species)
+        present_species = np.zeros((len(mask), S), bool)
+        for s in range(S):
+            present_species[:, s] = (mask >> np.uint64(s)) & np.uint64(1)
+        if "soap_type" not in calculator.hypers:
+            return [s for s in range(S) if present_species[:, s].any()], present_species
+        keys = []
+        k = 0
+        for a in range(S):
+            for b in range(a, S):
+                if (present_species[:, a] & present_species[:, b]).any():
+                    keys.append(k)
+                k += 1
+        return keys, present_species
+
+    def get_features(self, calculator=None, species=None):
+        """Dense feature matrix (n_centres, n_keys * inner) with the keys of
+        the batch (or of the requested species list)."""
+        calculator = self._require(calculator)
+        ctx = calculator._context
+        N = calculator.hypers["max_radial"]
+        L1 = calculator.hypers["max_angular"] + 1
+        if "soap_type" in calculator.hypers:
+            X, _ = ctx.features()
+            inner = N * N * L1
+        else:
+            c, _ = ctx.expansion()
+            X = c.reshape(c.shape[0], -1)
+            inner = N * L1 * L1
+        nk = X.shape[1] // inner
+        X = X.reshape(X.shape[0], nk, inner)
+        if species is None:
+            keys, _ = self._present_keys(calculator)
+            return np.ascontiguousarray(X[:, keys, :].reshape(X.shape[0], -1))
+        return self._features_for_species(calculator, X, species)
+
+    def _features_for_species(self, calculator, X, species):
+        sp = list(self.species)
+        out_keys = calculator.get_keys(species)
+        inner = X.shape[2]
+        out = np.zeros((X.shape[0], len(out_keys), inner))
+        all_keys = calculator.get_keys(sp)
+        index = {k: i for i, k in enumerate(all_keys)}
+        for i, k in enumerate(out_keys):
+            if k in index:
+                out[:, i, :] = X[:, index[k], :]
+        return out.reshape(X.shape[0], -1)
+
+    def get_dense_features(self, calculator=None):
+        """Features over all (a<=b) keys of the batch species, no dropping."""
+        calculator = self._require(calculator)
+        X, _ = calculator._context.features()
+        return X
+
+    def get_neighbours(self, calculator=None):
+        calculator = self._require(calculator)
+        return calculator._context.neighbours()
+
+    def get_gradients_info(self, calculator=None):
+        """(n_pairs, 5) rows [structure, i, j, type_i, type_j] with centre and
+        neighbour indices within the structure, self pair first
+        (bind_py_structure_manager.cc:622-744)."""
+        calculator = self._require(calculator)
+        off, pj, _ = calculator._context.neighbours(with_vectors=False)
+        n = len(off) - 1
+        centre = np.repeat(np.arange(n), np.diff(off))
+        sidx = np.searchsorted(self.atom_offsets, centre, side="right") - 1
+        a0 = self.atom_offsets[sidx]
+        info = np.stack([sidx, centre - a0, pj - a0, self.numbers[centre], self.numbers[pj]], axis=1)
+        return info.astype(np.int32)

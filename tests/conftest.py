@@ -1,0 +1,77 @@
+import json
+import os
+import sys
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (HIP device) and the built library")
+
+
+def golden_structures():
+    with open(os.path.join(GOLDEN, "structures.json")) as fh:
+        d = json.load(fh)
+    out = {}
+    for k, v in d.items():
+        out[k] = [dict(positions=np.asarray(s["positions"], float), numbers=np.asarray(s["numbers"]),
+                       cell=np.asarray(s["cell"], float), pbc=np.asarray(s["pbc"], int),
+                       energy=s.get("energy")) for s in v]
+    return out
+
+
+def wrap_cubic_periodic(st):
+    """Shift + wrap a periodic structure into its cell (environments are
+    unchanged)."""
+    cell = np.asarray(st["cell"], float)
+    inv = np.linalg.inv(cell)
+    f = np.mod(st["positions"] @ inv.T, 1.0)
+    return dict(st, positions=f @ cell.T)
+
+
+def soap_hypers(rc=3.0, sw=0.5, sigma=0.4, N=4, L=3, spline=None, normalize=True,
+                cutoff_type="ShiftedCosine", rs=None, invariants=True):
+    fc = dict(type=cutoff_type, cutoff=dict(value=rc, unit="AA"),
+              smooth_width=dict(value=sw, unit="AA"))
+    if cutoff_type == "RadialScaling":
+        rate, scale, expo = rs
+        fc.update(rate=dict(value=rate, unit="AA"), scale=dict(value=scale, unit="AA"),
+                  exponent=dict(value=expo, unit="AA"))
+    h = dict(max_radial=N, max_angular=L, expansion_by_species_method="environment wise",
+             global_species=[], compute_gradients=False, cutoff_function=fc,
+             gaussian_density=dict(type="Constant", gaussian_sigma=dict(value=sigma, unit="AA")),
+             radial_contribution=dict(type="GTO", optimization={} if spline is None else
+                                      dict(Spline=dict(accuracy=spline))))
+    if invariants:
+        h.update(soap_type="PowerSpectrum", normalize=normalize, inversion_symmetry=True)
+    return h
+
+
+def rel_err(ref, test, eps=1e-14):
+    """librascal's relative_error metric (math/utils.hh:152-194)."""
+    ref = np.asarray(ref)
+    test = np.asarray(test)
+    diff = np.abs(ref - test)
+    zr = np.abs(ref) < eps
+    zt = np.abs(test) < eps
+    out = np.where(zr & zt, 0.0, np.where(~zr, diff / np.where(zr, 1.0, np.abs(ref)), diff))
+    return out
+
+
+def gpu_available():
+    try:
+        import torch
+        return torch.cuda.is_available()
+    except Exception:
+        return False
+
+
+@pytest.fixture(scope="session")
+def structures():
+    return golden_structures()

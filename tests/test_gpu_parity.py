@@ -1,0 +1,225 @@
+"""Parity of the HIP path (through the C ABI) with the FP64 oracle.
+
+Bars: neighbour lists bit-identical (same pairs, same order, same vectors);
+FP64 GPU path within 1e-10 relative (librascal's relative_error metric with
+epsilon 1e-14 for values that vanish); mixed path within the tolerances
+stated per test.
+"""
+import json
+
+import numpy as np
+import pytest
+
+import oracle as O
+from conftest import golden_structures, rel_err, soap_hypers, wrap_cubic_periodic
+from librascal_amd import _capi
+from librascal_amd.synthetic import diamond_cells, molecules, pack, periodic_box, triclinic_cells
+
+pytestmark = pytest.mark.gpu
+
+FP64_TOL = 1e-10
+
+
+def make_ctx(hp, structures, precision=_capi.FP64, species=None):
+    ctx = _capi.Context(json.dumps(hp))
+    ctx.set_precision(precision)
+    off, pos, num, cells, pbc = pack(structures)
+    sp = sorted(set(num.tolist()) | set(species or []))
+    ctx.set_structures(off, pos, num, cells, pbc, sp)
+    return ctx, sp
+
+
+def structure_sets():
+    g = golden_structures()
+    sets = {
+        "CaCrP2O7_rc3": (g["CaCrP2O7"], 3.0),
+        "small_molecule_rc3": (g["small_molecule"], 3.0),
+        "diamond_small_cell_rc5": (g["diamond_cubic_distorted"], 5.0),
+        "methane_dimers_rc4": ([wrap_cubic_periodic(s) for s in g["methane_dimers"][:6]], 4.0),
+        "SiC_rc4": (g["SiC_moissanite_supercell"], 4.0),
+        "diamond_jitter_rc5": (diamond_cells(n_cells=2), 5.0),
+        "molecules_rc4": (molecules(n_mol=6), 4.0),
+        "triclinic_rc6": (triclinic_cells(n_cells=3, n_max=80), 6.0),
+    }
+    return sets
+
+
+@pytest.mark.parametrize("name", list(structure_sets().keys()))
+def test_neighbour_list_bit_exact(name):
+    sts, rc = structure_sets()[name]
+    ctx, _ = make_ctx(soap_hypers(rc=rc, N=2, L=1), sts)
+    off, pj, vec = ctx.neighbours()
+    a0 = 0
+    for st in sts:
+        n = len(st["numbers"])
+        ro, rj, rv, _ = O.neighbours(st["positions"], st["cell"], st["pbc"], rc)
+        go = off[a0:a0 + n + 1] - off[a0]
+        assert np.array_equal(go, ro), name
+        seg = slice(off[a0], off[a0 + n])
+        assert np.array_equal(pj[seg] - a0, rj), name
+        assert np.array_equal(vec[seg], rv), name
+        a0 += n
+
+
+def test_ylm_parity_fp64_and_fp32():
+    rng = np.random.default_rng(3)
+    dirs = rng.normal(size=(200, 3))
+    dirs /= np.linalg.norm(dirs, axis=1)[:, None]
+    dirs[:4] = [[0, 0, 1], [0, 0, -1], [1, 0, 0], [0.0, 0.05, 0.99874921777]]
+    for L in (3, 6, 9):
+        ctx, _ = make_ctx(soap_hypers(rc=3.0, N=2, L=L), golden_structures()["small_molecule"])
+        Y, dY = ctx.eval_ylm(dirs, derivative=True)
+        for k, d in enumerate(dirs):
+            y, dy, _ = O.ylm(L, d, derivatives=True)
+            assert np.max(np.abs(Y[k] - y)) < 1e-13
+            assert np.max(np.abs(dY[k] - dy)) < 1e-12
+        ctx.set_precision(_capi.MIXED)
+        Y32, dY32 = ctx.eval_ylm(dirs, derivative=True)
+        assert np.max(np.abs(Y32 - Y)) < 2e-6
+        assert np.max(np.abs(dY32 - dY)) < 2e-5
+
+
+@pytest.mark.parametrize("spline", [None, 1e-8])
+@pytest.mark.parametrize("N,L,sigma,rc", [(4, 3, 0.5, 3.0), (8, 6, 0.5, 5.0), (12, 9, 0.5, 5.0),
+                                          (6, 2, 0.3, 4.0)])
+def test_radial_parity(spline, N, L, sigma, rc):
+    hp = soap_hypers(rc=rc, N=N, L=L, sigma=sigma, spline=spline)
+    ctx, _ = make_ctx(hp, golden_structures()["small_molecule"])
+    calc = O.Calculator(hp)
+    if spline is not None:
+        assert ctx.info("spline_grid") == calc.radial_info()["grid"]
+    r = np.linspace(0.05, rc, 137)
+    R, dR = ctx.eval_radial(r, derivative=True)
+    for k, x in enumerate(r):
+        Rr, dRr = calc.radial(x, derivative=True)
+        assert rel_err(Rr, R[k]).max() < FP64_TOL
+        assert rel_err(dRr, dR[k]).max() < 1e-9
+
+
+EXP_CASES = [
+    ("CaCrP2O7", 3.0, 0.5, 4, 3, None),
+    ("CaCrP2O7", 3.0, 0.2, 4, 3, 1e-8),
+    ("small_molecule", 3.0, 0.3, 6, 4, None),
+    ("diamond_cubic_distorted", 5.0, 0.5, 8, 6, 1e-8),
+    ("SiC_moissanite_supercell", 4.0, 0.4, 5, 5, None),
+]
+
+
+@pytest.mark.parametrize("case", EXP_CASES)
+def test_expansion_and_power_spectrum_fp64(case):
+    name, rc, sigma, N, L, spline = case
+    sts = golden_structures()[name]
+    hp = soap_hypers(rc=rc, sigma=sigma, N=N, L=L, spline=spline)
+    ctx, sp = make_ctx(hp, sts)
+    calc = O.Calculator(hp)
+    b = O.BatchData(sts, species=sp)
+    c_ref, mask_ref = calc.expansion(b)
+    c, mask = ctx.expansion()
+    assert np.array_equal(mask, mask_ref)
+    assert rel_err(c_ref, c).max() < FP64_TOL
+    p_ref, inv_ref = calc.power_spectrum(b)
+    X, inv = ctx.features()
+    assert rel_err(p_ref, X).max() < FP64_TOL
+    assert rel_err(inv_ref, inv).max() < FP64_TOL
+
+
+def oracle_backward(calc, batch, G):
+    grads, info = calc.power_spectrum_gradients(batch)
+    n = batch.n_atoms
+    out = np.zeros((n, 3))
+    ns = np.zeros((batch.n_struct, 6))
+    off = batch.atom_offsets
+    vo = [(4, 2), (5, 0), (3, 1)]
+    row = 0
+    for s in range(batch.n_struct):
+        st = slice(off[s], off[s + 1])
+        ro, rj, rv, _ = O.neighbours(batch.positions[st], batch.cells[s], batch.pbc[s], calc._h.cutoff)
+        for i in range(off[s + 1] - off[s]):
+            for p in range(ro[i], ro[i + 1]):
+                f = grads[row] @ G[off[s] + i]
+                out[off[s] + rj[p]] += f
+                rji = -rv[p]
+                for x in range(3):
+                    ns[s, x] += rji[x] * f[x]
+                    ns[s, vo[x][0]] += rji[vo[x][1]] * f[x]
+                row += 1
+        ns[s] /= abs(np.linalg.det(batch.cells[s]))
+    return out, ns
+
+
+@pytest.mark.parametrize("case", [("CaCrP2O7", 3.0, 0.5, 4, 3, None),
+                                  ("diamond_cubic_distorted", 5.0, 0.5, 4, 4, 1e-8),
+                                  ("small_molecule", 3.5, 0.4, 5, 3, 1e-8)])
+def test_features_backward_fp64(case):
+    name, rc, sigma, N, L, spline = case
+    sts = golden_structures()[name]
+    hp = soap_hypers(rc=rc, sigma=sigma, N=N, L=L, spline=spline)
+    ctx, sp = make_ctx(hp, sts)
+    calc = O.Calculator(hp)
+    b = O.BatchData(sts, species=sp)
+    rng = np.random.default_rng(7)
+    G = rng.normal(size=(b.n_atoms, calc.D(len(sp))))
+    ref_g, ref_s = oracle_backward(calc, b, G)
+    g, s = ctx.features_backward(G)
+    scale = np.abs(ref_g).max()
+    assert np.abs(g - ref_g).max() < FP64_TOL * scale
+    assert np.abs(s - ref_s).max() < FP64_TOL * np.abs(ref_s).max()
+
+
+def random_sparse(calc, batch, n_per_species, rng, zeta):
+    X, _ = calc.power_spectrum(batch)
+    species, values = [], []
+    for t in sorted(set(batch.types.tolist())):
+        idx = np.where(batch.types == t)[0]
+        pick = rng.choice(idx, size=n_per_species, replace=True)
+        species += [t] * n_per_species
+        values.append(X[pick] + 0.01 * rng.normal(size=(n_per_species, X.shape[1])))
+    species = np.asarray(species)
+    values = np.concatenate(values)
+    perm = rng.permutation(len(species))
+    return species[perm], values[perm], rng.normal(size=len(species))
+
+
+@pytest.mark.parametrize("zeta", [1, 2, 4])
+def test_gap_predict_fp64(zeta):
+    sts = golden_structures()["CaCrP2O7"] + golden_structures()["SiC_moissanite_supercell"]
+    hp = soap_hypers(rc=3.5, sigma=0.5, N=4, L=3, spline=1e-8)
+    ctx, sp = make_ctx(hp, sts)
+    calc = O.Calculator(hp)
+    b = O.BatchData(sts, species=sp)
+    rng = np.random.default_rng(11 + zeta)
+    spec, vals, w = random_sparse(calc, b, 5, rng, zeta)
+    ctx.set_sparse_points(zeta, spec, vals, w)
+    ss = O.SparseSet(spec, vals, zeta)
+    e_ref, g_ref, s_ref = O.gap_predict(calc, b, ss, w)
+    e, g, s = ctx.gap_predict()
+    assert rel_err(e_ref, e).max() < FP64_TOL
+    assert np.abs(g - g_ref).max() < FP64_TOL * np.abs(g_ref).max()
+    assert np.abs(s - s_ref).max() < FP64_TOL * np.abs(s_ref).max()
+    for target in ("Structure", "Atom"):
+        k_ref = O.gap_knm(calc, b, ss, target)
+        # the oracle orders columns species-sorted; map back to input order
+        k_ref_in = np.zeros_like(k_ref)
+        k_ref_in[:, ss.order] = k_ref
+        k = ctx.gap_knm(target)
+        assert rel_err(k_ref_in, k).max() < FP64_TOL
+
+
+def test_mixed_precision_power_spectrum_and_forces():
+    sts = periodic_box(256, 0.06, (1, 6, 7, 8), seed=5)
+    hp = soap_hypers(rc=5.0, sigma=0.5, N=8, L=6, spline=1e-8)
+    calc = O.Calculator(hp)
+    ctx, sp = make_ctx(hp, sts)
+    b = O.BatchData(sts, species=sp)
+    rng = np.random.default_rng(5)
+    spec, vals, w = random_sparse(calc, b, 16, rng, 4)
+    ctx.set_sparse_points(4, spec, vals, w)
+    e64, g64, s64 = ctx.gap_predict()
+    X64, _ = ctx.features()
+    ctx.set_precision(_capi.MIXED)
+    X32, _ = ctx.features()
+    e32, g32, s32 = ctx.gap_predict()
+    assert np.abs(X32 - X64).max() < 1e-5
+    assert abs(e32[0] - e64[0]) / abs(e64[0]) < 1e-4
+    rms = np.sqrt(np.mean((g32 - g64) ** 2)) / np.sqrt(np.mean(g64 ** 2))
+    assert rms < 1e-4
