@@ -52,6 +52,15 @@ extern "C" {
 #define RGPU_INFO_IS_INVARIANTS 9 /* 1 PowerSpectrum, 0 SphericalExpansion */
 #define RGPU_INFO_PRECISION 10
 
+/* stages timed by rgpu_enable_timing */
+#define RGPU_STAGE_NEIGHBOURS 0
+#define RGPU_STAGE_EXPANSION 1
+#define RGPU_STAGE_POWER_SPECTRUM 2
+#define RGPU_STAGE_GAP_KERNEL 3  /* K = (X T^T)^zeta GEMM + epilogue */
+#define RGPU_STAGE_GAP_FORCE 4   /* T~ = W~ T GEMM */
+#define RGPU_STAGE_BACKWARD 5    /* dL/dX -> dL/dc -> per-pair forces */
+#define RGPU_N_STAGES 6
+
 typedef struct rgpu_context rgpu_context;
 
 const char* rgpu_last_error(void);
@@ -134,6 +143,15 @@ int rgpu_gap_predict(rgpu_context* ctx, double* energy, double* gradient, double
  * dR[n][N][L+1] (may be NULL), evaluated by the GPU path of the current
  * precision; Y_lm and gradients at n unit vectors: Y[n][(L+1)^2],
  * dY[n][3][(L+1)^2] (may be NULL). */
+/* Stage timers: HIP events recorded on the context stream around each
+ * stage's kernels.  rgpu_get_timing synchronises and returns the summed
+ * milliseconds and the number of launches since the last reset.  No
+ * counterpart in the reference (its profilers time whole calls,
+ * performance/profiles/profile_spherical_expansion.cc:110-171). */
+int rgpu_enable_timing(rgpu_context* ctx, int enable);
+int rgpu_get_timing(rgpu_context* ctx, int stage, double* total_ms, int64_t* launches);
+int rgpu_reset_timing(rgpu_context* ctx);
+
 int rgpu_eval_radial(rgpu_context* ctx, int64_t n, const double* r, double* R, double* dR);
 int rgpu_eval_ylm(rgpu_context* ctx, int64_t n, const double* dirs, double* Y, double* dY);
 

@@ -20,6 +20,8 @@ RGPU_ERR_STATE = 4
 FP64 = 0
 MIXED = 1
 
+STAGES = dict(neighbours=0, expansion=1, power_spectrum=2, gap_kernel=3, gap_force=4, backward=5)
+
 INFO = dict(max_radial=0, max_angular=1, n_species=2, feature_size=3, n_centres=4,
             n_pairs=5, n_struct=6, spline_grid=7, n_sparse=8, is_invariants=9, precision=10)
 
@@ -62,11 +64,15 @@ def lib():
     L.rgpu_gap_predict.argtypes = [p, p, p, p]
     L.rgpu_eval_radial.argtypes = [p, i64, p, p, p]
     L.rgpu_eval_ylm.argtypes = [p, i64, p, p, p]
+    L.rgpu_enable_timing.argtypes = [p, i]
+    L.rgpu_get_timing.argtypes = [p, i, C.POINTER(C.c_double), C.POINTER(C.c_int64)]
+    L.rgpu_reset_timing.argtypes = [p]
     for name in ("rgpu_context_create", "rgpu_set_stream", "rgpu_set_precision", "rgpu_get_info",
                  "rgpu_set_structures", "rgpu_update_positions", "rgpu_get_neighbours",
                  "rgpu_compute", "rgpu_get_expansion", "rgpu_get_features",
                  "rgpu_features_backward", "rgpu_set_sparse_points", "rgpu_gap_knm",
-                 "rgpu_gap_predict", "rgpu_eval_radial", "rgpu_eval_ylm"):
+                 "rgpu_gap_predict", "rgpu_eval_radial", "rgpu_eval_ylm", "rgpu_enable_timing",
+                 "rgpu_get_timing", "rgpu_reset_timing"):
         getattr(L, name).restype = C.c_int
     _lib = L
     return L
@@ -194,6 +200,22 @@ class Context:
         s = np.zeros((ns, 6)) if neg_stress else None
         check(lib().rgpu_gap_predict(self._p, ptr(e), ptr(g), ptr(s)))
         return e, g, s
+
+    def enable_timing(self, enable=True):
+        check(lib().rgpu_enable_timing(self._p, int(enable)))
+
+    def reset_timing(self):
+        check(lib().rgpu_reset_timing(self._p))
+
+    def timing(self):
+        """{stage: (total_ms, launches)} since the last reset."""
+        out = {}
+        for name, k in STAGES.items():
+            ms = C.c_double(0)
+            n = C.c_int64(0)
+            check(lib().rgpu_get_timing(self._p, k, C.byref(ms), C.byref(n)))
+            out[name] = (ms.value, n.value)
+        return out
 
     def eval_radial(self, r, derivative=False):
         r = np.ascontiguousarray(r, np.float64)

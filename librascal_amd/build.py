@@ -28,7 +28,7 @@ COMMON = [
 # geometry of the neighbour list must round like separate multiply/add
 SOURCE_FLAGS = {"neighbours.hip": ["-ffp-contract=off"]}
 SOURCES = ["hypers.cc", "radial_host.cc", "neighbours.hip", "soap_kernels.hip",
-           "gap_kernels.hip", "gap_mfma.hip", "context.cc"]
+           "gap_kernels.hip", "gap_mfma.hip", "gap_f64.hip", "context.cc"]
 
 
 def _hipcc():

@@ -12,6 +12,7 @@
 #include "../../include/rascal_mi355x.h"
 #include "common.hh"
 #include "device_buffer.hh"
+#include "gap_f64.hh"
 #include "gap_kernels.hh"
 #include "gap_mfma.hh"
 #include "hypers.hh"
@@ -80,6 +81,48 @@ struct Context {
   DeviceBuffer<float> wt32, G32, W32;
   DeviceBuffer<int64_t> knm_perm;
 
+  // stage timers (HIP events on `stream`)
+  bool timing = false;
+  struct Pending {
+    hipEvent_t a, b;
+    int stage;
+  };
+  std::vector<Pending> pending;
+  std::vector<hipEvent_t> ev_pool;
+  double stage_ms[RGPU_N_STAGES] = {0, 0, 0, 0, 0, 0};
+  int64_t stage_n[RGPU_N_STAGES] = {0, 0, 0, 0, 0, 0};
+
+  hipEvent_t event() {
+    if (!ev_pool.empty()) {
+      hipEvent_t e = ev_pool.back();
+      ev_pool.pop_back();
+      return e;
+    }
+    hipEvent_t e;
+    RGPU_HIP(hipEventCreate(&e));
+    return e;
+  }
+  void collect() {
+    if (pending.empty()) return;
+    RGPU_HIP(hipStreamSynchronize(stream));
+    for (auto& p : pending) {
+      float ms = 0;
+      RGPU_HIP(hipEventElapsedTime(&ms, p.a, p.b));
+      stage_ms[p.stage] += ms;
+      stage_n[p.stage] += 1;
+      ev_pool.push_back(p.a);
+      ev_pool.push_back(p.b);
+    }
+    pending.clear();
+  }
+  ~Context() {
+    for (auto& p : pending) {
+      (void)hipEventDestroy(p.a);
+      (void)hipEventDestroy(p.b);
+    }
+    for (auto e : ev_pool) (void)hipEventDestroy(e);
+  }
+
   int64_t D() const {
     const int S = int(species.size());
     const int N = hp.max_radial, L1 = hp.max_angular + 1;
@@ -89,6 +132,27 @@ struct Context {
 };
 
 namespace {
+
+// Records HIP events around a stage's kernels when timing is enabled.
+class StageTimer {
+ public:
+  StageTimer(Context& c, int stage) : c_(c), stage_(stage) {
+    if (c_.timing) {
+      a_ = c_.event();
+      RGPU_HIP(hipEventRecord(a_, c_.stream));
+    }
+  }
+  ~StageTimer() {
+    if (!a_) return;
+    hipEvent_t b = c_.event();
+    if (hipEventRecord(b, c_.stream) == hipSuccess) c_.pending.push_back({a_, b, stage_});
+  }
+
+ private:
+  Context& c_;
+  int stage_;
+  hipEvent_t a_ = nullptr;
+};
 
 void build_tables(Context& c) {
   c.rh.reset(new RadialHost(c.hp));
@@ -203,6 +267,7 @@ void require_batch(const Context& c) {
 }
 
 void build_batch_nl(Context& c, const double* positions) {
+  StageTimer t(c, RGPU_STAGE_NEIGHBOURS);
   build_pair_list(c.pl, c.n_struct, c.atom_off.data(), positions, c.cells.data(), c.pbc.data(),
                   c.hp.cutoff, c.stream);
   c.have_rep = false;
@@ -247,8 +312,12 @@ void compute_rep(Context& c) {
   const int64_t E = int64_t(S) * c.rd.N * c.rd.LL;
   T* cc = cbuf<T>(c).reserve(size_t(std::max<int64_t>(n * E, 1)));
   unsigned long long* km = c.keymask.reserve(size_t(std::max<int64_t>(n, 1)));
-  launch_expansion<T>(c.rd, S, pair_view(c), cc, km, c.stream);
+  {
+    StageTimer t(c, RGPU_STAGE_EXPANSION);
+    launch_expansion<T>(c.rd, S, pair_view(c), cc, km, c.stream);
+  }
   if (c.hp.kind == SoapHypers::kPowerSpectrum) {
+    StageTimer t(c, RGPU_STAGE_POWER_SPECTRUM);
     const int64_t D = c.D();
     T* pp = pbuf<T>(c).reserve(size_t(std::max<int64_t>(n * D, 1)));
     double* n2 = c.norm2.reserve(size_t(std::max<int64_t>(n, 1)));
@@ -354,6 +423,7 @@ void backward_device(Context& c, const T* G, double* grad, double* cvir) {
   const int S = int(c.species.size());
   const int64_t E = int64_t(S) * c.rd.N * c.rd.LL;
   T* W = Wbuf<T>(c).reserve(size_t(std::max<int64_t>(c.n_atoms * E, 1)));
+  StageTimer t(c, RGPU_STAGE_BACKWARD);
   launch_backward<T>(c.rd, S, pair_view(c), nullptr, cbuf<T>(c).get(), pbuf<T>(c).get(), c.norm2.get(),
                      c.hp.normalize, G, W, grad, cvir, c.stream);
 }
@@ -397,20 +467,57 @@ void gap_predict_t(Context& c, double* energy, double* gradient, double* neg_str
     RGPU_HIP(hipMemsetAsync(G, 0, sizeof(T) * n * D, c.stream));
   }
   const T* X = pbuf<T>(c).get();
+  // W~ of all species blocks in one buffer
+  std::vector<GapBlock<T>> blks;
+  std::vector<int64_t> wt_off;
+  int64_t wt_total = 0;
   for (int s = 0; s < S; ++s) {
-    GapBlock<T> blk = gap_block<T>(c, s, X, sparse_values<T>(c));
-    if (blk.n_rows == 0 || blk.n_cols == 0) continue;
-    T* wt = want_grad ? wtbuf<T>(c).reserve(size_t(blk.n_rows * blk.n_cols)) : nullptr;
-    bool done = false;
-    if constexpr (std::is_same<T, float>::value) {
-      if (mfma_available()) {
-        run_gap_mixed(blk, c.mixed_sparse, ec, wt, G, c.stream);
-        done = true;
+    blks.push_back(gap_block<T>(c, s, X, sparse_values<T>(c)));
+    wt_off.push_back(wt_total);
+    wt_total += blks.back().n_rows * blks.back().n_cols;
+  }
+  T* wt_all = want_grad ? wtbuf<T>(c).reserve(size_t(std::max<int64_t>(wt_total, 1))) : nullptr;
+  std::vector<T*> wts(size_t(S), nullptr);
+  if (want_grad)
+    for (int s = 0; s < S; ++s) wts[s] = wt_all + wt_off[s];
+  bool grouped_done = false;
+  if constexpr (std::is_same<T, float>::value) {
+    if (mfma_available()) {
+      {
+        StageTimer t(c, RGPU_STAGE_GAP_KERNEL);
+        run_gap_mixed_kernel(blks.data(), S, c.mixed_sparse, ec, want_grad ? wts.data() : nullptr,
+                             c.stream);
       }
+      if (want_grad) {
+        StageTimer t(c, RGPU_STAGE_GAP_FORCE);
+        run_gap_mixed_force(blks.data(), S, c.mixed_sparse, wts.data(), G, c.stream);
+      }
+      grouped_done = true;
     }
-    if (!done) {
-      launch_gap_kernel<T>(blk, c.M, ec, wt, nullptr, nullptr, nullptr, c.stream);
-      if (want_grad) launch_gap_force_contraction<T>(blk, wt, G, c.stream);
+  } else {
+    {
+      StageTimer t(c, RGPU_STAGE_GAP_KERNEL);
+      run_gap_f64_kernel(blks.data(), S, ec, want_grad ? wts.data() : nullptr, c.stream);
+    }
+    if (want_grad) {
+      StageTimer t(c, RGPU_STAGE_GAP_FORCE);
+      run_gap_f64_force(blks.data(), S, wts.data(), G, c.stream);
+    }
+    grouped_done = true;
+  }
+  for (int s = 0; s < S && !grouped_done; ++s) {
+    const GapBlock<T>& blk = blks[s];
+    if (blk.n_rows == 0 || blk.n_cols == 0) continue;
+    T* wt = wts[s];
+    {
+      {
+        StageTimer t(c, RGPU_STAGE_GAP_KERNEL);
+        launch_gap_kernel<T>(blk, c.M, ec, wt, nullptr, nullptr, nullptr, c.stream);
+      }
+      if (want_grad) {
+        StageTimer t(c, RGPU_STAGE_GAP_FORCE);
+        launch_gap_force_contraction<T>(blk, wt, G, c.stream);
+      }
     }
   }
   if (energy) {
@@ -787,6 +894,34 @@ int rgpu_gap_predict(rgpu_context* ctx, double* energy, double* gradient, double
       gap_predict_t<double>(c, energy, gradient, neg_stress);
     else
       gap_predict_t<float>(c, energy, gradient, neg_stress);
+  });
+}
+
+int rgpu_enable_timing(rgpu_context* ctx, int enable) {
+  return guarded([&] {
+    if (!ctx) throw InvalidArgument("null context");
+    ctx->c.timing = enable != 0;
+  });
+}
+
+int rgpu_get_timing(rgpu_context* ctx, int stage, double* total_ms, int64_t* launches) {
+  return guarded([&] {
+    if (!ctx) throw InvalidArgument("null context");
+    if (stage < 0 || stage >= RGPU_N_STAGES) throw InvalidArgument("unknown stage");
+    ctx->c.collect();
+    if (total_ms) *total_ms = ctx->c.stage_ms[stage];
+    if (launches) *launches = ctx->c.stage_n[stage];
+  });
+}
+
+int rgpu_reset_timing(rgpu_context* ctx) {
+  return guarded([&] {
+    if (!ctx) throw InvalidArgument("null context");
+    ctx->c.collect();
+    for (int k = 0; k < RGPU_N_STAGES; ++k) {
+      ctx->c.stage_ms[k] = 0;
+      ctx->c.stage_n[k] = 0;
+    }
   });
 }
 

@@ -1,0 +1,18 @@
+// FP64 GAP GEMMs on the gfx950 matrix cores (parity path).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "gap_kernels.hh"
+
+namespace rgpu {
+
+// K GEMM + epilogue for all species blocks (grouped): energy per centre (+=)
+// and wts[b] [n_rows][n_cols] (wts may be null)
+void run_gap_f64_kernel(const GapBlock<double>* blks, int n_blocks, double* energy,
+                        double* const* wts, hipStream_t stream);
+// G[row_idx[r]][:] = sum_m wts[b][r][m] T[col0+m][:]
+void run_gap_f64_force(const GapBlock<double>* blks, int n_blocks, double* const* wts, double* G,
+                       hipStream_t stream);
+
+}  // namespace rgpu

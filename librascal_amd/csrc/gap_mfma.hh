@@ -24,8 +24,13 @@ struct MixedSparse {
 bool mfma_available();
 void prepare_mixed_sparse(MixedSparse& ms, const std::vector<double>& T, int64_t M, int64_t D,
                           hipStream_t stream);
-// energy (per centre, +=), wt [n_rows][n_cols] and G rows (when non-null)
-void run_gap_mixed(const GapBlock<float>& blk, const MixedSparse& ms, double* energy, float* wt,
-                   float* G, hipStream_t stream);
+// K tile GEMM + epilogue for all species blocks in grouped launches:
+// energy (per centre, +=) and wts[b] [n_rows][n_cols] (wts or any entry may
+// be null)
+void run_gap_mixed_kernel(const GapBlock<float>* blks, int n_blocks, const MixedSparse& ms,
+                          double* energy, float* const* wts, hipStream_t stream);
+// T~ rows: G[row_idx[r]][:] = sum_m wts[b][r][m] T[col0+m][:]
+void run_gap_mixed_force(const GapBlock<float>* blks, int n_blocks, const MixedSparse& ms,
+                         float* const* wts, float* G, hipStream_t stream);
 
 }  // namespace rgpu

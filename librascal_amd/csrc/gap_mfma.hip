@@ -1,16 +1,405 @@
+// Mixed-precision GAP GEMMs on the gfx950 matrix cores.
+//
+// Both products of the GAP prediction are computed as C = A B^T with A and B
+// stored row-major with the contraction index contiguous:
+//   kernel:  K[r][m] = sum_d X[row(r)][d] T[col0+m][d]       (B = T, [M][D])
+//   forces:  G[row(r)][d] = sum_m W~[r][m] T[col0+m][d]      (B = T^T, [D][M])
+// FP32 operands are split as x = hi + lo with hi = bf16(x), lo = bf16(x-hi);
+// each algorithmic multiply-add is three v_mfma_f32_32x32x16_bf16 products
+// (hi.hi + hi.lo + lo.hi) accumulated in FP32.  The sparse points (model
+// data) are split once at load time; X and W~ are split while staging
+// through LDS.
+//
+// Tiling: 256 threads (4 wavefronts, 2 x 2), block tile 128 x 128, K step 32;
+// each wavefront owns 64 x 64 = 2 x 2 MFMA tiles.  Global loads of the next K
+// step are issued before the MFMAs of the current one (register prefetch).
+// All species blocks of one product go in one grouped launch.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+
+#include "common.hh"
 #include "gap_mfma.hh"
 
 namespace rgpu {
 
-bool mfma_available() { return false; }
+namespace {
 
-void prepare_mixed_sparse(MixedSparse& ms, const std::vector<double>&, int64_t M, int64_t D, hipStream_t) {
-  ms.M = M;
-  ms.D = D;
+typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
+typedef __attribute__((__vector_size__(16 * sizeof(float)))) float f32x16;
+
+constexpr int BM = 128, BN = 128, BK = 32, LDK = BK + 8, NT = 256;
+constexpr int kMaxProblems = 16;
+
+__device__ __forceinline__ unsigned short f2bf(float x) {
+  unsigned u = __float_as_uint(x);
+  u += 0x7FFFu + ((u >> 16) & 1u);  // round to nearest even
+  return (unsigned short)(u >> 16);
+}
+__device__ __forceinline__ float bf2f(unsigned short h) { return __uint_as_float(unsigned(h) << 16); }
+
+struct Problem {
+  int64_t M, N, K;             // C (M x N) = A (M x K) B (N x K)^T
+  const float* A;
+  int64_t lda;
+  const int* a_rows;           // gathered A rows (null: identity)
+  const unsigned short* Bhi;   // B rows, K contiguous, already offset
+  const unsigned short* Blo;
+  int64_t ldb;
+  const int* c_rows;           // output row ids (centre indices)
+  int64_t tile0;               // first linear tile of this problem
+  int tiles_n;                 // tiles along N
+  // kernel epilogue
+  const double* norm2;
+  const double* alpha;         // offset to the block's first sparse point
+  double* energy;
+  float* wt;                   // [M][N]
+  // force epilogue
+  float* G;
+  int64_t ldg;
+};
+
+struct ProblemSet {
+  Problem p[kMaxProblems];
+  int n;
+  int normalize;
+  int zeta;
+};
+
+template <class T>
+__device__ __forceinline__ T pow_int(T v, int n) {
+  if (n == 0) return T(1);
+  if (n == 1) return v;
+  if (n == 2) return v * v;
+  if (n == 3) return v * v * v;
+  T r = T(1), x = v;
+  unsigned u = unsigned(n);
+  do {
+    if (u & 1) r *= x;
+    u >>= 1;
+    x *= x;
+  } while (u);
+  return r;
 }
 
-void run_gap_mixed(const GapBlock<float>&, const MixedSparse&, double*, float*, float*, hipStream_t) {
-  throw RuntimeError("mixed-precision matrix-core GAP kernel not built");
+template <int EPI>  // 0: GAP kernel epilogue, 1: store rows of G
+__global__ __launch_bounds__(NT, 2) void bf16x3_gemm(ProblemSet ps) {
+  __shared__ __align__(16) unsigned short sAh[BM * LDK];
+  __shared__ __align__(16) unsigned short sAl[BM * LDK];
+  __shared__ __align__(16) unsigned short sBh[BN * LDK];
+  __shared__ __align__(16) unsigned short sBl[BN * LDK];
+  __shared__ double esum[2][BM];
+
+  // problem and tile of this block
+  const int64_t bid = blockIdx.x;
+  int pi = 0;
+  while (pi + 1 < ps.n && bid >= ps.p[pi + 1].tile0) ++pi;
+  const Problem& P = ps.p[pi];
+  const int64_t local = bid - P.tile0;
+  const int64_t tm = local / P.tiles_n, tn = local - tm * P.tiles_n;
+  const int64_t r0 = tm * BM, c0 = tn * BN;
+
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+
+  // global load assignment: two threads per row, 16 K values each
+  const int lr = tid >> 1, lk = (tid & 1) * 16;
+  const int64_t arow = r0 + lr, brow = c0 + lr;
+  const bool a_ok = arow < P.M, b_ok = brow < P.N;
+  const float* Ap = nullptr;
+  if (a_ok) Ap = P.A + (P.a_rows ? int64_t(P.a_rows[arow]) : arow) * P.lda;
+  const unsigned short* Bhp = b_ok ? P.Bhi + brow * P.ldb : nullptr;
+  const unsigned short* Blp = b_ok ? P.Blo + brow * P.ldb : nullptr;
+
+  float4 ra[4];
+  uint4 rbh[2], rbl[2];
+  auto gload = [&](int64_t k0) {
+    const int64_t k = k0 + lk;
+    if (a_ok && k + 16 <= P.K && ((reinterpret_cast<uintptr_t>(Ap + k) & 15) == 0)) {
+      const float4* src = reinterpret_cast<const float4*>(Ap + k);
+#pragma unroll
+      for (int q = 0; q < 4; ++q) ra[q] = src[q];
+    } else {
+      float tmp[16];
+#pragma unroll
+      for (int q = 0; q < 16; ++q) tmp[q] = (a_ok && k + q < P.K) ? Ap[k + q] : 0.f;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) ra[q] = make_float4(tmp[4 * q], tmp[4 * q + 1], tmp[4 * q + 2], tmp[4 * q + 3]);
+    }
+    if (b_ok && k + 16 <= P.K && ((reinterpret_cast<uintptr_t>(Bhp + k) & 15) == 0) &&
+        ((reinterpret_cast<uintptr_t>(Blp + k) & 15) == 0)) {
+      const uint4* sh = reinterpret_cast<const uint4*>(Bhp + k);
+      const uint4* sl = reinterpret_cast<const uint4*>(Blp + k);
+      rbh[0] = sh[0];
+      rbh[1] = sh[1];
+      rbl[0] = sl[0];
+      rbl[1] = sl[1];
+    } else {
+      unsigned short th[16], tl[16];
+#pragma unroll
+      for (int q = 0; q < 16; ++q) {
+        const bool ok = b_ok && k + q < P.K;
+        th[q] = ok ? Bhp[k + q] : (unsigned short)0;
+        tl[q] = ok ? Blp[k + q] : (unsigned short)0;
+      }
+      memcpy(rbh, th, 32);
+      memcpy(rbl, tl, 32);
+    }
+  };
+  auto lstore = [&]() {
+    unsigned short hi[16], lo[16];
+    const float* f = reinterpret_cast<const float*>(ra);
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const unsigned short h = f2bf(f[q]);
+      hi[q] = h;
+      lo[q] = f2bf(f[q] - bf2f(h));
+    }
+    uint4* dah = reinterpret_cast<uint4*>(sAh + lr * LDK + lk);
+    uint4* dal = reinterpret_cast<uint4*>(sAl + lr * LDK + lk);
+    dah[0] = *reinterpret_cast<const uint4*>(hi);
+    dah[1] = *reinterpret_cast<const uint4*>(hi + 8);
+    dal[0] = *reinterpret_cast<const uint4*>(lo);
+    dal[1] = *reinterpret_cast<const uint4*>(lo + 8);
+    uint4* dbh = reinterpret_cast<uint4*>(sBh + lr * LDK + lk);
+    uint4* dbl = reinterpret_cast<uint4*>(sBl + lr * LDK + lk);
+    dbh[0] = rbh[0];
+    dbh[1] = rbh[1];
+    dbl[0] = rbl[0];
+    dbl[1] = rbl[1];
+  };
+
+  f32x16 acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int e = 0; e < 16; ++e) acc[i][j][e] = 0.f;
+
+  const int fr = lane & 31, fk = (lane >> 5) * 8;
+  gload(0);
+  for (int64_t k0 = 0; k0 < P.K; k0 += BK) {
+    __syncthreads();
+    lstore();
+    __syncthreads();
+    if (k0 + BK < P.K) gload(k0 + BK);
+#pragma unroll
+    for (int ks = 0; ks < BK; ks += 16) {
+      bf16x8 ah[2], al[2], bh[2], bl[2];
+#pragma unroll
+      for (int i = 0; i < 2; ++i) {
+        const int row = wm * 64 + i * 32 + fr;
+        ah[i] = *reinterpret_cast<const bf16x8*>(sAh + row * LDK + ks + fk);
+        al[i] = *reinterpret_cast<const bf16x8*>(sAl + row * LDK + ks + fk);
+      }
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        const int col = wn * 64 + j * 32 + fr;
+        bh[j] = *reinterpret_cast<const bf16x8*>(sBh + col * LDK + ks + fk);
+        bl[j] = *reinterpret_cast<const bf16x8*>(sBl + col * LDK + ks + fk);
+      }
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+          acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
+          acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
+          acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
+        }
+    }
+  }
+
+  // epilogue; element e = 4*g + q of a 32x32 tile sits at
+  // row 8*g + 4*(lane/32) + q, column lane%32
+  const int half = lane >> 5;
+  if (EPI == 0) {
+    double rowsum[2][16];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int e = 0; e < 16; ++e) rowsum[i][e] = 0.0;
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+#pragma unroll
+      for (int e = 0; e < 16; ++e) {
+        const int rl = wm * 64 + i * 32 + 8 * (e >> 2) + 4 * half + (e & 3);
+        const int64_t r = r0 + rl;
+        if (r >= P.M) continue;
+        const int centre = P.c_rows[r];
+        double inv = 1.0;
+        if (ps.normalize) {
+          const double n2 = P.norm2[centre];
+          inv = n2 > 0 ? 1.0 / sqrt(n2) : 1.0;
+        }
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+          const int64_t c = c0 + wn * 64 + j * 32 + fr;
+          if (c >= P.N) continue;
+          const float k = float(double(acc[i][j][e]) * inv);
+          const double a = P.alpha[c];
+          rowsum[i][e] += a * double(pow_int(k, ps.zeta));
+          if (P.wt) {
+            const float g = ps.zeta > 1 ? float(ps.zeta) * pow_int(k, ps.zeta - 1) : 1.f;
+            P.wt[r * P.N + c] = float(a) * g;
+          }
+        }
+      }
+    }
+    // reduce over the 32 columns held by the lanes of each half
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int e = 0; e < 16; ++e) {
+        double v = rowsum[i][e];
+        for (int off = 16; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+        rowsum[i][e] = v;
+      }
+    if (fr == 0) {
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int e = 0; e < 16; ++e)
+          esum[wn][wm * 64 + i * 32 + 8 * (e >> 2) + 4 * half + (e & 3)] = rowsum[i][e];
+    }
+    __syncthreads();
+    if (P.energy && tid < BM) {
+      const int64_t r = r0 + tid;
+      if (r < P.M) atomicAdd(P.energy + P.c_rows[r], esum[0][tid] + esum[1][tid]);
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int e = 0; e < 16; ++e) {
+        const int64_t r = r0 + wm * 64 + i * 32 + 8 * (e >> 2) + 4 * half + (e & 3);
+        if (r >= P.M) continue;
+        float* g = P.G + int64_t(P.c_rows[r]) * P.ldg;
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+          const int64_t c = c0 + wn * 64 + j * 32 + fr;
+          if (c < P.N) g[c] = acc[i][j][e];
+        }
+      }
+  }
+}
+
+unsigned short bf16_rne_host(float x) {
+  unsigned u;
+  std::memcpy(&u, &x, 4);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (unsigned short)(u >> 16);
+}
+float bf16_to_float_host(unsigned short h) {
+  unsigned u = unsigned(h) << 16;
+  float f;
+  std::memcpy(&f, &u, 4);
+  return f;
+}
+
+void launch(const ProblemSet& ps, int64_t total_tiles, int epi, hipStream_t stream) {
+  if (total_tiles == 0) return;
+  if (epi == 0)
+    hipLaunchKernelGGL(bf16x3_gemm<0>, dim3(unsigned(total_tiles)), dim3(NT), 0, stream, ps);
+  else
+    hipLaunchKernelGGL(bf16x3_gemm<1>, dim3(unsigned(total_tiles)), dim3(NT), 0, stream, ps);
+  RGPU_LAUNCH_CHECK();
+}
+
+}  // namespace
+
+bool mfma_available() { return true; }
+
+void prepare_mixed_sparse(MixedSparse& ms, const std::vector<double>& T, int64_t M, int64_t D,
+                          hipStream_t stream) {
+  ms.M = M;
+  ms.D = D;
+  const size_t n = size_t(M * D);
+  std::vector<unsigned short> hi(std::max<size_t>(n, 1)), lo(std::max<size_t>(n, 1)),
+      thi(std::max<size_t>(n, 1)), tlo(std::max<size_t>(n, 1));
+  for (int64_t m = 0; m < M; ++m)
+    for (int64_t d = 0; d < D; ++d) {
+      const float x = float(T[size_t(m * D + d)]);
+      const unsigned short h = bf16_rne_host(x);
+      const unsigned short l = bf16_rne_host(x - bf16_to_float_host(h));
+      hi[size_t(m * D + d)] = h;
+      lo[size_t(m * D + d)] = l;
+      thi[size_t(d * M + m)] = h;
+      tlo[size_t(d * M + m)] = l;
+    }
+  auto up = [&](DeviceBuffer<unsigned short>& b, const std::vector<unsigned short>& v) {
+    unsigned short* p = b.reserve(v.size());
+    RGPU_HIP(hipMemcpyAsync(p, v.data(), v.size() * 2, hipMemcpyHostToDevice, stream));
+  };
+  up(ms.t_hi, hi);
+  up(ms.t_lo, lo);
+  up(ms.tt_hi, thi);
+  up(ms.tt_lo, tlo);
+  RGPU_HIP(hipStreamSynchronize(stream));
+}
+
+void run_gap_mixed_kernel(const GapBlock<float>* blks, int n_blocks, const MixedSparse& ms,
+                          double* energy, float* const* wts, hipStream_t stream) {
+  for (int b0 = 0; b0 < n_blocks; b0 += kMaxProblems) {
+    ProblemSet ps{};
+    int64_t tiles = 0;
+    for (int b = b0; b < std::min(n_blocks, b0 + kMaxProblems); ++b) {
+      const GapBlock<float>& blk = blks[b];
+      if (blk.n_rows == 0 || blk.n_cols == 0) continue;
+      Problem& P = ps.p[ps.n++];
+      P.M = blk.n_rows;
+      P.N = blk.n_cols;
+      P.K = blk.D;
+      P.A = blk.X;
+      P.lda = blk.D;
+      P.a_rows = blk.row_idx;
+      P.Bhi = ms.t_hi.get() + blk.col0 * blk.D;
+      P.Blo = ms.t_lo.get() + blk.col0 * blk.D;
+      P.ldb = blk.D;
+      P.c_rows = blk.row_idx;
+      P.tile0 = tiles;
+      P.tiles_n = int((P.N + BN - 1) / BN);
+      P.norm2 = blk.norm2;
+      if (!blk.alpha) throw InvalidArgument("the GAP energy kernel needs weights");
+      P.alpha = blk.alpha + blk.col0;
+      P.energy = energy;
+      P.wt = wts ? wts[b] : nullptr;
+      tiles += ((P.M + BM - 1) / BM) * P.tiles_n;
+      ps.normalize = blk.normalize;
+      ps.zeta = blk.zeta;
+    }
+    launch(ps, tiles, 0, stream);
+  }
+}
+
+void run_gap_mixed_force(const GapBlock<float>* blks, int n_blocks, const MixedSparse& ms,
+                         float* const* wts, float* G, hipStream_t stream) {
+  for (int b0 = 0; b0 < n_blocks; b0 += kMaxProblems) {
+    ProblemSet ps{};
+    int64_t tiles = 0;
+    for (int b = b0; b < std::min(n_blocks, b0 + kMaxProblems); ++b) {
+      const GapBlock<float>& blk = blks[b];
+      if (blk.n_rows == 0 || blk.n_cols == 0) continue;
+      Problem& P = ps.p[ps.n++];
+      P.M = blk.n_rows;
+      P.N = blk.D;
+      P.K = blk.n_cols;
+      P.A = wts[b];
+      P.lda = blk.n_cols;
+      P.a_rows = nullptr;
+      P.Bhi = ms.tt_hi.get() + blk.col0;
+      P.Blo = ms.tt_lo.get() + blk.col0;
+      P.ldb = ms.M;
+      P.c_rows = blk.row_idx;
+      P.tile0 = tiles;
+      P.tiles_n = int((P.N + BN - 1) / BN);
+      P.G = G;
+      P.ldg = blk.D;
+      tiles += ((P.M + BM - 1) / BM) * P.tiles_n;
+    }
+    launch(ps, tiles, 1, stream);
+  }
 }
 
 }  // namespace rgpu

@@ -75,3 +75,52 @@ def gpu_available():
 @pytest.fixture(scope="session")
 def structures():
     return golden_structures()
+
+
+def _block_rel(ref, test, scale):
+    """max |test-ref| / scale with scale broadcast over the block; blocks that
+    are identically zero in the reference are compared against the row
+    scale."""
+    return float(np.max(np.abs(np.asarray(test) - np.asarray(ref)) / scale))
+
+
+FLOOR = 1e-4
+
+
+def expansion_rel(ref, test, L):
+    """Relative error per (centre, species, l) block of c[n][s][N][(L+1)^2]:
+    max |diff| / max |ref| over the block's (n, m) entries.  Blocks smaller
+    than FLOOR times the centre's largest coefficient (e.g. zero by symmetry,
+    holding only rounding noise of the large entries) are measured against
+    that floor."""
+    ref = np.asarray(ref)
+    test = np.asarray(test)
+    worst = 0.0
+    row = np.abs(ref).reshape(ref.shape[0], -1).max(axis=1)
+    for l in range(L + 1):
+        sl = slice(l * l, (l + 1) * (l + 1))
+        r = ref[..., sl]
+        sc = np.abs(r).max(axis=(2, 3), keepdims=True)
+        sc = np.maximum(sc, FLOOR * row[:, None, None, None] + 1e-300)
+        worst = max(worst, _block_rel(r, test[..., sl], sc))
+    return worst
+
+
+def ps_rel(ref, test, S, N, L):
+    """Relative error per (centre, key, l) block of p[n][K][N*N][L+1]."""
+    K = S * (S + 1) // 2
+    ref = np.asarray(ref).reshape(-1, K, N * N, L + 1)
+    test = np.asarray(test).reshape(ref.shape)
+    sc = np.abs(ref).max(axis=2, keepdims=True)
+    row = np.abs(ref).reshape(ref.shape[0], -1).max(axis=1)
+    sc = np.maximum(sc, FLOOR * row[:, None, None, None] + 1e-300)
+    return _block_rel(ref, test, sc)
+
+
+def radial_rel(ref, test):
+    """Relative error per radial function (n, l) over the sampled r:
+    ref/test [n_r][N][L+1]."""
+    ref = np.asarray(ref)
+    sc = np.abs(ref).max(axis=0, keepdims=True)
+    sc = np.where(sc > 0, sc, 1e-300)
+    return _block_rel(ref, test, sc)
