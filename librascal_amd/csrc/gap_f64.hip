@@ -155,7 +155,7 @@ __global__ __launch_bounds__(NT, 2) void f64_gemm(ProblemSet ps) {
     }
   }
 
-  // element e of a 16x16 tile: row 4*(lane/16) + e, column lane%16
+  // element e of a 16x16 tile: row lane/16 + 4*e, column lane%16
   if (EPI == 0) {
     double rowsum[2][4];
 #pragma unroll
@@ -163,7 +163,7 @@ __global__ __launch_bounds__(NT, 2) void f64_gemm(ProblemSet ps) {
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
         rowsum[i][e] = 0.0;
-        const int64_t r = r0 + wm * 32 + i * 16 + 4 * fq + e;
+        const int64_t r = r0 + wm * 32 + i * 16 + fq + 4 * e;
         if (r >= P.M) continue;
         const int centre = P.c_rows[r];
         double inv = 1.0;
@@ -193,7 +193,7 @@ __global__ __launch_bounds__(NT, 2) void f64_gemm(ProblemSet ps) {
 #pragma unroll
       for (int i = 0; i < 2; ++i)
 #pragma unroll
-        for (int e = 0; e < 4; ++e) esum[wn][wm * 32 + i * 16 + 4 * fq + e] = rowsum[i][e];
+        for (int e = 0; e < 4; ++e) esum[wn][wm * 32 + i * 16 + fq + 4 * e] = rowsum[i][e];
     }
     __syncthreads();
     if (P.energy && tid < BM) {
@@ -205,7 +205,7 @@ __global__ __launch_bounds__(NT, 2) void f64_gemm(ProblemSet ps) {
     for (int i = 0; i < 2; ++i)
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
-        const int64_t r = r0 + wm * 32 + i * 16 + 4 * fq + e;
+        const int64_t r = r0 + wm * 32 + i * 16 + fq + 4 * e;
         if (r >= P.M) continue;
         double* g = P.G + int64_t(P.c_rows[r]) * P.ldg;
 #pragma unroll

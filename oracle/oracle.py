@@ -67,7 +67,7 @@ def lib():
         L.orc_hyp1f1.argtypes = [d, d, d, i, i, d]
         L.orc_hyp1f1_switch_point.restype = d
         L.orc_hyp1f1_switch_point.argtypes = [d, d]
-        L.orc_hyp1f1_sph.argtypes = [i, i, d, d, p, i, p, p]
+        L.orc_hyp1f1_sph.argtypes = [i, i, d, d, p, i, d, p, p]
         L.orc_ylm.argtypes = [i, p, p, p, p]
         L.orc_cutoff_fc.restype = d
         L.orc_cutoff_fc.argtypes = [C.POINTER(Hypers), d]
@@ -113,12 +113,12 @@ def hyp1f1_switch_point(a, b):
     return lib().orc_hyp1f1_switch_point(a, b)
 
 
-def hyp1f1_sph(max_radial, max_angular, r, alpha, fac_b, recursion=True):
+def hyp1f1_sph(max_radial, max_angular, r, alpha, fac_b, recursion=True, tol=1e-13):
     fac_b = np.ascontiguousarray(fac_b, dtype=np.float64)
     v = np.zeros((max_radial, max_angular + 1))
     dv = np.zeros_like(v)
     _check(lib().orc_hyp1f1_sph(max_radial, max_angular, r, alpha, _ptr(fac_b),
-                                int(recursion), _ptr(v), _ptr(dv)))
+                                int(recursion), float(tol), _ptr(v), _ptr(dv)))
     return v, dv
 
 

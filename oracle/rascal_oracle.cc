@@ -1306,9 +1306,9 @@ double orc_hyp1f1_switch_point(double a, double b) {
 }
 
 int orc_hyp1f1_sph(int N, int L, double r, double alpha, const double* fac_b,
-                   int recursion, double* values, double* derivatives) {
+                   int recursion, double tol, double* values, double* derivatives) {
   return guarded([&] {
-    Hyp1f1Sph h(recursion != 0, 1e-13, 200);
+    Hyp1f1Sph h(recursion != 0, tol, 200);
     h.precompute(N, L);
     h.calc(r, alpha, fac_b, true);
     std::copy(h.vals.begin(), h.vals.end(), values);

@@ -48,7 +48,7 @@ double orc_hyp1f1(double a, double b, double z, int derivative, int mmax,
 double orc_hyp1f1_switch_point(double a, double b);
 /* values/derivatives [N][L+1] of G(n,l) as in the GTO radial integral */
 int orc_hyp1f1_sph(int max_radial, int max_angular, double r, double alpha,
-                   const double* fac_b, int recursion, double* values,
+                   const double* fac_b, int recursion, double tol, double* values,
                    double* derivatives);
 /* harmonics[(L+1)^2]; derivatives[3][(L+1)^2] (may be NULL); alp[(L+1)*(L+2)]
  * (may be NULL) */
