@@ -598,11 +598,14 @@ int rgpu_context_create(const char* hypers_json, rgpu_context** out) {
   return guarded([&] {
     if (!hypers_json || !out) throw InvalidArgument("null argument");
     *out = nullptr;
+    // hypers are validated before any device call, so bad JSON is reported
+    // as an argument error on any host
+    SoapHypers hp = parse_soap_hypers(hypers_json);
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
       throw HipError("no HIP device available: the MI355X library needs a GPU");
     std::unique_ptr<rgpu_context> ctx(new rgpu_context());
-    ctx->c.hp = parse_soap_hypers(hypers_json);
+    ctx->c.hp = hp;
     build_tables(ctx->c);
     RGPU_HIP(hipDeviceSynchronize());
     *out = ctx.release();
