@@ -68,6 +68,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--cpu-centres", type=int, default=48)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="first run a plain 256 MiB -> 256 MiB elementwise copy, the reference "
+                         "kernel for scaling FETCH_SIZE / WRITE_SIZE counters to bytes")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: every box of this build has one MI355X, and the "
@@ -77,6 +80,15 @@ def main():
     import torch  # device plumbing only (synchronisation)
     from librascal_amd import _capi
     from librascal_amd.synthetic import pack, periodic_box
+
+    if args.calibrate_copy:
+        src = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            torch.mul(src, 1.0, out=dst)
+        torch.cuda.synchronize()
+        del src, dst
+        log("copy calibration: 3 x (256 MiB read + 256 MiB write)")
 
     w = WORKLOAD
     t0 = time.time()

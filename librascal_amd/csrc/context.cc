@@ -77,8 +77,8 @@ struct Context {
 
   // scratch
   DeviceBuffer<double> e_centre, grad, cvir, tmp64, out64;
-  DeviceBuffer<double> wt64, G64, W64;
-  DeviceBuffer<float> wt32, G32, W32;
+  DeviceBuffer<double> wt64, G64;
+  DeviceBuffer<float> wt32, G32;
   DeviceBuffer<int64_t> knm_perm;
 
   // stage timers (HIP events on `stream`)
@@ -183,7 +183,7 @@ void build_tables(Context& c) {
   r.hc = up(c.d_hc, h.hc);
   r.ha = up(c.d_ha, h.ha);
   r.use_spline = 0;
-  r.center = up(c.d_center, h.center_raw);
+  r.center = up(c.d_center, h.center_fin);  // finalised: R is finalised per pair
   if (c.hp.optimization != 1) return;
 
   // Spline of O^T R(r) on [1e-5, rc] (calculator_spherical_expansion.hh
@@ -406,26 +406,15 @@ template <>
 DeviceBuffer<float>& Gbuf<float>(Context& c) {
   return c.G32;
 }
-template <class T>
-DeviceBuffer<T>& Wbuf(Context& c);
-template <>
-DeviceBuffer<double>& Wbuf<double>(Context& c) {
-  return c.W64;
-}
-template <>
-DeviceBuffer<float>& Wbuf<float>(Context& c) {
-  return c.W32;
-}
-
 // Backward from G (device, [n][D] in precision T) to grad/virial (device).
+// xg (may be null): X_i . G_i per centre, times xg_scale.
 template <class T>
-void backward_device(Context& c, const T* G, double* grad, double* cvir) {
+void backward_device(Context& c, const T* G, double* grad, double* cvir, const double* xg = nullptr,
+                     double xg_scale = 1.0) {
   const int S = int(c.species.size());
-  const int64_t E = int64_t(S) * c.rd.N * c.rd.LL;
-  T* W = Wbuf<T>(c).reserve(size_t(std::max<int64_t>(c.n_atoms * E, 1)));
   StageTimer t(c, RGPU_STAGE_BACKWARD);
-  launch_backward<T>(c.rd, S, pair_view(c), nullptr, cbuf<T>(c).get(), pbuf<T>(c).get(), c.norm2.get(),
-                     c.hp.normalize, G, W, grad, cvir, c.stream);
+  launch_backward<T>(c.rd, S, pair_view(c), cbuf<T>(c).get(), pbuf<T>(c).get(), c.norm2.get(),
+                     c.hp.normalize, xg, xg_scale, G, grad, cvir, c.stream);
 }
 
 double cell_volume(const double* C) {
@@ -531,7 +520,8 @@ void gap_predict_t(Context& c, double* energy, double* gradient, double* neg_str
   double* gd = c.grad.reserve(size_t(std::max<int64_t>(3 * n, 1)));
   double* cv = c.cvir.reserve(size_t(std::max<int64_t>(6 * n, 1)));
   RGPU_HIP(hipMemsetAsync(gd, 0, sizeof(double) * 3 * n, c.stream));
-  backward_device<T>(c, G, gd, cv);
+  // the GAP energy is homogeneous of degree zeta in X, so X_i . G_i = zeta E_i
+  backward_device<T>(c, G, gd, cv, ec, double(c.zeta));
   emit_forces(c, gd, cv, gradient, neg_stress);
 }
 

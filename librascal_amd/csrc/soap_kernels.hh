@@ -36,13 +36,15 @@ void launch_normalise(int64_t n_centres, int64_t D, const T* p, const double* no
                       double* out, hipStream_t stream);
 
 // Adjoint: given G = dL/dX per centre (X the exported features), accumulate
-// dL/dr on atoms (grad[n_atoms][3], += ) and the structure virial
-// neg_virial[n_struct][6] (+= sum over pairs r_ji (x) dL/dr_j, Voigt
-// xx,yy,zz,yz,xz,xy, not divided by the volume).
+// dL/dr on atoms (grad[n_atoms][3], +=) and write per-centre virials
+// centre_virial[n_centres][6] (sum over the centre's pairs of r_ji (x)
+// dL/dr_j, Voigt xx,yy,zz,yz,xz,xy, not divided by the volume).  xg, when
+// not null, holds X_i . G_i per centre (times xg_scale); otherwise it is
+// reduced from p and G.
 template <class T>
-void launch_backward(const RadialDev& rad, int S, const PairView& pv, const int* atom_struct,
-                     const T* c, const T* p, const double* norm2, bool normalize, const T* G,
-                     T* dLdc_scratch, double* grad, double* neg_virial, hipStream_t stream);
+void launch_backward(const RadialDev& rad, int S, const PairView& pv, const T* c, const T* p,
+                     const double* norm2, bool normalize, const double* xg, double xg_scale,
+                     const T* G, double* grad, double* centre_virial, hipStream_t stream);
 
 // out[s][:w] += sum of in[a][:w] over a in [seg_off[s], seg_off[s+1])
 void launch_segment_sum(const double* in, int w, const int64_t* seg_off, int n_seg, double* out,

@@ -1,14 +1,25 @@
 // SOAP expansion, power spectrum and their adjoint on gfx950.
 //
-// One workgroup (256 threads, 4 wavefronts) owns one centre.  Pairs are
-// processed in batches: per pair the radial integrals come from the spline
-// table (coalesced row reads) or from FP64 Hyp1f1, the real spherical
-// harmonics from a wavefront-level recursion (one lane per m, lockstep over
-// l, neighbouring-m values by lane shuffles), and the per-centre coefficients
-// accumulate in registers, each thread owning a fixed set of (species, n, lm)
-// entries.  The result is identical to the reference's per-pair outer-product
-// accumulation (calculator_spherical_expansion.hh:2234-2410) up to rounding.
+// One workgroup (256 threads, 4 wavefronts) owns one centre.  Its pairs are
+// staged in LDS in batches: per pair the real spherical harmonics come from
+// a wavefront-level recursion (one lane per m, lockstep over l,
+// neighbouring-m values by lane shuffles) and the radial functions from the
+// spline table (or FP64 Hyp1f1 followed by the O^T orthonormalisation).
+// Pairs of a batch are grouped by neighbour species with wavefront ballots,
+// so the accumulation walks one species block at a time and each thread
+// keeps only a few (n, lm) accumulators in registers.  The result equals
+// the reference's per-pair outer-product accumulation
+// (calculator_spherical_expansion.hh:2234-2410) up to rounding.
+//
+// The adjoint keeps dL/dc of the centre in LDS: it is formed key block by
+// key block from dL/dp (power-spectrum transpose, register-tiled 4 x 1),
+// then the same workgroup recomputes Y_lm, dY_lm, R and dR per pair and
+// contracts them with dL/dc, so the per-pair gradient tensor is never
+// materialised.
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <string>
 
 #include "soap_device.hh"
 #include "soap_kernels.hh"
@@ -18,6 +29,38 @@ namespace rgpu {
 namespace {
 
 constexpr int kThreads = 256;
+constexpr int kWaves = kThreads / 64;
+
+template <class T>
+struct Vec4;
+template <>
+struct Vec4<float> {
+  typedef float type __attribute__((ext_vector_type(4)));
+};
+template <>
+struct Vec4<double> {
+  typedef double type __attribute__((ext_vector_type(4)));
+};
+
+// floor(sqrt(x)) for small non-negative x
+__device__ __forceinline__ int isqrt_small(int x) {
+  int r = int(sqrtf(float(x)));
+  while (r * r > x) --r;
+  while ((r + 1) * (r + 1) <= x) ++r;
+  return r;
+}
+
+// index (a, b), a <= b, of a species-pair key in the order (0,0), (0,1), ...
+__device__ __forceinline__ void key_species(int key, int S, int* a, int* b) {
+  int aa = 0, kk = key;
+  while (kk >= S - aa) {
+    kk -= S - aa;
+    ++aa;
+  }
+  *a = aa;
+  *b = aa + kk;
+}
+
 
 // ---------------------------------------------------------------------------
 // Real spherical harmonics of nq pairs, one wavefront segment of (L+1) lanes
@@ -143,158 +186,254 @@ __device__ void ylm_wave(int L, const T* dirs, int q0, int nq, T* Y, T* dY, int 
   }
 }
 
-// Radial values (and derivatives) of the batch: R[q][n*L1+l].
+// ---------------------------------------------------------------------------
+// LDS layout of one batch of PB pairs (FP64 values first for alignment).
 template <class T>
-__device__ void radial_batch(const RadialDev& rad, const double* dist, int nb, T* R, T* dR) {
-  const int F = rad.N * rad.L1;
-  if (rad.use_spline) {
-    for (int it = threadIdx.x; it < nb * F; it += blockDim.x) {
-      const int q = it / F, f = it - q * F;
-      SplineCell<T> cell = spline_cell<T>(rad, dist[q]);
-      T v, d;
-      spline_eval<T>(rad, cell, f, &v, dR ? &d : nullptr);
-      R[q * F + f] = v;
-      if (dR) dR[q * F + f] = d;
-    }
-  } else {
-    for (int it = threadIdx.x; it < nb * rad.N; it += blockDim.x) {
-      const int q = it / rad.N, n = it - q * rad.N;
-      double v[kMaxL1], d[kMaxL1];
-      gto_radial_n(rad, n, dist[q], v, dR ? d : nullptr);
-      for (int l = 0; l < rad.L1; ++l) {
-        R[q * F + n * rad.L1 + l] = T(v[l]);
-        if (dR) dR[q * F + n * rad.L1 + l] = T(d[l]);
-      }
-    }
-  }
+struct PairBatch {
+  double* dist;  // [PB]
+  double* fc;    // [PB] cutoff function
+  double* dfc;   // [PB] its derivative (adjoint)
+  double* raw;   // [PB][F] raw radial integrals (direct Hyp1f1)
+  double* draw;  // [PB][F] their derivatives (direct, adjoint)
+  T* Y;          // [PB][LL]
+  T* dY;         // [PB][3][LL] (adjoint)
+  T* R;          // [PB][F] finalised radial functions (x fc in the forward)
+  T* dR;         // [PB][F] (adjoint)
+  T* dir;        // [PB][3]
+  int* sp;       // [PB] neighbour species
+  int* lst;      // [PB] batch positions grouped by species (forward)
+};
+
+__host__ __device__ inline size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
+__host__ __device__ inline size_t pair_batch_bytes(int PB, int LL, int F, int tsize, bool direct,
+                                                   bool adjoint) {
+  size_t b = 8 * size_t(PB) * (adjoint ? 3 : 2);
+  if (direct) b += 8 * size_t(PB) * F * (adjoint ? 2 : 1);
+  b += size_t(tsize) * PB * (size_t(LL) * (adjoint ? 4 : 1) + size_t(F) * (adjoint ? 2 : 1) + 3);
+  b = align_up(b, 8);
+  b += 4 * size_t(PB) * 2;
+  return align_up(b, 16);
 }
 
-// ---------------------------------------------------------------------------
-// Expansion forward.
-constexpr int kPB = 16;  // pairs per batch
+template <class T, bool DIRECT, bool ADJ>
+__device__ PairBatch<T> carve_batch(unsigned char* base, int PB, int LL, int F) {
+  PairBatch<T> pb;
+  double* d = reinterpret_cast<double*>(base);
+  pb.dist = d;
+  d += PB;
+  pb.fc = d;
+  d += PB;
+  pb.dfc = nullptr;
+  if (ADJ) {
+    pb.dfc = d;
+    d += PB;
+  }
+  pb.raw = pb.draw = nullptr;
+  if (DIRECT) {
+    pb.raw = d;
+    d += size_t(PB) * F;
+    if (ADJ) {
+      pb.draw = d;
+      d += size_t(PB) * F;
+    }
+  }
+  T* t = reinterpret_cast<T*>(d);
+  pb.Y = t;
+  t += PB * LL;
+  pb.dY = nullptr;
+  if (ADJ) {
+    pb.dY = t;
+    t += 3 * PB * LL;
+  }
+  pb.R = t;
+  t += PB * F;
+  pb.dR = nullptr;
+  if (ADJ) {
+    pb.dR = t;
+    t += PB * F;
+  }
+  pb.dir = t;
+  t += 3 * PB;
+  const size_t off = align_up(size_t(reinterpret_cast<unsigned char*>(t) - base), 8);
+  pb.sp = reinterpret_cast<int*>(base + off);
+  pb.lst = pb.sp + PB;
+  return pb;
+}
 
-template <class T, int KPT>
-__global__ __launch_bounds__(kThreads) void expansion_kernel(RadialDev rad, int S, PairView pv,
-                                                             T* __restrict__ cout,
-                                                             unsigned long long* __restrict__ keymask) {
-  const int64_t i = blockIdx.x;
-  const int N = rad.N, L = rad.L, L1 = rad.L1, LL = rad.LL;
-  const int F = N * L1;
-  const int B = N * LL;          // entries per species block
-  const int E = S * B;           // entries per centre
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  T* Ysh = reinterpret_cast<T*>(smem_raw);               // [kPB][LL]
-  T* Rsh = Ysh + kPB * LL;                               // [kPB][F]
-  T* dirsh = Rsh + kPB * F;                              // [kPB][3]
-  T* fcsh = dirsh + 3 * kPB;                             // [kPB]
-  double* distsh = reinterpret_cast<double*>(fcsh + kPB);  // [kPB]
-  int* spsh = reinterpret_cast<int*>(distsh + kPB);        // [kPB]
-  __shared__ unsigned long long kmask;
-
-  const int64_t p0 = pv.centre_off[i], p1 = pv.centre_off[i + 1];
-  const int np = int(p1 - p0) - 1;  // without the self pair
-  const int si = pv.species_of[i];
-  if (threadIdx.x == 0) kmask = 1ull << si;
-
-  // thread-owned entries
-  T acc[KPT];
-  int ent_s[KPT], ent_nl[KPT], ent_lm[KPT];
-  const double inv_s4pi = 0.28209479177387814347;  // 1/sqrt(4 pi)
-#pragma unroll
-  for (int k = 0; k < KPT; ++k) {
-    const int e = threadIdx.x + kThreads * k;
-    acc[k] = T(0);
-    ent_s[k] = -1;
-    ent_nl[k] = 0;
-    ent_lm[k] = 0;
-    if (e < E) {
-      const int s = e / B, r = e - s * B, n = r / LL, lm = r - n * LL;
-      int l = 0;
-      while ((l + 1) * (l + 1) <= lm) ++l;
-      ent_s[k] = s;
-      ent_nl[k] = n * L1 + l;
-      ent_lm[k] = lm;
-      // centre contribution first (expansion.hh:2244-2247)
-      if (s == si && lm == 0) acc[k] = T(rad.center[n] * inv_s4pi);
+// Stages pairs [pbase, pbase + nb) of the pair list: distances, directions,
+// cutoff values, neighbour species, Y_lm (and gradients) and the finalised
+// radial functions (and derivatives); in the forward R is multiplied by fc.
+// When lists are requested (forward), wavefront 0 groups the batch by
+// species: s_count[s] pairs listed from lst[s_start[s]].  All threads call;
+// returns after a barrier.
+template <class T, bool DIRECT, bool ADJ>
+__device__ void stage_pairs(const RadialDev& rad, const PairView& pv, int64_t pbase, int nb,
+                            PairBatch<T>& b, int* s_start, int* s_count,
+                            unsigned long long* kmask) {
+  const int N = rad.N, L = rad.L, L1 = rad.L1, LL = rad.LL, F = N * L1;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  if (tid < nb) {
+    const int64_t p = pbase + tid;
+    const double d = pv.pair_dist[p];
+    b.dist[tid] = d;
+    b.dir[3 * tid] = T(pv.pair_vec[3 * p] / d);
+    b.dir[3 * tid + 1] = T(pv.pair_vec[3 * p + 1] / d);
+    b.dir[3 * tid + 2] = T(pv.pair_vec[3 * p + 2] / d);
+    double fc, dfc;
+    cutoff_eval(rad, d, &fc, ADJ ? &dfc : nullptr);
+    b.fc[tid] = fc;
+    if (ADJ) b.dfc[tid] = dfc;
+    b.sp[tid] = pv.species_of[pv.pair_j[p]];
+  }
+  __syncthreads();
+  if (s_count && wave == 0) {
+    const bool act = lane < nb;
+    const int s = act ? b.sp[lane] : -1;
+    unsigned long long rem = __ballot(act);
+    int base = 0;
+    while (rem) {
+      const int leader = __ffsll((long long)rem) - 1;
+      const int sl = __shfl(s, leader, 64);
+      const unsigned long long m = __ballot(act && s == sl);
+      const int cnt = __popcll(m);
+      if (act && s == sl) b.lst[base + __popcll(m & ((1ull << lane) - 1ull))] = lane;
+      if (lane == 0) {
+        s_start[sl] = base;
+        s_count[sl] = cnt;
+        *kmask |= 1ull << sl;
+      }
+      base += cnt;
+      rem &= ~m;
+    }
+  }
+  const int per_wave = 64 / L1;
+  for (int qb = wave * per_wave; qb < nb; qb += kWaves * per_wave)
+    ylm_wave<T>(L, b.dir, qb, nb, b.Y, ADJ ? b.dY : nullptr, LL, 3 * LL);
+  if (!DIRECT) {
+    for (int it = tid; it < nb * F; it += kThreads) {
+      const int q = it / F, f = it - q * F;
+      const SplineCell<T> cell = spline_cell<T>(rad, b.dist[q]);
+      T v, dv;
+      spline_eval<T>(rad, cell, f, &v, ADJ ? &dv : nullptr);
+      if (ADJ) {
+        b.R[it] = v;
+        b.dR[it] = dv;
+      } else {
+        b.R[it] = v * T(b.fc[q]);
+      }
+    }
+  } else {
+    for (int it = tid; it < nb * N; it += kThreads) {
+      const int q = it / N, n = it - q * N;
+      double v[kMaxL1], dv[kMaxL1];
+      gto_radial_n(rad, n, b.dist[q], v, ADJ ? dv : nullptr);
+      for (int l = 0; l < L1; ++l) {
+        b.raw[q * F + n * L1 + l] = v[l];
+        if (ADJ) b.draw[q * F + n * L1 + l] = dv[l];
+      }
+    }
+    __syncthreads();
+    // O^T R per pair (calculator_spherical_expansion.hh:605-618 applied
+    // before the accumulation, which is linear)
+    for (int it = tid; it < nb * F; it += kThreads) {
+      const int q = it / F, f = it - q * F, n = f / L1, l = f - n * L1;
+      double s = 0, ds = 0;
+      for (int m = 0; m < N; ++m) {
+        const double o = rad.O[m * N + n];
+        s += o * b.raw[q * F + m * L1 + l];
+        if (ADJ) ds += o * b.draw[q * F + m * L1 + l];
+      }
+      if (ADJ) {
+        b.R[it] = T(s);
+        b.dR[it] = T(ds);
+      } else {
+        b.R[it] = T(s * b.fc[q]);
+      }
     }
   }
   __syncthreads();
+}
 
-  for (int q0 = 0; q0 < np; q0 += kPB) {
-    const int nb = min(kPB, np - q0);
-    if (threadIdx.x < nb) {
-      const int q = threadIdx.x;
-      const int64_t p = p0 + 1 + q0 + q;
-      const double d = pv.pair_dist[p];
-      const int j = pv.pair_j[p];
-      distsh[q] = d;
-      dirsh[3 * q] = T(pv.pair_vec[3 * p] / d);
-      dirsh[3 * q + 1] = T(pv.pair_vec[3 * p + 1] / d);
-      dirsh[3 * q + 2] = T(pv.pair_vec[3 * p + 2] / d);
-      double fc;
-      cutoff_eval(rad, d, &fc, nullptr);
-      fcsh[q] = T(fc);
-      const int sj = pv.species_of[j];
-      spsh[q] = sj;
-      atomicOr(&kmask, 1ull << sj);
-    }
-    __syncthreads();
-    {
-      const int per_wave = 64 / L1;
-      const int wave = threadIdx.x >> 6;
-      for (int qb = wave * per_wave; qb < nb; qb += 4 * per_wave)
-        ylm_wave<T>(L, dirsh, qb, nb, Ysh, nullptr, LL, 0);
-    }
-    radial_batch<T>(rad, distsh, nb, Rsh, nullptr);
-    __syncthreads();
-    for (int q = 0; q < nb; ++q) {
-      const int s = spsh[q];
-      const T fq = fcsh[q];
-      const T* Yq = Ysh + q * LL;
-      const T* Rq = Rsh + q * F;
+// ---------------------------------------------------------------------------
+// Expansion forward: c[i][s][n][lm] = [s == s_i][lm == 0] R0[n] / sqrt(4 pi)
+//   + sum over pairs (i, j), s_j = s, of fc(r) R_nl(r) Y_lm(r_hat)
+// KPT = entries of one species block per thread.
+template <class T, int KPT, bool DIRECT>
+__global__ __launch_bounds__(kThreads) void expansion_kernel(RadialDev rad, int S, int PB,
+                                                             PairView pv, T* __restrict__ cout,
+                                                             unsigned long long* __restrict__ keymask) {
+  const int64_t i = blockIdx.x;
+  const int N = rad.N, L1 = rad.L1, LL = rad.LL, F = N * L1, B = N * LL;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  PairBatch<T> pb = carve_batch<T, DIRECT, false>(smem_raw, PB, LL, F);
+  __shared__ int s_start[kMaxSpecies], s_count[kMaxSpecies];
+  __shared__ unsigned long long kmask;
+  const int tid = threadIdx.x;
+  const int64_t p0 = pv.centre_off[i] + 1;  // the self pair contributes nothing here
+  const int np = int(pv.centre_off[i + 1] - p0);
+  const int si = pv.species_of[i];
+  const double inv_s4pi = 0.28209479177387814347;  // 1/sqrt(4 pi)
+
+  int ent[KPT];  // (n*L1 + l) << 16 | lm, or -1
 #pragma unroll
-      for (int k = 0; k < KPT; ++k)
-        if (ent_s[k] == s) acc[k] += (Rq[ent_nl[k]] * Yq[ent_lm[k]]) * fq;
+  for (int k = 0; k < KPT; ++k) {
+    const int e = tid + kThreads * k;
+    ent[k] = -1;
+    if (e < B) {
+      const int n = e / LL, lm = e - n * LL;
+      ent[k] = ((n * L1 + isqrt_small(lm)) << 16) | lm;
     }
-    __syncthreads();
   }
+  if (tid == 0) kmask = 1ull << si;
+  T* ci = cout + i * int64_t(S) * B;
 
-  T* ci = cout + i * int64_t(E);
-  if (rad.use_spline) {
-#pragma unroll
-    for (int k = 0; k < KPT; ++k) {
-      const int e = threadIdx.x + kThreads * k;
-      if (e < E) ci[e] = acc[k];
-    }
-  } else {
-    // c <- O^T c per species block (expansion.hh:605-618)
-    T* blk = Ysh;  // reuse: needs N*LL entries
+  for (int q0 = 0; q0 == 0 || q0 < np; q0 += PB) {
+    const int nb = max(0, min(PB, np - q0));
+    if (tid < S) s_count[tid] = 0;
+    stage_pairs<T, DIRECT, false>(rad, pv, p0 + q0, nb, pb, s_start, s_count, &kmask);
     for (int s = 0; s < S; ++s) {
+      const int cnt = s_count[s];
+      if (q0 > 0 && cnt == 0) continue;
+      const int st = s_start[s];
+      T* cs = ci + int64_t(s) * B;
+      T acc[KPT];
 #pragma unroll
       for (int k = 0; k < KPT; ++k) {
-        const int e = threadIdx.x + kThreads * k;
-        if (ent_s[k] == s) blk[e - s * B] = acc[k];
+        const int e = tid + kThreads * k;
+        acc[k] = T(0);
+        if (ent[k] >= 0) {
+          if (q0 > 0)
+            acc[k] = cs[e];
+          else if (s == si && (ent[k] & 0xffff) == 0)
+            acc[k] = T(rad.center[e / LL] * inv_s4pi);
+        }
       }
-      __syncthreads();
-      for (int r = threadIdx.x; r < B; r += kThreads) {
-        const int n = r / LL, lm = r - n * LL;
-        double v = 0;
-        for (int mm = 0; mm < N; ++mm) v += rad.O[mm * N + n] * double(blk[mm * LL + lm]);
-        ci[s * B + r] = T(v);
+      for (int t = 0; t < cnt; ++t) {
+        const int q = pb.lst[st + t];
+        const T* Yq = pb.Y + q * LL;
+        const T* Rq = pb.R + q * F;
+#pragma unroll
+        for (int k = 0; k < KPT; ++k)
+          if (ent[k] >= 0) acc[k] += Rq[ent[k] >> 16] * Yq[ent[k] & 0xffff];
       }
-      __syncthreads();
+#pragma unroll
+      for (int k = 0; k < KPT; ++k)
+        if (ent[k] >= 0) cs[tid + kThreads * k] = acc[k];
     }
+    __syncthreads();
   }
-  if (threadIdx.x == 0) keymask[i] = kmask;
+  if (tid == 0) keymask[i] = kmask;
 }
 
 // ---------------------------------------------------------------------------
 // Power spectrum: p^{ab}[n1 N + n2][l] = (2l+1)^{-1/2} (sqrt 2 if a<b)
 // sum_m c_a[n1,lm] c_b[n2,lm] (calculator_spherical_invariants.hh:747-788),
-// 4x4 register tiles over (n1, n2) per (key, l).  Un-normalised; |p|^2 is
+// 4x4 register tiles over (n1, n2) per (key, l); KC keys at a time are
+// staged in LDS and written out contiguously.  Un-normalised; |p|^2 is
 // returned per centre.
 template <class T>
-__global__ __launch_bounds__(kThreads) void power_spectrum_kernel(int N, int L, int S,
+__global__ __launch_bounds__(kThreads) void power_spectrum_kernel(int N, int L, int S, int KC,
                                                                   const T* __restrict__ c,
                                                                   T* __restrict__ p,
                                                                   double* __restrict__ norm2) {
@@ -302,69 +441,73 @@ __global__ __launch_bounds__(kThreads) void power_spectrum_kernel(int N, int L, 
   const int L1 = L + 1, LL = L1 * L1, B = N * LL;
   const int K = S * (S + 1) / 2;
   const int NT = (N + 3) / 4;
+  const int KB = N * N * L1;  // one key block
   extern __shared__ __align__(16) unsigned char smem_raw[];
   T* cs = reinterpret_cast<T*>(smem_raw);  // [S][N][LL]
-  __shared__ double red[kThreads / 64];
+  T* stage = cs + S * B;                   // [KC][N][N][L1]
+  __shared__ double red[kWaves];
   const T* ci = c + i * int64_t(S) * B;
   for (int e = threadIdx.x; e < S * B; e += kThreads) cs[e] = ci[e];
   __syncthreads();
-  const int64_t D = int64_t(K) * N * N * L1;
-  T* pi = p + i * D;
+  T* pi = p + i * int64_t(K) * KB;
   double ss = 0;
-  const int items = K * L1 * NT * NT;
-  for (int it = threadIdx.x; it < items; it += kThreads) {
-    int r = it;
-    const int t2 = r % NT;
-    r /= NT;
-    const int t1 = r % NT;
-    r /= NT;
-    const int l = r % L1;
-    const int key = r / L1;
-    int a = 0, kk = key;
-    while (kk >= S - a) {
-      kk -= S - a;
-      ++a;
-    }
-    const int b = a + kk;
-    const T* ca = cs + a * B;
-    const T* cb = cs + b * B;
-    const int n1 = 4 * t1, n2 = 4 * t2;
-    T acc[4][4];
-#pragma unroll
-    for (int u = 0; u < 4; ++u)
-#pragma unroll
-      for (int v = 0; v < 4; ++v) acc[u][v] = T(0);
-    for (int lm = l * l; lm < (l + 1) * (l + 1); ++lm) {
-      T va[4], vb[4];
-#pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        va[u] = (n1 + u < N) ? ca[(n1 + u) * LL + lm] : T(0);
-        vb[u] = (n2 + u < N) ? cb[(n2 + u) * LL + lm] : T(0);
-      }
+  for (int k0 = 0; k0 < K; k0 += KC) {
+    const int kc = min(KC, K - k0);
+    const int items = kc * L1 * NT * NT;
+    for (int it = threadIdx.x; it < items; it += kThreads) {
+      int r = it;
+      const int t2 = r % NT;
+      r /= NT;
+      const int t1 = r % NT;
+      r /= NT;
+      const int l = r % L1;
+      const int kk = r / L1;
+      int a, b;
+      key_species(k0 + kk, S, &a, &b);
+      const T* ca = cs + a * B;
+      const T* cb = cs + b * B;
+      const int n1 = 4 * t1, n2 = 4 * t2;
+      T acc[4][4];
 #pragma unroll
       for (int u = 0; u < 4; ++u)
 #pragma unroll
-        for (int v = 0; v < 4; ++v) acc[u][v] += va[u] * vb[v];
-    }
-    T f = T(1) / sqrt(T(2 * l + 1));
-    if (a < b) f *= T(1.41421356237309504880);
-    T* pk = pi + int64_t(key) * N * N * L1;
+        for (int v = 0; v < 4; ++v) acc[u][v] = T(0);
+      for (int lm = l * l; lm < (l + 1) * (l + 1); ++lm) {
+        T va[4], vb[4];
 #pragma unroll
-    for (int u = 0; u < 4; ++u)
-#pragma unroll
-      for (int v = 0; v < 4; ++v)
-        if (n1 + u < N && n2 + v < N) {
-          const T val = acc[u][v] * f;
-          pk[((n1 + u) * N + n2 + v) * L1 + l] = val;
-          ss += double(val) * double(val);
+        for (int u = 0; u < 4; ++u) {
+          va[u] = (n1 + u < N) ? ca[(n1 + u) * LL + lm] : T(0);
+          vb[u] = (n2 + u < N) ? cb[(n2 + u) * LL + lm] : T(0);
         }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+          for (int v = 0; v < 4; ++v) acc[u][v] += va[u] * vb[v];
+      }
+      T f = T(1) / sqrt(T(2 * l + 1));
+      if (a < b) f *= T(1.41421356237309504880);
+      T* sk = stage + kk * KB;
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+          if (n1 + u < N && n2 + v < N) sk[((n1 + u) * N + n2 + v) * L1 + l] = acc[u][v] * f;
+    }
+    __syncthreads();
+    T* out = pi + int64_t(k0) * KB;
+    for (int o = threadIdx.x; o < kc * KB; o += kThreads) {
+      const T v = stage[o];
+      out[o] = v;
+      ss += double(v) * double(v);
+    }
+    __syncthreads();
   }
   for (int off = 32; off > 0; off >>= 1) ss += __shfl_down(ss, off, 64);
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = ss;
   __syncthreads();
   if (threadIdx.x == 0) {
     double t = 0;
-    for (int w = 0; w < kThreads / 64; ++w) t += red[w];
+    for (int w = 0; w < kWaves; ++w) t += red[w];
     norm2[i] = t;
   }
 }
@@ -387,188 +530,156 @@ __global__ void normalise_kernel(int64_t n, int64_t D, const T* __restrict__ p,
 }
 
 // ---------------------------------------------------------------------------
-// Adjoint, step 1: dL/dX -> dL/dc_raw per centre.
-//   dL/dp = (G - X (X.G)) / |p| when normalised (calculator_spherical_
-//   invariants.hh:588-652 is the forward form of this projection);
-//   dL/dc_a[k,lm] = sum over keys containing a of the transposed contraction
-//   of the power-spectrum product; finally dL/dc_raw = O dL/dc for the direct
-//   (un-splined) radial basis.
-template <class T, int KPT>
-__global__ __launch_bounds__(kThreads) void backward_dc_kernel(RadialDev rad, int S,
-                                                               const T* __restrict__ c,
-                                                               const T* __restrict__ p,
-                                                               const double* __restrict__ norm2,
-                                                               int normalize, const T* __restrict__ G,
-                                                               T* __restrict__ W) {
+// Fused adjoint.  Given G = dL/dX (X = p/|p| the exported features):
+//  1. dL/dp = G/|p| - (X.G) p/|p|^2 (the transpose of the normalisation
+//     derivative, calculator_spherical_invariants.hh:588-652); X.G is either
+//     passed per centre (for the GAP energy X.G = zeta E_i, the energy being
+//     homogeneous of degree zeta in X) or reduced here;
+//  2. dL/dc_a[n,lm] = sum over keys (a,b) of (2l+1)^{-1/2} (sqrt 2 if a<b)
+//     sum_n2 dL/dp^{ab}[n,n2,l] c_b[n2,lm] (both sides of each key; the
+//     diagonal keys symmetrised), kept in LDS;
+//  3. per pair (i, j): g = sum_{n,lm} dL/dc[s_j][n][lm]
+//       [ (R' fc + R fc') Y r_hat + R fc grad_Y / r ]
+//     (the pair-gradient formula of calculator_spherical_expansion.hh
+//     :2306-2366 contracted with dL/dc), grad[j] += g, and the centre gets
+//     -sum g unless j is a periodic image of i; the virial sum r_ji (x) g is
+//     reduced per centre (sparse_kernel_predict.hh:354-364).
+template <class T, bool DIRECT>
+__global__ __launch_bounds__(kThreads) void adjoint_kernel(
+    RadialDev rad, int S, int PB, int c_in_lds, PairView pv, const T* __restrict__ c,
+    const T* __restrict__ p, const double* __restrict__ norm2, int normalize,
+    const double* __restrict__ xg, double xg_scale, const T* __restrict__ G,
+    double* __restrict__ grad, double* __restrict__ centre_virial) {
+  typedef typename Vec4<T>::type V4;
   const int64_t i = blockIdx.x;
-  const int N = rad.N, L1 = rad.L1, LL = rad.LL, B = N * LL, E = S * B;
+  const int N = rad.N, L1 = rad.L1, LL = rad.LL, F = N * L1, B = N * LL, E = S * B;
+  const int N4 = (N + 3) & ~3;
   const int K = S * (S + 1) / 2;
-  const int64_t D = int64_t(K) * N * N * L1;
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  T* cs = reinterpret_cast<T*>(smem_raw);  // [S][N][LL]
-  __shared__ double red[kThreads / 64];
-  __shared__ double sh_alpha, sh_beta;
-  const T* ci = c + i * int64_t(E);
-  const T* pi = p + i * D;
-  const T* gi = G + i * D;
-  for (int e = threadIdx.x; e < E; e += kThreads) cs[e] = ci[e];
-  // alpha = 1/|p|, beta = (p.G)/|p|^3 so that dL/dp = alpha G - beta p
-  double dot = 0;
-  if (normalize)
-    for (int64_t o = threadIdx.x; o < D; o += kThreads) dot += double(pi[o]) * double(gi[o]);
-  for (int off = 32; off > 0; off >>= 1) dot += __shfl_down(dot, off, 64);
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = dot;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double t = 0;
-    for (int w = 0; w < kThreads / 64; ++w) t += red[w];
-    const double n2 = norm2[i];
-    if (normalize && n2 > 0) {
-      const double inv = 1.0 / sqrt(n2);
-      sh_alpha = inv;
-      sh_beta = t * inv * inv * inv;
-    } else {
-      sh_alpha = 1.0;
-      sh_beta = 0.0;
-    }
-  }
-  __syncthreads();
-  const T alpha = T(sh_alpha), beta = T(sh_beta);
-  T out[KPT];
-#pragma unroll
-  for (int k = 0; k < KPT; ++k) {
-    const int e = threadIdx.x + kThreads * k;
-    out[k] = T(0);
-    if (e >= E) continue;
-    const int a = e / B, r = e - a * B, kn = r / LL, lm = r - kn * LL;
-    int l = 0;
-    while ((l + 1) * (l + 1) <= lm) ++l;
-    const T lf = T(1) / sqrt(T(2 * l + 1));
-    T sum = T(0);
-    for (int b = 0; b < S; ++b) {
-      const int lo = a < b ? a : b, hi = a < b ? b : a;
-      const int key = lo * S - lo * (lo - 1) / 2 + (hi - lo);
-      const T w = (lo < hi) ? T(1.41421356237309504880) : T(1);
-      const int64_t kb = int64_t(key) * N * N * L1;
-      const T* cb = cs + b * B;
-      T s1 = T(0);
-      for (int n = 0; n < N; ++n) {
-        // element (a's index k, b's index n) of the key block
-        int64_t o1 = (a <= b) ? kb + (int64_t(kn) * N + n) * L1 + l : kb + (int64_t(n) * N + kn) * L1 + l;
-        T g1 = alpha * gi[o1] - beta * pi[o1];
-        T term = g1;
-        if (a == b) {
-          int64_t o2 = kb + (int64_t(n) * N + kn) * L1 + l;
-          term += alpha * gi[o2] - beta * pi[o2];
-        }
-        s1 += term * cb[n * LL + lm];
-      }
-      sum += w * lf * s1;
-    }
-    out[k] = sum;
-  }
-  T* Wi = W + i * int64_t(E);
-  if (rad.use_spline) {
-#pragma unroll
-    for (int k = 0; k < KPT; ++k) {
-      const int e = threadIdx.x + kThreads * k;
-      if (e < E) Wi[e] = out[k];
-    }
+  const int KB = N * N * L1;
+  const int64_t D = int64_t(K) * KB;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  extern __shared__ __align__(64) unsigned char smem_raw[];
+  T* dc = reinterpret_cast<T*>(smem_raw);  // [S][N][LL]
+  unsigned char* region = smem_raw + align_up(sizeof(T) * E, 64);
+  const T* cs;
+  T* gA;
+  if (c_in_lds) {
+    T* cl = reinterpret_cast<T*>(region);
+    for (int e = tid; e < E; e += kThreads) cl[e] = c[i * int64_t(E) + e];
+    cs = cl;
+    gA = reinterpret_cast<T*>(region + align_up(sizeof(T) * E, 64));
   } else {
-    __syncthreads();
-    // reuse cs as the staging block for dL/dc -> O dL/dc
-#pragma unroll
-    for (int k = 0; k < KPT; ++k) {
-      const int e = threadIdx.x + kThreads * k;
-      if (e < E) cs[e] = out[k];
+    cs = c + i * int64_t(E);
+    gA = reinterpret_cast<T*>(region);
+  }
+  T* gB = gA + L1 * N * N4;  // gA[l][x][y] = g[y][x][l], gB[l][x][y] = g[x][y][l]
+  __shared__ double red[kWaves][9];
+  __shared__ double sh_ab[2];
+
+  const T* Gi = G + i * D;
+  const T* Pi = p + i * D;
+  double alpha = 1.0, beta = 0.0;
+  if (normalize) {
+    const double n2 = norm2[i];
+    double dotxg = 0;
+    if (xg) {
+      dotxg = xg_scale * xg[i];
+    } else {
+      double dot = 0;
+      for (int64_t o = tid; o < D; o += kThreads) dot += double(Pi[o]) * double(Gi[o]);
+      for (int off = 32; off > 0; off >>= 1) dot += __shfl_down(dot, off, 64);
+      if (lane == 0) red[wave][0] = dot;
+      __syncthreads();
+      if (tid == 0) {
+        double t = 0;
+        for (int w = 0; w < kWaves; ++w) t += red[w][0];
+        sh_ab[0] = t;
+      }
+      __syncthreads();
+      dotxg = n2 > 0 ? sh_ab[0] / sqrt(n2) : 0.0;
     }
-    __syncthreads();
-    for (int e = threadIdx.x; e < E; e += kThreads) {
-      const int s = e / B, r = e - s * B, mrow = r / LL, lm = r - mrow * LL;
-      double v = 0;
-      for (int n = 0; n < N; ++n) v += rad.O[mrow * N + n] * double(cs[s * B + n * LL + lm]);
-      Wi[e] = T(v);
+    if (n2 > 0) {
+      alpha = 1.0 / sqrt(n2);
+      beta = dotxg / n2;
     }
   }
-}
+  const T ta = T(alpha), tb = T(beta);
+  for (int e = tid; e < E; e += kThreads) dc[e] = T(0);
+  for (int e = tid; e < 2 * L1 * N * N4; e += kThreads) gA[e] = T(0);  // zero pads
+  __syncthreads();
 
-// Adjoint, step 2: per pair g = dL/dr_j for the pair (i, j):
-//   g = sum_{n,lm} W[s_j][n][lm] [ (R' fc + R fc') Y r_hat + R fc grad_Y / r ]
-// (the pair-gradient formula of calculator_spherical_expansion.hh:2306-2366
-// contracted with W), scattered to grad[j] += g and grad[i] -= g unless j is
-// a periodic image of i; the virial sum r_ji (x) g is reduced per centre.
-constexpr int kPBB = 4;  // pairs per batch in the adjoint (one per wavefront)
-
-template <class T>
-__global__ __launch_bounds__(kThreads) void backward_pairs_kernel(RadialDev rad, int S, PairView pv,
-                                                                  const T* __restrict__ W,
-                                                                  double* __restrict__ grad,
-                                                                  double* __restrict__ centre_virial) {
-  const int64_t i = blockIdx.x;
-  const int N = rad.N, L = rad.L, L1 = rad.L1, LL = rad.LL;
-  const int F = N * L1, B = N * LL;
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  T* Ysh = reinterpret_cast<T*>(smem_raw);  // [kPBB][LL]
-  T* dYsh = Ysh + kPBB * LL;                // [kPBB][3][LL]
-  T* Rsh = dYsh + kPBB * 3 * LL;            // [kPBB][F]
-  T* dRsh = Rsh + kPBB * F;                 // [kPBB][F]
-  T* dirsh = dRsh + kPBB * F;               // [kPBB][3]
-  double* distsh = reinterpret_cast<double*>(dirsh + 3 * kPBB);  // [kPBB]
-  double* fcsh = distsh + kPBB;                                  // [kPBB]
-  double* dfcsh = fcsh + kPBB;                                   // [kPBB]
-  int* spsh = reinterpret_cast<int*>(dfcsh + kPBB);              // [kPBB]
-  __shared__ double vir[kThreads / 64][6];
-
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int64_t p0 = pv.centre_off[i], p1 = pv.centre_off[i + 1];
-  const int np = int(p1 - p0) - 1;
-  const T* Wi = W + i * int64_t(S) * B;
-  double v6[6] = {0, 0, 0, 0, 0, 0};
-  for (int q0 = 0; q0 < np; q0 += kPBB) {
-    const int nb = min(kPBB, np - q0);
-    if (threadIdx.x < nb) {
-      const int q = threadIdx.x;
-      const int64_t p = p0 + 1 + q0 + q;
-      const double d = pv.pair_dist[p];
-      distsh[q] = d;
-      dirsh[3 * q] = T(pv.pair_vec[3 * p] / d);
-      dirsh[3 * q + 1] = T(pv.pair_vec[3 * p + 1] / d);
-      dirsh[3 * q + 2] = T(pv.pair_vec[3 * p + 2] / d);
-      double fc, dfc;
-      cutoff_eval(rad, d, &fc, &dfc);
-      fcsh[q] = fc;
-      dfcsh[q] = dfc;
-      spsh[q] = pv.species_of[pv.pair_j[p]];
+  const int units_a = (N4 / 4) * LL;
+  for (int key = 0; key < K; ++key) {
+    int a, b;
+    key_species(key, S, &a, &b);
+    const T* Gk = Gi + int64_t(key) * KB;
+    const T* Pk = Pi + int64_t(key) * KB;
+    for (int o = tid; o < KB; o += kThreads) {
+      const int x = o / (N * L1), r = o - x * N * L1, y = r / L1, l = r - y * L1;
+      const T g = ta * Gk[o] - tb * Pk[o];
+      if (a == b) {
+        const int o2 = (y * N + x) * L1 + l;
+        gA[(l * N + y) * N4 + x] = g + (ta * Gk[o2] - tb * Pk[o2]);
+      } else {
+        gA[(l * N + y) * N4 + x] = g;
+        gB[(l * N + x) * N4 + y] = g;
+      }
     }
     __syncthreads();
-    {
-      const int per_wave = 64 / L1;
-      for (int qb = wave * per_wave; qb < nb; qb += 4 * per_wave)
-        ylm_wave<T>(L, dirsh, qb, nb, Ysh, dYsh, LL, 3 * LL);
+    const T w = (a < b) ? T(1.41421356237309504880) : T(1);
+    const int units = units_a * (a < b ? 2 : 1);
+    for (int u = tid; u < units; u += kThreads) {
+      const bool side_b = u >= units_a;
+      const int uu = side_b ? u - units_a : u;
+      const int nt = uu / LL, lm = uu - nt * LL;
+      const int l = isqrt_small(lm);
+      const T* gt = (side_b ? gB : gA) + l * N * N4 + 4 * nt;
+      const T* cx = cs + (side_b ? a : b) * B + lm;
+      V4 s4 = {T(0), T(0), T(0), T(0)};
+      for (int x = 0; x < N; ++x) {
+        const T cv = cx[x * LL];
+        const V4 g4 = *reinterpret_cast<const V4*>(gt + x * N4);
+        s4 += g4 * cv;
+      }
+      const T f = w / sqrt(T(2 * l + 1));
+      T* out = dc + (side_b ? b : a) * B + lm;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int n = 4 * nt + v;
+        if (n < N) out[n * LL] += f * s4[v];
+      }
     }
-    radial_batch<T>(rad, distsh, nb, Rsh, dRsh);
     __syncthreads();
-    if (wave < nb) {
-      const int q = wave;
-      const T fc = T(fcsh[q]), dfc = T(dfcsh[q]);
-      const T* Ws = Wi + spsh[q] * B;
-      const T* Yq = Ysh + q * LL;
-      const T* dYq = dYsh + q * 3 * LL;
-      const T* Rq = Rsh + q * F;
-      const T* dRq = dRsh + q * F;
+  }
+
+  // per-pair contraction; the batch reuses the region after dL/dc
+  PairBatch<T> pb = carve_batch<T, DIRECT, true>(region, PB, LL, F);
+  const int64_t p0 = pv.centre_off[i] + 1;
+  const int np = int(pv.centre_off[i + 1] - p0);
+  double fi[3] = {0, 0, 0}, v6[6] = {0, 0, 0, 0, 0, 0};
+  for (int q0 = 0; q0 < np; q0 += PB) {
+    const int nb = min(PB, np - q0);
+    stage_pairs<T, DIRECT, true>(rad, pv, p0 + q0, nb, pb, nullptr, nullptr, nullptr);
+    for (int q = wave; q < nb; q += kWaves) {
+      const T* W = dc + pb.sp[q] * B;
+      const T* Yq = pb.Y + q * LL;
+      const T* dYq = pb.dY + q * 3 * LL;
+      const T* Rq = pb.R + q * F;
+      const T* dRq = pb.dR + q * F;
+      const T fc = T(pb.fc[q]), dfc = T(pb.dfc[q]);
       T A = T(0), Bx = T(0), By = T(0), Bz = T(0);
-      for (int e = lane; e < B; e += 64) {
-        const int n = e / LL, lm = e - n * LL;
-        int l = 0;
-        while ((l + 1) * (l + 1) <= lm) ++l;
-        const T w = Ws[e];
-        const T r = Rq[n * L1 + l];
-        const T dr = dRq[n * L1 + l];
-        A += w * (dr * fc + r * dfc) * Yq[lm];
-        const T wr = w * r;
-        Bx += wr * dYq[lm];
-        By += wr * dYq[LL + lm];
-        Bz += wr * dYq[2 * LL + lm];
+      for (int lm = lane; lm < LL; lm += 64) {
+        const int l = isqrt_small(lm);
+        T Dn = T(0), Ddn = T(0);
+        for (int n = 0; n < N; ++n) {
+          const T wv = W[n * LL + lm];
+          Dn += wv * Rq[n * L1 + l];
+          Ddn += wv * dRq[n * L1 + l];
+        }
+        A += (Ddn * fc + Dn * dfc) * Yq[lm];
+        Bx += Dn * dYq[lm];
+        By += Dn * dYq[LL + lm];
+        Bz += Dn * dYq[2 * LL + lm];
       }
       for (int off = 32; off > 0; off >>= 1) {
         A += __shfl_down(A, off, 64);
@@ -577,40 +688,48 @@ __global__ __launch_bounds__(kThreads) void backward_pairs_kernel(RadialDev rad,
         Bz += __shfl_down(Bz, off, 64);
       }
       if (lane == 0) {
-        const double d = distsh[q];
-        const double sc = double(fc) / d;
-        const double g[3] = {double(A) * double(dirsh[3 * q]) + sc * double(Bx),
-                             double(A) * double(dirsh[3 * q + 1]) + sc * double(By),
-                             double(A) * double(dirsh[3 * q + 2]) + sc * double(Bz)};
-        const int64_t p = p0 + 1 + q0 + q;
-        const int j = pv.pair_j[p];
-        atomicAdd(grad + 3 * j, g[0]);
-        atomicAdd(grad + 3 * j + 1, g[1]);
-        atomicAdd(grad + 3 * j + 2, g[2]);
+        const double d = pb.dist[q];
+        const double sc = pb.fc[q] / d;
+        const int64_t pidx = p0 + q0 + q;
+        const double ux = pv.pair_vec[3 * pidx] / d, uy = pv.pair_vec[3 * pidx + 1] / d,
+                     uz = pv.pair_vec[3 * pidx + 2] / d;
+        const double g[3] = {double(A) * ux + sc * double(Bx), double(A) * uy + sc * double(By),
+                             double(A) * uz + sc * double(Bz)};
+        const int j = pv.pair_j[pidx];
+        unsafeAtomicAdd(grad + 3 * j, g[0]);
+        unsafeAtomicAdd(grad + 3 * j + 1, g[1]);
+        unsafeAtomicAdd(grad + 3 * j + 2, g[2]);
         if (j != int(i)) {
-          atomicAdd(grad + 3 * i, -g[0]);
-          atomicAdd(grad + 3 * i + 1, -g[1]);
-          atomicAdd(grad + 3 * i + 2, -g[2]);
+          fi[0] -= g[0];
+          fi[1] -= g[1];
+          fi[2] -= g[2];
         }
-        // r_ji = r_i - r_j(image) = -pair vector (sparse_kernel_predict.hh:354-364)
-        const double rji[3] = {-pv.pair_vec[3 * p], -pv.pair_vec[3 * p + 1], -pv.pair_vec[3 * p + 2]};
-        v6[0] += rji[0] * g[0];
-        v6[4] += rji[2] * g[0];
-        v6[1] += rji[1] * g[1];
-        v6[5] += rji[0] * g[1];
-        v6[2] += rji[2] * g[2];
-        v6[3] += rji[1] * g[2];
+        // r_ji = r_i - r_j(image) = -pair vector
+        const double rx = -pv.pair_vec[3 * pidx], ry = -pv.pair_vec[3 * pidx + 1],
+                     rz = -pv.pair_vec[3 * pidx + 2];
+        v6[0] += rx * g[0];
+        v6[1] += ry * g[1];
+        v6[2] += rz * g[2];
+        v6[3] += ry * g[2];
+        v6[4] += rz * g[0];
+        v6[5] += rx * g[1];
       }
     }
     __syncthreads();
   }
-  if (lane == 0)
-    for (int k = 0; k < 6; ++k) vir[wave][k] = v6[k];
+  if (lane == 0) {
+    for (int k = 0; k < 3; ++k) red[wave][k] = fi[k];
+    for (int k = 0; k < 6; ++k) red[wave][3 + k] = v6[k];
+  }
   __syncthreads();
-  if (threadIdx.x < 6) {
+  if (tid < 9) {
     double t = 0;
-    for (int w = 0; w < kThreads / 64; ++w) t += vir[w][threadIdx.x];
-    centre_virial[6 * i + threadIdx.x] = t;
+    for (int w = 0; w < kWaves; ++w) t += red[w][tid];
+    if (tid < 3) {
+      if (t != 0.0) unsafeAtomicAdd(grad + 3 * i + tid, t);
+    } else {
+      centre_virial[6 * i + tid - 3] = t;
+    }
   }
 }
 
@@ -620,7 +739,7 @@ __global__ void segment_sum_kernel(const double* __restrict__ in, int w,
                                    double* __restrict__ out) {
   const int s = blockIdx.x;
   if (s >= n_seg) return;
-  __shared__ double red[kThreads / 64][8];
+  __shared__ double red[kWaves][8];
   double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   for (int64_t a = seg_off[s] + threadIdx.x; a < seg_off[s + 1]; a += kThreads)
     for (int k = 0; k < w; ++k) v[k] += in[int64_t(w) * a + k];
@@ -631,7 +750,7 @@ __global__ void segment_sum_kernel(const double* __restrict__ in, int w,
   __syncthreads();
   if (threadIdx.x < w) {
     double t = 0;
-    for (int x = 0; x < kThreads / 64; ++x) t += red[x][threadIdx.x];
+    for (int x = 0; x < kWaves; ++x) t += red[x][threadIdx.x];
     out[int64_t(w) * s + threadIdx.x] += t;
   }
 }
@@ -704,24 +823,70 @@ __global__ __launch_bounds__(kThreads) void ylm_eval_kernel(int L, const double*
     for (int t = threadIdx.x; t < nb * 3 * LL; t += kThreads) dY[q0 * 3 * LL + t] = double(dYsh[t]);
 }
 
-template <class T, int KPT>
-void launch_expansion_kpt(const RadialDev& rad, int S, const PairView& pv, T* c,
-                          unsigned long long* keymask, hipStream_t stream) {
+// LDS per workgroup: the forward and the power spectrum stay small enough
+// for several workgroups per CU; the adjoint may use more.
+constexpr size_t kFwdBudget = 32 * 1024;
+constexpr size_t kPsBudget = 48 * 1024;
+constexpr size_t kAdjBudget = 64 * 1024;
+
+size_t device_lds_limit() {
+  int dev = 0, v = 0;
+  RGPU_HIP(hipGetDevice(&dev));
+  RGPU_HIP(hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+  return size_t(v);
+}
+
+template <class K>
+void allow_lds(K kernel, size_t smem) {
+  if (smem > device_lds_limit())
+    throw InvalidArgument("hypers need " + std::to_string(smem) +
+                          " bytes of LDS per workgroup, more than the device provides");
+  if (smem > 64 * 1024)
+    RGPU_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, int(smem)));
+}
+
+template <class T, int KPT, bool DIRECT>
+void launch_expansion_t(const RadialDev& rad, int S, const PairView& pv, T* c,
+                        unsigned long long* keymask, hipStream_t stream) {
   const int F = rad.N * rad.L1;
-  size_t smem = sizeof(T) * (kPB * rad.LL + kPB * F + 4 * kPB) + sizeof(double) * kPB + sizeof(int) * kPB;
-  size_t fin = sizeof(T) * rad.N * rad.LL;
-  if (fin > smem) smem = fin;
-  hipLaunchKernelGGL((expansion_kernel<T, KPT>), dim3(unsigned(pv.n_centres)), dim3(kThreads), smem,
-                     stream, rad, S, pv, c, keymask);
+  int PB = 64;
+  while (PB > 1 && pair_batch_bytes(PB, rad.LL, F, sizeof(T), DIRECT, false) > kFwdBudget) PB /= 2;
+  const size_t smem = pair_batch_bytes(PB, rad.LL, F, sizeof(T), DIRECT, false);
+  auto kern = expansion_kernel<T, KPT, DIRECT>;
+  allow_lds(kern, smem);
+  hipLaunchKernelGGL(kern, dim3(unsigned(pv.n_centres)), dim3(kThreads), smem, stream, rad, S, PB, pv,
+                     c, keymask);
   RGPU_LAUNCH_CHECK();
 }
 
 template <class T, int KPT>
-void launch_backward_dc_kpt(const RadialDev& rad, int S, int64_t n, const T* c, const T* p,
-                            const double* norm2, bool normalize, const T* G, T* W, hipStream_t stream) {
-  size_t smem = sizeof(T) * S * rad.N * rad.LL;
-  hipLaunchKernelGGL((backward_dc_kernel<T, KPT>), dim3(unsigned(n)), dim3(kThreads), smem, stream, rad,
-                     S, c, p, norm2, int(normalize), G, W);
+void launch_expansion_k(const RadialDev& rad, int S, const PairView& pv, T* c,
+                        unsigned long long* keymask, hipStream_t stream) {
+  if (rad.use_spline)
+    launch_expansion_t<T, KPT, false>(rad, S, pv, c, keymask, stream);
+  else
+    launch_expansion_t<T, KPT, true>(rad, S, pv, c, keymask, stream);
+}
+
+template <class T, bool DIRECT>
+void launch_adjoint_t(const RadialDev& rad, int S, const PairView& pv, const T* c, const T* p,
+                      const double* norm2, bool normalize, const double* xg, double xg_scale,
+                      const T* G, double* grad, double* centre_virial, hipStream_t stream) {
+  const int N = rad.N, L1 = rad.L1, LL = rad.LL, F = N * L1, E = S * N * LL;
+  const int N4 = (N + 3) & ~3;
+  const size_t blk = align_up(sizeof(T) * size_t(E), 64);
+  const size_t gbytes = sizeof(T) * 2 * size_t(L1) * N * N4;
+  const int c_in_lds = blk + blk + gbytes <= kAdjBudget ? 1 : 0;
+  const size_t region = (c_in_lds ? blk : 0) + gbytes;
+  const size_t batch_room = std::max(region, kAdjBudget > blk ? kAdjBudget - blk : size_t(0));
+  int PB = 8;
+  while (PB > 1 && pair_batch_bytes(PB, LL, F, sizeof(T), DIRECT, true) > batch_room) --PB;
+  const size_t smem = blk + std::max(region, pair_batch_bytes(PB, LL, F, sizeof(T), DIRECT, true));
+  auto kern = adjoint_kernel<T, DIRECT>;
+  allow_lds(kern, smem);
+  hipLaunchKernelGGL(kern, dim3(unsigned(pv.n_centres)), dim3(kThreads), smem, stream, rad, S, PB,
+                     c_in_lds, pv, c, p, norm2, int(normalize), xg, xg_scale, G, grad, centre_virial);
   RGPU_LAUNCH_CHECK();
 }
 
@@ -731,32 +896,41 @@ template <class T>
 void launch_expansion(const RadialDev& rad, int S, const PairView& pv, T* c,
                       unsigned long long* keymask, hipStream_t stream) {
   if (pv.n_centres == 0) return;
-  const int E = S * rad.N * rad.LL;
-  const int kpt = (E + kThreads - 1) / kThreads;
-  if (kpt <= 4)
-    launch_expansion_kpt<T, 4>(rad, S, pv, c, keymask, stream);
+  if (S > kMaxSpecies) throw InvalidArgument("at most 64 species");
+  const int B = rad.N * rad.LL;
+  const int kpt = (B + kThreads - 1) / kThreads;
+  if (kpt <= 2)
+    launch_expansion_k<T, 2>(rad, S, pv, c, keymask, stream);
+  else if (kpt <= 4)
+    launch_expansion_k<T, 4>(rad, S, pv, c, keymask, stream);
+  else if (kpt <= 5)
+    launch_expansion_k<T, 5>(rad, S, pv, c, keymask, stream);
   else if (kpt <= 8)
-    launch_expansion_kpt<T, 8>(rad, S, pv, c, keymask, stream);
+    launch_expansion_k<T, 8>(rad, S, pv, c, keymask, stream);
+  else if (kpt <= 12)
+    launch_expansion_k<T, 12>(rad, S, pv, c, keymask, stream);
   else if (kpt <= 16)
-    launch_expansion_kpt<T, 16>(rad, S, pv, c, keymask, stream);
-  else if (kpt <= 24)
-    launch_expansion_kpt<T, 24>(rad, S, pv, c, keymask, stream);
-  else if (kpt <= 32)
-    launch_expansion_kpt<T, 32>(rad, S, pv, c, keymask, stream);
-  else if (kpt <= 64)
-    launch_expansion_kpt<T, 64>(rad, S, pv, c, keymask, stream);
+    launch_expansion_k<T, 16>(rad, S, pv, c, keymask, stream);
+  else if (kpt <= 28)
+    launch_expansion_k<T, 28>(rad, S, pv, c, keymask, stream);
   else
-    throw InvalidArgument("species x max_radial x (max_angular+1)^2 too large (> 16384)");
+    throw InvalidArgument("max_radial x (max_angular+1)^2 too large (> 7168)");
 }
 
 template <class T>
 void launch_power_spectrum(const RadialDev& rad, int S, int64_t n, const T* c, T* p, double* norm2,
                            hipStream_t stream) {
   if (n == 0) return;
-  size_t smem = sizeof(T) * S * rad.N * rad.LL;
-  if (smem > 64 * 1024) throw InvalidArgument("expansion block too large for the power spectrum");
-  hipLaunchKernelGGL(power_spectrum_kernel<T>, dim3(unsigned(n)), dim3(kThreads), smem, stream, rad.N,
-                     rad.L, S, c, p, norm2);
+  const size_t cbytes = sizeof(T) * size_t(S) * rad.N * rad.LL;
+  const size_t kb = sizeof(T) * size_t(rad.N) * rad.N * rad.L1;
+  const int K = S * (S + 1) / 2;
+  int KC = cbytes + kb < kPsBudget ? int((kPsBudget - cbytes) / kb) : 1;
+  KC = std::max(1, std::min(KC, K));
+  const size_t smem = cbytes + size_t(KC) * kb;
+  auto kern = power_spectrum_kernel<T>;
+  allow_lds(kern, smem);
+  hipLaunchKernelGGL(kern, dim3(unsigned(n)), dim3(kThreads), smem, stream, rad.N, rad.L, S, KC, c, p,
+                     norm2);
   RGPU_LAUNCH_CHECK();
 }
 
@@ -772,33 +946,16 @@ void launch_normalise(int64_t n, int64_t D, const T* p, const double* norm2, boo
 }
 
 template <class T>
-void launch_backward(const RadialDev& rad, int S, const PairView& pv, const int* /*atom_struct*/,
-                     const T* c, const T* p, const double* norm2, bool normalize, const T* G, T* W,
-                     double* grad, double* centre_virial, hipStream_t stream) {
-  const int64_t n = pv.n_centres;
-  if (n == 0) return;
-  const int E = S * rad.N * rad.LL;
-  const int kpt = (E + kThreads - 1) / kThreads;
-  if (kpt <= 4)
-    launch_backward_dc_kpt<T, 4>(rad, S, n, c, p, norm2, normalize, G, W, stream);
-  else if (kpt <= 8)
-    launch_backward_dc_kpt<T, 8>(rad, S, n, c, p, norm2, normalize, G, W, stream);
-  else if (kpt <= 16)
-    launch_backward_dc_kpt<T, 16>(rad, S, n, c, p, norm2, normalize, G, W, stream);
-  else if (kpt <= 24)
-    launch_backward_dc_kpt<T, 24>(rad, S, n, c, p, norm2, normalize, G, W, stream);
-  else if (kpt <= 32)
-    launch_backward_dc_kpt<T, 32>(rad, S, n, c, p, norm2, normalize, G, W, stream);
-  else if (kpt <= 64)
-    launch_backward_dc_kpt<T, 64>(rad, S, n, c, p, norm2, normalize, G, W, stream);
+void launch_backward(const RadialDev& rad, int S, const PairView& pv, const T* c, const T* p,
+                     const double* norm2, bool normalize, const double* xg, double xg_scale,
+                     const T* G, double* grad, double* centre_virial, hipStream_t stream) {
+  if (pv.n_centres == 0) return;
+  if (rad.use_spline)
+    launch_adjoint_t<T, false>(rad, S, pv, c, p, norm2, normalize, xg, xg_scale, G, grad, centre_virial,
+                               stream);
   else
-    throw InvalidArgument("species x max_radial x (max_angular+1)^2 too large (> 16384)");
-  const int F = rad.N * rad.L1;
-  size_t smem = sizeof(T) * (kPBB * rad.LL * 4 + 2 * kPBB * F + 3 * kPBB) + sizeof(double) * 3 * kPBB +
-                sizeof(int) * kPBB;
-  hipLaunchKernelGGL(backward_pairs_kernel<T>, dim3(unsigned(n)), dim3(kThreads), smem, stream, rad, S, pv,
-                     W, grad, centre_virial);
-  RGPU_LAUNCH_CHECK();
+    launch_adjoint_t<T, true>(rad, S, pv, c, p, norm2, normalize, xg, xg_scale, G, grad, centre_virial,
+                              stream);
 }
 
 void launch_segment_sum(const double* in, int w, const int64_t* seg_off, int n_seg, double* out,
@@ -866,11 +1023,11 @@ template void launch_normalise<double>(int64_t, int64_t, const double*, const do
                                        hipStream_t);
 template void launch_normalise<float>(int64_t, int64_t, const float*, const double*, bool, double*,
                                       hipStream_t);
-template void launch_backward<double>(const RadialDev&, int, const PairView&, const int*, const double*,
-                                      const double*, const double*, bool, const double*, double*,
-                                      double*, double*, hipStream_t);
-template void launch_backward<float>(const RadialDev&, int, const PairView&, const int*, const float*,
-                                     const float*, const double*, bool, const float*, float*, double*,
-                                     double*, hipStream_t);
+template void launch_backward<double>(const RadialDev&, int, const PairView&, const double*,
+                                      const double*, const double*, bool, const double*, double,
+                                      const double*, double*, double*, hipStream_t);
+template void launch_backward<float>(const RadialDev&, int, const PairView&, const float*,
+                                     const float*, const double*, bool, const double*, double,
+                                     const float*, double*, double*, hipStream_t);
 
 }  // namespace rgpu
