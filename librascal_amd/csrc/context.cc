@@ -77,6 +77,7 @@ struct Context {
 
   // scratch
   DeviceBuffer<double> e_centre, grad, cvir, tmp64, out64;
+  DeviceBuffer<double> pair_fc, pair_dfc;  // cutoff function per pair
   DeviceBuffer<double> wt64, G64;
   DeviceBuffer<float> wt32, G32;
   DeviceBuffer<int64_t> knm_perm;
@@ -236,12 +237,16 @@ void build_tables(Context& c) {
     ++dof;
   }
   const int npts = int(grid.size());
-  std::vector<double> tab(size_t(npts) * 2 * F);
+  // packed per cell: {y_k, d_k, y_k+1, d_k+1}[f]
+  std::vector<double> tab(size_t(npts - 1) * F * 4);
   std::vector<float> tab32(tab.size());
-  for (int k = 0; k < npts; ++k)
+  for (int k = 0; k + 1 < npts; ++k)
     for (int f = 0; f < F; ++f) {
-      tab[size_t(k) * 2 * F + f] = y[size_t(k) * F + f];
-      tab[size_t(k) * 2 * F + F + f] = d2[size_t(k) * F + f];
+      double* t = &tab[(size_t(k) * F + f) * 4];
+      t[0] = y[size_t(k) * F + f];
+      t[1] = d2[size_t(k) * F + f];
+      t[2] = y[size_t(k + 1) * F + f];
+      t[3] = d2[size_t(k + 1) * F + f];
     }
   for (size_t k = 0; k < tab.size(); ++k) tab32[k] = float(tab[k]);
   h.grid_n = npts;
@@ -281,6 +286,8 @@ PairView pair_view(Context& c) {
   pv.pair_vec = c.pl.pair_vec.get();
   pv.pair_dist = c.pl.pair_dist.get();
   pv.species_of = c.d_species_of.get();
+  pv.pair_fc = c.pair_fc.get();
+  pv.pair_dfc = c.pair_dfc.get();
   return pv;
 }
 
@@ -314,6 +321,10 @@ void compute_rep(Context& c) {
   unsigned long long* km = c.keymask.reserve(size_t(std::max<int64_t>(n, 1)));
   {
     StageTimer t(c, RGPU_STAGE_EXPANSION);
+    const int64_t np = c.pl.n_pairs;
+    double* fc = c.pair_fc.reserve(size_t(std::max<int64_t>(np, 1)));
+    double* dfc = c.pair_dfc.reserve(size_t(std::max<int64_t>(np, 1)));
+    launch_pair_cutoff(c.rd, np, c.pl.pair_dist.get(), fc, dfc, c.stream);
     launch_expansion<T>(c.rd, S, pair_view(c), cc, km, c.stream);
   }
   if (c.hp.kind == SoapHypers::kPowerSpectrum) {

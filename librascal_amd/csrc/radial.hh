@@ -43,13 +43,14 @@ struct RadialDev {
   const HypCoef* hyp;     // [N][L+1]
   const double* hc;       // [N][L+1][2][kHypTerms] series coeff, derivative coeff
   const double* ha;       // [N][L+1][2][kHypTerms] asymptotic coeff, derivative coeff
-  // spline on [x1, x2], grid_n points; row k = {y_k[F], d_k[F]} with
-  // d = y'' h^2/6, function index f = n*L1 + l
+  // spline on [x1, x2], grid_n points, packed per cell k and function
+  // f = n*L1 + l as {y_k, d_k, y_k+1, d_k+1} with d = y'' h^2/6, so that one
+  // vector load serves one evaluation
   int grid_n, search_size;
   double x1, h, per_unit;
   const double* grid;     // [grid_n] grid points
-  const double* spl64;    // [grid_n][2F]
-  const float* spl32;     // [grid_n][2F]
+  const double* spl64;    // [grid_n-1][F][4]
+  const float* spl32;     // [grid_n-1][F][4]
 };
 
 // Host-side constants (all FP64).

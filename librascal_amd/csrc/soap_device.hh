@@ -221,24 +221,31 @@ __device__ inline SplineCell<T> spline_cell(const RadialDev& R, double r) {
 }
 
 template <class T>
-__device__ __forceinline__ const T* spline_table(const RadialDev& R);
+struct Spl4;
 template <>
-__device__ __forceinline__ const double* spline_table<double>(const RadialDev& R) {
-  return R.spl64;
-}
+struct Spl4<float> {
+  typedef float type __attribute__((ext_vector_type(4)));
+  static __device__ __forceinline__ const type* table(const RadialDev& R) {
+    return reinterpret_cast<const type*>(R.spl32);
+  }
+};
 template <>
-__device__ __forceinline__ const float* spline_table<float>(const RadialDev& R) {
-  return R.spl32;
-}
+struct Spl4<double> {
+  typedef double type __attribute__((ext_vector_type(4)));
+  static __device__ __forceinline__ const type* table(const RadialDev& R) {
+    return reinterpret_cast<const type*>(R.spl64);
+  }
+};
 
+// cubic of cell c.klo for function f at the point the cell was made for
+// (math/interpolator.hh:166-186)
 template <class T>
 __device__ __forceinline__ void spline_eval(const RadialDev& R, const SplineCell<T>& c, int f,
                                             T* val, T* der) {
   const int F = R.N * R.L1;
-  const T* lo = spline_table<T>(R) + size_t(c.klo) * 2 * F;
-  const T* hi = lo + 2 * F;
-  T ylo = lo[f], dlo = lo[F + f], yhi = hi[f], dhi = hi[F + f];
-  T a = c.a, b = c.b;
+  const typename Spl4<T>::type v = Spl4<T>::table(R)[size_t(c.klo) * F + f];
+  const T ylo = v.x, dlo = v.y, yhi = v.z, dhi = v.w;
+  const T a = c.a, b = c.b;
   *val = a * (ylo + (a * a - T(1)) * dlo) + b * (yhi + (b * b - T(1)) * dhi);
   if (der)
     *der = (yhi - ylo - (T(3) * a * a - T(1)) * dlo + (T(3) * b * b - T(1)) * dhi) * c.inv_h;

@@ -18,7 +18,13 @@ struct PairView {
   const double* pair_vec;     // [n_pairs][3]
   const double* pair_dist;    // [n_pairs]
   const int* species_of;      // [n_atoms] species index in the dense layout
+  const double* pair_fc;      // [n_pairs] cutoff function at the pair distance
+  const double* pair_dfc;     // [n_pairs] its derivative
 };
+
+// pair_fc / pair_dfc of all pairs (cutoff_functions.hh:69-283)
+void launch_pair_cutoff(const RadialDev& rad, int64_t n_pairs, const double* dist, double* fc,
+                        double* dfc, hipStream_t stream);
 
 // c[i][s][n][l*l+l+m] (finalised) and the per-centre key mask.
 template <class T>

@@ -200,8 +200,11 @@ struct PairBatch {
   T* R;          // [PB][F] finalised radial functions (x fc in the forward)
   T* dR;         // [PB][F] (adjoint)
   T* dir;        // [PB][3]
+  T* ca;         // [PB] spline weights of the pair's cell
+  T* cb;         // [PB]
   int* sp;       // [PB] neighbour species
   int* lst;      // [PB] batch positions grouped by species (forward)
+  int* klo;      // [PB] spline cell
 };
 
 __host__ __device__ inline size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
@@ -210,9 +213,9 @@ __host__ __device__ inline size_t pair_batch_bytes(int PB, int LL, int F, int ts
                                                    bool adjoint) {
   size_t b = 8 * size_t(PB) * (adjoint ? 3 : 2);
   if (direct) b += 8 * size_t(PB) * F * (adjoint ? 2 : 1);
-  b += size_t(tsize) * PB * (size_t(LL) * (adjoint ? 4 : 1) + size_t(F) * (adjoint ? 2 : 1) + 3);
+  b += size_t(tsize) * PB * (size_t(LL) * (adjoint ? 4 : 1) + size_t(F) * (adjoint ? 2 : 1) + 5);
   b = align_up(b, 8);
-  b += 4 * size_t(PB) * 2;
+  b += 4 * size_t(PB) * 3;
   return align_up(b, 16);
 }
 
@@ -255,9 +258,14 @@ __device__ PairBatch<T> carve_batch(unsigned char* base, int PB, int LL, int F) 
   }
   pb.dir = t;
   t += 3 * PB;
+  pb.ca = t;
+  t += PB;
+  pb.cb = t;
+  t += PB;
   const size_t off = align_up(size_t(reinterpret_cast<unsigned char*>(t) - base), 8);
   pb.sp = reinterpret_cast<int*>(base + off);
   pb.lst = pb.sp + PB;
+  pb.klo = pb.lst + PB;
   return pb;
 }
 
@@ -280,11 +288,15 @@ __device__ void stage_pairs(const RadialDev& rad, const PairView& pv, int64_t pb
     b.dir[3 * tid] = T(pv.pair_vec[3 * p] / d);
     b.dir[3 * tid + 1] = T(pv.pair_vec[3 * p + 1] / d);
     b.dir[3 * tid + 2] = T(pv.pair_vec[3 * p + 2] / d);
-    double fc, dfc;
-    cutoff_eval(rad, d, &fc, ADJ ? &dfc : nullptr);
-    b.fc[tid] = fc;
-    if (ADJ) b.dfc[tid] = dfc;
+    b.fc[tid] = pv.pair_fc[p];
+    if (ADJ) b.dfc[tid] = pv.pair_dfc[p];
     b.sp[tid] = pv.species_of[pv.pair_j[p]];
+    if (!DIRECT) {
+      const SplineCell<T> cell = spline_cell<T>(rad, d);
+      b.klo[tid] = cell.klo;
+      b.ca[tid] = cell.a;
+      b.cb[tid] = cell.b;
+    }
   }
   __syncthreads();
   if (s_count && wave == 0) {
@@ -311,9 +323,15 @@ __device__ void stage_pairs(const RadialDev& rad, const PairView& pv, int64_t pb
   for (int qb = wave * per_wave; qb < nb; qb += kWaves * per_wave)
     ylm_wave<T>(L, b.dir, qb, nb, b.Y, ADJ ? b.dY : nullptr, LL, 3 * LL);
   if (!DIRECT) {
+    const T inv_h = T(1.0 / rad.h);
+#pragma unroll 4
     for (int it = tid; it < nb * F; it += kThreads) {
       const int q = it / F, f = it - q * F;
-      const SplineCell<T> cell = spline_cell<T>(rad, b.dist[q]);
+      SplineCell<T> cell;
+      cell.klo = b.klo[q];
+      cell.a = b.ca[q];
+      cell.b = b.cb[q];
+      cell.inv_h = inv_h;
       T v, dv;
       spline_eval<T>(rad, cell, f, &v, ADJ ? &dv : nullptr);
       if (ADJ) {
@@ -355,12 +373,19 @@ __device__ void stage_pairs(const RadialDev& rad, const PairView& pv, int64_t pb
   __syncthreads();
 }
 
+// register budget of the forward: small-KPT spline kernels are held to
+// 4-5 waves per SIMD (their LDS allows that many workgroups per CU)
+template <class T, int KPT, bool DIRECT>
+constexpr int expansion_min_waves() {
+  return (!DIRECT && KPT <= 8) ? (sizeof(T) == 4 ? 5 : 4) : 1;
+}
+
 // ---------------------------------------------------------------------------
 // Expansion forward: c[i][s][n][lm] = [s == s_i][lm == 0] R0[n] / sqrt(4 pi)
 //   + sum over pairs (i, j), s_j = s, of fc(r) R_nl(r) Y_lm(r_hat)
 // KPT = entries of one species block per thread.
 template <class T, int KPT, bool DIRECT>
-__global__ __launch_bounds__(kThreads) void expansion_kernel(RadialDev rad, int S, int PB,
+__global__ __launch_bounds__(kThreads, (expansion_min_waves<T, KPT, DIRECT>())) void expansion_kernel(RadialDev rad, int S, int PB,
                                                              PairView pv, T* __restrict__ cout,
                                                              unsigned long long* __restrict__ keymask) {
   const int64_t i = blockIdx.x;
@@ -733,6 +758,16 @@ __global__ __launch_bounds__(kThreads) void adjoint_kernel(
   }
 }
 
+__global__ void pair_cutoff_kernel(RadialDev rad, int64_t n_pairs, const double* __restrict__ dist,
+                                   double* __restrict__ fc, double* __restrict__ dfc) {
+  const int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (p >= n_pairs) return;
+  double f, df;
+  cutoff_eval(rad, dist[p], &f, &df);
+  fc[p] = f;
+  dfc[p] = df;
+}
+
 // per-structure (segment) sums of per-centre rows of width w <= 8 (+=)
 __global__ void segment_sum_kernel(const double* __restrict__ in, int w,
                                    const int64_t* __restrict__ seg_off, int n_seg,
@@ -956,6 +991,14 @@ void launch_backward(const RadialDev& rad, int S, const PairView& pv, const T* c
   else
     launch_adjoint_t<T, true>(rad, S, pv, c, p, norm2, normalize, xg, xg_scale, G, grad, centre_virial,
                               stream);
+}
+
+void launch_pair_cutoff(const RadialDev& rad, int64_t n_pairs, const double* dist, double* fc,
+                        double* dfc, hipStream_t stream) {
+  if (n_pairs == 0) return;
+  hipLaunchKernelGGL(pair_cutoff_kernel, dim3(unsigned((n_pairs + 255) / 256)), dim3(256), 0, stream,
+                     rad, n_pairs, dist, fc, dfc);
+  RGPU_LAUNCH_CHECK();
 }
 
 void launch_segment_sum(const double* in, int w, const int64_t* seg_off, int n_seg, double* out,
