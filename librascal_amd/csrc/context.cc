@@ -35,7 +35,8 @@ struct Context {
 
   // radial tables on the device
   DeviceBuffer<double> d_fac_b, d_O, d_center, d_hc, d_ha, d_grid, d_spl64;
-  DeviceBuffer<float> d_spl32;
+  DeviceBuffer<float> d_spl32, d_ylm32;
+  DeviceBuffer<double> d_ylm64;
   DeviceBuffer<HypCoef> d_hyp;
 
   // batch
@@ -183,6 +184,12 @@ void build_tables(Context& c) {
   r.hyp = up(c.d_hyp, h.hyp);
   r.hc = up(c.d_hc, h.hc);
   r.ha = up(c.d_ha, h.ha);
+  {
+    const std::vector<double> yt = ylm_coefficients(h.L);
+    std::vector<float> yt32(yt.begin(), yt.end());
+    r.ylm64 = up(c.d_ylm64, yt);
+    r.ylm32 = up(c.d_ylm32, yt32);
+  }
   r.use_spline = 0;
   r.center = up(c.d_center, h.center_fin);  // finalised: R is finalised per pair
   if (c.hp.optimization != 1) return;
@@ -964,9 +971,9 @@ int rgpu_eval_ylm(rgpu_context* ctx, int64_t n, const double* dirs, double* Y, d
     double* py = dy.reserve(size_t(std::max<int64_t>(n * LL, 1)));
     double* pdy = dY ? ddy.reserve(size_t(std::max<int64_t>(3 * n * LL, 1))) : nullptr;
     if (c.precision == RGPU_FP64)
-      launch_ylm_eval<double>(c.rd.L, pd, n, py, pdy, c.stream);
+      launch_ylm_eval<double>(c.rd, pd, n, py, pdy, c.stream);
     else
-      launch_ylm_eval<float>(c.rd.L, pd, n, py, pdy, c.stream);
+      launch_ylm_eval<float>(c.rd, pd, n, py, pdy, c.stream);
     RGPU_HIP(hipStreamSynchronize(c.stream));
     RGPU_HIP(hipMemcpy(Y, py, sizeof(double) * n * LL, hipMemcpyDefault));
     if (dY) RGPU_HIP(hipMemcpy(dY, pdy, sizeof(double) * 3 * n * LL, hipMemcpyDefault));

@@ -32,12 +32,6 @@ typedef __attribute__((__vector_size__(16 * sizeof(float)))) float f32x16;
 constexpr int BM = 128, BN = 128, BK = 32, LDK = BK + 8, NT = 256;
 constexpr int kMaxProblems = 16;
 
-__device__ __forceinline__ unsigned short f2bf(float x) {
-  unsigned u = __float_as_uint(x);
-  u += 0x7FFFu + ((u >> 16) & 1u);  // round to nearest even
-  return (unsigned short)(u >> 16);
-}
-__device__ __forceinline__ float bf2f(unsigned short h) { return __uint_as_float(unsigned(h) << 16); }
 
 struct Problem {
   int64_t M, N, K;             // C (M x N) = A (M x K) B (N x K)^T
@@ -62,6 +56,7 @@ struct Problem {
 
 struct ProblemSet {
   Problem p[kMaxProblems];
+  int64_t total_tiles;
   int n;
   int normalize;
   int zeta;
@@ -91,8 +86,12 @@ __global__ __launch_bounds__(NT, 2) void bf16x3_gemm(ProblemSet ps) {
   __shared__ __align__(16) unsigned short sBl[BN * LDK];
   __shared__ double esum[2][BM];
 
-  // problem and tile of this block
-  const int64_t bid = blockIdx.x;
+  // problem and tile of this block.  Workgroups are dealt to the 8 XCDs
+  // round robin; consecutive tiles (same row panel) go to the same XCD so
+  // the row panel is shared through that XCD's L2.
+  const int64_t per_xcd = (int64_t(gridDim.x) + 7) / 8;
+  const int64_t bid = int64_t(blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
+  if (bid >= ps.total_tiles) return;
   int pi = 0;
   while (pi + 1 < ps.n && bid >= ps.p[pi + 1].tile0) ++pi;
   const Problem& P = ps.p[pi];
@@ -148,20 +147,22 @@ __global__ __launch_bounds__(NT, 2) void bf16x3_gemm(ProblemSet ps) {
     }
   };
   auto lstore = [&]() {
-    unsigned short hi[16], lo[16];
+    // x = hi + lo, hi = bf16(x), lo = bf16(x - hi), both round-to-nearest-even
+    // (hardware conversions on gfx950)
+    bf16x8 hi[2], lo[2];
     const float* f = reinterpret_cast<const float*>(ra);
 #pragma unroll
     for (int q = 0; q < 16; ++q) {
-      const unsigned short h = f2bf(f[q]);
-      hi[q] = h;
-      lo[q] = f2bf(f[q] - bf2f(h));
+      const __bf16 h = (__bf16)f[q];
+      hi[q >> 3][q & 7] = h;
+      lo[q >> 3][q & 7] = (__bf16)(f[q] - (float)h);
     }
-    uint4* dah = reinterpret_cast<uint4*>(sAh + lr * LDK + lk);
-    uint4* dal = reinterpret_cast<uint4*>(sAl + lr * LDK + lk);
-    dah[0] = *reinterpret_cast<const uint4*>(hi);
-    dah[1] = *reinterpret_cast<const uint4*>(hi + 8);
-    dal[0] = *reinterpret_cast<const uint4*>(lo);
-    dal[1] = *reinterpret_cast<const uint4*>(lo + 8);
+    bf16x8* dah = reinterpret_cast<bf16x8*>(sAh + lr * LDK + lk);
+    bf16x8* dal = reinterpret_cast<bf16x8*>(sAl + lr * LDK + lk);
+    dah[0] = hi[0];
+    dah[1] = hi[1];
+    dal[0] = lo[0];
+    dal[1] = lo[1];
     uint4* dbh = reinterpret_cast<uint4*>(sBh + lr * LDK + lk);
     uint4* dbl = reinterpret_cast<uint4*>(sBl + lr * LDK + lk);
     dbh[0] = rbh[0];
@@ -298,12 +299,14 @@ float bf16_to_float_host(unsigned short h) {
   return f;
 }
 
-void launch(const ProblemSet& ps, int64_t total_tiles, int epi, hipStream_t stream) {
+void launch(ProblemSet& ps, int64_t total_tiles, int epi, hipStream_t stream) {
   if (total_tiles == 0) return;
+  ps.total_tiles = total_tiles;
+  const unsigned grid = unsigned((total_tiles + 7) / 8 * 8);  // whole rounds over the 8 XCDs
   if (epi == 0)
-    hipLaunchKernelGGL(bf16x3_gemm<0>, dim3(unsigned(total_tiles)), dim3(NT), 0, stream, ps);
+    hipLaunchKernelGGL(bf16x3_gemm<0>, dim3(grid), dim3(NT), 0, stream, ps);
   else
-    hipLaunchKernelGGL(bf16x3_gemm<1>, dim3(unsigned(total_tiles)), dim3(NT), 0, stream, ps);
+    hipLaunchKernelGGL(bf16x3_gemm<1>, dim3(grid), dim3(NT), 0, stream, ps);
   RGPU_LAUNCH_CHECK();
 }
 

@@ -51,7 +51,21 @@ struct RadialDev {
   const double* grid;     // [grid_n] grid points
   const double* spl64;    // [grid_n-1][F][4]
   const float* spl32;     // [grid_n-1][F][4]
+  // real spherical harmonics recursion coefficients, ylm_table_size(L+1)
+  // values: [L1][L1][4] = {a_lm, b_lm, f_lm, f'_lm}, then d_m, sqrt(2l+1),
+  // sqrt(l(l+1)) ([L1] each); see ylm_coefficients()
+  const double* ylm64;
+  const float* ylm32;
 };
+
+__host__ __device__ inline int ylm_table_size(int L1) { return 4 * L1 * L1 + 3 * L1; }
+
+// Coefficients of the associated-Legendre recursion and of the harmonics'
+// gradient (math/spherical_harmonics.cc:36-96, 249-300), all FP64:
+//   a_lm = sqrt((4l^2-1)/(l^2-m^2)), b_lm = -sqrt(((l-1)^2-m^2)/(4(l-1)^2-1))
+//   (m < l-1); f_lm = sqrt((l-m)(l+m+1)), f'_lm = sqrt((l-m-1)(l+m+2))
+//   (m < l); d_1 = -sqrt(3/2), d_m = -sqrt(1 + 1/(2m)).
+std::vector<double> ylm_coefficients(int L);
 
 // Host-side constants (all FP64).
 struct RadialHost {

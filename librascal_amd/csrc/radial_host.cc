@@ -279,4 +279,30 @@ RadialHost::RadialHost(const SoapHypers& hp) : N(hp.max_radial), L(hp.max_angula
     }
 }
 
+std::vector<double> ylm_coefficients(int L) {
+  const int L1 = L + 1;
+  std::vector<double> t(size_t(ylm_table_size(L1)), 0.0);
+  for (int l = 0; l <= L; ++l)
+    for (int m = 0; m <= L; ++m) {
+      double* c = &t[size_t(4 * (l * L1 + m))];
+      const double ll = double(l) * l, mm = double(m) * m, l1 = double(l - 1) * (l - 1);
+      if (m < l - 1) {
+        c[0] = std::sqrt((4.0 * ll - 1.0) / (ll - mm));
+        c[1] = -std::sqrt((l1 - mm) / (4.0 * l1 - 1.0));
+      }
+      if (m < l) {
+        c[2] = std::sqrt(double(l - m) * double(l + m + 1));
+        c[3] = std::sqrt(double(l - m - 1) * double(l + m + 2));
+      }
+    }
+  double* d = &t[size_t(4 * L1 * L1)];
+  d[0] = 1.0;
+  for (int m = 1; m <= L; ++m) d[m] = m == 1 ? -1.22474487139158904910 : -std::sqrt(1.0 + 0.5 / m);
+  for (int l = 0; l <= L; ++l) {
+    d[L1 + l] = std::sqrt(double(2 * l + 1));
+    d[2 * L1 + l] = std::sqrt(double(l) * double(l + 1));
+  }
+  return t;
+}
+
 }  // namespace rgpu

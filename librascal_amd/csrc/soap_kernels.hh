@@ -67,6 +67,7 @@ void launch_radial_spline_eval(const RadialDev& rad, const double* r, int64_t n_
                                double* dR, hipStream_t stream);
 // Y_lm (+ gradients) at unit vectors in precision T
 template <class T>
-void launch_ylm_eval(int L, const double* dirs, int64_t n, double* Y, double* dY, hipStream_t stream);
+void launch_ylm_eval(const RadialDev& rad, const double* dirs, int64_t n, double* Y, double* dY,
+                     hipStream_t stream);
 
 }  // namespace rgpu

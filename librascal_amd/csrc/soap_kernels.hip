@@ -3,19 +3,24 @@
 // One workgroup (256 threads, 4 wavefronts) owns one centre.  Its pairs are
 // staged in LDS in batches: per pair the real spherical harmonics come from
 // a wavefront-level recursion (one lane per m, lockstep over l,
-// neighbouring-m values by lane shuffles) and the radial functions from the
-// spline table (or FP64 Hyp1f1 followed by the O^T orthonormalisation).
-// Pairs of a batch are grouped by neighbour species with wavefront ballots,
-// so the accumulation walks one species block at a time and each thread
-// keeps only a few (n, lm) accumulators in registers.  The result equals
-// the reference's per-pair outer-product accumulation
-// (calculator_spherical_expansion.hh:2234-2410) up to rounding.
+// neighbouring-m values by lane shuffles, recursion coefficients from a
+// table in LDS) and the radial functions from the spline table (or FP64
+// Hyp1f1 followed by the O^T orthonormalisation).  Pairs of a batch are
+// grouped by neighbour species with wavefront ballots, so the accumulation
+// walks one species block at a time and each thread keeps only a few
+// (n, lm) accumulators in registers.  The result equals the reference's
+// per-pair outer-product accumulation (calculator_spherical_expansion.hh
+// :2234-2410) up to rounding.
 //
 // The adjoint keeps dL/dc of the centre in LDS: it is formed key block by
-// key block from dL/dp (power-spectrum transpose, register-tiled 4 x 1),
-// then the same workgroup recomputes Y_lm, dY_lm, R and dR per pair and
-// contracts them with dL/dc, so the per-pair gradient tensor is never
-// materialised.
+// key block from dL/dp (power-spectrum transpose, register-tiled 4 x 1,
+// next key block prefetched into registers), then the same workgroup
+// recomputes Y_lm, dY_lm, R and dR per pair and contracts them with dL/dc,
+// so the per-pair gradient tensor is never materialised.
+//
+// Index arithmetic avoids integer division by runtime values (tens of VALU
+// instructions each on this GPU): quotients come from an FP32 reciprocal
+// with one correction step, l of a packed lm from a square root.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -42,13 +47,21 @@ struct Vec4<double> {
   typedef double type __attribute__((ext_vector_type(4)));
 };
 
-// floor(sqrt(x)) for small non-negative x
-__device__ __forceinline__ int isqrt_small(int x) {
-  int r = int(sqrtf(float(x)));
-  while (r * r > x) --r;
-  while ((r + 1) * (r + 1) <= x) ++r;
-  return r;
+// n / d for 0 <= n < 2^22: FP32 reciprocal, then one correction step
+struct FastDiv {
+  int d;
+  float inv;
+};
+__device__ __forceinline__ FastDiv fast_div(int d) { return FastDiv{d, 1.0f / float(d)}; }
+__device__ __forceinline__ int operator/(int n, const FastDiv& f) {
+  int q = int(float(n) * f.inv);
+  const int r = n - q * f.d;
+  q += (r >= f.d) - (r < 0);
+  return q;
 }
+
+// l of the packed index lm = l*l + l + m (exact for lm < 2^20)
+__device__ __forceinline__ int l_of_lm(int lm) { return int(sqrtf(float(lm) + 0.5f)); }
 
 // index (a, b), a <= b, of a species-pair key in the order (0,0), (0,1), ...
 __device__ __forceinline__ void key_species(int key, int S, int* a, int* b) {
@@ -61,14 +74,43 @@ __device__ __forceinline__ void key_species(int key, int S, int* a, int* b) {
   *b = aa + kk;
 }
 
+__host__ __device__ inline size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
+// Y_lm coefficient table in LDS (copied from the context's device table)
+template <class T>
+__host__ __device__ inline size_t ylm_table_bytes(int L1) {
+  return align_up(sizeof(T) * size_t(ylm_table_size(L1)), 64);
+}
+template <class T>
+__device__ __forceinline__ const T* ylm_table_src(const RadialDev& R);
+template <>
+__device__ __forceinline__ const double* ylm_table_src<double>(const RadialDev& R) {
+  return R.ylm64;
+}
+template <>
+__device__ __forceinline__ const float* ylm_table_src<float>(const RadialDev& R) {
+  return R.ylm32;
+}
+template <class T>
+__device__ void load_ylm_table(const RadialDev& R, T* dst) {
+  const T* src = ylm_table_src<T>(R);
+  const int n = ylm_table_size(R.L1);
+  for (int e = threadIdx.x; e < n; e += blockDim.x) dst[e] = src[e];
+}
 
 // ---------------------------------------------------------------------------
 // Real spherical harmonics of nq pairs, one wavefront segment of (L+1) lanes
-// per pair.  dirs[q][3] (unit vectors), writes Y[q][LL] and optionally
-// dY[q][3][LL].  Every lane of the wavefront must call this (shuffles).
+// per pair (lane = m >= 0 of its segment; it writes m and -m).  dirs[q][3]
+// (unit vectors), writes Y[q][LL] and optionally dY[q][3][LL] (gradient on
+// the unit sphere, no 1/r).  Every lane of the wavefront must call this
+// (shuffles).  Follows math/spherical_harmonics.cc:135-363: associated
+// Legendre recursion with the sqrt(1/(2 pi)) normalisation folded in, m = 0
+// scaled by 1/sqrt(2), cos/sin(m phi) by Chebyshev recurrence, phi = 0 on
+// the z axis, and the pole/equator branch of the gradient at sin(theta) 0.1.
 template <class T>
-__device__ void ylm_wave(int L, const T* dirs, int q0, int nq, T* Y, T* dY, int ystride,
-                         int dystride) {
+__device__ void ylm_wave(int L, const T* __restrict__ yc, const T* dirs, int q0, int nq, T* Y, T* dY,
+                         int ystride, int dystride) {
+  typedef typename Vec4<T>::type V4;
   const int L1 = L + 1;
   const int lane = threadIdx.x & 63;
   const int seg = lane / L1;
@@ -76,6 +118,9 @@ __device__ void ylm_wave(int L, const T* dirs, int q0, int nq, T* Y, T* dY, int 
   const int per_wave = 64 / L1;
   const int q = q0 + seg;
   const bool active = seg < per_wave && q < nq;
+  const T* dfac = yc + 4 * L1 * L1;
+  const T* s2l1 = dfac + L1;
+  const T* sll1 = s2l1 + L1;
   T x = T(0), y = T(0), z = T(1);
   if (active) {
     x = dirs[3 * q];
@@ -110,30 +155,26 @@ __device__ void ylm_wave(int L, const T* dirs, int q0, int nq, T* Y, T* dY, int 
   // diagonal P(m, m) with the sqrt(1/(2 pi)) normalisation folded in
   const T s2pi = T(0.39894228040143267794);
   T diag = s2pi;
-  for (int k = 1; k <= m; ++k) {
-    if (k == 1)
-      diag = diag * -T(1.22474487139158904910) * sin_t;
-    else
-      diag = diag * sin_t * (-sqrt(T(1) + T(0.5) / T(k)));
-  }
+  for (int k = 1; k <= m; ++k) diag = diag * sin_t * dfac[k];
   const T isq2 = T(0.70710678118654752440);
+  const bool dy_on = dY != nullptr;
+  const T inv_sxy = sxy > T(0.1) ? T(1) / sxy : T(0);
+  const T inv_ct = sxy > T(0.1) ? T(0) : T(-0.5) / cos_t;
   T P1 = T(0), P2 = T(0);
   T* Yq = Y + q * ystride;
-  T* dYq = dY ? dY + q * dystride : nullptr;
+  T* dYq = dy_on ? dY + q * dystride : nullptr;
   const int LL = L1 * L1;
   for (int l = 0; l <= L; ++l) {
+    const V4 c4 = reinterpret_cast<const V4*>(yc)[l * L1 + m];  // a, b, f_k, f_k+1
     T P = T(0);
     if (m == l) {
       P = diag;
     } else if (m == l - 1) {
-      P = (l == 1) ? cos_t * T(1.73205080756887729353) * s2pi : diag * cos_t * sqrt(T(2 * l + 1));
+      P = (l == 1) ? cos_t * T(1.73205080756887729353) * s2pi : diag * cos_t * s2l1[l];
     } else if (m < l - 1) {
-      const T ll = T(l) * T(l), mm = T(m) * T(m), l1 = T(l - 1) * T(l - 1);
-      const T a = sqrt((T(4) * ll - T(1)) / (ll - mm));
-      const T b = -sqrt((l1 - mm) / (T(4) * l1 - T(1)));
       T t = cos_t * P1;
-      t += b * P2;
-      t *= a;
+      t += c4.y * P2;
+      t *= c4.x;
       P = t;
     }
     const int base = l * l + l;
@@ -145,27 +186,22 @@ __device__ void ylm_wave(int L, const T* dirs, int q0, int nq, T* Y, T* dY, int 
         Yq[base - m] = P * s_m;
       }
     }
-    if (dYq) {
+    if (dy_on) {
       const T Pn1 = __shfl(P, lane + 1, 64);
       const T Pn2 = __shfl(P, lane + 2, 64);
       if (active && l >= 1 && m <= l) {
         if (m == 0) {
-          const T qv = sqrt(T(l) * T(l + 1)) * isq2 * Pn1;
+          const T qv = sll1[l] * isq2 * Pn1;
           dYq[base] = cos_t * cos_p * qv;
           dYq[LL + base] = cos_t * sin_p * qv;
           dYq[2 * LL + base] = -sxy * qv;
         }
         if (m < l) {
-          const int k = m, M = m + 1;
-          const T fk = sqrt(T(l - k) * T(l + k + 1));
-          const T fk1 = sqrt(T(l - k - 1) * T(l + k + 2));
-          const T pk2 = (k + 2 <= l) ? Pn2 : T(0);
+          const int M = m + 1;
+          const T fk = c4.z, fk1 = c4.w;
+          const T pk2 = (m + 2 <= l) ? Pn2 : T(0);
           const T lpd = fk * P - fk1 * pk2;
-          T pdf;
-          if (sxy > T(0.1))
-            pdf = T(k + 1) / sxy * Pn1;
-          else
-            pdf = T(-0.5) / cos_t * (fk * P + fk1 * pk2);
+          const T pdf = sxy > T(0.1) ? T(m + 1) * inv_sxy * Pn1 : inv_ct * (fk * P + fk1 * pk2);
           const T hc = T(-0.5) * cos_t;
           dYq[base + M] = sin_p * pdf * s_m1 + hc * cos_p * c_m1 * lpd;
           dYq[LL + base + M] = -cos_p * pdf * s_m1 + hc * sin_p * c_m1 * lpd;
@@ -179,7 +215,7 @@ __device__ void ylm_wave(int L, const T* dirs, int q0, int nq, T* Y, T* dY, int 
     P2 = P1;
     P1 = P;
   }
-  if (dYq && active && m == 0) {
+  if (dy_on && active && m == 0) {
     dYq[0] = T(0);
     dYq[LL] = T(0);
     dYq[2 * LL] = T(0);
@@ -207,8 +243,6 @@ struct PairBatch {
   int* klo;      // [PB] spline cell
 };
 
-__host__ __device__ inline size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
-
 __host__ __device__ inline size_t pair_batch_bytes(int PB, int LL, int F, int tsize, bool direct,
                                                    bool adjoint) {
   size_t b = 8 * size_t(PB) * (adjoint ? 3 : 2);
@@ -216,7 +250,7 @@ __host__ __device__ inline size_t pair_batch_bytes(int PB, int LL, int F, int ts
   b += size_t(tsize) * PB * (size_t(LL) * (adjoint ? 4 : 1) + size_t(F) * (adjoint ? 2 : 1) + 5);
   b = align_up(b, 8);
   b += 4 * size_t(PB) * 3;
-  return align_up(b, 16);
+  return align_up(b, 64);
 }
 
 template <class T, bool DIRECT, bool ADJ>
@@ -273,11 +307,12 @@ __device__ PairBatch<T> carve_batch(unsigned char* base, int PB, int LL, int F) 
 // cutoff values, neighbour species, Y_lm (and gradients) and the finalised
 // radial functions (and derivatives); in the forward R is multiplied by fc.
 // When lists are requested (forward), wavefront 0 groups the batch by
-// species: s_count[s] pairs listed from lst[s_start[s]].  All threads call;
-// returns after a barrier.
+// species: s_count[s] pairs listed from lst[s_start[s]].  Wavefronts with
+// no Y_lm work take the radial functions.  All threads call; returns after
+// a barrier.
 template <class T, bool DIRECT, bool ADJ>
-__device__ void stage_pairs(const RadialDev& rad, const PairView& pv, int64_t pbase, int nb,
-                            PairBatch<T>& b, int* s_start, int* s_count,
+__device__ void stage_pairs(const RadialDev& rad, const T* yc, const PairView& pv, int64_t pbase,
+                            int nb, PairBatch<T>& b, int* s_start, int* s_count,
                             unsigned long long* kmask) {
   const int N = rad.N, L = rad.L, L1 = rad.L1, LL = rad.LL, F = N * L1;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -321,24 +356,31 @@ __device__ void stage_pairs(const RadialDev& rad, const PairView& pv, int64_t pb
   }
   const int per_wave = 64 / L1;
   for (int qb = wave * per_wave; qb < nb; qb += kWaves * per_wave)
-    ylm_wave<T>(L, b.dir, qb, nb, b.Y, ADJ ? b.dY : nullptr, LL, 3 * LL);
+    ylm_wave<T>(L, yc, b.dir, qb, nb, b.Y, ADJ ? b.dY : nullptr, LL, 3 * LL);
+  // radial functions on the wavefronts left idle by the Y_lm pass
+  const int ywaves = (nb + per_wave - 1) / per_wave;
+  const int r0 = ywaves >= kWaves ? 0 : ywaves * 64;
+  const int rstride = kThreads - r0;
   if (!DIRECT) {
-    const T inv_h = T(1.0 / rad.h);
+    if (tid >= r0) {
+      const FastDiv fF = fast_div(F);
+      const T inv_h = T(1.0 / rad.h);
 #pragma unroll 4
-    for (int it = tid; it < nb * F; it += kThreads) {
-      const int q = it / F, f = it - q * F;
-      SplineCell<T> cell;
-      cell.klo = b.klo[q];
-      cell.a = b.ca[q];
-      cell.b = b.cb[q];
-      cell.inv_h = inv_h;
-      T v, dv;
-      spline_eval<T>(rad, cell, f, &v, ADJ ? &dv : nullptr);
-      if (ADJ) {
-        b.R[it] = v;
-        b.dR[it] = dv;
-      } else {
-        b.R[it] = v * T(b.fc[q]);
+      for (int it = tid - r0; it < nb * F; it += rstride) {
+        const int q = it / fF, f = it - q * F;
+        SplineCell<T> cell;
+        cell.klo = b.klo[q];
+        cell.a = b.ca[q];
+        cell.b = b.cb[q];
+        cell.inv_h = inv_h;
+        T v, dv;
+        spline_eval<T>(rad, cell, f, &v, ADJ ? &dv : nullptr);
+        if (ADJ) {
+          b.R[it] = v;
+          b.dR[it] = dv;
+        } else {
+          b.R[it] = v * T(b.fc[q]);
+        }
       }
     }
   } else {
@@ -385,13 +427,14 @@ constexpr int expansion_min_waves() {
 //   + sum over pairs (i, j), s_j = s, of fc(r) R_nl(r) Y_lm(r_hat)
 // KPT = entries of one species block per thread.
 template <class T, int KPT, bool DIRECT>
-__global__ __launch_bounds__(kThreads, (expansion_min_waves<T, KPT, DIRECT>())) void expansion_kernel(RadialDev rad, int S, int PB,
-                                                             PairView pv, T* __restrict__ cout,
-                                                             unsigned long long* __restrict__ keymask) {
+__global__ __launch_bounds__(kThreads, (expansion_min_waves<T, KPT, DIRECT>())) void expansion_kernel(
+    RadialDev rad, int S, int PB, PairView pv, T* __restrict__ cout,
+    unsigned long long* __restrict__ keymask) {
   const int64_t i = blockIdx.x;
   const int N = rad.N, L1 = rad.L1, LL = rad.LL, F = N * L1, B = N * LL;
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  PairBatch<T> pb = carve_batch<T, DIRECT, false>(smem_raw, PB, LL, F);
+  extern __shared__ __align__(64) unsigned char smem_raw[];
+  T* yc = reinterpret_cast<T*>(smem_raw);
+  PairBatch<T> pb = carve_batch<T, DIRECT, false>(smem_raw + ylm_table_bytes<T>(L1), PB, LL, F);
   __shared__ int s_start[kMaxSpecies], s_count[kMaxSpecies];
   __shared__ unsigned long long kmask;
   const int tid = threadIdx.x;
@@ -399,15 +442,17 @@ __global__ __launch_bounds__(kThreads, (expansion_min_waves<T, KPT, DIRECT>())) 
   const int np = int(pv.centre_off[i + 1] - p0);
   const int si = pv.species_of[i];
   const double inv_s4pi = 0.28209479177387814347;  // 1/sqrt(4 pi)
+  load_ylm_table<T>(rad, yc);
 
+  const FastDiv fLL = fast_div(LL);
   int ent[KPT];  // (n*L1 + l) << 16 | lm, or -1
 #pragma unroll
   for (int k = 0; k < KPT; ++k) {
     const int e = tid + kThreads * k;
     ent[k] = -1;
     if (e < B) {
-      const int n = e / LL, lm = e - n * LL;
-      ent[k] = ((n * L1 + isqrt_small(lm)) << 16) | lm;
+      const int n = e / fLL, lm = e - n * LL;
+      ent[k] = ((n * L1 + l_of_lm(lm)) << 16) | lm;
     }
   }
   if (tid == 0) kmask = 1ull << si;
@@ -416,7 +461,7 @@ __global__ __launch_bounds__(kThreads, (expansion_min_waves<T, KPT, DIRECT>())) 
   for (int q0 = 0; q0 == 0 || q0 < np; q0 += PB) {
     const int nb = max(0, min(PB, np - q0));
     if (tid < S) s_count[tid] = 0;
-    stage_pairs<T, DIRECT, false>(rad, pv, p0 + q0, nb, pb, s_start, s_count, &kmask);
+    stage_pairs<T, DIRECT, false>(rad, yc, pv, p0 + q0, nb, pb, s_start, s_count, &kmask);
     for (int s = 0; s < S; ++s) {
       const int cnt = s_count[s];
       if (q0 > 0 && cnt == 0) continue;
@@ -431,7 +476,7 @@ __global__ __launch_bounds__(kThreads, (expansion_min_waves<T, KPT, DIRECT>())) 
           if (q0 > 0)
             acc[k] = cs[e];
           else if (s == si && (ent[k] & 0xffff) == 0)
-            acc[k] = T(rad.center[e / LL] * inv_s4pi);
+            acc[k] = T(rad.center[e / fLL] * inv_s4pi);
         }
       }
       for (int t = 0; t < cnt; ++t) {
@@ -455,42 +500,53 @@ __global__ __launch_bounds__(kThreads, (expansion_min_waves<T, KPT, DIRECT>())) 
 // Power spectrum: p^{ab}[n1 N + n2][l] = (2l+1)^{-1/2} (sqrt 2 if a<b)
 // sum_m c_a[n1,lm] c_b[n2,lm] (calculator_spherical_invariants.hh:747-788),
 // 4x4 register tiles over (n1, n2) per (key, l); KC keys at a time are
-// staged in LDS and written out contiguously.  Un-normalised; |p|^2 is
-// returned per centre.
+// staged in LDS and written out contiguously.  The expansion rows sit in
+// LDS with an odd stride (no bank conflicts between the tile rows).
+// Un-normalised; |p|^2 is returned per centre.
 template <class T>
 __global__ __launch_bounds__(kThreads) void power_spectrum_kernel(int N, int L, int S, int KC,
                                                                   const T* __restrict__ c,
                                                                   T* __restrict__ p,
                                                                   double* __restrict__ norm2) {
   const int64_t i = blockIdx.x;
-  const int L1 = L + 1, LL = L1 * L1, B = N * LL;
+  const int L1 = L + 1, LL = L1 * L1, B = N * LL, LS = LL | 1;
   const int K = S * (S + 1) / 2;
   const int NT = (N + 3) / 4;
   const int KB = N * N * L1;  // one key block
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  T* cs = reinterpret_cast<T*>(smem_raw);  // [S][N][LL]
-  T* stage = cs + S * B;                   // [KC][N][N][L1]
+  extern __shared__ __align__(64) unsigned char smem_raw[];
+  T* cs = reinterpret_cast<T*>(smem_raw);  // [S][N][LS]
+  T* stage = cs + S * N * LS;              // [KC][N][N][L1]
+  int* key_ab = reinterpret_cast<int*>(stage + KC * KB);  // [K] a << 8 | b
   __shared__ double red[kWaves];
   const T* ci = c + i * int64_t(S) * B;
-  for (int e = threadIdx.x; e < S * B; e += kThreads) cs[e] = ci[e];
+  const FastDiv fLL = fast_div(LL);
+  for (int e = threadIdx.x; e < S * B; e += kThreads) {
+    const int row = e / fLL;
+    cs[row * LS + (e - row * LL)] = ci[e];
+  }
+  for (int k = threadIdx.x; k < K; k += kThreads) {
+    int a, b;
+    key_species(k, S, &a, &b);
+    key_ab[k] = (a << 8) | b;
+  }
   __syncthreads();
   T* pi = p + i * int64_t(K) * KB;
+  const FastDiv fNT = fast_div(NT), fL1 = fast_div(L1);
   double ss = 0;
   for (int k0 = 0; k0 < K; k0 += KC) {
     const int kc = min(KC, K - k0);
     const int items = kc * L1 * NT * NT;
     for (int it = threadIdx.x; it < items; it += kThreads) {
-      int r = it;
-      const int t2 = r % NT;
-      r /= NT;
-      const int t1 = r % NT;
-      r /= NT;
-      const int l = r % L1;
-      const int kk = r / L1;
-      int a, b;
-      key_species(k0 + kk, S, &a, &b);
-      const T* ca = cs + a * B;
-      const T* cb = cs + b * B;
+      const int r = it / fNT;
+      const int t2 = it - r * NT;
+      const int r2 = r / fNT;
+      const int t1 = r - r2 * NT;
+      const int kk = r2 / fL1;
+      const int l = r2 - kk * L1;
+      const int ab = key_ab[k0 + kk];
+      const int a = ab >> 8, b = ab & 255;
+      const T* ca = cs + a * N * LS;
+      const T* cb = cs + b * N * LS;
       const int n1 = 4 * t1, n2 = 4 * t2;
       T acc[4][4];
 #pragma unroll
@@ -501,8 +557,8 @@ __global__ __launch_bounds__(kThreads) void power_spectrum_kernel(int N, int L, 
         T va[4], vb[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
-          va[u] = (n1 + u < N) ? ca[(n1 + u) * LL + lm] : T(0);
-          vb[u] = (n2 + u < N) ? cb[(n2 + u) * LL + lm] : T(0);
+          va[u] = (n1 + u < N) ? ca[(n1 + u) * LS + lm] : T(0);
+          vb[u] = (n2 + u < N) ? cb[(n2 + u) * LS + lm] : T(0);
         }
 #pragma unroll
         for (int u = 0; u < 4; ++u)
@@ -569,6 +625,8 @@ __global__ void normalise_kernel(int64_t n, int64_t D, const T* __restrict__ p,
 //     :2306-2366 contracted with dL/dc), grad[j] += g, and the centre gets
 //     -sum g unless j is a periodic image of i; the virial sum r_ji (x) g is
 //     reduced per centre (sparse_kernel_predict.hh:354-364).
+constexpr int kAdjPre = 8;  // key-block elements per thread held for the prefetch
+
 template <class T, bool DIRECT>
 __global__ __launch_bounds__(kThreads) void adjoint_kernel(
     RadialDev rad, int S, int PB, int c_in_lds, PairView pv, const T* __restrict__ c,
@@ -584,8 +642,10 @@ __global__ __launch_bounds__(kThreads) void adjoint_kernel(
   const int64_t D = int64_t(K) * KB;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   extern __shared__ __align__(64) unsigned char smem_raw[];
-  T* dc = reinterpret_cast<T*>(smem_raw);  // [S][N][LL]
-  unsigned char* region = smem_raw + align_up(sizeof(T) * E, 64);
+  T* yc = reinterpret_cast<T*>(smem_raw);
+  T* dc = reinterpret_cast<T*>(smem_raw + ylm_table_bytes<T>(L1));  // [S][N][LL]
+  unsigned char* region = reinterpret_cast<unsigned char*>(dc) + align_up(sizeof(T) * E, 64);
+  load_ylm_table<T>(rad, yc);
   const T* cs;
   T* gA;
   if (c_in_lds) {
@@ -631,33 +691,61 @@ __global__ __launch_bounds__(kThreads) void adjoint_kernel(
   const T ta = T(alpha), tb = T(beta);
   for (int e = tid; e < E; e += kThreads) dc[e] = T(0);
   for (int e = tid; e < 2 * L1 * N * N4; e += kThreads) gA[e] = T(0);  // zero pads
-  __syncthreads();
 
+  // key blocks: element o = (x*N + y)*L1 + l; thread holds o = tid + 256 r
+  const FastDiv fNL1 = fast_div(N * L1), fL1 = fast_div(L1), fLL = fast_div(LL);
+  const bool prefetch = KB <= kAdjPre * kThreads;
+  T pre[kAdjPre];
+  auto fetch = [&](int key) {
+    if (!prefetch) return;
+    const T* Gk = Gi + int64_t(key) * KB;
+    const T* Pk = Pi + int64_t(key) * KB;
+#pragma unroll
+    for (int r = 0; r < kAdjPre; ++r) {
+      const int o = tid + kThreads * r;
+      pre[r] = o < KB ? ta * Gk[o] - tb * Pk[o] : T(0);
+    }
+  };
+  fetch(0);
+  __syncthreads();
   const int units_a = (N4 / 4) * LL;
   for (int key = 0; key < K; ++key) {
     int a, b;
     key_species(key, S, &a, &b);
-    const T* Gk = Gi + int64_t(key) * KB;
-    const T* Pk = Pi + int64_t(key) * KB;
-    for (int o = tid; o < KB; o += kThreads) {
-      const int x = o / (N * L1), r = o - x * N * L1, y = r / L1, l = r - y * L1;
-      const T g = ta * Gk[o] - tb * Pk[o];
-      if (a == b) {
-        const int o2 = (y * N + x) * L1 + l;
-        gA[(l * N + y) * N4 + x] = g + (ta * Gk[o2] - tb * Pk[o2]);
-      } else {
+    if (prefetch) {
+#pragma unroll
+      for (int r = 0; r < kAdjPre; ++r) {
+        const int o = tid + kThreads * r;
+        if (o < KB) {
+          const int x = o / fNL1, rr = o - x * N * L1, y = rr / fL1, l = rr - y * L1;
+          gA[(l * N + y) * N4 + x] = pre[r];
+          gB[(l * N + x) * N4 + y] = pre[r];
+        }
+      }
+    } else {
+      const T* Gk = Gi + int64_t(key) * KB;
+      const T* Pk = Pi + int64_t(key) * KB;
+      for (int o = tid; o < KB; o += kThreads) {
+        const T g = ta * Gk[o] - tb * Pk[o];
+        const int x = o / fNL1, rr = o - x * N * L1, y = rr / fL1, l = rr - y * L1;
         gA[(l * N + y) * N4 + x] = g;
         gB[(l * N + x) * N4 + y] = g;
       }
     }
     __syncthreads();
+    if (a == b) {
+      // symmetrise: gA[l][x][y] = g[y][x] + g[x][y]
+      for (int e = tid; e < L1 * N * N4; e += kThreads) gA[e] += gB[e];
+      __syncthreads();
+    }
+    if (key + 1 < K) fetch(key + 1);
     const T w = (a < b) ? T(1.41421356237309504880) : T(1);
     const int units = units_a * (a < b ? 2 : 1);
     for (int u = tid; u < units; u += kThreads) {
       const bool side_b = u >= units_a;
       const int uu = side_b ? u - units_a : u;
-      const int nt = uu / LL, lm = uu - nt * LL;
-      const int l = isqrt_small(lm);
+      const int nt = uu / fLL, lm = uu - nt * LL;
+      const int l = l_of_lm(lm);
       const T* gt = (side_b ? gB : gA) + l * N * N4 + 4 * nt;
       const T* cx = cs + (side_b ? a : b) * B + lm;
       V4 s4 = {T(0), T(0), T(0), T(0)};
@@ -684,7 +772,7 @@ __global__ __launch_bounds__(kThreads) void adjoint_kernel(
   double fi[3] = {0, 0, 0}, v6[6] = {0, 0, 0, 0, 0, 0};
   for (int q0 = 0; q0 < np; q0 += PB) {
     const int nb = min(PB, np - q0);
-    stage_pairs<T, DIRECT, true>(rad, pv, p0 + q0, nb, pb, nullptr, nullptr, nullptr);
+    stage_pairs<T, DIRECT, true>(rad, yc, pv, p0 + q0, nb, pb, nullptr, nullptr, nullptr);
     for (int q = wave; q < nb; q += kWaves) {
       const T* W = dc + pb.sp[q] * B;
       const T* Yq = pb.Y + q * LL;
@@ -694,7 +782,7 @@ __global__ __launch_bounds__(kThreads) void adjoint_kernel(
       const T fc = T(pb.fc[q]), dfc = T(pb.dfc[q]);
       T A = T(0), Bx = T(0), By = T(0), Bz = T(0);
       for (int lm = lane; lm < LL; lm += 64) {
-        const int l = isqrt_small(lm);
+        const int l = l_of_lm(lm);
         T Dn = T(0), Ddn = T(0);
         for (int n = 0; n < N; ++n) {
           const T wv = W[n * LL + lm];
@@ -819,7 +907,7 @@ __global__ void radial_finalise_kernel(RadialDev rad, const double* __restrict__
   out[it] = s;
 }
 
-// neighbour radial contribution as used by the expansion (spline or raw)
+// neighbour radial contribution as used by the expansion (spline)
 template <class T>
 __global__ void radial_eval_kernel(RadialDev rad, const double* __restrict__ r, int64_t n_pts,
                                    double* __restrict__ R, double* __restrict__ dR) {
@@ -836,21 +924,23 @@ __global__ void radial_eval_kernel(RadialDev rad, const double* __restrict__ r, 
 }
 
 template <class T>
-__global__ __launch_bounds__(kThreads) void ylm_eval_kernel(int L, const double* __restrict__ dirs,
+__global__ __launch_bounds__(kThreads) void ylm_eval_kernel(RadialDev rad, const double* __restrict__ dirs,
                                                             int64_t n, double* __restrict__ Y,
                                                             double* __restrict__ dY) {
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  const int L1 = L + 1, LL = L1 * L1;
-  const int per_wave = 64 / L1, per_block = 4 * per_wave;
-  T* dsh = reinterpret_cast<T*>(smem_raw);
+  extern __shared__ __align__(64) unsigned char smem_raw[];
+  const int L = rad.L, L1 = L + 1, LL = L1 * L1;
+  const int per_wave = 64 / L1, per_block = kWaves * per_wave;
+  T* yc = reinterpret_cast<T*>(smem_raw);
+  T* dsh = reinterpret_cast<T*>(smem_raw + ylm_table_bytes<T>(L1));
   T* Ysh = dsh + 3 * per_block;
   T* dYsh = Ysh + per_block * LL;
+  load_ylm_table<T>(rad, yc);
   const int64_t q0 = int64_t(blockIdx.x) * per_block;
   const int nb = int(min<int64_t>(per_block, n - q0));
   for (int t = threadIdx.x; t < 3 * nb; t += kThreads) dsh[t] = T(dirs[3 * q0 + t]);
   __syncthreads();
   const int wave = threadIdx.x >> 6;
-  ylm_wave<T>(L, dsh, wave * per_wave, min(nb, (wave + 1) * per_wave), Ysh, dY ? dYsh : nullptr, LL,
+  ylm_wave<T>(L, yc, dsh, wave * per_wave, min(nb, (wave + 1) * per_wave), Ysh, dY ? dYsh : nullptr, LL,
               3 * LL);
   __syncthreads();
   for (int t = threadIdx.x; t < nb * LL; t += kThreads) Y[q0 * LL + t] = double(Ysh[t]);
@@ -885,9 +975,10 @@ template <class T, int KPT, bool DIRECT>
 void launch_expansion_t(const RadialDev& rad, int S, const PairView& pv, T* c,
                         unsigned long long* keymask, hipStream_t stream) {
   const int F = rad.N * rad.L1;
+  const size_t yt = ylm_table_bytes<T>(rad.L1);
   int PB = 64;
-  while (PB > 1 && pair_batch_bytes(PB, rad.LL, F, sizeof(T), DIRECT, false) > kFwdBudget) PB /= 2;
-  const size_t smem = pair_batch_bytes(PB, rad.LL, F, sizeof(T), DIRECT, false);
+  while (PB > 1 && yt + pair_batch_bytes(PB, rad.LL, F, sizeof(T), DIRECT, false) > kFwdBudget) PB /= 2;
+  const size_t smem = yt + pair_batch_bytes(PB, rad.LL, F, sizeof(T), DIRECT, false);
   auto kern = expansion_kernel<T, KPT, DIRECT>;
   allow_lds(kern, smem);
   hipLaunchKernelGGL(kern, dim3(unsigned(pv.n_centres)), dim3(kThreads), smem, stream, rad, S, PB, pv,
@@ -910,14 +1001,16 @@ void launch_adjoint_t(const RadialDev& rad, int S, const PairView& pv, const T* 
                       const T* G, double* grad, double* centre_virial, hipStream_t stream) {
   const int N = rad.N, L1 = rad.L1, LL = rad.LL, F = N * L1, E = S * N * LL;
   const int N4 = (N + 3) & ~3;
+  const size_t yt = ylm_table_bytes<T>(L1);
   const size_t blk = align_up(sizeof(T) * size_t(E), 64);
   const size_t gbytes = sizeof(T) * 2 * size_t(L1) * N * N4;
-  const int c_in_lds = blk + blk + gbytes <= kAdjBudget ? 1 : 0;
+  const int c_in_lds = yt + blk + blk + gbytes <= kAdjBudget ? 1 : 0;
   const size_t region = (c_in_lds ? blk : 0) + gbytes;
-  const size_t batch_room = std::max(region, kAdjBudget > blk ? kAdjBudget - blk : size_t(0));
+  const size_t fixed = yt + blk;
+  const size_t batch_room = std::max(region, kAdjBudget > fixed ? kAdjBudget - fixed : size_t(0));
   int PB = 8;
   while (PB > 1 && pair_batch_bytes(PB, LL, F, sizeof(T), DIRECT, true) > batch_room) --PB;
-  const size_t smem = blk + std::max(region, pair_batch_bytes(PB, LL, F, sizeof(T), DIRECT, true));
+  const size_t smem = fixed + std::max(region, pair_batch_bytes(PB, LL, F, sizeof(T), DIRECT, true));
   auto kern = adjoint_kernel<T, DIRECT>;
   allow_lds(kern, smem);
   hipLaunchKernelGGL(kern, dim3(unsigned(pv.n_centres)), dim3(kThreads), smem, stream, rad, S, PB,
@@ -956,12 +1049,13 @@ template <class T>
 void launch_power_spectrum(const RadialDev& rad, int S, int64_t n, const T* c, T* p, double* norm2,
                            hipStream_t stream) {
   if (n == 0) return;
-  const size_t cbytes = sizeof(T) * size_t(S) * rad.N * rad.LL;
-  const size_t kb = sizeof(T) * size_t(rad.N) * rad.N * rad.L1;
   const int K = S * (S + 1) / 2;
+  if (S > kMaxSpecies) throw InvalidArgument("at most 64 species");
+  const size_t cbytes = sizeof(T) * size_t(S) * rad.N * (rad.LL | 1);
+  const size_t kb = sizeof(T) * size_t(rad.N) * rad.N * rad.L1;
   int KC = cbytes + kb < kPsBudget ? int((kPsBudget - cbytes) / kb) : 1;
   KC = std::max(1, std::min(KC, K));
-  const size_t smem = cbytes + size_t(KC) * kb;
+  const size_t smem = cbytes + size_t(KC) * kb + sizeof(int) * size_t(K);
   auto kern = power_spectrum_kernel<T>;
   allow_lds(kern, smem);
   hipLaunchKernelGGL(kern, dim3(unsigned(n)), dim3(kThreads), smem, stream, rad.N, rad.L, S, KC, c, p,
@@ -1038,12 +1132,13 @@ void launch_radial_spline_eval(const RadialDev& rad, const double* r, int64_t n_
 }
 
 template <class T>
-void launch_ylm_eval(int L, const double* dirs, int64_t n, double* Y, double* dY, hipStream_t stream) {
+void launch_ylm_eval(const RadialDev& rad, const double* dirs, int64_t n, double* Y, double* dY,
+                     hipStream_t stream) {
   if (n == 0) return;
-  const int L1 = L + 1, LL = L1 * L1, per_block = 4 * (64 / L1);
-  size_t smem = sizeof(T) * (3 * per_block + per_block * LL + per_block * 3 * LL);
+  const int L1 = rad.L + 1, LL = L1 * L1, per_block = kWaves * (64 / L1);
+  size_t smem = ylm_table_bytes<T>(L1) + sizeof(T) * (3 * per_block + per_block * LL + per_block * 3 * LL);
   hipLaunchKernelGGL(ylm_eval_kernel<T>, dim3(unsigned((n + per_block - 1) / per_block)), dim3(kThreads),
-                     smem, stream, L, dirs, n, Y, dY);
+                     smem, stream, rad, dirs, n, Y, dY);
   RGPU_LAUNCH_CHECK();
 }
 
@@ -1051,8 +1146,10 @@ template void launch_radial_spline_eval<double>(const RadialDev&, const double*,
                                                 hipStream_t);
 template void launch_radial_spline_eval<float>(const RadialDev&, const double*, int64_t, double*, double*,
                                                hipStream_t);
-template void launch_ylm_eval<double>(int, const double*, int64_t, double*, double*, hipStream_t);
-template void launch_ylm_eval<float>(int, const double*, int64_t, double*, double*, hipStream_t);
+template void launch_ylm_eval<double>(const RadialDev&, const double*, int64_t, double*, double*,
+                                      hipStream_t);
+template void launch_ylm_eval<float>(const RadialDev&, const double*, int64_t, double*, double*,
+                                     hipStream_t);
 
 template void launch_expansion<double>(const RadialDev&, int, const PairView&, double*,
                                        unsigned long long*, hipStream_t);
