@@ -138,11 +138,6 @@ int rgpu_gap_knm(rgpu_context* ctx, int target, double* knm);
  * sparse_kernel_predict.hh:206-377). */
 int rgpu_gap_predict(rgpu_context* ctx, double* energy, double* gradient, double* neg_stress);
 
-/* Diagnostics (unit-level parity): radial integrals O^T R(r) (or raw R for
- * the direct basis, see the library docs) at n points: R[n][N][L+1],
- * dR[n][N][L+1] (may be NULL), evaluated by the GPU path of the current
- * precision; Y_lm and gradients at n unit vectors: Y[n][(L+1)^2],
- * dY[n][3][(L+1)^2] (may be NULL). */
 /* Stage timers: HIP events recorded on the context stream around each
  * stage's kernels.  rgpu_get_timing synchronises and returns the summed
  * milliseconds and the number of launches since the last reset.  No
@@ -152,7 +147,15 @@ int rgpu_enable_timing(rgpu_context* ctx, int enable);
 int rgpu_get_timing(rgpu_context* ctx, int stage, double* total_ms, int64_t* launches);
 int rgpu_reset_timing(rgpu_context* ctx);
 
+/* Diagnostics (unit-level parity).  Radial functions at n distances r[n]:
+ * R[n][N][L+1] and dR/dr (dR may be NULL), evaluated by the GPU path of the
+ * current precision: the splined O^T R(r) for Spline hypers
+ * (calculator_spherical_expansion.hh:1107-1156), the raw GTO integrals
+ * before orthonormalisation for the direct Hyp1f1 basis (:518-584). */
 int rgpu_eval_radial(rgpu_context* ctx, int64_t n, const double* r, double* R, double* dR);
+/* Real spherical harmonics at n unit vectors dirs[n][3]: Y[n][(L+1)^2] and
+ * the gradient on the unit sphere dY[n][3][(L+1)^2] (may be NULL)
+ * (math/spherical_harmonics.cc:135-363). */
 int rgpu_eval_ylm(rgpu_context* ctx, int64_t n, const double* dirs, double* Y, double* dY);
 
 #ifdef __cplusplus

@@ -7,6 +7,19 @@
  * tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load it.
  * The product (librascal_amd) never links or calls it.
  *
+ * How each part is pinned (tests/test_oracle_golden.py, test_oracle_math.py):
+ *   Hyp1f1 - 4800 mpmath values of the reference (15 DBL_FTOL), Kummer
+ *     identity, recursion = direct; Y_lm / ALPs - reference ubjson (2 / 10
+ *     DBL_FTOL), scipy closed form, finite differences; cutoffs, radial
+ *     derivatives, representation gradients - finite differences; GTO
+ *     radial - closed form with scaled Bessel functions; expansion - 32 GTO
+ *     rows of spherical_expansion_reference (1e-8); power spectrum - cosine
+ *     kernels of kernel_reference (1e-10) and the 30 sparse points of
+ *     simple_gap_model.json; neighbour list - brute force and cell-skew
+ *     invariance; K_NM, K_MM - closed forms; GAP forces / stress / K_NM
+ *     derivative - finite differences and K_NM_der . w = gradient.
+ *   Parity unpinned: none.
+ *
  * Data conventions (shared with the HIP library):
  *   - positions: [n_atoms][3] row-major, Cartesian, Angstrom
  *   - cell: [3][3] with cell[d*3 + k] = component d of lattice vector k

@@ -367,3 +367,37 @@ def test_gap_forces_and_stress_finite_difference(zeta):
     kd = O.gap_knm_derivative(calc, b, ss, neg_stress=True)
     assert np.max(np.abs(kd[:3 * b.n_atoms] @ w[ss.order] - g.ravel())) < 1e-12 * np.abs(g).max()
     assert np.max(np.abs(kd[3 * b.n_atoms:] @ w[ss.order] - s.ravel())) < 1e-12 * np.abs(s).max()
+
+
+@pytest.mark.parametrize("zeta", [1, 2, 3])
+def test_gap_kmm_closed_form(zeta):
+    """sparse_kernels.hh:136-150: K_MM is block diagonal by species with
+    blocks (T_a T_a^T)^zeta, rows and columns in species-sorted order."""
+    _, _, ss, _ = _gap_setup(zeta)
+    K = O.gap_kmm(ss)
+    T, s = ss.values, ss.species  # stored species-sorted
+    ref = np.where(s[:, None] == s[None, :], (T @ T.T) ** zeta, 0.0)
+    assert np.max(np.abs(K - ref)) < 1e-13 * max(1.0, np.abs(ref).max())
+    assert np.allclose(K, K.T)
+
+
+def test_gap_knm_derivative_columns_finite_difference():
+    """Each column of the K_NM derivative (sparse_kernels.hh:258-499) against
+    central differences of the Structure kernel column, for two columns."""
+    sts, calc, ss, _ = _gap_setup(2)
+    b = O.BatchData(sts, species=[1, 8])
+    kd = O.gap_knm_derivative(calc, b, ss, neg_stress=False)
+    st = sts[0]
+    h = 1e-5
+    for m in (0, 3):
+        fd = np.zeros(3 * len(st["numbers"]))
+        for k in range(len(st["numbers"])):
+            for c in range(3):
+                vals = []
+                for sgn in (1, -1):
+                    p = st["positions"].copy()
+                    p[k, c] += sgn * h
+                    bb = O.BatchData([dict(st, positions=p)], species=[1, 8])
+                    vals.append(O.gap_knm(calc, bb, ss, "Structure")[0, m])
+                fd[3 * k + c] = (vals[0] - vals[1]) / (2 * h)
+        assert np.max(np.abs(fd - kd[:, m])) < 1e-6 * max(1e-12, np.abs(kd[:, m]).max())
