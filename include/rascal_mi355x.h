@@ -131,6 +131,11 @@ int rgpu_set_sparse_points(rgpu_context* ctx, int zeta, int64_t n_sparse, const 
  * 1 = Atom [n_centres][n_sparse]; columns in the order the sparse points were
  * given (SparseKernelImpl<GAP>::compute, sparse_kernels.hh:104-190). */
 int rgpu_gap_knm(rgpu_context* ctx, int target, double* knm);
+/* K_MM [n_sparse][n_sparse] in FP64: (T_a T_a^T)^zeta between sparse points
+ * of the same species a, 0 across species; rows and columns in the order the
+ * sparse points were given (SparseKernelImpl<GAP>::compute(sparse_points),
+ * models/sparse_kernels.hh:136-150). */
+int rgpu_gap_kmm(rgpu_context* ctx, double* kmm);
 /* Prediction without K_NM rows: energy[n_struct] = K_NM w, gradient
  * [n_atoms][3] = dE/dr, neg_stress [n_struct][6] (/ volume).  Any output may
  * be NULL; gradient/neg_stress trigger the adjoint

@@ -61,6 +61,7 @@ def lib():
     L.rgpu_features_backward.argtypes = [p, p, p, p]
     L.rgpu_set_sparse_points.argtypes = [p, i, i64, p, p, p]
     L.rgpu_gap_knm.argtypes = [p, i, p]
+    L.rgpu_gap_kmm.argtypes = [p, p]
     L.rgpu_gap_predict.argtypes = [p, p, p, p]
     L.rgpu_eval_radial.argtypes = [p, i64, p, p, p]
     L.rgpu_eval_ylm.argtypes = [p, i64, p, p, p]
@@ -70,7 +71,7 @@ def lib():
     for name in ("rgpu_context_create", "rgpu_set_stream", "rgpu_set_precision", "rgpu_get_info",
                  "rgpu_set_structures", "rgpu_update_positions", "rgpu_get_neighbours",
                  "rgpu_compute", "rgpu_get_expansion", "rgpu_get_features",
-                 "rgpu_features_backward", "rgpu_set_sparse_points", "rgpu_gap_knm",
+                 "rgpu_features_backward", "rgpu_set_sparse_points", "rgpu_gap_knm", "rgpu_gap_kmm",
                  "rgpu_gap_predict", "rgpu_eval_radial", "rgpu_eval_ylm", "rgpu_enable_timing",
                  "rgpu_get_timing", "rgpu_reset_timing"):
         getattr(L, name).restype = C.c_int
@@ -190,6 +191,12 @@ class Context:
         rows = self.info("n_struct") if t == 0 else self.info("n_centres")
         k = np.zeros((rows, self.info("n_sparse")))
         check(lib().rgpu_gap_knm(self._p, t, ptr(k)))
+        return k
+
+    def gap_kmm(self):
+        M = self.info("n_sparse")
+        k = np.zeros((M, M))
+        check(lib().rgpu_gap_kmm(self._p, ptr(k)))
         return k
 
     def gap_predict(self, energy=True, gradient=True, neg_stress=True):

@@ -79,6 +79,7 @@ struct Context {
   // scratch
   DeviceBuffer<double> e_centre, grad, cvir, tmp64, out64;
   DeviceBuffer<double> pair_fc, pair_dfc;  // cutoff function per pair
+  DeviceBuffer<int> d_sp_rows;             // 0..M-1 (K_MM row gather)
   DeviceBuffer<double> wt64, G64;
   DeviceBuffer<float> wt32, G32;
   DeviceBuffer<int64_t> knm_perm;
@@ -280,9 +281,16 @@ void require_batch(const Context& c) {
 
 void build_batch_nl(Context& c, const double* positions) {
   StageTimer t(c, RGPU_STAGE_NEIGHBOURS);
+  c.have_rep = false;
+  if (c.n_atoms == 0) {
+    // empty batch: only the species list (feature layout) is defined
+    c.pl.n_atoms = c.pl.n_pairs = 0;
+    int64_t* off = c.pl.centre_off.reserve(1);
+    RGPU_HIP(hipMemsetAsync(off, 0, sizeof(int64_t), c.stream));
+    return;
+  }
   build_pair_list(c.pl, c.n_struct, c.atom_off.data(), positions, c.cells.data(), c.pbc.data(),
                   c.hp.cutoff, c.stream);
-  c.have_rep = false;
 }
 
 PairView pair_view(Context& c) {
@@ -849,7 +857,12 @@ int rgpu_set_sparse_points(rgpu_context* ctx, int zeta, int64_t n_sparse, const 
       auto it = std::lower_bound(c.species.begin(), c.species.end(), species[m]);
       sidx[m] = (it == c.species.end() || *it != species[m]) ? S : int(it - c.species.begin());
     }
-    std::stable_sort(order.begin(), order.end(), [&](int64_t a, int64_t b) { return sidx[a] < sidx[b]; });
+    // species-sorted: batch species first (in the context's order), then any
+    // other species grouped by value (they contribute no K_NM columns)
+    std::stable_sort(order.begin(), order.end(), [&](int64_t a, int64_t b) {
+      if (sidx[a] != sidx[b]) return sidx[a] < sidx[b];
+      return sidx[a] == S && species[a] < species[b];
+    });
     c.sp_order = order;
     c.sp_species.resize(size_t(n_sparse));
     c.sp_off.assign(S + 1, 0);
@@ -875,6 +888,46 @@ int rgpu_set_sparse_points(rgpu_context* ctx, int zeta, int64_t n_sparse, const 
     c.have_weights = weights != nullptr;
     prepare_mixed_sparse(c.mixed_sparse, T, n_sparse, D, c.stream);
     c.have_sparse = true;
+  });
+}
+
+int rgpu_gap_kmm(rgpu_context* ctx, double* kmm) {
+  return guarded([&] {
+    if (!ctx || !kmm) throw InvalidArgument("null argument");
+    Context& c = ctx->c;
+    if (!c.have_sparse) throw InvalidArgument("no sparse points set");
+    const int64_t M = c.M, D = c.D();
+    std::vector<int> iota(static_cast<size_t>(std::max<int64_t>(M, 1)));
+    for (int64_t m = 0; m < M; ++m) iota[size_t(m)] = int(m);
+    int* rows = c.d_sp_rows.reserve(iota.size());
+    RGPU_HIP(hipMemcpyAsync(rows, iota.data(), sizeof(int) * iota.size(), hipMemcpyHostToDevice, c.stream));
+    double* k = c.tmp64.reserve(size_t(std::max<int64_t>(M * M, 1)));
+    RGPU_HIP(hipMemsetAsync(k, 0, sizeof(double) * M * M, c.stream));
+    // one block per run of equal species (sparse points are species-sorted)
+    for (int64_t m0 = 0; m0 < M;) {
+      int64_t m1 = m0 + 1;
+      while (m1 < M && c.sp_species[size_t(m1)] == c.sp_species[size_t(m0)]) ++m1;
+      GapBlock<double> b;
+      b.n_rows = b.n_cols = m1 - m0;
+      b.row_idx = rows + m0;
+      b.col0 = m0;
+      b.X = c.T64.get();
+      b.norm2 = nullptr;
+      b.normalize = 0;
+      b.Tm = c.T64.get();
+      b.alpha = nullptr;
+      b.zeta = c.zeta;
+      b.D = D;
+      launch_gap_kernel<double>(b, M, nullptr, nullptr, k, nullptr, nullptr, c.stream);
+      m0 = m1;
+    }
+    std::vector<double> h(static_cast<size_t>(M * M)), o(h.size());
+    RGPU_HIP(hipMemcpyAsync(h.data(), k, sizeof(double) * h.size(), hipMemcpyDeviceToHost, c.stream));
+    RGPU_HIP(hipStreamSynchronize(c.stream));
+    for (int64_t r = 0; r < M; ++r)
+      for (int64_t m = 0; m < M; ++m)
+        o[size_t(c.sp_order[r] * M + c.sp_order[m])] = h[size_t(r * M + m)];
+    RGPU_HIP(hipMemcpy(kmm, o.data(), sizeof(double) * o.size(), hipMemcpyDefault));
   });
 }
 

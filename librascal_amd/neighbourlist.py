@@ -71,13 +71,15 @@ class AtomsList:
         return int(self.atom_offsets[-1])
 
     def compute(self, calculator, species=None):
-        sp = sorted(set(self.numbers.tolist()) | set(species or []))
-        ctx = calculator._context
-        ctx.set_structures(self.atom_offsets, self.positions, self.numbers, self.cells, self.pbc, sp)
-        ctx.compute()
+        self.species = sorted(set(self.numbers.tolist()) | set(species or []))
         self.calculator = calculator
-        self.species = sp
+        calculator._loaded = None
+        calculator._load(self)
         return self
+
+    def _upload(self, calculator):
+        calculator._context.set_structures(self.atom_offsets, self.positions, self.numbers,
+                                           self.cells, self.pbc, self.species)
 
     def _require(self, calculator):
         if calculator is None:
@@ -85,7 +87,13 @@ class AtomsList:
         if calculator is None or calculator is not self.calculator:
             raise ValueError("the representation of this AtomsList was computed with another "
                              "calculator")
+        calculator._load(self)
         return calculator
+
+    @property
+    def managers(self):
+        """The reference's ManagerCollection handle; here the list itself."""
+        return self
 
     def _present_keys(self, calculator):
         """Keys present in at least one centre (dense export drops the rest,

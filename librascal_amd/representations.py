@@ -61,6 +61,10 @@ class _SphericalBase:
         # validates the hypers on the device side (raises like the reference)
         self._context = _capi.Context(json.dumps(self.hypers))
         self.precision = _capi.FP64
+        # the batch whose structures are loaded in the context (an AtomsList,
+        # or a species tuple for an empty batch that only fixes the layout)
+        self._loaded = None
+        self._sparse_token = None
 
     @property
     def cutoff(self):
@@ -71,7 +75,28 @@ class _SphericalBase:
         invariants, BF16 hi+lo kernel GEMM)."""
         self.precision = {"fp64": _capi.FP64, "mixed": _capi.MIXED}[precision]
         self._context.set_precision(self.precision)
+        if self._loaded is not None and not isinstance(self._loaded, tuple):
+            self._context.compute()
         return self
+
+    def _load(self, atoms_list):
+        """Make atoms_list the batch held by the context (recomputing its
+        representation if another batch was loaded since)."""
+        if self._loaded is not atoms_list:
+            atoms_list._upload(self)
+            self._context.compute()
+            self._loaded = atoms_list
+            self._sparse_token = None
+
+    def _load_species(self, species):
+        """Empty batch with the given species list (feature layout only)."""
+        sp = tuple(sorted(set(int(z) for z in species)))
+        if self._loaded != sp:
+            self._context.set_structures(np.zeros(1, np.int64), np.zeros((0, 3)),
+                                         np.zeros(0, np.int32), np.zeros((0, 9)),
+                                         np.zeros((0, 3), np.int32), list(sp))
+            self._loaded = sp
+            self._sparse_token = None
 
     def get_name(self):
         return json.dumps(self.hypers, sort_keys=True)

@@ -124,3 +124,44 @@ def radial_rel(ref, test):
     sc = np.abs(ref).max(axis=0, keepdims=True)
     sc = np.where(sc > 0, sc, 1e-300)
     return _block_rel(ref, test, sc)
+
+
+def oracle_backward(calc, batch, G):
+    """dL/dr and neg stress of L = sum_i G_i . X_i from the oracle's stored
+    per-pair feature gradients (the reference's gradient layout)."""
+    import oracle as O
+    grads, info = calc.power_spectrum_gradients(batch)
+    n = batch.n_atoms
+    out = np.zeros((n, 3))
+    ns = np.zeros((batch.n_struct, 6))
+    off = batch.atom_offsets
+    vo = [(4, 2), (5, 0), (3, 1)]
+    row = 0
+    for s in range(batch.n_struct):
+        st = slice(off[s], off[s + 1])
+        ro, rj, rv, _ = O.neighbours(batch.positions[st], batch.cells[s], batch.pbc[s], calc._h.cutoff)
+        for i in range(off[s + 1] - off[s]):
+            for p in range(ro[i], ro[i + 1]):
+                f = grads[row] @ G[off[s] + i]
+                out[off[s] + rj[p]] += f
+                rji = -rv[p]
+                for x in range(3):
+                    ns[s, x] += rji[x] * f[x]
+                    ns[s, vo[x][0]] += rji[vo[x][1]] * f[x]
+                row += 1
+        ns[s] /= abs(np.linalg.det(batch.cells[s]))
+    return out, ns
+
+
+def random_sparse(calc, batch, n_per_species, rng, zeta):
+    X, _ = calc.power_spectrum(batch)
+    species, values = [], []
+    for t in sorted(set(batch.types.tolist())):
+        idx = np.where(batch.types == t)[0]
+        pick = rng.choice(idx, size=n_per_species, replace=True)
+        species += [t] * n_per_species
+        values.append(X[pick] + 0.01 * rng.normal(size=(n_per_species, X.shape[1])))
+    species = np.asarray(species)
+    values = np.concatenate(values)
+    perm = rng.permutation(len(species))
+    return species[perm], values[perm], rng.normal(size=len(species))

@@ -1,0 +1,148 @@
+"""The reference's Python model API (SparsePoints, Kernel, KRR, module
+gradient functions) and the autograd bindings, on the GPU, against the FP64
+oracle (tolerance 1e-10 relative, FP64 path)."""
+import numpy as np
+import pytest
+
+import oracle as O
+from conftest import golden_structures, oracle_backward, rel_err
+from librascal_amd.models import (KRR, Kernel, SparsePoints, compute_sparse_kernel_gradients,
+                                  compute_sparse_kernel_neg_stress)
+from librascal_amd.representations import SphericalInvariants
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-10
+
+
+def _rep():
+    return SphericalInvariants(interaction_cutoff=3.5, cutoff_smooth_width=0.5, max_radial=4,
+                               max_angular=3, gaussian_sigma_type="Constant",
+                               gaussian_sigma_constant=0.5)
+
+
+def _frames():
+    g = golden_structures()
+    return g["CaCrP2O7"] + g["SiC_moissanite_supercell"]
+
+
+def _model(rep, zeta, target="Structure"):
+    frames = _frames()
+    al = rep.transform(frames)
+    sp = SparsePoints(rep)
+    sel = [list(range(0, al.atom_offsets[s + 1] - al.atom_offsets[s], 3)) for s in range(len(al))]
+    sp.extend(al, sel)
+    kernel = Kernel(rep, name="GAP", zeta=zeta, target_type=target, kernel_type="Sparse")
+    rng = np.random.default_rng(5 + zeta)
+    w = rng.normal(size=sp.size())
+    return frames, al, sp, kernel, w
+
+
+@pytest.mark.parametrize("zeta", [1, 2, 3])
+def test_kernel_kmm_and_knm(zeta):
+    rep = _rep()
+    frames, al, sp, kernel, _ = _model(rep, zeta)
+    calc = O.Calculator(rep.hypers)
+    b = O.BatchData(frames, species=al.species)
+    ss = O.SparseSet(sp.numbers, sp.values_for(al.species), zeta)
+    # K_MM (rows/columns in the order the points were added)
+    kmm = kernel(sp)
+    ref = np.zeros_like(kmm)
+    ref[np.ix_(ss.order, ss.order)] = O.gap_kmm(ss)
+    assert np.abs(kmm - ref).max() < TOL * np.abs(ref).max()
+    # K_NM for both targets
+    for target in ("Structure", "Atom"):
+        kernel.target_type = target
+        k = kernel(al, sp)
+        kr = np.zeros_like(k)
+        kr[:, ss.order] = O.gap_knm(calc, b, ss, target)
+        assert rel_err(kr, k).max() < TOL
+
+
+def test_krr_predictions_and_gradient_functions():
+    rep = _rep()
+    frames, al, sp, kernel, w = _model(rep, 2)
+    self_contrib = {int(z): 0.1 * int(z) for z in set(al.numbers.tolist())}
+    model = KRR(w, kernel, sp, self_contrib)
+    calc = O.Calculator(rep.hypers)
+    b = O.BatchData(frames, species=al.species)
+    ss = O.SparseSet(sp.numbers, sp.values_for(al.species), 2)
+    e_ref, g_ref, s_ref = O.gap_predict(calc, b, ss, w)
+    y0 = np.array([sum(self_contrib[int(z)] for z in f["numbers"]) for f in frames])
+    e = model.predict(al)
+    assert rel_err(e_ref + y0, e).max() < TOL
+    f = model.predict_forces(al)
+    assert np.abs(f + g_ref).max() < TOL * np.abs(g_ref).max()
+    s = model.predict_stress(al)
+    assert np.abs(s + s_ref).max() < TOL * np.abs(s_ref).max()
+    g = compute_sparse_kernel_gradients(rep, kernel, al, sp, w)
+    ns = compute_sparse_kernel_neg_stress(rep, kernel, al, sp, w)
+    assert np.abs(g - g_ref).max() < TOL * np.abs(g_ref).max()
+    assert np.abs(ns - s_ref).max() < TOL * np.abs(s_ref).max()
+    # a precomputed K_NM gives the same energies (krr.py:110-126)
+    knm = kernel(al, sp)
+    assert rel_err(e, model.predict(al, KNM=knm)).max() < TOL
+
+
+def test_context_reloads_its_batch():
+    """Two AtomsLists computed with one calculator: each one's features stay
+    its own (the context is reloaded on demand)."""
+    rep = _rep()
+    g = golden_structures()
+    a1 = rep.transform(g["CaCrP2O7"])
+    x1 = a1.get_dense_features(rep).copy()
+    a2 = rep.transform(g["small_molecule"])
+    x2 = a2.get_dense_features(rep)
+    assert x1.shape != x2.shape or np.abs(x1 - x2).max() > 0
+    assert np.array_equal(a1.get_dense_features(rep), x1)
+
+
+def test_autograd_features_backward_and_finite_differences():
+    import torch
+    from librascal_amd.autograd import soap_features
+    rep = _rep()
+    frames = golden_structures()["CaCrP2O7"]
+    al = rep.transform(frames)
+    pos = torch.tensor(al.positions, dtype=torch.float64, device="cuda", requires_grad=True)
+    X = soap_features(pos, rep, al)
+    rng = np.random.default_rng(3)
+    R = torch.tensor(rng.normal(size=tuple(X.shape)), dtype=torch.float64, device="cuda")
+    (X * R).sum().backward()
+    calc = O.Calculator(rep.hypers)
+    b = O.BatchData(frames, species=al.species)
+    ref, _ = oracle_backward(calc, b, R.cpu().numpy())
+    grad = pos.grad.cpu().numpy()
+    assert np.abs(grad - ref).max() < TOL * np.abs(ref).max()
+    # central differences of the GPU forward itself (h = 1e-4 A)
+    h = 1e-4
+    base = al.positions.copy()
+    for atom, c in ((0, 0), (3, 2), (7, 1)):
+        vals = []
+        for sgn in (1, -1):
+            p = base.copy()
+            p[atom, c] += sgn * h
+            Xs = soap_features(torch.tensor(p, device="cuda"), rep, al)
+            vals.append(float((Xs * R).sum()))
+        fd = (vals[0] - vals[1]) / (2 * h)
+        assert abs(fd - grad[atom, c]) < 1e-6 * np.abs(grad).max()
+
+
+def test_autograd_gap_energy_matches_forces():
+    import torch
+    from librascal_amd.autograd import gap_energy
+    rep = _rep()
+    frames, al, sp, kernel, w = _model(rep, 2)
+    model = KRR(w, kernel, sp, {int(z): 0.0 for z in set(al.numbers.tolist())})
+    forces = model.predict_forces(al)
+    pos = torch.tensor(al.positions, dtype=torch.float64, device="cuda", requires_grad=True)
+    E = gap_energy(pos, model, al)
+    assert rel_err(model.predict(al), E.detach().cpu().numpy()).max() < TOL
+    (2.0 * E.sum()).backward()
+    assert np.abs(pos.grad.cpu().numpy() + 2.0 * forces).max() < TOL * np.abs(forces).max()
+
+
+def test_empty_batch_fixes_the_layout():
+    rep = _rep()
+    rep._load_species([1, 6, 8])
+    X, inv = rep._context.features()
+    assert X.shape == (0, rep.get_num_coefficients(3)) and inv.shape == (0,)

@@ -15,8 +15,8 @@ import numpy as np
 import pytest
 
 import oracle as O
-from conftest import (expansion_rel, golden_structures, ps_rel, radial_rel, rel_err, soap_hypers,
-                      wrap_cubic_periodic)
+from conftest import (expansion_rel, golden_structures, oracle_backward, ps_rel, radial_rel,
+                      random_sparse, rel_err, soap_hypers, wrap_cubic_periodic)
 from librascal_amd import _capi
 from librascal_amd.synthetic import diamond_cells, molecules, pack, periodic_box, triclinic_cells
 
@@ -128,30 +128,6 @@ def test_expansion_and_power_spectrum_fp64(case):
     assert rel_err(inv_ref, inv).max() < FP64_TOL
 
 
-def oracle_backward(calc, batch, G):
-    grads, info = calc.power_spectrum_gradients(batch)
-    n = batch.n_atoms
-    out = np.zeros((n, 3))
-    ns = np.zeros((batch.n_struct, 6))
-    off = batch.atom_offsets
-    vo = [(4, 2), (5, 0), (3, 1)]
-    row = 0
-    for s in range(batch.n_struct):
-        st = slice(off[s], off[s + 1])
-        ro, rj, rv, _ = O.neighbours(batch.positions[st], batch.cells[s], batch.pbc[s], calc._h.cutoff)
-        for i in range(off[s + 1] - off[s]):
-            for p in range(ro[i], ro[i + 1]):
-                f = grads[row] @ G[off[s] + i]
-                out[off[s] + rj[p]] += f
-                rji = -rv[p]
-                for x in range(3):
-                    ns[s, x] += rji[x] * f[x]
-                    ns[s, vo[x][0]] += rji[vo[x][1]] * f[x]
-                row += 1
-        ns[s] /= abs(np.linalg.det(batch.cells[s]))
-    return out, ns
-
-
 @pytest.mark.parametrize("case", [("CaCrP2O7", 3.0, 0.5, 4, 3, None),
                                   ("diamond_cubic_distorted", 5.0, 0.5, 4, 4, 1e-8),
                                   ("small_molecule", 3.5, 0.4, 5, 3, 1e-8)])
@@ -169,20 +145,6 @@ def test_features_backward_fp64(case):
     scale = np.abs(ref_g).max()
     assert np.abs(g - ref_g).max() < FP64_TOL * scale
     assert np.abs(s - ref_s).max() < FP64_TOL * np.abs(ref_s).max()
-
-
-def random_sparse(calc, batch, n_per_species, rng, zeta):
-    X, _ = calc.power_spectrum(batch)
-    species, values = [], []
-    for t in sorted(set(batch.types.tolist())):
-        idx = np.where(batch.types == t)[0]
-        pick = rng.choice(idx, size=n_per_species, replace=True)
-        species += [t] * n_per_species
-        values.append(X[pick] + 0.01 * rng.normal(size=(n_per_species, X.shape[1])))
-    species = np.asarray(species)
-    values = np.concatenate(values)
-    perm = rng.permutation(len(species))
-    return species[perm], values[perm], rng.normal(size=len(species))
 
 
 @pytest.mark.parametrize("zeta", [1, 2, 4])
