@@ -29,7 +29,7 @@ namespace {
 typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
 typedef __attribute__((__vector_size__(16 * sizeof(float)))) float f32x16;
 
-constexpr int BM = 128, BN = 128, BK = 32, LDK = BK + 8, NT = 256;
+constexpr int BM = 128, BN = 128, BK = 32, NT = 256;
 constexpr int kMaxProblems = 16;
 
 
@@ -80,10 +80,15 @@ __device__ __forceinline__ T pow_int(T v, int n) {
 
 template <int EPI>  // 0: GAP kernel epilogue, 1: store rows of G
 __global__ __launch_bounds__(NT, 2) void bf16x3_gemm(ProblemSet ps) {
-  __shared__ __align__(16) unsigned short sAh[BM * LDK];
-  __shared__ __align__(16) unsigned short sAl[BM * LDK];
-  __shared__ __align__(16) unsigned short sBh[BN * LDK];
-  __shared__ __align__(16) unsigned short sBl[BN * LDK];
+  // Two LDS stages.  Each operand tile is kept in the matrix-core fragment
+  // order: for a 32-row block rb and a 16-wide K chunk c, the 64 lanes'
+  // 16-byte fragments (lane l: row l%32, k 8*(l/32) .. +8) are contiguous, so
+  // fragment reads are conflict free and one K step needs one barrier.
+  constexpr int FR = (BM / 32) * (BK / 16) * 64;  // fragments per operand per stage
+  __shared__ __align__(16) bf16x8 sAh[2][FR];
+  __shared__ __align__(16) bf16x8 sAl[2][FR];
+  __shared__ __align__(16) bf16x8 sBh[2][FR];
+  __shared__ __align__(16) bf16x8 sBl[2][FR];
   __shared__ double esum[2][BM];
 
   // problem and tile of this block.  Workgroups are dealt to the 8 XCDs
@@ -102,14 +107,20 @@ __global__ __launch_bounds__(NT, 2) void bf16x3_gemm(ProblemSet ps) {
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave >> 1, wn = wave & 1;
 
-  // global load assignment: two threads per row, 16 K values each
-  const int lr = tid >> 1, lk = (tid & 1) * 16;
+  // global load assignment: thread -> (row lr, K half kh); a wavefront
+  // covers 64 consecutive rows of one K half, so its fragment stores are
+  // consecutive 16-byte words
+  const int lr = tid & (BM - 1), kh = tid >> 7;
+  const int lk = kh * 16;
   const int64_t arow = r0 + lr, brow = c0 + lr;
   const bool a_ok = arow < P.M, b_ok = brow < P.N;
   const float* Ap = nullptr;
   if (a_ok) Ap = P.A + (P.a_rows ? int64_t(P.a_rows[arow]) : arow) * P.lda;
   const unsigned short* Bhp = b_ok ? P.Bhi + brow * P.ldb : nullptr;
   const unsigned short* Blp = b_ok ? P.Blo + brow * P.ldb : nullptr;
+  // destination fragments of this thread: chunk kh, row block lr/32, lanes
+  // lr%32 (k 0..7 of the chunk) and lr%32 + 32 (k 8..15)
+  const int frag0 = ((lr >> 5) * (BK / 16) + kh) * 64 + (lr & 31);
 
   float4 ra[4];
   uint4 rbh[2], rbl[2];
@@ -146,7 +157,7 @@ __global__ __launch_bounds__(NT, 2) void bf16x3_gemm(ProblemSet ps) {
       memcpy(rbl, tl, 32);
     }
   };
-  auto lstore = [&]() {
+  auto lstore = [&](int st) {
     // x = hi + lo, hi = bf16(x), lo = bf16(x - hi), both round-to-nearest-even
     // (hardware conversions on gfx950)
     bf16x8 hi[2], lo[2];
@@ -157,18 +168,14 @@ __global__ __launch_bounds__(NT, 2) void bf16x3_gemm(ProblemSet ps) {
       hi[q >> 3][q & 7] = h;
       lo[q >> 3][q & 7] = (__bf16)(f[q] - (float)h);
     }
-    bf16x8* dah = reinterpret_cast<bf16x8*>(sAh + lr * LDK + lk);
-    bf16x8* dal = reinterpret_cast<bf16x8*>(sAl + lr * LDK + lk);
-    dah[0] = hi[0];
-    dah[1] = hi[1];
-    dal[0] = lo[0];
-    dal[1] = lo[1];
-    uint4* dbh = reinterpret_cast<uint4*>(sBh + lr * LDK + lk);
-    uint4* dbl = reinterpret_cast<uint4*>(sBl + lr * LDK + lk);
-    dbh[0] = rbh[0];
-    dbh[1] = rbh[1];
-    dbl[0] = rbl[0];
-    dbl[1] = rbl[1];
+    sAh[st][frag0] = hi[0];
+    sAh[st][frag0 + 32] = hi[1];
+    sAl[st][frag0] = lo[0];
+    sAl[st][frag0 + 32] = lo[1];
+    sBh[st][frag0] = *reinterpret_cast<const bf16x8*>(&rbh[0]);
+    sBh[st][frag0 + 32] = *reinterpret_cast<const bf16x8*>(&rbh[1]);
+    sBl[st][frag0] = *reinterpret_cast<const bf16x8*>(&rbl[0]);
+    sBl[st][frag0 + 32] = *reinterpret_cast<const bf16x8*>(&rbl[1]);
   };
 
   f32x16 acc[2][2];
@@ -179,27 +186,27 @@ __global__ __launch_bounds__(NT, 2) void bf16x3_gemm(ProblemSet ps) {
 #pragma unroll
       for (int e = 0; e < 16; ++e) acc[i][j][e] = 0.f;
 
-  const int fr = lane & 31, fk = (lane >> 5) * 8;
+  const int64_t nk = (P.K + BK - 1) / BK;
   gload(0);
-  for (int64_t k0 = 0; k0 < P.K; k0 += BK) {
-    __syncthreads();
-    lstore();
-    __syncthreads();
-    if (k0 + BK < P.K) gload(k0 + BK);
+  lstore(0);
+  __syncthreads();
+  for (int64_t kt = 0; kt < nk; ++kt) {
+    const int st = int(kt & 1);
+    if (kt + 1 < nk) gload((kt + 1) * BK);
 #pragma unroll
-    for (int ks = 0; ks < BK; ks += 16) {
+    for (int c = 0; c < BK / 16; ++c) {
       bf16x8 ah[2], al[2], bh[2], bl[2];
 #pragma unroll
       for (int i = 0; i < 2; ++i) {
-        const int row = wm * 64 + i * 32 + fr;
-        ah[i] = *reinterpret_cast<const bf16x8*>(sAh + row * LDK + ks + fk);
-        al[i] = *reinterpret_cast<const bf16x8*>(sAl + row * LDK + ks + fk);
+        const int f = ((wm * 2 + i) * (BK / 16) + c) * 64 + lane;
+        ah[i] = sAh[st][f];
+        al[i] = sAl[st][f];
       }
 #pragma unroll
       for (int j = 0; j < 2; ++j) {
-        const int col = wn * 64 + j * 32 + fr;
-        bh[j] = *reinterpret_cast<const bf16x8*>(sBh + col * LDK + ks + fk);
-        bl[j] = *reinterpret_cast<const bf16x8*>(sBl + col * LDK + ks + fk);
+        const int f = ((wn * 2 + j) * (BK / 16) + c) * 64 + lane;
+        bh[j] = sBh[st][f];
+        bl[j] = sBl[st][f];
       }
 #pragma unroll
       for (int i = 0; i < 2; ++i)
@@ -210,11 +217,14 @@ __global__ __launch_bounds__(NT, 2) void bf16x3_gemm(ProblemSet ps) {
           acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
         }
     }
+    // the other stage was last read before the previous barrier
+    if (kt + 1 < nk) lstore(st ^ 1);
+    __syncthreads();
   }
 
   // epilogue; element e = 4*g + q of a 32x32 tile sits at
   // row 8*g + 4*(lane/32) + q, column lane%32
-  const int half = lane >> 5;
+  const int half = lane >> 5, fr = lane & 31;
   if (EPI == 0) {
     double rowsum[2][16];
 #pragma unroll
