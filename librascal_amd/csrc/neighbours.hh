@@ -41,7 +41,9 @@ struct PairList {
   DeviceBuffer<double> pair_vec;     // [n_pairs][3]  r_j(image) - r_i
   DeviceBuffer<double> pair_dist;    // [n_pairs]
   DeviceBuffer<int> err;             // [1]
-  DeviceBuffer<int64_t> scratch;
+  DeviceBuffer<int64_t> scratch, scratch2, scratch3;  // scan tile sums (one per scan)
+  DeviceBuffer<int64_t> totals;      // [3] ghosts, pairs, error flag
+  int64_t cap_ext = 0, cap_pairs = 0;  // allocated sizes of the extended set / pair list
 };
 
 // Host inputs: positions [n_atoms][3], cells [n_struct][9] (lattice vectors

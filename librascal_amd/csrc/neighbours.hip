@@ -46,7 +46,8 @@ template <bool WRITE>
 __global__ void ghosts_kernel(const NLGeom* __restrict__ geom, const int* __restrict__ atom_struct,
                               const double* __restrict__ pos, int64_t n_atoms,
                               int* __restrict__ count, const int64_t* __restrict__ off,
-                              double* __restrict__ ext_pos, int* __restrict__ ext_orig) {
+                              double* __restrict__ ext_pos, int* __restrict__ ext_orig,
+                              int64_t cap) {
   int64_t a = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (a >= n_atoms) return;
   const NLGeom& g = geom[atom_struct[a]];
@@ -72,7 +73,7 @@ __global__ void ghosts_kernel(const NLGeom* __restrict__ geom, const int* __rest
       if (!((xg[d] - g.ghost_min[d]) > g.tol && (xg[d] - g.ghost_max[d]) < -g.tol)) inside = false;
     }
     if (!inside) continue;
-    if (WRITE) {
+    if (WRITE && w + c < cap) {
       ext_pos[3 * (w + c)] = xg[0];
       ext_pos[3 * (w + c) + 1] = xg[1];
       ext_pos[3 * (w + c) + 2] = xg[2];
@@ -97,10 +98,12 @@ __global__ void originals_kernel(const double* __restrict__ pos, int64_t n_atoms
 // :395-422 range check)
 __global__ void bin_count_kernel(const NLGeom* __restrict__ geom, const int* __restrict__ atom_struct,
                                  const double* __restrict__ ext_pos, const int* __restrict__ ext_orig,
-                                 int64_t n_ext, double rc, int64_t* __restrict__ ext_box,
+                                 int64_t n_atoms, const int64_t* __restrict__ n_ghosts, int64_t cap,
+                                 double rc, int64_t* __restrict__ ext_box,
                                  int* __restrict__ box_count, int* __restrict__ err) {
   int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (t >= n_ext) return;
+  const int64_t n_ext = n_atoms + *n_ghosts;
+  if (t >= n_ext || t >= cap) return;
   const NLGeom& g = geom[atom_struct[ext_orig[t]]];
   int b[3];
   box_of(g, ext_pos + 3 * t, rc, b);
@@ -115,11 +118,13 @@ __global__ void bin_count_kernel(const NLGeom* __restrict__ geom, const int* __r
   atomicAdd(box_count + bl, 1);
 }
 
-__global__ void bin_fill_kernel(const int64_t* __restrict__ ext_box, int64_t n_ext,
+__global__ void bin_fill_kernel(const int64_t* __restrict__ ext_box, int64_t n_atoms,
+                                const int64_t* __restrict__ n_ghosts, int64_t cap,
                                 const int64_t* __restrict__ box_start, int* __restrict__ box_fill,
                                 int* __restrict__ box_items) {
   int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (t >= n_ext) return;
+  const int64_t n_ext = n_atoms + *n_ghosts;
+  if (t >= n_ext || t >= cap) return;
   int64_t bl = ext_box[t];
   if (bl < 0) return;
   int k = atomicAdd(box_fill + bl, 1);
@@ -153,7 +158,8 @@ __global__ __launch_bounds__(256) void pairs_kernel(
     const double* __restrict__ ext_pos, const int* __restrict__ ext_orig,
     const int64_t* __restrict__ box_start, const int* __restrict__ box_items, int64_t n_atoms,
     double rc, int* __restrict__ pair_count, const int64_t* __restrict__ centre_off,
-    int* __restrict__ pair_j, double* __restrict__ pair_vec, double* __restrict__ pair_dist) {
+    int* __restrict__ pair_j, double* __restrict__ pair_vec, double* __restrict__ pair_dist,
+    int64_t cap) {
   const int lane = threadIdx.x & 63;
   const int64_t i = int64_t(blockIdx.x) * (blockDim.x / 64) + (threadIdx.x >> 6);
   if (i >= n_atoms) return;
@@ -164,7 +170,7 @@ __global__ __launch_bounds__(256) void pairs_kernel(
   box_of(g, xi, rc, b);
   const double rc2 = rc * rc;
   int64_t w = WRITE ? centre_off[i] : 0;
-  if (WRITE && lane == 0) {
+  if (WRITE && lane == 0 && w < cap) {
     pair_j[w] = int(i);
     pair_vec[3 * w] = 0.0;
     pair_vec[3 * w + 1] = 0.0;
@@ -203,16 +209,28 @@ __global__ __launch_bounds__(256) void pairs_kernel(
       if (WRITE && keep) {
         unsigned long long below = mask & ((1ull << lane) - 1ull);
         int64_t o = w + n + __popcll(below);
-        pair_j[o] = ext_orig[t];
-        pair_vec[3 * o] = v0;
-        pair_vec[3 * o + 1] = v1;
-        pair_vec[3 * o + 2] = v2;
-        pair_dist[o] = sqrt(d2);
+        if (o < cap) {
+          pair_j[o] = ext_orig[t];
+          pair_vec[3 * o] = v0;
+          pair_vec[3 * o + 1] = v1;
+          pair_vec[3 * o + 2] = v2;
+          pair_dist[o] = sqrt(d2);
+        }
       }
       n += __popcll(mask);
     }
   }
   if (!WRITE && lane == 0) pair_count[i] = int(n);
+}
+
+__global__ void nl_totals_kernel(const int64_t* __restrict__ n_ghosts,
+                                 const int64_t* __restrict__ n_pairs, const int* __restrict__ err,
+                                 int64_t* __restrict__ out) {
+  if (threadIdx.x == 0) {
+    out[0] = *n_ghosts;
+    out[1] = *n_pairs;
+    out[2] = *err;
+  }
 }
 
 }  // namespace
@@ -327,66 +345,89 @@ void build_pair_list(PairList& pl, int n_struct, const int64_t* atom_offsets,
   double* dpos = pl.pos.reserve(3 * n_atoms);
   RGPU_HIP(hipMemcpyAsync(dpos, positions, sizeof(double) * 3 * n_atoms, hipMemcpyHostToDevice,
                           stream));
-  int* err = pl.err.reserve(1);
-  RGPU_HIP(hipMemsetAsync(err, 0, sizeof(int), stream));
 
+  // Sizes of the extended (ghost) set and of the pair list are only known on
+  // the device; the buffers are sized by capacities kept from earlier builds,
+  // every write beyond a capacity is dropped, and one synchronisation at the
+  // end reads the true sizes.  When a capacity was short, the build is run
+  // again with enough room (the first build of a batch, or a large change).
+  if (pl.cap_ext < n_atoms) pl.cap_ext = 2 * n_atoms + 1024;
+  if (pl.cap_pairs < n_atoms) pl.cap_pairs = 48 * n_atoms + 1024;
   const int TB = 256;
   auto blocks = [](int64_t n, int tb) { return dim3(unsigned((n + tb - 1) / tb)); };
-  // ghosts
-  int* gcount = pl.ghost_count.reserve(n_atoms);
-  int64_t* goff = pl.ghost_off.reserve(n_atoms + 1);
-  hipLaunchKernelGGL(ghosts_kernel<false>, blocks(n_atoms, TB), dim3(TB), 0, stream, dgeom, dstruct,
-                     dpos, n_atoms, gcount, nullptr, nullptr, nullptr);
-  RGPU_LAUNCH_CHECK();
-  int64_t n_ghosts = exclusive_scan<int>(gcount, goff, n_atoms, stream, pl.scratch);
-  pl.n_ext = n_atoms + n_ghosts;
-  double* epos = pl.ext_pos.reserve(3 * pl.n_ext);
-  int* eorig = pl.ext_orig.reserve(pl.n_ext);
-  hipLaunchKernelGGL(originals_kernel, blocks(n_atoms, TB), dim3(TB), 0, stream, dpos, n_atoms, epos,
-                     eorig);
-  RGPU_LAUNCH_CHECK();
-  hipLaunchKernelGGL(ghosts_kernel<true>, blocks(n_atoms, TB), dim3(TB), 0, stream, dgeom, dstruct,
-                     dpos, n_atoms, nullptr, goff, epos, eorig);
-  RGPU_LAUNCH_CHECK();
-  // binning
-  int* bcount = pl.box_count.reserve(boxes);
-  int64_t* bstart = pl.box_start.reserve(boxes + 1);
-  int* bfill = pl.box_fill.reserve(boxes);
-  int* bitems = pl.box_items.reserve(pl.n_ext);
-  int64_t* ebox = pl.ext_box.reserve(pl.n_ext);
-  RGPU_HIP(hipMemsetAsync(bcount, 0, sizeof(int) * boxes, stream));
-  RGPU_HIP(hipMemsetAsync(bfill, 0, sizeof(int) * boxes, stream));
-  hipLaunchKernelGGL(bin_count_kernel, blocks(pl.n_ext, TB), dim3(TB), 0, stream, dgeom, dstruct, epos,
-                     eorig, pl.n_ext, rc, ebox, bcount, err);
-  RGPU_LAUNCH_CHECK();
-  int herr = 0;
-  RGPU_HIP(hipMemcpyAsync(&herr, err, sizeof(int), hipMemcpyDeviceToHost, stream));
-  RGPU_HIP(hipStreamSynchronize(stream));
-  if (herr)
-    throw RuntimeError(
-        "Error: this atom does not fall in one of the linked cell bin. It might have not been "
-        "wrapped properly.");
-  exclusive_scan<int>(bcount, bstart, boxes, stream, pl.scratch);
-  hipLaunchKernelGGL(bin_fill_kernel, blocks(pl.n_ext, TB), dim3(TB), 0, stream, ebox, pl.n_ext, bstart,
-                     bfill, bitems);
-  RGPU_LAUNCH_CHECK();
-  hipLaunchKernelGGL(bin_sort_kernel, blocks(boxes, TB), dim3(TB), 0, stream, bstart, boxes, bitems);
-  RGPU_LAUNCH_CHECK();
-  // pairs
-  int* pcount = pl.pair_count.reserve(n_atoms);
-  int64_t* coff = pl.centre_off.reserve(n_atoms + 1);
-  const int WPB = TB / 64;
-  dim3 pgrid(unsigned((n_atoms + WPB - 1) / WPB));
-  hipLaunchKernelGGL(pairs_kernel<false>, pgrid, dim3(TB), 0, stream, dgeom, dstruct, epos, eorig,
-                     bstart, bitems, n_atoms, rc, pcount, nullptr, nullptr, nullptr, nullptr);
-  RGPU_LAUNCH_CHECK();
-  pl.n_pairs = exclusive_scan<int>(pcount, coff, n_atoms, stream, pl.scratch);
-  int* pj = pl.pair_j.reserve(pl.n_pairs);
-  double* pv = pl.pair_vec.reserve(3 * pl.n_pairs);
-  double* pd = pl.pair_dist.reserve(pl.n_pairs);
-  hipLaunchKernelGGL(pairs_kernel<true>, pgrid, dim3(TB), 0, stream, dgeom, dstruct, epos, eorig, bstart,
-                     bitems, n_atoms, rc, nullptr, coff, pj, pv, pd);
-  RGPU_LAUNCH_CHECK();
+  for (int pass = 0; pass < 2; ++pass) {
+    const int64_t cap_ext = pl.cap_ext, cap_pairs = pl.cap_pairs;
+    int* err = pl.err.reserve(1);
+    RGPU_HIP(hipMemsetAsync(err, 0, sizeof(int), stream));
+    // ghosts
+    int* gcount = pl.ghost_count.reserve(n_atoms);
+    int64_t* goff = pl.ghost_off.reserve(n_atoms + 1);
+    hipLaunchKernelGGL(ghosts_kernel<false>, blocks(n_atoms, TB), dim3(TB), 0, stream, dgeom, dstruct,
+                       dpos, n_atoms, gcount, nullptr, nullptr, nullptr, int64_t(0));
+    RGPU_LAUNCH_CHECK();
+    exclusive_scan_device<int>(gcount, goff, n_atoms, stream, pl.scratch);
+    double* epos = pl.ext_pos.reserve(3 * cap_ext);
+    int* eorig = pl.ext_orig.reserve(cap_ext);
+    hipLaunchKernelGGL(originals_kernel, blocks(n_atoms, TB), dim3(TB), 0, stream, dpos, n_atoms, epos,
+                       eorig);
+    RGPU_LAUNCH_CHECK();
+    hipLaunchKernelGGL(ghosts_kernel<true>, blocks(n_atoms, TB), dim3(TB), 0, stream, dgeom, dstruct,
+                       dpos, n_atoms, nullptr, goff, epos, eorig, cap_ext);
+    RGPU_LAUNCH_CHECK();
+    // binning
+    int* bcount = pl.box_count.reserve(boxes);
+    int64_t* bstart = pl.box_start.reserve(boxes + 1);
+    int* bfill = pl.box_fill.reserve(boxes);
+    int* bitems = pl.box_items.reserve(cap_ext);
+    int64_t* ebox = pl.ext_box.reserve(cap_ext);
+    RGPU_HIP(hipMemsetAsync(bcount, 0, sizeof(int) * boxes, stream));
+    RGPU_HIP(hipMemsetAsync(bfill, 0, sizeof(int) * boxes, stream));
+    hipLaunchKernelGGL(bin_count_kernel, blocks(cap_ext, TB), dim3(TB), 0, stream, dgeom, dstruct, epos,
+                       eorig, n_atoms, goff + n_atoms, cap_ext, rc, ebox, bcount, err);
+    RGPU_LAUNCH_CHECK();
+    exclusive_scan_device<int>(bcount, bstart, boxes, stream, pl.scratch2);
+    hipLaunchKernelGGL(bin_fill_kernel, blocks(cap_ext, TB), dim3(TB), 0, stream, ebox, n_atoms,
+                       goff + n_atoms, cap_ext, bstart, bfill, bitems);
+    RGPU_LAUNCH_CHECK();
+    hipLaunchKernelGGL(bin_sort_kernel, blocks(boxes, TB), dim3(TB), 0, stream, bstart, boxes, bitems);
+    RGPU_LAUNCH_CHECK();
+    // pairs
+    int* pcount = pl.pair_count.reserve(n_atoms);
+    int64_t* coff = pl.centre_off.reserve(n_atoms + 1);
+    const int WPB = TB / 64;
+    dim3 pgrid(unsigned((n_atoms + WPB - 1) / WPB));
+    hipLaunchKernelGGL(pairs_kernel<false>, pgrid, dim3(TB), 0, stream, dgeom, dstruct, epos, eorig,
+                       bstart, bitems, n_atoms, rc, pcount, nullptr, nullptr, nullptr, nullptr,
+                       int64_t(0));
+    RGPU_LAUNCH_CHECK();
+    exclusive_scan_device<int>(pcount, coff, n_atoms, stream, pl.scratch3);
+    int* pj = pl.pair_j.reserve(cap_pairs);
+    double* pv = pl.pair_vec.reserve(3 * cap_pairs);
+    double* pd = pl.pair_dist.reserve(cap_pairs);
+    hipLaunchKernelGGL(pairs_kernel<true>, pgrid, dim3(TB), 0, stream, dgeom, dstruct, epos, eorig,
+                       bstart, bitems, n_atoms, rc, nullptr, coff, pj, pv, pd, cap_pairs);
+    RGPU_LAUNCH_CHECK();
+    int64_t* dtot = pl.totals.reserve(3);
+    hipLaunchKernelGGL(nl_totals_kernel, dim3(1), dim3(64), 0, stream, goff + n_atoms, coff + n_atoms,
+                       err, dtot);
+    RGPU_LAUNCH_CHECK();
+    int64_t tot[3];
+    RGPU_HIP(hipMemcpyAsync(tot, dtot, sizeof(tot), hipMemcpyDeviceToHost, stream));
+    RGPU_HIP(hipStreamSynchronize(stream));
+    const int64_t n_ext = n_atoms + tot[0], n_pairs = tot[1];
+    if (n_ext <= cap_ext && n_pairs <= cap_pairs) {
+      if (tot[2])
+        throw RuntimeError(
+            "Error: this atom does not fall in one of the linked cell bin. It might have not been "
+            "wrapped properly.");
+      pl.n_ext = n_ext;
+      pl.n_pairs = n_pairs;
+      return;
+    }
+    pl.cap_ext = std::max(cap_ext, n_ext + n_ext / 8 + 64);
+    pl.cap_pairs = std::max(cap_pairs, n_pairs + n_pairs / 8 + 64);
+  }
+  throw RuntimeError("neighbour list buffers could not be sized");
 }
 
 }  // namespace rgpu

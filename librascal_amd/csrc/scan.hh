@@ -5,7 +5,6 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
-#include <vector>
 
 #include "device_buffer.hh"
 
@@ -55,48 +54,59 @@ __global__ static void scan_add_kernel(int64_t* __restrict__ out, const int64_t*
   if (idx < n) out[idx] += tile_off[idx / kScanTile];
 }
 
-// out[i] = sum_{k<i} in[k] for i < n, and out[n] = total (out has n+1
-// elements).  Returns the total (synchronises the stream).
+// Exclusive scan of per-tile sums by one workgroup, chunk by chunk with a
+// running carry; out_excl[i] = sum_{k<i} in[k], *total = sum of all.
+__global__ static __launch_bounds__(kScanThreads) void scan_small_kernel(
+    const int64_t* __restrict__ in, int64_t* __restrict__ out_excl, int64_t n,
+    int64_t* __restrict__ total) {
+  __shared__ int64_t wsum[kScanThreads / 64];
+  __shared__ int64_t carry_sh;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (threadIdx.x == 0) carry_sh = 0;
+  __syncthreads();
+  for (int64_t c0 = 0; c0 < n; c0 += kScanThreads) {
+    const int64_t idx = c0 + threadIdx.x;
+    const int64_t v = idx < n ? in[idx] : 0;
+    int64_t incl = v;
+    for (int off = 1; off < 64; off <<= 1) {
+      const int64_t o = __shfl_up(incl, off, 64);
+      if (lane >= off) incl += o;
+    }
+    if (lane == 63) wsum[wave] = incl;
+    __syncthreads();
+    int64_t wbase = carry_sh;
+    for (int w = 0; w < wave; ++w) wbase += wsum[w];
+    if (idx < n) out_excl[idx] = wbase + incl - v;
+    __syncthreads();
+    if (threadIdx.x == kScanThreads - 1) carry_sh = wbase + incl;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *total = carry_sh;
+}
+
+// out[i] = sum_{k<i} in[k] for i < n and out[n] = total, all on the device
+// (no synchronisation); out has n+1 elements.
 template <class In>
-int64_t exclusive_scan(const In* in, int64_t* out, int64_t n, hipStream_t stream,
-                       DeviceBuffer<int64_t>& scratch) {
+void exclusive_scan_device(const In* in, int64_t* out, int64_t n, hipStream_t stream,
+                           DeviceBuffer<int64_t>& scratch) {
   if (n == 0) {
     RGPU_HIP(hipMemsetAsync(out, 0, sizeof(int64_t), stream));
-    RGPU_HIP(hipStreamSynchronize(stream));
-    return 0;
+    return;
   }
-  int64_t tiles = (n + kScanTile - 1) / kScanTile;
-  // level storage: tile sums and their scan
-  std::vector<int64_t> level_sizes;
-  int64_t t = tiles;
-  size_t need = 0;
-  while (true) {
-    level_sizes.push_back(t);
-    need += size_t(2 * t + 1);
-    if (t <= 1) break;
-    t = (t + kScanTile - 1) / kScanTile;
-  }
-  int64_t* buf = scratch.reserve(need);
-  int64_t* sums = buf;
-  int64_t* sums_scan = buf + tiles;
+  const int64_t tiles = (n + kScanTile - 1) / kScanTile;
+  int64_t* sums = scratch.reserve(size_t(2 * tiles));
+  int64_t* sums_scan = sums + tiles;
   hipLaunchKernelGGL(scan_tiles_kernel<In>, dim3(unsigned(tiles)), dim3(kScanThreads), 0, stream, in,
                      out, sums, n);
   RGPU_LAUNCH_CHECK();
-  int64_t total;
+  hipLaunchKernelGGL(scan_small_kernel, dim3(1), dim3(kScanThreads), 0, stream, sums, sums_scan, tiles,
+                     out + n);
+  RGPU_LAUNCH_CHECK();
   if (tiles > 1) {
-    DeviceBuffer<int64_t> inner;
-    total = exclusive_scan<int64_t>(sums, sums_scan, tiles, stream, inner);
-    int64_t blocks = (n + 255) / 256;
-    hipLaunchKernelGGL(scan_add_kernel, dim3(unsigned(blocks)), dim3(256), 0, stream, out, sums_scan,
-                       n);
+    hipLaunchKernelGGL(scan_add_kernel, dim3(unsigned((n + 255) / 256)), dim3(256), 0, stream, out,
+                       sums_scan, n);
     RGPU_LAUNCH_CHECK();
-  } else {
-    RGPU_HIP(hipMemcpyAsync(&total, sums, sizeof(int64_t), hipMemcpyDeviceToHost, stream));
-    RGPU_HIP(hipStreamSynchronize(stream));
   }
-  RGPU_HIP(hipMemcpyAsync(out + n, &total, sizeof(int64_t), hipMemcpyHostToDevice, stream));
-  RGPU_HIP(hipStreamSynchronize(stream));
-  return total;
 }
 
 }  // namespace rgpu
