@@ -51,6 +51,8 @@ extern "C" {
 #define RGPU_INFO_N_SPARSE 8
 #define RGPU_INFO_IS_INVARIANTS 9 /* 1 PowerSpectrum, 0 SphericalExpansion */
 #define RGPU_INFO_PRECISION 10
+#define RGPU_INFO_GRAPH_CAPTURES 11 /* graphs captured by rgpu_compute */
+#define RGPU_INFO_GRAPH_LAUNCHES 12 /* graph replays by rgpu_compute */
 
 /* stages timed by rgpu_enable_timing */
 #define RGPU_STAGE_NEIGHBOURS 0
@@ -76,6 +78,12 @@ void rgpu_context_destroy(rgpu_context* ctx);
 
 /* HIP stream used by every later call (default: the null stream). */
 int rgpu_set_stream(rgpu_context* ctx, void* hip_stream);
+/* Replay the representation stage (cutoff, expansion, invariants) of
+ * rgpu_compute as a captured HIP graph while the batch's buffers and sizes
+ * are unchanged (positions may change).  Creates a private stream when no
+ * stream was set.  Not used while stage timing is on.  No counterpart in
+ * the reference (a CPU library). */
+int rgpu_enable_graphs(rgpu_context* ctx, int enable);
 /* RGPU_FP64 (default) or RGPU_MIXED; invalidates computed results. */
 int rgpu_set_precision(rgpu_context* ctx, int precision);
 int rgpu_get_info(rgpu_context* ctx, int key, int64_t* value);

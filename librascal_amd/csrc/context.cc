@@ -84,6 +84,18 @@ struct Context {
   DeviceBuffer<float> wt32, G32;
   DeviceBuffer<int64_t> knm_perm;
 
+  // HIP graph of the representation stage (cutoff, expansion, invariants)
+  bool use_graphs = false;
+  hipStream_t own_stream = nullptr;  // created for capture when no stream was given
+  hipGraphExec_t rep_exec = nullptr;
+  std::vector<uintptr_t> rep_sig;
+  int64_t graph_captures = 0, graph_launches = 0;
+  void drop_graph() {
+    if (rep_exec) (void)hipGraphExecDestroy(rep_exec);
+    rep_exec = nullptr;
+    rep_sig.clear();
+  }
+
   // stage timers (HIP events on `stream`)
   bool timing = false;
   struct Pending {
@@ -119,11 +131,14 @@ struct Context {
     pending.clear();
   }
   ~Context() {
+    if (stream) (void)hipStreamSynchronize(stream);
+    drop_graph();
     for (auto& p : pending) {
       (void)hipEventDestroy(p.a);
       (void)hipEventDestroy(p.b);
     }
     for (auto e : ev_pool) (void)hipEventDestroy(e);
+    if (own_stream) (void)hipStreamDestroy(own_stream);
   }
 
   int64_t D() const {
@@ -332,22 +347,62 @@ void compute_rep(Context& c) {
   const int S = int(c.species.size());
   const int64_t n = c.n_atoms;
   const int64_t E = int64_t(S) * c.rd.N * c.rd.LL;
+  const int64_t D = c.D();
+  const bool ps = c.hp.kind == SoapHypers::kPowerSpectrum;
+  const int64_t cap = n > 0 ? c.pl.cap_pairs : 0;
   T* cc = cbuf<T>(c).reserve(size_t(std::max<int64_t>(n * E, 1)));
   unsigned long long* km = c.keymask.reserve(size_t(std::max<int64_t>(n, 1)));
-  {
-    StageTimer t(c, RGPU_STAGE_EXPANSION);
-    const int64_t np = c.pl.n_pairs;
-    double* fc = c.pair_fc.reserve(size_t(std::max<int64_t>(np, 1)));
-    double* dfc = c.pair_dfc.reserve(size_t(std::max<int64_t>(np, 1)));
-    launch_pair_cutoff(c.rd, np, c.pl.pair_dist.get(), fc, dfc, c.stream);
+  double* fc = c.pair_fc.reserve(size_t(std::max<int64_t>(cap, 1)));
+  double* dfc = c.pair_dfc.reserve(size_t(std::max<int64_t>(cap, 1)));
+  T* pp = ps ? pbuf<T>(c).reserve(size_t(std::max<int64_t>(n * D, 1))) : nullptr;
+  double* n2 = ps ? c.norm2.reserve(size_t(std::max<int64_t>(n, 1))) : nullptr;
+  const int64_t* n_pairs_dev = c.pl.centre_off.get() + n;
+  auto cutoff_and_expansion = [&] {
+    launch_pair_cutoff(c.rd, n_pairs_dev, cap, c.pl.pair_dist.get(), fc, dfc, c.stream);
     launch_expansion<T>(c.rd, S, pair_view(c), cc, km, c.stream);
-  }
-  if (c.hp.kind == SoapHypers::kPowerSpectrum) {
-    StageTimer t(c, RGPU_STAGE_POWER_SPECTRUM);
-    const int64_t D = c.D();
-    T* pp = pbuf<T>(c).reserve(size_t(std::max<int64_t>(n * D, 1)));
-    double* n2 = c.norm2.reserve(size_t(std::max<int64_t>(n, 1)));
-    launch_power_spectrum<T>(c.rd, S, n, cc, pp, n2, c.stream);
+  };
+  auto invariants = [&] {
+    if (ps) launch_power_spectrum<T>(c.rd, S, n, cc, pp, n2, c.stream);
+  };
+  if (c.use_graphs && !c.timing && n > 0) {
+    // replay a captured graph of the stage while every buffer, size and
+    // precision it was captured with is unchanged
+    const std::vector<uintptr_t> sig = {
+        uintptr_t(c.precision), uintptr_t(S), uintptr_t(n), uintptr_t(cap), uintptr_t(cc),
+        uintptr_t(km), uintptr_t(fc), uintptr_t(dfc), uintptr_t(pp), uintptr_t(n2),
+        uintptr_t(c.pl.centre_off.get()), uintptr_t(c.pl.pair_j.get()),
+        uintptr_t(c.pl.pair_vec.get()), uintptr_t(c.pl.pair_dist.get()),
+        uintptr_t(c.d_species_of.get())};
+    if (!c.rep_exec || sig != c.rep_sig) {
+      c.drop_graph();
+      hipGraph_t g = nullptr;
+      RGPU_HIP(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
+      try {
+        cutoff_and_expansion();
+        invariants();
+      } catch (...) {
+        (void)hipStreamEndCapture(c.stream, &g);
+        if (g) (void)hipGraphDestroy(g);
+        throw;
+      }
+      RGPU_HIP(hipStreamEndCapture(c.stream, &g));
+      const hipError_t e = hipGraphInstantiate(&c.rep_exec, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      RGPU_HIP(e);
+      c.rep_sig = sig;
+      ++c.graph_captures;
+    }
+    RGPU_HIP(hipGraphLaunch(c.rep_exec, c.stream));
+    ++c.graph_launches;
+  } else {
+    {
+      StageTimer t(c, RGPU_STAGE_EXPANSION);
+      cutoff_and_expansion();
+    }
+    {
+      StageTimer t(c, RGPU_STAGE_POWER_SPECTRUM);
+      invariants();
+    }
   }
   c.have_rep = true;
 }
@@ -637,6 +692,24 @@ int rgpu_set_stream(rgpu_context* ctx, void* s) {
   });
 }
 
+int rgpu_enable_graphs(rgpu_context* ctx, int enable) {
+  return guarded([&] {
+    if (!ctx) throw InvalidArgument("null context");
+    Context& c = ctx->c;
+    c.use_graphs = enable != 0;
+    if (!c.use_graphs) {
+      c.drop_graph();
+      return;
+    }
+    if (!c.stream) {  // capture needs a stream other than the null stream
+      hipStream_t s = nullptr;
+      RGPU_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+      c.own_stream = s;
+      c.stream = s;
+    }
+  });
+}
+
 int rgpu_set_precision(rgpu_context* ctx, int precision) {
   return guarded([&] {
     if (!ctx) throw InvalidArgument("null context");
@@ -665,6 +738,8 @@ int rgpu_get_info(rgpu_context* ctx, int key, int64_t* value) {
       case RGPU_INFO_N_SPARSE: *value = c.M; break;
       case RGPU_INFO_IS_INVARIANTS: *value = c.hp.kind == SoapHypers::kPowerSpectrum; break;
       case RGPU_INFO_PRECISION: *value = c.precision; break;
+      case RGPU_INFO_GRAPH_CAPTURES: *value = c.graph_captures; break;
+      case RGPU_INFO_GRAPH_LAUNCHES: *value = c.graph_launches; break;
       default: throw InvalidArgument("unknown info key");
     }
   });

@@ -22,9 +22,10 @@ struct PairView {
   const double* pair_dfc;     // [n_pairs] its derivative
 };
 
-// pair_fc / pair_dfc of all pairs (cutoff_functions.hh:69-283)
-void launch_pair_cutoff(const RadialDev& rad, int64_t n_pairs, const double* dist, double* fc,
-                        double* dfc, hipStream_t stream);
+// pair_fc / pair_dfc of the first *n_pairs (device value) of cap pairs
+// (cutoff_functions.hh:69-283)
+void launch_pair_cutoff(const RadialDev& rad, const int64_t* n_pairs, int64_t cap, const double* dist,
+                        double* fc, double* dfc, hipStream_t stream);
 
 // c[i][s][n][l*l+l+m] (finalised) and the per-centre key mask.
 template <class T>

@@ -846,10 +846,13 @@ __global__ __launch_bounds__(kThreads) void adjoint_kernel(
   }
 }
 
-__global__ void pair_cutoff_kernel(RadialDev rad, int64_t n_pairs, const double* __restrict__ dist,
-                                   double* __restrict__ fc, double* __restrict__ dfc) {
+// n_pairs is read on the device (the last entry of the pair offsets), so one
+// launch shape (the pair capacity) serves every neighbour-list rebuild
+__global__ void pair_cutoff_kernel(RadialDev rad, const int64_t* __restrict__ n_pairs, int64_t cap,
+                                   const double* __restrict__ dist, double* __restrict__ fc,
+                                   double* __restrict__ dfc) {
   const int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (p >= n_pairs) return;
+  if (p >= cap || p >= *n_pairs) return;
   double f, df;
   cutoff_eval(rad, dist[p], &f, &df);
   fc[p] = f;
@@ -954,11 +957,19 @@ constexpr size_t kFwdBudget = 32 * 1024;
 constexpr size_t kPsBudget = 48 * 1024;
 constexpr size_t kAdjBudget = 64 * 1024;
 
+// LDS limit per workgroup of the current device (queried once per device;
+// launches may be captured into graphs, where device queries are avoided)
 size_t device_lds_limit() {
-  int dev = 0, v = 0;
+  static int cached[64] = {0};
+  int dev = 0;
   RGPU_HIP(hipGetDevice(&dev));
-  RGPU_HIP(hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
-  return size_t(v);
+  if (dev < 0 || dev >= 64) throw RuntimeError("device index out of range");
+  if (!cached[dev]) {
+    int v = 0;
+    RGPU_HIP(hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+    cached[dev] = v;
+  }
+  return size_t(cached[dev]);
 }
 
 template <class K>
@@ -1087,11 +1098,11 @@ void launch_backward(const RadialDev& rad, int S, const PairView& pv, const T* c
                               stream);
 }
 
-void launch_pair_cutoff(const RadialDev& rad, int64_t n_pairs, const double* dist, double* fc,
-                        double* dfc, hipStream_t stream) {
-  if (n_pairs == 0) return;
-  hipLaunchKernelGGL(pair_cutoff_kernel, dim3(unsigned((n_pairs + 255) / 256)), dim3(256), 0, stream,
-                     rad, n_pairs, dist, fc, dfc);
+void launch_pair_cutoff(const RadialDev& rad, const int64_t* n_pairs, int64_t cap, const double* dist,
+                        double* fc, double* dfc, hipStream_t stream) {
+  if (cap == 0) return;
+  hipLaunchKernelGGL(pair_cutoff_kernel, dim3(unsigned((cap + 255) / 256)), dim3(256), 0, stream, rad,
+                     n_pairs, cap, dist, fc, dfc);
   RGPU_LAUNCH_CHECK();
 }
 
