@@ -53,6 +53,9 @@ extern "C" {
 #define RGPU_INFO_PRECISION 10
 #define RGPU_INFO_GRAPH_CAPTURES 11 /* graphs captured by rgpu_compute */
 #define RGPU_INFO_GRAPH_LAUNCHES 12 /* graph replays by rgpu_compute */
+#define RGPU_INFO_TILE_PAIRS_FWD 13 /* autotuned pairs per batch, expansion (0 = default) */
+#define RGPU_INFO_TILE_KEYS_PS 14   /* autotuned key blocks per pass, power spectrum */
+#define RGPU_INFO_TILE_PAIRS_ADJ 15 /* autotuned pairs per batch, adjoint */
 
 /* stages timed by rgpu_enable_timing */
 #define RGPU_STAGE_NEIGHBOURS 0
@@ -84,6 +87,11 @@ int rgpu_set_stream(rgpu_context* ctx, void* hip_stream);
  * stream was set.  Not used while stage timing is on.  No counterpart in
  * the reference (a CPU library). */
 int rgpu_enable_graphs(rgpu_context* ctx, int enable);
+/* Autotuner: times the candidate work partitions (pairs staged per batch in
+ * the expansion and the adjoint, key blocks per pass in the power spectrum)
+ * on the current batch and precision and keeps the fastest for later calls
+ * (RGPU_INFO_TILE_*).  No counterpart in the reference. */
+int rgpu_autotune(rgpu_context* ctx);
 /* RGPU_FP64 (default) or RGPU_MIXED; invalidates computed results. */
 int rgpu_set_precision(rgpu_context* ctx, int precision);
 int rgpu_get_info(rgpu_context* ctx, int key, int64_t* value);

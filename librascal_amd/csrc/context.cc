@@ -84,6 +84,10 @@ struct Context {
   DeviceBuffer<float> wt32, G32;
   DeviceBuffer<int64_t> knm_perm;
 
+  // kernel work partitions chosen by the autotuner, per precision
+  SoapTiles tiles[2];
+  SoapTiles& tile() { return tiles[precision == RGPU_FP64 ? 0 : 1]; }
+
   // HIP graph of the representation stage (cutoff, expansion, invariants)
   bool use_graphs = false;
   hipStream_t own_stream = nullptr;  // created for capture when no stream was given
@@ -359,10 +363,10 @@ void compute_rep(Context& c) {
   const int64_t* n_pairs_dev = c.pl.centre_off.get() + n;
   auto cutoff_and_expansion = [&] {
     launch_pair_cutoff(c.rd, n_pairs_dev, cap, c.pl.pair_dist.get(), fc, dfc, c.stream);
-    launch_expansion<T>(c.rd, S, pair_view(c), cc, km, c.stream);
+    launch_expansion<T>(c.rd, S, pair_view(c), cc, km, c.stream, c.tile());
   };
   auto invariants = [&] {
-    if (ps) launch_power_spectrum<T>(c.rd, S, n, cc, pp, n2, c.stream);
+    if (ps) launch_power_spectrum<T>(c.rd, S, n, cc, pp, n2, c.stream, c.tile());
   };
   if (c.use_graphs && !c.timing && n > 0) {
     // replay a captured graph of the stage while every buffer, size and
@@ -372,7 +376,7 @@ void compute_rep(Context& c) {
         uintptr_t(km), uintptr_t(fc), uintptr_t(dfc), uintptr_t(pp), uintptr_t(n2),
         uintptr_t(c.pl.centre_off.get()), uintptr_t(c.pl.pair_j.get()),
         uintptr_t(c.pl.pair_vec.get()), uintptr_t(c.pl.pair_dist.get()),
-        uintptr_t(c.d_species_of.get())};
+        uintptr_t(c.d_species_of.get()), uintptr_t(c.tile().pb_fwd), uintptr_t(c.tile().kc)};
     if (!c.rep_exec || sig != c.rep_sig) {
       c.drop_graph();
       hipGraph_t g = nullptr;
@@ -495,7 +499,7 @@ void backward_device(Context& c, const T* G, double* grad, double* cvir, const d
   const int S = int(c.species.size());
   StageTimer t(c, RGPU_STAGE_BACKWARD);
   launch_backward<T>(c.rd, S, pair_view(c), cbuf<T>(c).get(), pbuf<T>(c).get(), c.norm2.get(),
-                     c.hp.normalize, xg, xg_scale, G, grad, cvir, c.stream);
+                     c.hp.normalize, xg, xg_scale, G, grad, cvir, c.stream, c.tile());
 }
 
 double cell_volume(const double* C) {
@@ -657,6 +661,103 @@ int guarded(F&& f) {
 
 using namespace rgpu;
 
+// Autotuner: times the candidate work partitions of the expansion, the
+// power spectrum and the adjoint on the current batch and precision (HIP
+// events, best of 3 after a warm-up launch) and keeps the fastest.
+template <class T>
+void autotune_t(Context& c) {
+  const int S = int(c.species.size());
+  const int64_t n = c.n_atoms;
+  if (n == 0) return;
+  const bool ps = c.hp.kind == SoapHypers::kPowerSpectrum;
+  compute_rep<T>(c);  // buffers and pair cutoffs in place
+  hipEvent_t a = c.event(), b = c.event();
+  auto time_ms = [&](auto&& launch) {
+    launch();
+    float best = 1e30f;
+    for (int r = 0; r < 3; ++r) {
+      RGPU_HIP(hipEventRecord(a, c.stream));
+      launch();
+      RGPU_HIP(hipEventRecord(b, c.stream));
+      RGPU_HIP(hipEventSynchronize(b));
+      float ms = 0;
+      RGPU_HIP(hipEventElapsedTime(&ms, a, b));
+      best = std::min(best, ms);
+    }
+    return best;
+  };
+  SoapTiles& t = c.tile();
+  const PairView pv = pair_view(c);
+  T* cc = cbuf<T>(c).get();
+  unsigned long long* km = c.keymask.get();
+  {
+    float best = 1e30f;
+    int pick = 0;
+    for (int pb : {8, 16, 32, 48, 64}) {
+      SoapTiles cand = t;
+      cand.pb_fwd = pb;
+      float ms;
+      try {
+        ms = time_ms([&] { launch_expansion<T>(c.rd, S, pv, cc, km, c.stream, cand); });
+      } catch (const InvalidArgument&) {
+        continue;  // needs more LDS than the device provides
+      }
+      if (ms < best) best = ms, pick = pb;
+    }
+    t.pb_fwd = pick;
+  }
+  if (ps) {
+    const int K = S * (S + 1) / 2;
+    T* pp = pbuf<T>(c).get();
+    double* n2 = c.norm2.get();
+    float best = 1e30f;
+    int pick = 0;
+    for (int kc = 1; kc <= std::min(K, 8); ++kc) {
+      SoapTiles cand = t;
+      cand.kc = kc;
+      float ms;
+      try {
+        ms = time_ms([&] { launch_power_spectrum<T>(c.rd, S, n, cc, pp, n2, c.stream, cand); });
+      } catch (const InvalidArgument&) {
+        continue;
+      }
+      if (ms < best) best = ms, pick = kc;
+    }
+    t.kc = pick;
+    // adjoint on a zero dL/dX (its time does not depend on the values)
+    const int64_t D = c.D();
+    DeviceBuffer<T> G;
+    DeviceBuffer<double> gr, cv;
+    T* g = G.reserve(size_t(n * D));
+    RGPU_HIP(hipMemsetAsync(g, 0, sizeof(T) * n * D, c.stream));
+    double* grad = gr.reserve(size_t(3 * n));
+    double* cvir = cv.reserve(size_t(6 * n));
+    RGPU_HIP(hipMemsetAsync(grad, 0, sizeof(double) * 3 * n, c.stream));
+    best = 1e30f;
+    pick = 0;
+    for (int pb : {2, 4, 6, 8, 12, 16}) {
+      SoapTiles cand = t;
+      cand.pb_adj = pb;
+      float ms;
+      try {
+        ms = time_ms([&] {
+          launch_backward<T>(c.rd, S, pv, cc, pp, n2, c.hp.normalize, nullptr, 1.0, g, grad, cvir,
+                             c.stream, cand);
+        });
+      } catch (const InvalidArgument&) {
+        continue;
+      }
+      if (ms < best) best = ms, pick = pb;
+    }
+    t.pb_adj = pick;
+    RGPU_HIP(hipStreamSynchronize(c.stream));
+  }
+  c.ev_pool.push_back(a);
+  c.ev_pool.push_back(b);
+  c.have_rep = false;
+  c.drop_graph();
+}
+
 struct rgpu_context {
   Context c;
 };
@@ -689,6 +790,18 @@ int rgpu_set_stream(rgpu_context* ctx, void* s) {
   return guarded([&] {
     if (!ctx) throw InvalidArgument("null context");
     ctx->c.stream = static_cast<hipStream_t>(s);
+  });
+}
+
+int rgpu_autotune(rgpu_context* ctx) {
+  return guarded([&] {
+    if (!ctx) throw InvalidArgument("null context");
+    Context& c = ctx->c;
+    require_batch(c);
+    if (c.precision == RGPU_FP64)
+      autotune_t<double>(c);
+    else
+      autotune_t<float>(c);
   });
 }
 
@@ -740,6 +853,9 @@ int rgpu_get_info(rgpu_context* ctx, int key, int64_t* value) {
       case RGPU_INFO_PRECISION: *value = c.precision; break;
       case RGPU_INFO_GRAPH_CAPTURES: *value = c.graph_captures; break;
       case RGPU_INFO_GRAPH_LAUNCHES: *value = c.graph_launches; break;
+      case RGPU_INFO_TILE_PAIRS_FWD: *value = c.tile().pb_fwd; break;
+      case RGPU_INFO_TILE_KEYS_PS: *value = c.tile().kc; break;
+      case RGPU_INFO_TILE_PAIRS_ADJ: *value = c.tile().pb_adj; break;
       default: throw InvalidArgument("unknown info key");
     }
   });

@@ -27,15 +27,24 @@ struct PairView {
 void launch_pair_cutoff(const RadialDev& rad, const int64_t* n_pairs, int64_t cap, const double* dist,
                         double* fc, double* dfc, hipStream_t stream);
 
+// Work partition of the SOAP kernels; 0 selects the built-in choice.  Set
+// per batch by the autotuner (rgpu_autotune).
+struct SoapTiles {
+  int pb_fwd = 0;  // pairs staged per batch in the expansion (<= 64)
+  int kc = 0;      // key blocks staged per pass in the power spectrum
+  int pb_adj = 0;  // pairs staged per batch in the adjoint (<= 64)
+};
+
 // c[i][s][n][l*l+l+m] (finalised) and the per-centre key mask.
 template <class T>
 void launch_expansion(const RadialDev& rad, int S, const PairView& pv, T* c,
-                      unsigned long long* keymask, hipStream_t stream);
+                      unsigned long long* keymask, hipStream_t stream,
+                      const SoapTiles& tiles = SoapTiles());
 
 // p[i][key(a<=b)][n1*N+n2][l] UN-normalised, and norm2[i] = |p_i|^2 (FP64).
 template <class T>
 void launch_power_spectrum(const RadialDev& rad, int S, int64_t n_centres, const T* c, T* p,
-                           double* norm2, hipStream_t stream);
+                           double* norm2, hipStream_t stream, const SoapTiles& tiles = SoapTiles());
 
 // Normalised copy X = p / |p| (or X = p), FP64 output for export.
 template <class T>
@@ -51,7 +60,8 @@ void launch_normalise(int64_t n_centres, int64_t D, const T* p, const double* no
 template <class T>
 void launch_backward(const RadialDev& rad, int S, const PairView& pv, const T* c, const T* p,
                      const double* norm2, bool normalize, const double* xg, double xg_scale,
-                     const T* G, double* grad, double* centre_virial, hipStream_t stream);
+                     const T* G, double* grad, double* centre_virial, hipStream_t stream,
+                     const SoapTiles& tiles = SoapTiles());
 
 // out[s][:w] += sum of in[a][:w] over a in [seg_off[s], seg_off[s+1])
 void launch_segment_sum(const double* in, int w, const int64_t* seg_off, int n_seg, double* out,

@@ -984,11 +984,15 @@ void allow_lds(K kernel, size_t smem) {
 
 template <class T, int KPT, bool DIRECT>
 void launch_expansion_t(const RadialDev& rad, int S, const PairView& pv, T* c,
-                        unsigned long long* keymask, hipStream_t stream) {
+                        unsigned long long* keymask, hipStream_t stream, const SoapTiles& tiles) {
   const int F = rad.N * rad.L1;
   const size_t yt = ylm_table_bytes<T>(rad.L1);
   int PB = 64;
-  while (PB > 1 && yt + pair_batch_bytes(PB, rad.LL, F, sizeof(T), DIRECT, false) > kFwdBudget) PB /= 2;
+  if (tiles.pb_fwd > 0)
+    PB = std::min(tiles.pb_fwd, 64);
+  else
+    while (PB > 1 && yt + pair_batch_bytes(PB, rad.LL, F, sizeof(T), DIRECT, false) > kFwdBudget)
+      PB /= 2;
   const size_t smem = yt + pair_batch_bytes(PB, rad.LL, F, sizeof(T), DIRECT, false);
   auto kern = expansion_kernel<T, KPT, DIRECT>;
   allow_lds(kern, smem);
@@ -999,17 +1003,18 @@ void launch_expansion_t(const RadialDev& rad, int S, const PairView& pv, T* c,
 
 template <class T, int KPT>
 void launch_expansion_k(const RadialDev& rad, int S, const PairView& pv, T* c,
-                        unsigned long long* keymask, hipStream_t stream) {
+                        unsigned long long* keymask, hipStream_t stream, const SoapTiles& tiles) {
   if (rad.use_spline)
-    launch_expansion_t<T, KPT, false>(rad, S, pv, c, keymask, stream);
+    launch_expansion_t<T, KPT, false>(rad, S, pv, c, keymask, stream, tiles);
   else
-    launch_expansion_t<T, KPT, true>(rad, S, pv, c, keymask, stream);
+    launch_expansion_t<T, KPT, true>(rad, S, pv, c, keymask, stream, tiles);
 }
 
 template <class T, bool DIRECT>
 void launch_adjoint_t(const RadialDev& rad, int S, const PairView& pv, const T* c, const T* p,
                       const double* norm2, bool normalize, const double* xg, double xg_scale,
-                      const T* G, double* grad, double* centre_virial, hipStream_t stream) {
+                      const T* G, double* grad, double* centre_virial, hipStream_t stream,
+                      const SoapTiles& tiles) {
   const int N = rad.N, L1 = rad.L1, LL = rad.LL, F = N * L1, E = S * N * LL;
   const int N4 = (N + 3) & ~3;
   const size_t yt = ylm_table_bytes<T>(L1);
@@ -1020,7 +1025,10 @@ void launch_adjoint_t(const RadialDev& rad, int S, const PairView& pv, const T* 
   const size_t fixed = yt + blk;
   const size_t batch_room = std::max(region, kAdjBudget > fixed ? kAdjBudget - fixed : size_t(0));
   int PB = 8;
-  while (PB > 1 && pair_batch_bytes(PB, LL, F, sizeof(T), DIRECT, true) > batch_room) --PB;
+  if (tiles.pb_adj > 0)
+    PB = std::min(tiles.pb_adj, 64);
+  else
+    while (PB > 1 && pair_batch_bytes(PB, LL, F, sizeof(T), DIRECT, true) > batch_room) --PB;
   const size_t smem = fixed + std::max(region, pair_batch_bytes(PB, LL, F, sizeof(T), DIRECT, true));
   auto kern = adjoint_kernel<T, DIRECT>;
   allow_lds(kern, smem);
@@ -1033,38 +1041,39 @@ void launch_adjoint_t(const RadialDev& rad, int S, const PairView& pv, const T* 
 
 template <class T>
 void launch_expansion(const RadialDev& rad, int S, const PairView& pv, T* c,
-                      unsigned long long* keymask, hipStream_t stream) {
+                      unsigned long long* keymask, hipStream_t stream, const SoapTiles& tiles) {
   if (pv.n_centres == 0) return;
   if (S > kMaxSpecies) throw InvalidArgument("at most 64 species");
   const int B = rad.N * rad.LL;
   const int kpt = (B + kThreads - 1) / kThreads;
   if (kpt <= 2)
-    launch_expansion_k<T, 2>(rad, S, pv, c, keymask, stream);
+    launch_expansion_k<T, 2>(rad, S, pv, c, keymask, stream, tiles);
   else if (kpt <= 4)
-    launch_expansion_k<T, 4>(rad, S, pv, c, keymask, stream);
+    launch_expansion_k<T, 4>(rad, S, pv, c, keymask, stream, tiles);
   else if (kpt <= 5)
-    launch_expansion_k<T, 5>(rad, S, pv, c, keymask, stream);
+    launch_expansion_k<T, 5>(rad, S, pv, c, keymask, stream, tiles);
   else if (kpt <= 8)
-    launch_expansion_k<T, 8>(rad, S, pv, c, keymask, stream);
+    launch_expansion_k<T, 8>(rad, S, pv, c, keymask, stream, tiles);
   else if (kpt <= 12)
-    launch_expansion_k<T, 12>(rad, S, pv, c, keymask, stream);
+    launch_expansion_k<T, 12>(rad, S, pv, c, keymask, stream, tiles);
   else if (kpt <= 16)
-    launch_expansion_k<T, 16>(rad, S, pv, c, keymask, stream);
+    launch_expansion_k<T, 16>(rad, S, pv, c, keymask, stream, tiles);
   else if (kpt <= 28)
-    launch_expansion_k<T, 28>(rad, S, pv, c, keymask, stream);
+    launch_expansion_k<T, 28>(rad, S, pv, c, keymask, stream, tiles);
   else
     throw InvalidArgument("max_radial x (max_angular+1)^2 too large (> 7168)");
 }
 
 template <class T>
 void launch_power_spectrum(const RadialDev& rad, int S, int64_t n, const T* c, T* p, double* norm2,
-                           hipStream_t stream) {
+                           hipStream_t stream, const SoapTiles& tiles) {
   if (n == 0) return;
   const int K = S * (S + 1) / 2;
   if (S > kMaxSpecies) throw InvalidArgument("at most 64 species");
   const size_t cbytes = sizeof(T) * size_t(S) * rad.N * (rad.LL | 1);
   const size_t kb = sizeof(T) * size_t(rad.N) * rad.N * rad.L1;
   int KC = cbytes + kb < kPsBudget ? int((kPsBudget - cbytes) / kb) : 1;
+  if (tiles.kc > 0) KC = tiles.kc;
   KC = std::max(1, std::min(KC, K));
   const size_t smem = cbytes + size_t(KC) * kb + sizeof(int) * size_t(K);
   auto kern = power_spectrum_kernel<T>;
@@ -1088,14 +1097,15 @@ void launch_normalise(int64_t n, int64_t D, const T* p, const double* norm2, boo
 template <class T>
 void launch_backward(const RadialDev& rad, int S, const PairView& pv, const T* c, const T* p,
                      const double* norm2, bool normalize, const double* xg, double xg_scale,
-                     const T* G, double* grad, double* centre_virial, hipStream_t stream) {
+                     const T* G, double* grad, double* centre_virial, hipStream_t stream,
+                     const SoapTiles& tiles) {
   if (pv.n_centres == 0) return;
   if (rad.use_spline)
     launch_adjoint_t<T, false>(rad, S, pv, c, p, norm2, normalize, xg, xg_scale, G, grad, centre_virial,
-                               stream);
+                               stream, tiles);
   else
     launch_adjoint_t<T, true>(rad, S, pv, c, p, norm2, normalize, xg, xg_scale, G, grad, centre_virial,
-                              stream);
+                              stream, tiles);
 }
 
 void launch_pair_cutoff(const RadialDev& rad, const int64_t* n_pairs, int64_t cap, const double* dist,
@@ -1163,22 +1173,22 @@ template void launch_ylm_eval<float>(const RadialDev&, const double*, int64_t, d
                                      hipStream_t);
 
 template void launch_expansion<double>(const RadialDev&, int, const PairView&, double*,
-                                       unsigned long long*, hipStream_t);
+                                       unsigned long long*, hipStream_t, const SoapTiles&);
 template void launch_expansion<float>(const RadialDev&, int, const PairView&, float*,
-                                      unsigned long long*, hipStream_t);
+                                      unsigned long long*, hipStream_t, const SoapTiles&);
 template void launch_power_spectrum<double>(const RadialDev&, int, int64_t, const double*, double*,
-                                            double*, hipStream_t);
+                                            double*, hipStream_t, const SoapTiles&);
 template void launch_power_spectrum<float>(const RadialDev&, int, int64_t, const float*, float*,
-                                           double*, hipStream_t);
+                                           double*, hipStream_t, const SoapTiles&);
 template void launch_normalise<double>(int64_t, int64_t, const double*, const double*, bool, double*,
                                        hipStream_t);
 template void launch_normalise<float>(int64_t, int64_t, const float*, const double*, bool, double*,
                                       hipStream_t);
 template void launch_backward<double>(const RadialDev&, int, const PairView&, const double*,
                                       const double*, const double*, bool, const double*, double,
-                                      const double*, double*, double*, hipStream_t);
+                                      const double*, double*, double*, hipStream_t, const SoapTiles&);
 template void launch_backward<float>(const RadialDev&, int, const PairView&, const float*,
                                      const float*, const double*, bool, const double*, double,
-                                     const float*, double*, double*, hipStream_t);
+                                     const float*, double*, double*, hipStream_t, const SoapTiles&);
 
 }  // namespace rgpu

@@ -176,3 +176,25 @@ def test_ragged_batch_graph_replays_match_eager():
         assert np.array_equal(Xg, Xe) and np.array_equal(ng, ne)
     assert g_ctx.info("graph_captures") >= 1
     assert g_ctx.info("graph_launches") == 4
+
+
+def test_autotuned_tiles_keep_results():
+    """The autotuner picks work partitions on the triclinic 3-species batch
+    (rc 6, N 12, L 9); features are bit-identical and the adjoint's forces
+    agree to rounding with the default partitions."""
+    sts = triclinic_cells(n_cells=8, n_min=32, n_max=128, seed=31)
+    hp = soap_hypers(rc=6.0, sw=0.5, sigma=0.5, N=12, L=9, spline=1e-8)
+    for precision in (_capi.FP64, _capi.MIXED):
+        ctx, sp = make_ctx(hp, sts, precision=precision)
+        X0, _ = ctx.features()
+        rng = np.random.default_rng(2)
+        G = rng.normal(size=X0.shape)
+        g0, s0 = ctx.features_backward(G)
+        tiles = ctx.autotune()
+        assert tiles["pairs_fwd"] > 0 and tiles["keys_ps"] > 0 and tiles["pairs_adj"] > 0
+        X1, _ = ctx.features()
+        assert np.array_equal(X0, X1)
+        g1, s1 = ctx.features_backward(G)
+        tol = 1e-12 if precision == _capi.FP64 else 1e-5
+        assert np.abs(g1 - g0).max() < tol * np.abs(g0).max()
+        assert np.abs(s1 - s0).max() < tol * np.abs(s0).max()
