@@ -160,11 +160,27 @@ def main():
     k_avg_ms = k_ms / max(k_n, 1)
     achieved = flop_gemm / (k_avg_ms * 1e-3)
     peak = PEAK_BF16 / 3.0  # three BF16 products per algorithmic multiply-add
+    traffic = profiled_traffic("bf16x3_gemm<0>")
     roofline = dict(bound="mfma", achieved=achieved / 1e12, peak=peak / 1e12, unit="TFLOP/s",
-                    frac=achieved / peak, traffic=None,
+                    frac=achieved / peak, traffic=traffic,
                     kernel="gap_kernel (K = (X T^T)^zeta, all species blocks in one launch)",
                     avg_launch_ms=k_avg_ms, flop_per_launch=flop_gemm,
-                    bf16_mfma_utilisation=3.0 * achieved / PEAK_BF16)
+                    bf16_mfma_utilisation=3.0 * achieved / PEAK_BF16,
+                    traffic_source=TRAFFIC_FILE if traffic is not None else None)
+    # HBM-bound stages: algorithmic bytes per step / stage time (FP32 path)
+    n_p = n_pairs
+    S_, N_, L1_ = len(species), w["N"], w["L"] + 1
+    c_bytes = 4.0 * n_centres * S_ * N_ * L1_ * L1_
+    p_bytes = 4.0 * n_centres * D
+    pair_bytes = 56.0 * n_p
+    stage_bytes = dict(expansion=c_bytes + pair_bytes, power_spectrum=c_bytes + p_bytes,
+                       backward=2 * p_bytes + c_bytes + pair_bytes)
+    hbm = {}
+    for k, b in stage_bytes.items():
+        ms = stm[k][0] / max(args.steps, 1)
+        hbm[k] = dict(bytes=b, ms=ms, achieved_GBps=b / (ms * 1e-3) / 1e9,
+                      frac=b / (ms * 1e-3) / PEAK_HBM)
+    roofline["hbm_stages"] = hbm
 
     stage_ms = {k: round(v[0] / args.steps, 4) for k, v in stm.items()}
     stage_ms64 = {k: round(v[0] / steps64, 4) for k, v in st64.items()}
@@ -190,6 +206,26 @@ def main():
                             neg_stress_max_rel=stress_rel),
         stage_ms=stage_ms, roofline=roofline, cpu_baseline=cpu)
     print(json.dumps(line), flush=True)
+
+
+TRAFFIC_FILE = "profiles/hbm_traffic_periodic_box_8192.md"
+
+
+def profiled_traffic(kernel):
+    """HBM bytes per launch of `kernel` (FETCH_SIZE + WRITE_SIZE, scaled by the
+    copy-kernel calibration) from the committed rocprofv3 --pmc summary, or
+    None when it is absent."""
+    path = os.path.join(ROOT, TRAFFIC_FILE)
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) >= 5 and cells[0].strip("`") == kernel:
+            try:
+                return float(cells[3]) + float(cells[4])
+            except ValueError:
+                return None
+    return None
 
 
 def cpu_baseline(structures, hp, sp_species, sp_values, weights, zeta, n_centres):
