@@ -42,6 +42,13 @@ def lib():
     global _lib
     if _lib is not None:
         return _lib
+    # When PyTorch shares the process, its HIP runtime must be the one loaded
+    # first (the library then binds to the same runtime instead of loading a
+    # second copy).
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     if not os.path.exists(LIB_PATH):
         raise ImportError(
             "librascal_mi355x.so is not built (%s); run `python -m librascal_amd.build`" % LIB_PATH)

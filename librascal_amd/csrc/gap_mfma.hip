@@ -27,6 +27,7 @@ namespace rgpu {
 namespace {
 
 typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
+typedef __attribute__((__vector_size__(4 * sizeof(__bf16)))) __bf16 bf16x4;
 typedef __attribute__((__vector_size__(16 * sizeof(float)))) float f32x16;
 
 constexpr int BM = 128, BN = 128, BK = 32, NT = 256;
@@ -107,75 +108,96 @@ __global__ __launch_bounds__(NT, 2) void bf16x3_gemm(ProblemSet ps) {
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave >> 1, wn = wave & 1;
 
-  // global load assignment: thread -> (row lr, K half kh); a wavefront
-  // covers 64 consecutive rows of one K half, so its fragment stores are
-  // consecutive 16-byte words
-  const int lr = tid & (BM - 1), kh = tid >> 7;
-  const int lk = kh * 16;
-  const int64_t arow = r0 + lr, brow = c0 + lr;
-  const bool a_ok = arow < P.M, b_ok = brow < P.N;
-  const float* Ap = nullptr;
-  if (a_ok) Ap = P.A + (P.a_rows ? int64_t(P.a_rows[arow]) : arow) * P.lda;
-  const unsigned short* Bhp = b_ok ? P.Bhi + brow * P.ldb : nullptr;
-  const unsigned short* Blp = b_ok ? P.Blo + brow * P.ldb : nullptr;
-  // destination fragments of this thread: chunk kh, row block lr/32, lanes
-  // lr%32 (k 0..7 of the chunk) and lr%32 + 32 (k 8..15)
-  const int frag0 = ((lr >> 5) * (BK / 16) + kh) * 64 + (lr & 31);
+  // Global loads are coalesced along K: for A (FP32) eight lanes read the 32
+  // K values (128 B) of one row, rows rs + 32 j (j < 4); for B (BF16 hi and
+  // lo) four lanes read 16-byte pieces of one row, rows rs + 64 j (j < 2).
+  // Fragments sit in LDS with their 64 lane slots XOR-swizzled by
+  // 4 c + 2 (lane / 32) inside aligned groups of 8, so both these stores and
+  // the fragment reads hit distinct banks.
+  const int ars = tid >> 3, akq = tid & 7;
+  const int brs = tid >> 2, bkq = tid & 3;
+  const float* Ap[4];
+  bool a_ok[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int64_t row = r0 + ars + 32 * j;
+    a_ok[j] = row < P.M;
+    Ap[j] = a_ok[j] ? P.A + (P.a_rows ? int64_t(P.a_rows[row]) : row) * P.lda : nullptr;
+  }
+  const unsigned short *Bhp[2], *Blp[2];
+  bool b_ok[2];
+#pragma unroll
+  for (int j = 0; j < 2; ++j) {
+    const int64_t row = c0 + brs + 64 * j;
+    b_ok[j] = row < P.N;
+    Bhp[j] = b_ok[j] ? P.Bhi + row * P.ldb : nullptr;
+    Blp[j] = b_ok[j] ? P.Blo + row * P.ldb : nullptr;
+  }
+  auto slot = [](int rb, int c, int l) {
+    return (rb * (BK / 16) + c) * 64 + (l ^ (4 * c + 2 * (l >> 5)));
+  };
 
   float4 ra[4];
   uint4 rbh[2], rbl[2];
   auto gload = [&](int64_t k0) {
-    const int64_t k = k0 + lk;
-    if (a_ok && k + 16 <= P.K && ((reinterpret_cast<uintptr_t>(Ap + k) & 15) == 0)) {
-      const float4* src = reinterpret_cast<const float4*>(Ap + k);
+    const int64_t ka = k0 + 4 * akq;
 #pragma unroll
-      for (int q = 0; q < 4; ++q) ra[q] = src[q];
-    } else {
-      float tmp[16];
+    for (int j = 0; j < 4; ++j) {
+      if (a_ok[j] && ka + 4 <= P.K && ((reinterpret_cast<uintptr_t>(Ap[j] + ka) & 15) == 0)) {
+        ra[j] = *reinterpret_cast<const float4*>(Ap[j] + ka);
+      } else {
+        float t[4];
 #pragma unroll
-      for (int q = 0; q < 16; ++q) tmp[q] = (a_ok && k + q < P.K) ? Ap[k + q] : 0.f;
-#pragma unroll
-      for (int q = 0; q < 4; ++q) ra[q] = make_float4(tmp[4 * q], tmp[4 * q + 1], tmp[4 * q + 2], tmp[4 * q + 3]);
-    }
-    if (b_ok && k + 16 <= P.K && ((reinterpret_cast<uintptr_t>(Bhp + k) & 15) == 0) &&
-        ((reinterpret_cast<uintptr_t>(Blp + k) & 15) == 0)) {
-      const uint4* sh = reinterpret_cast<const uint4*>(Bhp + k);
-      const uint4* sl = reinterpret_cast<const uint4*>(Blp + k);
-      rbh[0] = sh[0];
-      rbh[1] = sh[1];
-      rbl[0] = sl[0];
-      rbl[1] = sl[1];
-    } else {
-      unsigned short th[16], tl[16];
-#pragma unroll
-      for (int q = 0; q < 16; ++q) {
-        const bool ok = b_ok && k + q < P.K;
-        th[q] = ok ? Bhp[k + q] : (unsigned short)0;
-        tl[q] = ok ? Blp[k + q] : (unsigned short)0;
+        for (int q = 0; q < 4; ++q) t[q] = (a_ok[j] && ka + q < P.K) ? Ap[j][ka + q] : 0.f;
+        ra[j] = make_float4(t[0], t[1], t[2], t[3]);
       }
-      memcpy(rbh, th, 32);
-      memcpy(rbl, tl, 32);
+    }
+    const int64_t kb = k0 + 8 * bkq;
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      if (b_ok[j] && kb + 8 <= P.K && ((reinterpret_cast<uintptr_t>(Bhp[j] + kb) & 15) == 0) &&
+          ((reinterpret_cast<uintptr_t>(Blp[j] + kb) & 15) == 0)) {
+        rbh[j] = *reinterpret_cast<const uint4*>(Bhp[j] + kb);
+        rbl[j] = *reinterpret_cast<const uint4*>(Blp[j] + kb);
+      } else {
+        unsigned short th[8], tl[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+          const bool ok = b_ok[j] && kb + q < P.K;
+          th[q] = ok ? Bhp[j][kb + q] : (unsigned short)0;
+          tl[q] = ok ? Blp[j][kb + q] : (unsigned short)0;
+        }
+        memcpy(&rbh[j], th, 16);
+        memcpy(&rbl[j], tl, 16);
+      }
     }
   };
   auto lstore = [&](int st) {
     // x = hi + lo, hi = bf16(x), lo = bf16(x - hi), both round-to-nearest-even
     // (hardware conversions on gfx950)
-    bf16x8 hi[2], lo[2];
-    const float* f = reinterpret_cast<const float*>(ra);
+    const int ac = akq >> 2, ahalf = (akq >> 1) & 1, asub = akq & 1;
 #pragma unroll
-    for (int q = 0; q < 16; ++q) {
-      const __bf16 h = (__bf16)f[q];
-      hi[q >> 3][q & 7] = h;
-      lo[q >> 3][q & 7] = (__bf16)(f[q] - (float)h);
+    for (int j = 0; j < 4; ++j) {
+      const float f[4] = {ra[j].x, ra[j].y, ra[j].z, ra[j].w};
+      bf16x4 hi, lo;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const __bf16 h = (__bf16)f[q];
+        hi[q] = h;
+        lo[q] = (__bf16)(f[q] - (float)h);
+      }
+      const int fs = slot(j, ac, ars + 32 * ahalf);
+      reinterpret_cast<bf16x4*>(&sAh[st][fs])[asub] = hi;
+      reinterpret_cast<bf16x4*>(&sAl[st][fs])[asub] = lo;
     }
-    sAh[st][frag0] = hi[0];
-    sAh[st][frag0 + 32] = hi[1];
-    sAl[st][frag0] = lo[0];
-    sAl[st][frag0 + 32] = lo[1];
-    sBh[st][frag0] = *reinterpret_cast<const bf16x8*>(&rbh[0]);
-    sBh[st][frag0 + 32] = *reinterpret_cast<const bf16x8*>(&rbh[1]);
-    sBl[st][frag0] = *reinterpret_cast<const bf16x8*>(&rbl[0]);
-    sBl[st][frag0 + 32] = *reinterpret_cast<const bf16x8*>(&rbl[1]);
+    const int bc = bkq >> 1, bhalf = bkq & 1;
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const int r = brs + 64 * j;
+      const int fs = slot(r >> 5, bc, (r & 31) + 32 * bhalf);
+      sBh[st][fs] = *reinterpret_cast<const bf16x8*>(&rbh[j]);
+      sBl[st][fs] = *reinterpret_cast<const bf16x8*>(&rbl[j]);
+    }
   };
 
   f32x16 acc[2][2];
@@ -198,13 +220,13 @@ __global__ __launch_bounds__(NT, 2) void bf16x3_gemm(ProblemSet ps) {
       bf16x8 ah[2], al[2], bh[2], bl[2];
 #pragma unroll
       for (int i = 0; i < 2; ++i) {
-        const int f = ((wm * 2 + i) * (BK / 16) + c) * 64 + lane;
+        const int f = slot(wm * 2 + i, c, lane);
         ah[i] = sAh[st][f];
         al[i] = sAl[st][f];
       }
 #pragma unroll
       for (int j = 0; j < 2; ++j) {
-        const int f = ((wn * 2 + j) * (BK / 16) + c) * 64 + lane;
+        const int f = slot(wn * 2 + j, c, lane);
         bh[j] = sBh[st][f];
         bl[j] = sBl[st][f];
       }

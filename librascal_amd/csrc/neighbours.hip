@@ -355,7 +355,8 @@ void build_pair_list(PairList& pl, int n_struct, const int64_t* atom_offsets,
   if (pl.cap_pairs < n_atoms) pl.cap_pairs = 48 * n_atoms + 1024;
   const int TB = 256;
   auto blocks = [](int64_t n, int tb) { return dim3(unsigned((n + tb - 1) / tb)); };
-  for (int pass = 0; pass < 2; ++pass) {
+  // a short ghost capacity also hides pairs, so up to three passes
+  for (int pass = 0; pass < 4; ++pass) {
     const int64_t cap_ext = pl.cap_ext, cap_pairs = pl.cap_pairs;
     int* err = pl.err.reserve(1);
     RGPU_HIP(hipMemsetAsync(err, 0, sizeof(int), stream));

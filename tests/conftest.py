@@ -5,6 +5,11 @@ import sys
 import numpy as np
 import pytest
 
+try:  # load PyTorch's HIP runtime before the library's (one runtime per process)
+    import torch  # noqa: F401
+except ImportError:
+    pass
+
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 sys.path.insert(0, ROOT)
 sys.path.insert(0, os.path.join(ROOT, "oracle"))

