@@ -194,9 +194,16 @@ class Context:
         return X, inv
 
     def features_backward(self, dL_dX, want_stress=True):
+        """dL/dr [n_atoms][3] (and the negative virial) for dL/dX given as a
+        host array or a contiguous float64 torch tensor (host or device)."""
         n = self.info("n_centres")
         ns = self.info("n_struct")
-        g = np.ascontiguousarray(dL_dX, np.float64)
+        if hasattr(dL_dX, "data_ptr"):
+            g = dL_dX
+            if str(g.dtype) != "torch.float64" or not g.is_contiguous():
+                raise ValueError("dL/dX tensors must be contiguous float64")
+        else:
+            g = np.ascontiguousarray(dL_dX, np.float64)
         out = np.zeros((n, 3))
         st = np.zeros((ns, 6)) if want_stress else None
         check(lib().rgpu_features_backward(self._p, ptr(g), ptr(out), ptr(st)))

@@ -5,8 +5,8 @@ size-independent properties (finite differences, FP64 vs mixed) elsewhere.
   diamond  8 diamond cells of 64 Si atoms, rc 5, N 8, L 6 (direct Hyp1f1 and
            Spline 1e-8), FP64 expansion + invariants <= 1e-10
   ragged   non-periodic molecules of 8..64 atoms, 4 species, rc 4, N 8, L 6,
-           sigma 0.4: FP32 expansion / invariants vs the oracle (<= 1e-5
-           block-relative), FP64 <= 1e-10
+           sigma 0.4: FP64 <= 1e-10 (block-relative); FP32 expansion <= 1e-5
+           block-relative, FP32 normalised invariants <= 1e-5 absolute (|X| = 1)
   forces   periodic 2-species box (1024 atoms, rho 0.05), rc 5, N 10, L 8,
            zeta 2, 1000 sparse points per species: GAP forces of the GPU
            adjoint vs central differences (h = 1e-4 A) of the GPU energy,
@@ -69,7 +69,7 @@ def test_ragged_molecules_fp32_and_fp64():
     c32, _ = ctx.expansion()
     X32, _ = ctx.features()
     assert expansion_rel(c_ref, c32, 6) < 1e-5
-    assert ps_rel(p_ref, X32, len(sp), 8, 6) < 1e-5
+    assert np.abs(X32 - p_ref).max() < 1e-5
 
 
 def _gap_box(seed=20262):
