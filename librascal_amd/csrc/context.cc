@@ -81,7 +81,11 @@ struct Context {
   DeviceBuffer<double> pair_fc, pair_dfc;  // cutoff function per pair
   DeviceBuffer<int> d_sp_rows;             // 0..M-1 (K_MM row gather)
   DeviceBuffer<double> wt64, G64;
-  DeviceBuffer<float> wt32, G32;
+  DeviceBuffer<float> G32;
+  // mixed path: the invariants and W~ as BF16 operand planes; FP32
+  // invariants are formed only for export and K_NM (p32_valid)
+  DeviceBuffer<unsigned short> p_hi, p_lo, wth, wtl;
+  bool p32_valid = false;
   DeviceBuffer<int64_t> knm_perm;
 
   // kernel work partitions chosen by the autotuner, per precision
@@ -358,24 +362,34 @@ void compute_rep(Context& c) {
   unsigned long long* km = c.keymask.reserve(size_t(std::max<int64_t>(n, 1)));
   double* fc = c.pair_fc.reserve(size_t(std::max<int64_t>(cap, 1)));
   double* dfc = c.pair_dfc.reserve(size_t(std::max<int64_t>(cap, 1)));
-  T* pp = ps ? pbuf<T>(c).reserve(size_t(std::max<int64_t>(n * D, 1))) : nullptr;
+  // FP64 keeps the invariants in FP64; the mixed path stores them as the
+  // BF16 hi/lo operand planes of the matrix-core GEMMs
+  constexpr bool split = std::is_same<T, float>::value;
+  T* pp = ps && !split ? pbuf<T>(c).reserve(size_t(std::max<int64_t>(n * D, 1))) : nullptr;
+  unsigned short* ph = ps && split ? c.p_hi.reserve(size_t(std::max<int64_t>(n * D, 1))) : nullptr;
+  unsigned short* pl = ps && split ? c.p_lo.reserve(size_t(std::max<int64_t>(n * D, 1))) : nullptr;
   double* n2 = ps ? c.norm2.reserve(size_t(std::max<int64_t>(n, 1))) : nullptr;
+  c.p32_valid = false;
   const int64_t* n_pairs_dev = c.pl.centre_off.get() + n;
   auto cutoff_and_expansion = [&] {
     launch_pair_cutoff(c.rd, n_pairs_dev, cap, c.pl.pair_dist.get(), fc, dfc, c.stream);
     launch_expansion<T>(c.rd, S, pair_view(c), cc, km, c.stream, c.tile());
   };
   auto invariants = [&] {
-    if (ps) launch_power_spectrum<T>(c.rd, S, n, cc, pp, n2, c.stream, c.tile());
+    if (!ps) return;
+    if constexpr (split)
+      launch_power_spectrum_split(c.rd, S, n, cc, ph, pl, n2, c.stream, c.tile());
+    else
+      launch_power_spectrum<T>(c.rd, S, n, cc, pp, n2, c.stream, c.tile());
   };
   if (c.use_graphs && !c.timing && n > 0) {
     // replay a captured graph of the stage while every buffer, size and
     // precision it was captured with is unchanged
     const std::vector<uintptr_t> sig = {
         uintptr_t(c.precision), uintptr_t(S), uintptr_t(n), uintptr_t(cap), uintptr_t(cc),
-        uintptr_t(km), uintptr_t(fc), uintptr_t(dfc), uintptr_t(pp), uintptr_t(n2),
-        uintptr_t(c.pl.centre_off.get()), uintptr_t(c.pl.pair_j.get()),
-        uintptr_t(c.pl.pair_vec.get()), uintptr_t(c.pl.pair_dist.get()),
+        uintptr_t(km), uintptr_t(fc), uintptr_t(dfc), uintptr_t(pp), uintptr_t(ph),
+        uintptr_t(pl), uintptr_t(n2), uintptr_t(c.pl.centre_off.get()),
+        uintptr_t(c.pl.pair_j.get()), uintptr_t(c.pl.pair_vec.get()), uintptr_t(c.pl.pair_dist.get()),
         uintptr_t(c.d_species_of.get()), uintptr_t(c.tile().pb_fwd), uintptr_t(c.tile().kc)};
     if (!c.rep_exec || sig != c.rep_sig) {
       c.drop_graph();
@@ -472,16 +486,6 @@ const float* sparse_values<float>(Context& c) {
   return c.T32.get();
 }
 template <class T>
-DeviceBuffer<T>& wtbuf(Context& c);
-template <>
-DeviceBuffer<double>& wtbuf<double>(Context& c) {
-  return c.wt64;
-}
-template <>
-DeviceBuffer<float>& wtbuf<float>(Context& c) {
-  return c.wt32;
-}
-template <class T>
 DeviceBuffer<T>& Gbuf(Context& c);
 template <>
 DeviceBuffer<double>& Gbuf<double>(Context& c) {
@@ -498,8 +502,38 @@ void backward_device(Context& c, const T* G, double* grad, double* cvir, const d
                      double xg_scale = 1.0) {
   const int S = int(c.species.size());
   StageTimer t(c, RGPU_STAGE_BACKWARD);
-  launch_backward<T>(c.rd, S, pair_view(c), cbuf<T>(c).get(), pbuf<T>(c).get(), c.norm2.get(),
-                     c.hp.normalize, xg, xg_scale, G, grad, cvir, c.stream, c.tile());
+  if constexpr (std::is_same<T, float>::value)
+    launch_backward_split(c.rd, S, pair_view(c), c.c32.get(), c.p_hi.get(), c.p_lo.get(),
+                          c.norm2.get(), c.hp.normalize, xg, xg_scale, G, grad, cvir, c.stream,
+                          c.tile());
+  else
+    launch_backward<T>(c.rd, S, pair_view(c), cbuf<T>(c).get(), pbuf<T>(c).get(), c.norm2.get(),
+                       c.hp.normalize, xg, xg_scale, G, grad, cvir, c.stream, c.tile());
+}
+
+// FP32 invariants of the mixed path (for export and K_NM), formed on demand
+// from the stored expansion; norm2 is recomputed with identical values
+const float* ensure_p32(Context& c) {
+  if (!c.p32_valid) {
+    const int64_t n = c.n_atoms;
+    float* p = c.p32.reserve(size_t(std::max<int64_t>(n * c.D(), 1)));
+    if (n > 0)
+      launch_power_spectrum<float>(c.rd, int(c.species.size()), n, c.c32.get(), p, c.norm2.get(),
+                                   c.stream, c.tile());
+    c.p32_valid = true;
+  }
+  return c.p32.get();
+}
+
+template <class T>
+const T* invariants_of(Context& c);
+template <>
+const double* invariants_of<double>(Context& c) {
+  return c.p64.get();
+}
+template <>
+const float* invariants_of<float>(Context& c) {
+  return ensure_p32(c);
 }
 
 double cell_volume(const double* C) {
@@ -535,40 +569,46 @@ void gap_predict_t(Context& c, double* energy, double* gradient, double* neg_str
   const bool want_grad = gradient || neg_stress;
   double* ec = c.e_centre.reserve(size_t(std::max<int64_t>(n, 1)));
   RGPU_HIP(hipMemsetAsync(ec, 0, sizeof(double) * n, c.stream));
-  T* G = nullptr;
-  if (want_grad) {
-    G = Gbuf<T>(c).reserve(size_t(std::max<int64_t>(n * D, 1)));
-    RGPU_HIP(hipMemsetAsync(G, 0, sizeof(T) * n * D, c.stream));
-  }
-  const T* X = pbuf<T>(c).get();
-  // W~ of all species blocks in one buffer
+  const T* X = std::is_same<T, double>::value ? pbuf<T>(c).get() : nullptr;
   std::vector<GapBlock<T>> blks;
   std::vector<int64_t> wt_off;
   int64_t wt_total = 0;
+  bool rows_without_points = false;
   for (int s = 0; s < S; ++s) {
     blks.push_back(gap_block<T>(c, s, X, sparse_values<T>(c)));
     wt_off.push_back(wt_total);
-    wt_total += blks.back().n_rows * blks.back().n_cols;
+    // blocks start on 8-element (16-byte) boundaries
+    wt_total += (blks.back().n_rows * blks.back().n_cols + 7) / 8 * 8;
+    if (blks.back().n_rows > 0 && blks.back().n_cols == 0) rows_without_points = true;
   }
-  T* wt_all = want_grad ? wtbuf<T>(c).reserve(size_t(std::max<int64_t>(wt_total, 1))) : nullptr;
-  std::vector<T*> wts(size_t(S), nullptr);
-  if (want_grad)
-    for (int s = 0; s < S; ++s) wts[s] = wt_all + wt_off[s];
-  bool grouped_done = false;
+  T* G = nullptr;
+  if (want_grad) {
+    // the force product writes every row of a species with sparse points
+    G = Gbuf<T>(c).reserve(size_t(std::max<int64_t>(n * D, 1)));
+    if (rows_without_points) RGPU_HIP(hipMemsetAsync(G, 0, sizeof(T) * n * D, c.stream));
+  }
   if constexpr (std::is_same<T, float>::value) {
-    if (mfma_available()) {
-      {
-        StageTimer t(c, RGPU_STAGE_GAP_KERNEL);
-        run_gap_mixed_kernel(blks.data(), S, c.mixed_sparse, ec, want_grad ? wts.data() : nullptr,
-                             c.stream);
-      }
-      if (want_grad) {
-        StageTimer t(c, RGPU_STAGE_GAP_FORCE);
-        run_gap_mixed_force(blks.data(), S, c.mixed_sparse, wts.data(), G, c.stream);
-      }
-      grouped_done = true;
+    std::vector<SplitPlanes> wts(static_cast<size_t>(S));
+    if (want_grad) {
+      unsigned short* wh = c.wth.reserve(size_t(std::max<int64_t>(wt_total, 1)));
+      unsigned short* wl = c.wtl.reserve(size_t(std::max<int64_t>(wt_total, 1)));
+      for (int s = 0; s < S; ++s) wts[s] = SplitPlanes{wh + wt_off[s], wl + wt_off[s]};
+    }
+    const SplitPlanes xs{c.p_hi.get(), c.p_lo.get()};
+    {
+      StageTimer t(c, RGPU_STAGE_GAP_KERNEL);
+      run_gap_mixed_kernel(blks.data(), S, c.mixed_sparse, xs, ec, want_grad ? wts.data() : nullptr,
+                           c.stream);
+    }
+    if (want_grad) {
+      StageTimer t(c, RGPU_STAGE_GAP_FORCE);
+      run_gap_mixed_force(blks.data(), S, c.mixed_sparse, wts.data(), G, c.stream);
     }
   } else {
+    double* wt_all = want_grad ? c.wt64.reserve(size_t(std::max<int64_t>(wt_total, 1))) : nullptr;
+    std::vector<double*> wts(static_cast<size_t>(S), nullptr);
+    if (want_grad)
+      for (int s = 0; s < S; ++s) wts[s] = wt_all + wt_off[s];
     {
       StageTimer t(c, RGPU_STAGE_GAP_KERNEL);
       run_gap_f64_kernel(blks.data(), S, ec, want_grad ? wts.data() : nullptr, c.stream);
@@ -576,22 +616,6 @@ void gap_predict_t(Context& c, double* energy, double* gradient, double* neg_str
     if (want_grad) {
       StageTimer t(c, RGPU_STAGE_GAP_FORCE);
       run_gap_f64_force(blks.data(), S, wts.data(), G, c.stream);
-    }
-    grouped_done = true;
-  }
-  for (int s = 0; s < S && !grouped_done; ++s) {
-    const GapBlock<T>& blk = blks[s];
-    if (blk.n_rows == 0 || blk.n_cols == 0) continue;
-    T* wt = wts[s];
-    {
-      {
-        StageTimer t(c, RGPU_STAGE_GAP_KERNEL);
-        launch_gap_kernel<T>(blk, c.M, ec, wt, nullptr, nullptr, nullptr, c.stream);
-      }
-      if (want_grad) {
-        StageTimer t(c, RGPU_STAGE_GAP_FORCE);
-        launch_gap_force_contraction<T>(blk, wt, G, c.stream);
-      }
     }
   }
   if (energy) {
@@ -617,7 +641,7 @@ void gap_knm_t(Context& c, int target, double* knm) {
   const int64_t rows = target == 0 ? c.n_struct : n;
   double* k = c.tmp64.reserve(size_t(std::max<int64_t>(rows * c.M, 1)));
   RGPU_HIP(hipMemsetAsync(k, 0, sizeof(double) * rows * c.M, c.stream));
-  const T* X = pbuf<T>(c).get();
+  const T* X = invariants_of<T>(c);
   for (int s = 0; s < S; ++s) {
     GapBlock<T> blk = gap_block<T>(c, s, X, sparse_values<T>(c));
     blk.alpha = nullptr;
@@ -708,7 +732,10 @@ void autotune_t(Context& c) {
   }
   if (ps) {
     const int K = S * (S + 1) / 2;
-    T* pp = pbuf<T>(c).get();
+    constexpr bool split = std::is_same<T, float>::value;
+    T* pp = split ? nullptr : pbuf<T>(c).get();
+    unsigned short* ph = c.p_hi.get();
+    unsigned short* pl = c.p_lo.get();
     double* n2 = c.norm2.get();
     float best = 1e30f;
     int pick = 0;
@@ -717,7 +744,12 @@ void autotune_t(Context& c) {
       cand.kc = kc;
       float ms;
       try {
-        ms = time_ms([&] { launch_power_spectrum<T>(c.rd, S, n, cc, pp, n2, c.stream, cand); });
+        ms = time_ms([&] {
+          if constexpr (split)
+            launch_power_spectrum_split(c.rd, S, n, cc, ph, pl, n2, c.stream, cand);
+          else
+            launch_power_spectrum<T>(c.rd, S, n, cc, pp, n2, c.stream, cand);
+        });
       } catch (const InvalidArgument&) {
         continue;
       }
@@ -741,8 +773,12 @@ void autotune_t(Context& c) {
       float ms;
       try {
         ms = time_ms([&] {
-          launch_backward<T>(c.rd, S, pv, cc, pp, n2, c.hp.normalize, nullptr, 1.0, g, grad, cvir,
-                             c.stream, cand);
+          if constexpr (split)
+            launch_backward_split(c.rd, S, pv, cc, ph, pl, n2, c.hp.normalize, nullptr, 1.0, g,
+                                  grad, cvir, c.stream, cand);
+          else
+            launch_backward<T>(c.rd, S, pv, cc, pp, n2, c.hp.normalize, nullptr, 1.0, g, grad,
+                               cvir, c.stream, cand);
         });
       } catch (const InvalidArgument&) {
         continue;
@@ -986,7 +1022,7 @@ int rgpu_get_features(rgpu_context* ctx, double* features, double* inv_norms) {
       if (c.precision == RGPU_FP64)
         launch_normalise<double>(n, D, c.p64.get(), c.norm2.get(), c.hp.normalize, o, c.stream);
       else
-        launch_normalise<float>(n, D, c.p32.get(), c.norm2.get(), c.hp.normalize, o, c.stream);
+        launch_normalise<float>(n, D, ensure_p32(c), c.norm2.get(), c.hp.normalize, o, c.stream);
       RGPU_HIP(hipMemcpyAsync(features, o, sizeof(double) * n * D, hipMemcpyDefault, c.stream));
     }
     if (inv_norms && n) {

@@ -21,16 +21,24 @@ struct MixedSparse {
   DeviceBuffer<unsigned short> tt_hi, tt_lo;  // [D][M]
 };
 
+// A BF16 hi/lo pair of planes (same shape); bit patterns of __bf16.
+struct SplitPlanes {
+  unsigned short* hi = nullptr;
+  unsigned short* lo = nullptr;
+};
+
 bool mfma_available();
 void prepare_mixed_sparse(MixedSparse& ms, const std::vector<double>& T, int64_t M, int64_t D,
                           hipStream_t stream);
-// K tile GEMM + epilogue for all species blocks in grouped launches:
-// energy (per centre, +=) and wts[b] [n_rows][n_cols] (wts or any entry may
-// be null)
+// K tile GEMM + epilogue for all species blocks in grouped launches.  The
+// features are given split: X = xs.hi + xs.lo, [n_centres][D] (rows gathered
+// through the blocks' row_idx, un-normalised, scaled by 1/|p| in the
+// epilogue).  Outputs: energy (per centre, +=) and the weighted kernel
+// gradient W~ of each block, split, [n_rows][n_cols] (wts may be null).
 void run_gap_mixed_kernel(const GapBlock<float>* blks, int n_blocks, const MixedSparse& ms,
-                          double* energy, float* const* wts, hipStream_t stream);
-// T~ rows: G[row_idx[r]][:] = sum_m wts[b][r][m] T[col0+m][:]
+                          SplitPlanes xs, double* energy, const SplitPlanes* wts, hipStream_t stream);
+// T~ rows: G[row_idx[r]][:] = sum_m W~_b[r][m] T[col0+m][:]
 void run_gap_mixed_force(const GapBlock<float>* blks, int n_blocks, const MixedSparse& ms,
-                         float* const* wts, float* G, hipStream_t stream);
+                         const SplitPlanes* wts, float* G, hipStream_t stream);
 
 }  // namespace rgpu

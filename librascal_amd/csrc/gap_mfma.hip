@@ -4,16 +4,18 @@
 // stored row-major with the contraction index contiguous:
 //   kernel:  K[r][m] = sum_d X[row(r)][d] T[col0+m][d]       (B = T, [M][D])
 //   forces:  G[row(r)][d] = sum_m W~[r][m] T[col0+m][d]      (B = T^T, [D][M])
-// FP32 operands are split as x = hi + lo with hi = bf16(x), lo = bf16(x-hi);
-// each algorithmic multiply-add is three v_mfma_f32_32x32x16_bf16 products
-// (hi.hi + hi.lo + lo.hi) accumulated in FP32.  The sparse points (model
-// data) are split once at load time; X and W~ are split while staging
-// through LDS.
+// Every operand arrives split into BF16 planes, x = hi + lo with
+// hi = bf16(x), lo = bf16(x - hi) (the power spectrum and this kernel's own
+// energy epilogue write them that way; the sparse points are split once at
+// load time), so the main loop does no conversions: each algorithmic
+// multiply-add is three v_mfma_f32_32x32x16_bf16 products (hi.hi + hi.lo +
+// lo.hi) accumulated in FP32.
 //
 // Tiling: 256 threads (4 wavefronts, 2 x 2), block tile 128 x 128, K step 32;
-// each wavefront owns 64 x 64 = 2 x 2 MFMA tiles.  Global loads of the next K
-// step are issued before the MFMAs of the current one (register prefetch).
-// All species blocks of one product go in one grouped launch.
+// each wavefront owns 64 x 64 = 2 x 2 MFMA tiles.  Two LDS stages (one
+// barrier per K step), the next K step's global loads issued before the
+// MFMAs of the current one.  All species blocks of one product go in one
+// grouped launch, dealt to the XCDs so that a row panel stays in one L2.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -27,16 +29,19 @@ namespace rgpu {
 namespace {
 
 typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
-typedef __attribute__((__vector_size__(4 * sizeof(__bf16)))) __bf16 bf16x4;
 typedef __attribute__((__vector_size__(16 * sizeof(float)))) float f32x16;
 
 constexpr int BM = 128, BN = 128, BK = 32, NT = 256;
 constexpr int kMaxProblems = 16;
 
+__device__ __forceinline__ unsigned short bf16_bits(__bf16 h) {
+  return __builtin_bit_cast(unsigned short, h);
+}
 
 struct Problem {
   int64_t M, N, K;             // C (M x N) = A (M x K) B (N x K)^T
-  const float* A;
+  const unsigned short* Ahi;   // A rows, K contiguous
+  const unsigned short* Alo;
   int64_t lda;
   const int* a_rows;           // gathered A rows (null: identity)
   const unsigned short* Bhi;   // B rows, K contiguous, already offset
@@ -45,11 +50,13 @@ struct Problem {
   const int* c_rows;           // output row ids (centre indices)
   int64_t tile0;               // first linear tile of this problem
   int tiles_n;                 // tiles along N
+  int fast;                    // K % 32 == 0 and every row 16-byte aligned
   // kernel epilogue
   const double* norm2;
   const double* alpha;         // offset to the block's first sparse point
   double* energy;
-  float* wt;                   // [M][N]
+  unsigned short* wth;         // [M][N] W~ split (may be null)
+  unsigned short* wtl;
   // force epilogue
   float* G;
   int64_t ldg;
@@ -79,124 +86,89 @@ __device__ __forceinline__ T pow_int(T v, int n) {
   return r;
 }
 
-template <int EPI>  // 0: GAP kernel epilogue, 1: store rows of G
-__global__ __launch_bounds__(NT, 2) void bf16x3_gemm(ProblemSet ps) {
-  // Two LDS stages.  Each operand tile is kept in the matrix-core fragment
-  // order: for a 32-row block rb and a 16-wide K chunk c, the 64 lanes'
-  // 16-byte fragments (lane l: row l%32, k 8*(l/32) .. +8) are contiguous, so
-  // fragment reads are conflict free and one K step needs one barrier.
-  constexpr int FR = (BM / 32) * (BK / 16) * 64;  // fragments per operand per stage
-  __shared__ __align__(16) bf16x8 sAh[2][FR];
-  __shared__ __align__(16) bf16x8 sAl[2][FR];
-  __shared__ __align__(16) bf16x8 sBh[2][FR];
-  __shared__ __align__(16) bf16x8 sBl[2][FR];
-  __shared__ double esum[2][BM];
+// 32-row block rb, 16-wide K chunk c, lane l (row l%32, K 8*(l/32) .. +8):
+// fragment slot in an LDS stage, XOR-swizzled by 4 c + 2 (l / 32) inside
+// aligned groups of 8 so the K-coalesced stores and the fragment reads hit
+// distinct banks
+__device__ __forceinline__ int frag_slot(int rb, int c, int l) {
+  return (rb * (BK / 16) + c) * 64 + (l ^ (4 * c + 2 * (l >> 5)));
+}
 
-  // problem and tile of this block.  Workgroups are dealt to the 8 XCDs
-  // round robin; consecutive tiles (same row panel) go to the same XCD so
-  // the row panel is shared through that XCD's L2.
-  const int64_t per_xcd = (int64_t(gridDim.x) + 7) / 8;
-  const int64_t bid = int64_t(blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
-  if (bid >= ps.total_tiles) return;
-  int pi = 0;
-  while (pi + 1 < ps.n && bid >= ps.p[pi + 1].tile0) ++pi;
-  const Problem& P = ps.p[pi];
-  const int64_t local = bid - P.tile0;
-  const int64_t tm = local / P.tiles_n, tn = local - tm * P.tiles_n;
-  const int64_t r0 = tm * BM, c0 = tn * BN;
+constexpr int FR = (BM / 32) * (BK / 16) * 64;  // fragments per operand per LDS stage
+
+// LDS of one workgroup: two stages of the four operand planes
+struct GemmLds {
+  bf16x8 Ah[2][FR];
+  bf16x8 Al[2][FR];
+  bf16x8 Bh[2][FR];
+  bf16x8 Bl[2][FR];
+  double esum[2][BM];
+};
+
+template <int EPI, bool FAST>
+__device__ __forceinline__ void gemm_tile(const ProblemSet& ps, const Problem& P, int64_t r0, int64_t c0,
+                                          GemmLds& sm) {
+  auto& sAh = sm.Ah;
+  auto& sAl = sm.Al;
+  auto& sBh = sm.Bh;
+  auto& sBl = sm.Bl;
+  auto& esum = sm.esum;
 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave >> 1, wn = wave & 1;
 
-  // Global loads are coalesced along K: for A (FP32) eight lanes read the 32
-  // K values (128 B) of one row, rows rs + 32 j (j < 4); for B (BF16 hi and
-  // lo) four lanes read 16-byte pieces of one row, rows rs + 64 j (j < 2).
-  // Fragments sit in LDS with their 64 lane slots XOR-swizzled by
-  // 4 c + 2 (lane / 32) inside aligned groups of 8, so both these stores and
-  // the fragment reads hit distinct banks.
-  const int ars = tid >> 3, akq = tid & 7;
-  const int brs = tid >> 2, bkq = tid & 3;
-  const float* Ap[4];
-  bool a_ok[4];
-#pragma unroll
-  for (int j = 0; j < 4; ++j) {
-    const int64_t row = r0 + ars + 32 * j;
-    a_ok[j] = row < P.M;
-    Ap[j] = a_ok[j] ? P.A + (P.a_rows ? int64_t(P.a_rows[row]) : row) * P.lda : nullptr;
-  }
-  const unsigned short *Bhp[2], *Blp[2];
-  bool b_ok[2];
+  // Global loads are coalesced along K: four lanes read the 32 K values
+  // (64 B per plane) of one row, rows rs + 64 j (j < 2).  Rows past the
+  // matrix edge repeat its last row (their results are never stored).
+  const int rs = tid >> 2, kq = tid & 3;
+  const unsigned short *Ah[2], *Al[2], *Bh[2], *Bl[2];
 #pragma unroll
   for (int j = 0; j < 2; ++j) {
-    const int64_t row = c0 + brs + 64 * j;
-    b_ok[j] = row < P.N;
-    Bhp[j] = b_ok[j] ? P.Bhi + row * P.ldb : nullptr;
-    Blp[j] = b_ok[j] ? P.Blo + row * P.ldb : nullptr;
+    int64_t ra = r0 + rs + 64 * j;
+    ra = ra < P.M ? ra : P.M - 1;
+    if (P.a_rows) ra = P.a_rows[ra];
+    Ah[j] = P.Ahi + ra * P.lda;
+    Al[j] = P.Alo + ra * P.lda;
+    int64_t rb = c0 + rs + 64 * j;
+    rb = rb < P.N ? rb : P.N - 1;
+    Bh[j] = P.Bhi + rb * P.ldb;
+    Bl[j] = P.Blo + rb * P.ldb;
   }
-  auto slot = [](int rb, int c, int l) {
-    return (rb * (BK / 16) + c) * 64 + (l ^ (4 * c + 2 * (l >> 5)));
-  };
+  const int sc = kq >> 1, shalf = kq & 1;
+  int fs[2];
+#pragma unroll
+  for (int j = 0; j < 2; ++j) {
+    const int r = rs + 64 * j;
+    fs[j] = frag_slot(r >> 5, sc, (r & 31) + 32 * shalf);
+  }
 
-  float4 ra[4];
-  uint4 rbh[2], rbl[2];
+  uint4 rah[2], ral[2], rbh[2], rbl[2];
+  auto load8 = [&](const unsigned short* p, int64_t k) -> uint4 {
+    if (FAST) return *reinterpret_cast<const uint4*>(p + k);
+    unsigned short t[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) t[q] = (k + q < P.K) ? p[k + q] : (unsigned short)0;
+    uint4 v;
+    memcpy(&v, t, 16);
+    return v;
+  };
   auto gload = [&](int64_t k0) {
-    const int64_t ka = k0 + 4 * akq;
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      if (a_ok[j] && ka + 4 <= P.K && ((reinterpret_cast<uintptr_t>(Ap[j] + ka) & 15) == 0)) {
-        ra[j] = *reinterpret_cast<const float4*>(Ap[j] + ka);
-      } else {
-        float t[4];
-#pragma unroll
-        for (int q = 0; q < 4; ++q) t[q] = (a_ok[j] && ka + q < P.K) ? Ap[j][ka + q] : 0.f;
-        ra[j] = make_float4(t[0], t[1], t[2], t[3]);
-      }
-    }
-    const int64_t kb = k0 + 8 * bkq;
+    const int64_t k = k0 + 8 * kq;
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
-      if (b_ok[j] && kb + 8 <= P.K && ((reinterpret_cast<uintptr_t>(Bhp[j] + kb) & 15) == 0) &&
-          ((reinterpret_cast<uintptr_t>(Blp[j] + kb) & 15) == 0)) {
-        rbh[j] = *reinterpret_cast<const uint4*>(Bhp[j] + kb);
-        rbl[j] = *reinterpret_cast<const uint4*>(Blp[j] + kb);
-      } else {
-        unsigned short th[8], tl[8];
-#pragma unroll
-        for (int q = 0; q < 8; ++q) {
-          const bool ok = b_ok[j] && kb + q < P.K;
-          th[q] = ok ? Bhp[j][kb + q] : (unsigned short)0;
-          tl[q] = ok ? Blp[j][kb + q] : (unsigned short)0;
-        }
-        memcpy(&rbh[j], th, 16);
-        memcpy(&rbl[j], tl, 16);
-      }
+      rah[j] = load8(Ah[j], k);
+      ral[j] = load8(Al[j], k);
+      rbh[j] = load8(Bh[j], k);
+      rbl[j] = load8(Bl[j], k);
     }
   };
   auto lstore = [&](int st) {
-    // x = hi + lo, hi = bf16(x), lo = bf16(x - hi), both round-to-nearest-even
-    // (hardware conversions on gfx950)
-    const int ac = akq >> 2, ahalf = (akq >> 1) & 1, asub = akq & 1;
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const float f[4] = {ra[j].x, ra[j].y, ra[j].z, ra[j].w};
-      bf16x4 hi, lo;
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const __bf16 h = (__bf16)f[q];
-        hi[q] = h;
-        lo[q] = (__bf16)(f[q] - (float)h);
-      }
-      const int fs = slot(j, ac, ars + 32 * ahalf);
-      reinterpret_cast<bf16x4*>(&sAh[st][fs])[asub] = hi;
-      reinterpret_cast<bf16x4*>(&sAl[st][fs])[asub] = lo;
-    }
-    const int bc = bkq >> 1, bhalf = bkq & 1;
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
-      const int r = brs + 64 * j;
-      const int fs = slot(r >> 5, bc, (r & 31) + 32 * bhalf);
-      sBh[st][fs] = *reinterpret_cast<const bf16x8*>(&rbh[j]);
-      sBl[st][fs] = *reinterpret_cast<const bf16x8*>(&rbl[j]);
+      sAh[st][fs[j]] = __builtin_bit_cast(bf16x8, rah[j]);
+      sAl[st][fs[j]] = __builtin_bit_cast(bf16x8, ral[j]);
+      sBh[st][fs[j]] = __builtin_bit_cast(bf16x8, rbh[j]);
+      sBl[st][fs[j]] = __builtin_bit_cast(bf16x8, rbl[j]);
     }
   };
 
@@ -220,13 +192,13 @@ __global__ __launch_bounds__(NT, 2) void bf16x3_gemm(ProblemSet ps) {
       bf16x8 ah[2], al[2], bh[2], bl[2];
 #pragma unroll
       for (int i = 0; i < 2; ++i) {
-        const int f = slot(wm * 2 + i, c, lane);
+        const int f = frag_slot(wm * 2 + i, c, lane);
         ah[i] = sAh[st][f];
         al[i] = sAl[st][f];
       }
 #pragma unroll
       for (int j = 0; j < 2; ++j) {
-        const int f = slot(wn * 2 + j, c, lane);
+        const int f = frag_slot(wn * 2 + j, c, lane);
         bh[j] = sBh[st][f];
         bl[j] = sBl[st][f];
       }
@@ -273,9 +245,12 @@ __global__ __launch_bounds__(NT, 2) void bf16x3_gemm(ProblemSet ps) {
           const float k = float(double(acc[i][j][e]) * inv);
           const double a = P.alpha[c];
           rowsum[i][e] += a * double(pow_int(k, ps.zeta));
-          if (P.wt) {
+          if (P.wth) {
             const float g = ps.zeta > 1 ? float(ps.zeta) * pow_int(k, ps.zeta - 1) : 1.f;
-            P.wt[r * P.N + c] = float(a) * g;
+            const float v = float(a) * g;
+            const __bf16 h = (__bf16)v;
+            P.wth[r * P.N + c] = bf16_bits(h);
+            P.wtl[r * P.N + c] = bf16_bits((__bf16)(v - (float)h));
           }
         }
       }
@@ -318,6 +293,26 @@ __global__ __launch_bounds__(NT, 2) void bf16x3_gemm(ProblemSet ps) {
   }
 }
 
+template <int EPI>  // 0: GAP kernel epilogue, 1: store rows of G
+__global__ __launch_bounds__(NT, 2) void bf16x3_gemm(ProblemSet ps) {
+  // problem and tile of this block.  Workgroups are dealt to the 8 XCDs
+  // round robin; consecutive tiles (same row panel) go to the same XCD so
+  // the row panel is shared through that XCD's L2.
+  const int64_t per_xcd = (int64_t(gridDim.x) + 7) / 8;
+  const int64_t bid = int64_t(blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
+  if (bid >= ps.total_tiles) return;
+  int pi = 0;
+  while (pi + 1 < ps.n && bid >= ps.p[pi + 1].tile0) ++pi;
+  const Problem& P = ps.p[pi];
+  const int64_t local = bid - P.tile0;
+  const int64_t tm = local / P.tiles_n, tn = local - tm * P.tiles_n;
+  __shared__ __align__(16) GemmLds sm;
+  if (P.fast)
+    gemm_tile<EPI, true>(ps, P, tm * BM, tn * BN, sm);
+  else
+    gemm_tile<EPI, false>(ps, P, tm * BM, tn * BN, sm);
+}
+
 unsigned short bf16_rne_host(float x) {
   unsigned u;
   std::memcpy(&u, &x, 4);
@@ -329,6 +324,13 @@ float bf16_to_float_host(unsigned short h) {
   float f;
   std::memcpy(&f, &u, 4);
   return f;
+}
+
+bool aligned16(const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; }
+
+void finish_problem(Problem& P) {
+  P.fast = (P.K % BK == 0) && (P.lda % 8 == 0) && (P.ldb % 8 == 0) && aligned16(P.Ahi) &&
+           aligned16(P.Alo) && aligned16(P.Bhi) && aligned16(P.Blo);
 }
 
 void launch(ProblemSet& ps, int64_t total_tiles, int epi, hipStream_t stream) {
@@ -375,7 +377,7 @@ void prepare_mixed_sparse(MixedSparse& ms, const std::vector<double>& T, int64_t
 }
 
 void run_gap_mixed_kernel(const GapBlock<float>* blks, int n_blocks, const MixedSparse& ms,
-                          double* energy, float* const* wts, hipStream_t stream) {
+                          SplitPlanes xs, double* energy, const SplitPlanes* wts, hipStream_t stream) {
   for (int b0 = 0; b0 < n_blocks; b0 += kMaxProblems) {
     ProblemSet ps{};
     int64_t tiles = 0;
@@ -386,7 +388,8 @@ void run_gap_mixed_kernel(const GapBlock<float>* blks, int n_blocks, const Mixed
       P.M = blk.n_rows;
       P.N = blk.n_cols;
       P.K = blk.D;
-      P.A = blk.X;
+      P.Ahi = xs.hi;
+      P.Alo = xs.lo;
       P.lda = blk.D;
       P.a_rows = blk.row_idx;
       P.Bhi = ms.t_hi.get() + blk.col0 * blk.D;
@@ -399,7 +402,9 @@ void run_gap_mixed_kernel(const GapBlock<float>* blks, int n_blocks, const Mixed
       if (!blk.alpha) throw InvalidArgument("the GAP energy kernel needs weights");
       P.alpha = blk.alpha + blk.col0;
       P.energy = energy;
-      P.wt = wts ? wts[b] : nullptr;
+      P.wth = wts ? wts[b].hi : nullptr;
+      P.wtl = wts ? wts[b].lo : nullptr;
+      finish_problem(P);
       tiles += ((P.M + BM - 1) / BM) * P.tiles_n;
       ps.normalize = blk.normalize;
       ps.zeta = blk.zeta;
@@ -409,7 +414,7 @@ void run_gap_mixed_kernel(const GapBlock<float>* blks, int n_blocks, const Mixed
 }
 
 void run_gap_mixed_force(const GapBlock<float>* blks, int n_blocks, const MixedSparse& ms,
-                         float* const* wts, float* G, hipStream_t stream) {
+                         const SplitPlanes* wts, float* G, hipStream_t stream) {
   for (int b0 = 0; b0 < n_blocks; b0 += kMaxProblems) {
     ProblemSet ps{};
     int64_t tiles = 0;
@@ -420,7 +425,8 @@ void run_gap_mixed_force(const GapBlock<float>* blks, int n_blocks, const MixedS
       P.M = blk.n_rows;
       P.N = blk.D;
       P.K = blk.n_cols;
-      P.A = wts[b];
+      P.Ahi = wts[b].hi;
+      P.Alo = wts[b].lo;
       P.lda = blk.n_cols;
       P.a_rows = nullptr;
       P.Bhi = ms.tt_hi.get() + blk.col0;
@@ -431,6 +437,7 @@ void run_gap_mixed_force(const GapBlock<float>* blks, int n_blocks, const MixedS
       P.tiles_n = int((P.N + BN - 1) / BN);
       P.G = G;
       P.ldg = blk.D;
+      finish_problem(P);
       tiles += ((P.M + BM - 1) / BM) * P.tiles_n;
     }
     launch(ps, tiles, 1, stream);

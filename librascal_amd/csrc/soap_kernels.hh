@@ -46,6 +46,13 @@ template <class T>
 void launch_power_spectrum(const RadialDev& rad, int S, int64_t n_centres, const T* c, T* p,
                            double* norm2, hipStream_t stream, const SoapTiles& tiles = SoapTiles());
 
+// The same power spectrum written as the matrix-core operand planes
+// p = p_hi + p_lo (BF16 bit patterns, each rounded to nearest even) instead
+// of FP32 values; norm2 from the FP32 values.
+void launch_power_spectrum_split(const RadialDev& rad, int S, int64_t n_centres, const float* c,
+                                 unsigned short* p_hi, unsigned short* p_lo, double* norm2,
+                                 hipStream_t stream, const SoapTiles& tiles = SoapTiles());
+
 // Normalised copy X = p / |p| (or X = p), FP64 output for export.
 template <class T>
 void launch_normalise(int64_t n_centres, int64_t D, const T* p, const double* norm2, bool normalize,
@@ -62,6 +69,13 @@ void launch_backward(const RadialDev& rad, int S, const PairView& pv, const T* c
                      const double* norm2, bool normalize, const double* xg, double xg_scale,
                      const T* G, double* grad, double* centre_virial, hipStream_t stream,
                      const SoapTiles& tiles = SoapTiles());
+
+// launch_backward<float> with p given as BF16 operand planes
+void launch_backward_split(const RadialDev& rad, int S, const PairView& pv, const float* c,
+                           const unsigned short* p_hi, const unsigned short* p_lo,
+                           const double* norm2, bool normalize, const double* xg, double xg_scale,
+                           const float* G, double* grad, double* centre_virial, hipStream_t stream,
+                           const SoapTiles& tiles = SoapTiles());
 
 // out[s][:w] += sum of in[a][:w] over a in [seg_off[s], seg_off[s+1])
 void launch_segment_sum(const double* in, int w, const int64_t* seg_off, int n_seg, double* out,

@@ -503,10 +503,12 @@ __global__ __launch_bounds__(kThreads, (expansion_min_waves<T, KPT, DIRECT>())) 
 // staged in LDS and written out contiguously.  The expansion rows sit in
 // LDS with an odd stride (no bank conflicts between the tile rows).
 // Un-normalised; |p|^2 is returned per centre.
-template <class T>
+template <class T, bool SPLIT>
 __global__ __launch_bounds__(kThreads) void power_spectrum_kernel(int N, int L, int S, int KC,
                                                                   const T* __restrict__ c,
                                                                   T* __restrict__ p,
+                                                                  unsigned short* __restrict__ p_hi,
+                                                                  unsigned short* __restrict__ p_lo,
                                                                   double* __restrict__ norm2) {
   const int64_t i = blockIdx.x;
   const int L1 = L + 1, LL = L1 * L1, B = N * LL, LS = LL | 1;
@@ -530,7 +532,7 @@ __global__ __launch_bounds__(kThreads) void power_spectrum_kernel(int N, int L, 
     key_ab[k] = (a << 8) | b;
   }
   __syncthreads();
-  T* pi = p + i * int64_t(K) * KB;
+  const int64_t pbase = i * int64_t(K) * KB;
   const FastDiv fNT = fast_div(NT), fL1 = fast_div(L1);
   double ss = 0;
   for (int k0 = 0; k0 < K; k0 += KC) {
@@ -575,10 +577,17 @@ __global__ __launch_bounds__(kThreads) void power_spectrum_kernel(int N, int L, 
           if (n1 + u < N && n2 + v < N) sk[((n1 + u) * N + n2 + v) * L1 + l] = acc[u][v] * f;
     }
     __syncthreads();
-    T* out = pi + int64_t(k0) * KB;
+    const int64_t ob = pbase + int64_t(k0) * KB;
     for (int o = threadIdx.x; o < kc * KB; o += kThreads) {
       const T v = stage[o];
-      out[o] = v;
+      if (SPLIT) {
+        // the matrix-core operand planes: v = hi + lo (RNE BF16 each)
+        const __bf16 h = (__bf16)float(v);
+        p_hi[ob + o] = __builtin_bit_cast(unsigned short, h);
+        p_lo[ob + o] = __builtin_bit_cast(unsigned short, (__bf16)(float(v) - (float)h));
+      } else {
+        p[ob + o] = v;
+      }
       ss += double(v) * double(v);
     }
     __syncthreads();
@@ -627,10 +636,11 @@ __global__ void normalise_kernel(int64_t n, int64_t D, const T* __restrict__ p,
 //     reduced per centre (sparse_kernel_predict.hh:354-364).
 constexpr int kAdjPre = 8;  // key-block elements per thread held for the prefetch
 
-template <class T, bool DIRECT>
+template <class T, bool DIRECT, bool PSPLIT>
 __global__ __launch_bounds__(kThreads) void adjoint_kernel(
     RadialDev rad, int S, int PB, int c_in_lds, PairView pv, const T* __restrict__ c,
-    const T* __restrict__ p, const double* __restrict__ norm2, int normalize,
+    const T* __restrict__ p, const unsigned short* __restrict__ p_hi,
+    const unsigned short* __restrict__ p_lo, const double* __restrict__ norm2, int normalize,
     const double* __restrict__ xg, double xg_scale, const T* __restrict__ G,
     double* __restrict__ grad, double* __restrict__ centre_virial) {
   typedef typename Vec4<T>::type V4;
@@ -662,7 +672,13 @@ __global__ __launch_bounds__(kThreads) void adjoint_kernel(
   __shared__ double sh_ab[2];
 
   const T* Gi = G + i * D;
-  const T* Pi = p + i * D;
+  // p[o] of this centre (from the BF16 operand planes when split)
+  auto pval = [&](int64_t o) -> T {
+    if (PSPLIT)
+      return T(__builtin_bit_cast(float, unsigned(p_hi[i * D + o]) << 16) +
+               __builtin_bit_cast(float, unsigned(p_lo[i * D + o]) << 16));
+    return p[i * D + o];
+  };
   double alpha = 1.0, beta = 0.0;
   if (normalize) {
     const double n2 = norm2[i];
@@ -671,7 +687,7 @@ __global__ __launch_bounds__(kThreads) void adjoint_kernel(
       dotxg = xg_scale * xg[i];
     } else {
       double dot = 0;
-      for (int64_t o = tid; o < D; o += kThreads) dot += double(Pi[o]) * double(Gi[o]);
+      for (int64_t o = tid; o < D; o += kThreads) dot += double(pval(o)) * double(Gi[o]);
       for (int off = 32; off > 0; off >>= 1) dot += __shfl_down(dot, off, 64);
       if (lane == 0) red[wave][0] = dot;
       __syncthreads();
@@ -699,11 +715,11 @@ __global__ __launch_bounds__(kThreads) void adjoint_kernel(
   auto fetch = [&](int key) {
     if (!prefetch) return;
     const T* Gk = Gi + int64_t(key) * KB;
-    const T* Pk = Pi + int64_t(key) * KB;
+    const int64_t pk = int64_t(key) * KB;
 #pragma unroll
     for (int r = 0; r < kAdjPre; ++r) {
       const int o = tid + kThreads * r;
-      pre[r] = o < KB ? ta * Gk[o] - tb * Pk[o] : T(0);
+      pre[r] = o < KB ? ta * Gk[o] - tb * pval(pk + o) : T(0);
     }
   };
   fetch(0);
@@ -724,9 +740,9 @@ __global__ __launch_bounds__(kThreads) void adjoint_kernel(
       }
     } else {
       const T* Gk = Gi + int64_t(key) * KB;
-      const T* Pk = Pi + int64_t(key) * KB;
+      const int64_t pk = int64_t(key) * KB;
       for (int o = tid; o < KB; o += kThreads) {
-        const T g = ta * Gk[o] - tb * Pk[o];
+        const T g = ta * Gk[o] - tb * pval(pk + o);
         const int x = o / fNL1, rr = o - x * N * L1, y = rr / fL1, l = rr - y * L1;
         gA[(l * N + y) * N4 + x] = g;
         gB[(l * N + x) * N4 + y] = g;
@@ -1012,6 +1028,7 @@ void launch_expansion_k(const RadialDev& rad, int S, const PairView& pv, T* c,
 
 template <class T, bool DIRECT>
 void launch_adjoint_t(const RadialDev& rad, int S, const PairView& pv, const T* c, const T* p,
+                      const unsigned short* p_hi, const unsigned short* p_lo,
                       const double* norm2, bool normalize, const double* xg, double xg_scale,
                       const T* G, double* grad, double* centre_virial, hipStream_t stream,
                       const SoapTiles& tiles) {
@@ -1030,10 +1047,19 @@ void launch_adjoint_t(const RadialDev& rad, int S, const PairView& pv, const T* 
   else
     while (PB > 1 && pair_batch_bytes(PB, LL, F, sizeof(T), DIRECT, true) > batch_room) --PB;
   const size_t smem = fixed + std::max(region, pair_batch_bytes(PB, LL, F, sizeof(T), DIRECT, true));
-  auto kern = adjoint_kernel<T, DIRECT>;
-  allow_lds(kern, smem);
-  hipLaunchKernelGGL(kern, dim3(unsigned(pv.n_centres)), dim3(kThreads), smem, stream, rad, S, PB,
-                     c_in_lds, pv, c, p, norm2, int(normalize), xg, xg_scale, G, grad, centre_virial);
+  if (p_hi) {
+    auto kern = adjoint_kernel<T, DIRECT, true>;
+    allow_lds(kern, smem);
+    hipLaunchKernelGGL(kern, dim3(unsigned(pv.n_centres)), dim3(kThreads), smem, stream, rad, S, PB,
+                       c_in_lds, pv, c, p, p_hi, p_lo, norm2, int(normalize), xg, xg_scale, G, grad,
+                       centre_virial);
+  } else {
+    auto kern = adjoint_kernel<T, DIRECT, false>;
+    allow_lds(kern, smem);
+    hipLaunchKernelGGL(kern, dim3(unsigned(pv.n_centres)), dim3(kThreads), smem, stream, rad, S, PB,
+                       c_in_lds, pv, c, p, p_hi, p_lo, norm2, int(normalize), xg, xg_scale, G, grad,
+                       centre_virial);
+  }
   RGPU_LAUNCH_CHECK();
 }
 
@@ -1065,8 +1091,9 @@ void launch_expansion(const RadialDev& rad, int S, const PairView& pv, T* c,
 }
 
 template <class T>
-void launch_power_spectrum(const RadialDev& rad, int S, int64_t n, const T* c, T* p, double* norm2,
-                           hipStream_t stream, const SoapTiles& tiles) {
+void launch_power_spectrum_impl(const RadialDev& rad, int S, int64_t n, const T* c, T* p,
+                                unsigned short* p_hi, unsigned short* p_lo, double* norm2,
+                                hipStream_t stream, const SoapTiles& tiles) {
   if (n == 0) return;
   const int K = S * (S + 1) / 2;
   if (S > kMaxSpecies) throw InvalidArgument("at most 64 species");
@@ -1076,11 +1103,30 @@ void launch_power_spectrum(const RadialDev& rad, int S, int64_t n, const T* c, T
   if (tiles.kc > 0) KC = tiles.kc;
   KC = std::max(1, std::min(KC, K));
   const size_t smem = cbytes + size_t(KC) * kb + sizeof(int) * size_t(K);
-  auto kern = power_spectrum_kernel<T>;
-  allow_lds(kern, smem);
-  hipLaunchKernelGGL(kern, dim3(unsigned(n)), dim3(kThreads), smem, stream, rad.N, rad.L, S, KC, c, p,
-                     norm2);
+  if (p_hi) {
+    auto kern = power_spectrum_kernel<T, true>;
+    allow_lds(kern, smem);
+    hipLaunchKernelGGL(kern, dim3(unsigned(n)), dim3(kThreads), smem, stream, rad.N, rad.L, S, KC, c,
+                       p, p_hi, p_lo, norm2);
+  } else {
+    auto kern = power_spectrum_kernel<T, false>;
+    allow_lds(kern, smem);
+    hipLaunchKernelGGL(kern, dim3(unsigned(n)), dim3(kThreads), smem, stream, rad.N, rad.L, S, KC, c,
+                       p, p_hi, p_lo, norm2);
+  }
   RGPU_LAUNCH_CHECK();
+}
+
+template <class T>
+void launch_power_spectrum(const RadialDev& rad, int S, int64_t n, const T* c, T* p, double* norm2,
+                           hipStream_t stream, const SoapTiles& tiles) {
+  launch_power_spectrum_impl<T>(rad, S, n, c, p, nullptr, nullptr, norm2, stream, tiles);
+}
+
+void launch_power_spectrum_split(const RadialDev& rad, int S, int64_t n, const float* c,
+                                 unsigned short* p_hi, unsigned short* p_lo, double* norm2,
+                                 hipStream_t stream, const SoapTiles& tiles) {
+  launch_power_spectrum_impl<float>(rad, S, n, c, nullptr, p_hi, p_lo, norm2, stream, tiles);
 }
 
 template <class T>
@@ -1101,11 +1147,25 @@ void launch_backward(const RadialDev& rad, int S, const PairView& pv, const T* c
                      const SoapTiles& tiles) {
   if (pv.n_centres == 0) return;
   if (rad.use_spline)
-    launch_adjoint_t<T, false>(rad, S, pv, c, p, norm2, normalize, xg, xg_scale, G, grad, centre_virial,
-                               stream, tiles);
+    launch_adjoint_t<T, false>(rad, S, pv, c, p, nullptr, nullptr, norm2, normalize, xg, xg_scale, G,
+                               grad, centre_virial, stream, tiles);
   else
-    launch_adjoint_t<T, true>(rad, S, pv, c, p, norm2, normalize, xg, xg_scale, G, grad, centre_virial,
-                              stream, tiles);
+    launch_adjoint_t<T, true>(rad, S, pv, c, p, nullptr, nullptr, norm2, normalize, xg, xg_scale, G,
+                              grad, centre_virial, stream, tiles);
+}
+
+void launch_backward_split(const RadialDev& rad, int S, const PairView& pv, const float* c,
+                           const unsigned short* p_hi, const unsigned short* p_lo,
+                           const double* norm2, bool normalize, const double* xg, double xg_scale,
+                           const float* G, double* grad, double* centre_virial, hipStream_t stream,
+                           const SoapTiles& tiles) {
+  if (pv.n_centres == 0) return;
+  if (rad.use_spline)
+    launch_adjoint_t<float, false>(rad, S, pv, c, nullptr, p_hi, p_lo, norm2, normalize, xg, xg_scale,
+                                   G, grad, centre_virial, stream, tiles);
+  else
+    launch_adjoint_t<float, true>(rad, S, pv, c, nullptr, p_hi, p_lo, norm2, normalize, xg, xg_scale,
+                                  G, grad, centre_virial, stream, tiles);
 }
 
 void launch_pair_cutoff(const RadialDev& rad, const int64_t* n_pairs, int64_t cap, const double* dist,
