@@ -767,9 +767,13 @@ void autotune_t(Context& c) {
     RGPU_HIP(hipMemsetAsync(grad, 0, sizeof(double) * 3 * n, c.stream));
     best = 1e30f;
     pick = 0;
-    for (int pb : {2, 4, 6, 8, 12, 16}) {
+    int pick_res = -1;
+    for (int res : {1, 0})
+    for (int pb : {2, 4, 6, 8, 12, 16, 24, 32}) {
+      if (res == 1 && !split) continue;  // the resident variant is FP32 only
       SoapTiles cand = t;
       cand.pb_adj = pb;
+      cand.adj_resident = res;
       float ms;
       try {
         ms = time_ms([&] {
@@ -783,9 +787,10 @@ void autotune_t(Context& c) {
       } catch (const InvalidArgument&) {
         continue;
       }
-      if (ms < best) best = ms, pick = pb;
+      if (ms < best) best = ms, pick = pb, pick_res = res;
     }
     t.pb_adj = pick;
+    t.adj_resident = pick_res;
     RGPU_HIP(hipStreamSynchronize(c.stream));
   }
   c.ev_pool.push_back(a);
@@ -841,6 +846,24 @@ int rgpu_autotune(rgpu_context* ctx) {
   });
 }
 
+int rgpu_set_tiles(rgpu_context* ctx, int pairs_fwd, int keys_ps, int pairs_adj,
+                   int adjoint_resident) {
+  return guarded([&] {
+    if (!ctx) throw InvalidArgument("null context");
+    if (pairs_fwd > 64 || pairs_adj > 64) throw InvalidArgument("at most 64 pairs per batch");
+    if (adjoint_resident < -1 || adjoint_resident > 1)
+      throw InvalidArgument("adjoint_resident must be -1, 0 or 1");
+    Context& c = ctx->c;
+    SoapTiles& t = c.tile();
+    t.pb_fwd = std::max(pairs_fwd, 0);
+    t.kc = std::max(keys_ps, 0);
+    t.pb_adj = std::max(pairs_adj, 0);
+    t.adj_resident = adjoint_resident;
+    c.have_rep = false;
+    c.drop_graph();
+  });
+}
+
 int rgpu_enable_graphs(rgpu_context* ctx, int enable) {
   return guarded([&] {
     if (!ctx) throw InvalidArgument("null context");
@@ -892,6 +915,7 @@ int rgpu_get_info(rgpu_context* ctx, int key, int64_t* value) {
       case RGPU_INFO_TILE_PAIRS_FWD: *value = c.tile().pb_fwd; break;
       case RGPU_INFO_TILE_KEYS_PS: *value = c.tile().kc; break;
       case RGPU_INFO_TILE_PAIRS_ADJ: *value = c.tile().pb_adj; break;
+      case RGPU_INFO_TILE_ADJ_RESIDENT: *value = c.tile().adj_resident; break;
       default: throw InvalidArgument("unknown info key");
     }
   });

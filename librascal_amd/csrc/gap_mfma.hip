@@ -33,6 +33,7 @@ typedef __attribute__((__vector_size__(16 * sizeof(float)))) float f32x16;
 
 constexpr int BM = 128, BN = 128, BK = 32, NT = 256;
 constexpr int kMaxProblems = 16;
+constexpr int kGroupM = 8;  // row panels per tile group
 
 __device__ __forceinline__ unsigned short bf16_bits(__bf16 h) {
   return __builtin_bit_cast(unsigned short, h);
@@ -49,7 +50,7 @@ struct Problem {
   int64_t ldb;
   const int* c_rows;           // output row ids (centre indices)
   int64_t tile0;               // first linear tile of this problem
-  int tiles_n;                 // tiles along N
+  int tiles_m, tiles_n;        // tiles along M and N
   int fast;                    // K % 32 == 0 and every row 16-byte aligned
   // kernel epilogue
   const double* norm2;
@@ -296,8 +297,11 @@ __device__ __forceinline__ void gemm_tile(const ProblemSet& ps, const Problem& P
 template <int EPI>  // 0: GAP kernel epilogue, 1: store rows of G
 __global__ __launch_bounds__(NT, 2) void bf16x3_gemm(ProblemSet ps) {
   // problem and tile of this block.  Workgroups are dealt to the 8 XCDs
-  // round robin; consecutive tiles (same row panel) go to the same XCD so
-  // the row panel is shared through that XCD's L2.
+  // round robin; each XCD takes a contiguous range of linear tiles, and the
+  // linear order walks groups of kGroupM row panels column by column, so
+  // the tiles resident on one XCD at a time (about 64) form a square of
+  // row and column panels that share their operand panels through the
+  // XCD's L2.
   const int64_t per_xcd = (int64_t(gridDim.x) + 7) / 8;
   const int64_t bid = int64_t(blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
   if (bid >= ps.total_tiles) return;
@@ -305,7 +309,10 @@ __global__ __launch_bounds__(NT, 2) void bf16x3_gemm(ProblemSet ps) {
   while (pi + 1 < ps.n && bid >= ps.p[pi + 1].tile0) ++pi;
   const Problem& P = ps.p[pi];
   const int64_t local = bid - P.tile0;
-  const int64_t tm = local / P.tiles_n, tn = local - tm * P.tiles_n;
+  const int64_t per_group = int64_t(kGroupM) * P.tiles_n;
+  const int64_t grp = local / per_group, in = local - grp * per_group;
+  const int64_t gm = std::min<int64_t>(kGroupM, P.tiles_m - grp * kGroupM);
+  const int64_t tn = in / gm, tm = grp * kGroupM + (in - tn * gm);
   __shared__ __align__(16) GemmLds sm;
   if (P.fast)
     gemm_tile<EPI, true>(ps, P, tm * BM, tn * BN, sm);
@@ -397,6 +404,7 @@ void run_gap_mixed_kernel(const GapBlock<float>* blks, int n_blocks, const Mixed
       P.ldb = blk.D;
       P.c_rows = blk.row_idx;
       P.tile0 = tiles;
+      P.tiles_m = int((P.M + BM - 1) / BM);
       P.tiles_n = int((P.N + BN - 1) / BN);
       P.norm2 = blk.norm2;
       if (!blk.alpha) throw InvalidArgument("the GAP energy kernel needs weights");
@@ -405,7 +413,7 @@ void run_gap_mixed_kernel(const GapBlock<float>* blks, int n_blocks, const Mixed
       P.wth = wts ? wts[b].hi : nullptr;
       P.wtl = wts ? wts[b].lo : nullptr;
       finish_problem(P);
-      tiles += ((P.M + BM - 1) / BM) * P.tiles_n;
+      tiles += int64_t(P.tiles_m) * P.tiles_n;
       ps.normalize = blk.normalize;
       ps.zeta = blk.zeta;
     }
@@ -434,11 +442,12 @@ void run_gap_mixed_force(const GapBlock<float>* blks, int n_blocks, const MixedS
       P.ldb = ms.M;
       P.c_rows = blk.row_idx;
       P.tile0 = tiles;
+      P.tiles_m = int((P.M + BM - 1) / BM);
       P.tiles_n = int((P.N + BN - 1) / BN);
       P.G = G;
       P.ldg = blk.D;
       finish_problem(P);
-      tiles += ((P.M + BM - 1) / BM) * P.tiles_n;
+      tiles += int64_t(P.tiles_m) * P.tiles_n;
     }
     launch(ps, tiles, 1, stream);
   }

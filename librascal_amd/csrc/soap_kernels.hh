@@ -33,6 +33,9 @@ struct SoapTiles {
   int pb_fwd = 0;  // pairs staged per batch in the expansion (<= 64)
   int kc = 0;      // key blocks staged per pass in the power spectrum
   int pb_adj = 0;  // pairs staged per batch in the adjoint (<= 64)
+  // adjoint variant: -1 chooses (dL/dp resident in LDS when it fits),
+  // 0 one LDS pass per key block, 1 resident (when it fits)
+  int adj_resident = -1;
 };
 
 // c[i][s][n][l*l+l+m] (finalised) and the per-centre key mask.

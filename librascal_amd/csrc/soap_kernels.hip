@@ -619,6 +619,97 @@ __global__ void normalise_kernel(int64_t n, int64_t D, const T* __restrict__ p,
   }
 }
 
+// Step 3 of the adjoint for centre i: per-pair contraction of dL/dc (in
+// LDS) with the pair gradients, force atomics and the centre virial.  The
+// pair batches are staged in `region`.
+template <class T, bool DIRECT>
+__device__ __forceinline__ void adjoint_pairs(const RadialDev& rad, int S, int PB, const PairView& pv,
+                                              const T* yc, const T* dc, unsigned char* region,
+                                              int64_t i, double* __restrict__ grad,
+                                              double* __restrict__ centre_virial,
+                                              double (*red)[9]) {
+  const int N = rad.N, L1 = rad.L1, LL = rad.LL, F = N * L1, B = N * LL;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  PairBatch<T> pb = carve_batch<T, DIRECT, true>(region, PB, LL, F);
+  const int64_t p0 = pv.centre_off[i] + 1;
+  const int np = int(pv.centre_off[i + 1] - p0);
+  double fi[3] = {0, 0, 0}, v6[6] = {0, 0, 0, 0, 0, 0};
+  for (int q0 = 0; q0 < np; q0 += PB) {
+    const int nb = min(PB, np - q0);
+    stage_pairs<T, DIRECT, true>(rad, yc, pv, p0 + q0, nb, pb, nullptr, nullptr, nullptr);
+    for (int q = wave; q < nb; q += kWaves) {
+      const T* W = dc + pb.sp[q] * B;
+      const T* Yq = pb.Y + q * LL;
+      const T* dYq = pb.dY + q * 3 * LL;
+      const T* Rq = pb.R + q * F;
+      const T* dRq = pb.dR + q * F;
+      const T fc = T(pb.fc[q]), dfc = T(pb.dfc[q]);
+      T A = T(0), Bx = T(0), By = T(0), Bz = T(0);
+      for (int lm = lane; lm < LL; lm += 64) {
+        const int l = l_of_lm(lm);
+        T Dn = T(0), Ddn = T(0);
+        for (int n = 0; n < N; ++n) {
+          const T wv = W[n * LL + lm];
+          Dn += wv * Rq[n * L1 + l];
+          Ddn += wv * dRq[n * L1 + l];
+        }
+        A += (Ddn * fc + Dn * dfc) * Yq[lm];
+        Bx += Dn * dYq[lm];
+        By += Dn * dYq[LL + lm];
+        Bz += Dn * dYq[2 * LL + lm];
+      }
+      for (int off = 32; off > 0; off >>= 1) {
+        A += __shfl_down(A, off, 64);
+        Bx += __shfl_down(Bx, off, 64);
+        By += __shfl_down(By, off, 64);
+        Bz += __shfl_down(Bz, off, 64);
+      }
+      if (lane == 0) {
+        const double d = pb.dist[q];
+        const double sc = pb.fc[q] / d;
+        const int64_t pidx = p0 + q0 + q;
+        const double ux = pv.pair_vec[3 * pidx] / d, uy = pv.pair_vec[3 * pidx + 1] / d,
+                     uz = pv.pair_vec[3 * pidx + 2] / d;
+        const double g[3] = {double(A) * ux + sc * double(Bx), double(A) * uy + sc * double(By),
+                             double(A) * uz + sc * double(Bz)};
+        const int j = pv.pair_j[pidx];
+        unsafeAtomicAdd(grad + 3 * j, g[0]);
+        unsafeAtomicAdd(grad + 3 * j + 1, g[1]);
+        unsafeAtomicAdd(grad + 3 * j + 2, g[2]);
+        if (j != int(i)) {
+          fi[0] -= g[0];
+          fi[1] -= g[1];
+          fi[2] -= g[2];
+        }
+        // r_ji = r_i - r_j(image) = -pair vector
+        const double rx = -pv.pair_vec[3 * pidx], ry = -pv.pair_vec[3 * pidx + 1],
+                     rz = -pv.pair_vec[3 * pidx + 2];
+        v6[0] += rx * g[0];
+        v6[1] += ry * g[1];
+        v6[2] += rz * g[2];
+        v6[3] += ry * g[2];
+        v6[4] += rz * g[0];
+        v6[5] += rx * g[1];
+      }
+    }
+    __syncthreads();
+  }
+  if (lane == 0) {
+    for (int k = 0; k < 3; ++k) red[wave][k] = fi[k];
+    for (int k = 0; k < 6; ++k) red[wave][3 + k] = v6[k];
+  }
+  __syncthreads();
+  if (tid < 9) {
+    double t = 0;
+    for (int w = 0; w < kWaves; ++w) t += red[w][tid];
+    if (tid < 3) {
+      if (t != 0.0) unsafeAtomicAdd(grad + 3 * i + tid, t);
+    } else {
+      centre_virial[6 * i + tid - 3] = t;
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Fused adjoint.  Given G = dL/dX (X = p/|p| the exported features):
 //  1. dL/dp = G/|p| - (X.G) p/|p|^2 (the transpose of the normalisation
@@ -645,7 +736,7 @@ __global__ __launch_bounds__(kThreads) void adjoint_kernel(
     double* __restrict__ grad, double* __restrict__ centre_virial) {
   typedef typename Vec4<T>::type V4;
   const int64_t i = blockIdx.x;
-  const int N = rad.N, L1 = rad.L1, LL = rad.LL, F = N * L1, B = N * LL, E = S * B;
+  const int N = rad.N, L1 = rad.L1, LL = rad.LL, B = N * LL, E = S * B;
   const int N4 = (N + 3) & ~3;
   const int K = S * (S + 1) / 2;
   const int KB = N * N * L1;
@@ -782,84 +873,188 @@ __global__ __launch_bounds__(kThreads) void adjoint_kernel(
   }
 
   // per-pair contraction; the batch reuses the region after dL/dc
-  PairBatch<T> pb = carve_batch<T, DIRECT, true>(region, PB, LL, F);
-  const int64_t p0 = pv.centre_off[i] + 1;
-  const int np = int(pv.centre_off[i + 1] - p0);
-  double fi[3] = {0, 0, 0}, v6[6] = {0, 0, 0, 0, 0, 0};
-  for (int q0 = 0; q0 < np; q0 += PB) {
-    const int nb = min(PB, np - q0);
-    stage_pairs<T, DIRECT, true>(rad, yc, pv, p0 + q0, nb, pb, nullptr, nullptr, nullptr);
-    for (int q = wave; q < nb; q += kWaves) {
-      const T* W = dc + pb.sp[q] * B;
-      const T* Yq = pb.Y + q * LL;
-      const T* dYq = pb.dY + q * 3 * LL;
-      const T* Rq = pb.R + q * F;
-      const T* dRq = pb.dR + q * F;
-      const T fc = T(pb.fc[q]), dfc = T(pb.dfc[q]);
-      T A = T(0), Bx = T(0), By = T(0), Bz = T(0);
-      for (int lm = lane; lm < LL; lm += 64) {
-        const int l = l_of_lm(lm);
-        T Dn = T(0), Ddn = T(0);
-        for (int n = 0; n < N; ++n) {
-          const T wv = W[n * LL + lm];
-          Dn += wv * Rq[n * L1 + l];
-          Ddn += wv * dRq[n * L1 + l];
-        }
-        A += (Ddn * fc + Dn * dfc) * Yq[lm];
-        Bx += Dn * dYq[lm];
-        By += Dn * dYq[LL + lm];
-        Bz += Dn * dYq[2 * LL + lm];
+  adjoint_pairs<T, DIRECT>(rad, S, PB, pv, yc, dc, region, i, grad, centre_virial, red);
+}
+
+// Adjoint with the whole dL/dp of the centre resident in LDS (used when it
+// fits beside the expansion coefficients).  Same mathematics as
+// adjoint_kernel; the dL/dc contraction runs as one pass over every key
+// with the outputs in registers instead of one LDS pass per key:
+//  1. g = alpha G - beta p staged for all keys, natural layout
+//     [key][x][y][l]; diagonal keys then replaced by g + g^T;
+//  2. thread unit (a, 4 radial channels n, lm):
+//       dL/dc_a[n][lm] = (2l+1)^{-1/2} sum_b w_ab sum_y g^(ab)[n][y][l] c_b[y][lm]
+//     (g^(ab) read transposed for a > b, w_ab = sqrt 2 for a != b), written
+//     over c once every unit has read it;
+//  3. the pair phase of adjoint_kernel, its batches over the g region.
+// UPT = units per thread (ceil(S * ceil(N/4) * LL / 256)).
+template <class T, bool DIRECT, bool PSPLIT, int UPT>
+__global__ __launch_bounds__(kThreads) void adjoint_resident_kernel(
+    RadialDev rad, int S, int PB, PairView pv, const T* __restrict__ c, const T* __restrict__ p,
+    const unsigned short* __restrict__ p_hi, const unsigned short* __restrict__ p_lo,
+    const double* __restrict__ norm2, int normalize, const double* __restrict__ xg, double xg_scale,
+    const T* __restrict__ G, double* __restrict__ grad, double* __restrict__ centre_virial) {
+  typedef typename Vec4<T>::type V4;
+  const int64_t i = blockIdx.x;
+  const int N = rad.N, L1 = rad.L1, LL = rad.LL, B = N * LL, E = S * B;
+  const int K = S * (S + 1) / 2;
+  const int KB = N * N * L1;
+  const int D = K * KB;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  extern __shared__ __align__(64) unsigned char smem_raw[];
+  T* yc = reinterpret_cast<T*>(smem_raw);
+  T* cd = reinterpret_cast<T*>(smem_raw + ylm_table_bytes<T>(L1));  // c, then dL/dc
+  unsigned char* region = reinterpret_cast<unsigned char*>(cd) + align_up(sizeof(T) * E, 64);
+  T* gl = reinterpret_cast<T*>(region);
+  __shared__ double red[kWaves][9];
+  __shared__ double sh_ab[2];
+  load_ylm_table<T>(rad, yc);
+  for (int e = tid; e < E; e += kThreads) cd[e] = c[i * int64_t(E) + e];
+
+  const T* Gi = G + i * int64_t(D);
+  auto pval = [&](int64_t o) -> T {
+    if (PSPLIT)
+      return T(__builtin_bit_cast(float, unsigned(p_hi[i * D + o]) << 16) +
+               __builtin_bit_cast(float, unsigned(p_lo[i * D + o]) << 16));
+    return p[i * D + o];
+  };
+  double alpha = 1.0, beta = 0.0;
+  if (normalize) {
+    const double n2 = norm2[i];
+    double dotxg = 0;
+    if (xg) {
+      dotxg = xg_scale * xg[i];
+    } else {
+      double dot = 0;
+      for (int o = tid; o < D; o += kThreads) dot += double(pval(o)) * double(Gi[o]);
+      for (int off = 32; off > 0; off >>= 1) dot += __shfl_down(dot, off, 64);
+      if (lane == 0) red[wave][0] = dot;
+      __syncthreads();
+      if (tid == 0) {
+        double t = 0;
+        for (int w = 0; w < kWaves; ++w) t += red[w][0];
+        sh_ab[0] = t;
       }
-      for (int off = 32; off > 0; off >>= 1) {
-        A += __shfl_down(A, off, 64);
-        Bx += __shfl_down(Bx, off, 64);
-        By += __shfl_down(By, off, 64);
-        Bz += __shfl_down(Bz, off, 64);
-      }
-      if (lane == 0) {
-        const double d = pb.dist[q];
-        const double sc = pb.fc[q] / d;
-        const int64_t pidx = p0 + q0 + q;
-        const double ux = pv.pair_vec[3 * pidx] / d, uy = pv.pair_vec[3 * pidx + 1] / d,
-                     uz = pv.pair_vec[3 * pidx + 2] / d;
-        const double g[3] = {double(A) * ux + sc * double(Bx), double(A) * uy + sc * double(By),
-                             double(A) * uz + sc * double(Bz)};
-        const int j = pv.pair_j[pidx];
-        unsafeAtomicAdd(grad + 3 * j, g[0]);
-        unsafeAtomicAdd(grad + 3 * j + 1, g[1]);
-        unsafeAtomicAdd(grad + 3 * j + 2, g[2]);
-        if (j != int(i)) {
-          fi[0] -= g[0];
-          fi[1] -= g[1];
-          fi[2] -= g[2];
-        }
-        // r_ji = r_i - r_j(image) = -pair vector
-        const double rx = -pv.pair_vec[3 * pidx], ry = -pv.pair_vec[3 * pidx + 1],
-                     rz = -pv.pair_vec[3 * pidx + 2];
-        v6[0] += rx * g[0];
-        v6[1] += ry * g[1];
-        v6[2] += rz * g[2];
-        v6[3] += ry * g[2];
-        v6[4] += rz * g[0];
-        v6[5] += rx * g[1];
-      }
+      __syncthreads();
+      dotxg = n2 > 0 ? sh_ab[0] / sqrt(n2) : 0.0;
     }
-    __syncthreads();
+    if (n2 > 0) {
+      alpha = 1.0 / sqrt(n2);
+      beta = dotxg / n2;
+    }
   }
-  if (lane == 0) {
-    for (int k = 0; k < 3; ++k) red[wave][k] = fi[k];
-    for (int k = 0; k < 6; ++k) red[wave][3 + k] = v6[k];
+  const T ta = T(alpha), tb = T(beta);
+
+  // 1. stage g (four elements per thread and step when rows are 16-byte
+  //    aligned, D % 4 == 0)
+  if ((D & 3) == 0) {
+    const V4* G4 = reinterpret_cast<const V4*>(Gi);
+    V4* g4 = reinterpret_cast<V4*>(gl);
+    for (int q = tid; q < (D >> 2); q += kThreads) {
+      V4 gv = G4[q];
+      V4 pv4;
+      if (PSPLIT) {
+        const uint2 h = reinterpret_cast<const uint2*>(p_hi + i * int64_t(D))[q];
+        const uint2 lo = reinterpret_cast<const uint2*>(p_lo + i * int64_t(D))[q];
+        pv4[0] = T(__builtin_bit_cast(float, h.x << 16) + __builtin_bit_cast(float, lo.x << 16));
+        pv4[1] = T(__builtin_bit_cast(float, h.x & 0xffff0000u) +
+                   __builtin_bit_cast(float, lo.x & 0xffff0000u));
+        pv4[2] = T(__builtin_bit_cast(float, h.y << 16) + __builtin_bit_cast(float, lo.y << 16));
+        pv4[3] = T(__builtin_bit_cast(float, h.y & 0xffff0000u) +
+                   __builtin_bit_cast(float, lo.y & 0xffff0000u));
+      } else {
+        pv4 = reinterpret_cast<const V4*>(p + i * int64_t(D))[q];
+      }
+      g4[q] = ta * gv - tb * pv4;
+    }
+  } else {
+    for (int o = tid; o < D; o += kThreads) gl[o] = ta * Gi[o] - tb * pval(o);
   }
   __syncthreads();
-  if (tid < 9) {
-    double t = 0;
-    for (int w = 0; w < kWaves; ++w) t += red[w][tid];
-    if (tid < 3) {
-      if (t != 0.0) unsafeAtomicAdd(grad + 3 * i + tid, t);
-    } else {
-      centre_virial[6 * i + tid - 3] = t;
+  // diagonal keys: g[x][y] + g[y][x] on both triangles
+  {
+    const int tri = N * (N + 1) / 2 * L1;
+    for (int it = tid; it < S * tri; it += kThreads) {
+      const int a = it / tri;
+      int r = it - a * tri;
+      const int l = r % L1;
+      r /= L1;
+      int x = 0;
+      while (r >= N - x) {
+        r -= N - x;
+        ++x;
+      }
+      const int y = x + r;
+      T* gk = gl + (a * S - a * (a - 1) / 2) * KB + l;
+      const T v = gk[(x * N + y) * L1] + gk[(y * N + x) * L1];
+      gk[(x * N + y) * L1] = v;
+      gk[(y * N + x) * L1] = v;
     }
   }
+  __syncthreads();
+
+  // 2. dL/dc units with register accumulators
+  const int NQ = (N + 3) >> 2;
+  const int units = S * NQ * LL;
+  const int sx = N * L1;  // stride of the first radial index of a key block
+  T acc[UPT][4];
+#pragma unroll
+  for (int k = 0; k < UPT; ++k) {
+    acc[k][0] = acc[k][1] = acc[k][2] = acc[k][3] = T(0);
+    const int u = tid + kThreads * k;
+    if (u < units) {
+      const int anq = u / LL, lm = u - anq * LL;
+      const int a = anq / NQ, n0 = 4 * (anq - a * NQ);
+      const int l = l_of_lm(lm);
+      int na[4];
+#pragma unroll
+      for (int v = 0; v < 4; ++v) na[v] = min(n0 + v, N - 1);
+      for (int b = 0; b < S; ++b) {
+        const int lo = min(a, b), hi = max(a, b);
+        const T* gk = gl + (lo * S - lo * (lo - 1) / 2 + (hi - lo)) * KB + l;
+        // g^(ab)[n][y] for a <= b, g^(ba)[y][n] for a > b
+        const int sn = a <= b ? sx : L1, sy = a <= b ? L1 : sx;
+        const T* cb = cd + b * B + lm;
+        T s0 = T(0), s1 = T(0), s2 = T(0), s3 = T(0);
+        const T* g0 = gk + na[0] * sn;
+        const T* g1 = gk + na[1] * sn;
+        const T* g2 = gk + na[2] * sn;
+        const T* g3 = gk + na[3] * sn;
+        for (int y = 0; y < N; ++y) {
+          const T cv = cb[y * LL];
+          const int oy = y * sy;
+          s0 += g0[oy] * cv;
+          s1 += g1[oy] * cv;
+          s2 += g2[oy] * cv;
+          s3 += g3[oy] * cv;
+        }
+        const T w = a == b ? T(1) : T(1.41421356237309504880);
+        acc[k][0] += w * s0;
+        acc[k][1] += w * s1;
+        acc[k][2] += w * s2;
+        acc[k][3] += w * s3;
+      }
+      const T f = T(1) / sqrt(T(2 * l + 1));
+#pragma unroll
+      for (int v = 0; v < 4; ++v) acc[k][v] *= f;
+    }
+  }
+  __syncthreads();  // every unit has read c
+#pragma unroll
+  for (int k = 0; k < UPT; ++k) {
+    const int u = tid + kThreads * k;
+    if (u < units) {
+      const int anq = u / LL, lm = u - anq * LL;
+      const int a = anq / NQ, n0 = 4 * (anq - a * NQ);
+#pragma unroll
+      for (int v = 0; v < 4; ++v)
+        if (n0 + v < N) cd[a * B + (n0 + v) * LL + lm] = acc[k][v];
+    }
+  }
+  __syncthreads();
+
+  // 3. pairs; the batches reuse the g region
+  adjoint_pairs<T, DIRECT>(rad, S, PB, pv, yc, cd, region, i, grad, centre_virial, red);
 }
 
 // n_pairs is read on the device (the last entry of the pair offsets), so one
@@ -972,6 +1167,7 @@ __global__ __launch_bounds__(kThreads) void ylm_eval_kernel(RadialDev rad, const
 constexpr size_t kFwdBudget = 32 * 1024;
 constexpr size_t kPsBudget = 48 * 1024;
 constexpr size_t kAdjBudget = 64 * 1024;
+constexpr size_t kResidentBudget = 80 * 1024;  // two workgroups per CU
 
 // LDS limit per workgroup of the current device (queried once per device;
 // launches may be captured into graphs, where device queries are avoided)
@@ -1036,6 +1232,45 @@ void launch_adjoint_t(const RadialDev& rad, int S, const PairView& pv, const T* 
   const int N4 = (N + 3) & ~3;
   const size_t yt = ylm_table_bytes<T>(L1);
   const size_t blk = align_up(sizeof(T) * size_t(E), 64);
+  if constexpr (sizeof(T) == 4) {
+    // dL/dp of a centre resident in LDS beside c
+    const int K = S * (S + 1) / 2;
+    const size_t dbytes = align_up(sizeof(T) * size_t(K) * N * N * L1, 64);
+    const int upt = (S * ((N + 3) / 4) * LL + kThreads - 1) / kThreads;
+    if (tiles.adj_resident != 0 && upt <= 8 && yt + blk + dbytes <= kResidentBudget) {
+      int PB = tiles.pb_adj > 0 ? std::min(tiles.pb_adj, 64) : 16;
+      if (tiles.pb_adj <= 0)
+        while (PB > 1 && pair_batch_bytes(PB, LL, F, sizeof(T), DIRECT, true) > dbytes) --PB;
+      const size_t smem = yt + blk + std::max(dbytes, pair_batch_bytes(PB, LL, F, sizeof(T), DIRECT, true));
+      auto go = [&](auto kern) {
+        allow_lds(kern, smem);
+        hipLaunchKernelGGL(kern, dim3(unsigned(pv.n_centres)), dim3(kThreads), smem, stream, rad, S, PB,
+                           pv, c, p, p_hi, p_lo, norm2, int(normalize), xg, xg_scale, G, grad,
+                           centre_virial);
+      };
+      if (p_hi) {
+        if (upt <= 2)
+          go(adjoint_resident_kernel<T, DIRECT, true, 2>);
+        else if (upt <= 4)
+          go(adjoint_resident_kernel<T, DIRECT, true, 4>);
+        else if (upt <= 6)
+          go(adjoint_resident_kernel<T, DIRECT, true, 6>);
+        else
+          go(adjoint_resident_kernel<T, DIRECT, true, 8>);
+      } else {
+        if (upt <= 2)
+          go(adjoint_resident_kernel<T, DIRECT, false, 2>);
+        else if (upt <= 4)
+          go(adjoint_resident_kernel<T, DIRECT, false, 4>);
+        else if (upt <= 6)
+          go(adjoint_resident_kernel<T, DIRECT, false, 6>);
+        else
+          go(adjoint_resident_kernel<T, DIRECT, false, 8>);
+      }
+      RGPU_LAUNCH_CHECK();
+      return;
+    }
+  }
   const size_t gbytes = sizeof(T) * 2 * size_t(L1) * N * N4;
   const int c_in_lds = yt + blk + blk + gbytes <= kAdjBudget ? 1 : 0;
   const size_t region = (c_in_lds ? blk : 0) + gbytes;
