@@ -765,18 +765,9 @@ void autotune_t(Context& c) {
     double* grad = gr.reserve(size_t(3 * n));
     double* cvir = cv.reserve(size_t(6 * n));
     RGPU_HIP(hipMemsetAsync(grad, 0, sizeof(double) * 3 * n, c.stream));
-    best = 1e30f;
-    pick = 0;
-    int pick_res = -1;
-    for (int res : {1, 0})
-    for (int pb : {2, 4, 6, 8, 12, 16, 24, 32}) {
-      if (res == 1 && !split) continue;  // the resident variant is FP32 only
-      SoapTiles cand = t;
-      cand.pb_adj = pb;
-      cand.adj_resident = res;
-      float ms;
+    auto time_adjoint = [&](const SoapTiles& cand) {
       try {
-        ms = time_ms([&] {
+        return time_ms([&] {
           if constexpr (split)
             launch_backward_split(c.rd, S, pv, cc, ph, pl, n2, c.hp.normalize, nullptr, 1.0, g,
                                   grad, cvir, c.stream, cand);
@@ -785,12 +776,40 @@ void autotune_t(Context& c) {
                                cvir, c.stream, cand);
         });
       } catch (const InvalidArgument&) {
-        continue;
+        return 1e30f;  // needs more LDS than the device provides
       }
-      if (ms < best) best = ms, pick = pb, pick_res = res;
+    };
+    // variant (and key blocks per pass of the staged variant), then pairs
+    // per batch for the chosen variant
+    SoapTiles bestv = t;
+    bestv.pb_adj = 0;
+    bestv.adj_resident = 0;
+    bestv.keys_adj = 0;
+    best = time_adjoint(bestv);
+    if (split) {
+      int last = -1;
+      for (int div = 1; div <= 4; ++div) {
+        const int kch = (K + div - 1) / div;
+        if (kch == last) continue;
+        last = kch;
+        SoapTiles cand = bestv;
+        cand.adj_resident = 1;
+        cand.keys_adj = kch;
+        const float ms = time_adjoint(cand);
+        if (ms < best) best = ms, bestv = cand;
+      }
+    }
+    t.adj_resident = bestv.adj_resident;
+    t.keys_adj = bestv.keys_adj;
+    best = 1e30f;
+    pick = 0;
+    for (int pb : {2, 4, 6, 8, 12, 16, 24, 32}) {
+      SoapTiles cand = t;
+      cand.pb_adj = pb;
+      const float ms = time_adjoint(cand);
+      if (ms < best) best = ms, pick = pb;
     }
     t.pb_adj = pick;
-    t.adj_resident = pick_res;
     RGPU_HIP(hipStreamSynchronize(c.stream));
   }
   c.ev_pool.push_back(a);
@@ -847,7 +866,7 @@ int rgpu_autotune(rgpu_context* ctx) {
 }
 
 int rgpu_set_tiles(rgpu_context* ctx, int pairs_fwd, int keys_ps, int pairs_adj,
-                   int adjoint_resident) {
+                   int adjoint_resident, int keys_adj) {
   return guarded([&] {
     if (!ctx) throw InvalidArgument("null context");
     if (pairs_fwd > 64 || pairs_adj > 64) throw InvalidArgument("at most 64 pairs per batch");
@@ -859,6 +878,7 @@ int rgpu_set_tiles(rgpu_context* ctx, int pairs_fwd, int keys_ps, int pairs_adj,
     t.kc = std::max(keys_ps, 0);
     t.pb_adj = std::max(pairs_adj, 0);
     t.adj_resident = adjoint_resident;
+    t.keys_adj = std::max(keys_adj, 0);
     c.have_rep = false;
     c.drop_graph();
   });
@@ -916,6 +936,7 @@ int rgpu_get_info(rgpu_context* ctx, int key, int64_t* value) {
       case RGPU_INFO_TILE_KEYS_PS: *value = c.tile().kc; break;
       case RGPU_INFO_TILE_PAIRS_ADJ: *value = c.tile().pb_adj; break;
       case RGPU_INFO_TILE_ADJ_RESIDENT: *value = c.tile().adj_resident; break;
+      case RGPU_INFO_TILE_KEYS_ADJ: *value = c.tile().keys_adj; break;
       default: throw InvalidArgument("unknown info key");
     }
   });

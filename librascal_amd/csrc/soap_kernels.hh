@@ -36,6 +36,7 @@ struct SoapTiles {
   // adjoint variant: -1 chooses (dL/dp resident in LDS when it fits),
   // 0 one LDS pass per key block, 1 resident (when it fits)
   int adj_resident = -1;
+  int keys_adj = 0;  // key blocks staged per pass in the resident adjoint
 };
 
 // c[i][s][n][l*l+l+m] (finalised) and the per-centre key mask.

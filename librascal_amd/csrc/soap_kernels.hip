@@ -229,6 +229,7 @@ struct PairBatch {
   double* dist;  // [PB]
   double* fc;    // [PB] cutoff function
   double* dfc;   // [PB] its derivative (adjoint)
+  double* vec;   // [PB][3] pair vectors (adjoint)
   double* raw;   // [PB][F] raw radial integrals (direct Hyp1f1)
   double* draw;  // [PB][F] their derivatives (direct, adjoint)
   T* Y;          // [PB][LL]
@@ -238,18 +239,21 @@ struct PairBatch {
   T* dir;        // [PB][3]
   T* ca;         // [PB] spline weights of the pair's cell
   T* cb;         // [PB]
+  T* res;        // [PB][4] per-pair contractions (adjoint)
   int* sp;       // [PB] neighbour species
   int* lst;      // [PB] batch positions grouped by species (forward)
   int* klo;      // [PB] spline cell
+  int* pj;       // [PB] neighbour atom (adjoint)
 };
 
 __host__ __device__ inline size_t pair_batch_bytes(int PB, int LL, int F, int tsize, bool direct,
                                                    bool adjoint) {
-  size_t b = 8 * size_t(PB) * (adjoint ? 3 : 2);
+  size_t b = 8 * size_t(PB) * (adjoint ? 6 : 2);
   if (direct) b += 8 * size_t(PB) * F * (adjoint ? 2 : 1);
-  b += size_t(tsize) * PB * (size_t(LL) * (adjoint ? 4 : 1) + size_t(F) * (adjoint ? 2 : 1) + 5);
+  b += size_t(tsize) * PB *
+       (size_t(LL) * (adjoint ? 4 : 1) + size_t(F) * (adjoint ? 2 : 1) + 5 + (adjoint ? 4 : 0));
   b = align_up(b, 8);
-  b += 4 * size_t(PB) * 3;
+  b += 4 * size_t(PB) * (adjoint ? 4 : 3);
   return align_up(b, 64);
 }
 
@@ -261,10 +265,12 @@ __device__ PairBatch<T> carve_batch(unsigned char* base, int PB, int LL, int F) 
   d += PB;
   pb.fc = d;
   d += PB;
-  pb.dfc = nullptr;
+  pb.dfc = pb.vec = nullptr;
   if (ADJ) {
     pb.dfc = d;
     d += PB;
+    pb.vec = d;
+    d += 3 * PB;
   }
   pb.raw = pb.draw = nullptr;
   if (DIRECT) {
@@ -296,10 +302,16 @@ __device__ PairBatch<T> carve_batch(unsigned char* base, int PB, int LL, int F) 
   t += PB;
   pb.cb = t;
   t += PB;
+  pb.res = nullptr;
+  if (ADJ) {
+    pb.res = t;
+    t += 4 * PB;
+  }
   const size_t off = align_up(size_t(reinterpret_cast<unsigned char*>(t) - base), 8);
   pb.sp = reinterpret_cast<int*>(base + off);
   pb.lst = pb.sp + PB;
   pb.klo = pb.lst + PB;
+  pb.pj = ADJ ? pb.klo + PB : nullptr;
   return pb;
 }
 
@@ -324,8 +336,15 @@ __device__ void stage_pairs(const RadialDev& rad, const T* yc, const PairView& p
     b.dir[3 * tid + 1] = T(pv.pair_vec[3 * p + 1] / d);
     b.dir[3 * tid + 2] = T(pv.pair_vec[3 * p + 2] / d);
     b.fc[tid] = pv.pair_fc[p];
-    if (ADJ) b.dfc[tid] = pv.pair_dfc[p];
-    b.sp[tid] = pv.species_of[pv.pair_j[p]];
+    const int j = pv.pair_j[p];
+    if (ADJ) {
+      b.dfc[tid] = pv.pair_dfc[p];
+      b.vec[3 * tid] = pv.pair_vec[3 * p];
+      b.vec[3 * tid + 1] = pv.pair_vec[3 * p + 1];
+      b.vec[3 * tid + 2] = pv.pair_vec[3 * p + 2];
+      b.pj[tid] = j;
+    }
+    b.sp[tid] = pv.species_of[j];
     if (!DIRECT) {
       const SplineCell<T> cell = spline_cell<T>(rad, d);
       b.klo[tid] = cell.klo;
@@ -621,15 +640,18 @@ __global__ void normalise_kernel(int64_t n, int64_t D, const T* __restrict__ p,
 
 // Step 3 of the adjoint for centre i: per-pair contraction of dL/dc (in
 // LDS) with the pair gradients, force atomics and the centre virial.  The
-// pair batches are staged in `region`.
+// pair batches are staged in `region`.  Each pair is contracted by a group
+// of 16 lanes (four pairs per wavefront at a time, reduced within the
+// group); the per-pair FP64 tail (force, atomics, virial) then runs one
+// lane per pair on wavefront 0 (PB <= 64).
 template <class T, bool DIRECT>
 __device__ __forceinline__ void adjoint_pairs(const RadialDev& rad, int S, int PB, const PairView& pv,
                                               const T* yc, const T* dc, unsigned char* region,
                                               int64_t i, double* __restrict__ grad,
-                                              double* __restrict__ centre_virial,
-                                              double (*red)[9]) {
+                                              double* __restrict__ centre_virial) {
   const int N = rad.N, L1 = rad.L1, LL = rad.LL, F = N * L1, B = N * LL;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int grp = lane >> 4, gl = lane & 15;
   PairBatch<T> pb = carve_batch<T, DIRECT, true>(region, PB, LL, F);
   const int64_t p0 = pv.centre_off[i] + 1;
   const int np = int(pv.centre_off[i + 1] - p0);
@@ -637,75 +659,83 @@ __device__ __forceinline__ void adjoint_pairs(const RadialDev& rad, int S, int P
   for (int q0 = 0; q0 < np; q0 += PB) {
     const int nb = min(PB, np - q0);
     stage_pairs<T, DIRECT, true>(rad, yc, pv, p0 + q0, nb, pb, nullptr, nullptr, nullptr);
-    for (int q = wave; q < nb; q += kWaves) {
-      const T* W = dc + pb.sp[q] * B;
-      const T* Yq = pb.Y + q * LL;
-      const T* dYq = pb.dY + q * 3 * LL;
-      const T* Rq = pb.R + q * F;
-      const T* dRq = pb.dR + q * F;
-      const T fc = T(pb.fc[q]), dfc = T(pb.dfc[q]);
+    for (int qb = 4 * wave; qb < nb; qb += 4 * kWaves) {
+      const int q = qb + grp;
+      const bool act = q < nb;
       T A = T(0), Bx = T(0), By = T(0), Bz = T(0);
-      for (int lm = lane; lm < LL; lm += 64) {
-        const int l = l_of_lm(lm);
-        T Dn = T(0), Ddn = T(0);
-        for (int n = 0; n < N; ++n) {
-          const T wv = W[n * LL + lm];
-          Dn += wv * Rq[n * L1 + l];
-          Ddn += wv * dRq[n * L1 + l];
+      if (act) {
+        const T* W = dc + pb.sp[q] * B;
+        const T* Yq = pb.Y + q * LL;
+        const T* dYq = pb.dY + q * 3 * LL;
+        const T* Rq = pb.R + q * F;
+        const T* dRq = pb.dR + q * F;
+        const T fc = T(pb.fc[q]), dfc = T(pb.dfc[q]);
+        for (int lm = gl; lm < LL; lm += 16) {
+          const int l = l_of_lm(lm);
+          T Dn = T(0), Ddn = T(0);
+          for (int n = 0; n < N; ++n) {
+            const T wv = W[n * LL + lm];
+            Dn += wv * Rq[n * L1 + l];
+            Ddn += wv * dRq[n * L1 + l];
+          }
+          A += (Ddn * fc + Dn * dfc) * Yq[lm];
+          Bx += Dn * dYq[lm];
+          By += Dn * dYq[LL + lm];
+          Bz += Dn * dYq[2 * LL + lm];
         }
-        A += (Ddn * fc + Dn * dfc) * Yq[lm];
-        Bx += Dn * dYq[lm];
-        By += Dn * dYq[LL + lm];
-        Bz += Dn * dYq[2 * LL + lm];
       }
-      for (int off = 32; off > 0; off >>= 1) {
-        A += __shfl_down(A, off, 64);
-        Bx += __shfl_down(Bx, off, 64);
-        By += __shfl_down(By, off, 64);
-        Bz += __shfl_down(Bz, off, 64);
+      for (int off = 8; off > 0; off >>= 1) {
+        A += __shfl_xor(A, off, 16);
+        Bx += __shfl_xor(Bx, off, 16);
+        By += __shfl_xor(By, off, 16);
+        Bz += __shfl_xor(Bz, off, 16);
       }
-      if (lane == 0) {
-        const double d = pb.dist[q];
-        const double sc = pb.fc[q] / d;
-        const int64_t pidx = p0 + q0 + q;
-        const double ux = pv.pair_vec[3 * pidx] / d, uy = pv.pair_vec[3 * pidx + 1] / d,
-                     uz = pv.pair_vec[3 * pidx + 2] / d;
-        const double g[3] = {double(A) * ux + sc * double(Bx), double(A) * uy + sc * double(By),
-                             double(A) * uz + sc * double(Bz)};
-        const int j = pv.pair_j[pidx];
-        unsafeAtomicAdd(grad + 3 * j, g[0]);
-        unsafeAtomicAdd(grad + 3 * j + 1, g[1]);
-        unsafeAtomicAdd(grad + 3 * j + 2, g[2]);
-        if (j != int(i)) {
-          fi[0] -= g[0];
-          fi[1] -= g[1];
-          fi[2] -= g[2];
-        }
-        // r_ji = r_i - r_j(image) = -pair vector
-        const double rx = -pv.pair_vec[3 * pidx], ry = -pv.pair_vec[3 * pidx + 1],
-                     rz = -pv.pair_vec[3 * pidx + 2];
-        v6[0] += rx * g[0];
-        v6[1] += ry * g[1];
-        v6[2] += rz * g[2];
-        v6[3] += ry * g[2];
-        v6[4] += rz * g[0];
-        v6[5] += rx * g[1];
+      if (act && gl == 0) {
+        pb.res[4 * q] = A;
+        pb.res[4 * q + 1] = Bx;
+        pb.res[4 * q + 2] = By;
+        pb.res[4 * q + 3] = Bz;
       }
     }
     __syncthreads();
+    if (tid < nb) {
+      const int q = tid;
+      const double d = pb.dist[q];
+      const double sc = pb.fc[q] / d;
+      const double vx = pb.vec[3 * q], vy = pb.vec[3 * q + 1], vz = pb.vec[3 * q + 2];
+      const double A = double(pb.res[4 * q]);
+      const double g[3] = {A * vx / d + sc * double(pb.res[4 * q + 1]),
+                           A * vy / d + sc * double(pb.res[4 * q + 2]),
+                           A * vz / d + sc * double(pb.res[4 * q + 3])};
+      const int j = pb.pj[q];
+      unsafeAtomicAdd(grad + 3 * j, g[0]);
+      unsafeAtomicAdd(grad + 3 * j + 1, g[1]);
+      unsafeAtomicAdd(grad + 3 * j + 2, g[2]);
+      if (j != int(i)) {
+        fi[0] -= g[0];
+        fi[1] -= g[1];
+        fi[2] -= g[2];
+      }
+      // r_ji = r_i - r_j(image) = -pair vector
+      v6[0] -= vx * g[0];
+      v6[1] -= vy * g[1];
+      v6[2] -= vz * g[2];
+      v6[3] -= vy * g[2];
+      v6[4] -= vz * g[0];
+      v6[5] -= vx * g[1];
+    }
+    __syncthreads();  // the next batch overwrites this one
   }
-  if (lane == 0) {
-    for (int k = 0; k < 3; ++k) red[wave][k] = fi[k];
-    for (int k = 0; k < 6; ++k) red[wave][3 + k] = v6[k];
-  }
-  __syncthreads();
-  if (tid < 9) {
-    double t = 0;
-    for (int w = 0; w < kWaves; ++w) t += red[w][tid];
-    if (tid < 3) {
-      if (t != 0.0) unsafeAtomicAdd(grad + 3 * i + tid, t);
-    } else {
-      centre_virial[6 * i + tid - 3] = t;
+  // only wavefront 0 holds partial sums
+  if (wave == 0) {
+    double r[9] = {fi[0], fi[1], fi[2], v6[0], v6[1], v6[2], v6[3], v6[4], v6[5]};
+#pragma unroll
+    for (int k = 0; k < 9; ++k)
+      for (int off = 32; off > 0; off >>= 1) r[k] += __shfl_down(r[k], off, 64);
+    if (lane == 0) {
+      for (int k = 0; k < 3; ++k)
+        if (r[k] != 0.0) unsafeAtomicAdd(grad + 3 * i + k, r[k]);
+      for (int k = 0; k < 6; ++k) centre_virial[6 * i + k] = r[3 + k];
     }
   }
 }
@@ -873,24 +903,25 @@ __global__ __launch_bounds__(kThreads) void adjoint_kernel(
   }
 
   // per-pair contraction; the batch reuses the region after dL/dc
-  adjoint_pairs<T, DIRECT>(rad, S, PB, pv, yc, dc, region, i, grad, centre_virial, red);
+  adjoint_pairs<T, DIRECT>(rad, S, PB, pv, yc, dc, region, i, grad, centre_virial);
 }
 
-// Adjoint with the whole dL/dp of the centre resident in LDS (used when it
-// fits beside the expansion coefficients).  Same mathematics as
-// adjoint_kernel; the dL/dc contraction runs as one pass over every key
-// with the outputs in registers instead of one LDS pass per key:
-//  1. g = alpha G - beta p staged for all keys, natural layout
-//     [key][x][y][l]; diagonal keys then replaced by g + g^T;
+// Adjoint with dL/dp of the centre staged in LDS KCH key blocks at a time
+// (all of them when they fit) beside the expansion coefficients.  Same
+// mathematics as adjoint_kernel; the dL/dc contraction keeps its outputs
+// in registers across the key blocks instead of one LDS update per key:
+//  1. g = alpha G - beta p staged for keys [k0, k0 + KCH) in the natural
+//     layout [key][x][y][l]; diagonal keys then replaced by g + g^T;
 //  2. thread unit (a, 4 radial channels n, lm):
-//       dL/dc_a[n][lm] = (2l+1)^{-1/2} sum_b w_ab sum_y g^(ab)[n][y][l] c_b[y][lm]
-//     (g^(ab) read transposed for a > b, w_ab = sqrt 2 for a != b), written
-//     over c once every unit has read it;
-//  3. the pair phase of adjoint_kernel, its batches over the g region.
+//       dL/dc_a[n][lm] += (2l+1)^{-1/2} w_ab sum_y g^(ab)[n][y][l] c_b[y][lm]
+//     over the species b whose key is in the chunk (g^(ab) read transposed
+//     for a > b, w_ab = sqrt 2 for a != b); after the last chunk written
+//     over c;
+//  3. the pair phase, its batches over the g region.
 // UPT = units per thread (ceil(S * ceil(N/4) * LL / 256)).
 template <class T, bool DIRECT, bool PSPLIT, int UPT>
 __global__ __launch_bounds__(kThreads) void adjoint_resident_kernel(
-    RadialDev rad, int S, int PB, PairView pv, const T* __restrict__ c, const T* __restrict__ p,
+    RadialDev rad, int S, int PB, int KCH, PairView pv, const T* __restrict__ c, const T* __restrict__ p,
     const unsigned short* __restrict__ p_hi, const unsigned short* __restrict__ p_lo,
     const double* __restrict__ norm2, int normalize, const double* __restrict__ xg, double xg_scale,
     const T* __restrict__ G, double* __restrict__ grad, double* __restrict__ centre_virial) {
@@ -945,98 +976,109 @@ __global__ __launch_bounds__(kThreads) void adjoint_resident_kernel(
   }
   const T ta = T(alpha), tb = T(beta);
 
-  // 1. stage g (four elements per thread and step when rows are 16-byte
-  //    aligned, D % 4 == 0)
-  if ((D & 3) == 0) {
-    const V4* G4 = reinterpret_cast<const V4*>(Gi);
-    V4* g4 = reinterpret_cast<V4*>(gl);
-    for (int q = tid; q < (D >> 2); q += kThreads) {
-      V4 gv = G4[q];
-      V4 pv4;
-      if (PSPLIT) {
-        const uint2 h = reinterpret_cast<const uint2*>(p_hi + i * int64_t(D))[q];
-        const uint2 lo = reinterpret_cast<const uint2*>(p_lo + i * int64_t(D))[q];
-        pv4[0] = T(__builtin_bit_cast(float, h.x << 16) + __builtin_bit_cast(float, lo.x << 16));
-        pv4[1] = T(__builtin_bit_cast(float, h.x & 0xffff0000u) +
-                   __builtin_bit_cast(float, lo.x & 0xffff0000u));
-        pv4[2] = T(__builtin_bit_cast(float, h.y << 16) + __builtin_bit_cast(float, lo.y << 16));
-        pv4[3] = T(__builtin_bit_cast(float, h.y & 0xffff0000u) +
-                   __builtin_bit_cast(float, lo.y & 0xffff0000u));
-      } else {
-        pv4 = reinterpret_cast<const V4*>(p + i * int64_t(D))[q];
-      }
-      g4[q] = ta * gv - tb * pv4;
-    }
-  } else {
-    for (int o = tid; o < D; o += kThreads) gl[o] = ta * Gi[o] - tb * pval(o);
-  }
-  __syncthreads();
-  // diagonal keys: g[x][y] + g[y][x] on both triangles
-  {
-    const int tri = N * (N + 1) / 2 * L1;
-    for (int it = tid; it < S * tri; it += kThreads) {
-      const int a = it / tri;
-      int r = it - a * tri;
-      const int l = r % L1;
-      r /= L1;
-      int x = 0;
-      while (r >= N - x) {
-        r -= N - x;
-        ++x;
-      }
-      const int y = x + r;
-      T* gk = gl + (a * S - a * (a - 1) / 2) * KB + l;
-      const T v = gk[(x * N + y) * L1] + gk[(y * N + x) * L1];
-      gk[(x * N + y) * L1] = v;
-      gk[(y * N + x) * L1] = v;
-    }
-  }
-  __syncthreads();
-
-  // 2. dL/dc units with register accumulators
+  // dL/dc units: thread unit (a, radial channels n0..n0+3, lm)
   const int NQ = (N + 3) >> 2;
   const int units = S * NQ * LL;
   const int sx = N * L1;  // stride of the first radial index of a key block
   T acc[UPT][4];
 #pragma unroll
-  for (int k = 0; k < UPT; ++k) {
-    acc[k][0] = acc[k][1] = acc[k][2] = acc[k][3] = T(0);
-    const int u = tid + kThreads * k;
-    if (u < units) {
-      const int anq = u / LL, lm = u - anq * LL;
-      const int a = anq / NQ, n0 = 4 * (anq - a * NQ);
-      const int l = l_of_lm(lm);
-      int na[4];
-#pragma unroll
-      for (int v = 0; v < 4; ++v) na[v] = min(n0 + v, N - 1);
-      for (int b = 0; b < S; ++b) {
-        const int lo = min(a, b), hi = max(a, b);
-        const T* gk = gl + (lo * S - lo * (lo - 1) / 2 + (hi - lo)) * KB + l;
-        // g^(ab)[n][y] for a <= b, g^(ba)[y][n] for a > b
-        const int sn = a <= b ? sx : L1, sy = a <= b ? L1 : sx;
-        const T* cb = cd + b * B + lm;
-        T s0 = T(0), s1 = T(0), s2 = T(0), s3 = T(0);
-        const T* g0 = gk + na[0] * sn;
-        const T* g1 = gk + na[1] * sn;
-        const T* g2 = gk + na[2] * sn;
-        const T* g3 = gk + na[3] * sn;
-        for (int y = 0; y < N; ++y) {
-          const T cv = cb[y * LL];
-          const int oy = y * sy;
-          s0 += g0[oy] * cv;
-          s1 += g1[oy] * cv;
-          s2 += g2[oy] * cv;
-          s3 += g3[oy] * cv;
+  for (int k = 0; k < UPT; ++k) acc[k][0] = acc[k][1] = acc[k][2] = acc[k][3] = T(0);
+
+  for (int k0 = 0; k0 < K; k0 += KCH) {
+    const int k1 = min(K, k0 + KCH);
+    const int e0 = k0 * KB, ne = (k1 - k0) * KB;
+    if (k0 > 0) __syncthreads();  // every unit has read the previous chunk
+    // 1. stage g of keys [k0, k1) (four elements per thread and step when
+    //    the key blocks keep 16-byte alignment, KB % 4 == 0)
+    if ((KB & 3) == 0) {
+      const V4* G4 = reinterpret_cast<const V4*>(Gi + e0);
+      V4* g4 = reinterpret_cast<V4*>(gl);
+      const unsigned short* ph = PSPLIT ? p_hi + i * int64_t(D) + e0 : nullptr;
+      const unsigned short* plo = PSPLIT ? p_lo + i * int64_t(D) + e0 : nullptr;
+      for (int q = tid; q < (ne >> 2); q += kThreads) {
+        V4 gv = G4[q];
+        V4 pv4;
+        if (PSPLIT) {
+          const uint2 h = reinterpret_cast<const uint2*>(ph)[q];
+          const uint2 lo = reinterpret_cast<const uint2*>(plo)[q];
+          pv4[0] = T(__builtin_bit_cast(float, h.x << 16) + __builtin_bit_cast(float, lo.x << 16));
+          pv4[1] = T(__builtin_bit_cast(float, h.x & 0xffff0000u) +
+                     __builtin_bit_cast(float, lo.x & 0xffff0000u));
+          pv4[2] = T(__builtin_bit_cast(float, h.y << 16) + __builtin_bit_cast(float, lo.y << 16));
+          pv4[3] = T(__builtin_bit_cast(float, h.y & 0xffff0000u) +
+                     __builtin_bit_cast(float, lo.y & 0xffff0000u));
+        } else {
+          pv4 = reinterpret_cast<const V4*>(p + i * int64_t(D) + e0)[q];
         }
-        const T w = a == b ? T(1) : T(1.41421356237309504880);
-        acc[k][0] += w * s0;
-        acc[k][1] += w * s1;
-        acc[k][2] += w * s2;
-        acc[k][3] += w * s3;
+        g4[q] = ta * gv - tb * pv4;
       }
-      const T f = T(1) / sqrt(T(2 * l + 1));
+    } else {
+      for (int o = tid; o < ne; o += kThreads) gl[o] = ta * Gi[e0 + o] - tb * pval(e0 + o);
+    }
+    __syncthreads();
+    // diagonal keys of the chunk: g[x][y] + g[y][x] on both triangles
+    {
+      const int tri = N * (N + 1) / 2 * L1;
+      for (int it = tid; it < S * tri; it += kThreads) {
+        const int a = it / tri;
+        const int kd = a * S - a * (a - 1) / 2;
+        if (kd < k0 || kd >= k1) continue;
+        int r = it - a * tri;
+        const int l = r % L1;
+        r /= L1;
+        int x = 0;
+        while (r >= N - x) {
+          r -= N - x;
+          ++x;
+        }
+        const int y = x + r;
+        T* gk = gl + (kd - k0) * KB + l;
+        const T v = gk[(x * N + y) * L1] + gk[(y * N + x) * L1];
+        gk[(x * N + y) * L1] = v;
+        gk[(y * N + x) * L1] = v;
+      }
+    }
+    __syncthreads();
+
+    // 2. contraction of the chunk's keys into the register accumulators
 #pragma unroll
-      for (int v = 0; v < 4; ++v) acc[k][v] *= f;
+    for (int k = 0; k < UPT; ++k) {
+      const int u = tid + kThreads * k;
+      if (u < units) {
+        const int anq = u / LL, lm = u - anq * LL;
+        const int a = anq / NQ, n0 = 4 * (anq - a * NQ);
+        const int l = l_of_lm(lm);
+        int na[4];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) na[v] = min(n0 + v, N - 1);
+        for (int b = 0; b < S; ++b) {
+          const int lo = min(a, b), hi = max(a, b);
+          const int key = lo * S - lo * (lo - 1) / 2 + (hi - lo);
+          if (key < k0 || key >= k1) continue;
+          const T* gk = gl + (key - k0) * KB + l;
+          // g^(ab)[n][y] for a <= b, g^(ba)[y][n] for a > b
+          const int sn = a <= b ? sx : L1, sy = a <= b ? L1 : sx;
+          const T* cb = cd + b * B + lm;
+          T s0 = T(0), s1 = T(0), s2 = T(0), s3 = T(0);
+          const T* g0 = gk + na[0] * sn;
+          const T* g1 = gk + na[1] * sn;
+          const T* g2 = gk + na[2] * sn;
+          const T* g3 = gk + na[3] * sn;
+          for (int y = 0; y < N; ++y) {
+            const T cv = cb[y * LL];
+            const int oy = y * sy;
+            s0 += g0[oy] * cv;
+            s1 += g1[oy] * cv;
+            s2 += g2[oy] * cv;
+            s3 += g3[oy] * cv;
+          }
+          const T w = a == b ? T(1) : T(1.41421356237309504880);
+          acc[k][0] += w * s0;
+          acc[k][1] += w * s1;
+          acc[k][2] += w * s2;
+          acc[k][3] += w * s3;
+        }
+      }
     }
   }
   __syncthreads();  // every unit has read c
@@ -1046,15 +1088,16 @@ __global__ __launch_bounds__(kThreads) void adjoint_resident_kernel(
     if (u < units) {
       const int anq = u / LL, lm = u - anq * LL;
       const int a = anq / NQ, n0 = 4 * (anq - a * NQ);
+      const T f = T(1) / sqrt(T(2 * l_of_lm(lm) + 1));
 #pragma unroll
       for (int v = 0; v < 4; ++v)
-        if (n0 + v < N) cd[a * B + (n0 + v) * LL + lm] = acc[k][v];
+        if (n0 + v < N) cd[a * B + (n0 + v) * LL + lm] = f * acc[k][v];
     }
   }
   __syncthreads();
 
   // 3. pairs; the batches reuse the g region
-  adjoint_pairs<T, DIRECT>(rad, S, PB, pv, yc, cd, region, i, grad, centre_virial, red);
+  adjoint_pairs<T, DIRECT>(rad, S, PB, pv, yc, cd, region, i, grad, centre_virial);
 }
 
 // n_pairs is read on the device (the last entry of the pair offsets), so one
@@ -1167,7 +1210,7 @@ __global__ __launch_bounds__(kThreads) void ylm_eval_kernel(RadialDev rad, const
 constexpr size_t kFwdBudget = 32 * 1024;
 constexpr size_t kPsBudget = 48 * 1024;
 constexpr size_t kAdjBudget = 64 * 1024;
-constexpr size_t kResidentBudget = 80 * 1024;  // two workgroups per CU
+constexpr size_t kResidentBudget = 52 * 1024;  // three workgroups per CU
 
 // LDS limit per workgroup of the current device (queried once per device;
 // launches may be captured into graphs, where device queries are avoided)
@@ -1233,19 +1276,25 @@ void launch_adjoint_t(const RadialDev& rad, int S, const PairView& pv, const T* 
   const size_t yt = ylm_table_bytes<T>(L1);
   const size_t blk = align_up(sizeof(T) * size_t(E), 64);
   if constexpr (sizeof(T) == 4) {
-    // dL/dp of a centre resident in LDS beside c
+    // dL/dp of a centre staged in LDS, KCH key blocks at a time, beside c
     const int K = S * (S + 1) / 2;
-    const size_t dbytes = align_up(sizeof(T) * size_t(K) * N * N * L1, 64);
+    const size_t kbytes = sizeof(T) * size_t(N) * N * L1;
     const int upt = (S * ((N + 3) / 4) * LL + kThreads - 1) / kThreads;
-    if (tiles.adj_resident != 0 && upt <= 8 && yt + blk + dbytes <= kResidentBudget) {
+    int KCH = tiles.keys_adj > 0 ? std::min(tiles.keys_adj, K) : K;
+    if (tiles.keys_adj <= 0)
+      while (KCH > 1 && yt + blk + align_up(KCH * kbytes, 64) > kResidentBudget) --KCH;
+    const size_t dbytes = align_up(KCH * kbytes, 64);
+    if (tiles.adj_resident != 0 && upt <= 8 &&
+        (yt + blk + dbytes <= kResidentBudget || tiles.keys_adj > 0)) {
       int PB = tiles.pb_adj > 0 ? std::min(tiles.pb_adj, 64) : 16;
       if (tiles.pb_adj <= 0)
         while (PB > 1 && pair_batch_bytes(PB, LL, F, sizeof(T), DIRECT, true) > dbytes) --PB;
       const size_t smem = yt + blk + std::max(dbytes, pair_batch_bytes(PB, LL, F, sizeof(T), DIRECT, true));
+      if (smem > device_lds_limit()) throw InvalidArgument("adjoint partition exceeds the LDS");
       auto go = [&](auto kern) {
         allow_lds(kern, smem);
         hipLaunchKernelGGL(kern, dim3(unsigned(pv.n_centres)), dim3(kThreads), smem, stream, rad, S, PB,
-                           pv, c, p, p_hi, p_lo, norm2, int(normalize), xg, xg_scale, G, grad,
+                           KCH, pv, c, p, p_hi, p_lo, norm2, int(normalize), xg, xg_scale, G, grad,
                            centre_virial);
       };
       if (p_hi) {

@@ -203,9 +203,10 @@ def test_autotuned_tiles_keep_results():
 
 @pytest.mark.parametrize("N,L", [(12, 9), (5, 4), (6, 3)])
 def test_adjoint_variants_agree(N, L):
-    """FP32 adjoint with dL/dp resident in LDS vs one LDS pass per key block
-    vs the FP64 adjoint (pinned against the oracle elsewhere), on ragged
-    molecules with 4 species.  N = 5, L = 4 gives D = 10 * 125 (not a
+    """FP32 adjoint with dL/dp staged in LDS (register accumulation over key
+    blocks, one or several key blocks per pass) vs one LDS update per key
+    block vs the FP64 adjoint (pinned against the oracle elsewhere), on
+    ragged molecules with 4 species.  N = 5, L = 4 gives D = 10 * 125 (not a
     multiple of 4: element-wise staging) and a partial group of radial
     channels; N = 6 another partial group."""
     sts = molecules(n_mol=24, seed=5)
@@ -217,14 +218,15 @@ def test_adjoint_variants_agree(N, L):
     g64, s64 = ctx.features_backward(G)
     ctx.set_precision(_capi.MIXED)
     out = {}
-    for res in (0, 1):
-        ctx.set_tiles(adjoint_resident=res)
-        assert ctx.tiles()["adjoint_resident"] == res
+    # (variant, key blocks per pass): per-key updates; staged, all keys
+    # (default budget) / one key / three keys per pass
+    for res, kch in ((0, 0), (1, 0), (1, 1), (1, 3)):
+        ctx.set_tiles(adjoint_resident=res, keys_adj=kch)
+        assert ctx.tiles()["adjoint_resident"] == res and ctx.tiles()["keys_adj"] == kch
         ctx.features()
-        out[res] = ctx.features_backward(G)
+        out[res, kch] = ctx.features_backward(G)
     scale = np.abs(g64).max()
-    for res in (0, 1):
-        g, s = out[res]
-        assert np.abs(g - g64).max() < 2e-5 * scale, res
-        assert np.abs(s - s64).max() < 2e-5 * np.abs(s64).max(), res
-    assert np.abs(out[0][0] - out[1][0]).max() < 1e-5 * scale
+    for key, (g, s) in out.items():
+        assert np.abs(g - g64).max() < 2e-5 * scale, key
+        assert np.abs(s - s64).max() < 2e-5 * np.abs(s64).max(), key
+        assert np.abs(g - out[0, 0][0]).max() < 1e-5 * scale, key
