@@ -77,7 +77,9 @@ struct Context {
   DeviceBuffer<int> d_rows;
 
   // scratch
-  DeviceBuffer<double> e_centre, grad, cvir, tmp64, out64;
+  DeviceBuffer<double> e_centre, grad, cvir, tmp64, out64, out_e;
+  DeviceBuffer<double> d_inv_vol;  // [n_struct] 1 / cell volume
+  std::vector<double> inv_vol;     // host copy of the uploaded values
   DeviceBuffer<double> pair_fc, pair_dfc;  // cutoff function per pair
   DeviceBuffer<int> d_sp_rows;             // 0..M-1 (K_MM row gather)
   DeviceBuffer<double> wt64, G64;
@@ -302,6 +304,11 @@ void require_batch(const Context& c) {
   if (!c.have_batch) throw InvalidArgument("no structures set (call rgpu_set_structures first)");
 }
 
+double cell_volume(const double* C) {
+  return std::fabs(C[0] * (C[4] * C[8] - C[5] * C[7]) - C[1] * (C[3] * C[8] - C[5] * C[6]) +
+                   C[2] * (C[3] * C[7] - C[4] * C[6]));
+}
+
 void build_batch_nl(Context& c, const double* positions) {
   StageTimer t(c, RGPU_STAGE_NEIGHBOURS);
   c.have_rep = false;
@@ -314,6 +321,15 @@ void build_batch_nl(Context& c, const double* positions) {
   }
   build_pair_list(c.pl, c.n_struct, c.atom_off.data(), positions, c.cells.data(), c.pbc.data(),
                   c.hp.cutoff, c.stream);
+  // cell volumes for the stress (uploaded when the cells change)
+  std::vector<double> iv(static_cast<size_t>(c.n_struct));
+  for (int s = 0; s < c.n_struct; ++s) iv[size_t(s)] = 1.0 / cell_volume(&c.cells[9 * s]);
+  const double* before = c.d_inv_vol.get();
+  double* d = c.d_inv_vol.reserve(iv.size());
+  if (d != before || iv != c.inv_vol) {
+    RGPU_HIP(hipMemcpy(d, iv.data(), sizeof(double) * iv.size(), hipMemcpyHostToDevice));
+    c.inv_vol = iv;
+  }
 }
 
 PairView pair_view(Context& c) {
@@ -536,29 +552,28 @@ const float* invariants_of<float>(Context& c) {
   return ensure_p32(c);
 }
 
-double cell_volume(const double* C) {
-  return std::fabs(C[0] * (C[4] * C[8] - C[5] * C[7]) - C[1] * (C[3] * C[8] - C[5] * C[6]) +
-                   C[2] * (C[3] * C[7] - C[4] * C[6]));
-}
 
-// gradient/neg_stress outputs from device grad [n_atoms][3] and per-centre
-// virials
-void emit_forces(Context& c, double* grad_d, double* cvir_d, double* gradient, double* neg_stress) {
-  if (gradient)
-    RGPU_HIP(hipMemcpyAsync(gradient, grad_d, sizeof(double) * 3 * c.n_atoms, hipMemcpyDefault, c.stream));
-  if (neg_stress) {
-    double* ns = c.out64.reserve(size_t(6 * std::max(c.n_struct, 1)));
-    RGPU_HIP(hipMemsetAsync(ns, 0, sizeof(double) * 6 * c.n_struct, c.stream));
-    launch_segment_sum(cvir_d, 6, c.d_atom_off.get(), c.n_struct, ns, c.stream);
-    std::vector<double> h(static_cast<size_t>(6 * c.n_struct));
-    RGPU_HIP(hipMemcpyAsync(h.data(), ns, sizeof(double) * h.size(), hipMemcpyDeviceToHost, c.stream));
-    RGPU_HIP(hipStreamSynchronize(c.stream));
-    for (int s = 0; s < c.n_struct; ++s) {
-      const double vol = cell_volume(&c.cells[9 * s]);
-      for (int k = 0; k < 6; ++k) h[6 * s + k] /= vol;
-    }
-    RGPU_HIP(hipMemcpyAsync(neg_stress, h.data(), sizeof(double) * h.size(), hipMemcpyDefault, c.stream));
+// Outputs of a prediction: per-structure energies from per-centre energies
+// e_centre (device, may be null), gradient from device grad [n_atoms][3],
+// neg_stress from per-centre virials (summed per structure and divided by
+// the cell volume on the device).  One synchronisation at the end.
+void emit_outputs(Context& c, const double* e_centre, double* energy, double* grad_d,
+                  double* cvir_d, double* gradient, double* neg_stress) {
+  const int ns = c.n_struct;
+  double* es = energy && e_centre ? c.out_e.reserve(size_t(std::max(ns, 1))) : nullptr;
+  double* st = neg_stress ? c.out64.reserve(size_t(6 * std::max(ns, 1))) : nullptr;
+  if (es) {
+    RGPU_HIP(hipMemsetAsync(es, 0, sizeof(double) * ns, c.stream));
+    launch_segment_sum(e_centre, 1, c.d_atom_off.get(), ns, es, c.stream);
   }
+  if (st) {
+    RGPU_HIP(hipMemsetAsync(st, 0, sizeof(double) * 6 * ns, c.stream));
+    launch_segment_sum(cvir_d, 6, c.d_atom_off.get(), ns, st, c.stream, c.d_inv_vol.get());
+  }
+  if (es) RGPU_HIP(hipMemcpyAsync(energy, es, sizeof(double) * ns, hipMemcpyDefault, c.stream));
+  if (gradient && grad_d)
+    RGPU_HIP(hipMemcpyAsync(gradient, grad_d, sizeof(double) * 3 * c.n_atoms, hipMemcpyDefault, c.stream));
+  if (st) RGPU_HIP(hipMemcpyAsync(neg_stress, st, sizeof(double) * 6 * ns, hipMemcpyDefault, c.stream));
   RGPU_HIP(hipStreamSynchronize(c.stream));
 }
 
@@ -618,20 +633,15 @@ void gap_predict_t(Context& c, double* energy, double* gradient, double* neg_str
       run_gap_f64_force(blks.data(), S, wts.data(), G, c.stream);
     }
   }
-  if (energy) {
-    double* es = c.out64.reserve(size_t(std::max(c.n_struct, 1)));
-    RGPU_HIP(hipMemsetAsync(es, 0, sizeof(double) * c.n_struct, c.stream));
-    launch_segment_sum(ec, 1, c.d_atom_off.get(), c.n_struct, es, c.stream);
-    RGPU_HIP(hipMemcpyAsync(energy, es, sizeof(double) * c.n_struct, hipMemcpyDefault, c.stream));
-    RGPU_HIP(hipStreamSynchronize(c.stream));
+  double *gd = nullptr, *cv = nullptr;
+  if (want_grad) {
+    gd = c.grad.reserve(size_t(std::max<int64_t>(3 * n, 1)));
+    cv = c.cvir.reserve(size_t(std::max<int64_t>(6 * n, 1)));
+    RGPU_HIP(hipMemsetAsync(gd, 0, sizeof(double) * 3 * n, c.stream));
+    // the GAP energy is homogeneous of degree zeta in X, so X_i . G_i = zeta E_i
+    backward_device<T>(c, G, gd, cv, ec, double(c.zeta));
   }
-  if (!want_grad) return;
-  double* gd = c.grad.reserve(size_t(std::max<int64_t>(3 * n, 1)));
-  double* cv = c.cvir.reserve(size_t(std::max<int64_t>(6 * n, 1)));
-  RGPU_HIP(hipMemsetAsync(gd, 0, sizeof(double) * 3 * n, c.stream));
-  // the GAP energy is homogeneous of degree zeta in X, so X_i . G_i = zeta E_i
-  backward_device<T>(c, G, gd, cv, ec, double(c.zeta));
-  emit_forces(c, gd, cv, gradient, neg_stress);
+  emit_outputs(c, ec, energy, gd, cv, gradient, neg_stress);
 }
 
 template <class T>
@@ -1106,7 +1116,7 @@ int rgpu_features_backward(rgpu_context* ctx, const double* dL_dX, double* dL_dr
       RGPU_HIP(hipStreamSynchronize(c.stream));
       backward_device<float>(c, g32, gd, cv);
     }
-    emit_forces(c, gd, cv, dL_dr, neg_stress);
+    emit_outputs(c, nullptr, nullptr, gd, cv, dL_dr, neg_stress);
   });
 }
 

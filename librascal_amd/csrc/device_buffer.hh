@@ -42,4 +42,35 @@ class DeviceBuffer {
   size_t cap_ = 0, size_ = 0;
 };
 
+// Growable page-locked host buffer (hipHostMalloc / hipHostFree) for
+// staging small transfers asynchronously.
+template <class T>
+class PinnedBuffer {
+ public:
+  PinnedBuffer() = default;
+  PinnedBuffer(const PinnedBuffer&) = delete;
+  PinnedBuffer& operator=(const PinnedBuffer&) = delete;
+  ~PinnedBuffer() { release(); }
+
+  T* reserve(size_t n) {
+    if (n > cap_) {
+      release();
+      RGPU_HIP(hipHostMalloc(reinterpret_cast<void**>(&ptr_), (n == 0 ? 1 : n) * sizeof(T),
+                             hipHostMallocDefault));
+      cap_ = n;
+    }
+    return ptr_;
+  }
+  void release() {
+    if (ptr_) (void)hipHostFree(ptr_);
+    ptr_ = nullptr;
+    cap_ = 0;
+  }
+  T* get() const { return ptr_; }
+
+ private:
+  T* ptr_ = nullptr;
+  size_t cap_ = 0;
+};
+
 }  // namespace rgpu

@@ -44,6 +44,12 @@ struct PairList {
   DeviceBuffer<int64_t> scratch, scratch2, scratch3;  // scan tile sums (one per scan)
   DeviceBuffer<int64_t> totals;      // [3] ghosts, pairs, error flag
   int64_t cap_ext = 0, cap_pairs = 0;  // allocated sizes of the extended set / pair list
+  // host staging: positions and totals (page-locked), and the last uploaded
+  // structure map and geometry (not uploaded again while unchanged)
+  PinnedBuffer<double> pos_stage;
+  PinnedBuffer<int64_t> tot_stage;
+  std::vector<int> astruct_uploaded;
+  std::vector<NLGeom> geom_uploaded;
 };
 
 // Host inputs: positions [n_atoms][3], cells [n_struct][9] (lattice vectors

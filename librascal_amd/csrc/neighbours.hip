@@ -4,6 +4,7 @@
 // inclusive cutoff test takes the same decisions.
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 
 #include "neighbours.hh"
 #include "scan.hh"
@@ -336,15 +337,30 @@ void build_pair_list(PairList& pl, int n_struct, const int64_t* atom_offsets,
     RGPU_HIP(hipMemsetAsync(pl.centre_off.get(), 0, sizeof(int64_t), stream));
     return;
   }
+  // geometry and structure map: uploaded again only when they change (the
+  // buffers keep their addresses while they do not grow)
+  const NLGeom* geom_before = pl.geom.get();
   NLGeom* dgeom = pl.geom.reserve(n_struct);
-  RGPU_HIP(hipMemcpyAsync(dgeom, pl.geom_host.data(), sizeof(NLGeom) * n_struct,
-                          hipMemcpyHostToDevice, stream));
+  if (dgeom != geom_before || pl.geom_uploaded.size() != pl.geom_host.size() ||
+      std::memcmp(pl.geom_uploaded.data(), pl.geom_host.data(), sizeof(NLGeom) * n_struct) != 0) {
+    RGPU_HIP(hipMemcpyAsync(dgeom, pl.geom_host.data(), sizeof(NLGeom) * n_struct,
+                            hipMemcpyHostToDevice, stream));
+    pl.geom_uploaded = pl.geom_host;
+  }
+  const int* struct_before = pl.atom_struct.get();
   int* dstruct = pl.atom_struct.reserve(n_atoms);
-  RGPU_HIP(hipMemcpyAsync(dstruct, astruct.data(), sizeof(int) * n_atoms, hipMemcpyHostToDevice,
-                          stream));
+  astruct.resize(size_t(n_atoms));
+  if (dstruct != struct_before || pl.astruct_uploaded != astruct) {
+    RGPU_HIP(hipMemcpyAsync(dstruct, astruct.data(), sizeof(int) * n_atoms, hipMemcpyHostToDevice,
+                            stream));
+    pl.astruct_uploaded = astruct;
+  }
+  // positions through page-locked staging (the previous build synchronised,
+  // so the staging buffer is free)
   double* dpos = pl.pos.reserve(3 * n_atoms);
-  RGPU_HIP(hipMemcpyAsync(dpos, positions, sizeof(double) * 3 * n_atoms, hipMemcpyHostToDevice,
-                          stream));
+  double* ps = pl.pos_stage.reserve(size_t(3 * n_atoms));
+  std::memcpy(ps, positions, sizeof(double) * 3 * n_atoms);
+  RGPU_HIP(hipMemcpyAsync(dpos, ps, sizeof(double) * 3 * n_atoms, hipMemcpyHostToDevice, stream));
 
   // Sizes of the extended (ghost) set and of the pair list are only known on
   // the device; the buffers are sized by capacities kept from earlier builds,
@@ -412,8 +428,8 @@ void build_pair_list(PairList& pl, int n_struct, const int64_t* atom_offsets,
     hipLaunchKernelGGL(nl_totals_kernel, dim3(1), dim3(64), 0, stream, goff + n_atoms, coff + n_atoms,
                        err, dtot);
     RGPU_LAUNCH_CHECK();
-    int64_t tot[3];
-    RGPU_HIP(hipMemcpyAsync(tot, dtot, sizeof(tot), hipMemcpyDeviceToHost, stream));
+    int64_t* tot = pl.tot_stage.reserve(3);
+    RGPU_HIP(hipMemcpyAsync(tot, dtot, sizeof(int64_t) * 3, hipMemcpyDeviceToHost, stream));
     RGPU_HIP(hipStreamSynchronize(stream));
     const int64_t n_ext = n_atoms + tot[0], n_pairs = tot[1];
     if (n_ext <= cap_ext && n_pairs <= cap_pairs) {

@@ -84,6 +84,49 @@ __global__ static __launch_bounds__(kScanThreads) void scan_small_kernel(
   if (threadIdx.x == 0) *total = carry_sh;
 }
 
+// Whole scan in one workgroup of 1024 threads (8 items each per chunk,
+// running carry between chunks), out[n] = total; for short arrays, where
+// one launch beats three.
+constexpr int kScan1Threads = 1024;
+template <class In>
+__global__ __launch_bounds__(kScan1Threads) void scan_single_kernel(const In* __restrict__ in,
+                                                                    int64_t* __restrict__ out,
+                                                                    int64_t n) {
+  __shared__ int64_t wsum[kScan1Threads / 64];
+  __shared__ int64_t carry_sh;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (threadIdx.x == 0) carry_sh = 0;
+  __syncthreads();
+  for (int64_t c0 = 0; c0 < n; c0 += int64_t(kScan1Threads) * kScanItems) {
+    const int64_t base = c0 + int64_t(threadIdx.x) * kScanItems;
+    int64_t v[kScanItems];
+    int64_t tsum = 0;
+    for (int k = 0; k < kScanItems; ++k) {
+      v[k] = base + k < n ? int64_t(in[base + k]) : 0;
+      tsum += v[k];
+    }
+    int64_t incl = tsum;
+    for (int off = 1; off < 64; off <<= 1) {
+      const int64_t o = __shfl_up(incl, off, 64);
+      if (lane >= off) incl += o;
+    }
+    if (lane == 63) wsum[wave] = incl;
+    __syncthreads();
+    int64_t run = carry_sh;
+    for (int w = 0; w < wave; ++w) run += wsum[w];
+    const int64_t last = run + incl;  // inclusive end of this thread
+    run += incl - tsum;
+    for (int k = 0; k < kScanItems; ++k) {
+      if (base + k < n) out[base + k] = run;
+      run += v[k];
+    }
+    __syncthreads();
+    if (threadIdx.x == kScan1Threads - 1) carry_sh = last;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[n] = carry_sh;
+}
+
 // out[i] = sum_{k<i} in[k] for i < n and out[n] = total, all on the device
 // (no synchronisation); out has n+1 elements.
 template <class In>
@@ -91,6 +134,11 @@ void exclusive_scan_device(const In* in, int64_t* out, int64_t n, hipStream_t st
                            DeviceBuffer<int64_t>& scratch) {
   if (n == 0) {
     RGPU_HIP(hipMemsetAsync(out, 0, sizeof(int64_t), stream));
+    return;
+  }
+  if (n <= 8 * int64_t(kScan1Threads) * kScanItems) {
+    hipLaunchKernelGGL(scan_single_kernel<In>, dim3(1), dim3(kScan1Threads), 0, stream, in, out, n);
+    RGPU_LAUNCH_CHECK();
     return;
   }
   const int64_t tiles = (n + kScanTile - 1) / kScanTile;

@@ -81,9 +81,10 @@ void launch_backward_split(const RadialDev& rad, int S, const PairView& pv, cons
                            const float* G, double* grad, double* centre_virial, hipStream_t stream,
                            const SoapTiles& tiles = SoapTiles());
 
-// out[s][:w] += sum of in[a][:w] over a in [seg_off[s], seg_off[s+1])
+// out[s][:w] += (scale ? scale[s] : 1) * sum of in[a][:w] over a in
+// [seg_off[s], seg_off[s+1])
 void launch_segment_sum(const double* in, int w, const int64_t* seg_off, int n_seg, double* out,
-                        hipStream_t stream);
+                        hipStream_t stream, const double* scale = nullptr);
 // raw radial integrals (FP64 Hyp1f1) raw[pt][n][l] (+ derivatives, may be null)
 void launch_radial_direct(const RadialDev& rad, const double* r, int64_t n_pts, double* raw,
                           double* draw, hipStream_t stream);

@@ -1113,10 +1113,11 @@ __global__ void pair_cutoff_kernel(RadialDev rad, const int64_t* __restrict__ n_
   dfc[p] = df;
 }
 
-// per-structure (segment) sums of per-centre rows of width w <= 8 (+=)
+// per-structure (segment) sums of per-centre rows of width w <= 8, times
+// scale[s] when given (+=)
 __global__ void segment_sum_kernel(const double* __restrict__ in, int w,
                                    const int64_t* __restrict__ seg_off, int n_seg,
-                                   double* __restrict__ out) {
+                                   const double* __restrict__ scale, double* __restrict__ out) {
   const int s = blockIdx.x;
   if (s >= n_seg) return;
   __shared__ double red[kWaves][8];
@@ -1131,7 +1132,7 @@ __global__ void segment_sum_kernel(const double* __restrict__ in, int w,
   if (threadIdx.x < w) {
     double t = 0;
     for (int x = 0; x < kWaves; ++x) t += red[x][threadIdx.x];
-    out[int64_t(w) * s + threadIdx.x] += t;
+    out[int64_t(w) * s + threadIdx.x] += scale ? t * scale[s] : t;
   }
 }
 
@@ -1461,10 +1462,10 @@ void launch_pair_cutoff(const RadialDev& rad, const int64_t* n_pairs, int64_t ca
 }
 
 void launch_segment_sum(const double* in, int w, const int64_t* seg_off, int n_seg, double* out,
-                        hipStream_t stream) {
+                        hipStream_t stream, const double* scale) {
   if (n_seg == 0) return;
   hipLaunchKernelGGL(segment_sum_kernel, dim3(unsigned(n_seg)), dim3(kThreads), 0, stream, in, w, seg_off,
-                     n_seg, out);
+                     n_seg, scale, out);
   RGPU_LAUNCH_CHECK();
 }
 

@@ -66,6 +66,25 @@ def test_neighbour_list_bit_exact(name):
         a0 += n
 
 
+def test_neighbour_list_bit_exact_large_box():
+    """74088 atoms (jittered simple cubic lattice, periodic): more atoms than
+    one workgroup scans, so the ghost and pair offsets take the multi-block
+    scan."""
+    m, a = 42, 1.5
+    rng = np.random.default_rng(11)
+    grid = np.stack(np.meshgrid(*[np.arange(m)] * 3, indexing="ij"), -1).reshape(-1, 3)
+    pos = (grid + 0.5 + rng.uniform(-0.2, 0.2, size=grid.shape)) * a
+    st = dict(positions=pos, numbers=np.where(rng.random(len(pos)) < 0.5, 1, 8).astype(np.int32),
+              cell=np.eye(3) * m * a, pbc=np.ones(3, np.int32))
+    ctx, _ = make_ctx(soap_hypers(rc=3.0, N=2, L=1), [st])
+    off, pj, vec = ctx.neighbours()
+    ro, rj, rv, _ = O.neighbours(st["positions"], st["cell"], st["pbc"], 3.0)
+    assert len(pos) > 65536
+    assert np.array_equal(off, ro)
+    assert np.array_equal(pj, rj)
+    assert np.array_equal(vec, rv)
+
+
 def test_ylm_parity_fp64_and_fp32():
     rng = np.random.default_rng(3)
     dirs = rng.normal(size=(200, 3))
