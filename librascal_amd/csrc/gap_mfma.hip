@@ -143,7 +143,9 @@ __device__ __forceinline__ void gemm_tile(const ProblemSet& ps, const Problem& P
     fs[j] = frag_slot(r >> 5, sc, (r & 31) + 32 * shalf);
   }
 
-  uint4 rah[2], ral[2], rbh[2], rbl[2];
+  // two register stages: the global loads of K step kt + 2 are issued while
+  // step kt computes, and stored to LDS at the end of step kt + 1
+  uint4 rah[2][2], ral[2][2], rbh[2][2], rbl[2][2];
   auto load8 = [&](const unsigned short* p, int64_t k) -> uint4 {
     if (FAST) return *reinterpret_cast<const uint4*>(p + k);
     unsigned short t[8];
@@ -153,23 +155,23 @@ __device__ __forceinline__ void gemm_tile(const ProblemSet& ps, const Problem& P
     memcpy(&v, t, 16);
     return v;
   };
-  auto gload = [&](int64_t k0) {
+  auto gload = [&](int rs_, int64_t k0) {
     const int64_t k = k0 + 8 * kq;
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
-      rah[j] = load8(Ah[j], k);
-      ral[j] = load8(Al[j], k);
-      rbh[j] = load8(Bh[j], k);
-      rbl[j] = load8(Bl[j], k);
+      rah[rs_][j] = load8(Ah[j], k);
+      ral[rs_][j] = load8(Al[j], k);
+      rbh[rs_][j] = load8(Bh[j], k);
+      rbl[rs_][j] = load8(Bl[j], k);
     }
   };
-  auto lstore = [&](int st) {
+  auto lstore = [&](int rs_, int st) {
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
-      sAh[st][fs[j]] = __builtin_bit_cast(bf16x8, rah[j]);
-      sAl[st][fs[j]] = __builtin_bit_cast(bf16x8, ral[j]);
-      sBh[st][fs[j]] = __builtin_bit_cast(bf16x8, rbh[j]);
-      sBl[st][fs[j]] = __builtin_bit_cast(bf16x8, rbl[j]);
+      sAh[st][fs[j]] = __builtin_bit_cast(bf16x8, rah[rs_][j]);
+      sAl[st][fs[j]] = __builtin_bit_cast(bf16x8, ral[rs_][j]);
+      sBh[st][fs[j]] = __builtin_bit_cast(bf16x8, rbh[rs_][j]);
+      sBl[st][fs[j]] = __builtin_bit_cast(bf16x8, rbl[rs_][j]);
     }
   };
 
@@ -181,13 +183,7 @@ __device__ __forceinline__ void gemm_tile(const ProblemSet& ps, const Problem& P
 #pragma unroll
       for (int e = 0; e < 16; ++e) acc[i][j][e] = 0.f;
 
-  const int64_t nk = (P.K + BK - 1) / BK;
-  gload(0);
-  lstore(0);
-  __syncthreads();
-  for (int64_t kt = 0; kt < nk; ++kt) {
-    const int st = int(kt & 1);
-    if (kt + 1 < nk) gload((kt + 1) * BK);
+  auto compute = [&](int st) {
 #pragma unroll
     for (int c = 0; c < BK / 16; ++c) {
       bf16x8 ah[2], al[2], bh[2], bl[2];
@@ -212,8 +208,25 @@ __device__ __forceinline__ void gemm_tile(const ProblemSet& ps, const Problem& P
           acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
         }
     }
-    // the other stage was last read before the previous barrier
-    if (kt + 1 < nk) lstore(st ^ 1);
+  };
+
+  const int64_t nk = (P.K + BK - 1) / BK;
+  gload(0, 0);
+  if (nk > 1) gload(1, BK);
+  lstore(0, 0);
+  __syncthreads();
+  // K steps in pairs so that the register stage indices are constants;
+  // step kt reads LDS stage kt & 1 and register stage kt & 1 is refilled
+  for (int64_t kt = 0; kt < nk; kt += 2) {
+    if (kt + 2 < nk) gload(0, (kt + 2) * BK);
+    compute(0);
+    // the other LDS stage was last read before the previous barrier
+    if (kt + 1 < nk) lstore(1, 1);
+    __syncthreads();
+    if (kt + 1 >= nk) break;
+    if (kt + 3 < nk) gload(1, (kt + 3) * BK);
+    compute(1);
+    if (kt + 2 < nk) lstore(0, 0);
     __syncthreads();
   }
 

@@ -1115,14 +1115,15 @@ __global__ void pair_cutoff_kernel(RadialDev rad, const int64_t* __restrict__ n_
 
 // per-structure (segment) sums of per-centre rows of width w <= 8, times
 // scale[s] when given (+=)
-__global__ void segment_sum_kernel(const double* __restrict__ in, int w,
+constexpr int kSegThreads = 1024;
+__global__ __launch_bounds__(kSegThreads) void segment_sum_kernel(const double* __restrict__ in, int w,
                                    const int64_t* __restrict__ seg_off, int n_seg,
                                    const double* __restrict__ scale, double* __restrict__ out) {
   const int s = blockIdx.x;
   if (s >= n_seg) return;
-  __shared__ double red[kWaves][8];
+  __shared__ double red[kSegThreads / 64][8];
   double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-  for (int64_t a = seg_off[s] + threadIdx.x; a < seg_off[s + 1]; a += kThreads)
+  for (int64_t a = seg_off[s] + threadIdx.x; a < seg_off[s + 1]; a += kSegThreads)
     for (int k = 0; k < w; ++k) v[k] += in[int64_t(w) * a + k];
   for (int k = 0; k < w; ++k)
     for (int off = 32; off > 0; off >>= 1) v[k] += __shfl_down(v[k], off, 64);
@@ -1131,7 +1132,7 @@ __global__ void segment_sum_kernel(const double* __restrict__ in, int w,
   __syncthreads();
   if (threadIdx.x < w) {
     double t = 0;
-    for (int x = 0; x < kWaves; ++x) t += red[x][threadIdx.x];
+    for (int x = 0; x < kSegThreads / 64; ++x) t += red[x][threadIdx.x];
     out[int64_t(w) * s + threadIdx.x] += scale ? t * scale[s] : t;
   }
 }
@@ -1464,7 +1465,7 @@ void launch_pair_cutoff(const RadialDev& rad, const int64_t* n_pairs, int64_t ca
 void launch_segment_sum(const double* in, int w, const int64_t* seg_off, int n_seg, double* out,
                         hipStream_t stream, const double* scale) {
   if (n_seg == 0) return;
-  hipLaunchKernelGGL(segment_sum_kernel, dim3(unsigned(n_seg)), dim3(kThreads), 0, stream, in, w, seg_off,
+  hipLaunchKernelGGL(segment_sum_kernel, dim3(unsigned(n_seg)), dim3(kSegThreads), 0, stream, in, w, seg_off,
                      n_seg, scale, out);
   RGPU_LAUNCH_CHECK();
 }
