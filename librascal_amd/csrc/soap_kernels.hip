@@ -315,7 +315,38 @@ __device__ PairBatch<T> carve_batch(unsigned char* base, int PB, int LL, int F) 
   return pb;
 }
 
-// Stages pairs [pbase, pbase + nb) of the pair list: distances, directions,
+// Inputs of one batch of pairs held in registers (thread t holds pair t of
+// the batch), fetched a batch ahead so that their global latency overlaps
+// the work on the previous batch.
+struct PairPre {
+  double d, vx, vy, vz, fc, dfc, xhi;
+  int j, sp, klo;
+};
+
+template <bool DIRECT, bool ADJ>
+__device__ __forceinline__ void pair_fetch(const RadialDev& rad, const PairView& pv, int64_t pbase,
+                                           int nb, PairPre& f) {
+  if (int(threadIdx.x) < nb) {
+    const int64_t p = pbase + threadIdx.x;
+    f.d = pv.pair_dist[p];
+    f.vx = pv.pair_vec[3 * p];
+    f.vy = pv.pair_vec[3 * p + 1];
+    f.vz = pv.pair_vec[3 * p + 2];
+    f.fc = pv.pair_fc[p];
+    if (ADJ) f.dfc = pv.pair_dfc[p];
+    f.j = pv.pair_j[p];
+    f.sp = pv.species_of[f.j];
+    if (!DIRECT) {
+      // spline cell as in spline_cell()
+      int k = int((f.d - rad.x1) * rad.per_unit) - 1;
+      k = k < 0 ? 0 : (k > rad.search_size ? rad.search_size : k);
+      f.klo = k;
+      f.xhi = rad.grid[k + 1];
+    }
+  }
+}
+
+// Stages a batch of nb pairs (inputs fetched by pair_fetch): distances, directions,
 // cutoff values, neighbour species, Y_lm (and gradients) and the finalised
 // radial functions (and derivatives); in the forward R is multiplied by fc.
 // When lists are requested (forward), wavefront 0 groups the batch by
@@ -323,33 +354,31 @@ __device__ PairBatch<T> carve_batch(unsigned char* base, int PB, int LL, int F) 
 // no Y_lm work take the radial functions.  All threads call; returns after
 // a barrier.
 template <class T, bool DIRECT, bool ADJ>
-__device__ void stage_pairs(const RadialDev& rad, const T* yc, const PairView& pv, int64_t pbase,
-                            int nb, PairBatch<T>& b, int* s_start, int* s_count,
+__device__ void stage_pairs(const RadialDev& rad, const T* yc, const PairPre& f, int nb,
+                            PairBatch<T>& b, int* s_start, int* s_count,
                             unsigned long long* kmask) {
   const int N = rad.N, L = rad.L, L1 = rad.L1, LL = rad.LL, F = N * L1;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   if (tid < nb) {
-    const int64_t p = pbase + tid;
-    const double d = pv.pair_dist[p];
+    const double d = f.d;
     b.dist[tid] = d;
-    b.dir[3 * tid] = T(pv.pair_vec[3 * p] / d);
-    b.dir[3 * tid + 1] = T(pv.pair_vec[3 * p + 1] / d);
-    b.dir[3 * tid + 2] = T(pv.pair_vec[3 * p + 2] / d);
-    b.fc[tid] = pv.pair_fc[p];
-    const int j = pv.pair_j[p];
+    b.dir[3 * tid] = T(f.vx / d);
+    b.dir[3 * tid + 1] = T(f.vy / d);
+    b.dir[3 * tid + 2] = T(f.vz / d);
+    b.fc[tid] = f.fc;
     if (ADJ) {
-      b.dfc[tid] = pv.pair_dfc[p];
-      b.vec[3 * tid] = pv.pair_vec[3 * p];
-      b.vec[3 * tid + 1] = pv.pair_vec[3 * p + 1];
-      b.vec[3 * tid + 2] = pv.pair_vec[3 * p + 2];
-      b.pj[tid] = j;
+      b.dfc[tid] = f.dfc;
+      b.vec[3 * tid] = f.vx;
+      b.vec[3 * tid + 1] = f.vy;
+      b.vec[3 * tid + 2] = f.vz;
+      b.pj[tid] = f.j;
     }
-    b.sp[tid] = pv.species_of[j];
+    b.sp[tid] = f.sp;
     if (!DIRECT) {
-      const SplineCell<T> cell = spline_cell<T>(rad, d);
-      b.klo[tid] = cell.klo;
-      b.ca[tid] = cell.a;
-      b.cb[tid] = cell.b;
+      const double a = (f.xhi - d) / rad.h;
+      b.klo[tid] = f.klo;
+      b.ca[tid] = T(a);
+      b.cb[tid] = T(1 - a);
     }
   }
   __syncthreads();
@@ -477,10 +506,14 @@ __global__ __launch_bounds__(kThreads, (expansion_min_waves<T, KPT, DIRECT>())) 
   if (tid == 0) kmask = 1ull << si;
   T* ci = cout + i * int64_t(S) * B;
 
+  PairPre pre;
+  pair_fetch<DIRECT, false>(rad, pv, p0, max(0, min(PB, np)), pre);
   for (int q0 = 0; q0 == 0 || q0 < np; q0 += PB) {
     const int nb = max(0, min(PB, np - q0));
     if (tid < S) s_count[tid] = 0;
-    stage_pairs<T, DIRECT, false>(rad, yc, pv, p0 + q0, nb, pb, s_start, s_count, &kmask);
+    stage_pairs<T, DIRECT, false>(rad, yc, pre, nb, pb, s_start, s_count, &kmask);
+    // next batch's inputs in flight during this batch's accumulation
+    if (q0 + PB < np) pair_fetch<DIRECT, false>(rad, pv, p0 + q0 + PB, min(PB, np - q0 - PB), pre);
     for (int s = 0; s < S; ++s) {
       const int cnt = s_count[s];
       if (q0 > 0 && cnt == 0) continue;
@@ -656,9 +689,13 @@ __device__ __forceinline__ void adjoint_pairs(const RadialDev& rad, int S, int P
   const int64_t p0 = pv.centre_off[i] + 1;
   const int np = int(pv.centre_off[i + 1] - p0);
   double fi[3] = {0, 0, 0}, v6[6] = {0, 0, 0, 0, 0, 0};
+  PairPre pre;
+  pair_fetch<DIRECT, true>(rad, pv, p0, min(PB, np), pre);
   for (int q0 = 0; q0 < np; q0 += PB) {
     const int nb = min(PB, np - q0);
-    stage_pairs<T, DIRECT, true>(rad, yc, pv, p0 + q0, nb, pb, nullptr, nullptr, nullptr);
+    stage_pairs<T, DIRECT, true>(rad, yc, pre, nb, pb, nullptr, nullptr, nullptr);
+    // next batch's inputs in flight during this batch's contraction
+    if (q0 + PB < np) pair_fetch<DIRECT, true>(rad, pv, p0 + q0 + PB, min(PB, np - q0 - PB), pre);
     for (int qb = 4 * wave; qb < nb; qb += 4 * kWaves) {
       const int q = qb + grp;
       const bool act = q < nb;
@@ -920,7 +957,7 @@ __global__ __launch_bounds__(kThreads) void adjoint_kernel(
 //  3. the pair phase, its batches over the g region.
 // UPT = units per thread (ceil(S * ceil(N/4) * LL / 256)).
 template <class T, bool DIRECT, bool PSPLIT, int UPT>
-__global__ __launch_bounds__(kThreads) void adjoint_resident_kernel(
+__global__ __launch_bounds__(kThreads, (UPT <= 5 ? 4 : 1)) void adjoint_resident_kernel(
     RadialDev rad, int S, int PB, int KCH, PairView pv, const T* __restrict__ c, const T* __restrict__ p,
     const unsigned short* __restrict__ p_hi, const unsigned short* __restrict__ p_lo,
     const double* __restrict__ norm2, int normalize, const double* __restrict__ xg, double xg_scale,
@@ -995,22 +1032,42 @@ __global__ __launch_bounds__(kThreads) void adjoint_resident_kernel(
       V4* g4 = reinterpret_cast<V4*>(gl);
       const unsigned short* ph = PSPLIT ? p_hi + i * int64_t(D) + e0 : nullptr;
       const unsigned short* plo = PSPLIT ? p_lo + i * int64_t(D) + e0 : nullptr;
-      for (int q = tid; q < (ne >> 2); q += kThreads) {
-        V4 gv = G4[q];
-        V4 pv4;
-        if (PSPLIT) {
-          const uint2 h = reinterpret_cast<const uint2*>(ph)[q];
-          const uint2 lo = reinterpret_cast<const uint2*>(plo)[q];
-          pv4[0] = T(__builtin_bit_cast(float, h.x << 16) + __builtin_bit_cast(float, lo.x << 16));
-          pv4[1] = T(__builtin_bit_cast(float, h.x & 0xffff0000u) +
-                     __builtin_bit_cast(float, lo.x & 0xffff0000u));
-          pv4[2] = T(__builtin_bit_cast(float, h.y << 16) + __builtin_bit_cast(float, lo.y << 16));
-          pv4[3] = T(__builtin_bit_cast(float, h.y & 0xffff0000u) +
-                     __builtin_bit_cast(float, lo.y & 0xffff0000u));
-        } else {
-          pv4 = reinterpret_cast<const V4*>(p + i * int64_t(D) + e0)[q];
+      const int nq = ne >> 2;
+      // kStageQ quads per thread per round, all loads issued before use
+      constexpr int kStageQ = 4;
+      for (int qr = 0; qr < nq; qr += kStageQ * kThreads) {
+        V4 gv[kStageQ], pv4[kStageQ];
+        uint2 h[kStageQ], lo[kStageQ];
+#pragma unroll
+        for (int r = 0; r < kStageQ; ++r) {
+          const int q = qr + r * kThreads + tid;
+          if (q < nq) {
+            gv[r] = G4[q];
+            if (PSPLIT) {
+              h[r] = reinterpret_cast<const uint2*>(ph)[q];
+              lo[r] = reinterpret_cast<const uint2*>(plo)[q];
+            } else {
+              pv4[r] = reinterpret_cast<const V4*>(p + i * int64_t(D) + e0)[q];
+            }
+          }
         }
-        g4[q] = ta * gv - tb * pv4;
+#pragma unroll
+        for (int r = 0; r < kStageQ; ++r) {
+          const int q = qr + r * kThreads + tid;
+          if (q < nq) {
+            if (PSPLIT) {
+              pv4[r][0] = T(__builtin_bit_cast(float, h[r].x << 16) +
+                            __builtin_bit_cast(float, lo[r].x << 16));
+              pv4[r][1] = T(__builtin_bit_cast(float, h[r].x & 0xffff0000u) +
+                            __builtin_bit_cast(float, lo[r].x & 0xffff0000u));
+              pv4[r][2] = T(__builtin_bit_cast(float, h[r].y << 16) +
+                            __builtin_bit_cast(float, lo[r].y << 16));
+              pv4[r][3] = T(__builtin_bit_cast(float, h[r].y & 0xffff0000u) +
+                            __builtin_bit_cast(float, lo[r].y & 0xffff0000u));
+            }
+            g4[q] = ta * gv[r] - tb * pv4[r];
+          }
+        }
       }
     } else {
       for (int o = tid; o < ne; o += kThreads) gl[o] = ta * Gi[e0 + o] - tb * pval(e0 + o);
@@ -1302,19 +1359,23 @@ void launch_adjoint_t(const RadialDev& rad, int S, const PairView& pv, const T* 
       if (p_hi) {
         if (upt <= 2)
           go(adjoint_resident_kernel<T, DIRECT, true, 2>);
+        else if (upt <= 3)
+          go(adjoint_resident_kernel<T, DIRECT, true, 3>);
         else if (upt <= 4)
           go(adjoint_resident_kernel<T, DIRECT, true, 4>);
-        else if (upt <= 6)
-          go(adjoint_resident_kernel<T, DIRECT, true, 6>);
+        else if (upt <= 5)
+          go(adjoint_resident_kernel<T, DIRECT, true, 5>);
         else
           go(adjoint_resident_kernel<T, DIRECT, true, 8>);
       } else {
         if (upt <= 2)
           go(adjoint_resident_kernel<T, DIRECT, false, 2>);
+        else if (upt <= 3)
+          go(adjoint_resident_kernel<T, DIRECT, false, 3>);
         else if (upt <= 4)
           go(adjoint_resident_kernel<T, DIRECT, false, 4>);
-        else if (upt <= 6)
-          go(adjoint_resident_kernel<T, DIRECT, false, 6>);
+        else if (upt <= 5)
+          go(adjoint_resident_kernel<T, DIRECT, false, 5>);
         else
           go(adjoint_resident_kernel<T, DIRECT, false, 8>);
       }
