@@ -689,13 +689,11 @@ __device__ __forceinline__ void adjoint_pairs(const RadialDev& rad, int S, int P
   const int64_t p0 = pv.centre_off[i] + 1;
   const int np = int(pv.centre_off[i + 1] - p0);
   double fi[3] = {0, 0, 0}, v6[6] = {0, 0, 0, 0, 0, 0};
-  PairPre pre;
-  pair_fetch<DIRECT, true>(rad, pv, p0, min(PB, np), pre);
   for (int q0 = 0; q0 < np; q0 += PB) {
     const int nb = min(PB, np - q0);
+    PairPre pre;
+    pair_fetch<DIRECT, true>(rad, pv, p0 + q0, nb, pre);
     stage_pairs<T, DIRECT, true>(rad, yc, pre, nb, pb, nullptr, nullptr, nullptr);
-    // next batch's inputs in flight during this batch's contraction
-    if (q0 + PB < np) pair_fetch<DIRECT, true>(rad, pv, p0 + q0 + PB, min(PB, np - q0 - PB), pre);
     for (int qb = 4 * wave; qb < nb; qb += 4 * kWaves) {
       const int q = qb + grp;
       const bool act = q < nb;
@@ -957,7 +955,7 @@ __global__ __launch_bounds__(kThreads) void adjoint_kernel(
 //  3. the pair phase, its batches over the g region.
 // UPT = units per thread (ceil(S * ceil(N/4) * LL / 256)).
 template <class T, bool DIRECT, bool PSPLIT, int UPT>
-__global__ __launch_bounds__(kThreads, (UPT <= 5 ? 4 : 1)) void adjoint_resident_kernel(
+__global__ __launch_bounds__(kThreads) void adjoint_resident_kernel(
     RadialDev rad, int S, int PB, int KCH, PairView pv, const T* __restrict__ c, const T* __restrict__ p,
     const unsigned short* __restrict__ p_hi, const unsigned short* __restrict__ p_lo,
     const double* __restrict__ norm2, int normalize, const double* __restrict__ xg, double xg_scale,
@@ -1032,42 +1030,22 @@ __global__ __launch_bounds__(kThreads, (UPT <= 5 ? 4 : 1)) void adjoint_resident
       V4* g4 = reinterpret_cast<V4*>(gl);
       const unsigned short* ph = PSPLIT ? p_hi + i * int64_t(D) + e0 : nullptr;
       const unsigned short* plo = PSPLIT ? p_lo + i * int64_t(D) + e0 : nullptr;
-      const int nq = ne >> 2;
-      // kStageQ quads per thread per round, all loads issued before use
-      constexpr int kStageQ = 4;
-      for (int qr = 0; qr < nq; qr += kStageQ * kThreads) {
-        V4 gv[kStageQ], pv4[kStageQ];
-        uint2 h[kStageQ], lo[kStageQ];
-#pragma unroll
-        for (int r = 0; r < kStageQ; ++r) {
-          const int q = qr + r * kThreads + tid;
-          if (q < nq) {
-            gv[r] = G4[q];
-            if (PSPLIT) {
-              h[r] = reinterpret_cast<const uint2*>(ph)[q];
-              lo[r] = reinterpret_cast<const uint2*>(plo)[q];
-            } else {
-              pv4[r] = reinterpret_cast<const V4*>(p + i * int64_t(D) + e0)[q];
-            }
-          }
+      for (int q = tid; q < (ne >> 2); q += kThreads) {
+        V4 gv = G4[q];
+        V4 pv4;
+        if (PSPLIT) {
+          const uint2 h = reinterpret_cast<const uint2*>(ph)[q];
+          const uint2 lo = reinterpret_cast<const uint2*>(plo)[q];
+          pv4[0] = T(__builtin_bit_cast(float, h.x << 16) + __builtin_bit_cast(float, lo.x << 16));
+          pv4[1] = T(__builtin_bit_cast(float, h.x & 0xffff0000u) +
+                     __builtin_bit_cast(float, lo.x & 0xffff0000u));
+          pv4[2] = T(__builtin_bit_cast(float, h.y << 16) + __builtin_bit_cast(float, lo.y << 16));
+          pv4[3] = T(__builtin_bit_cast(float, h.y & 0xffff0000u) +
+                     __builtin_bit_cast(float, lo.y & 0xffff0000u));
+        } else {
+          pv4 = reinterpret_cast<const V4*>(p + i * int64_t(D) + e0)[q];
         }
-#pragma unroll
-        for (int r = 0; r < kStageQ; ++r) {
-          const int q = qr + r * kThreads + tid;
-          if (q < nq) {
-            if (PSPLIT) {
-              pv4[r][0] = T(__builtin_bit_cast(float, h[r].x << 16) +
-                            __builtin_bit_cast(float, lo[r].x << 16));
-              pv4[r][1] = T(__builtin_bit_cast(float, h[r].x & 0xffff0000u) +
-                            __builtin_bit_cast(float, lo[r].x & 0xffff0000u));
-              pv4[r][2] = T(__builtin_bit_cast(float, h[r].y << 16) +
-                            __builtin_bit_cast(float, lo[r].y << 16));
-              pv4[r][3] = T(__builtin_bit_cast(float, h[r].y & 0xffff0000u) +
-                            __builtin_bit_cast(float, lo[r].y & 0xffff0000u));
-            }
-            g4[q] = ta * gv[r] - tb * pv4[r];
-          }
-        }
+        g4[q] = ta * gv - tb * pv4;
       }
     } else {
       for (int o = tid; o < ne; o += kThreads) gl[o] = ta * Gi[e0 + o] - tb * pval(e0 + o);
