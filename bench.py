@@ -155,11 +155,15 @@ def main():
     value = n_centres * args.steps / elm
     value64 = n_centres * steps64 / el64
 
-    # roofline of the dominant kernel: the K = (X T^T)^zeta GEMM
+    # roofline of the dominant kernel: the K = (X T^T)^zeta GEMM.  The mixed
+    # path multiplies the packed invariants (diagonal species blocks stored
+    # once, n1 <= n2): Dp columns instead of D, the same products.
     M_s = w["n_sparse_per_species"]
     rows = [int(np.sum(num == t)) for t in species]
-    # one grouped launch covers every species block: 2 * rows_s * M_s * D flop each
-    flop_gemm = sum(2.0 * r * M_s * D for r in rows)
+    S_, N_, L1_ = len(species), w["N"], w["L"] + 1
+    Dp = S_ * N_ * (N_ + 1) // 2 * L1_ + (S_ * (S_ + 1) // 2 - S_) * N_ * N_ * L1_
+    # one grouped launch covers every species block: 2 * rows_s * M_s * Dp flop each
+    flop_gemm = sum(2.0 * r * M_s * Dp for r in rows)
     k_ms, k_n = stm["gap_kernel"]
     k_avg_ms = k_ms / max(k_n, 1)
     achieved = flop_gemm / (k_avg_ms * 1e-3)
@@ -168,14 +172,13 @@ def main():
     roofline = dict(bound="mfma", achieved=achieved / 1e12, peak=peak / 1e12, unit="TFLOP/s",
                     frac=achieved / peak, traffic=traffic,
                     kernel="gap_kernel (K = (X T^T)^zeta, all species blocks in one launch)",
-                    avg_launch_ms=k_avg_ms, flop_per_launch=flop_gemm,
+                    avg_launch_ms=k_avg_ms, flop_per_launch=flop_gemm, packed_columns=Dp,
                     bf16_mfma_utilisation=3.0 * achieved / PEAK_BF16,
                     traffic_source=TRAFFIC_FILE if traffic is not None else None)
     # HBM-bound stages: algorithmic bytes per step / stage time (FP32 path)
     n_p = n_pairs
-    S_, N_, L1_ = len(species), w["N"], w["L"] + 1
     c_bytes = 4.0 * n_centres * S_ * N_ * L1_ * L1_
-    p_bytes = 4.0 * n_centres * D
+    p_bytes = 4.0 * n_centres * Dp  # packed invariants (hi + lo planes) or dL/dX
     pair_bytes = 56.0 * n_p
     stage_bytes = dict(expansion=c_bytes + pair_bytes, power_spectrum=c_bytes + p_bytes,
                        backward=2 * p_bytes + c_bytes + pair_bytes)

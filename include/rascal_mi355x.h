@@ -56,8 +56,7 @@ extern "C" {
 #define RGPU_INFO_TILE_PAIRS_FWD 13 /* autotuned pairs per batch, expansion (0 = default) */
 #define RGPU_INFO_TILE_KEYS_PS 14   /* autotuned key blocks per pass, power spectrum */
 #define RGPU_INFO_TILE_PAIRS_ADJ 15 /* autotuned pairs per batch, adjoint */
-#define RGPU_INFO_TILE_ADJ_RESIDENT 16 /* adjoint variant: -1 auto, 0 per key, 1 staged */
-#define RGPU_INFO_TILE_KEYS_ADJ 17  /* key blocks per pass of the staged adjoint (0 = auto) */
+#define RGPU_INFO_TILE_KEYS_ADJ 16  /* key blocks per pass of the FP32 adjoint (0 = auto) */
 
 /* stages timed by rgpu_enable_timing */
 #define RGPU_STAGE_NEIGHBOURS 0
@@ -90,18 +89,16 @@ int rgpu_set_stream(rgpu_context* ctx, void* hip_stream);
  * the reference (a CPU library). */
 int rgpu_enable_graphs(rgpu_context* ctx, int enable);
 /* Autotuner: times the candidate work partitions (pairs staged per batch in
- * the expansion and the adjoint, key blocks per pass in the power spectrum,
- * the adjoint variant) on the current batch and precision and keeps the
- * fastest for later calls
+ * the expansion and the adjoint, key blocks per pass in the power spectrum
+ * and in the mixed-precision adjoint) on the current batch and precision and
+ * keeps the fastest for later calls
  * (RGPU_INFO_TILE_*).  No counterpart in the reference. */
 int rgpu_autotune(rgpu_context* ctx);
 /* Sets the work partitions of the current precision directly (values <= 0:
- * built-in choice; adjoint_resident -1 auto, 0 one LDS update per key
- * block, 1 dL/dp staged in LDS keys_adj key blocks at a time with register
- * accumulation (FP32)), e.g. to reuse an autotuned set.  Results agree
- * between partitions to rounding.  No counterpart in the reference. */
-int rgpu_set_tiles(rgpu_context* ctx, int pairs_fwd, int keys_ps, int pairs_adj,
-                   int adjoint_resident, int keys_adj);
+ * built-in choice; keys_adj: key blocks of dL/dp staged in LDS per pass of
+ * the mixed-precision adjoint), e.g. to reuse an autotuned set.  Results
+ * agree between partitions to rounding.  No counterpart in the reference. */
+int rgpu_set_tiles(rgpu_context* ctx, int pairs_fwd, int keys_ps, int pairs_adj, int keys_adj);
 /* RGPU_FP64 (default) or RGPU_MIXED; invalidates computed results. */
 int rgpu_set_precision(rgpu_context* ctx, int precision);
 int rgpu_get_info(rgpu_context* ctx, int key, int64_t* value);

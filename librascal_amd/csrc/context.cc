@@ -17,6 +17,7 @@
 #include "gap_mfma.hh"
 #include "hypers.hh"
 #include "neighbours.hh"
+#include "packing.hh"
 #include "radial.hh"
 #include "soap_kernels.hh"
 
@@ -57,7 +58,7 @@ struct Context {
   bool have_rep = false;
   DeviceBuffer<double> c64, p64;
   DeviceBuffer<float> c32, p32;
-  DeviceBuffer<double> norm2;
+  DeviceBuffer<double> norm2, tmp_norm2;
   DeviceBuffer<unsigned long long> keymask;
 
   // sparse GAP model
@@ -156,6 +157,10 @@ struct Context {
     const int N = hp.max_radial, L1 = hp.max_angular + 1;
     if (hp.kind == SoapHypers::kPowerSpectrum) return int64_t(S) * (S + 1) / 2 * N * N * L1;
     return int64_t(S) * N * L1 * L1;
+  }
+  // columns of the packed invariants of the mixed path (PowerSpectrum)
+  int64_t Dp() const {
+    return packed_feature_size(int(species.size()), hp.max_radial, hp.max_angular + 1);
   }
 };
 
@@ -382,8 +387,9 @@ void compute_rep(Context& c) {
   // BF16 hi/lo operand planes of the matrix-core GEMMs
   constexpr bool split = std::is_same<T, float>::value;
   T* pp = ps && !split ? pbuf<T>(c).reserve(size_t(std::max<int64_t>(n * D, 1))) : nullptr;
-  unsigned short* ph = ps && split ? c.p_hi.reserve(size_t(std::max<int64_t>(n * D, 1))) : nullptr;
-  unsigned short* pl = ps && split ? c.p_lo.reserve(size_t(std::max<int64_t>(n * D, 1))) : nullptr;
+  const int64_t Dp = c.Dp();
+  unsigned short* ph = ps && split ? c.p_hi.reserve(size_t(std::max<int64_t>(n * Dp, 1))) : nullptr;
+  unsigned short* pl = ps && split ? c.p_lo.reserve(size_t(std::max<int64_t>(n * Dp, 1))) : nullptr;
   double* n2 = ps ? c.norm2.reserve(size_t(std::max<int64_t>(n, 1))) : nullptr;
   c.p32_valid = false;
   const int64_t* n_pairs_dev = c.pl.centre_off.get() + n;
@@ -527,15 +533,17 @@ void backward_device(Context& c, const T* G, double* grad, double* cvir, const d
                        c.hp.normalize, xg, xg_scale, G, grad, cvir, c.stream, c.tile());
 }
 
-// FP32 invariants of the mixed path (for export and K_NM), formed on demand
-// from the stored expansion; norm2 is recomputed with identical values
+// FP32 invariants of the mixed path in the full layout (for export and
+// K_NM), formed on demand from the stored expansion (the norms of the packed
+// pass are kept; this pass's equal them to rounding)
 const float* ensure_p32(Context& c) {
   if (!c.p32_valid) {
     const int64_t n = c.n_atoms;
     float* p = c.p32.reserve(size_t(std::max<int64_t>(n * c.D(), 1)));
+    double* n2 = c.tmp_norm2.reserve(size_t(std::max<int64_t>(n, 1)));
     if (n > 0)
-      launch_power_spectrum<float>(c.rd, int(c.species.size()), n, c.c32.get(), p, c.norm2.get(),
-                                   c.stream, c.tile());
+      launch_power_spectrum<float>(c.rd, int(c.species.size()), n, c.c32.get(), p, n2, c.stream,
+                                   c.tile());
     c.p32_valid = true;
   }
   return c.p32.get();
@@ -580,7 +588,8 @@ void emit_outputs(Context& c, const double* e_centre, double* energy, double* gr
 template <class T>
 void gap_predict_t(Context& c, double* energy, double* gradient, double* neg_stress) {
   const int S = int(c.species.size());
-  const int64_t n = c.n_atoms, D = c.D();
+  // the mixed path works on the packed invariants
+  const int64_t n = c.n_atoms, D = std::is_same<T, float>::value ? c.Dp() : c.D();
   const bool want_grad = gradient || neg_stress;
   double* ec = c.e_centre.reserve(size_t(std::max<int64_t>(n, 1)));
   RGPU_HIP(hipMemsetAsync(ec, 0, sizeof(double) * n, c.stream));
@@ -591,6 +600,7 @@ void gap_predict_t(Context& c, double* energy, double* gradient, double* neg_str
   bool rows_without_points = false;
   for (int s = 0; s < S; ++s) {
     blks.push_back(gap_block<T>(c, s, X, sparse_values<T>(c)));
+    blks.back().D = D;
     wt_off.push_back(wt_total);
     // blocks start on 8-element (16-byte) boundaries
     wt_total += (blks.back().n_rows * blks.back().n_cols + 7) / 8 * 8;
@@ -789,28 +799,25 @@ void autotune_t(Context& c) {
         return 1e30f;  // needs more LDS than the device provides
       }
     };
-    // variant (and key blocks per pass of the staged variant), then pairs
-    // per batch for the chosen variant
-    SoapTiles bestv = t;
-    bestv.pb_adj = 0;
-    bestv.adj_resident = 0;
-    bestv.keys_adj = 0;
-    best = time_adjoint(bestv);
+    // key blocks per pass of the FP32 adjoint (the built-in choice fits an
+    // LDS budget), then pairs per batch
     if (split) {
+      SoapTiles bestv = t;
+      bestv.pb_adj = 0;
+      bestv.keys_adj = 0;
+      best = time_adjoint(bestv);
       int last = -1;
       for (int div = 1; div <= 4; ++div) {
         const int kch = (K + div - 1) / div;
         if (kch == last) continue;
         last = kch;
         SoapTiles cand = bestv;
-        cand.adj_resident = 1;
         cand.keys_adj = kch;
         const float ms = time_adjoint(cand);
         if (ms < best) best = ms, bestv = cand;
       }
+      t.keys_adj = bestv.keys_adj;
     }
-    t.adj_resident = bestv.adj_resident;
-    t.keys_adj = bestv.keys_adj;
     best = 1e30f;
     pick = 0;
     for (int pb : {2, 4, 6, 8, 12, 16, 24, 32}) {
@@ -875,19 +882,15 @@ int rgpu_autotune(rgpu_context* ctx) {
   });
 }
 
-int rgpu_set_tiles(rgpu_context* ctx, int pairs_fwd, int keys_ps, int pairs_adj,
-                   int adjoint_resident, int keys_adj) {
+int rgpu_set_tiles(rgpu_context* ctx, int pairs_fwd, int keys_ps, int pairs_adj, int keys_adj) {
   return guarded([&] {
     if (!ctx) throw InvalidArgument("null context");
     if (pairs_fwd > 64 || pairs_adj > 64) throw InvalidArgument("at most 64 pairs per batch");
-    if (adjoint_resident < -1 || adjoint_resident > 1)
-      throw InvalidArgument("adjoint_resident must be -1, 0 or 1");
     Context& c = ctx->c;
     SoapTiles& t = c.tile();
     t.pb_fwd = std::max(pairs_fwd, 0);
     t.kc = std::max(keys_ps, 0);
     t.pb_adj = std::max(pairs_adj, 0);
-    t.adj_resident = adjoint_resident;
     t.keys_adj = std::max(keys_adj, 0);
     c.have_rep = false;
     c.drop_graph();
@@ -945,7 +948,6 @@ int rgpu_get_info(rgpu_context* ctx, int key, int64_t* value) {
       case RGPU_INFO_TILE_PAIRS_FWD: *value = c.tile().pb_fwd; break;
       case RGPU_INFO_TILE_KEYS_PS: *value = c.tile().kc; break;
       case RGPU_INFO_TILE_PAIRS_ADJ: *value = c.tile().pb_adj; break;
-      case RGPU_INFO_TILE_ADJ_RESIDENT: *value = c.tile().adj_resident; break;
       case RGPU_INFO_TILE_KEYS_ADJ: *value = c.tile().keys_adj; break;
       default: throw InvalidArgument("unknown info key");
     }
@@ -1107,13 +1109,9 @@ int rgpu_features_backward(rgpu_context* ctx, const double* dL_dX, double* dL_dr
     if (c.precision == RGPU_FP64) {
       backward_device<double>(c, g64, gd, cv);
     } else {
-      std::vector<double> h(static_cast<size_t>(n * D));
-      RGPU_HIP(hipMemcpyAsync(h.data(), g64, sizeof(double) * h.size(), hipMemcpyDeviceToHost, c.stream));
-      RGPU_HIP(hipStreamSynchronize(c.stream));
-      std::vector<float> hf(h.begin(), h.end());
-      float* g32 = c.G32.reserve(size_t(std::max<int64_t>(n * D, 1)));
-      RGPU_HIP(hipMemcpyAsync(g32, hf.data(), sizeof(float) * hf.size(), hipMemcpyHostToDevice, c.stream));
-      RGPU_HIP(hipStreamSynchronize(c.stream));
+      // dL/dX in the packed layout of the mixed path
+      float* g32 = c.G32.reserve(size_t(std::max<int64_t>(n * c.Dp(), 1)));
+      launch_pack_gradient(int(c.species.size()), c.rd.N, c.rd.L1, n, g64, g32, c.stream);
       backward_device<float>(c, g32, gd, cv);
     }
     emit_outputs(c, nullptr, nullptr, gd, cv, dL_dr, neg_stress);
@@ -1168,7 +1166,7 @@ int rgpu_set_sparse_points(rgpu_context* ctx, int zeta, int64_t n_sparse, const 
     c.zeta = zeta;
     c.M = n_sparse;
     c.have_weights = weights != nullptr;
-    prepare_mixed_sparse(c.mixed_sparse, T, n_sparse, D, c.stream);
+    prepare_mixed_sparse(c.mixed_sparse, T, n_sparse, S, c.rd.N, c.rd.L1, c.stream);
     c.have_sparse = true;
   });
 }

@@ -12,9 +12,10 @@
 
 namespace rgpu {
 
-// Sparse points pre-split into BF16 hi/lo planes, in the two layouts the
-// two GEMMs read: [M][D] (K-contiguous rows for K = X T^T) and [D][M]
-// (K-contiguous columns for T~ = W~ T).
+// Sparse points in the packed layout of the invariants (packing.hh),
+// pre-split into BF16 hi/lo planes, in the two layouts the two GEMMs read:
+// [M][D] (K-contiguous rows for K = X T^T) and [D][M] (K-contiguous
+// columns for T~ = W~ T); D here is the packed width.
 struct MixedSparse {
   int64_t M = 0, D = 0;
   DeviceBuffer<unsigned short> t_hi, t_lo;    // [M][D]
@@ -28,8 +29,10 @@ struct SplitPlanes {
 };
 
 bool mfma_available();
-void prepare_mixed_sparse(MixedSparse& ms, const std::vector<double>& T, int64_t M, int64_t D,
-                          hipStream_t stream);
+// T: [M][D] FP64 rows in the full layout of S species, N radial channels,
+// L1 = max_angular + 1
+void prepare_mixed_sparse(MixedSparse& ms, const std::vector<double>& T, int64_t M, int S, int N,
+                          int L1, hipStream_t stream);
 // K tile GEMM + epilogue for all species blocks in grouped launches.  The
 // features are given split: X = xs.hi + xs.lo, [n_centres][D] (rows gathered
 // through the blocks' row_idx, un-normalised, scaled by 1/|p| in the

@@ -23,6 +23,7 @@
 
 #include "common.hh"
 #include "gap_mfma.hh"
+#include "packing.hh"
 
 namespace rgpu {
 
@@ -368,23 +369,48 @@ void launch(ProblemSet& ps, int64_t total_tiles, int epi, hipStream_t stream) {
 
 bool mfma_available() { return true; }
 
-void prepare_mixed_sparse(MixedSparse& ms, const std::vector<double>& T, int64_t M, int64_t D,
-                          hipStream_t stream) {
+void prepare_mixed_sparse(MixedSparse& ms, const std::vector<double>& T, int64_t M, int S, int N,
+                          int L1, hipStream_t stream) {
+  // rows of T in the packed layout (packing.hh); a sparse point's diagonal
+  // blocks need not be symmetric: (t[n1][n2] + t[n2][n1]) / sqrt 2 keeps
+  // every product with a symmetric feature row exact
+  const int K = S * (S + 1) / 2, KB = N * N * L1, KD = packed_diag_block(N, L1);
+  const int64_t D = int64_t(K) * KB, Dp = packed_feature_size(S, N, L1);
   ms.M = M;
-  ms.D = D;
-  const size_t n = size_t(M * D);
+  ms.D = Dp;
+  const size_t n = size_t(M * Dp);
   std::vector<unsigned short> hi(std::max<size_t>(n, 1)), lo(std::max<size_t>(n, 1)),
       thi(std::max<size_t>(n, 1)), tlo(std::max<size_t>(n, 1));
-  for (int64_t m = 0; m < M; ++m)
-    for (int64_t d = 0; d < D; ++d) {
-      const float x = float(T[size_t(m * D + d)]);
+  std::vector<double> row(size_t(std::max<int64_t>(Dp, 1)));
+  for (int64_t m = 0; m < M; ++m) {
+    const double* src = T.data() + m * D;
+    for (int k = 0; k < K; ++k) {
+      int a, b;
+      key_pair(k, S, &a, &b);
+      const double* sk = src + int64_t(k) * KB;
+      double* dk = row.data() + packed_key_offset(a, b, S, N, L1);
+      if (a < b) {
+        std::copy(sk, sk + KB, dk);
+      } else {
+        for (int o = 0; o < KD; ++o) {
+          const int t = o / L1, l = o - t * L1;
+          int n1, n2;
+          packed_pair(t, N, &n1, &n2);
+          dk[o] = n1 < n2 ? (sk[(n1 * N + n2) * L1 + l] + sk[(n2 * N + n1) * L1 + l]) * 0.70710678118654752440
+                          : sk[(n1 * N + n1) * L1 + l];
+        }
+      }
+    }
+    for (int64_t d = 0; d < Dp; ++d) {
+      const float x = float(row[size_t(d)]);
       const unsigned short h = bf16_rne_host(x);
       const unsigned short l = bf16_rne_host(x - bf16_to_float_host(h));
-      hi[size_t(m * D + d)] = h;
-      lo[size_t(m * D + d)] = l;
+      hi[size_t(m * Dp + d)] = h;
+      lo[size_t(m * Dp + d)] = l;
       thi[size_t(d * M + m)] = h;
       tlo[size_t(d * M + m)] = l;
     }
+  }
   auto up = [&](DeviceBuffer<unsigned short>& b, const std::vector<unsigned short>& v) {
     unsigned short* p = b.reserve(v.size());
     RGPU_HIP(hipMemcpyAsync(p, v.data(), v.size() * 2, hipMemcpyHostToDevice, stream));

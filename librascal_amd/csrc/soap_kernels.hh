@@ -33,10 +33,7 @@ struct SoapTiles {
   int pb_fwd = 0;  // pairs staged per batch in the expansion (<= 64)
   int kc = 0;      // key blocks staged per pass in the power spectrum
   int pb_adj = 0;  // pairs staged per batch in the adjoint (<= 64)
-  // adjoint variant: -1 chooses (dL/dp resident in LDS when it fits),
-  // 0 one LDS pass per key block, 1 resident (when it fits)
-  int adj_resident = -1;
-  int keys_adj = 0;  // key blocks staged per pass in the resident adjoint
+  int keys_adj = 0;  // key blocks staged per pass in the FP32 adjoint
 };
 
 // c[i][s][n][l*l+l+m] (finalised) and the per-centre key mask.
@@ -50,9 +47,9 @@ template <class T>
 void launch_power_spectrum(const RadialDev& rad, int S, int64_t n_centres, const T* c, T* p,
                            double* norm2, hipStream_t stream, const SoapTiles& tiles = SoapTiles());
 
-// The same power spectrum written as the matrix-core operand planes
-// p = p_hi + p_lo (BF16 bit patterns, each rounded to nearest even) instead
-// of FP32 values; norm2 from the FP32 values.
+// The same power spectrum in the packed layout (packing.hh) written as the
+// matrix-core operand planes p = p_hi + p_lo (BF16 bit patterns, each
+// rounded to nearest even), [n][Dp]; norm2 from the FP32 values.
 void launch_power_spectrum_split(const RadialDev& rad, int S, int64_t n_centres, const float* c,
                                  unsigned short* p_hi, unsigned short* p_lo, double* norm2,
                                  hipStream_t stream, const SoapTiles& tiles = SoapTiles());
@@ -62,7 +59,7 @@ template <class T>
 void launch_normalise(int64_t n_centres, int64_t D, const T* p, const double* norm2, bool normalize,
                       double* out, hipStream_t stream);
 
-// Adjoint: given G = dL/dX per centre (X the exported features), accumulate
+// Adjoint (FP64): given G = dL/dX per centre (X the exported features), accumulate
 // dL/dr on atoms (grad[n_atoms][3], +=) and write per-centre virials
 // centre_virial[n_centres][6] (sum over the centre's pairs of r_ji (x)
 // dL/dr_j, Voigt xx,yy,zz,yz,xz,xy, not divided by the volume).  xg, when
@@ -74,7 +71,14 @@ void launch_backward(const RadialDev& rad, int S, const PairView& pv, const T* c
                      const T* G, double* grad, double* centre_virial, hipStream_t stream,
                      const SoapTiles& tiles = SoapTiles());
 
-// launch_backward<float> with p given as BF16 operand planes
+// Pack dL/dX [n][D] (full layout, FP64) into the packed layout of the mixed
+// path [n][Dp] (FP32): diagonal-key entries (G[n1][n2] + G[n2][n1]) / sqrt 2
+// for n1 < n2, G[n][n] on the diagonal.
+void launch_pack_gradient(int S, int N, int L1, int64_t n, const double* G, float* Gp,
+                          hipStream_t stream);
+
+// Adjoint of the mixed path: dL/dX (G) and p in the packed layout, p as its
+// BF16 operand planes (launch_power_spectrum_split)
 void launch_backward_split(const RadialDev& rad, int S, const PairView& pv, const float* c,
                            const unsigned short* p_hi, const unsigned short* p_lo,
                            const double* norm2, bool normalize, const double* xg, double xg_scale,

@@ -27,6 +27,7 @@
 #include <string>
 
 #include "soap_device.hh"
+#include "packing.hh"
 #include "soap_kernels.hh"
 
 namespace rgpu {
@@ -556,7 +557,7 @@ __global__ __launch_bounds__(kThreads, (expansion_min_waves<T, KPT, DIRECT>())) 
 // LDS with an odd stride (no bank conflicts between the tile rows).
 // Un-normalised; |p|^2 is returned per centre.
 template <class T, bool SPLIT>
-__global__ __launch_bounds__(kThreads) void power_spectrum_kernel(int N, int L, int S, int KC,
+__global__ __launch_bounds__(kThreads) void power_spectrum_kernel(int N, int L, int S, int KC, int64_t Dp,
                                                                   const T* __restrict__ c,
                                                                   T* __restrict__ p,
                                                                   unsigned short* __restrict__ p_hi,
@@ -599,6 +600,8 @@ __global__ __launch_bounds__(kThreads) void power_spectrum_kernel(int N, int L, 
       const int l = r2 - kk * L1;
       const int ab = key_ab[k0 + kk];
       const int a = ab >> 8, b = ab & 255;
+      // the packed output of a diagonal key needs n1 <= n2 only
+      if (SPLIT && a == b && t1 > t2) continue;
       const T* ca = cs + a * N * LS;
       const T* cb = cs + b * N * LS;
       const int n1 = 4 * t1, n2 = 4 * t2;
@@ -629,18 +632,42 @@ __global__ __launch_bounds__(kThreads) void power_spectrum_kernel(int N, int L, 
           if (n1 + u < N && n2 + v < N) sk[((n1 + u) * N + n2 + v) * L1 + l] = acc[u][v] * f;
     }
     __syncthreads();
-    const int64_t ob = pbase + int64_t(k0) * KB;
-    for (int o = threadIdx.x; o < kc * KB; o += kThreads) {
-      const T v = stage[o];
-      if (SPLIT) {
-        // the matrix-core operand planes: v = hi + lo (RNE BF16 each)
-        const __bf16 h = (__bf16)float(v);
-        p_hi[ob + o] = __builtin_bit_cast(unsigned short, h);
-        p_lo[ob + o] = __builtin_bit_cast(unsigned short, (__bf16)(float(v) - (float)h));
-      } else {
-        p[ob + o] = v;
+    if (SPLIT) {
+      // packed row (off-diagonal keys whole, diagonal keys n1 <= n2 with the
+      // n1 < n2 entries times sqrt 2) as the matrix-core operand planes
+      // v = hi + lo (RNE BF16 each); the norm from the full layout
+      const int KD = packed_diag_block(N, L1);
+      for (int kk = 0; kk < kc; ++kk) {
+        const int ab = key_ab[k0 + kk];
+        const int a = ab >> 8, b = ab & 255;
+        const T* sk = stage + kk * KB;
+        const int64_t ob = i * Dp + packed_key_offset(a, b, S, N, L1);
+        const int nk = a < b ? KB : KD;
+        for (int o = threadIdx.x; o < nk; o += kThreads) {
+          T v;
+          if (a < b) {
+            v = sk[o];
+            ss += double(v) * double(v);
+          } else {
+            const int t = o / fL1, l = o - t * L1;
+            int n1, n2;
+            packed_pair(t, N, &n1, &n2);
+            v = sk[(n1 * N + n2) * L1 + l];
+            ss += (n1 < n2 ? 2.0 : 1.0) * double(v) * double(v);
+            if (n1 < n2) v *= T(1.41421356237309504880);
+          }
+          const __bf16 h = (__bf16)float(v);
+          p_hi[ob + o] = __builtin_bit_cast(unsigned short, h);
+          p_lo[ob + o] = __builtin_bit_cast(unsigned short, (__bf16)(float(v) - (float)h));
+        }
       }
-      ss += double(v) * double(v);
+    } else {
+      const int64_t ob = pbase + int64_t(k0) * KB;
+      for (int o = threadIdx.x; o < kc * KB; o += kThreads) {
+        const T v = stage[o];
+        p[ob + o] = v;
+        ss += double(v) * double(v);
+      }
     }
     __syncthreads();
   }
@@ -792,11 +819,10 @@ __device__ __forceinline__ void adjoint_pairs(const RadialDev& rad, int S, int P
 //     reduced per centre (sparse_kernel_predict.hh:354-364).
 constexpr int kAdjPre = 8;  // key-block elements per thread held for the prefetch
 
-template <class T, bool DIRECT, bool PSPLIT>
+template <class T, bool DIRECT>
 __global__ __launch_bounds__(kThreads) void adjoint_kernel(
     RadialDev rad, int S, int PB, int c_in_lds, PairView pv, const T* __restrict__ c,
-    const T* __restrict__ p, const unsigned short* __restrict__ p_hi,
-    const unsigned short* __restrict__ p_lo, const double* __restrict__ norm2, int normalize,
+    const T* __restrict__ p, const double* __restrict__ norm2, int normalize,
     const double* __restrict__ xg, double xg_scale, const T* __restrict__ G,
     double* __restrict__ grad, double* __restrict__ centre_virial) {
   typedef typename Vec4<T>::type V4;
@@ -828,13 +854,7 @@ __global__ __launch_bounds__(kThreads) void adjoint_kernel(
   __shared__ double sh_ab[2];
 
   const T* Gi = G + i * D;
-  // p[o] of this centre (from the BF16 operand planes when split)
-  auto pval = [&](int64_t o) -> T {
-    if (PSPLIT)
-      return T(__builtin_bit_cast(float, unsigned(p_hi[i * D + o]) << 16) +
-               __builtin_bit_cast(float, unsigned(p_lo[i * D + o]) << 16));
-    return p[i * D + o];
-  };
+  auto pval = [&](int64_t o) -> T { return p[i * D + o]; };
   double alpha = 1.0, beta = 0.0;
   if (normalize) {
     const double n2 = norm2[i];
@@ -941,12 +961,15 @@ __global__ __launch_bounds__(kThreads) void adjoint_kernel(
   adjoint_pairs<T, DIRECT>(rad, S, PB, pv, yc, dc, region, i, grad, centre_virial);
 }
 
-// Adjoint with dL/dp of the centre staged in LDS KCH key blocks at a time
-// (all of them when they fit) beside the expansion coefficients.  Same
-// mathematics as adjoint_kernel; the dL/dc contraction keeps its outputs
-// in registers across the key blocks instead of one LDS update per key:
-//  1. g = alpha G - beta p staged for keys [k0, k0 + KCH) in the natural
-//     layout [key][x][y][l]; diagonal keys then replaced by g + g^T;
+// Adjoint of the mixed path: dL/dX (G) and p in the packed layout (p as its
+// BF16 operand planes), the dL/dp blocks staged in LDS in the full layout
+// KCH key blocks at a time beside the expansion coefficients.  Same
+// mathematics as adjoint_kernel; the dL/dc contraction keeps its outputs in
+// registers across the key blocks instead of one LDS update per key:
+//  1. g = alpha G - beta p of keys [k0, k0 + KCH) in the layout
+//     [key][x][y][l]; a diagonal key is unpacked symmetrised, g + g^T
+//     (sqrt 2 (alpha G' - beta p') off the diagonal, 2 (alpha G' - beta p')
+//     on it, from the packed values G', p');
 //  2. thread unit (a, 4 radial channels n, lm):
 //       dL/dc_a[n][lm] += (2l+1)^{-1/2} w_ab sum_y g^(ab)[n][y][l] c_b[y][lm]
 //     over the species b whose key is in the chunk (g^(ab) read transposed
@@ -954,35 +977,36 @@ __global__ __launch_bounds__(kThreads) void adjoint_kernel(
 //     over c;
 //  3. the pair phase, its batches over the g region.
 // UPT = units per thread (ceil(S * ceil(N/4) * LL / 256)).
-template <class T, bool DIRECT, bool PSPLIT, int UPT>
-__global__ __launch_bounds__(kThreads) void adjoint_resident_kernel(
-    RadialDev rad, int S, int PB, int KCH, PairView pv, const T* __restrict__ c, const T* __restrict__ p,
+template <bool DIRECT, int UPT>
+__global__ __launch_bounds__(kThreads) void adjoint_staged_kernel(
+    RadialDev rad, int S, int PB, int KCH, PairView pv, const float* __restrict__ c,
     const unsigned short* __restrict__ p_hi, const unsigned short* __restrict__ p_lo,
     const double* __restrict__ norm2, int normalize, const double* __restrict__ xg, double xg_scale,
-    const T* __restrict__ G, double* __restrict__ grad, double* __restrict__ centre_virial) {
+    const float* __restrict__ G, double* __restrict__ grad, double* __restrict__ centre_virial) {
+  typedef float T;
   typedef typename Vec4<T>::type V4;
   const int64_t i = blockIdx.x;
   const int N = rad.N, L1 = rad.L1, LL = rad.LL, B = N * LL, E = S * B;
   const int K = S * (S + 1) / 2;
-  const int KB = N * N * L1;
-  const int D = K * KB;
+  const int KB = N * N * L1, KD = packed_diag_block(N, L1);
+  const int64_t Dp = packed_feature_size(S, N, L1);
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   extern __shared__ __align__(64) unsigned char smem_raw[];
   T* yc = reinterpret_cast<T*>(smem_raw);
   T* cd = reinterpret_cast<T*>(smem_raw + ylm_table_bytes<T>(L1));  // c, then dL/dc
   unsigned char* region = reinterpret_cast<unsigned char*>(cd) + align_up(sizeof(T) * E, 64);
   T* gl = reinterpret_cast<T*>(region);
-  __shared__ double red[kWaves][9];
+  __shared__ double red[kWaves];
   __shared__ double sh_ab[2];
   load_ylm_table<T>(rad, yc);
   for (int e = tid; e < E; e += kThreads) cd[e] = c[i * int64_t(E) + e];
 
-  const T* Gi = G + i * int64_t(D);
+  const T* Gi = G + i * Dp;
+  const unsigned short* Hi = p_hi + i * Dp;
+  const unsigned short* Lo = p_lo + i * Dp;
   auto pval = [&](int64_t o) -> T {
-    if (PSPLIT)
-      return T(__builtin_bit_cast(float, unsigned(p_hi[i * D + o]) << 16) +
-               __builtin_bit_cast(float, unsigned(p_lo[i * D + o]) << 16));
-    return p[i * D + o];
+    return __builtin_bit_cast(float, unsigned(Hi[o]) << 16) +
+           __builtin_bit_cast(float, unsigned(Lo[o]) << 16);
   };
   double alpha = 1.0, beta = 0.0;
   if (normalize) {
@@ -991,14 +1015,15 @@ __global__ __launch_bounds__(kThreads) void adjoint_resident_kernel(
     if (xg) {
       dotxg = xg_scale * xg[i];
     } else {
+      // X . G of the full rows equals the packed dot product
       double dot = 0;
-      for (int o = tid; o < D; o += kThreads) dot += double(pval(o)) * double(Gi[o]);
+      for (int64_t o = tid; o < Dp; o += kThreads) dot += double(pval(o)) * double(Gi[o]);
       for (int off = 32; off > 0; off >>= 1) dot += __shfl_down(dot, off, 64);
-      if (lane == 0) red[wave][0] = dot;
+      if (lane == 0) red[wave] = dot;
       __syncthreads();
       if (tid == 0) {
         double t = 0;
-        for (int w = 0; w < kWaves; ++w) t += red[w][0];
+        for (int w = 0; w < kWaves; ++w) t += red[w];
         sh_ab[0] = t;
       }
       __syncthreads();
@@ -1010,6 +1035,9 @@ __global__ __launch_bounds__(kThreads) void adjoint_resident_kernel(
     }
   }
   const T ta = T(alpha), tb = T(beta);
+  const FastDiv fL1 = fast_div(L1);
+  // four elements per step where a key block keeps 16-byte alignment
+  const bool vec4 = ((KB | KD | int(Dp & 3)) & 3) == 0;
 
   // dL/dc units: thread unit (a, radial channels n0..n0+3, lm)
   const int NQ = (N + 3) >> 2;
@@ -1021,56 +1049,44 @@ __global__ __launch_bounds__(kThreads) void adjoint_resident_kernel(
 
   for (int k0 = 0; k0 < K; k0 += KCH) {
     const int k1 = min(K, k0 + KCH);
-    const int e0 = k0 * KB, ne = (k1 - k0) * KB;
     if (k0 > 0) __syncthreads();  // every unit has read the previous chunk
-    // 1. stage g of keys [k0, k1) (four elements per thread and step when
-    //    the key blocks keep 16-byte alignment, KB % 4 == 0)
-    if ((KB & 3) == 0) {
-      const V4* G4 = reinterpret_cast<const V4*>(Gi + e0);
-      V4* g4 = reinterpret_cast<V4*>(gl);
-      const unsigned short* ph = PSPLIT ? p_hi + i * int64_t(D) + e0 : nullptr;
-      const unsigned short* plo = PSPLIT ? p_lo + i * int64_t(D) + e0 : nullptr;
-      for (int q = tid; q < (ne >> 2); q += kThreads) {
-        V4 gv = G4[q];
-        V4 pv4;
-        if (PSPLIT) {
-          const uint2 h = reinterpret_cast<const uint2*>(ph)[q];
-          const uint2 lo = reinterpret_cast<const uint2*>(plo)[q];
-          pv4[0] = T(__builtin_bit_cast(float, h.x << 16) + __builtin_bit_cast(float, lo.x << 16));
-          pv4[1] = T(__builtin_bit_cast(float, h.x & 0xffff0000u) +
-                     __builtin_bit_cast(float, lo.x & 0xffff0000u));
-          pv4[2] = T(__builtin_bit_cast(float, h.y << 16) + __builtin_bit_cast(float, lo.y << 16));
-          pv4[3] = T(__builtin_bit_cast(float, h.y & 0xffff0000u) +
-                     __builtin_bit_cast(float, lo.y & 0xffff0000u));
+    // 1. stage g of keys [k0, k1)
+    for (int k = k0; k < k1; ++k) {
+      int a, b;
+      key_pair(k, S, &a, &b);
+      T* gk = gl + (k - k0) * KB;
+      const int64_t src = packed_key_offset(a, b, S, N, L1);
+      if (a < b) {
+        if (vec4) {
+          const V4* G4 = reinterpret_cast<const V4*>(Gi + src);
+          const uint2* H2 = reinterpret_cast<const uint2*>(Hi + src);
+          const uint2* L2 = reinterpret_cast<const uint2*>(Lo + src);
+          V4* g4 = reinterpret_cast<V4*>(gk);
+          for (int q = tid; q < (KB >> 2); q += kThreads) {
+            const V4 gv = G4[q];
+            const uint2 h = H2[q], lo = L2[q];
+            V4 pv4;
+            pv4[0] = __builtin_bit_cast(float, h.x << 16) + __builtin_bit_cast(float, lo.x << 16);
+            pv4[1] = __builtin_bit_cast(float, h.x & 0xffff0000u) +
+                     __builtin_bit_cast(float, lo.x & 0xffff0000u);
+            pv4[2] = __builtin_bit_cast(float, h.y << 16) + __builtin_bit_cast(float, lo.y << 16);
+            pv4[3] = __builtin_bit_cast(float, h.y & 0xffff0000u) +
+                     __builtin_bit_cast(float, lo.y & 0xffff0000u);
+            g4[q] = ta * gv - tb * pv4;
+          }
         } else {
-          pv4 = reinterpret_cast<const V4*>(p + i * int64_t(D) + e0)[q];
+          for (int o = tid; o < KB; o += kThreads) gk[o] = ta * Gi[src + o] - tb * pval(src + o);
         }
-        g4[q] = ta * gv - tb * pv4;
-      }
-    } else {
-      for (int o = tid; o < ne; o += kThreads) gl[o] = ta * Gi[e0 + o] - tb * pval(e0 + o);
-    }
-    __syncthreads();
-    // diagonal keys of the chunk: g[x][y] + g[y][x] on both triangles
-    {
-      const int tri = N * (N + 1) / 2 * L1;
-      for (int it = tid; it < S * tri; it += kThreads) {
-        const int a = it / tri;
-        const int kd = a * S - a * (a - 1) / 2;
-        if (kd < k0 || kd >= k1) continue;
-        int r = it - a * tri;
-        const int l = r % L1;
-        r /= L1;
-        int x = 0;
-        while (r >= N - x) {
-          r -= N - x;
-          ++x;
+      } else {
+        for (int o = tid; o < KD; o += kThreads) {
+          const int t = o / fL1, l = o - t * L1;
+          int n1, n2;
+          packed_pair(t, N, &n1, &n2);
+          const T v = ta * Gi[src + o] - tb * pval(src + o);
+          const T w = n1 < n2 ? T(1.41421356237309504880) * v : T(2) * v;
+          gk[(n1 * N + n2) * L1 + l] = w;
+          gk[(n2 * N + n1) * L1 + l] = w;
         }
-        const int y = x + r;
-        T* gk = gl + (kd - k0) * KB + l;
-        const T v = gk[(x * N + y) * L1] + gk[(y * N + x) * L1];
-        gk[(x * N + y) * L1] = v;
-        gk[(y * N + x) * L1] = v;
       }
     }
     __syncthreads();
@@ -1133,6 +1149,33 @@ __global__ __launch_bounds__(kThreads) void adjoint_resident_kernel(
 
   // 3. pairs; the batches reuse the g region
   adjoint_pairs<T, DIRECT>(rad, S, PB, pv, yc, cd, region, i, grad, centre_virial);
+}
+
+// dL/dX of one centre per workgroup into the packed layout (packing.hh)
+__global__ void pack_gradient_kernel(int S, int N, int L1, const double* __restrict__ G,
+                                     float* __restrict__ Gp) {
+  const int64_t i = blockIdx.x;
+  const int K = S * (S + 1) / 2, KB = N * N * L1, KD = packed_diag_block(N, L1);
+  const double* g = G + i * int64_t(K) * KB;
+  float* gp = Gp + i * packed_feature_size(S, N, L1);
+  for (int k = 0; k < K; ++k) {
+    int a, b;
+    key_pair(k, S, &a, &b);
+    const double* src = g + int64_t(k) * KB;
+    float* dst = gp + packed_key_offset(a, b, S, N, L1);
+    if (a < b) {
+      for (int o = threadIdx.x; o < KB; o += blockDim.x) dst[o] = float(src[o]);
+    } else {
+      for (int o = threadIdx.x; o < KD; o += blockDim.x) {
+        const int t = o / L1, l = o - t * L1;
+        int n1, n2;
+        packed_pair(t, N, &n1, &n2);
+        dst[o] = n1 < n2 ? float((src[(n1 * N + n2) * L1 + l] + src[(n2 * N + n1) * L1 + l]) *
+                                 0.70710678118654752440)
+                         : float(src[(n1 * N + n1) * L1 + l]);
+      }
+    }
+  }
 }
 
 // n_pairs is read on the device (the last entry of the pair offsets), so one
@@ -1309,83 +1352,61 @@ void launch_adjoint_t(const RadialDev& rad, int S, const PairView& pv, const T* 
                       const T* G, double* grad, double* centre_virial, hipStream_t stream,
                       const SoapTiles& tiles) {
   const int N = rad.N, L1 = rad.L1, LL = rad.LL, F = N * L1, E = S * N * LL;
-  const int N4 = (N + 3) & ~3;
   const size_t yt = ylm_table_bytes<T>(L1);
   const size_t blk = align_up(sizeof(T) * size_t(E), 64);
   if constexpr (sizeof(T) == 4) {
-    // dL/dp of a centre staged in LDS, KCH key blocks at a time, beside c
+    // mixed path: packed dL/dX and BF16 planes of p, staged in LDS KCH key
+    // blocks at a time beside c
+    if (!p_hi || !p_lo) throw InvalidArgument("the FP32 adjoint takes the split invariants");
     const int K = S * (S + 1) / 2;
     const size_t kbytes = sizeof(T) * size_t(N) * N * L1;
     const int upt = (S * ((N + 3) / 4) * LL + kThreads - 1) / kThreads;
+    if (upt > 8) throw InvalidArgument("max_radial x (max_angular+1)^2 too large for the FP32 adjoint");
     int KCH = tiles.keys_adj > 0 ? std::min(tiles.keys_adj, K) : K;
     if (tiles.keys_adj <= 0)
       while (KCH > 1 && yt + blk + align_up(KCH * kbytes, 64) > kResidentBudget) --KCH;
     const size_t dbytes = align_up(KCH * kbytes, 64);
-    if (tiles.adj_resident != 0 && upt <= 8 &&
-        (yt + blk + dbytes <= kResidentBudget || tiles.keys_adj > 0)) {
-      int PB = tiles.pb_adj > 0 ? std::min(tiles.pb_adj, 64) : 16;
-      if (tiles.pb_adj <= 0)
-        while (PB > 1 && pair_batch_bytes(PB, LL, F, sizeof(T), DIRECT, true) > dbytes) --PB;
-      const size_t smem = yt + blk + std::max(dbytes, pair_batch_bytes(PB, LL, F, sizeof(T), DIRECT, true));
-      if (smem > device_lds_limit()) throw InvalidArgument("adjoint partition exceeds the LDS");
-      auto go = [&](auto kern) {
-        allow_lds(kern, smem);
-        hipLaunchKernelGGL(kern, dim3(unsigned(pv.n_centres)), dim3(kThreads), smem, stream, rad, S, PB,
-                           KCH, pv, c, p, p_hi, p_lo, norm2, int(normalize), xg, xg_scale, G, grad,
-                           centre_virial);
-      };
-      if (p_hi) {
-        if (upt <= 2)
-          go(adjoint_resident_kernel<T, DIRECT, true, 2>);
-        else if (upt <= 3)
-          go(adjoint_resident_kernel<T, DIRECT, true, 3>);
-        else if (upt <= 4)
-          go(adjoint_resident_kernel<T, DIRECT, true, 4>);
-        else if (upt <= 5)
-          go(adjoint_resident_kernel<T, DIRECT, true, 5>);
-        else
-          go(adjoint_resident_kernel<T, DIRECT, true, 8>);
-      } else {
-        if (upt <= 2)
-          go(adjoint_resident_kernel<T, DIRECT, false, 2>);
-        else if (upt <= 3)
-          go(adjoint_resident_kernel<T, DIRECT, false, 3>);
-        else if (upt <= 4)
-          go(adjoint_resident_kernel<T, DIRECT, false, 4>);
-        else if (upt <= 5)
-          go(adjoint_resident_kernel<T, DIRECT, false, 5>);
-        else
-          go(adjoint_resident_kernel<T, DIRECT, false, 8>);
-      }
-      RGPU_LAUNCH_CHECK();
-      return;
-    }
-  }
-  const size_t gbytes = sizeof(T) * 2 * size_t(L1) * N * N4;
-  const int c_in_lds = yt + blk + blk + gbytes <= kAdjBudget ? 1 : 0;
-  const size_t region = (c_in_lds ? blk : 0) + gbytes;
-  const size_t fixed = yt + blk;
-  const size_t batch_room = std::max(region, kAdjBudget > fixed ? kAdjBudget - fixed : size_t(0));
-  int PB = 8;
-  if (tiles.pb_adj > 0)
-    PB = std::min(tiles.pb_adj, 64);
-  else
-    while (PB > 1 && pair_batch_bytes(PB, LL, F, sizeof(T), DIRECT, true) > batch_room) --PB;
-  const size_t smem = fixed + std::max(region, pair_batch_bytes(PB, LL, F, sizeof(T), DIRECT, true));
-  if (p_hi) {
-    auto kern = adjoint_kernel<T, DIRECT, true>;
-    allow_lds(kern, smem);
-    hipLaunchKernelGGL(kern, dim3(unsigned(pv.n_centres)), dim3(kThreads), smem, stream, rad, S, PB,
-                       c_in_lds, pv, c, p, p_hi, p_lo, norm2, int(normalize), xg, xg_scale, G, grad,
-                       centre_virial);
+    int PB = tiles.pb_adj > 0 ? std::min(tiles.pb_adj, 64) : 16;
+    if (tiles.pb_adj <= 0)
+      while (PB > 1 && pair_batch_bytes(PB, LL, F, sizeof(T), DIRECT, true) > dbytes) --PB;
+    const size_t smem = yt + blk + std::max(dbytes, pair_batch_bytes(PB, LL, F, sizeof(T), DIRECT, true));
+    auto go = [&](auto kern) {
+      allow_lds(kern, smem);
+      hipLaunchKernelGGL(kern, dim3(unsigned(pv.n_centres)), dim3(kThreads), smem, stream, rad, S, PB, KCH,
+                         pv, c, p_hi, p_lo, norm2, int(normalize), xg, xg_scale, G, grad, centre_virial);
+    };
+    if (upt <= 2)
+      go(adjoint_staged_kernel<DIRECT, 2>);
+    else if (upt <= 3)
+      go(adjoint_staged_kernel<DIRECT, 3>);
+    else if (upt <= 4)
+      go(adjoint_staged_kernel<DIRECT, 4>);
+    else if (upt <= 5)
+      go(adjoint_staged_kernel<DIRECT, 5>);
+    else
+      go(adjoint_staged_kernel<DIRECT, 8>);
+    RGPU_LAUNCH_CHECK();
   } else {
-    auto kern = adjoint_kernel<T, DIRECT, false>;
+    (void)p_hi;
+    (void)p_lo;
+    const int N4 = (N + 3) & ~3;
+    const size_t gbytes = sizeof(T) * 2 * size_t(L1) * N * N4;
+    const int c_in_lds = yt + blk + blk + gbytes <= kAdjBudget ? 1 : 0;
+    const size_t region = (c_in_lds ? blk : 0) + gbytes;
+    const size_t fixed = yt + blk;
+    const size_t batch_room = std::max(region, kAdjBudget > fixed ? kAdjBudget - fixed : size_t(0));
+    int PB = 8;
+    if (tiles.pb_adj > 0)
+      PB = std::min(tiles.pb_adj, 64);
+    else
+      while (PB > 1 && pair_batch_bytes(PB, LL, F, sizeof(T), DIRECT, true) > batch_room) --PB;
+    const size_t smem = fixed + std::max(region, pair_batch_bytes(PB, LL, F, sizeof(T), DIRECT, true));
+    auto kern = adjoint_kernel<T, DIRECT>;
     allow_lds(kern, smem);
     hipLaunchKernelGGL(kern, dim3(unsigned(pv.n_centres)), dim3(kThreads), smem, stream, rad, S, PB,
-                       c_in_lds, pv, c, p, p_hi, p_lo, norm2, int(normalize), xg, xg_scale, G, grad,
-                       centre_virial);
+                       c_in_lds, pv, c, p, norm2, int(normalize), xg, xg_scale, G, grad, centre_virial);
+    RGPU_LAUNCH_CHECK();
   }
-  RGPU_LAUNCH_CHECK();
 }
 
 }  // namespace
@@ -1431,13 +1452,13 @@ void launch_power_spectrum_impl(const RadialDev& rad, int S, int64_t n, const T*
   if (p_hi) {
     auto kern = power_spectrum_kernel<T, true>;
     allow_lds(kern, smem);
-    hipLaunchKernelGGL(kern, dim3(unsigned(n)), dim3(kThreads), smem, stream, rad.N, rad.L, S, KC, c,
-                       p, p_hi, p_lo, norm2);
+    hipLaunchKernelGGL(kern, dim3(unsigned(n)), dim3(kThreads), smem, stream, rad.N, rad.L, S, KC,
+                       packed_feature_size(S, rad.N, rad.L1), c, p, p_hi, p_lo, norm2);
   } else {
     auto kern = power_spectrum_kernel<T, false>;
     allow_lds(kern, smem);
-    hipLaunchKernelGGL(kern, dim3(unsigned(n)), dim3(kThreads), smem, stream, rad.N, rad.L, S, KC, c,
-                       p, p_hi, p_lo, norm2);
+    hipLaunchKernelGGL(kern, dim3(unsigned(n)), dim3(kThreads), smem, stream, rad.N, rad.L, S, KC,
+                       packed_feature_size(S, rad.N, rad.L1), c, p, p_hi, p_lo, norm2);
   }
   RGPU_LAUNCH_CHECK();
 }
@@ -1491,6 +1512,13 @@ void launch_backward_split(const RadialDev& rad, int S, const PairView& pv, cons
   else
     launch_adjoint_t<float, true>(rad, S, pv, c, nullptr, p_hi, p_lo, norm2, normalize, xg, xg_scale,
                                   G, grad, centre_virial, stream, tiles);
+}
+
+void launch_pack_gradient(int S, int N, int L1, int64_t n, const double* G, float* Gp,
+                          hipStream_t stream) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(pack_gradient_kernel, dim3(unsigned(n)), dim3(kThreads), 0, stream, S, N, L1, G, Gp);
+  RGPU_LAUNCH_CHECK();
 }
 
 void launch_pair_cutoff(const RadialDev& rad, const int64_t* n_pairs, int64_t cap, const double* dist,
@@ -1572,8 +1600,5 @@ template void launch_normalise<float>(int64_t, int64_t, const float*, const doub
 template void launch_backward<double>(const RadialDev&, int, const PairView&, const double*,
                                       const double*, const double*, bool, const double*, double,
                                       const double*, double*, double*, hipStream_t, const SoapTiles&);
-template void launch_backward<float>(const RadialDev&, int, const PairView&, const float*,
-                                     const float*, const double*, bool, const double*, double,
-                                     const float*, double*, double*, hipStream_t, const SoapTiles&);
 
 }  // namespace rgpu

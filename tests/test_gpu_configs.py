@@ -202,13 +202,14 @@ def test_autotuned_tiles_keep_results():
 
 
 @pytest.mark.parametrize("N,L", [(12, 9), (5, 4), (6, 3)])
-def test_adjoint_variants_agree(N, L):
-    """FP32 adjoint with dL/dp staged in LDS (register accumulation over key
-    blocks, one or several key blocks per pass) vs one LDS update per key
-    block vs the FP64 adjoint (pinned against the oracle elsewhere), on
-    ragged molecules with 4 species.  N = 5, L = 4 gives D = 10 * 125 (not a
-    multiple of 4: element-wise staging) and a partial group of radial
-    channels; N = 6 another partial group."""
+def test_mixed_adjoint_partitions_agree(N, L):
+    """Mixed-precision adjoint (packed dL/dX and invariants, dL/dp staged in
+    LDS one, three or all key blocks per pass, several pair batch sizes) vs
+    the FP64 adjoint (pinned against the oracle elsewhere), on ragged
+    molecules with 4 species and a random dL/dX that is not symmetric in the
+    diagonal-key blocks.  N = 5, L = 4 gives key blocks of 125 and 75
+    elements (element-wise staging) and a partial group of radial channels;
+    N = 6 another partial group."""
     sts = molecules(n_mol=24, seed=5)
     hp = soap_hypers(rc=4.0, sw=0.5, sigma=0.4, N=N, L=L, spline=1e-8)
     ctx, sp = make_ctx(hp, sts)
@@ -218,13 +219,11 @@ def test_adjoint_variants_agree(N, L):
     g64, s64 = ctx.features_backward(G)
     ctx.set_precision(_capi.MIXED)
     out = {}
-    # (variant, key blocks per pass): per-key updates; staged, all keys
-    # (default budget) / one key / three keys per pass
-    for res, kch in ((0, 0), (1, 0), (1, 1), (1, 3)):
-        ctx.set_tiles(adjoint_resident=res, keys_adj=kch)
-        assert ctx.tiles()["adjoint_resident"] == res and ctx.tiles()["keys_adj"] == kch
+    for kch, pb in ((0, 0), (1, 4), (3, 16), (10, 6)):
+        ctx.set_tiles(pairs_adj=pb, keys_adj=kch)
+        assert ctx.tiles()["keys_adj"] == kch and ctx.tiles()["pairs_adj"] == pb
         ctx.features()
-        out[res, kch] = ctx.features_backward(G)
+        out[kch, pb] = ctx.features_backward(G)
     scale = np.abs(g64).max()
     for key, (g, s) in out.items():
         assert np.abs(g - g64).max() < 2e-5 * scale, key
