@@ -52,7 +52,7 @@ struct Problem {
   const int* c_rows;           // output row ids (centre indices)
   int64_t tile0;               // first linear tile of this problem
   int tiles_m, tiles_n;        // tiles along M and N
-  int fast;                    // K % 32 == 0 and every row 16-byte aligned
+  int fast;                    // K % 8 == 0 and every row 16-byte aligned
   // kernel epilogue
   const double* norm2;
   const double* alpha;         // offset to the block's first sparse point
@@ -148,7 +148,8 @@ __device__ __forceinline__ void gemm_tile(const ProblemSet& ps, const Problem& P
   // step kt computes, and stored to LDS at the end of step kt + 1
   uint4 rah[2][2], ral[2][2], rbh[2][2], rbl[2][2];
   auto load8 = [&](const unsigned short* p, int64_t k) -> uint4 {
-    if (FAST) return *reinterpret_cast<const uint4*>(p + k);
+    // FAST: eight K values are all inside or all past the end
+    if (FAST) return k < P.K ? *reinterpret_cast<const uint4*>(p + k) : uint4{0u, 0u, 0u, 0u};
     unsigned short t[8];
 #pragma unroll
     for (int q = 0; q < 8; ++q) t[q] = (k + q < P.K) ? p[k + q] : (unsigned short)0;
@@ -350,7 +351,7 @@ float bf16_to_float_host(unsigned short h) {
 bool aligned16(const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; }
 
 void finish_problem(Problem& P) {
-  P.fast = (P.K % BK == 0) && (P.lda % 8 == 0) && (P.ldb % 8 == 0) && aligned16(P.Ahi) &&
+  P.fast = (P.K % 8 == 0) && (P.lda % 8 == 0) && (P.ldb % 8 == 0) && aligned16(P.Ahi) &&
            aligned16(P.Alo) && aligned16(P.Bhi) && aligned16(P.Blo);
 }
 

@@ -572,6 +572,7 @@ __global__ __launch_bounds__(kThreads) void power_spectrum_kernel(int N, int L, 
   T* cs = reinterpret_cast<T*>(smem_raw);  // [S][N][LS]
   T* stage = cs + S * N * LS;              // [KC][N][N][L1]
   int* key_ab = reinterpret_cast<int*>(stage + KC * KB);  // [K] a << 8 | b
+  int* tri = key_ab + K;  // [N(N+1)/2] n1 << 8 | n2 of the packed diagonal pairs
   __shared__ double red[kWaves];
   const T* ci = c + i * int64_t(S) * B;
   const FastDiv fLL = fast_div(LL);
@@ -584,6 +585,12 @@ __global__ __launch_bounds__(kThreads) void power_spectrum_kernel(int N, int L, 
     key_species(k, S, &a, &b);
     key_ab[k] = (a << 8) | b;
   }
+  if (SPLIT)
+    for (int t = threadIdx.x; t < N * (N + 1) / 2; t += kThreads) {
+      int n1, n2;
+      packed_pair(t, N, &n1, &n2);
+      tri[t] = (n1 << 8) | n2;
+    }
   __syncthreads();
   const int64_t pbase = i * int64_t(K) * KB;
   const FastDiv fNT = fast_div(NT), fL1 = fast_div(L1);
@@ -650,8 +657,7 @@ __global__ __launch_bounds__(kThreads) void power_spectrum_kernel(int N, int L, 
             ss += double(v) * double(v);
           } else {
             const int t = o / fL1, l = o - t * L1;
-            int n1, n2;
-            packed_pair(t, N, &n1, &n2);
+            const int n1 = tri[t] >> 8, n2 = tri[t] & 255;
             v = sk[(n1 * N + n2) * L1 + l];
             ss += (n1 < n2 ? 2.0 : 1.0) * double(v) * double(v);
             if (n1 < n2) v *= T(1.41421356237309504880);
@@ -998,8 +1004,14 @@ __global__ __launch_bounds__(kThreads) void adjoint_staged_kernel(
   T* gl = reinterpret_cast<T*>(region);
   __shared__ double red[kWaves];
   __shared__ double sh_ab[2];
+  __shared__ int tri[kMaxN * (kMaxN + 1) / 2];  // n1 << 8 | n2 of the packed diagonal pairs
   load_ylm_table<T>(rad, yc);
   for (int e = tid; e < E; e += kThreads) cd[e] = c[i * int64_t(E) + e];
+  for (int t = tid; t < N * (N + 1) / 2; t += kThreads) {
+    int n1, n2;
+    packed_pair(t, N, &n1, &n2);
+    tri[t] = (n1 << 8) | n2;
+  }
 
   const T* Gi = G + i * Dp;
   const unsigned short* Hi = p_hi + i * Dp;
@@ -1080,8 +1092,7 @@ __global__ __launch_bounds__(kThreads) void adjoint_staged_kernel(
       } else {
         for (int o = tid; o < KD; o += kThreads) {
           const int t = o / fL1, l = o - t * L1;
-          int n1, n2;
-          packed_pair(t, N, &n1, &n2);
+          const int n1 = tri[t] >> 8, n2 = tri[t] & 255;
           const T v = ta * Gi[src + o] - tb * pval(src + o);
           const T w = n1 < n2 ? T(1.41421356237309504880) * v : T(2) * v;
           gk[(n1 * N + n2) * L1 + l] = w;
@@ -1448,7 +1459,7 @@ void launch_power_spectrum_impl(const RadialDev& rad, int S, int64_t n, const T*
   int KC = cbytes + kb < kPsBudget ? int((kPsBudget - cbytes) / kb) : 1;
   if (tiles.kc > 0) KC = tiles.kc;
   KC = std::max(1, std::min(KC, K));
-  const size_t smem = cbytes + size_t(KC) * kb + sizeof(int) * size_t(K);
+  const size_t smem = cbytes + size_t(KC) * kb + sizeof(int) * size_t(K + rad.N * (rad.N + 1) / 2);
   if (p_hi) {
     auto kern = power_spectrum_kernel<T, true>;
     allow_lds(kern, smem);
