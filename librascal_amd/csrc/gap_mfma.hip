@@ -148,8 +148,6 @@ __device__ __forceinline__ void gemm_tile(const ProblemSet& ps, const Problem& P
   // step kt computes, and stored to LDS at the end of step kt + 1
   uint4 rah[2][2], ral[2][2], rbh[2][2], rbl[2][2];
   auto load8 = [&](const unsigned short* p, int64_t k) -> uint4 {
-    // FAST: eight K values are all inside or all past the end
-    if (FAST) return k < P.K ? *reinterpret_cast<const uint4*>(p + k) : uint4{0u, 0u, 0u, 0u};
     unsigned short t[8];
 #pragma unroll
     for (int q = 0; q < 8; ++q) t[q] = (k + q < P.K) ? p[k + q] : (unsigned short)0;
@@ -159,12 +157,33 @@ __device__ __forceinline__ void gemm_tile(const ProblemSet& ps, const Problem& P
   };
   auto gload = [&](int rs_, int64_t k0) {
     const int64_t k = k0 + 8 * kq;
+    if (FAST && k0 + BK <= P.K) {
+      // whole K step inside the rows (a wave-uniform branch)
 #pragma unroll
-    for (int j = 0; j < 2; ++j) {
-      rah[rs_][j] = load8(Ah[j], k);
-      ral[rs_][j] = load8(Al[j], k);
-      rbh[rs_][j] = load8(Bh[j], k);
-      rbl[rs_][j] = load8(Bl[j], k);
+      for (int j = 0; j < 2; ++j) {
+        rah[rs_][j] = *reinterpret_cast<const uint4*>(Ah[j] + k);
+        ral[rs_][j] = *reinterpret_cast<const uint4*>(Al[j] + k);
+        rbh[rs_][j] = *reinterpret_cast<const uint4*>(Bh[j] + k);
+        rbl[rs_][j] = *reinterpret_cast<const uint4*>(Bl[j] + k);
+      }
+    } else if (FAST) {
+      // last K step: eight K values are all inside or all past the end
+      const uint4 z = {0u, 0u, 0u, 0u};
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        rah[rs_][j] = k < P.K ? *reinterpret_cast<const uint4*>(Ah[j] + k) : z;
+        ral[rs_][j] = k < P.K ? *reinterpret_cast<const uint4*>(Al[j] + k) : z;
+        rbh[rs_][j] = k < P.K ? *reinterpret_cast<const uint4*>(Bh[j] + k) : z;
+        rbl[rs_][j] = k < P.K ? *reinterpret_cast<const uint4*>(Bl[j] + k) : z;
+      }
+    } else {
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        rah[rs_][j] = load8(Ah[j], k);
+        ral[rs_][j] = load8(Al[j], k);
+        rbh[rs_][j] = load8(Bh[j], k);
+        rbl[rs_][j] = load8(Bl[j], k);
+      }
     }
   };
   auto lstore = [&](int rs_, int st) {
