@@ -158,9 +158,10 @@ struct Context {
     if (hp.kind == SoapHypers::kPowerSpectrum) return int64_t(S) * (S + 1) / 2 * N * N * L1;
     return int64_t(S) * N * L1 * L1;
   }
-  // columns of the packed invariants of the mixed path (PowerSpectrum)
+  // row stride of the packed invariants of the mixed path (PowerSpectrum):
+  // the packed columns rounded up to 32, pad columns zero
   int64_t Dp() const {
-    return packed_feature_size(int(species.size()), hp.max_radial, hp.max_angular + 1);
+    return packed_row_stride(int(species.size()), hp.max_radial, hp.max_angular + 1);
   }
 };
 
@@ -777,7 +778,7 @@ void autotune_t(Context& c) {
     }
     t.kc = pick;
     // adjoint on a zero dL/dX (its time does not depend on the values)
-    const int64_t D = c.D();
+    const int64_t D = std::max(c.D(), c.Dp());
     DeviceBuffer<T> G;
     DeviceBuffer<double> gr, cv;
     T* g = G.reserve(size_t(n * D));

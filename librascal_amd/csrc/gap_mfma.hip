@@ -52,7 +52,7 @@ struct Problem {
   const int* c_rows;           // output row ids (centre indices)
   int64_t tile0;               // first linear tile of this problem
   int tiles_m, tiles_n;        // tiles along M and N
-  int fast;                    // K % 8 == 0 and every row 16-byte aligned
+  int fast;                    // K % 32 == 0 and every row 16-byte aligned
   // kernel epilogue
   const double* norm2;
   const double* alpha;         // offset to the block's first sparse point
@@ -148,6 +148,7 @@ __device__ __forceinline__ void gemm_tile(const ProblemSet& ps, const Problem& P
   // step kt computes, and stored to LDS at the end of step kt + 1
   uint4 rah[2][2], ral[2][2], rbh[2][2], rbl[2][2];
   auto load8 = [&](const unsigned short* p, int64_t k) -> uint4 {
+    if (FAST) return *reinterpret_cast<const uint4*>(p + k);
     unsigned short t[8];
 #pragma unroll
     for (int q = 0; q < 8; ++q) t[q] = (k + q < P.K) ? p[k + q] : (unsigned short)0;
@@ -157,33 +158,12 @@ __device__ __forceinline__ void gemm_tile(const ProblemSet& ps, const Problem& P
   };
   auto gload = [&](int rs_, int64_t k0) {
     const int64_t k = k0 + 8 * kq;
-    if (FAST && k0 + BK <= P.K) {
-      // whole K step inside the rows (a wave-uniform branch)
 #pragma unroll
-      for (int j = 0; j < 2; ++j) {
-        rah[rs_][j] = *reinterpret_cast<const uint4*>(Ah[j] + k);
-        ral[rs_][j] = *reinterpret_cast<const uint4*>(Al[j] + k);
-        rbh[rs_][j] = *reinterpret_cast<const uint4*>(Bh[j] + k);
-        rbl[rs_][j] = *reinterpret_cast<const uint4*>(Bl[j] + k);
-      }
-    } else if (FAST) {
-      // last K step: eight K values are all inside or all past the end
-      const uint4 z = {0u, 0u, 0u, 0u};
-#pragma unroll
-      for (int j = 0; j < 2; ++j) {
-        rah[rs_][j] = k < P.K ? *reinterpret_cast<const uint4*>(Ah[j] + k) : z;
-        ral[rs_][j] = k < P.K ? *reinterpret_cast<const uint4*>(Al[j] + k) : z;
-        rbh[rs_][j] = k < P.K ? *reinterpret_cast<const uint4*>(Bh[j] + k) : z;
-        rbl[rs_][j] = k < P.K ? *reinterpret_cast<const uint4*>(Bl[j] + k) : z;
-      }
-    } else {
-#pragma unroll
-      for (int j = 0; j < 2; ++j) {
-        rah[rs_][j] = load8(Ah[j], k);
-        ral[rs_][j] = load8(Al[j], k);
-        rbh[rs_][j] = load8(Bh[j], k);
-        rbl[rs_][j] = load8(Bl[j], k);
-      }
+    for (int j = 0; j < 2; ++j) {
+      rah[rs_][j] = load8(Ah[j], k);
+      ral[rs_][j] = load8(Al[j], k);
+      rbh[rs_][j] = load8(Bh[j], k);
+      rbl[rs_][j] = load8(Bl[j], k);
     }
   };
   auto lstore = [&](int rs_, int st) {
@@ -370,7 +350,7 @@ float bf16_to_float_host(unsigned short h) {
 bool aligned16(const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; }
 
 void finish_problem(Problem& P) {
-  P.fast = (P.K % 8 == 0) && (P.lda % 8 == 0) && (P.ldb % 8 == 0) && aligned16(P.Ahi) &&
+  P.fast = (P.K % BK == 0) && (P.lda % 8 == 0) && (P.ldb % 8 == 0) && aligned16(P.Ahi) &&
            aligned16(P.Alo) && aligned16(P.Bhi) && aligned16(P.Blo);
 }
 
@@ -395,13 +375,14 @@ void prepare_mixed_sparse(MixedSparse& ms, const std::vector<double>& T, int64_t
   // blocks need not be symmetric: (t[n1][n2] + t[n2][n1]) / sqrt 2 keeps
   // every product with a symmetric feature row exact
   const int K = S * (S + 1) / 2, KB = N * N * L1, KD = packed_diag_block(N, L1);
-  const int64_t D = int64_t(K) * KB, Dp = packed_feature_size(S, N, L1);
+  const int64_t D = int64_t(K) * KB;
+  const int64_t Dq = packed_row_stride(S, N, L1);  // zero pad columns
   ms.M = M;
-  ms.D = Dp;
-  const size_t n = size_t(M * Dp);
+  ms.D = Dq;
+  const size_t n = size_t(M * Dq);
   std::vector<unsigned short> hi(std::max<size_t>(n, 1)), lo(std::max<size_t>(n, 1)),
       thi(std::max<size_t>(n, 1)), tlo(std::max<size_t>(n, 1));
-  std::vector<double> row(size_t(std::max<int64_t>(Dp, 1)));
+  std::vector<double> row(size_t(std::max<int64_t>(Dq, 1)), 0.0);
   for (int64_t m = 0; m < M; ++m) {
     const double* src = T.data() + m * D;
     for (int k = 0; k < K; ++k) {
@@ -421,12 +402,12 @@ void prepare_mixed_sparse(MixedSparse& ms, const std::vector<double>& T, int64_t
         }
       }
     }
-    for (int64_t d = 0; d < Dp; ++d) {
+    for (int64_t d = 0; d < Dq; ++d) {
       const float x = float(row[size_t(d)]);
       const unsigned short h = bf16_rne_host(x);
       const unsigned short l = bf16_rne_host(x - bf16_to_float_host(h));
-      hi[size_t(m * Dp + d)] = h;
-      lo[size_t(m * Dp + d)] = l;
+      hi[size_t(m * Dq + d)] = h;
+      lo[size_t(m * Dq + d)] = l;
       thi[size_t(d * M + m)] = h;
       tlo[size_t(d * M + m)] = l;
     }

@@ -43,6 +43,12 @@ __host__ __device__ inline int64_t packed_feature_size(int S, int N, int L1) {
   return int64_t(S) * packed_diag_block(N, L1) + int64_t(K - S) * N * N * L1;
 }
 
+// Row stride of packed rows in memory: the packed width rounded up to a
+// multiple of 32 (the K step of the matrix-core GEMMs), pad columns zero.
+__host__ __device__ inline int64_t packed_row_stride(int S, int N, int L1) {
+  return (packed_feature_size(S, N, L1) + 31) / 32 * 32;
+}
+
 // (n1, n2), n1 <= n2, of packed pair index t of a diagonal key (row x
 // starts at x N - x (x - 1) / 2; the smaller root of that quadratic, then
 // exact integer corrections)

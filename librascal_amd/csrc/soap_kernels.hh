@@ -49,7 +49,8 @@ void launch_power_spectrum(const RadialDev& rad, int S, int64_t n_centres, const
 
 // The same power spectrum in the packed layout (packing.hh) written as the
 // matrix-core operand planes p = p_hi + p_lo (BF16 bit patterns, each
-// rounded to nearest even), [n][Dp]; norm2 from the FP32 values.
+// rounded to nearest even), rows of packed_row_stride() columns (zero pad);
+// norm2 from the FP32 values.
 void launch_power_spectrum_split(const RadialDev& rad, int S, int64_t n_centres, const float* c,
                                  unsigned short* p_hi, unsigned short* p_lo, double* norm2,
                                  hipStream_t stream, const SoapTiles& tiles = SoapTiles());
@@ -72,7 +73,8 @@ void launch_backward(const RadialDev& rad, int S, const PairView& pv, const T* c
                      const SoapTiles& tiles = SoapTiles());
 
 // Pack dL/dX [n][D] (full layout, FP64) into the packed layout of the mixed
-// path [n][Dp] (FP32): diagonal-key entries (G[n1][n2] + G[n2][n1]) / sqrt 2
+// path, rows of packed_row_stride() columns (FP32): diagonal-key entries
+// (G[n1][n2] + G[n2][n1]) / sqrt 2
 // for n1 < n2, G[n][n] on the diagonal.
 void launch_pack_gradient(int S, int N, int L1, int64_t n, const double* G, float* Gp,
                           hipStream_t stream);

@@ -557,7 +557,7 @@ __global__ __launch_bounds__(kThreads, (expansion_min_waves<T, KPT, DIRECT>())) 
 // LDS with an odd stride (no bank conflicts between the tile rows).
 // Un-normalised; |p|^2 is returned per centre.
 template <class T, bool SPLIT>
-__global__ __launch_bounds__(kThreads) void power_spectrum_kernel(int N, int L, int S, int KC, int64_t Dp,
+__global__ __launch_bounds__(kThreads) void power_spectrum_kernel(int N, int L, int S, int KC, int64_t Dq,
                                                                   const T* __restrict__ c,
                                                                   T* __restrict__ p,
                                                                   unsigned short* __restrict__ p_hi,
@@ -648,7 +648,7 @@ __global__ __launch_bounds__(kThreads) void power_spectrum_kernel(int N, int L, 
         const int ab = key_ab[k0 + kk];
         const int a = ab >> 8, b = ab & 255;
         const T* sk = stage + kk * KB;
-        const int64_t ob = i * Dp + packed_key_offset(a, b, S, N, L1);
+        const int64_t ob = i * Dq + packed_key_offset(a, b, S, N, L1);
         const int nk = a < b ? KB : KD;
         for (int o = threadIdx.x; o < nk; o += kThreads) {
           T v;
@@ -676,6 +676,14 @@ __global__ __launch_bounds__(kThreads) void power_spectrum_kernel(int N, int L, 
       }
     }
     __syncthreads();
+  }
+  if (SPLIT) {
+    // zero pad columns of the packed row
+    const int64_t Dp = packed_feature_size(S, N, L1);
+    for (int64_t o = Dp + threadIdx.x; o < Dq; o += kThreads) {
+      p_hi[i * Dq + o] = 0;
+      p_lo[i * Dq + o] = 0;
+    }
   }
   for (int off = 32; off > 0; off >>= 1) ss += __shfl_down(ss, off, 64);
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = ss;
@@ -995,7 +1003,7 @@ __global__ __launch_bounds__(kThreads) void adjoint_staged_kernel(
   const int N = rad.N, L1 = rad.L1, LL = rad.LL, B = N * LL, E = S * B;
   const int K = S * (S + 1) / 2;
   const int KB = N * N * L1, KD = packed_diag_block(N, L1);
-  const int64_t Dp = packed_feature_size(S, N, L1);
+  const int64_t Dp = packed_feature_size(S, N, L1), Dq = packed_row_stride(S, N, L1);
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   extern __shared__ __align__(64) unsigned char smem_raw[];
   T* yc = reinterpret_cast<T*>(smem_raw);
@@ -1013,9 +1021,9 @@ __global__ __launch_bounds__(kThreads) void adjoint_staged_kernel(
     tri[t] = (n1 << 8) | n2;
   }
 
-  const T* Gi = G + i * Dp;
-  const unsigned short* Hi = p_hi + i * Dp;
-  const unsigned short* Lo = p_lo + i * Dp;
+  const T* Gi = G + i * Dq;
+  const unsigned short* Hi = p_hi + i * Dq;
+  const unsigned short* Lo = p_lo + i * Dq;
   auto pval = [&](int64_t o) -> T {
     return __builtin_bit_cast(float, unsigned(Hi[o]) << 16) +
            __builtin_bit_cast(float, unsigned(Lo[o]) << 16);
@@ -1049,7 +1057,7 @@ __global__ __launch_bounds__(kThreads) void adjoint_staged_kernel(
   const T ta = T(alpha), tb = T(beta);
   const FastDiv fL1 = fast_div(L1);
   // four elements per step where a key block keeps 16-byte alignment
-  const bool vec4 = ((KB | KD | int(Dp & 3)) & 3) == 0;
+  const bool vec4 = ((KB | KD) & 3) == 0;
 
   // dL/dc units: thread unit (a, radial channels n0..n0+3, lm)
   const int NQ = (N + 3) >> 2;
@@ -1168,7 +1176,7 @@ __global__ void pack_gradient_kernel(int S, int N, int L1, const double* __restr
   const int64_t i = blockIdx.x;
   const int K = S * (S + 1) / 2, KB = N * N * L1, KD = packed_diag_block(N, L1);
   const double* g = G + i * int64_t(K) * KB;
-  float* gp = Gp + i * packed_feature_size(S, N, L1);
+  float* gp = Gp + i * packed_row_stride(S, N, L1);
   for (int k = 0; k < K; ++k) {
     int a, b;
     key_pair(k, S, &a, &b);
@@ -1464,12 +1472,12 @@ void launch_power_spectrum_impl(const RadialDev& rad, int S, int64_t n, const T*
     auto kern = power_spectrum_kernel<T, true>;
     allow_lds(kern, smem);
     hipLaunchKernelGGL(kern, dim3(unsigned(n)), dim3(kThreads), smem, stream, rad.N, rad.L, S, KC,
-                       packed_feature_size(S, rad.N, rad.L1), c, p, p_hi, p_lo, norm2);
+                       packed_row_stride(S, rad.N, rad.L1), c, p, p_hi, p_lo, norm2);
   } else {
     auto kern = power_spectrum_kernel<T, false>;
     allow_lds(kern, smem);
     hipLaunchKernelGGL(kern, dim3(unsigned(n)), dim3(kThreads), smem, stream, rad.N, rad.L, S, KC,
-                       packed_feature_size(S, rad.N, rad.L1), c, p, p_hi, p_lo, norm2);
+                       packed_row_stride(S, rad.N, rad.L1), c, p, p_hi, p_lo, norm2);
   }
   RGPU_LAUNCH_CHECK();
 }
