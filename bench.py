@@ -67,7 +67,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
-    ap.add_argument("--cpu-centres", type=int, default=48)
+    ap.add_argument("--cpu-centres", type=int, default=720)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first run a plain 256 MiB -> 256 MiB elementwise copy, the reference "
                          "kernel for scaling FETCH_SIZE / WRITE_SIZE counters to bytes")
