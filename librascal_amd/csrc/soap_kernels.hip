@@ -715,9 +715,9 @@ __global__ void normalise_kernel(int64_t n, int64_t D, const T* __restrict__ p,
 // Step 3 of the adjoint for centre i: per-pair contraction of dL/dc (in
 // LDS) with the pair gradients, force atomics and the centre virial.  The
 // pair batches are staged in `region`.  Each pair is contracted by a group
-// of 16 lanes (four pairs per wavefront at a time, reduced within the
-// group); the per-pair FP64 tail (force, atomics, virial) then runs one
-// lane per pair on wavefront 0 (PB <= 64).
+// of 16, 32 or 64 lanes (reduced within the group), sized so that a batch
+// spreads over the workgroup; the per-pair FP64 tail (force, atomics,
+// virial) then runs one lane per pair on wavefront 0 (PB <= 64).
 template <class T, bool DIRECT>
 __device__ __forceinline__ void adjoint_pairs(const RadialDev& rad, int S, int PB, const PairView& pv,
                                               const T* yc, const T* dc, unsigned char* region,
@@ -725,7 +725,11 @@ __device__ __forceinline__ void adjoint_pairs(const RadialDev& rad, int S, int P
                                               double* __restrict__ centre_virial) {
   const int N = rad.N, L1 = rad.L1, LL = rad.LL, F = N * L1, B = N * LL;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int grp = lane >> 4, gl = lane & 15;
+  // lanes per pair: the largest power of two in [16, 64] that still gives
+  // every pair of a full batch its own group
+  const int GS = PB <= 4 ? 64 : (PB <= 8 ? 32 : 16);
+  const int gpw = 64 / GS;  // groups per wavefront
+  const int grp = lane / GS, gl = lane & (GS - 1);
   PairBatch<T> pb = carve_batch<T, DIRECT, true>(region, PB, LL, F);
   const int64_t p0 = pv.centre_off[i] + 1;
   const int np = int(pv.centre_off[i + 1] - p0);
@@ -735,7 +739,7 @@ __device__ __forceinline__ void adjoint_pairs(const RadialDev& rad, int S, int P
     PairPre pre;
     pair_fetch<DIRECT, true>(rad, pv, p0 + q0, nb, pre);
     stage_pairs<T, DIRECT, true>(rad, yc, pre, nb, pb, nullptr, nullptr, nullptr);
-    for (int qb = 4 * wave; qb < nb; qb += 4 * kWaves) {
+    for (int qb = gpw * wave; qb < nb; qb += gpw * kWaves) {
       const int q = qb + grp;
       const bool act = q < nb;
       T A = T(0), Bx = T(0), By = T(0), Bz = T(0);
@@ -746,7 +750,7 @@ __device__ __forceinline__ void adjoint_pairs(const RadialDev& rad, int S, int P
         const T* Rq = pb.R + q * F;
         const T* dRq = pb.dR + q * F;
         const T fc = T(pb.fc[q]), dfc = T(pb.dfc[q]);
-        for (int lm = gl; lm < LL; lm += 16) {
+        for (int lm = gl; lm < LL; lm += GS) {
           const int l = l_of_lm(lm);
           T Dn = T(0), Ddn = T(0);
           for (int n = 0; n < N; ++n) {
@@ -760,11 +764,11 @@ __device__ __forceinline__ void adjoint_pairs(const RadialDev& rad, int S, int P
           Bz += Dn * dYq[2 * LL + lm];
         }
       }
-      for (int off = 8; off > 0; off >>= 1) {
-        A += __shfl_xor(A, off, 16);
-        Bx += __shfl_xor(Bx, off, 16);
-        By += __shfl_xor(By, off, 16);
-        Bz += __shfl_xor(Bz, off, 16);
+      for (int off = GS >> 1; off > 0; off >>= 1) {
+        A += __shfl_xor(A, off, GS);
+        Bx += __shfl_xor(Bx, off, GS);
+        By += __shfl_xor(By, off, GS);
+        Bz += __shfl_xor(Bz, off, GS);
       }
       if (act && gl == 0) {
         pb.res[4 * q] = A;
