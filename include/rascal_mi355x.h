@@ -154,6 +154,14 @@ int rgpu_set_sparse_points(rgpu_context* ctx, int zeta, int64_t n_sparse, const 
  * 1 = Atom [n_centres][n_sparse]; columns in the order the sparse points were
  * given (SparseKernelImpl<GAP>::compute, sparse_kernels.hh:104-190). */
 int rgpu_gap_knm(rgpu_context* ctx, int target, double* knm);
+/* Gradient rows of K_NM (training with forces and stress): out
+ * [3 n_atoms (+ 6 n_struct)][n_sparse], row 3 j + x = d/dr_j,x of
+ * sum_i k(X_i, T_m) over the centres i of m's species; with neg_stress the
+ * per-structure Voigt rows (xx, yy, zz, yz, xz, xy) of r_ji (x) dk/dr_j
+ * divided by the cell volume follow.  FP64 precision only.  Replaces
+ * SparseKernelImpl<GAP>::compute_derivative (models/sparse_kernels.hh
+ * :258-499). */
+int rgpu_gap_knm_gradient(rgpu_context* ctx, int neg_stress, double* out);
 /* K_MM [n_sparse][n_sparse] in FP64: (T_a T_a^T)^zeta between sparse points
  * of the same species a, 0 across species; rows and columns in the order the
  * sparse points were given (SparseKernelImpl<GAP>::compute(sparse_points),

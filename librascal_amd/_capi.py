@@ -74,6 +74,7 @@ def lib():
     L.rgpu_set_sparse_points.argtypes = [p, i, i64, p, p, p]
     L.rgpu_gap_knm.argtypes = [p, i, p]
     L.rgpu_gap_kmm.argtypes = [p, p]
+    L.rgpu_gap_knm_gradient.argtypes = [p, i, p]
     L.rgpu_gap_predict.argtypes = [p, p, p, p]
     L.rgpu_eval_radial.argtypes = [p, i64, p, p, p]
     L.rgpu_eval_ylm.argtypes = [p, i64, p, p, p]
@@ -84,7 +85,7 @@ def lib():
                  "rgpu_enable_graphs", "rgpu_autotune", "rgpu_set_tiles",
                  "rgpu_set_structures", "rgpu_update_positions", "rgpu_get_neighbours",
                  "rgpu_compute", "rgpu_get_expansion", "rgpu_get_features",
-                 "rgpu_features_backward", "rgpu_set_sparse_points", "rgpu_gap_knm", "rgpu_gap_kmm",
+                 "rgpu_features_backward", "rgpu_set_sparse_points", "rgpu_gap_knm", "rgpu_gap_kmm", "rgpu_gap_knm_gradient",
                  "rgpu_gap_predict", "rgpu_eval_radial", "rgpu_eval_ylm", "rgpu_enable_timing",
                  "rgpu_get_timing", "rgpu_reset_timing"):
         getattr(L, name).restype = C.c_int
@@ -232,6 +233,13 @@ class Context:
         k = np.zeros((rows, self.info("n_sparse")))
         check(lib().rgpu_gap_knm(self._p, t, ptr(k)))
         return k
+
+    def gap_knm_gradient(self, neg_stress=False):
+        """K_NM gradient rows [3 n_atoms (+ 6 n_struct)][n_sparse] (FP64)."""
+        n, ns, M = self.info("n_centres"), self.info("n_struct"), self.info("n_sparse")
+        out = np.zeros((3 * n + (6 * ns if neg_stress else 0), M))
+        check(lib().rgpu_gap_knm_gradient(self._p, int(bool(neg_stress)), ptr(out)))
+        return out
 
     def gap_kmm(self):
         M = self.info("n_sparse")

@@ -78,7 +78,7 @@ struct Context {
   DeviceBuffer<int> d_rows;
 
   // scratch
-  DeviceBuffer<double> e_centre, grad, cvir, tmp64, out64, out_e;
+  DeviceBuffer<double> e_centre, grad, cvir, tmp64, out64, out_e, knm_grad, knm_scale;
   DeviceBuffer<double> d_inv_vol;  // [n_struct] 1 / cell volume
   std::vector<double> inv_vol;     // host copy of the uploaded values
   DeviceBuffer<double> pair_fc, pair_dfc;  // cutoff function per pair
@@ -1209,6 +1209,57 @@ int rgpu_gap_kmm(rgpu_context* ctx, double* kmm) {
       for (int64_t m = 0; m < M; ++m)
         o[size_t(c.sp_order[r] * M + c.sp_order[m])] = h[size_t(r * M + m)];
     RGPU_HIP(hipMemcpy(kmm, o.data(), sizeof(double) * o.size(), hipMemcpyDefault));
+  });
+}
+
+int rgpu_gap_knm_gradient(rgpu_context* ctx, int neg_stress, double* out) {
+  return guarded([&] {
+    if (!ctx || !out) throw InvalidArgument("null argument");
+    Context& c = ctx->c;
+    require_batch(c);
+    require_invariants(c);
+    if (!c.have_sparse) throw InvalidArgument("no sparse points set");
+    if (c.precision != RGPU_FP64)
+      throw InvalidArgument("the K_NM gradient rows are computed in FP64 (set RGPU_FP64)");
+    ensure_rep(c);
+    const int S = int(c.species.size());
+    const int64_t n = c.n_atoms, M = c.M, D = c.D(), ns = c.n_struct;
+    const int64_t rows = 3 * n + (neg_stress ? 6 * ns : 0);
+    double* o = c.knm_grad.reserve(size_t(std::max<int64_t>(rows * M, 1)));
+    if (rows * M == 0) return;
+    // d[i][m] = X_i . T_m for the centre's species (the K_NM of zeta 1)
+    double* d = c.tmp64.reserve(size_t(std::max<int64_t>(n * M, 1)));
+    RGPU_HIP(hipMemsetAsync(d, 0, sizeof(double) * n * M, c.stream));
+    for (int s = 0; s < S; ++s) {
+      GapBlock<double> blk = gap_block<double>(c, s, c.p64.get(), c.T64.get());
+      blk.alpha = nullptr;
+      blk.zeta = 1;
+      if (blk.n_rows == 0 || blk.n_cols == 0) continue;
+      launch_gap_kernel<double>(blk, M, nullptr, nullptr, d, nullptr, c.d_centre_struct.get(), c.stream);
+    }
+    double* gs = c.knm_scale.reserve(size_t(std::max<int64_t>(n, 1)));
+    double* gd = c.grad.reserve(size_t(std::max<int64_t>(3 * n, 1)));
+    double* cv = c.cvir.reserve(size_t(std::max<int64_t>(6 * n, 1)));
+    double* st = c.out64.reserve(size_t(6 * std::max<int64_t>(ns, 1)));
+    // column m (species-sorted): dL/dX_i = zeta d_im^(zeta-1) T_m for the
+    // centres of m's species, through the adjoint; rows written in the
+    // caller's sparse-point order
+    for (int s = 0; s < S; ++s)
+      for (int64_t m = c.sp_off[s]; m < c.sp_off[s + 1]; ++m) {
+        launch_knm_gradient_scale(n, d + m, M, c.d_species_of.get(), s, c.zeta, gs, c.stream);
+        RGPU_HIP(hipMemsetAsync(gd, 0, sizeof(double) * 3 * n, c.stream));
+        launch_backward_rank1(c.rd, S, pair_view(c), c.c64.get(), c.p64.get(), c.norm2.get(),
+                              c.hp.normalize, c.T64.get() + m * D, gs, gd, cv, c.stream, c.tile());
+        const int64_t col = c.sp_order[size_t(m)];
+        launch_strided_copy(gd, 3 * n, o + col, M, c.stream);
+        if (neg_stress) {
+          RGPU_HIP(hipMemsetAsync(st, 0, sizeof(double) * 6 * ns, c.stream));
+          launch_segment_sum(cv, 6, c.d_atom_off.get(), int(ns), st, c.stream, c.d_inv_vol.get());
+          launch_strided_copy(st, 6 * ns, o + 3 * n * M + col, M, c.stream);
+        }
+      }
+    RGPU_HIP(hipMemcpyAsync(out, o, sizeof(double) * rows * M, hipMemcpyDefault, c.stream));
+    RGPU_HIP(hipStreamSynchronize(c.stream));
   });
 }
 

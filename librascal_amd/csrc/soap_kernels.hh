@@ -72,6 +72,14 @@ void launch_backward(const RadialDev& rad, int S, const PairView& pv, const T* c
                      const T* G, double* grad, double* centre_virial, hipStream_t stream,
                      const SoapTiles& tiles = SoapTiles());
 
+// launch_backward<double> for a rank-one dL/dX: G_i = Gscale[i] * Grow
+// (Grow [D], one sparse point's features; centres with a zero scale are
+// skipped and get a zero virial).  Used for the K_NM gradient rows.
+void launch_backward_rank1(const RadialDev& rad, int S, const PairView& pv, const double* c,
+                           const double* p, const double* norm2, bool normalize, const double* Grow,
+                           const double* Gscale, double* grad, double* centre_virial,
+                           hipStream_t stream, const SoapTiles& tiles = SoapTiles());
+
 // Pack dL/dX [n][D] (full layout, FP64) into the packed layout of the mixed
 // path, rows of packed_row_stride() columns (FP32): diagonal-key entries
 // (G[n1][n2] + G[n2][n1]) / sqrt 2
@@ -86,6 +94,12 @@ void launch_backward_split(const RadialDev& rad, int S, const PairView& pv, cons
                            const double* norm2, bool normalize, const double* xg, double xg_scale,
                            const float* G, double* grad, double* centre_virial, hipStream_t stream,
                            const SoapTiles& tiles = SoapTiles());
+
+// scale[i] = zeta d[i*ld]^(zeta-1) for the centres of species s, else 0
+void launch_knm_gradient_scale(int64_t n, const double* d, int64_t ld, const int* species_of, int s,
+                               int zeta, double* scale, hipStream_t stream);
+// dst[r * ld] = src[r], r < n
+void launch_strided_copy(const double* src, int64_t n, double* dst, int64_t ld, hipStream_t stream);
 
 // out[s][:w] += (scale ? scale[s] : 1) * sum of in[a][:w] over a in
 // [seg_off[s], seg_off[s+1])

@@ -842,9 +842,16 @@ __global__ __launch_bounds__(kThreads) void adjoint_kernel(
     RadialDev rad, int S, int PB, int c_in_lds, PairView pv, const T* __restrict__ c,
     const T* __restrict__ p, const double* __restrict__ norm2, int normalize,
     const double* __restrict__ xg, double xg_scale, const T* __restrict__ G,
+    const T* __restrict__ Grow, const double* __restrict__ Gscale,
     double* __restrict__ grad, double* __restrict__ centre_virial) {
   typedef typename Vec4<T>::type V4;
   const int64_t i = blockIdx.x;
+  // rank-one dL/dX (Grow != null): G_i = Gscale[i] Grow; a centre with a
+  // zero scale contributes nothing
+  if (Grow && Gscale[i] == 0.0) {
+    if (threadIdx.x < 6) centre_virial[6 * i + threadIdx.x] = 0.0;
+    return;
+  }
   const int N = rad.N, L1 = rad.L1, LL = rad.LL, B = N * LL, E = S * B;
   const int N4 = (N + 3) & ~3;
   const int K = S * (S + 1) / 2;
@@ -871,7 +878,9 @@ __global__ __launch_bounds__(kThreads) void adjoint_kernel(
   __shared__ double red[kWaves][9];
   __shared__ double sh_ab[2];
 
-  const T* Gi = G + i * D;
+  const T* Gi = Grow ? nullptr : G + i * D;
+  const T gs = Grow ? T(Gscale[i]) : T(1);
+  auto gval = [&](int64_t o) -> T { return Grow ? gs * Grow[o] : Gi[o]; };
   auto pval = [&](int64_t o) -> T { return p[i * D + o]; };
   double alpha = 1.0, beta = 0.0;
   if (normalize) {
@@ -881,7 +890,7 @@ __global__ __launch_bounds__(kThreads) void adjoint_kernel(
       dotxg = xg_scale * xg[i];
     } else {
       double dot = 0;
-      for (int64_t o = tid; o < D; o += kThreads) dot += double(pval(o)) * double(Gi[o]);
+      for (int64_t o = tid; o < D; o += kThreads) dot += double(pval(o)) * double(gval(o));
       for (int off = 32; off > 0; off >>= 1) dot += __shfl_down(dot, off, 64);
       if (lane == 0) red[wave][0] = dot;
       __syncthreads();
@@ -908,12 +917,11 @@ __global__ __launch_bounds__(kThreads) void adjoint_kernel(
   T pre[kAdjPre];
   auto fetch = [&](int key) {
     if (!prefetch) return;
-    const T* Gk = Gi + int64_t(key) * KB;
     const int64_t pk = int64_t(key) * KB;
 #pragma unroll
     for (int r = 0; r < kAdjPre; ++r) {
       const int o = tid + kThreads * r;
-      pre[r] = o < KB ? ta * Gk[o] - tb * pval(pk + o) : T(0);
+      pre[r] = o < KB ? ta * gval(pk + o) - tb * pval(pk + o) : T(0);
     }
   };
   fetch(0);
@@ -933,10 +941,9 @@ __global__ __launch_bounds__(kThreads) void adjoint_kernel(
         }
       }
     } else {
-      const T* Gk = Gi + int64_t(key) * KB;
       const int64_t pk = int64_t(key) * KB;
       for (int o = tid; o < KB; o += kThreads) {
-        const T g = ta * Gk[o] - tb * pval(pk + o);
+        const T g = ta * gval(pk + o) - tb * pval(pk + o);
         const int x = o / fNL1, rr = o - x * N * L1, y = rr / fL1, l = rr - y * L1;
         gA[(l * N + y) * N4 + x] = g;
         gB[(l * N + x) * N4 + y] = g;
@@ -1373,7 +1380,8 @@ void launch_adjoint_t(const RadialDev& rad, int S, const PairView& pv, const T* 
                       const unsigned short* p_hi, const unsigned short* p_lo,
                       const double* norm2, bool normalize, const double* xg, double xg_scale,
                       const T* G, double* grad, double* centre_virial, hipStream_t stream,
-                      const SoapTiles& tiles) {
+                      const SoapTiles& tiles, const T* Grow = nullptr,
+                      const double* Gscale = nullptr) {
   const int N = rad.N, L1 = rad.L1, LL = rad.LL, F = N * L1, E = S * N * LL;
   const size_t yt = ylm_table_bytes<T>(L1);
   const size_t blk = align_up(sizeof(T) * size_t(E), 64);
@@ -1427,7 +1435,8 @@ void launch_adjoint_t(const RadialDev& rad, int S, const PairView& pv, const T* 
     auto kern = adjoint_kernel<T, DIRECT>;
     allow_lds(kern, smem);
     hipLaunchKernelGGL(kern, dim3(unsigned(pv.n_centres)), dim3(kThreads), smem, stream, rad, S, PB,
-                       c_in_lds, pv, c, p, norm2, int(normalize), xg, xg_scale, G, grad, centre_virial);
+                       c_in_lds, pv, c, p, norm2, int(normalize), xg, xg_scale, G, Grow, Gscale, grad,
+                       centre_virial);
     RGPU_LAUNCH_CHECK();
   }
 }
@@ -1523,6 +1532,19 @@ void launch_backward(const RadialDev& rad, int S, const PairView& pv, const T* c
                               grad, centre_virial, stream, tiles);
 }
 
+void launch_backward_rank1(const RadialDev& rad, int S, const PairView& pv, const double* c,
+                           const double* p, const double* norm2, bool normalize, const double* Grow,
+                           const double* Gscale, double* grad, double* centre_virial,
+                           hipStream_t stream, const SoapTiles& tiles) {
+  if (pv.n_centres == 0) return;
+  if (rad.use_spline)
+    launch_adjoint_t<double, false>(rad, S, pv, c, p, nullptr, nullptr, norm2, normalize, nullptr, 1.0,
+                                    nullptr, grad, centre_virial, stream, tiles, Grow, Gscale);
+  else
+    launch_adjoint_t<double, true>(rad, S, pv, c, p, nullptr, nullptr, norm2, normalize, nullptr, 1.0,
+                                   nullptr, grad, centre_virial, stream, tiles, Grow, Gscale);
+}
+
 void launch_backward_split(const RadialDev& rad, int S, const PairView& pv, const float* c,
                            const unsigned short* p_hi, const unsigned short* p_lo,
                            const double* norm2, bool normalize, const double* xg, double xg_scale,
@@ -1541,6 +1563,41 @@ void launch_pack_gradient(int S, int N, int L1, int64_t n, const double* G, floa
                           hipStream_t stream) {
   if (n == 0) return;
   hipLaunchKernelGGL(pack_gradient_kernel, dim3(unsigned(n)), dim3(kThreads), 0, stream, S, N, L1, G, Gp);
+  RGPU_LAUNCH_CHECK();
+}
+
+__global__ void knm_gradient_scale_kernel(int64_t n, const double* __restrict__ d, int64_t ld,
+                                          const int* __restrict__ species_of, int s, int zeta,
+                                          double* __restrict__ scale) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double v = 0.0;
+  if (species_of[i] == s) {
+    const double x = d[i * ld];
+    v = double(zeta);
+    for (int k = 1; k < zeta; ++k) v *= x;
+  }
+  scale[i] = v;
+}
+
+__global__ void strided_copy_kernel(const double* __restrict__ src, int64_t n, double* __restrict__ dst,
+                                    int64_t ld) {
+  const int64_t r = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (r < n) dst[r * ld] = src[r];
+}
+
+void launch_knm_gradient_scale(int64_t n, const double* d, int64_t ld, const int* species_of, int s,
+                               int zeta, double* scale, hipStream_t stream) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(knm_gradient_scale_kernel, dim3(unsigned((n + 255) / 256)), dim3(256), 0, stream, n,
+                     d, ld, species_of, s, zeta, scale);
+  RGPU_LAUNCH_CHECK();
+}
+
+void launch_strided_copy(const double* src, int64_t n, double* dst, int64_t ld, hipStream_t stream) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(strided_copy_kernel, dim3(unsigned((n + 255) / 256)), dim3(256), 0, stream, src, n,
+                     dst, ld);
   RGPU_LAUNCH_CHECK();
 }
 

@@ -15,6 +15,7 @@ product, as in the reference).
 """
 import numpy as np
 
+from . import _capi
 from .neighbourlist import AtomsList
 from .representations import SphericalInvariants
 
@@ -162,9 +163,21 @@ class Kernel:
                 rep._load_species(X.all_species)
             _set_model(rep, X, self.zeta, None)
             return rep._context.gap_kmm()
-        if tuple(grad) == (True, False):
-            raise NotImplementedError(
-                "K_NM derivative rows (training with forces) are not provided on the GPU yet")
+        if tuple(grad) == (True, False) and isinstance(Y, SparsePoints):
+            # K_NM gradient rows (SparseKernel::compute_derivative), in FP64
+            if not isinstance(X, AtomsList):
+                raise ValueError("X must be an AtomsList")
+            X._require(rep)
+            _set_model(rep, Y, self.zeta, None)
+            ctx = rep._context
+            prec = ctx.info("precision")
+            if prec != _capi.FP64:
+                ctx.set_precision(_capi.FP64)
+            try:
+                return ctx.gap_knm_gradient(compute_neg_stress)
+            finally:
+                if prec != _capi.FP64:
+                    ctx.set_precision(prec)
         if tuple(grad) != (False, False) or not isinstance(Y, SparsePoints):
             raise NotImplementedError("The configuration: {} is not implemented for kernel {} "
                                       "in {} mode.".format(grad, self.name, self.kernel_type))

@@ -59,6 +59,28 @@ def test_kernel_kmm_and_knm(zeta):
         assert rel_err(kr, k).max() < TOL
 
 
+@pytest.mark.parametrize("zeta", [1, 2, 3])
+def test_kernel_gradient_rows(zeta):
+    """K_NM gradient rows with the negative-stress rows
+    (SparseKernel::compute_derivative) vs the oracle, columns in the order
+    the sparse points were added; contracted with the weights they give the
+    model's energy gradient and negative stress."""
+    rep = _rep()
+    frames, al, sp, kernel, w = _model(rep, zeta)
+    calc = O.Calculator(rep.hypers)
+    b = O.BatchData(frames, species=al.species)
+    ss = O.SparseSet(sp.numbers, sp.values_for(al.species), zeta)
+    kd = kernel(al, sp, grad=(True, False), compute_neg_stress=True)
+    assert kd.shape == (3 * b.n_atoms + 6 * b.n_struct, len(sp.numbers))
+    ref = np.zeros_like(kd)
+    ref[:, ss.order] = O.gap_knm_derivative(calc, b, ss, neg_stress=True)
+    assert np.abs(kd - ref).max() < TOL * np.abs(ref).max()
+    g = compute_sparse_kernel_gradients(rep, kernel, al, sp, w)
+    ns = compute_sparse_kernel_neg_stress(rep, kernel, al, sp, w)
+    assert np.abs(kd[:3 * b.n_atoms] @ w - g.ravel()).max() < TOL * np.abs(g).max()
+    assert np.abs(kd[3 * b.n_atoms:] @ w - ns.ravel()).max() < TOL * np.abs(ns).max()
+
+
 def test_krr_predictions_and_gradient_functions():
     rep = _rep()
     frames, al, sp, kernel, w = _model(rep, 2)
