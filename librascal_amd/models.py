@@ -278,6 +278,16 @@ class KRR:
             s = (KNM @ self.weights).reshape(len(managers), 6)
         return -s
 
+    def predict_energy_forces_stress(self, managers):
+        """predict, predict_forces and predict_stress of a Structure-target
+        sparse model from one GPU prediction (energy, gradient and virial
+        together)."""
+        if self.kernel.kernel_type != "Sparse" or self.target_type != "Structure":
+            raise NotImplementedError("one-pass prediction needs a sparse Structure kernel")
+        ctx = _predict_raw(self.kernel._rep, self.kernel, managers, self.X_train, self.weights)
+        e, g, s = ctx.gap_predict(energy=True, gradient=True, neg_stress=True)
+        return self._get_property_baseline(managers) + e, -g, -s
+
     def get_weights(self):
         return self.weights
 

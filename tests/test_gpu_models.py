@@ -168,3 +168,69 @@ def test_empty_batch_fixes_the_layout():
     rep._load_species([1, 6, 8])
     X, inv = rep._context.features()
     assert X.shape == (0, rep.get_num_coefficients(3)) and inv.shape == (0,)
+
+
+def _model_file():
+    import os
+    from conftest import GOLDEN
+    return os.path.join(GOLDEN, "simple_gap_model.json")
+
+
+def test_reference_model_file_predicts_like_oracle():
+    """The reference's saved KRR model (simple_gap_model.json: methane
+    dimers, RadialScaling rc 7.5, n 4, l 2, direct Hyp1f1, 30 sparse points,
+    zeta 2) loaded with io.load_obj: energies, forces and stress of methane
+    dimers vs the oracle with the same hypers, sparse points and weights."""
+    from conftest import wrap_cubic_periodic
+    from librascal_amd.io import load_obj
+    model = load_obj(_model_file())
+    rep = model.get_representation_calculator()
+    sts = [wrap_cubic_periodic(s) for s in golden_structures()["methane_dimers"][:8]]
+    al = rep.transform(sts)
+    e = model.predict(al)
+    f = model.predict_forces(al)
+    s = model.predict_stress(al)
+    calc = O.Calculator(rep.hypers)
+    b = O.BatchData(sts, species=al.species)
+    ss = O.SparseSet(model.X_train.numbers, model.X_train.values_for(al.species), model.kernel.zeta)
+    e_ref, g_ref, s_ref = O.gap_predict(calc, b, ss, model.weights)
+    assert rel_err(e_ref, e).max() < TOL
+    assert np.abs(f + g_ref).max() < TOL * np.abs(g_ref).max()
+    assert np.abs(s + s_ref).max() < TOL * np.abs(s_ref).max()
+    e1, f1, s1 = model.predict_energy_forces_stress(al)
+    assert np.allclose(e1, e, rtol=1e-13) and np.allclose(f1, f, rtol=1e-13, atol=0)
+    assert np.allclose(s1, s, rtol=1e-13, atol=0)
+
+
+def test_generic_md_calculator():
+    """models/genericmd.py behaviour (tests/python/md_calculator_test.py of
+    the reference, on a methane dimer in its periodic box): repeated calls
+    agree, uniform translations leave energy, forces and stress unchanged, a
+    different cell changes them, the stress matrix is symmetric, and the
+    argument checks raise the reference's errors."""
+    from conftest import wrap_cubic_periodic
+    from librascal_amd.genericmd import GenericMDCalculator
+    st = wrap_cubic_periodic(golden_structures()["methane_dimers"][0])
+    calc = GenericMDCalculator(_model_file(), True, structure_template=st)
+    pos = np.asarray(st["positions"])
+    cell_rows = np.asarray(st["cell"]).T
+    e, f, s = calc.calculate(pos, cell_rows)
+    assert np.all(np.isfinite(f)) and np.all(np.isfinite(s)) and np.isfinite(e).all()
+    assert f.shape == pos.shape and s.shape == (3, 3) and np.allclose(s, s.T)
+    e2, f2, s2 = calc.calculate(pos, cell_rows)
+    assert np.allclose(e, e2) and np.allclose(f, f2) and np.allclose(s, s2)
+    e3, f3, s3 = calc.calculate(pos + 5.0, cell_rows)
+    assert np.allclose(e, e3) and np.allclose(f, f3) and np.allclose(s, s3)
+    e4, f4, _ = calc.calculate(pos, np.eye(3) * 1.2 * np.abs(cell_rows).max())
+    assert not np.allclose(e, e4)
+    other = GenericMDCalculator(_model_file(), True, atomic_numbers=st["numbers"])
+    e5, f5, s5 = other.calculate(pos, cell_rows)
+    assert np.allclose(e, e5) and np.allclose(f, f5) and np.allclose(s, s5)
+    with pytest.raises(ValueError, match="Must specify one of 'structure_template' or 'atomic_numbers'"):
+        GenericMDCalculator(_model_file(), True)
+    with pytest.raises(ValueError, match="Structure template PBC flags.*incompatible"):
+        GenericMDCalculator(_model_file(), False, structure_template=st)
+    with pytest.raises(ValueError, match="Improper shape of positions"):
+        calc.calculate(pos[:5], cell_rows)
+    with pytest.raises(ValueError, match=r"Improper shape of cell info \(expected 3x3 matrix\)"):
+        calc.calculate(pos, cell_rows[:2])
