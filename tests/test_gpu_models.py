@@ -201,9 +201,11 @@ def test_reference_model_file_predicts_like_oracle():
     assert np.all(np.abs(e - e_ref) < TOL * (np.abs(K) @ np.abs(model.weights)))
     assert np.abs(f + g_ref).max() < TOL * np.abs(g_ref).max()
     assert np.abs(s + s_ref).max() < TOL * np.abs(s_ref).max()
+    # one GPU pass for all three (force atomics sum in any order)
     e1, f1, s1 = model.predict_energy_forces_stress(al)
-    assert np.allclose(e1, e, rtol=1e-13) and np.allclose(f1, f, rtol=1e-13, atol=0)
-    assert np.allclose(s1, s, rtol=1e-13, atol=0)
+    assert np.abs(e1 - e).max() < 1e-12 * np.abs(e).max()
+    assert np.abs(f1 - f).max() < 1e-12 * np.abs(f).max()
+    assert np.abs(s1 - s).max() < 1e-12 * np.abs(s).max()
 
 
 def test_generic_md_calculator():
