@@ -229,3 +229,25 @@ def test_mixed_adjoint_partitions_agree(N, L):
         assert np.abs(g - g64).max() < 2e-5 * scale, key
         assert np.abs(s - s64).max() < 2e-5 * np.abs(s64).max(), key
         assert np.abs(g - out[0, 0][0]).max() < 1e-5 * scale, key
+
+
+def test_large_gemm_tiles_mixed_vs_fp64():
+    """A batch large enough for the 256 x 128 GEMM tiles in both products
+    (2 species x ~2048 centres, 2048 sparse points per species, N 12, L 9):
+    mixed energies, forces and stress vs the FP64 GPU path."""
+    sts = periodic_box(4096, 0.05, (1, 8), d_min=1.0, seed=99)
+    hp = soap_hypers(rc=4.0, sw=0.5, sigma=0.5, N=12, L=9, spline=1e-8)
+    ctx, sp = make_ctx(hp, sts)
+    X, _ = ctx.features()
+    num = pack(sts)[2]
+    rng = np.random.default_rng(3)
+    spec = np.concatenate([np.full(2048, t, np.int32) for t in sp])
+    vals = np.concatenate([X[rng.choice(np.where(num == t)[0], 2048)] for t in sp])
+    w = rng.normal(size=len(spec))
+    ctx.set_sparse_points(3, spec, vals, w)
+    e64, g64, s64 = ctx.gap_predict()
+    ctx.set_precision(_capi.MIXED)
+    e32, g32, s32 = ctx.gap_predict()
+    assert rel_err(e64, e32).max() < 1e-4
+    assert np.sqrt(np.mean((g32 - g64) ** 2)) / np.sqrt(np.mean(g64 ** 2)) < 1e-4
+    assert np.abs(s32 - s64).max() < 1e-4 * np.abs(s64).max()
