@@ -32,15 +32,7 @@ namespace {
 typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
 typedef __attribute__((__vector_size__(16 * sizeof(float)))) float f32x16;
 
-// block tile BM(WM) x BN, K step BK; WM wavefront rows of 64 (2: 128 x 128
-// tiles, 4 waves; 4: 256 x 128 tiles, 8 waves), 64 x 64 per wavefront
-constexpr int BN = 128, BK = 32;
-template <int WM>
-struct Tile {
-  static constexpr int BM = 64 * WM, NT = 128 * WM;
-  static constexpr int JA = 2, JB = 4 / WM;  // rows loaded per thread (A, B)
-  static constexpr int FRA = (BM / 32) * (BK / 16) * 64, FRB = (BN / 32) * (BK / 16) * 64;
-};
+constexpr int BM = 128, BN = 128, BK = 32, NT = 256;
 constexpr int kMaxProblems = 16;
 constexpr int kGroupM = 8;  // row panels per tile group
 
@@ -104,21 +96,20 @@ __device__ __forceinline__ int frag_slot(int rb, int c, int l) {
   return (rb * (BK / 16) + c) * 64 + (l ^ (4 * c + 2 * (l >> 5)));
 }
 
+constexpr int FR = (BM / 32) * (BK / 16) * 64;  // fragments per operand per LDS stage
+
 // LDS of one workgroup: two stages of the four operand planes
-template <int WM>
 struct GemmLds {
-  bf16x8 Ah[2][Tile<WM>::FRA];
-  bf16x8 Al[2][Tile<WM>::FRA];
-  bf16x8 Bh[2][Tile<WM>::FRB];
-  bf16x8 Bl[2][Tile<WM>::FRB];
-  double esum[2][Tile<WM>::BM];
+  bf16x8 Ah[2][FR];
+  bf16x8 Al[2][FR];
+  bf16x8 Bh[2][FR];
+  bf16x8 Bl[2][FR];
+  double esum[2][BM];
 };
 
-template <int EPI, bool FAST, int WM>
+template <int EPI, bool FAST>
 __device__ __forceinline__ void gemm_tile(const ProblemSet& ps, const Problem& P, int64_t r0, int64_t c0,
-                                          GemmLds<WM>& sm) {
-  constexpr int BM = Tile<WM>::BM, NT = Tile<WM>::NT, JA = Tile<WM>::JA, JB = Tile<WM>::JB;
-  constexpr int RS = NT / 4;  // rows covered by one load of all threads
+                                          GemmLds& sm) {
   auto& sAh = sm.Ah;
   auto& sAl = sm.Al;
   auto& sBh = sm.Bh;
@@ -129,41 +120,33 @@ __device__ __forceinline__ void gemm_tile(const ProblemSet& ps, const Problem& P
   const int wm = wave >> 1, wn = wave & 1;
 
   // Global loads are coalesced along K: four lanes read the 32 K values
-  // (64 B per plane) of one row, rows rs + RS j.  Rows past the matrix edge
-  // repeat its last row (their results are never stored).
+  // (64 B per plane) of one row, rows rs + 64 j (j < 2).  Rows past the
+  // matrix edge repeat its last row (their results are never stored).
   const int rs = tid >> 2, kq = tid & 3;
-  const unsigned short *Ah[JA], *Al[JA], *Bh[JB], *Bl[JB];
+  const unsigned short *Ah[2], *Al[2], *Bh[2], *Bl[2];
 #pragma unroll
-  for (int j = 0; j < JA; ++j) {
-    int64_t ra = r0 + rs + RS * j;
+  for (int j = 0; j < 2; ++j) {
+    int64_t ra = r0 + rs + 64 * j;
     ra = ra < P.M ? ra : P.M - 1;
     if (P.a_rows) ra = P.a_rows[ra];
     Ah[j] = P.Ahi + ra * P.lda;
     Al[j] = P.Alo + ra * P.lda;
-  }
-#pragma unroll
-  for (int j = 0; j < JB; ++j) {
-    int64_t rb = c0 + rs + RS * j;
+    int64_t rb = c0 + rs + 64 * j;
     rb = rb < P.N ? rb : P.N - 1;
     Bh[j] = P.Bhi + rb * P.ldb;
     Bl[j] = P.Blo + rb * P.ldb;
   }
   const int sc = kq >> 1, shalf = kq & 1;
-  int fsa[JA], fsb[JB];
+  int fs[2];
 #pragma unroll
-  for (int j = 0; j < JA; ++j) {
-    const int r = rs + RS * j;
-    fsa[j] = frag_slot(r >> 5, sc, (r & 31) + 32 * shalf);
-  }
-#pragma unroll
-  for (int j = 0; j < JB; ++j) {
-    const int r = rs + RS * j;
-    fsb[j] = frag_slot(r >> 5, sc, (r & 31) + 32 * shalf);
+  for (int j = 0; j < 2; ++j) {
+    const int r = rs + 64 * j;
+    fs[j] = frag_slot(r >> 5, sc, (r & 31) + 32 * shalf);
   }
 
   // two register stages: the global loads of K step kt + 2 are issued while
   // step kt computes, and stored to LDS at the end of step kt + 1
-  uint4 rah[2][JA], ral[2][JA], rbh[2][JB], rbl[2][JB];
+  uint4 rah[2][2], ral[2][2], rbh[2][2], rbl[2][2];
   auto load8 = [&](const unsigned short* p, int64_t k) -> uint4 {
     if (FAST) return *reinterpret_cast<const uint4*>(p + k);
     unsigned short t[8];
@@ -176,26 +159,20 @@ __device__ __forceinline__ void gemm_tile(const ProblemSet& ps, const Problem& P
   auto gload = [&](int rs_, int64_t k0) {
     const int64_t k = k0 + 8 * kq;
 #pragma unroll
-    for (int j = 0; j < JA; ++j) {
+    for (int j = 0; j < 2; ++j) {
       rah[rs_][j] = load8(Ah[j], k);
       ral[rs_][j] = load8(Al[j], k);
-    }
-#pragma unroll
-    for (int j = 0; j < JB; ++j) {
       rbh[rs_][j] = load8(Bh[j], k);
       rbl[rs_][j] = load8(Bl[j], k);
     }
   };
   auto lstore = [&](int rs_, int st) {
 #pragma unroll
-    for (int j = 0; j < JA; ++j) {
-      sAh[st][fsa[j]] = __builtin_bit_cast(bf16x8, rah[rs_][j]);
-      sAl[st][fsa[j]] = __builtin_bit_cast(bf16x8, ral[rs_][j]);
-    }
-#pragma unroll
-    for (int j = 0; j < JB; ++j) {
-      sBh[st][fsb[j]] = __builtin_bit_cast(bf16x8, rbh[rs_][j]);
-      sBl[st][fsb[j]] = __builtin_bit_cast(bf16x8, rbl[rs_][j]);
+    for (int j = 0; j < 2; ++j) {
+      sAh[st][fs[j]] = __builtin_bit_cast(bf16x8, rah[rs_][j]);
+      sAl[st][fs[j]] = __builtin_bit_cast(bf16x8, ral[rs_][j]);
+      sBh[st][fs[j]] = __builtin_bit_cast(bf16x8, rbh[rs_][j]);
+      sBl[st][fs[j]] = __builtin_bit_cast(bf16x8, rbl[rs_][j]);
     }
   };
 
@@ -331,9 +308,8 @@ __device__ __forceinline__ void gemm_tile(const ProblemSet& ps, const Problem& P
   }
 }
 
-template <int EPI, int WM>  // EPI 0: GAP kernel epilogue, 1: store rows of G
-__global__ __launch_bounds__(Tile<WM>::NT, 4 / WM) void bf16x3_gemm(ProblemSet ps) {
-  constexpr int BM = Tile<WM>::BM;
+template <int EPI>  // 0: GAP kernel epilogue, 1: store rows of G
+__global__ __launch_bounds__(NT, 2) void bf16x3_gemm(ProblemSet ps) {
   // problem and tile of this block.  Workgroups are dealt to the 8 XCDs
   // round robin; each XCD takes a contiguous range of linear tiles, and the
   // linear order walks groups of kGroupM row panels column by column, so
@@ -351,11 +327,11 @@ __global__ __launch_bounds__(Tile<WM>::NT, 4 / WM) void bf16x3_gemm(ProblemSet p
   const int64_t grp = local / per_group, in = local - grp * per_group;
   const int64_t gm = std::min<int64_t>(kGroupM, P.tiles_m - grp * kGroupM);
   const int64_t tn = in / gm, tm = grp * kGroupM + (in - tn * gm);
-  __shared__ __align__(16) GemmLds<WM> sm;
+  __shared__ __align__(16) GemmLds sm;
   if (P.fast)
-    gemm_tile<EPI, true, WM>(ps, P, tm * BM, tn * BN, sm);
+    gemm_tile<EPI, true>(ps, P, tm * BM, tn * BN, sm);
   else
-    gemm_tile<EPI, false, WM>(ps, P, tm * BM, tn * BN, sm);
+    gemm_tile<EPI, false>(ps, P, tm * BM, tn * BN, sm);
 }
 
 unsigned short bf16_rne_host(float x) {
@@ -378,39 +354,14 @@ void finish_problem(Problem& P) {
            aligned16(P.Alo) && aligned16(P.Bhi) && aligned16(P.Blo);
 }
 
-// tile grid of every problem for block rows BM; returns the total
-int64_t assign_tiles(ProblemSet& ps, int BM) {
-  int64_t tiles = 0;
-  for (int q = 0; q < ps.n; ++q) {
-    Problem& P = ps.p[q];
-    P.tile0 = tiles;
-    P.tiles_m = int((P.M + BM - 1) / BM);
-    P.tiles_n = int((P.N + BN - 1) / BN);
-    tiles += int64_t(P.tiles_m) * P.tiles_n;
-  }
-  return tiles;
-}
-
-// 256 x 128 tiles (8 wavefronts) when they still give every CU a tile,
-// else 128 x 128 tiles
-void launch(ProblemSet& ps, int epi, hipStream_t stream) {
-  int64_t tiles = assign_tiles(ps, Tile<4>::BM);
-  const bool big = tiles >= 256;
-  if (!big) tiles = assign_tiles(ps, Tile<2>::BM);
-  if (tiles == 0) return;
-  ps.total_tiles = tiles;
-  const unsigned grid = unsigned((tiles + 7) / 8 * 8);  // whole rounds over the 8 XCDs
-  if (big) {
-    if (epi == 0)
-      hipLaunchKernelGGL((bf16x3_gemm<0, 4>), dim3(grid), dim3(Tile<4>::NT), 0, stream, ps);
-    else
-      hipLaunchKernelGGL((bf16x3_gemm<1, 4>), dim3(grid), dim3(Tile<4>::NT), 0, stream, ps);
-  } else {
-    if (epi == 0)
-      hipLaunchKernelGGL((bf16x3_gemm<0, 2>), dim3(grid), dim3(Tile<2>::NT), 0, stream, ps);
-    else
-      hipLaunchKernelGGL((bf16x3_gemm<1, 2>), dim3(grid), dim3(Tile<2>::NT), 0, stream, ps);
-  }
+void launch(ProblemSet& ps, int64_t total_tiles, int epi, hipStream_t stream) {
+  if (total_tiles == 0) return;
+  ps.total_tiles = total_tiles;
+  const unsigned grid = unsigned((total_tiles + 7) / 8 * 8);  // whole rounds over the 8 XCDs
+  if (epi == 0)
+    hipLaunchKernelGGL(bf16x3_gemm<0>, dim3(grid), dim3(NT), 0, stream, ps);
+  else
+    hipLaunchKernelGGL(bf16x3_gemm<1>, dim3(grid), dim3(NT), 0, stream, ps);
   RGPU_LAUNCH_CHECK();
 }
 
@@ -476,6 +427,7 @@ void run_gap_mixed_kernel(const GapBlock<float>* blks, int n_blocks, const Mixed
                           SplitPlanes xs, double* energy, const SplitPlanes* wts, hipStream_t stream) {
   for (int b0 = 0; b0 < n_blocks; b0 += kMaxProblems) {
     ProblemSet ps{};
+    int64_t tiles = 0;
     for (int b = b0; b < std::min(n_blocks, b0 + kMaxProblems); ++b) {
       const GapBlock<float>& blk = blks[b];
       if (blk.n_rows == 0 || blk.n_cols == 0) continue;
@@ -491,7 +443,9 @@ void run_gap_mixed_kernel(const GapBlock<float>* blks, int n_blocks, const Mixed
       P.Blo = ms.t_lo.get() + blk.col0 * blk.D;
       P.ldb = blk.D;
       P.c_rows = blk.row_idx;
-      P.tile0 = 0;
+      P.tile0 = tiles;
+      P.tiles_m = int((P.M + BM - 1) / BM);
+      P.tiles_n = int((P.N + BN - 1) / BN);
       P.norm2 = blk.norm2;
       if (!blk.alpha) throw InvalidArgument("the GAP energy kernel needs weights");
       P.alpha = blk.alpha + blk.col0;
@@ -499,10 +453,11 @@ void run_gap_mixed_kernel(const GapBlock<float>* blks, int n_blocks, const Mixed
       P.wth = wts ? wts[b].hi : nullptr;
       P.wtl = wts ? wts[b].lo : nullptr;
       finish_problem(P);
+      tiles += int64_t(P.tiles_m) * P.tiles_n;
       ps.normalize = blk.normalize;
       ps.zeta = blk.zeta;
     }
-    launch(ps, 0, stream);
+    launch(ps, tiles, 0, stream);
   }
 }
 
@@ -510,6 +465,7 @@ void run_gap_mixed_force(const GapBlock<float>* blks, int n_blocks, const MixedS
                          const SplitPlanes* wts, float* G, hipStream_t stream) {
   for (int b0 = 0; b0 < n_blocks; b0 += kMaxProblems) {
     ProblemSet ps{};
+    int64_t tiles = 0;
     for (int b = b0; b < std::min(n_blocks, b0 + kMaxProblems); ++b) {
       const GapBlock<float>& blk = blks[b];
       if (blk.n_rows == 0 || blk.n_cols == 0) continue;
@@ -525,12 +481,15 @@ void run_gap_mixed_force(const GapBlock<float>* blks, int n_blocks, const MixedS
       P.Blo = ms.tt_lo.get() + blk.col0;
       P.ldb = ms.M;
       P.c_rows = blk.row_idx;
-      P.tile0 = 0;
+      P.tile0 = tiles;
+      P.tiles_m = int((P.M + BM - 1) / BM);
+      P.tiles_n = int((P.N + BN - 1) / BN);
       P.G = G;
       P.ldg = blk.D;
       finish_problem(P);
+      tiles += int64_t(P.tiles_m) * P.tiles_n;
     }
-    launch(ps, 1, stream);
+    launch(ps, tiles, 1, stream);
   }
 }
 

@@ -232,9 +232,9 @@ def test_mixed_adjoint_partitions_agree(N, L):
 
 
 def test_large_gemm_tiles_mixed_vs_fp64():
-    """A batch large enough for the 256 x 128 GEMM tiles in both products
-    (2 species x ~2048 centres, 2048 sparse points per species, N 12, L 9):
-    mixed energies, forces and stress vs the FP64 GPU path."""
+    """A larger mixed batch (2 species x ~2048 centres, 2048 sparse points
+    per species, N 12, L 9; a few hundred GEMM tiles per product): mixed
+    energies, forces and stress vs the FP64 GPU path."""
     sts = periodic_box(4096, 0.05, (1, 8), d_min=1.0, seed=99)
     hp = soap_hypers(rc=4.0, sw=0.5, sigma=0.5, N=12, L=9, spline=1e-8)
     ctx, sp = make_ctx(hp, sts)
