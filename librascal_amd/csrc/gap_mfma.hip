@@ -212,22 +212,27 @@ __device__ __forceinline__ void gemm_tile(const ProblemSet& ps, const Problem& P
   };
 
   const int64_t nk = (P.K + BK - 1) / BK;
+  // K step of a prefetch, clamped to the last one: the loads and LDS stores
+  // are issued unconditionally (the surplus ones at the end are never read),
+  // which keeps the wait before each LDS store at the loads it consumes
+  // rather than at every load in flight
+  auto step = [&](int64_t kt) { return (kt < nk ? kt : nk - 1) * BK; };
   gload(0, 0);
-  if (nk > 1) gload(1, BK);
+  gload(1, step(1));
   lstore(0, 0);
   __syncthreads();
   // K steps in pairs so that the register stage indices are constants;
   // step kt reads LDS stage kt & 1 and register stage kt & 1 is refilled
   for (int64_t kt = 0; kt < nk; kt += 2) {
-    if (kt + 2 < nk) gload(0, (kt + 2) * BK);
+    gload(0, step(kt + 2));
     compute(0);
     // the other LDS stage was last read before the previous barrier
-    if (kt + 1 < nk) lstore(1, 1);
+    lstore(1, 1);
     __syncthreads();
     if (kt + 1 >= nk) break;
-    if (kt + 3 < nk) gload(1, (kt + 3) * BK);
+    gload(1, step(kt + 3));
     compute(1);
-    if (kt + 2 < nk) lstore(0, 0);
+    lstore(0, 0);
     __syncthreads();
   }
 
