@@ -1031,6 +1031,9 @@ __global__ __launch_bounds__(kThreads) void adjoint_staged_kernel(
     packed_pair(t, N, &n1, &n2);
     tri[t] = (n1 << 8) | n2;
   }
+  // the first chunk's staging reads tri, and the X . G reduction below is
+  // skipped when xg is given or the rows are not normalised
+  __syncthreads();
 
   const T* Gi = G + i * Dq;
   const unsigned short* Hi = p_hi + i * Dq;

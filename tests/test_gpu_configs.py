@@ -251,3 +251,32 @@ def test_large_gemm_tiles_mixed_vs_fp64():
     assert rel_err(e64, e32).max() < 1e-4
     assert np.sqrt(np.mean((g32 - g64) ** 2)) / np.sqrt(np.mean(g64 ** 2)) < 1e-4
     assert np.abs(s32 - s64).max() < 1e-4 * np.abs(s64).max()
+
+
+@pytest.mark.parametrize("normalize", [True, False])
+def test_mixed_prediction_adjoint_repeatable(normalize):
+    """The prediction hands the adjoint X_i . G_i = zeta E_i (no in-kernel
+    reduction, so none of its barriers): repeated mixed predictions agree to
+    rounding, and their forces equal features_backward of the same dE/dX
+    (zeta 1: dE/dX_i = sum over the species' sparse points of w_m T_m), where
+    the adjoint reduces X . G itself.  Guards the ordering of the adjoint's
+    LDS tables against their first use."""
+    sts = periodic_box(4096, 0.05, (1, 8), d_min=1.0, seed=99)
+    hp = soap_hypers(rc=4.0, sw=0.5, sigma=0.5, N=12, L=9, spline=1e-8, normalize=normalize)
+    ctx, sp = make_ctx(hp, sts)
+    X, _ = ctx.features()
+    num = pack(sts)[2]
+    rng = np.random.default_rng(5)
+    spec = np.concatenate([np.full(2048, t, np.int32) for t in sp])
+    vals = np.concatenate([X[rng.choice(np.where(num == t)[0], 2048)] for t in sp])
+    w = rng.normal(size=len(spec))
+    ctx.set_sparse_points(1, spec, vals, w)
+    G = np.zeros_like(X)
+    for t in sp:
+        G[num == t] = (w[spec == t][:, None] * vals[spec == t]).sum(0)
+    ctx.set_precision(_capi.MIXED)
+    ref = ctx.features_backward(G)[0]
+    scale = np.abs(ref).max()
+    for _ in range(4):
+        g = ctx.gap_predict()[1]
+        assert np.abs(g - ref).max() < 1e-5 * scale
