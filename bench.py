@@ -168,10 +168,11 @@ def main():
     k_avg_ms = k_ms / max(k_n, 1)
     achieved = flop_gemm / (k_avg_ms * 1e-3)
     peak = PEAK_BF16 / 3.0  # three BF16 products per algorithmic multiply-add
-    traffic = profiled_traffic("bf16x3_gemm<0>")
+    traffic = profiled_traffic(GEMM_KERNEL)
     roofline = dict(bound="mfma", achieved=achieved / 1e12, peak=peak / 1e12, unit="TFLOP/s",
                     frac=achieved / peak, traffic=traffic,
-                    kernel="gap_kernel (K = (X T^T)^zeta, all species blocks in one launch)",
+                    kernel="gap_kernel (%s: K = (X T^T)^zeta, all species blocks in one launch)"
+                    % GEMM_KERNEL,
                     avg_launch_ms=k_avg_ms, flop_per_launch=flop_gemm, packed_columns=Dp,
                     bf16_mfma_utilisation=3.0 * achieved / PEAK_BF16,
                     traffic_source=TRAFFIC_FILE if traffic is not None else None)
@@ -217,6 +218,9 @@ def main():
 
 
 TRAFFIC_FILE = "profiles/hbm_traffic_periodic_box_8192.md"
+# the energy GEMM's kernel (the packed layout always takes the direct-to-LDS
+# variant)
+GEMM_KERNEL = "bf16x3_gemm_dma<0, 128, 2>"
 
 
 def profiled_traffic(kernel):

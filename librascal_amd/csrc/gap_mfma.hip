@@ -14,8 +14,12 @@
 // Tiling: 256 threads (4 wavefronts, 2 x 2), block tile 128 x 128, K step 32;
 // each wavefront owns 64 x 64 = 2 x 2 MFMA tiles.  Two LDS stages (one
 // barrier per K step), the next K step's global loads issued before the
-// MFMAs of the current one.  All species blocks of one product go in one
-// grouped launch, dealt to the XCDs so that a row panel stays in one L2.
+// MFMAs of the current one.  Where every row is 16-byte aligned and K is a
+// multiple of 32 (the packed layout guarantees both), the loads go straight
+// to LDS (global_load_lds_dwordx4) in a swizzled layout; otherwise they pass
+// through registers with a guarded K tail.  All species blocks of one
+// product go in one grouped launch, dealt to the XCDs so that a row panel
+// stays in one L2.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -107,6 +111,90 @@ struct GemmLds {
   double esum[2][BM];
 };
 
+// Epilogue of a TBM x 128 block tile whose wavefronts form (TBM / 64) x 2
+// and own 64 x 64 = 2 x 2 MFMA tiles each; esum: LDS [2][TBM]
+template <int EPI, int TBM>
+__device__ __forceinline__ void gemm_epilogue(const ProblemSet& ps, const Problem& P, int64_t r0,
+                                              int64_t c0, f32x16 (&acc)[2][2], double (*esum)[TBM]) {
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+  // epilogue; element e = 4*g + q of a 32x32 tile sits at
+  // row 8*g + 4*(lane/32) + q, column lane%32
+  const int half = lane >> 5, fr = lane & 31;
+  if (EPI == 0) {
+    double rowsum[2][16];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int e = 0; e < 16; ++e) rowsum[i][e] = 0.0;
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+#pragma unroll
+      for (int e = 0; e < 16; ++e) {
+        const int rl = wm * 64 + i * 32 + 8 * (e >> 2) + 4 * half + (e & 3);
+        const int64_t r = r0 + rl;
+        if (r >= P.M) continue;
+        const int centre = P.c_rows[r];
+        double inv = 1.0;
+        if (ps.normalize) {
+          const double n2 = P.norm2[centre];
+          inv = n2 > 0 ? 1.0 / sqrt(n2) : 1.0;
+        }
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+          const int64_t c = c0 + wn * 64 + j * 32 + fr;
+          if (c >= P.N) continue;
+          const float k = float(double(acc[i][j][e]) * inv);
+          const double a = P.alpha[c];
+          rowsum[i][e] += a * double(pow_int(k, ps.zeta));
+          if (P.wth) {
+            const float g = ps.zeta > 1 ? float(ps.zeta) * pow_int(k, ps.zeta - 1) : 1.f;
+            const float v = float(a) * g;
+            const __bf16 h = (__bf16)v;
+            P.wth[r * P.N + c] = bf16_bits(h);
+            P.wtl[r * P.N + c] = bf16_bits((__bf16)(v - (float)h));
+          }
+        }
+      }
+    }
+    // reduce over the 32 columns held by the lanes of each half
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int e = 0; e < 16; ++e) {
+        double v = rowsum[i][e];
+        for (int off = 16; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+        rowsum[i][e] = v;
+      }
+    if (fr == 0) {
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int e = 0; e < 16; ++e)
+          esum[wn][wm * 64 + i * 32 + 8 * (e >> 2) + 4 * half + (e & 3)] = rowsum[i][e];
+    }
+    __syncthreads();
+    if (P.energy && tid < TBM) {
+      const int64_t r = r0 + tid;
+      if (r < P.M) atomicAdd(P.energy + P.c_rows[r], esum[0][tid] + esum[1][tid]);
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int e = 0; e < 16; ++e) {
+        const int64_t r = r0 + wm * 64 + i * 32 + 8 * (e >> 2) + 4 * half + (e & 3);
+        if (r >= P.M) continue;
+        float* g = P.G + int64_t(P.c_rows[r]) * P.ldg;
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+          const int64_t c = c0 + wn * 64 + j * 32 + fr;
+          if (c < P.N) g[c] = acc[i][j][e];
+        }
+      }
+  }
+}
+
 template <int EPI, bool FAST>
 __device__ __forceinline__ void gemm_tile(const ProblemSet& ps, const Problem& P, int64_t r0, int64_t c0,
                                           GemmLds& sm) {
@@ -114,7 +202,6 @@ __device__ __forceinline__ void gemm_tile(const ProblemSet& ps, const Problem& P
   auto& sAl = sm.Al;
   auto& sBh = sm.Bh;
   auto& sBl = sm.Bl;
-  auto& esum = sm.esum;
 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave >> 1, wn = wave & 1;
@@ -236,94 +323,151 @@ __device__ __forceinline__ void gemm_tile(const ProblemSet& ps, const Problem& P
     __syncthreads();
   }
 
-  // epilogue; element e = 4*g + q of a 32x32 tile sits at
-  // row 8*g + 4*(lane/32) + q, column lane%32
-  const int half = lane >> 5, fr = lane & 31;
-  if (EPI == 0) {
-    double rowsum[2][16];
+  gemm_epilogue<EPI, BM>(ps, P, r0, c0, acc, sm.esum);
+}
+
+// s_waitcnt vmcnt(N), the other counters left alone (gfx9 encoding)
+template <int N>
+__device__ __forceinline__ void wait_vmcnt() {
+  static_assert(N >= 0 && N < 64, "vmcnt range");
+  __builtin_amdgcn_s_waitcnt((N & 15) | ((N >> 4) << 14) | (7 << 4) | (15 << 8));
+}
+
+// Direct-to-LDS variant (global_load_lds_dwordx4: the loads write LDS
+// without passing through VGPRs).  A stage of one operand plane holds the
+// tile's rows of one K step (BK = 32 values, 64 B) as groups of 16 rows, one
+// wavefront load instruction (64 lanes x 16 B) per group; inside a group
+// row r keeps its K quarter q (8 values) at slot 4 r + (q ^ dma_swz(r)), so
+// that each instruction reads whole 64-byte row segments and the fragment
+// reads (16 rows of one quarter per 16 lanes) hit distinct banks.
+__device__ __forceinline__ int dma_swz(int r) { return (r ^ (r >> 2)) & 3; }
+// slot of the fragment of lane l for 32-row block rb, 16-wide K chunk c
+__device__ __forceinline__ int dma_frag_slot(int rb, int c, int l) {
+  const int r = l & 15;
+  return (2 * rb + ((l >> 4) & 1)) * 64 + 4 * r + ((2 * c + (l >> 5)) ^ dma_swz(r));
+}
+
+template <int TBM, int NS>
+struct DmaLds {
+  bf16x8 A[NS][2][TBM / 16 * 64];  // [stage][hi, lo][slot]
+  bf16x8 B[NS][2][BN / 16 * 64];
+  double esum[2][TBM];
+};
+
+// TBM x 128 block tile, TBM / 32 wavefronts, NS LDS stages with NS - 1 K
+// steps in flight; requires the fast layout (K % 32 == 0, aligned rows)
+template <int EPI, int TBM, int NS>
+__device__ __forceinline__ void gemm_tile_dma(const ProblemSet& ps, const Problem& P, int64_t r0,
+                                              int64_t c0, DmaLds<TBM, NS>& sm) {
+  constexpr int W = TBM / 32;
+  constexpr int GA = TBM / 16 / W, GB = BN / 16 / W;  // groups per plane and wavefront
+  static_assert(GA * W * 16 == TBM && GB * W * 16 == BN, "group partition");
+  constexpr int PER = 2 * (GA + GB);  // load instructions per wavefront and K step
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+
+  // lane of a group load: row lane / 4, quarter q; rows past the matrix
+  // edge repeat its last row (their results are never stored)
+  const int rr = lane >> 2, q = (lane & 3) ^ dma_swz(rr);
+  int64_t offa[GA], offb[GB];
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+  for (int j = 0; j < GA; ++j) {
+    int64_t ra = r0 + (wave + W * j) * 16 + rr;
+    ra = ra < P.M ? ra : P.M - 1;
+    if (P.a_rows) ra = P.a_rows[ra];
+    offa[j] = ra * P.lda + 8 * q;
+  }
 #pragma unroll
-      for (int e = 0; e < 16; ++e) rowsum[i][e] = 0.0;
+  for (int j = 0; j < GB; ++j) {
+    int64_t rb = c0 + (wave + W * j) * 16 + rr;
+    rb = rb < P.N ? rb : P.N - 1;
+    offb[j] = rb * P.ldb + 8 * q;
+  }
+  const int64_t nk = P.K / BK;
+  // loads of K step kt (clamped to the last one: the surplus loads at the
+  // end land in stages that are never read) into stage s
+  auto issue = [&](int s, int64_t kt) {
+    const int64_t k = (kt < nk ? kt : nk - 1) * BK;
 #pragma unroll
-    for (int i = 0; i < 2; ++i) {
-#pragma unroll
-      for (int e = 0; e < 16; ++e) {
-        const int rl = wm * 64 + i * 32 + 8 * (e >> 2) + 4 * half + (e & 3);
-        const int64_t r = r0 + rl;
-        if (r >= P.M) continue;
-        const int centre = P.c_rows[r];
-        double inv = 1.0;
-        if (ps.normalize) {
-          const double n2 = P.norm2[centre];
-          inv = n2 > 0 ? 1.0 / sqrt(n2) : 1.0;
-        }
-#pragma unroll
-        for (int j = 0; j < 2; ++j) {
-          const int64_t c = c0 + wn * 64 + j * 32 + fr;
-          if (c >= P.N) continue;
-          const float k = float(double(acc[i][j][e]) * inv);
-          const double a = P.alpha[c];
-          rowsum[i][e] += a * double(pow_int(k, ps.zeta));
-          if (P.wth) {
-            const float g = ps.zeta > 1 ? float(ps.zeta) * pow_int(k, ps.zeta - 1) : 1.f;
-            const float v = float(a) * g;
-            const __bf16 h = (__bf16)v;
-            P.wth[r * P.N + c] = bf16_bits(h);
-            P.wtl[r * P.N + c] = bf16_bits((__bf16)(v - (float)h));
-          }
-        }
-      }
+    for (int j = 0; j < GA; ++j) {
+      const int g = (wave + W * j) * 64;
+      __builtin_amdgcn_global_load_lds(P.Ahi + offa[j] + k, &sm.A[s][0][g], 16, 0, 0);
+      __builtin_amdgcn_global_load_lds(P.Alo + offa[j] + k, &sm.A[s][1][g], 16, 0, 0);
     }
-    // reduce over the 32 columns held by the lanes of each half
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+    for (int j = 0; j < GB; ++j) {
+      const int g = (wave + W * j) * 64;
+      __builtin_amdgcn_global_load_lds(P.Bhi + offb[j] + k, &sm.B[s][0][g], 16, 0, 0);
+      __builtin_amdgcn_global_load_lds(P.Blo + offb[j] + k, &sm.B[s][1][g], 16, 0, 0);
+    }
+  };
+
+  f32x16 acc[2][2];
 #pragma unroll
-      for (int e = 0; e < 16; ++e) {
-        double v = rowsum[i][e];
-        for (int off = 16; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-        rowsum[i][e] = v;
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int e = 0; e < 16; ++e) acc[i][j][e] = 0.f;
+
+  auto compute = [&](int s) {
+#pragma unroll
+    for (int c = 0; c < BK / 16; ++c) {
+      bf16x8 ah[2], al[2], bh[2], bl[2];
+#pragma unroll
+      for (int i = 0; i < 2; ++i) {
+        const int f = dma_frag_slot(wm * 2 + i, c, lane);
+        ah[i] = sm.A[s][0][f];
+        al[i] = sm.A[s][1][f];
       }
-    if (fr == 0) {
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        const int f = dma_frag_slot(wn * 2 + j, c, lane);
+        bh[j] = sm.B[s][0][f];
+        bl[j] = sm.B[s][1][f];
+      }
 #pragma unroll
       for (int i = 0; i < 2; ++i)
 #pragma unroll
-        for (int e = 0; e < 16; ++e)
-          esum[wn][wm * 64 + i * 32 + 8 * (e >> 2) + 4 * half + (e & 3)] = rowsum[i][e];
-    }
-    __syncthreads();
-    if (P.energy && tid < BM) {
-      const int64_t r = r0 + tid;
-      if (r < P.M) atomicAdd(P.energy + P.c_rows[r], esum[0][tid] + esum[1][tid]);
-    }
-  } else {
-#pragma unroll
-    for (int i = 0; i < 2; ++i)
-#pragma unroll
-      for (int e = 0; e < 16; ++e) {
-        const int64_t r = r0 + wm * 64 + i * 32 + 8 * (e >> 2) + 4 * half + (e & 3);
-        if (r >= P.M) continue;
-        float* g = P.G + int64_t(P.c_rows[r]) * P.ldg;
-#pragma unroll
         for (int j = 0; j < 2; ++j) {
-          const int64_t c = c0 + wn * 64 + j * 32 + fr;
-          if (c < P.N) g[c] = acc[i][j][e];
+          acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
+          acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
+          acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
         }
-      }
+    }
+  };
+
+#pragma unroll
+  for (int s = 0; s < NS - 1; ++s) issue(s, s);
+  int s_read = 0, s_fill = NS - 1;
+  for (int64_t kt = 0; kt < nk; ++kt) {
+    // this wavefront's loads of step kt are done once at most the NS - 2
+    // later steps' remain; the barrier extends that to every wavefront and
+    // orders the previous step's fragment reads before the refill of its
+    // stage
+    wait_vmcnt<(NS - 2) * PER>();
+    __syncthreads();
+    issue(s_fill, kt + NS - 1);
+    compute(s_read);
+    s_read = s_read + 1 == NS ? 0 : s_read + 1;
+    s_fill = s_fill + 1 == NS ? 0 : s_fill + 1;
   }
+  // no load may still write this workgroup's LDS when it exits
+  wait_vmcnt<0>();
+  gemm_epilogue<EPI, TBM>(ps, P, r0, c0, acc, sm.esum);
 }
 
-template <int EPI>  // 0: GAP kernel epilogue, 1: store rows of G
-__global__ __launch_bounds__(NT, 2) void bf16x3_gemm(ProblemSet ps) {
-  // problem and tile of this block.  Workgroups are dealt to the 8 XCDs
-  // round robin; each XCD takes a contiguous range of linear tiles, and the
-  // linear order walks groups of kGroupM row panels column by column, so
-  // the tiles resident on one XCD at a time (about 64) form a square of
-  // row and column panels that share their operand panels through the
-  // XCD's L2.
+// problem index and tile (row, column) of linear block bid.  Workgroups
+// are dealt to the 8 XCDs round robin; each XCD takes a contiguous range of
+// linear tiles, and the linear order walks groups of kGroupM row panels
+// column by column, so the tiles resident on one XCD at a time form a
+// square of row and column panels that share their operand panels through
+// the XCD's L2.
+__device__ __forceinline__ bool tile_of(const ProblemSet& ps, int* pi_out, int64_t* tm_out,
+                                        int64_t* tn_out) {
   const int64_t per_xcd = (int64_t(gridDim.x) + 7) / 8;
   const int64_t bid = int64_t(blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
-  if (bid >= ps.total_tiles) return;
+  if (bid >= ps.total_tiles) return false;
   int pi = 0;
   while (pi + 1 < ps.n && bid >= ps.p[pi + 1].tile0) ++pi;
   const Problem& P = ps.p[pi];
@@ -331,12 +475,35 @@ __global__ __launch_bounds__(NT, 2) void bf16x3_gemm(ProblemSet ps) {
   const int64_t per_group = int64_t(kGroupM) * P.tiles_n;
   const int64_t grp = local / per_group, in = local - grp * per_group;
   const int64_t gm = std::min<int64_t>(kGroupM, P.tiles_m - grp * kGroupM);
-  const int64_t tn = in / gm, tm = grp * kGroupM + (in - tn * gm);
+  const int64_t tn = in / gm;
+  *pi_out = pi;
+  *tm_out = grp * kGroupM + (in - tn * gm);
+  *tn_out = tn;
+  return true;
+}
+
+template <int EPI>  // 0: GAP kernel epilogue, 1: store rows of G
+__global__ __launch_bounds__(NT, 2) void bf16x3_gemm(ProblemSet ps) {
+  int pi;
+  int64_t tm, tn;
+  if (!tile_of(ps, &pi, &tm, &tn)) return;
+  const Problem& P = ps.p[pi];
   __shared__ __align__(16) GemmLds sm;
   if (P.fast)
     gemm_tile<EPI, true>(ps, P, tm * BM, tn * BN, sm);
   else
     gemm_tile<EPI, false>(ps, P, tm * BM, tn * BN, sm);
+}
+
+// every problem of the set in the fast layout
+template <int EPI, int TBM, int NS>
+__global__ __launch_bounds__(TBM * 2, 512 / (TBM * 2)) void bf16x3_gemm_dma(ProblemSet ps) {
+  int pi;
+  int64_t tm, tn;
+  if (!tile_of(ps, &pi, &tm, &tn)) return;
+  const Problem& P = ps.p[pi];
+  __shared__ __align__(16) DmaLds<TBM, NS> sm;
+  gemm_tile_dma<EPI, TBM, NS>(ps, P, tm * TBM, tn * BN, sm);
 }
 
 unsigned short bf16_rne_host(float x) {
@@ -359,15 +526,37 @@ void finish_problem(Problem& P) {
            aligned16(P.Alo) && aligned16(P.Bhi) && aligned16(P.Blo);
 }
 
-void launch(ProblemSet& ps, int64_t total_tiles, int epi, hipStream_t stream) {
-  if (total_tiles == 0) return;
-  ps.total_tiles = total_tiles;
-  const unsigned grid = unsigned((total_tiles + 7) / 8 * 8);  // whole rounds over the 8 XCDs
-  if (epi == 0)
-    hipLaunchKernelGGL(bf16x3_gemm<0>, dim3(grid), dim3(NT), 0, stream, ps);
+// tile counts and first tiles of the set's problems for block rows bm
+int64_t layout_tiles(ProblemSet& ps, int bm) {
+  int64_t tiles = 0;
+  for (int i = 0; i < ps.n; ++i) {
+    Problem& P = ps.p[i];
+    P.tile0 = tiles;
+    P.tiles_m = int((P.M + bm - 1) / bm);
+    P.tiles_n = int((P.N + BN - 1) / BN);
+    tiles += int64_t(P.tiles_m) * P.tiles_n;
+  }
+  return tiles;
+}
+
+template <int EPI>
+void launch_set(ProblemSet& ps, hipStream_t stream) {
+  if (ps.n == 0) return;
+  bool fast = true;
+  for (int i = 0; i < ps.n; ++i) fast = fast && ps.p[i].fast;
+  auto go = [&](auto kernel, int bm, int threads) {
+    ps.total_tiles = layout_tiles(ps, bm);
+    if (ps.total_tiles == 0) return;
+    const unsigned grid = unsigned((ps.total_tiles + 7) / 8 * 8);  // whole rounds over the 8 XCDs
+    hipLaunchKernelGGL(kernel, dim3(grid), dim3(threads), 0, stream, ps);
+    RGPU_LAUNCH_CHECK();
+  };
+  // 128 x 128 tiles either way; two stages of direct-to-LDS loads measured
+  // faster than three (with 256-row tiles) and than the register-staged loop
+  if (fast)
+    go(bf16x3_gemm_dma<EPI, BM, 2>, BM, NT);
   else
-    hipLaunchKernelGGL(bf16x3_gemm<1>, dim3(grid), dim3(NT), 0, stream, ps);
-  RGPU_LAUNCH_CHECK();
+    go(bf16x3_gemm<EPI>, BM, NT);
 }
 
 }  // namespace
@@ -432,7 +621,6 @@ void run_gap_mixed_kernel(const GapBlock<float>* blks, int n_blocks, const Mixed
                           SplitPlanes xs, double* energy, const SplitPlanes* wts, hipStream_t stream) {
   for (int b0 = 0; b0 < n_blocks; b0 += kMaxProblems) {
     ProblemSet ps{};
-    int64_t tiles = 0;
     for (int b = b0; b < std::min(n_blocks, b0 + kMaxProblems); ++b) {
       const GapBlock<float>& blk = blks[b];
       if (blk.n_rows == 0 || blk.n_cols == 0) continue;
@@ -448,9 +636,6 @@ void run_gap_mixed_kernel(const GapBlock<float>* blks, int n_blocks, const Mixed
       P.Blo = ms.t_lo.get() + blk.col0 * blk.D;
       P.ldb = blk.D;
       P.c_rows = blk.row_idx;
-      P.tile0 = tiles;
-      P.tiles_m = int((P.M + BM - 1) / BM);
-      P.tiles_n = int((P.N + BN - 1) / BN);
       P.norm2 = blk.norm2;
       if (!blk.alpha) throw InvalidArgument("the GAP energy kernel needs weights");
       P.alpha = blk.alpha + blk.col0;
@@ -458,11 +643,10 @@ void run_gap_mixed_kernel(const GapBlock<float>* blks, int n_blocks, const Mixed
       P.wth = wts ? wts[b].hi : nullptr;
       P.wtl = wts ? wts[b].lo : nullptr;
       finish_problem(P);
-      tiles += int64_t(P.tiles_m) * P.tiles_n;
       ps.normalize = blk.normalize;
       ps.zeta = blk.zeta;
     }
-    launch(ps, tiles, 0, stream);
+    launch_set<0>(ps, stream);
   }
 }
 
@@ -470,7 +654,6 @@ void run_gap_mixed_force(const GapBlock<float>* blks, int n_blocks, const MixedS
                          const SplitPlanes* wts, float* G, hipStream_t stream) {
   for (int b0 = 0; b0 < n_blocks; b0 += kMaxProblems) {
     ProblemSet ps{};
-    int64_t tiles = 0;
     for (int b = b0; b < std::min(n_blocks, b0 + kMaxProblems); ++b) {
       const GapBlock<float>& blk = blks[b];
       if (blk.n_rows == 0 || blk.n_cols == 0) continue;
@@ -486,15 +669,11 @@ void run_gap_mixed_force(const GapBlock<float>* blks, int n_blocks, const MixedS
       P.Blo = ms.tt_lo.get() + blk.col0;
       P.ldb = ms.M;
       P.c_rows = blk.row_idx;
-      P.tile0 = tiles;
-      P.tiles_m = int((P.M + BM - 1) / BM);
-      P.tiles_n = int((P.N + BN - 1) / BN);
       P.G = G;
       P.ldg = blk.D;
       finish_problem(P);
-      tiles += int64_t(P.tiles_m) * P.tiles_n;
     }
-    launch(ps, tiles, 1, stream);
+    launch_set<1>(ps, stream);
   }
 }
 
