@@ -412,7 +412,7 @@ __device__ void stage_pairs(const RadialDev& rad, const T* yc, const PairPre& f,
   const int rstride = kThreads - r0;
   if (!DIRECT) {
     if (tid >= r0) {
-      const FastDiv fF = fast_div(F);
+      const FastDiv fF = fast_div(F), fL1 = fast_div(L1);
       const T inv_h = T(1.0 / rad.h);
 #pragma unroll 4
       for (int it = tid - r0; it < nb * F; it += rstride) {
@@ -425,8 +425,11 @@ __device__ void stage_pairs(const RadialDev& rad, const T* yc, const PairPre& f,
         T v, dv;
         spline_eval<T>(rad, cell, f, &v, ADJ ? &dv : nullptr);
         if (ADJ) {
-          b.R[it] = v;
-          b.dR[it] = dv;
+          // the adjoint keeps R[q][l][n]: one lane's radial channels are
+          // contiguous
+          const int n = f / fL1, o = q * F + (f - n * L1) * N + n;
+          b.R[o] = v;
+          b.dR[o] = dv;
         } else {
           b.R[it] = v * T(b.fc[q]);
         }
@@ -454,8 +457,8 @@ __device__ void stage_pairs(const RadialDev& rad, const T* yc, const PairPre& f,
         if (ADJ) ds += o * b.draw[q * F + m * L1 + l];
       }
       if (ADJ) {
-        b.R[it] = T(s);
-        b.dR[it] = T(ds);
+        b.R[q * F + l * N + n] = T(s);
+        b.dR[q * F + l * N + n] = T(ds);
       } else {
         b.R[it] = T(s * b.fc[q]);
       }
@@ -713,8 +716,9 @@ __global__ void normalise_kernel(int64_t n, int64_t D, const T* __restrict__ p,
 }
 
 // Step 3 of the adjoint for centre i: per-pair contraction of dL/dc (in
-// LDS) with the pair gradients, force atomics and the centre virial.  The
-// pair batches are staged in `region`.  Each pair is contracted by a group
+// LDS, [species][lm][n]) with the pair gradients, force atomics and the
+// centre virial.  The pair batches are staged in `region` (radial functions
+// as R[pair][l][n]).  Each pair is contracted by a group
 // of 16, 32 or 64 lanes (reduced within the group), sized so that a batch
 // spreads over the workgroup; the per-pair FP64 tail (force, atomics,
 // virial) then runs one lane per pair on wavefront 0 (PB <= 64).
@@ -733,12 +737,19 @@ __device__ __forceinline__ void adjoint_pairs(const RadialDev& rad, int S, int P
   PairBatch<T> pb = carve_batch<T, DIRECT, true>(region, PB, LL, F);
   const int64_t p0 = pv.centre_off[i] + 1;
   const int np = int(pv.centre_off[i + 1] - p0);
+  // dL/dc rows W[lm][n] and the pairs' R[l][n], dR[l][n] are read four
+  // radial channels at a time where N allows
+  const bool vec4 = (N & 3) == 0;
+  typedef typename Vec4<T>::type V4;
   double fi[3] = {0, 0, 0}, v6[6] = {0, 0, 0, 0, 0, 0};
+  // the inputs of each batch are fetched while the previous one is
+  // contracted
+  PairPre pre;
+  pair_fetch<DIRECT, true>(rad, pv, p0, max(0, min(PB, np)), pre);
   for (int q0 = 0; q0 < np; q0 += PB) {
     const int nb = min(PB, np - q0);
-    PairPre pre;
-    pair_fetch<DIRECT, true>(rad, pv, p0 + q0, nb, pre);
     stage_pairs<T, DIRECT, true>(rad, yc, pre, nb, pb, nullptr, nullptr, nullptr);
+    if (q0 + PB < np) pair_fetch<DIRECT, true>(rad, pv, p0 + q0 + PB, min(PB, np - q0 - PB), pre);
     for (int qb = gpw * wave; qb < nb; qb += gpw * kWaves) {
       const int q = qb + grp;
       const bool act = q < nb;
@@ -752,11 +763,23 @@ __device__ __forceinline__ void adjoint_pairs(const RadialDev& rad, int S, int P
         const T fc = T(pb.fc[q]), dfc = T(pb.dfc[q]);
         for (int lm = gl; lm < LL; lm += GS) {
           const int l = l_of_lm(lm);
+          const T* w = W + lm * N;
+          const T* r = Rq + l * N;
+          const T* dr = dRq + l * N;
           T Dn = T(0), Ddn = T(0);
-          for (int n = 0; n < N; ++n) {
-            const T wv = W[n * LL + lm];
-            Dn += wv * Rq[n * L1 + l];
-            Ddn += wv * dRq[n * L1 + l];
+          if (vec4) {
+            for (int n = 0; n < N; n += 4) {
+              const V4 wv = *reinterpret_cast<const V4*>(w + n);
+              const V4 rv = *reinterpret_cast<const V4*>(r + n);
+              const V4 dv = *reinterpret_cast<const V4*>(dr + n);
+              Dn += wv[0] * rv[0] + wv[1] * rv[1] + wv[2] * rv[2] + wv[3] * rv[3];
+              Ddn += wv[0] * dv[0] + wv[1] * dv[1] + wv[2] * dv[2] + wv[3] * dv[3];
+            }
+          } else {
+            for (int n = 0; n < N; ++n) {
+              Dn += w[n] * r[n];
+              Ddn += w[n] * dr[n];
+            }
           }
           A += (Ddn * fc + Dn * dfc) * Yq[lm];
           Bx += Dn * dYq[lm];
@@ -860,7 +883,7 @@ __global__ __launch_bounds__(kThreads) void adjoint_kernel(
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   extern __shared__ __align__(64) unsigned char smem_raw[];
   T* yc = reinterpret_cast<T*>(smem_raw);
-  T* dc = reinterpret_cast<T*>(smem_raw + ylm_table_bytes<T>(L1));  // [S][N][LL]
+  T* dc = reinterpret_cast<T*>(smem_raw + ylm_table_bytes<T>(L1));  // dL/dc [S][LL][N]
   unsigned char* region = reinterpret_cast<unsigned char*>(dc) + align_up(sizeof(T) * E, 64);
   load_ylm_table<T>(rad, yc);
   const T* cs;
@@ -972,11 +995,11 @@ __global__ __launch_bounds__(kThreads) void adjoint_kernel(
         s4 += g4 * cv;
       }
       const T f = w / sqrt(T(2 * l + 1));
-      T* out = dc + (side_b ? b : a) * B + lm;
+      T* out = dc + (side_b ? b : a) * B + lm * N;
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
         const int n = 4 * nt + v;
-        if (n < N) out[n * LL] += f * s4[v];
+        if (n < N) out[n] += f * s4[v];
       }
     }
     __syncthreads();
@@ -1018,7 +1041,8 @@ __global__ __launch_bounds__(kThreads) void adjoint_staged_kernel(
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   extern __shared__ __align__(64) unsigned char smem_raw[];
   T* yc = reinterpret_cast<T*>(smem_raw);
-  T* cd = reinterpret_cast<T*>(smem_raw + ylm_table_bytes<T>(L1));  // c, then dL/dc
+  // c [S][N][LL], then dL/dc [S][LL][N]
+  T* cd = reinterpret_cast<T*>(smem_raw + ylm_table_bytes<T>(L1));
   unsigned char* region = reinterpret_cast<unsigned char*>(cd) + align_up(sizeof(T) * E, 64);
   T* gl = reinterpret_cast<T*>(region);
   __shared__ double red[kWaves];
@@ -1175,7 +1199,7 @@ __global__ __launch_bounds__(kThreads) void adjoint_staged_kernel(
       const T f = T(1) / sqrt(T(2 * l_of_lm(lm) + 1));
 #pragma unroll
       for (int v = 0; v < 4; ++v)
-        if (n0 + v < N) cd[a * B + (n0 + v) * LL + lm] = f * acc[k][v];
+        if (n0 + v < N) cd[a * B + lm * N + n0 + v] = f * acc[k][v];
     }
   }
   __syncthreads();
