@@ -79,6 +79,7 @@ struct Context {
 
   // scratch
   DeviceBuffer<double> e_centre, grad, cvir, tmp64, out64, out_e, knm_grad, knm_scale;
+  PinnedBuffer<double> out_stage;  // page-locked staging of a prediction's outputs
   DeviceBuffer<double> d_inv_vol;  // [n_struct] 1 / cell volume
   std::vector<double> inv_vol;     // host copy of the uploaded values
   DeviceBuffer<double> pair_fc, pair_dfc;  // cutoff function per pair
@@ -306,8 +307,25 @@ void build_tables(Context& c) {
   r.center = up(c.d_center, h.center_fin);
 }
 
-void require_batch(const Context& c) {
+// the batch is set; a neighbour list built with its size check deferred
+// may still be unchecked (only the prediction path works on such a list)
+void require_batch_unsettled(const Context& c) {
   if (!c.have_batch) throw InvalidArgument("no structures set (call rgpu_set_structures first)");
+}
+
+// Completes a deferred neighbour-list build; when a capacity was short, the
+// list is built again (synchronously) and the representation, computed from
+// a list without pairs, is invalidated.  Returns false in that case.
+bool settle_nl(Context& c) {
+  if (finish_pair_list(c.pl, c.stream)) return true;
+  rebuild_pair_list(c.pl, c.hp.cutoff, c.stream);
+  c.have_rep = false;
+  return false;
+}
+
+void require_batch(Context& c) {
+  require_batch_unsettled(c);
+  settle_nl(c);
 }
 
 double cell_volume(const double* C) {
@@ -315,7 +333,7 @@ double cell_volume(const double* C) {
                    C[2] * (C[3] * C[7] - C[4] * C[6]));
 }
 
-void build_batch_nl(Context& c, const double* positions) {
+void build_batch_nl(Context& c, const double* positions, bool defer = false) {
   StageTimer t(c, RGPU_STAGE_NEIGHBOURS);
   c.have_rep = false;
   if (c.n_atoms == 0) {
@@ -326,7 +344,7 @@ void build_batch_nl(Context& c, const double* positions) {
     return;
   }
   build_pair_list(c.pl, c.n_struct, c.atom_off.data(), positions, c.cells.data(), c.pbc.data(),
-                  c.hp.cutoff, c.stream);
+                  c.hp.cutoff, c.stream, defer);
   // cell volumes for the stress (uploaded when the cells change)
   std::vector<double> iv(static_cast<size_t>(c.n_struct));
   for (int s = 0; s < c.n_struct; ++s) iv[size_t(s)] = 1.0 / cell_volume(&c.cells[9 * s]);
@@ -448,8 +466,9 @@ void compute_rep(Context& c) {
   c.have_rep = true;
 }
 
+// (callers other than the prediction have settled the neighbour list)
 void compute(Context& c) {
-  require_batch(c);
+  require_batch_unsettled(c);
   if (c.precision == RGPU_FP64)
     compute_rep<double>(c);
   else
@@ -579,11 +598,19 @@ void emit_outputs(Context& c, const double* e_centre, double* energy, double* gr
     RGPU_HIP(hipMemsetAsync(st, 0, sizeof(double) * 6 * ns, c.stream));
     launch_segment_sum(cvir_d, 6, c.d_atom_off.get(), ns, st, c.stream, c.d_inv_vol.get());
   }
-  if (es) RGPU_HIP(hipMemcpyAsync(energy, es, sizeof(double) * ns, hipMemcpyDefault, c.stream));
-  if (gradient && grad_d)
-    RGPU_HIP(hipMemcpyAsync(gradient, grad_d, sizeof(double) * 3 * c.n_atoms, hipMemcpyDefault, c.stream));
-  if (st) RGPU_HIP(hipMemcpyAsync(neg_stress, st, sizeof(double) * 6 * ns, hipMemcpyDefault, c.stream));
+  // device to page-locked staging (asynchronous copies, one wait), then into
+  // the caller's host arrays
+  const size_t ne = es ? size_t(ns) : 0, ng = gradient && grad_d ? size_t(3 * c.n_atoms) : 0,
+               nst = st ? size_t(6 * ns) : 0;
+  double* h = c.out_stage.reserve(std::max<size_t>(ne + ng + nst, 1));
+  if (ne) RGPU_HIP(hipMemcpyAsync(h, es, sizeof(double) * ne, hipMemcpyDeviceToHost, c.stream));
+  if (ng) RGPU_HIP(hipMemcpyAsync(h + ne, grad_d, sizeof(double) * ng, hipMemcpyDeviceToHost, c.stream));
+  if (nst)
+    RGPU_HIP(hipMemcpyAsync(h + ne + ng, st, sizeof(double) * nst, hipMemcpyDeviceToHost, c.stream));
   RGPU_HIP(hipStreamSynchronize(c.stream));
+  if (ne) std::memcpy(energy, h, sizeof(double) * ne);
+  if (ng) std::memcpy(gradient, h + ne, sizeof(double) * ng);
+  if (nst) std::memcpy(neg_stress, h + ne + ng, sizeof(double) * nst);
 }
 
 template <class T>
@@ -867,6 +894,8 @@ void rgpu_context_destroy(rgpu_context* ctx) { delete ctx; }
 int rgpu_set_stream(rgpu_context* ctx, void* s) {
   return guarded([&] {
     if (!ctx) throw InvalidArgument("null context");
+    // a deferred build is checked on the stream it was enqueued on
+    if (ctx->c.have_batch) settle_nl(ctx->c);
     ctx->c.stream = static_cast<hipStream_t>(s);
   });
 }
@@ -938,7 +967,10 @@ int rgpu_get_info(rgpu_context* ctx, int key, int64_t* value) {
       case RGPU_INFO_N_SPECIES: *value = int64_t(c.species.size()); break;
       case RGPU_INFO_FEATURE_SIZE: *value = c.D(); break;
       case RGPU_INFO_N_CENTRES: *value = c.n_atoms; break;
-      case RGPU_INFO_N_PAIRS: *value = c.have_batch ? c.pl.n_pairs : 0; break;
+      case RGPU_INFO_N_PAIRS:
+        if (c.have_batch) settle_nl(c);
+        *value = c.have_batch ? c.pl.n_pairs : 0;
+        break;
       case RGPU_INFO_N_STRUCT: *value = c.n_struct; break;
       case RGPU_INFO_SPLINE_GRID: *value = c.rd.use_spline ? c.rd.grid_n : 0; break;
       case RGPU_INFO_N_SPARSE: *value = c.M; break;
@@ -1010,10 +1042,11 @@ int rgpu_update_positions(rgpu_context* ctx, const double* positions, const doub
   return guarded([&] {
     if (!ctx) throw InvalidArgument("null context");
     Context& c = ctx->c;
-    require_batch(c);
+    require_batch_unsettled(c);
     if (cells) c.cells.assign(cells, cells + 9 * c.n_struct);
-    build_batch_nl(c, positions);
-    RGPU_HIP(hipStreamSynchronize(c.stream));
+    // the size check of the list waits for the next call that needs it (a
+    // prediction checks after its own work, so the step has no extra wait)
+    build_batch_nl(c, positions, true);
   });
 }
 
@@ -1038,6 +1071,7 @@ int rgpu_get_neighbours(rgpu_context* ctx, int64_t* centre_offsets, int32_t* pai
 int rgpu_compute(rgpu_context* ctx) {
   return guarded([&] {
     if (!ctx) throw InvalidArgument("null context");
+    require_batch(ctx->c);
     compute(ctx->c);
     RGPU_HIP(hipStreamSynchronize(ctx->c.stream));
   });
@@ -1282,14 +1316,20 @@ int rgpu_gap_predict(rgpu_context* ctx, double* energy, double* gradient, double
   return guarded([&] {
     if (!ctx) throw InvalidArgument("null context");
     Context& c = ctx->c;
-    require_batch(c);
+    require_batch_unsettled(c);
     if (!c.have_sparse) throw InvalidArgument("no sparse points set");
     if (!c.have_weights) throw InvalidArgument("sparse points were set without weights");
-    ensure_rep(c);
-    if (c.precision == RGPU_FP64)
-      gap_predict_t<double>(c, energy, gradient, neg_stress);
-    else
-      gap_predict_t<float>(c, energy, gradient, neg_stress);
+    // runs on an unchecked neighbour list, then checks it: a list whose
+    // capacity was short held no pairs, so it is built again and the
+    // prediction repeated
+    for (int attempt = 0; attempt < 2; ++attempt) {
+      ensure_rep(c);
+      if (c.precision == RGPU_FP64)
+        gap_predict_t<double>(c, energy, gradient, neg_stress);
+      else
+        gap_predict_t<float>(c, energy, gradient, neg_stress);
+      if (settle_nl(c)) break;
+    }
   });
 }
 

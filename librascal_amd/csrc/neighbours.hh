@@ -50,15 +50,30 @@ struct PairList {
   PinnedBuffer<int64_t> tot_stage;
   std::vector<int> astruct_uploaded;
   std::vector<NLGeom> geom_uploaded;
+  // a deferred build whose sizes have not been read yet, and its capacities
+  bool pending = false;
+  int64_t pending_caps[2] = {0, 0};
 };
 
 // Host inputs: positions [n_atoms][3], cells [n_struct][9] (lattice vectors
 // as columns), pbc [n_struct][3], atom_offsets [n_struct+1].  Validates the
 // structures (cell not zero, atoms inside the cell within 1e-10) and builds
-// the list on `stream` (synchronises for sizes).
+// the list on `stream`.  Without defer it synchronises for the sizes (n_ext,
+// n_pairs).  With defer (and capacities kept from an earlier build of the
+// batch) it returns at once: device consumers may use the list straight
+// away, and finish_pair_list reads the sizes later; if a capacity was short,
+// the list then holds no pairs (every centre's range empty) until
+// rebuild_pair_list.
 void build_pair_list(PairList& pl, int n_struct, const int64_t* atom_offsets,
                      const double* positions, const double* cells, const int* pbc,
-                     double cutoff, hipStream_t stream);
+                     double cutoff, hipStream_t stream, bool defer = false);
+// Reads the sizes of a deferred build (waits for `stream`); false when a
+// capacity was short (the capacities are grown for rebuild_pair_list).
+// Throws if an atom fell outside the cell bins.  True when nothing is pending.
+bool finish_pair_list(PairList& pl, hipStream_t stream);
+// The device passes again over the positions and geometry of the last
+// build, with the current capacities; synchronous.
+void rebuild_pair_list(PairList& pl, double cutoff, hipStream_t stream);
 
 // Per-structure linked-cell geometry (host, FP64).
 NLGeom make_geometry(const double* cell, const int* pbc, double cutoff);

@@ -224,14 +224,21 @@ __global__ __launch_bounds__(256) void pairs_kernel(
   if (!WRITE && lane == 0) pair_count[i] = int(n);
 }
 
-__global__ void nl_totals_kernel(const int64_t* __restrict__ n_ghosts,
-                                 const int64_t* __restrict__ n_pairs, const int* __restrict__ err,
-                                 int64_t* __restrict__ out) {
+// Sizes of the build; when a capacity was short, every centre's pair range
+// is emptied (centre_off = 0) so that consumers launched before the host
+// sees the sizes read nothing beyond the buffers.
+__global__ void nl_totals_kernel(const int64_t* __restrict__ n_ghosts, int64_t* __restrict__ centre_off,
+                                 int64_t n_atoms, int64_t cap_ext, int64_t cap_pairs,
+                                 const int* __restrict__ err, int64_t* __restrict__ out) {
+  const int64_t ng = *n_ghosts, np = centre_off[n_atoms];
+  __syncthreads();  // every thread has read the pair total before it may be cleared
   if (threadIdx.x == 0) {
-    out[0] = *n_ghosts;
-    out[1] = *n_pairs;
+    out[0] = ng;
+    out[1] = np;
     out[2] = *err;
   }
+  if (n_atoms + ng > cap_ext || np > cap_pairs)
+    for (int64_t a = threadIdx.x; a <= n_atoms; a += blockDim.x) centre_off[a] = 0;
 }
 
 }  // namespace
@@ -308,67 +315,33 @@ void check_structure(int n_atoms, const double* pos, const double* cell) {
     }
 }
 
-void build_pair_list(PairList& pl, int n_struct, const int64_t* atom_offsets,
-                     const double* positions, const double* cells, const int* pbc, double rc,
-                     hipStream_t stream) {
-  const int64_t n_atoms = atom_offsets[n_struct];
-  pl.n_struct = n_struct;
-  pl.n_atoms = n_atoms;
-  pl.geom_host.resize(n_struct);
-  std::vector<int> astruct(size_t(n_atoms > 0 ? n_atoms : 1));
-  int64_t boxes = 0;
-  for (int s = 0; s < n_struct; ++s) {
-    int64_t a0 = atom_offsets[s], a1 = atom_offsets[s + 1];
-    if (a1 < a0) throw InvalidArgument("atom_offsets must be non-decreasing");
-    check_structure(int(a1 - a0), positions + 3 * a0, cells + 9 * s);
-    NLGeom g = make_geometry(cells + 9 * s, pbc + 3 * s, rc);
-    g.atom0 = int(a0);
-    g.n_atoms = int(a1 - a0);
-    g.box0 = boxes;
-    boxes += int64_t(g.nboxes[0]) * g.nboxes[1] * g.nboxes[2];
-    pl.geom_host[s] = g;
-    for (int64_t a = a0; a < a1; ++a) astruct[a] = s;
-  }
-  pl.n_boxes = boxes;
-  if (n_atoms == 0) {
-    pl.n_pairs = 0;
-    pl.n_ext = 0;
-    pl.centre_off.reserve(1);
-    RGPU_HIP(hipMemsetAsync(pl.centre_off.get(), 0, sizeof(int64_t), stream));
-    return;
-  }
-  // geometry and structure map: uploaded again only when they change (the
-  // buffers keep their addresses while they do not grow)
-  const NLGeom* geom_before = pl.geom.get();
-  NLGeom* dgeom = pl.geom.reserve(n_struct);
-  if (dgeom != geom_before || pl.geom_uploaded.size() != pl.geom_host.size() ||
-      std::memcmp(pl.geom_uploaded.data(), pl.geom_host.data(), sizeof(NLGeom) * n_struct) != 0) {
-    RGPU_HIP(hipMemcpyAsync(dgeom, pl.geom_host.data(), sizeof(NLGeom) * n_struct,
-                            hipMemcpyHostToDevice, stream));
-    pl.geom_uploaded = pl.geom_host;
-  }
-  const int* struct_before = pl.atom_struct.get();
-  int* dstruct = pl.atom_struct.reserve(n_atoms);
-  astruct.resize(size_t(n_atoms));
-  if (dstruct != struct_before || pl.astruct_uploaded != astruct) {
-    RGPU_HIP(hipMemcpyAsync(dstruct, astruct.data(), sizeof(int) * n_atoms, hipMemcpyHostToDevice,
-                            stream));
-    pl.astruct_uploaded = astruct;
-  }
-  // positions through page-locked staging (the previous build synchronised,
-  // so the staging buffer is free)
-  double* dpos = pl.pos.reserve(3 * n_atoms);
-  double* ps = pl.pos_stage.reserve(size_t(3 * n_atoms));
-  std::memcpy(ps, positions, sizeof(double) * 3 * n_atoms);
-  RGPU_HIP(hipMemcpyAsync(dpos, ps, sizeof(double) * 3 * n_atoms, hipMemcpyHostToDevice, stream));
+namespace {
 
-  // Sizes of the extended (ghost) set and of the pair list are only known on
-  // the device; the buffers are sized by capacities kept from earlier builds,
-  // every write beyond a capacity is dropped, and one synchronisation at the
-  // end reads the true sizes.  When a capacity was short, the build is run
-  // again with enough room (the first build of a batch, or a large change).
-  if (pl.cap_ext < n_atoms) pl.cap_ext = 2 * n_atoms + 1024;
-  if (pl.cap_pairs < n_atoms) pl.cap_pairs = 48 * n_atoms + 1024;
+// true when the sizes read back fit the capacities (then recorded); else
+// the capacities are grown for the next pass
+bool settle(PairList& pl, int64_t cap_ext, int64_t cap_pairs) {
+  const int64_t* tot = pl.tot_stage.get();
+  const int64_t n_ext = pl.n_atoms + tot[0], n_pairs = tot[1];
+  if (n_ext <= cap_ext && n_pairs <= cap_pairs) {
+    if (tot[2])
+      throw RuntimeError(
+          "Error: this atom does not fall in one of the linked cell bin. It might have not been "
+          "wrapped properly.");
+    pl.n_ext = n_ext;
+    pl.n_pairs = n_pairs;
+    return true;
+  }
+  pl.cap_ext = std::max(cap_ext, n_ext + n_ext / 8 + 64);
+  pl.cap_pairs = std::max(cap_pairs, n_pairs + n_pairs / 8 + 64);
+  return false;
+}
+
+// the device passes over the uploaded positions and geometry
+void run_passes(PairList& pl, double rc, hipStream_t stream, bool defer) {
+  const int64_t n_atoms = pl.n_atoms, boxes = pl.n_boxes;
+  const NLGeom* dgeom = pl.geom.get();
+  const int* dstruct = pl.atom_struct.get();
+  const double* dpos = pl.pos.get();
   const int TB = 256;
   auto blocks = [](int64_t n, int tb) { return dim3(unsigned((n + tb - 1) / tb)); };
   // a short ghost capacity also hides pairs, so up to three passes
@@ -425,26 +398,103 @@ void build_pair_list(PairList& pl, int n_struct, const int64_t* atom_offsets,
                        bstart, bitems, n_atoms, rc, nullptr, coff, pj, pv, pd, cap_pairs);
     RGPU_LAUNCH_CHECK();
     int64_t* dtot = pl.totals.reserve(3);
-    hipLaunchKernelGGL(nl_totals_kernel, dim3(1), dim3(64), 0, stream, goff + n_atoms, coff + n_atoms,
-                       err, dtot);
+    hipLaunchKernelGGL(nl_totals_kernel, dim3(1), dim3(256), 0, stream, goff + n_atoms, coff, n_atoms,
+                       cap_ext, cap_pairs, err, dtot);
     RGPU_LAUNCH_CHECK();
     int64_t* tot = pl.tot_stage.reserve(3);
     RGPU_HIP(hipMemcpyAsync(tot, dtot, sizeof(int64_t) * 3, hipMemcpyDeviceToHost, stream));
-    RGPU_HIP(hipStreamSynchronize(stream));
-    const int64_t n_ext = n_atoms + tot[0], n_pairs = tot[1];
-    if (n_ext <= cap_ext && n_pairs <= cap_pairs) {
-      if (tot[2])
-        throw RuntimeError(
-            "Error: this atom does not fall in one of the linked cell bin. It might have not been "
-            "wrapped properly.");
-      pl.n_ext = n_ext;
-      pl.n_pairs = n_pairs;
+    if (defer) {
+      pl.pending = true;
+      pl.pending_caps[0] = cap_ext;
+      pl.pending_caps[1] = cap_pairs;
       return;
     }
-    pl.cap_ext = std::max(cap_ext, n_ext + n_ext / 8 + 64);
-    pl.cap_pairs = std::max(cap_pairs, n_pairs + n_pairs / 8 + 64);
+    RGPU_HIP(hipStreamSynchronize(stream));
+    if (settle(pl, cap_ext, cap_pairs)) return;
   }
   throw RuntimeError("neighbour list buffers could not be sized");
+}
+
+}  // namespace
+
+bool finish_pair_list(PairList& pl, hipStream_t stream) {
+  if (!pl.pending) return true;
+  RGPU_HIP(hipStreamSynchronize(stream));
+  pl.pending = false;
+  return settle(pl, pl.pending_caps[0], pl.pending_caps[1]);
+}
+
+void rebuild_pair_list(PairList& pl, double rc, hipStream_t stream) {
+  if (pl.n_atoms == 0) return;
+  run_passes(pl, rc, stream, false);
+}
+
+void build_pair_list(PairList& pl, int n_struct, const int64_t* atom_offsets,
+                     const double* positions, const double* cells, const int* pbc, double rc,
+                     hipStream_t stream, bool defer) {
+  // an unchecked earlier build still owns the staging buffers
+  if (pl.pending) (void)finish_pair_list(pl, stream);
+  const int64_t n_atoms = atom_offsets[n_struct];
+  pl.n_struct = n_struct;
+  pl.n_atoms = n_atoms;
+  pl.geom_host.resize(n_struct);
+  std::vector<int> astruct(size_t(n_atoms > 0 ? n_atoms : 1));
+  int64_t boxes = 0;
+  for (int s = 0; s < n_struct; ++s) {
+    int64_t a0 = atom_offsets[s], a1 = atom_offsets[s + 1];
+    if (a1 < a0) throw InvalidArgument("atom_offsets must be non-decreasing");
+    check_structure(int(a1 - a0), positions + 3 * a0, cells + 9 * s);
+    NLGeom g = make_geometry(cells + 9 * s, pbc + 3 * s, rc);
+    g.atom0 = int(a0);
+    g.n_atoms = int(a1 - a0);
+    g.box0 = boxes;
+    boxes += int64_t(g.nboxes[0]) * g.nboxes[1] * g.nboxes[2];
+    pl.geom_host[s] = g;
+    for (int64_t a = a0; a < a1; ++a) astruct[a] = s;
+  }
+  pl.n_boxes = boxes;
+  if (n_atoms == 0) {
+    pl.n_pairs = 0;
+    pl.n_ext = 0;
+    pl.centre_off.reserve(1);
+    RGPU_HIP(hipMemsetAsync(pl.centre_off.get(), 0, sizeof(int64_t), stream));
+    return;
+  }
+  // geometry and structure map: uploaded again only when they change (the
+  // buffers keep their addresses while they do not grow)
+  const NLGeom* geom_before = pl.geom.get();
+  NLGeom* dgeom = pl.geom.reserve(n_struct);
+  if (dgeom != geom_before || pl.geom_uploaded.size() != pl.geom_host.size() ||
+      std::memcmp(pl.geom_uploaded.data(), pl.geom_host.data(), sizeof(NLGeom) * n_struct) != 0) {
+    RGPU_HIP(hipMemcpyAsync(dgeom, pl.geom_host.data(), sizeof(NLGeom) * n_struct,
+                            hipMemcpyHostToDevice, stream));
+    pl.geom_uploaded = pl.geom_host;
+  }
+  const int* struct_before = pl.atom_struct.get();
+  int* dstruct = pl.atom_struct.reserve(n_atoms);
+  astruct.resize(size_t(n_atoms));
+  if (dstruct != struct_before || pl.astruct_uploaded != astruct) {
+    RGPU_HIP(hipMemcpyAsync(dstruct, astruct.data(), sizeof(int) * n_atoms, hipMemcpyHostToDevice,
+                            stream));
+    pl.astruct_uploaded = astruct;
+  }
+  // positions through page-locked staging (free: every earlier build has
+  // been waited for, the last one just above if it was deferred)
+  double* dpos = pl.pos.reserve(3 * n_atoms);
+  double* ps = pl.pos_stage.reserve(size_t(3 * n_atoms));
+  std::memcpy(ps, positions, sizeof(double) * 3 * n_atoms);
+  RGPU_HIP(hipMemcpyAsync(dpos, ps, sizeof(double) * 3 * n_atoms, hipMemcpyHostToDevice, stream));
+
+  // Sizes of the extended (ghost) set and of the pair list are only known on
+  // the device; the buffers are sized by capacities kept from earlier builds,
+  // every write beyond a capacity is dropped, and the true sizes are read
+  // back: at once, or (defer) by finish_pair_list after the consumers of the
+  // list have been enqueued.  When a capacity was short, the build is run
+  // again with enough room (the first build of a batch, or a large change).
+  if (pl.cap_ext < n_atoms) pl.cap_ext = 2 * n_atoms + 1024;
+  if (pl.cap_pairs < n_atoms) pl.cap_pairs = 48 * n_atoms + 1024;
+  // the first build of a batch sizes the buffers synchronously
+  run_passes(pl, rc, stream, defer && pl.n_pairs > 0);
 }
 
 }  // namespace rgpu

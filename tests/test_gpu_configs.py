@@ -280,3 +280,51 @@ def test_mixed_prediction_adjoint_repeatable(normalize):
     for _ in range(4):
         g = ctx.gap_predict()[1]
         assert np.abs(g - ref).max() < 1e-5 * scale
+
+
+def test_deferred_neighbour_check_rebuilds_a_short_list():
+    """update_positions leaves the size check of the neighbour list to the
+    prediction, which checks after its own work.  Compressing a dilute box
+    (about 10 neighbours per atom within rc 5) to about 63 per atom exceeds
+    the pair capacity kept from the first build (48 per atom + 1024): the
+    prediction has to build the list again and repeat itself, and then
+    matches a context built on the dense box directly (FP64).  Two updates
+    without a prediction between them are checked the same way."""
+    n = 512
+    dense = periodic_box(n, 0.12, (1, 8), d_min=1.0, seed=7)
+    hp = soap_hypers(rc=5.0, sw=0.5, sigma=0.5, N=4, L=3)
+    off, pos, num, cells, pbc = pack(dense)
+    sp = sorted(set(num.tolist()))
+    s = (0.12 / 0.02) ** (1.0 / 3.0)
+    ctx = _capi.Context(json.dumps(hp))
+    ctx.set_structures(off, pos * s, num, cells * s, pbc, sp)
+    n_dilute = ctx.info("n_pairs")
+    ref = _capi.Context(json.dumps(hp))
+    ref.set_structures(off, pos, num, cells, pbc, sp)
+    n_dense = ref.info("n_pairs")
+    assert n_dilute < 48 * n + 1024 < n_dense
+    X, _ = ref.features()
+    rng = np.random.default_rng(1)
+    idx = rng.choice(n, 96, replace=False)
+    spec = num[idx].astype(np.int32)
+    w = rng.normal(size=len(idx))
+    for c in (ctx, ref):
+        c.set_sparse_points(2, spec, X[idx], w)
+    e0, g0, s0 = ref.gap_predict()
+
+    def same(out):
+        e, g, st = out
+        np.testing.assert_allclose(e, e0, rtol=1e-12)
+        np.testing.assert_allclose(g, g0, rtol=0, atol=1e-12 * np.abs(g0).max())
+        np.testing.assert_allclose(st, s0, rtol=0, atol=1e-12 * np.abs(s0).max())
+
+    ctx.update_positions(pos, cells)
+    same(ctx.gap_predict())
+    assert ctx.info("n_pairs") == n_dense
+    ctx.update_positions(pos * s, cells * s)
+    ctx.update_positions(pos, cells)
+    same(ctx.gap_predict())
+    # the dilute box again: fits, no rebuild
+    ctx.update_positions(pos * s, cells * s)
+    ctx.gap_predict()
+    assert ctx.info("n_pairs") == n_dilute
