@@ -57,6 +57,7 @@ extern "C" {
 #define RGPU_INFO_TILE_KEYS_PS 14   /* autotuned key blocks per pass, power spectrum */
 #define RGPU_INFO_TILE_PAIRS_ADJ 15 /* autotuned pairs per batch, adjoint */
 #define RGPU_INFO_TILE_KEYS_ADJ 16  /* key blocks per pass of the FP32 adjoint (0 = auto) */
+#define RGPU_INFO_NL_GRAPH_LAUNCHES 17 /* neighbour-list passes replayed from a graph */
 
 /* stages timed by rgpu_enable_timing */
 #define RGPU_STAGE_NEIGHBOURS 0
@@ -84,9 +85,10 @@ void rgpu_context_destroy(rgpu_context* ctx);
 int rgpu_set_stream(rgpu_context* ctx, void* hip_stream);
 /* Replay the representation stage (cutoff, expansion, invariants) of
  * rgpu_compute as a captured HIP graph while the batch's buffers and sizes
- * are unchanged (positions may change).  Creates a private stream when no
- * stream was set.  Not used while stage timing is on.  No counterpart in
- * the reference (a CPU library). */
+ * are unchanged (positions may change; not used while stage timing is on),
+ * and the device pass of the neighbour list built by rgpu_update_positions
+ * likewise.  Creates a private stream when no stream was set.  No
+ * counterpart in the reference (a CPU library). */
 int rgpu_enable_graphs(rgpu_context* ctx, int enable);
 /* Autotuner: times the candidate work partitions (pairs staged per batch in
  * the expansion and the adjoint, key blocks per pass in the power spectrum

@@ -25,7 +25,7 @@ STAGES = dict(neighbours=0, expansion=1, power_spectrum=2, gap_kernel=3, gap_for
 INFO = dict(max_radial=0, max_angular=1, n_species=2, feature_size=3, n_centres=4,
             n_pairs=5, n_struct=6, spline_grid=7, n_sparse=8, is_invariants=9, precision=10,
             graph_captures=11, graph_launches=12, tile_pairs_fwd=13, tile_keys_ps=14,
-            tile_pairs_adj=15, tile_keys_adj=16)
+            tile_pairs_adj=15, tile_keys_adj=16, nl_graph_launches=17)
 
 _lib = None
 

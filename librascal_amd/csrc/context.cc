@@ -78,7 +78,7 @@ struct Context {
   DeviceBuffer<int> d_rows;
 
   // scratch
-  DeviceBuffer<double> e_centre, grad, cvir, tmp64, out64, out_e, knm_grad, knm_scale;
+  DeviceBuffer<double> e_centre, grad, cvir, tmp64, out64, knm_grad, knm_scale;
   PinnedBuffer<double> out_stage;  // page-locked staging of a prediction's outputs
   DeviceBuffer<double> d_inv_vol;  // [n_struct] 1 / cell volume
   std::vector<double> inv_vol;     // host copy of the uploaded values
@@ -344,7 +344,7 @@ void build_batch_nl(Context& c, const double* positions, bool defer = false) {
     return;
   }
   build_pair_list(c.pl, c.n_struct, c.atom_off.data(), positions, c.cells.data(), c.pbc.data(),
-                  c.hp.cutoff, c.stream, defer);
+                  c.hp.cutoff, c.stream, defer, c.use_graphs);
   // cell volumes for the stress (uploaded when the cells change)
   std::vector<double> iv(static_cast<size_t>(c.n_struct));
   for (int s = 0; s < c.n_struct; ++s) iv[size_t(s)] = 1.0 / cell_volume(&c.cells[9 * s]);
@@ -588,25 +588,17 @@ const float* invariants_of<float>(Context& c) {
 void emit_outputs(Context& c, const double* e_centre, double* energy, double* grad_d,
                   double* cvir_d, double* gradient, double* neg_stress) {
   const int ns = c.n_struct;
-  double* es = energy && e_centre ? c.out_e.reserve(size_t(std::max(ns, 1))) : nullptr;
-  double* st = neg_stress ? c.out64.reserve(size_t(6 * std::max(ns, 1))) : nullptr;
-  if (es) {
-    RGPU_HIP(hipMemsetAsync(es, 0, sizeof(double) * ns, c.stream));
-    launch_segment_sum(e_centre, 1, c.d_atom_off.get(), ns, es, c.stream);
-  }
-  if (st) {
-    RGPU_HIP(hipMemsetAsync(st, 0, sizeof(double) * 6 * ns, c.stream));
-    launch_segment_sum(cvir_d, 6, c.d_atom_off.get(), ns, st, c.stream, c.d_inv_vol.get());
-  }
-  // device to page-locked staging (asynchronous copies, one wait), then into
-  // the caller's host arrays
-  const size_t ne = es ? size_t(ns) : 0, ng = gradient && grad_d ? size_t(3 * c.n_atoms) : 0,
-               nst = st ? size_t(6 * ns) : 0;
+  // the reductions and the gradient copy write straight into page-locked
+  // staging ([energy | gradient | stress], zeroed here: the host owns it
+  // between predictions), one wait, then the caller's host arrays
+  const size_t ne = energy && e_centre ? size_t(ns) : 0,
+               ng = gradient && grad_d ? size_t(3 * c.n_atoms) : 0, nst = neg_stress ? size_t(6 * ns) : 0;
   double* h = c.out_stage.reserve(std::max<size_t>(ne + ng + nst, 1));
-  if (ne) RGPU_HIP(hipMemcpyAsync(h, es, sizeof(double) * ne, hipMemcpyDeviceToHost, c.stream));
-  if (ng) RGPU_HIP(hipMemcpyAsync(h + ne, grad_d, sizeof(double) * ng, hipMemcpyDeviceToHost, c.stream));
-  if (nst)
-    RGPU_HIP(hipMemcpyAsync(h + ne + ng, st, sizeof(double) * nst, hipMemcpyDeviceToHost, c.stream));
+  double* hd = c.out_stage.device();
+  std::memset(h, 0, sizeof(double) * (ne + ng + nst));
+  if (ne) launch_segment_sum(e_centre, 1, c.d_atom_off.get(), ns, hd, c.stream);
+  if (ng) launch_strided_copy(grad_d, int64_t(ng), hd + ne, 1, c.stream);
+  if (nst) launch_segment_sum(cvir_d, 6, c.d_atom_off.get(), ns, hd + ne + ng, c.stream, c.d_inv_vol.get());
   RGPU_HIP(hipStreamSynchronize(c.stream));
   if (ne) std::memcpy(energy, h, sizeof(double) * ne);
   if (ng) std::memcpy(gradient, h + ne, sizeof(double) * ng);
@@ -978,6 +970,7 @@ int rgpu_get_info(rgpu_context* ctx, int key, int64_t* value) {
       case RGPU_INFO_PRECISION: *value = c.precision; break;
       case RGPU_INFO_GRAPH_CAPTURES: *value = c.graph_captures; break;
       case RGPU_INFO_GRAPH_LAUNCHES: *value = c.graph_launches; break;
+      case RGPU_INFO_NL_GRAPH_LAUNCHES: *value = c.pl.graph_launches; break;
       case RGPU_INFO_TILE_PAIRS_FWD: *value = c.tile().pb_fwd; break;
       case RGPU_INFO_TILE_KEYS_PS: *value = c.tile().kc; break;
       case RGPU_INFO_TILE_PAIRS_ADJ: *value = c.tile().pb_adj; break;

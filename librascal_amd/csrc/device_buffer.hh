@@ -64,12 +64,21 @@ class PinnedBuffer {
   void release() {
     if (ptr_) (void)hipHostFree(ptr_);
     ptr_ = nullptr;
+    dev_ = nullptr;
     cap_ = 0;
   }
   T* get() const { return ptr_; }
+  // The device address of the buffer: page-locked memory is mapped into the
+  // device's address space, so kernels read and write it directly (no copy
+  // engine, no wait between engines).
+  T* device() {
+    if (ptr_ && !dev_) RGPU_HIP(hipHostGetDevicePointer(reinterpret_cast<void**>(&dev_), ptr_, 0));
+    return dev_;
+  }
 
  private:
   T* ptr_ = nullptr;
+  T* dev_ = nullptr;
   size_t cap_ = 0;
 };
 

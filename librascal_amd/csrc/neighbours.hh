@@ -9,6 +9,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cstdint>
 #include <vector>
 
 #include "common.hh"
@@ -42,17 +43,29 @@ struct PairList {
   DeviceBuffer<double> pair_dist;    // [n_pairs]
   DeviceBuffer<int> err;             // [1]
   DeviceBuffer<int64_t> scratch, scratch2, scratch3;  // scan tile sums (one per scan)
-  DeviceBuffer<int64_t> totals;      // [3] ghosts, pairs, error flag
   int64_t cap_ext = 0, cap_pairs = 0;  // allocated sizes of the extended set / pair list
-  // host staging: positions and totals (page-locked), and the last uploaded
+  // host staging: positions and totals [ghosts, pairs, error flag]
+  // (page-locked, read and written by kernels), and the last uploaded
   // structure map and geometry (not uploaded again while unchanged)
   PinnedBuffer<double> pos_stage;
   PinnedBuffer<int64_t> tot_stage;
-  std::vector<int> astruct_uploaded;
+  std::vector<int> astruct_host;           // source of the last structure-map upload
+  std::vector<int64_t> offsets_uploaded;   // the atom offsets it was made from
   std::vector<NLGeom> geom_uploaded;
   // a deferred build whose sizes have not been read yet, and its capacities
   bool pending = false;
   int64_t pending_caps[2] = {0, 0};
+  // HIP graph of one pass of the device build, replayed by deferred builds
+  // while every size, capacity and buffer it was captured with is unchanged
+  hipGraphExec_t graph = nullptr;
+  std::vector<uintptr_t> graph_sig;
+  int64_t graph_captures = 0, graph_launches = 0;
+  PairList() = default;
+  PairList(const PairList&) = delete;
+  PairList& operator=(const PairList&) = delete;
+  ~PairList() {
+    if (graph) (void)hipGraphExecDestroy(graph);
+  }
 };
 
 // Host inputs: positions [n_atoms][3], cells [n_struct][9] (lattice vectors
@@ -64,9 +77,11 @@ struct PairList {
 // away, and finish_pair_list reads the sizes later; if a capacity was short,
 // the list then holds no pairs (every centre's range empty) until
 // rebuild_pair_list.
+// With graph (a deferred build on a stream other than the null stream), the
+// device pass is replayed from a captured HIP graph.
 void build_pair_list(PairList& pl, int n_struct, const int64_t* atom_offsets,
                      const double* positions, const double* cells, const int* pbc,
-                     double cutoff, hipStream_t stream, bool defer = false);
+                     double cutoff, hipStream_t stream, bool defer = false, bool graph = false);
 // Reads the sizes of a deferred build (waits for `stream`); false when a
 // capacity was short (the capacities are grown for rebuild_pair_list).
 // Throws if an atom fell outside the cell bins.  True when nothing is pending.

@@ -224,6 +224,11 @@ __global__ __launch_bounds__(256) void pairs_kernel(
   if (!WRITE && lane == 0) pair_count[i] = int(n);
 }
 
+__global__ void copy_f64_kernel(const double* __restrict__ src, int64_t n, double* __restrict__ dst) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = src[i];
+}
+
 // Sizes of the build; when a capacity was short, every centre's pair range
 // is emptied (centre_off = 0) so that consumers launched before the host
 // sees the sizes read nothing beyond the buffers.
@@ -305,14 +310,19 @@ void check_structure(int n_atoms, const double* pos, const double* cell) {
         "even if the structure is not periodic.");
   double Ci[9];
   inverse3(cell, Ci);
-  for (int a = 0; a < n_atoms; ++a)
+  // branch-free over the atoms (vectorisable), one test at the end
+  bool outside = false;
+  for (int a = 0; a < n_atoms; ++a) {
+    const double x = pos[3 * a], y = pos[3 * a + 1], z = pos[3 * a + 2];
     for (int d = 0; d < 3; ++d) {
-      double s = Ci[d * 3] * pos[3 * a] + Ci[d * 3 + 1] * pos[3 * a + 1] + Ci[d * 3 + 2] * pos[3 * a + 2];
-      if (s < -1e-10 || s > 1. + 1e-10)
-        throw RuntimeError(
-            "Some of the atoms in the structure are not inside the unit cell. Please wrap them "
-            "inside it with at least tolerance of 1e-10.");
+      const double s = Ci[d * 3] * x + Ci[d * 3 + 1] * y + Ci[d * 3 + 2] * z;
+      outside |= (s < -1e-10) | (s > 1. + 1e-10);
     }
+  }
+  if (outside)
+    throw RuntimeError(
+        "Some of the atoms in the structure are not inside the unit cell. Please wrap them "
+        "inside it with at least tolerance of 1e-10.");
 }
 
 namespace {
@@ -336,73 +346,153 @@ bool settle(PairList& pl, int64_t cap_ext, int64_t cap_pairs) {
   return false;
 }
 
-// the device passes over the uploaded positions and geometry
-void run_passes(PairList& pl, double rc, hipStream_t stream, bool defer) {
+__global__ void zero_i32_kernel(int* __restrict__ p, int64_t n) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
+// every buffer of one device pass at the current capacities (reserved
+// before a capture, so that nothing is allocated inside it)
+void reserve_pass(PairList& pl) {
   const int64_t n_atoms = pl.n_atoms, boxes = pl.n_boxes;
+  pl.pos.reserve(3 * n_atoms);
+  pl.err.reserve(1);
+  pl.ghost_count.reserve(n_atoms);
+  pl.ghost_off.reserve(n_atoms + 1);
+  pl.ext_pos.reserve(3 * pl.cap_ext);
+  pl.ext_orig.reserve(pl.cap_ext);
+  pl.box_count.reserve(boxes);
+  pl.box_start.reserve(boxes + 1);
+  pl.box_fill.reserve(boxes);
+  pl.box_items.reserve(pl.cap_ext);
+  pl.ext_box.reserve(pl.cap_ext);
+  pl.pair_count.reserve(n_atoms);
+  pl.centre_off.reserve(n_atoms + 1);
+  pl.pair_j.reserve(pl.cap_pairs);
+  pl.pair_vec.reserve(3 * pl.cap_pairs);
+  pl.pair_dist.reserve(pl.cap_pairs);
+  pl.tot_stage.reserve(3);
+  exclusive_scan_reserve(n_atoms, pl.scratch);
+  exclusive_scan_reserve(boxes, pl.scratch2);
+  exclusive_scan_reserve(n_atoms, pl.scratch3);
+}
+
+// One device pass (kernels only): positions from the page-locked staging,
+// ghosts, binning, pairs, and the sizes written to page-locked memory.
+void launch_pass(PairList& pl, double rc, hipStream_t stream) {
+  const int64_t n_atoms = pl.n_atoms, boxes = pl.n_boxes;
+  const int64_t cap_ext = pl.cap_ext, cap_pairs = pl.cap_pairs;
   const NLGeom* dgeom = pl.geom.get();
   const int* dstruct = pl.atom_struct.get();
-  const double* dpos = pl.pos.get();
+  double* dpos = pl.pos.get();
   const int TB = 256;
   auto blocks = [](int64_t n, int tb) { return dim3(unsigned((n + tb - 1) / tb)); };
+  hipLaunchKernelGGL(copy_f64_kernel, blocks(3 * n_atoms, TB), dim3(TB), 0, stream, pl.pos_stage.device(),
+                     3 * n_atoms, dpos);
+  RGPU_LAUNCH_CHECK();
+  int* err = pl.err.get();
+  hipLaunchKernelGGL(zero_i32_kernel, dim3(1), dim3(64), 0, stream, err, int64_t(1));
+  RGPU_LAUNCH_CHECK();
+  // ghosts
+  int* gcount = pl.ghost_count.get();
+  int64_t* goff = pl.ghost_off.get();
+  hipLaunchKernelGGL(ghosts_kernel<false>, blocks(n_atoms, TB), dim3(TB), 0, stream, dgeom, dstruct,
+                     dpos, n_atoms, gcount, nullptr, nullptr, nullptr, int64_t(0));
+  RGPU_LAUNCH_CHECK();
+  exclusive_scan_device<int>(gcount, goff, n_atoms, stream, pl.scratch);
+  double* epos = pl.ext_pos.get();
+  int* eorig = pl.ext_orig.get();
+  hipLaunchKernelGGL(originals_kernel, blocks(n_atoms, TB), dim3(TB), 0, stream, dpos, n_atoms, epos,
+                     eorig);
+  RGPU_LAUNCH_CHECK();
+  hipLaunchKernelGGL(ghosts_kernel<true>, blocks(n_atoms, TB), dim3(TB), 0, stream, dgeom, dstruct,
+                     dpos, n_atoms, nullptr, goff, epos, eorig, cap_ext);
+  RGPU_LAUNCH_CHECK();
+  // binning
+  int* bcount = pl.box_count.get();
+  int64_t* bstart = pl.box_start.get();
+  int* bfill = pl.box_fill.get();
+  int* bitems = pl.box_items.get();
+  int64_t* ebox = pl.ext_box.get();
+  hipLaunchKernelGGL(zero_i32_kernel, blocks(boxes, TB), dim3(TB), 0, stream, bcount, boxes);
+  RGPU_LAUNCH_CHECK();
+  hipLaunchKernelGGL(zero_i32_kernel, blocks(boxes, TB), dim3(TB), 0, stream, bfill, boxes);
+  RGPU_LAUNCH_CHECK();
+  hipLaunchKernelGGL(bin_count_kernel, blocks(cap_ext, TB), dim3(TB), 0, stream, dgeom, dstruct, epos,
+                     eorig, n_atoms, goff + n_atoms, cap_ext, rc, ebox, bcount, err);
+  RGPU_LAUNCH_CHECK();
+  exclusive_scan_device<int>(bcount, bstart, boxes, stream, pl.scratch2);
+  hipLaunchKernelGGL(bin_fill_kernel, blocks(cap_ext, TB), dim3(TB), 0, stream, ebox, n_atoms,
+                     goff + n_atoms, cap_ext, bstart, bfill, bitems);
+  RGPU_LAUNCH_CHECK();
+  hipLaunchKernelGGL(bin_sort_kernel, blocks(boxes, TB), dim3(TB), 0, stream, bstart, boxes, bitems);
+  RGPU_LAUNCH_CHECK();
+  // pairs
+  int* pcount = pl.pair_count.get();
+  int64_t* coff = pl.centre_off.get();
+  const int WPB = TB / 64;
+  dim3 pgrid(unsigned((n_atoms + WPB - 1) / WPB));
+  hipLaunchKernelGGL(pairs_kernel<false>, pgrid, dim3(TB), 0, stream, dgeom, dstruct, epos, eorig,
+                     bstart, bitems, n_atoms, rc, pcount, nullptr, nullptr, nullptr, nullptr,
+                     int64_t(0));
+  RGPU_LAUNCH_CHECK();
+  exclusive_scan_device<int>(pcount, coff, n_atoms, stream, pl.scratch3);
+  hipLaunchKernelGGL(pairs_kernel<true>, pgrid, dim3(TB), 0, stream, dgeom, dstruct, epos, eorig,
+                     bstart, bitems, n_atoms, rc, nullptr, coff, pl.pair_j.get(), pl.pair_vec.get(),
+                     pl.pair_dist.get(), cap_pairs);
+  RGPU_LAUNCH_CHECK();
+  hipLaunchKernelGGL(nl_totals_kernel, dim3(1), dim3(256), 0, stream, goff + n_atoms, coff, n_atoms,
+                     cap_ext, cap_pairs, err, pl.tot_stage.device());
+  RGPU_LAUNCH_CHECK();
+}
+
+// one pass replayed from its graph (captured again when anything it depends
+// on changed)
+void launch_pass_graph(PairList& pl, double rc, hipStream_t stream) {
+  const std::vector<uintptr_t> sig = {
+      uintptr_t(stream), uintptr_t(pl.n_atoms), uintptr_t(pl.n_boxes), uintptr_t(pl.cap_ext),
+      uintptr_t(pl.cap_pairs), uintptr_t(std::llround(rc * 1e12)), uintptr_t(pl.geom.get()),
+      uintptr_t(pl.atom_struct.get()), uintptr_t(pl.pos.get()), uintptr_t(pl.pos_stage.device()),
+      uintptr_t(pl.err.get()), uintptr_t(pl.ghost_count.get()), uintptr_t(pl.ghost_off.get()),
+      uintptr_t(pl.ext_pos.get()), uintptr_t(pl.ext_orig.get()), uintptr_t(pl.box_count.get()),
+      uintptr_t(pl.box_start.get()), uintptr_t(pl.box_fill.get()), uintptr_t(pl.box_items.get()),
+      uintptr_t(pl.ext_box.get()), uintptr_t(pl.pair_count.get()), uintptr_t(pl.centre_off.get()),
+      uintptr_t(pl.pair_j.get()), uintptr_t(pl.pair_vec.get()), uintptr_t(pl.pair_dist.get()),
+      uintptr_t(pl.tot_stage.device()), uintptr_t(pl.scratch.get()), uintptr_t(pl.scratch2.get()),
+      uintptr_t(pl.scratch3.get())};
+  if (!pl.graph || sig != pl.graph_sig) {
+    if (pl.graph) (void)hipGraphExecDestroy(pl.graph);
+    pl.graph = nullptr;
+    hipGraph_t g = nullptr;
+    RGPU_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+    try {
+      launch_pass(pl, rc, stream);
+    } catch (...) {
+      (void)hipStreamEndCapture(stream, &g);
+      if (g) (void)hipGraphDestroy(g);
+      throw;
+    }
+    RGPU_HIP(hipStreamEndCapture(stream, &g));
+    const hipError_t e = hipGraphInstantiate(&pl.graph, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    RGPU_HIP(e);
+    pl.graph_sig = sig;
+    ++pl.graph_captures;
+  }
+  RGPU_HIP(hipGraphLaunch(pl.graph, stream));
+  ++pl.graph_launches;
+}
+
+// the device passes over the uploaded positions and geometry
+void run_passes(PairList& pl, double rc, hipStream_t stream, bool defer, bool graph) {
   // a short ghost capacity also hides pairs, so up to three passes
   for (int pass = 0; pass < 4; ++pass) {
     const int64_t cap_ext = pl.cap_ext, cap_pairs = pl.cap_pairs;
-    int* err = pl.err.reserve(1);
-    RGPU_HIP(hipMemsetAsync(err, 0, sizeof(int), stream));
-    // ghosts
-    int* gcount = pl.ghost_count.reserve(n_atoms);
-    int64_t* goff = pl.ghost_off.reserve(n_atoms + 1);
-    hipLaunchKernelGGL(ghosts_kernel<false>, blocks(n_atoms, TB), dim3(TB), 0, stream, dgeom, dstruct,
-                       dpos, n_atoms, gcount, nullptr, nullptr, nullptr, int64_t(0));
-    RGPU_LAUNCH_CHECK();
-    exclusive_scan_device<int>(gcount, goff, n_atoms, stream, pl.scratch);
-    double* epos = pl.ext_pos.reserve(3 * cap_ext);
-    int* eorig = pl.ext_orig.reserve(cap_ext);
-    hipLaunchKernelGGL(originals_kernel, blocks(n_atoms, TB), dim3(TB), 0, stream, dpos, n_atoms, epos,
-                       eorig);
-    RGPU_LAUNCH_CHECK();
-    hipLaunchKernelGGL(ghosts_kernel<true>, blocks(n_atoms, TB), dim3(TB), 0, stream, dgeom, dstruct,
-                       dpos, n_atoms, nullptr, goff, epos, eorig, cap_ext);
-    RGPU_LAUNCH_CHECK();
-    // binning
-    int* bcount = pl.box_count.reserve(boxes);
-    int64_t* bstart = pl.box_start.reserve(boxes + 1);
-    int* bfill = pl.box_fill.reserve(boxes);
-    int* bitems = pl.box_items.reserve(cap_ext);
-    int64_t* ebox = pl.ext_box.reserve(cap_ext);
-    RGPU_HIP(hipMemsetAsync(bcount, 0, sizeof(int) * boxes, stream));
-    RGPU_HIP(hipMemsetAsync(bfill, 0, sizeof(int) * boxes, stream));
-    hipLaunchKernelGGL(bin_count_kernel, blocks(cap_ext, TB), dim3(TB), 0, stream, dgeom, dstruct, epos,
-                       eorig, n_atoms, goff + n_atoms, cap_ext, rc, ebox, bcount, err);
-    RGPU_LAUNCH_CHECK();
-    exclusive_scan_device<int>(bcount, bstart, boxes, stream, pl.scratch2);
-    hipLaunchKernelGGL(bin_fill_kernel, blocks(cap_ext, TB), dim3(TB), 0, stream, ebox, n_atoms,
-                       goff + n_atoms, cap_ext, bstart, bfill, bitems);
-    RGPU_LAUNCH_CHECK();
-    hipLaunchKernelGGL(bin_sort_kernel, blocks(boxes, TB), dim3(TB), 0, stream, bstart, boxes, bitems);
-    RGPU_LAUNCH_CHECK();
-    // pairs
-    int* pcount = pl.pair_count.reserve(n_atoms);
-    int64_t* coff = pl.centre_off.reserve(n_atoms + 1);
-    const int WPB = TB / 64;
-    dim3 pgrid(unsigned((n_atoms + WPB - 1) / WPB));
-    hipLaunchKernelGGL(pairs_kernel<false>, pgrid, dim3(TB), 0, stream, dgeom, dstruct, epos, eorig,
-                       bstart, bitems, n_atoms, rc, pcount, nullptr, nullptr, nullptr, nullptr,
-                       int64_t(0));
-    RGPU_LAUNCH_CHECK();
-    exclusive_scan_device<int>(pcount, coff, n_atoms, stream, pl.scratch3);
-    int* pj = pl.pair_j.reserve(cap_pairs);
-    double* pv = pl.pair_vec.reserve(3 * cap_pairs);
-    double* pd = pl.pair_dist.reserve(cap_pairs);
-    hipLaunchKernelGGL(pairs_kernel<true>, pgrid, dim3(TB), 0, stream, dgeom, dstruct, epos, eorig,
-                       bstart, bitems, n_atoms, rc, nullptr, coff, pj, pv, pd, cap_pairs);
-    RGPU_LAUNCH_CHECK();
-    int64_t* dtot = pl.totals.reserve(3);
-    hipLaunchKernelGGL(nl_totals_kernel, dim3(1), dim3(256), 0, stream, goff + n_atoms, coff, n_atoms,
-                       cap_ext, cap_pairs, err, dtot);
-    RGPU_LAUNCH_CHECK();
-    int64_t* tot = pl.tot_stage.reserve(3);
-    RGPU_HIP(hipMemcpyAsync(tot, dtot, sizeof(int64_t) * 3, hipMemcpyDeviceToHost, stream));
+    reserve_pass(pl);
+    if (defer && graph)
+      launch_pass_graph(pl, rc, stream);
+    else
+      launch_pass(pl, rc, stream);
     if (defer) {
       pl.pending = true;
       pl.pending_caps[0] = cap_ext;
@@ -426,19 +516,18 @@ bool finish_pair_list(PairList& pl, hipStream_t stream) {
 
 void rebuild_pair_list(PairList& pl, double rc, hipStream_t stream) {
   if (pl.n_atoms == 0) return;
-  run_passes(pl, rc, stream, false);
+  run_passes(pl, rc, stream, false, false);
 }
 
 void build_pair_list(PairList& pl, int n_struct, const int64_t* atom_offsets,
                      const double* positions, const double* cells, const int* pbc, double rc,
-                     hipStream_t stream, bool defer) {
+                     hipStream_t stream, bool defer, bool graph) {
   // an unchecked earlier build still owns the staging buffers
   if (pl.pending) (void)finish_pair_list(pl, stream);
   const int64_t n_atoms = atom_offsets[n_struct];
   pl.n_struct = n_struct;
   pl.n_atoms = n_atoms;
   pl.geom_host.resize(n_struct);
-  std::vector<int> astruct(size_t(n_atoms > 0 ? n_atoms : 1));
   int64_t boxes = 0;
   for (int s = 0; s < n_struct; ++s) {
     int64_t a0 = atom_offsets[s], a1 = atom_offsets[s + 1];
@@ -450,7 +539,6 @@ void build_pair_list(PairList& pl, int n_struct, const int64_t* atom_offsets,
     g.box0 = boxes;
     boxes += int64_t(g.nboxes[0]) * g.nboxes[1] * g.nboxes[2];
     pl.geom_host[s] = g;
-    for (int64_t a = a0; a < a1; ++a) astruct[a] = s;
   }
   pl.n_boxes = boxes;
   if (n_atoms == 0) {
@@ -470,20 +558,23 @@ void build_pair_list(PairList& pl, int n_struct, const int64_t* atom_offsets,
                             hipMemcpyHostToDevice, stream));
     pl.geom_uploaded = pl.geom_host;
   }
+  // the structure of each atom, from the atom offsets
   const int* struct_before = pl.atom_struct.get();
   int* dstruct = pl.atom_struct.reserve(n_atoms);
-  astruct.resize(size_t(n_atoms));
-  if (dstruct != struct_before || pl.astruct_uploaded != astruct) {
-    RGPU_HIP(hipMemcpyAsync(dstruct, astruct.data(), sizeof(int) * n_atoms, hipMemcpyHostToDevice,
-                            stream));
-    pl.astruct_uploaded = astruct;
+  if (dstruct != struct_before || pl.offsets_uploaded.size() != size_t(n_struct + 1) ||
+      !std::equal(atom_offsets, atom_offsets + n_struct + 1, pl.offsets_uploaded.begin())) {
+    pl.astruct_host.resize(size_t(n_atoms));
+    for (int s = 0; s < n_struct; ++s)
+      std::fill(pl.astruct_host.begin() + atom_offsets[s], pl.astruct_host.begin() + atom_offsets[s + 1], s);
+    RGPU_HIP(hipMemcpyAsync(dstruct, pl.astruct_host.data(), sizeof(int) * n_atoms,
+                            hipMemcpyHostToDevice, stream));
+    pl.offsets_uploaded.assign(atom_offsets, atom_offsets + n_struct + 1);
   }
-  // positions through page-locked staging (free: every earlier build has
-  // been waited for, the last one just above if it was deferred)
-  double* dpos = pl.pos.reserve(3 * n_atoms);
+  // positions through page-locked staging, read by each device pass with a
+  // kernel (free: every earlier build has been waited for, the last one
+  // just above if it was deferred)
   double* ps = pl.pos_stage.reserve(size_t(3 * n_atoms));
   std::memcpy(ps, positions, sizeof(double) * 3 * n_atoms);
-  RGPU_HIP(hipMemcpyAsync(dpos, ps, sizeof(double) * 3 * n_atoms, hipMemcpyHostToDevice, stream));
 
   // Sizes of the extended (ghost) set and of the pair list are only known on
   // the device; the buffers are sized by capacities kept from earlier builds,
@@ -494,7 +585,7 @@ void build_pair_list(PairList& pl, int n_struct, const int64_t* atom_offsets,
   if (pl.cap_ext < n_atoms) pl.cap_ext = 2 * n_atoms + 1024;
   if (pl.cap_pairs < n_atoms) pl.cap_pairs = 48 * n_atoms + 1024;
   // the first build of a batch sizes the buffers synchronously
-  run_passes(pl, rc, stream, defer && pl.n_pairs > 0);
+  run_passes(pl, rc, stream, defer && pl.n_pairs > 0, graph && stream != nullptr);
 }
 
 }  // namespace rgpu

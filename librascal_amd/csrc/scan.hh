@@ -127,6 +127,12 @@ __global__ __launch_bounds__(kScan1Threads) void scan_single_kernel(const In* __
   if (threadIdx.x == 0) out[n] = carry_sh;
 }
 
+// scratch that exclusive_scan_device(n) uses (reserved ahead of a graph
+// capture, so that nothing is allocated inside it)
+inline void exclusive_scan_reserve(int64_t n, DeviceBuffer<int64_t>& scratch) {
+  if (n > 8 * int64_t(kScan1Threads) * kScanItems) scratch.reserve(size_t(2 * ((n + kScanTile - 1) / kScanTile)));
+}
+
 // out[i] = sum_{k<i} in[k] for i < n and out[n] = total, all on the device
 // (no synchronisation); out has n+1 elements.
 template <class In>

@@ -282,14 +282,17 @@ def test_mixed_prediction_adjoint_repeatable(normalize):
         assert np.abs(g - ref).max() < 1e-5 * scale
 
 
-def test_deferred_neighbour_check_rebuilds_a_short_list():
+@pytest.mark.parametrize("graphs", [False, True])
+def test_deferred_neighbour_check_rebuilds_a_short_list(graphs):
     """update_positions leaves the size check of the neighbour list to the
     prediction, which checks after its own work.  Compressing a dilute box
     (about 10 neighbours per atom within rc 5) to about 63 per atom exceeds
     the pair capacity kept from the first build (48 per atom + 1024): the
     prediction has to build the list again and repeat itself, and then
     matches a context built on the dense box directly (FP64).  Two updates
-    without a prediction between them are checked the same way."""
+    without a prediction between them are checked the same way.  With
+    graphs, the list passes of the updates are replayed from a captured
+    graph; every replay's list equals the eagerly built one."""
     n = 512
     dense = periodic_box(n, 0.12, (1, 8), d_min=1.0, seed=7)
     hp = soap_hypers(rc=5.0, sw=0.5, sigma=0.5, N=4, L=3)
@@ -297,12 +300,18 @@ def test_deferred_neighbour_check_rebuilds_a_short_list():
     sp = sorted(set(num.tolist()))
     s = (0.12 / 0.02) ** (1.0 / 3.0)
     ctx = _capi.Context(json.dumps(hp))
+    if graphs:
+        ctx.enable_graphs(True)
     ctx.set_structures(off, pos * s, num, cells * s, pbc, sp)
     n_dilute = ctx.info("n_pairs")
     ref = _capi.Context(json.dumps(hp))
     ref.set_structures(off, pos, num, cells, pbc, sp)
     n_dense = ref.info("n_pairs")
     assert n_dilute < 48 * n + 1024 < n_dense
+    nl_dense = ref.neighbours()
+    dil = _capi.Context(json.dumps(hp))
+    dil.set_structures(off, pos * s, num, cells * s, pbc, sp)
+    nl_dilute = dil.neighbours()
     X, _ = ref.features()
     rng = np.random.default_rng(1)
     idx = rng.choice(n, 96, replace=False)
@@ -318,13 +327,25 @@ def test_deferred_neighbour_check_rebuilds_a_short_list():
         np.testing.assert_allclose(g, g0, rtol=0, atol=1e-12 * np.abs(g0).max())
         np.testing.assert_allclose(st, s0, rtol=0, atol=1e-12 * np.abs(s0).max())
 
+    def same_list(a, b):
+        for x, y in zip(a, b):
+            assert np.array_equal(x, y)
+
     ctx.update_positions(pos, cells)
     same(ctx.gap_predict())
     assert ctx.info("n_pairs") == n_dense
+    same_list(ctx.neighbours(), nl_dense)
     ctx.update_positions(pos * s, cells * s)
     ctx.update_positions(pos, cells)
     same(ctx.gap_predict())
-    # the dilute box again: fits, no rebuild
-    ctx.update_positions(pos * s, cells * s)
-    ctx.gap_predict()
-    assert ctx.info("n_pairs") == n_dilute
+    same_list(ctx.neighbours(), nl_dense)
+    # the dilute box again, then the dense one twice: every list fits
+    for p, c, nl in ((pos * s, cells * s, nl_dilute), (pos, cells, nl_dense), (pos, cells, nl_dense)):
+        ctx.update_positions(p, c)
+        out = ctx.gap_predict()
+        same_list(ctx.neighbours(), nl)
+    same(out)
+    if graphs:
+        assert ctx.info("nl_graph_launches") >= 4
+    else:
+        assert ctx.info("nl_graph_launches") == 0
