@@ -585,12 +585,24 @@ const float* invariants_of<float>(Context& c) {
 // e_centre (device, may be null), gradient from device grad [n_atoms][3],
 // neg_stress from per-centre virials (summed per structure and divided by
 // the cell volume on the device).  One synchronisation at the end.
+// true for device (or managed) memory; host memory, registered or not, is
+// false
+bool is_device_memory(const void* p) {
+  hipPointerAttribute_t a{};
+  if (hipPointerGetAttributes(&a, p) != hipSuccess) {
+    (void)hipGetLastError();  // pageable host memory is unknown to the runtime
+    return false;
+  }
+  return a.type == hipMemoryTypeDevice || a.type == hipMemoryTypeManaged;
+}
+
 void emit_outputs(Context& c, const double* e_centre, double* energy, double* grad_d,
                   double* cvir_d, double* gradient, double* neg_stress) {
   const int ns = c.n_struct;
   // the reductions and the gradient copy write straight into page-locked
   // staging ([energy | gradient | stress], zeroed here: the host owns it
-  // between predictions), one wait, then the caller's host arrays
+  // between predictions); one wait, then the caller's arrays (host arrays
+  // by memcpy, device arrays by a copy enqueued before the wait)
   const size_t ne = energy && e_centre ? size_t(ns) : 0,
                ng = gradient && grad_d ? size_t(3 * c.n_atoms) : 0, nst = neg_stress ? size_t(6 * ns) : 0;
   double* h = c.out_stage.reserve(std::max<size_t>(ne + ng + nst, 1));
@@ -599,10 +611,19 @@ void emit_outputs(Context& c, const double* e_centre, double* energy, double* gr
   if (ne) launch_segment_sum(e_centre, 1, c.d_atom_off.get(), ns, hd, c.stream);
   if (ng) launch_strided_copy(grad_d, int64_t(ng), hd + ne, 1, c.stream);
   if (nst) launch_segment_sum(cvir_d, 6, c.d_atom_off.get(), ns, hd + ne + ng, c.stream, c.d_inv_vol.get());
+  struct Out {
+    double* dst;
+    size_t off, n;
+    bool device;
+  } outs[3] = {{energy, 0, ne, ne && is_device_memory(energy)},
+               {gradient, ne, ng, ng && is_device_memory(gradient)},
+               {neg_stress, ne + ng, nst, nst && is_device_memory(neg_stress)}};
+  for (const Out& o : outs)
+    if (o.n && o.device)
+      RGPU_HIP(hipMemcpyAsync(o.dst, hd + o.off, sizeof(double) * o.n, hipMemcpyDefault, c.stream));
   RGPU_HIP(hipStreamSynchronize(c.stream));
-  if (ne) std::memcpy(energy, h, sizeof(double) * ne);
-  if (ng) std::memcpy(gradient, h + ne, sizeof(double) * ng);
-  if (nst) std::memcpy(neg_stress, h + ne + ng, sizeof(double) * nst);
+  for (const Out& o : outs)
+    if (o.n && !o.device) std::memcpy(o.dst, h + o.off, sizeof(double) * o.n);
 }
 
 template <class T>

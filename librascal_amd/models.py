@@ -2,7 +2,8 @@
 
 Mirrors bindings/rascal/models/sparse_points.py (SparsePoints),
 kernels.py:16-170 (Kernel), krr.py:17-214 (KRR.predict / predict_forces /
-predict_stress) and the module functions compute_sparse_kernel_gradients /
+predict_stress), krr.py:350-477 (train_gap_model) and the module functions
+compute_sparse_kernel_gradients /
 compute_sparse_kernel_neg_stress (bind_py_models.cc:150-200).  Every kernel
 value, energy, force and stress is computed on the GPU through the
 calculator's context (C ABI in include/rascal_mi355x.h).
@@ -16,7 +17,7 @@ product, as in the reference).
 import numpy as np
 
 from . import _capi
-from .neighbourlist import AtomsList
+from .neighbourlist import AtomsList, _as_structure
 from .representations import SphericalInvariants
 
 
@@ -295,5 +296,54 @@ class KRR:
         return self.kernel._rep
 
 
+def _gram_on_device(A):
+    """A^T A in FP64 on the GPU (a plain library GEMM; torch is the device
+    plumbing here)."""
+    import torch
+    if not torch.cuda.is_available():
+        raise RuntimeError("train_gap_model needs the GPU (K_NM^T K_NM is formed there)")
+    t = torch.from_numpy(np.ascontiguousarray(A, np.float64)).to("cuda")
+    return (t.T.contiguous() @ t).cpu().numpy()
+
+
+def train_gap_model(kernel, frames, KNM_, X_sparse, y_train, self_contributions, grad_train=None,
+                    lambdas=None, jitter=1e-8):
+    """Sparse GAP training (krr.py:350-477): the weights solve
+    (K_MM + K_MN Lambda^-2 K_NM) alpha = K_MN Lambda^-2 y, with Lambda^-1 =
+    lambdas[0] / sigma_y * sqrt(N_atoms) on the rows of the per-structure
+    property (the first len(y_train) rows of KNM_) and lambdas[1] / sigma_y on
+    the gradient rows (grad_train, e.g. minus the forces), sigma_y the
+    standard deviation of the property after the self contributions are
+    subtracted; jitter on the diagonal of K_MM.  K_MM comes from the GPU
+    kernel, K_NM^T K_NM from an FP64 GEMM on the GPU; the M x M solve is
+    numpy's lstsq, as in the reference.  Returns a KRR model."""
+    KMM = kernel(X_sparse)
+    Y = np.asarray(y_train, np.float64).reshape((-1, 1)).copy()
+    KNM = np.array(KNM_, np.float64, copy=True)
+    n_centers = Y.shape[0]
+    Natoms = np.zeros(n_centers)
+    Y0 = np.zeros((n_centers, 1))
+    for iframe, frame in enumerate(frames):
+        numbers = _as_structure(frame)["numbers"]
+        Natoms[iframe] = len(numbers)
+        for sp in numbers:
+            Y0[iframe] += self_contributions[int(sp)]
+    Y = Y - Y0
+    delta = np.std(Y)
+    # lambdas[0] is per atom, hence sqrt(N_atoms) on the property rows
+    KNM[:n_centers] /= lambdas[0] / delta * np.sqrt(Natoms)[:, None]
+    Y /= lambdas[0] / delta * np.sqrt(Natoms)[:, None]
+    if grad_train is not None:
+        KNM[n_centers:] /= lambdas[1] / delta
+        F = np.asarray(grad_train, np.float64).reshape((-1, 1)).copy()
+        F /= lambdas[1] / delta
+        Y = np.vstack([Y, F])
+    KMM[np.diag_indices_from(KMM)] += jitter
+    K = KMM + _gram_on_device(KNM)
+    Y = KNM.T @ Y
+    weights = np.linalg.lstsq(K, Y, rcond=None)[0]
+    return KRR(weights, kernel, X_sparse, self_contributions)
+
+
 __all__ = ["SparsePoints", "Kernel", "KRR", "compute_sparse_kernel_gradients",
-           "compute_sparse_kernel_neg_stress"]
+           "compute_sparse_kernel_neg_stress", "train_gap_model"]

@@ -349,3 +349,31 @@ def test_deferred_neighbour_check_rebuilds_a_short_list(graphs):
         assert ctx.info("nl_graph_launches") >= 4
     else:
         assert ctx.info("nl_graph_launches") == 0
+
+
+def test_prediction_outputs_to_device_memory():
+    """Outputs given as device pointers (contiguous torch tensors on the GPU)
+    receive the same values as host arrays (to the rounding of the FP64
+    force atomics, whose order varies between predictions)."""
+    import torch
+    sts, hp, ctx, sp, spec, vals, w = _gap_box()
+    e, g, s = ctx.gap_predict()
+
+    def close(a, b):
+        return np.abs(a - b).max() <= 1e-12 * np.abs(b).max()
+
+    ed = torch.zeros(e.shape, dtype=torch.float64, device="cuda")
+    gd = torch.zeros(g.shape, dtype=torch.float64, device="cuda")
+    sd = torch.zeros(s.shape, dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    _capi.check(_capi.lib().rgpu_gap_predict(ctx._p, _capi.ptr(ed), _capi.ptr(gd), _capi.ptr(sd)))
+    torch.cuda.synchronize()
+    assert close(ed.cpu().numpy(), e) and close(gd.cpu().numpy(), g) and close(sd.cpu().numpy(), s)
+    # host energy, device gradient and stress in one call
+    eh = np.zeros_like(e)
+    gd.zero_()
+    sd.zero_()
+    torch.cuda.synchronize()
+    _capi.check(_capi.lib().rgpu_gap_predict(ctx._p, _capi.ptr(eh), _capi.ptr(gd), _capi.ptr(sd)))
+    torch.cuda.synchronize()
+    assert close(eh, e) and close(gd.cpu().numpy(), g) and close(sd.cpu().numpy(), s)

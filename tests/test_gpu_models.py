@@ -7,7 +7,7 @@ import pytest
 import oracle as O
 from conftest import golden_structures, oracle_backward, rel_err
 from librascal_amd.models import (KRR, Kernel, SparsePoints, compute_sparse_kernel_gradients,
-                                  compute_sparse_kernel_neg_stress)
+                                  compute_sparse_kernel_neg_stress, train_gap_model)
 from librascal_amd.representations import SphericalInvariants
 
 pytestmark = pytest.mark.gpu
@@ -104,6 +104,50 @@ def test_krr_predictions_and_gradient_functions():
     # a precomputed K_NM gives the same energies (krr.py:110-126)
     knm = kernel(al, sp)
     assert rel_err(e, model.predict(al, KNM=knm)).max() < TOL
+
+
+def test_train_gap_model_matches_regularised_least_squares():
+    """train_gap_model (krr.py:350-477) on noisy energies and energy
+    gradients of a known model: its weights minimise
+    |Lambda^-1 (K_NM a - y)|^2 + a^T (K_MM + jitter) a with Lambda^-1 =
+    lambdas[0] / sigma_y * sqrt(N_atoms) on the energy rows and lambdas[1] /
+    sigma_y on the gradient rows (sigma_y of the energies less the self
+    contributions).  The same minimiser is found here by another route: one
+    least-squares solve over the stacked matrix [Lambda^-1 K_NM; R], R^T R =
+    K_MM + jitter (Cholesky), instead of the normal equations.  The noise and
+    the moderate regularisation make the solution depend on every weight."""
+    rep = _rep()
+    frames, al, sp, kernel, w_true = _model(rep, 2)
+    self_contrib = {int(z): -0.5 * int(z) for z in set(al.numbers.tolist())}
+    true = KRR(w_true, kernel, sp, self_contrib)
+    rng = np.random.default_rng(11)
+    y = true.predict(al)
+    grad = -true.predict_forces(al)
+    y0 = np.array([sum(self_contrib[int(z)] for z in f["numbers"]) for f in frames])
+    y = y + 0.2 * np.std(y - y0) * rng.normal(size=y.shape)
+    grad = grad + 0.2 * np.std(grad) * rng.normal(size=grad.shape)
+    knm = kernel(al, sp)
+    kd = kernel(al, sp, grad=(True, False))
+    lambdas, jitter = (0.05, 0.2), 1e-8
+    model = train_gap_model(kernel, frames, np.vstack([knm, kd]), sp, y, self_contrib,
+                            grad_train=grad, lambdas=lambdas, jitter=jitter)
+    nat = np.diff(al.atom_offsets).astype(np.float64)
+    yc = y - y0
+    delta = np.std(yc)
+    s_e = lambdas[0] / delta * np.sqrt(nat)
+    s_g = lambdas[1] / delta
+    A = np.vstack([knm / s_e[:, None], kd / s_g])
+    b = np.concatenate([yc / s_e, grad.ravel() / s_g])
+    kmm = kernel(sp) + jitter * np.eye(sp.size())
+    R = np.linalg.cholesky(kmm).T
+    alpha = np.linalg.lstsq(np.vstack([A, R]), np.concatenate([b, np.zeros(sp.size())]),
+                            rcond=None)[0]
+    w = np.asarray(model.get_weights()).ravel()
+    # the predictions of the two solutions agree (the weights themselves
+    # only as far as K_MM's conditioning allows)
+    for K in (knm, kd):
+        p_ref = K @ alpha
+        assert np.abs(K @ w - p_ref).max() < 1e-6 * np.abs(p_ref).max()
 
 
 def test_context_reloads_its_batch():
