@@ -35,7 +35,7 @@ struct Context {
   int precision = RGPU_FP64;
 
   // radial tables on the device
-  DeviceBuffer<double> d_fac_b, d_O, d_center, d_hc, d_ha, d_grid, d_spl64;
+  DeviceBuffer<double> d_fac_b, d_O, d_center, d_hc, d_ha, d_grid, d_spl64, d_dvr_x, d_dvr_f;
   DeviceBuffer<float> d_spl32, d_ylm32;
   DeviceBuffer<double> d_ylm64;
   DeviceBuffer<HypCoef> d_hyp;
@@ -217,6 +217,9 @@ void build_tables(Context& c) {
   r.hyp = up(c.d_hyp, h.hyp);
   r.hc = up(c.d_hc, h.hc);
   r.ha = up(c.d_ha, h.ha);
+  r.basis = h.basis;
+  r.dvr_x = h.basis == 1 ? up(c.d_dvr_x, h.dvr_x) : nullptr;
+  r.dvr_f = h.basis == 1 ? up(c.d_dvr_f, h.dvr_f) : nullptr;
   {
     const std::vector<double> yt = ylm_coefficients(h.L);
     std::vector<float> yt32(yt.begin(), yt.end());
@@ -227,7 +230,8 @@ void build_tables(Context& c) {
   r.center = up(c.d_center, h.center_fin);  // finalised: R is finalised per pair
   if (c.hp.optimization != 1) return;
 
-  // Spline of O^T R(r) on [1e-5, rc] (calculator_spherical_expansion.hh
+  // Spline of O^T R(r) (the DVR functions as they are: O = identity) on
+  // [1e-5, rc] (calculator_spherical_expansion.hh
   // :1107-1135): grids of 2<<k points, k = 5, 6, ..., refined until the mean
   // absolute error at the cell midpoints is below the accuracy
   // (math/interpolator.hh:705-712, 1229-1248).  Function values come from

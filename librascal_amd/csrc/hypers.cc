@@ -68,9 +68,14 @@ SoapHypers parse_soap_hypers(const std::string& text) {
     o.gaussian_sigma = unit_value(gd.at("gaussian_sigma"));
     const json::Value& rcn = h.at("radial_contribution");
     const std::string& rt = rcn.at("type").as_string();
-    if (rt != "GTO")
+    // calculator_spherical_expansion.hh:1682-1699
+    if (rt == "GTO")
+      o.radial_basis = 0;
+    else if (rt == "DVR")
+      o.radial_basis = 1;
+    else
       throw InvalidArgument("Requested Radial contribution type '" + rt +
-                            "' is not provided by this library (GTO only)");
+                            "' has not been implemented.  Must be one of: 'GTO' or 'DVR'. ");
     o.optimization = 0;
     if (rcn.has("optimization")) {
       const json::Value& op = rcn.at("optimization");
@@ -111,6 +116,10 @@ SoapHypers parse_soap_hypers(const std::string& text) {
   if (!(o.gaussian_sigma > 0)) throw InvalidArgument("gaussian_sigma must be positive");
   if (o.optimization == 1 && !(o.spline_accuracy > 0))
     throw InvalidArgument("Error bound must be > 0");
+  // math/gauss_legendre.cc:33-37 (the DVR basis integrates on max_radial
+  // Gauss-Legendre points)
+  if (o.radial_basis == 1 && o.max_radial < 2)
+    throw InvalidArgument("there should be at least 2 integration points but: order_n < 2 == true");
   return o;
 }
 

@@ -19,7 +19,8 @@ struct SoapHypers {
   double cutoff = 0, smooth_width = 0, gaussian_sigma = 0;
   int cutoff_type = 0;  // 0 ShiftedCosine, 1 RadialScaling
   double rs_rate = 0, rs_scale = 1, rs_exponent = 0;
-  int optimization = 0;  // 0 direct Hyp1f1, 1 Spline
+  int radial_basis = 0;  // 0 GTO, 1 DVR
+  int optimization = 0;  // 0 direct evaluation, 1 Spline
   double spline_accuracy = 0;
   bool normalize = true;
   bool compute_gradients = false;

@@ -1,10 +1,12 @@
-// GTO radial basis tables shared by the host set-up and the HIP kernels.
+// Radial basis tables shared by the host set-up and the HIP kernels.
 //
 // Host side (radial_host.cc): Gaussian-type-orbital constants, the overlap
 // orthonormalisation O = diag(norm) S^{-1/2}, the per-(n,l) Hyp1f1 series /
 // asymptotic coefficients and switching points, and the uniform cubic-spline
-// tables of O^T R(r).  Device side (soap_device.hh): evaluation of R(r) by
-// Hyp1f1 (FP64) or by the spline (FP64 or FP32), Y_lm and cutoff functions.
+// tables of O^T R(r); for the DVR basis the Gauss-Legendre points and
+// weights and the Hyp1f1 of its top two Bessel orders (O = identity).
+// Device side (soap_device.hh): evaluation of R(r) directly (FP64) or by
+// the spline (FP64 or FP32), Y_lm and cutoff functions.
 #pragma once
 
 #include <vector>
@@ -31,7 +33,8 @@ struct HypCoef {
 // Everything a kernel needs to evaluate the radial part, passed by value.
 struct RadialDev {
   int N, L, L1, LL;       // max_radial, max_angular, L+1, (L+1)^2
-  int use_spline;         // 1: spline table, 0: direct Hyp1f1
+  int use_spline;         // 1: spline table, 0: direct evaluation
+  int basis;              // 0 GTO, 1 DVR
   int recursion;          // downward l-recursion (L >= 3)
   int cutoff_type;        // 0 ShiftedCosine, 1 RadialScaling
   double rc, sw, rs_rate, rs_scale, rs_exponent;
@@ -43,6 +46,11 @@ struct RadialDev {
   const HypCoef* hyp;     // [N][L+1]
   const double* hc;       // [N][L+1][2][kHypTerms] series coeff, derivative coeff
   const double* ha;       // [N][L+1][2][kHypTerms] asymptotic coeff, derivative coeff
+  // DVR (calculator_spherical_expansion.hh:738-930): Gauss-Legendre points
+  // x_n on [0, rc + 3 sigma] and factors sqrt(w_n) x_n; hyp / hc / ha hold
+  // 1F1(o + 1, 2 o + 2, z) of the Bessel orders o = L, L + 1 at entries 0, 1
+  const double* dvr_x;    // [N]
+  const double* dvr_f;    // [N]
   // spline on [x1, x2], grid_n points, packed per cell k and function
   // f = n*L1 + l as {y_k, d_k, y_k+1, d_k+1} with d = y'' h^2/6, so that one
   // vector load serves one evaluation
@@ -69,9 +77,10 @@ std::vector<double> ylm_coefficients(int L);
 
 // Host-side constants (all FP64).
 struct RadialHost {
-  int N = 0, L = 0;
+  int N = 0, L = 0, basis = 0;
   double fac_a = 0;
   std::vector<double> sigma, fac_b, norm, nfac, O, center_raw, center_fin;
+  std::vector<double> dvr_x, dvr_f;  // DVR points and sqrt(w) x factors
   std::vector<HypCoef> hyp;
   std::vector<double> hc, ha;
   // spline

@@ -196,7 +196,100 @@ void spline_second_derivatives(int n, int F, double h, const std::vector<double>
   for (size_t k = 0; k < y2.size(); ++k) d2[k] = y2[k] * h_sq_6;
 }
 
+// Series / asymptotic coefficients of 1F1(a, b, z) (and of its derivative)
+// into entry idx of the tables (hyp1f1.hh:49-229).
+static void fill_hyp(RadialHost& h, int idx, double a, double b, double tol) {
+  HypCoef& c = h.hyp[idx];
+  c.a = a;
+  c.b = b;
+  c.prefac_s = std::tgamma(a) / std::tgamma(b);
+  c.prefac_a = std::tgamma(b) / std::tgamma(a);
+  c.is_exp = std::fabs(1 - b / a) < kDblFtol;
+  double ip;
+  c.is_n_and_l = std::fabs(std::modf(2 * (a - b), &ip)) < 1e-14;
+  HostHyp hh(a, b, tol);
+  c.z_switch = hh.switching_point();
+  if (c.is_exp) return;
+  double* s0 = &h.hc[(size_t(idx) * 2 + 0) * kHypTerms];
+  double* s1 = &h.hc[(size_t(idx) * 2 + 1) * kHypTerms];
+  s0[0] = a / b;
+  s1[0] = (a + 1) / (b + 1);
+  for (int i = 1; i < kHypTerms; ++i) {
+    s0[i] = s0[i - 1] * ((a + i) / ((i + 1) * (b + i)));
+    s1[i] = s1[i - 1] * ((a + i + 1) / ((i + 1) * (b + i + 1)));
+  }
+  double* a0 = &h.ha[(size_t(idx) * 2 + 0) * kHypTerms];
+  double* a1 = &h.ha[(size_t(idx) * 2 + 1) * kHypTerms];
+  double bma = b - a, oma = 1 - a, oma1 = 1 - a - 1;
+  a0[0] = bma * oma;
+  a1[0] = bma * oma1;
+  for (int i = 1; i < kHypTerms; ++i) {
+    a0[i] = a0[i - 1] * ((bma + i) * (oma + i) / (i + 1));
+    a1[i] = a1[i - 1] * ((bma + i) * (oma1 + i) / (i + 1));
+  }
+}
+
+// Gauss-Legendre points and weights on [r0, r1], order_n points, ascending
+// (math/gauss_legendre.cc:30-73: Newton iteration on the Legendre
+// three-term recurrence from the starting guesses cos(pi (i + 3/4) / (n +
+// 1/2)), the symmetric half mirrored)
+static void gauss_legendre(double r0, double r1, int order_n, std::vector<double>& x,
+                    std::vector<double>& w) {
+  x.assign(order_n, 0.0);
+  w.assign(order_n, 0.0);
+  const double mid = 0.5 * (r0 + r1), half = 0.5 * (r1 - r0);
+  const int m = (order_n + 1) / 2;
+  for (int i = 0; i < m; ++i) {
+    double z = std::cos(3.14159265358979323846 * (i + 0.75) / (order_n + 0.5)), z1 = 0, pp = 0;
+    while (std::fabs(z - z1) > kDblFtol) {
+      double p1 = 1, p2 = 0;
+      for (int j = 0; j < order_n; ++j) {
+        const double p3 = p2;
+        p2 = p1;
+        p1 = ((2.0 * j + 1.0) * z * p2 - j * p3) / (j + 1);
+      }
+      pp = order_n * (z * p1 - p2) / (z * z - 1.0);
+      z1 = z;
+      z = z1 - p1 / pp;
+    }
+    x[i] = mid - half * z;
+    x[order_n - 1 - i] = mid + half * z;
+    w[i] = (2.0 * half) / ((1.0 - z * z) * pp * pp);
+    w[order_n - 1 - i] = w[i];
+  }
+}
+
 RadialHost::RadialHost(const SoapHypers& hp) : N(hp.max_radial), L(hp.max_angular) {
+  basis = hp.radial_basis;
+  fac_a = 0.5 * pow_i(hp.gaussian_sigma, -2);
+  const double tol = 1e-13;
+  if (basis == 1) {
+    // DVR (calculator_spherical_expansion.hh:809-848): points and weights on
+    // [0, rc + 3 sigma], factor sqrt(w) x, centre term sqrt(w) x exp(-a x^2),
+    // no orthonormalisation
+    std::vector<double> wts;
+    gauss_legendre(0.0, hp.cutoff + 3 * hp.gaussian_sigma, N, dvr_x, wts);
+    dvr_f.resize(N);
+    center_raw.resize(N);
+    for (int n = 0; n < N; ++n) {
+      dvr_f[n] = std::sqrt(wts[n]) * dvr_x[n];
+      center_raw[n] = dvr_f[n] * std::exp(-fac_a * dvr_x[n] * dvr_x[n]);
+    }
+    center_fin = center_raw;
+    sigma.assign(N, 0.0);
+    fac_b.assign(N, 0.0);
+    norm.assign(N, 1.0);
+    nfac.assign(N, 0.0);
+    O.assign(N * N, 0.0);
+    for (int n = 0; n < N; ++n) O[n * N + n] = 1.0;
+    // 1F1(o + 1, 2 o + 2, .) of the two top Bessel orders o = L, L + 1 (one
+    // order above L for the gradient recursion; math/bessel.cc:24-58)
+    hyp.resize(2);
+    hc.assign(size_t(2) * 2 * kHypTerms, 0.0);
+    ha.assign(size_t(2) * 2 * kHypTerms, 0.0);
+    for (int k = 0; k < 2; ++k) fill_hyp(*this, k, double(L + k + 1), double(2 * (L + k) + 2), tol);
+    return;
+  }
   // calculator_spherical_expansion.hh:621-636
   sigma.resize(N);
   fac_b.resize(N);
@@ -228,7 +321,6 @@ RadialHost::RadialHost(const SoapHypers& hp) : N(hp.max_radial), L(hp.max_angula
       for (int k = 0; k < N; ++k) s += V[a * N + k] * (1.0 / std::sqrt(w[k])) * V[b * N + k];
       O[a * N + b] = norm[a] * s;
     }
-  fac_a = 0.5 * pow_i(hp.gaussian_sigma, -2);
   // :478-501 centre contribution (Y_00 and 1F1(a,b,0) = 1)
   center_raw.resize(N);
   for (int n = 0; n < N; ++n) {
@@ -240,43 +332,12 @@ RadialHost::RadialHost(const SoapHypers& hp) : N(hp.max_radial), L(hp.max_angula
   for (int a = 0; a < N; ++a)
     for (int b = 0; b < N; ++b) center_fin[a] += O[b * N + a] * center_raw[b];
   // Hyp1f1 tables, tolerance 1e-13 and 200 terms (expansion.hh:697)
-  const double tol = 1e-13;
   const int L1 = L + 1;
   hyp.resize(N * L1);
   hc.assign(size_t(N) * L1 * 2 * kHypTerms, 0.0);
   ha.assign(size_t(N) * L1 * 2 * kHypTerms, 0.0);
   for (int n = 0; n < N; ++n)
-    for (int l = 0; l < L1; ++l) {
-      double a = 0.5 * (n + l + 3), b = l + 1.5;
-      HypCoef& c = hyp[n * L1 + l];
-      c.a = a;
-      c.b = b;
-      c.prefac_s = std::tgamma(a) / std::tgamma(b);
-      c.prefac_a = std::tgamma(b) / std::tgamma(a);
-      c.is_exp = std::fabs(1 - b / a) < kDblFtol;
-      double ip;
-      c.is_n_and_l = std::fabs(std::modf(2 * (a - b), &ip)) < 1e-14;
-      HostHyp hh(a, b, tol);
-      c.z_switch = hh.switching_point();
-      if (c.is_exp) continue;
-      double* s0 = &hc[(size_t(n * L1 + l) * 2 + 0) * kHypTerms];
-      double* s1 = &hc[(size_t(n * L1 + l) * 2 + 1) * kHypTerms];
-      s0[0] = a / b;
-      s1[0] = (a + 1) / (b + 1);
-      for (int i = 1; i < kHypTerms; ++i) {
-        s0[i] = s0[i - 1] * ((a + i) / ((i + 1) * (b + i)));
-        s1[i] = s1[i - 1] * ((a + i + 1) / ((i + 1) * (b + i + 1)));
-      }
-      double* a0 = &ha[(size_t(n * L1 + l) * 2 + 0) * kHypTerms];
-      double* a1 = &ha[(size_t(n * L1 + l) * 2 + 1) * kHypTerms];
-      double bma = b - a, oma = 1 - a, oma1 = 1 - a - 1;
-      a0[0] = bma * oma;
-      a1[0] = bma * oma1;
-      for (int i = 1; i < kHypTerms; ++i) {
-        a0[i] = a0[i - 1] * ((bma + i) * (oma + i) / (i + 1));
-        a1[i] = a1[i - 1] * ((bma + i) * (oma1 + i) / (i + 1));
-      }
-    }
+    for (int l = 0; l < L1; ++l) fill_hyp(*this, n * L1 + l, 0.5 * (n + l + 3), l + 1.5, tol);
 }
 
 std::vector<double> ylm_coefficients(int L) {

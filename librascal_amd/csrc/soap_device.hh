@@ -128,6 +128,74 @@ __device__ inline double hyp_g(const RadialDev& R, int idx, double z, double z2,
   return hc.prefac_s * f * ez2;
 }
 
+// Plain 1F1(a, b, z) from table entry idx (series below the switching
+// point, asymptotic above; hyp1f1.hh:185-195)
+__device__ inline double hyp_plain(const RadialDev& R, int idx, double z) {
+  const HypCoef hc = R.hyp[idx];
+  const double tol = 1e-13;
+  if (z > hc.z_switch) {
+    const double* c = R.ha + size_t(idx) * 2 * kHypTerms;
+    const int e = int(2 * (hc.a - hc.b));
+    const double zab = hc.is_n_and_l ? sqrt(e < 0 ? pow_u_dev(1.0 / z, unsigned(-e)) : pow_u_dev(z, unsigned(e)))
+                                     : pow(z, hc.a - hc.b);
+    return hc.prefac_a * exp(z) * zab * hyp_asym_sum(c, z, tol);
+  }
+  return hyp_series_sum(R.hc + size_t(idx) * 2 * kHypTerms, z, tol);
+}
+
+// DVR radial functions of Gauss-Legendre point n at distance r
+// (calculator_spherical_expansion.hh:809-900 with math/bessel.cc:30-190):
+// out[l] = sqrt(w_n) x_n exp(-a (r^2 + x_n^2)) i_l(2 a r x_n), i_l the
+// modified spherical Bessel function, from a downward recursion started by
+// 1F1 at the top two orders (argument <= 50) or an upward one from the
+// closed forms of i_0, i_1 (above); values below 1e-100 are zero.  dout[l]
+// its r-derivative (the recursion for the gradients, one order above L).
+__device__ inline void dvr_radial_n(const RadialDev& R, int n, double r, double* out,
+                                    double* dout) {
+  const int L = R.L, omax = L + 2;
+  const double a = R.fac_a, x = R.dvr_x[n], f = R.dvr_f[n];
+  double v[kMaxL1 + 1];
+  if (r < 1e-5) {
+    // zeroth-order approximation (bessel.cc:140-151)
+    const double v0 = exp(-x * x * a);
+    for (int l = 0; l <= L; ++l) {
+      out[l] = l == 0 && v0 >= 1e-100 ? f * v0 : 0.0;
+      if (dout) dout[l] = 0.0;
+    }
+    return;
+  }
+  const double z = 2.0 * a * r * x, iz = 1.0 / z;
+  if (z > 50) {
+    // upward (bessel.cc:60-82): i_0 = sinh(z) / z, i_1 = cosh(z) / z - i_0 / z
+    const double em = exp(-a * (x - r) * (x - r)), ep = exp(-a * (x + r) * (x + r));
+    v[0] = (em - ep) * 0.5 * iz;
+    v[1] = (em + ep) * 0.5 * iz - v[0] * iz;
+    for (int o = 2; o < omax; ++o) v[o] = v[o - 2] - v[o - 1] * (2.0 * o - 1.0) * iz;
+  } else {
+    // downward (bessel.cc:84-110)
+    const double efac = exp(-a * r * r) * exp(-a * x * x), ez = exp(-z);
+    for (int k = 0; k < 2; ++k) {
+      const int o = omax - 2 + k;
+      const double ig = 1.0 / tgamma(1.5 + o);
+      v[o] = ez * ig * pow_u_dev(z * 0.5, unsigned(o)) * 0.5 * 1.77245385090551602730 *
+             hyp_plain(R, k, 2.0 * z) * efac;
+    }
+    for (int o = omax - 3; o >= 0; --o) v[o] = v[o + 2] + v[o + 1] * (2.0 * o + 3.0) * iz;
+  }
+  for (int o = 0; o < omax; ++o)
+    if (v[o] < 1e-100) v[o] = 0.0;
+  for (int l = 0; l <= L; ++l) out[l] = f * v[l];
+  if (dout) {
+    // bessel.cc:112-130
+    const double g = 2.0 * a * x;
+    for (int l = 0; l <= L; ++l) {
+      double d = -2.0 * a * r * v[l];
+      d += l == 0 ? g * v[1] : g * (l * v[l - 1] + (l + 1) * v[l + 1]) / (2 * l + 1);
+      dout[l] = f * d;
+    }
+  }
+}
+
 // Raw GTO radial integral R[l] (and dR/dr) for one n at distance r
 // (calculator_spherical_expansion.hh:518-584 with hyp1f1.cc:395-491).
 // out/dout: L1 values each (dout may be null).
@@ -196,6 +264,15 @@ __device__ inline void gto_radial_n(const RadialDev& R, int n, double r, double*
       dout[l] = abl * dg * dfal + v * (double(l) / r);
     }
   }
+}
+
+// Raw radial functions of basis function n at distance r (GTO: before the
+// orthonormalisation, applied by the caller; DVR: final)
+__device__ inline void radial_n(const RadialDev& R, int n, double r, double* out, double* dout) {
+  if (R.basis == 1)
+    dvr_radial_n(R, n, r, out, dout);
+  else
+    gto_radial_n(R, n, r, out, dout);
 }
 
 // Cubic spline lookup of all F = N*L1 functions at r for function f

@@ -439,7 +439,7 @@ __device__ void stage_pairs(const RadialDev& rad, const T* yc, const PairPre& f,
     for (int it = tid; it < nb * N; it += kThreads) {
       const int q = it / N, n = it - q * N;
       double v[kMaxL1], dv[kMaxL1];
-      gto_radial_n(rad, n, b.dist[q], v, ADJ ? dv : nullptr);
+      radial_n(rad, n, b.dist[q], v, ADJ ? dv : nullptr);
       for (int l = 0; l < L1; ++l) {
         b.raw[q * F + n * L1 + l] = v[l];
         if (ADJ) b.draw[q * F + n * L1 + l] = dv[l];
@@ -1280,7 +1280,7 @@ __global__ void radial_direct_kernel(RadialDev rad, const double* __restrict__ r
   const int64_t pt = it / rad.N;
   const int n = int(it - pt * rad.N);
   double v[kMaxL1], d[kMaxL1];
-  gto_radial_n(rad, n, r[pt], v, dout ? d : nullptr);
+  radial_n(rad, n, r[pt], v, dout ? d : nullptr);
   const int64_t o = (pt * rad.N + n) * rad.L1;
   for (int l = 0; l < rad.L1; ++l) {
     out[o + l] = v[l];

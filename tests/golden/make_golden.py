@@ -92,23 +92,27 @@ def main():
 
     # spherical expansion features, GTO rows (test_calculator.cc:460-493)
     se = ubjson_read.load(os.path.join(tests_only, "spherical_expansion_reference.ubjson"))
-    rows = []
-    feats = {}
+    # GTO rows in one pair of files, DVR rows (direct and Spline) in another
+    sel = {"GTO": ([], {}), "DVR": ([], {})}
     idx = 0
     for fn in se["filenames"]:
         for rc in se["cutoffs"]:
             for k, e in enumerate(se["rep_info"][idx]):
                 hp = e["hypers"]
                 opt = hp["radial_contribution"].get("optimization", {})
-                if hp["radial_contribution"]["type"] != "GTO" or "RadialDimReduction" in opt:
+                basis = hp["radial_contribution"]["type"]
+                if basis not in sel or "RadialDimReduction" in opt:
                     continue
+                rows, feats = sel[basis]
                 name = "f%d_rc%s_%d" % (idx // len(se["cutoffs"]), rc, k)
                 rows.append(dict(name=name, structure=os.path.basename(fn), hypers=hp))
                 feats[name] = np.asarray(e["feature_matrix"])
             idx += 1
-    np.savez_compressed(os.path.join(HERE, "spherical_expansion_reference.npz"), **feats)
-    json.dump(rows, open(os.path.join(HERE, "spherical_expansion_reference.json"), "w"),
-              indent=1)
+    for basis, stem in (("GTO", "spherical_expansion_reference"),
+                        ("DVR", "spherical_expansion_dvr_reference")):
+        rows, feats = sel[basis]
+        np.savez_compressed(os.path.join(HERE, stem + ".npz"), **feats)
+        json.dump(rows, open(os.path.join(HERE, stem + ".json"), "w"), indent=1)
 
     # cosine kernels of the power spectrum (test_kernels.cc:62-101)
     kr = ubjson_read.load(os.path.join(tests_only, "kernel_reference.ubjson"))

@@ -62,7 +62,12 @@ def _create(hp):
 
 @pytest.mark.parametrize("mutate,needle", [
     (lambda h: h.update(soap_type="BiSpectrum"), "BiSpectrum"),
-    (lambda h: h["radial_contribution"].update(type="DVR"), "DVR"),
+    (lambda h: h["radial_contribution"].update(type="Chebyshev"), "Chebyshev"),
+    (lambda h: h.update(max_radial=1) or h["radial_contribution"].update(type="DVR"),
+     "integration points"),
+    (lambda h: h["radial_contribution"].update(
+        optimization=dict(Spline=dict(accuracy=1e-8),
+                          RadialDimReduction=dict(projection_matrices={}))), "RadialDimReduction"),
     (lambda h: h["cutoff_function"].update(type="Step"), "Step"),
     (lambda h: h["gaussian_density"].update(type="PieceWise"), "PieceWise"),
     (lambda h: h.update(expansion_by_species_method="magic"), "magic"),
