@@ -50,6 +50,9 @@ struct Context {
   std::vector<int> species;      // sorted
   std::vector<int> species_of;   // [n_atoms] index into species
   DeviceBuffer<int> d_species_of;
+  DeviceBuffer<int> d_sp_tab;  // batch species -> spline table (RadialDimReduction)
+  DeviceBuffer<int> d_tab_n, d_tab_cell0, d_tab_grid0;
+  DeviceBuffer<double> d_tab_h, d_tab_pu;
   DeviceBuffer<int64_t> d_atom_off;
   DeviceBuffer<int> d_centre_struct;
   PairList pl;
@@ -227,19 +230,22 @@ void build_tables(Context& c) {
     r.ylm32 = up(c.d_ylm32, yt32);
   }
   r.use_spline = 0;
-  r.center = up(c.d_center, h.center_fin);  // finalised: R is finalised per pair
+  r.n_tabs = h.n_tabs;
+  r.sp_tab = nullptr;  // set with the batch's species
+  // finalised: R is finalised per pair; one centre term per table
+  r.center = up(c.d_center, h.tab_centers);
   if (c.hp.optimization != 1) return;
 
-  // Spline of O^T R(r) (the DVR functions as they are: O = identity) on
-  // [1e-5, rc] (calculator_spherical_expansion.hh
-  // :1107-1135): grids of 2<<k points, k = 5, 6, ..., refined until the mean
-  // absolute error at the cell midpoints is below the accuracy
+  // Spline of O^T R(r) (the DVR functions as they are: O = identity; with
+  // RadialDimReduction one spline per species of its projection P_k[l]
+  // O^T R[:, l], calculator_spherical_expansion.hh:1431-1454) on [1e-5, rc]
+  // (:1107-1135): grids of 2<<k points, k = 5, 6, ..., refined until the
+  // mean absolute error at the cell midpoints is below the accuracy
   // (math/interpolator.hh:705-712, 1229-1248).  Function values come from
-  // the FP64 Hyp1f1 kernel; the tridiagonal solve runs on the host.
+  // the FP64 direct kernel; the tridiagonal solve runs on the host.
   const int F = N * L1;
   const double x1 = 1e-5, x2 = c.hp.cutoff;
   DeviceBuffer<double> d_r, d_raw, d_fin;
-  std::vector<double> grid, y, d2, test, ref, inter(F);
   auto eval_direct = [&](const std::vector<double>& xs, std::vector<double>& out) {
     int64_t n = int64_t(xs.size());
     double* dr = d_r.reserve(n);
@@ -251,64 +257,111 @@ void build_tables(Context& c) {
     out.resize(size_t(n) * F);
     RGPU_HIP(hipMemcpy(out.data(), fin, sizeof(double) * n * F, hipMemcpyDeviceToHost));
   };
-  int dof = 5;
-  double h_step = 0, per_unit = 0;
-  while (true) {
-    const int npts = 2 << dof;
-    grid = linspaced(npts, x1, x2);
-    eval_direct(grid, y);
-    h_step = grid[1] - grid[0];
-    spline_second_derivatives(npts, F, h_step, y, d2);
-    per_unit = double(npts) / (grid[npts - 1] - grid[0]);
-    const int tdof = std::max(dof, 5);
-    const int tn = 2 << tdof;
-    const double off = (x2 - x1) / (2 * (tn - 1));
-    test = linspaced(tn - 1, x1 + off, x2 - off);
-    eval_direct(test, ref);
-    double err = 0;
-    for (size_t t = 0; t < test.size(); ++t) {
-      const double x = test[t];
-      int k = std::max(0, std::min(npts - 2, int((x - x1) * per_unit) - 1));
-      const double a = (grid[k + 1] - x) / h_step, b = 1 - a;
-      for (int f = 0; f < F; ++f) {
-        const double v = a * (y[size_t(k) * F + f] + (a * a - 1) * d2[size_t(k) * F + f]) +
-                         b * (y[size_t(k + 1) * F + f] + (b * b - 1) * d2[size_t(k + 1) * F + f]);
-        err += std::fabs(v - ref[t * F + f]);
+  // values of table t at the points xs: out[pt][c * L1 + l]
+  auto eval_table = [&](int t, const std::vector<double>& xs, std::vector<double>& out) {
+    eval_direct(xs, out);
+    if (h.tab_species.empty()) return;
+    std::vector<double> fin = out;
+    for (size_t pt = 0; pt < xs.size(); ++pt)
+      for (int l = 0; l < L1; ++l)
+        for (int cc = 0; cc < N; ++cc) {
+          const double* P = &h.proj[((size_t(t) * L1 + l) * N + cc) * N];
+          double v = 0;
+          for (int n = 0; n < N; ++n) v += P[n] * fin[pt * F + n * L1 + l];
+          out[pt * F + cc * L1 + l] = v;
+        }
+  };
+  struct Fit {
+    std::vector<double> grid, y, d2;
+    double h = 0, per_unit = 0;
+  };
+  // spline of table t on 2 << dof points (fixed_npts > 0: that grid, no
+  // refinement)
+  auto fit_table = [&](int t, int fixed_npts) {
+    Fit f;
+    std::vector<double> test, ref;
+    int dof = 5;
+    while (true) {
+      const int npts = fixed_npts > 0 ? fixed_npts : 2 << dof;
+      f.grid = linspaced(npts, x1, x2);
+      eval_table(t, f.grid, f.y);
+      f.h = f.grid[1] - f.grid[0];
+      spline_second_derivatives(npts, F, f.h, f.y, f.d2);
+      f.per_unit = double(npts) / (f.grid[npts - 1] - f.grid[0]);
+      if (fixed_npts > 0) return f;
+      const int tdof = std::max(dof, 5);
+      const int tn = 2 << tdof;
+      const double off = (x2 - x1) / (2 * (tn - 1));
+      test = linspaced(tn - 1, x1 + off, x2 - off);
+      eval_table(t, test, ref);
+      double err = 0;
+      for (size_t q = 0; q < test.size(); ++q) {
+        const double x = test[q];
+        int k = std::max(0, std::min(npts - 2, int((x - x1) * f.per_unit) - 1));
+        const double a = (f.grid[k + 1] - x) / f.h, b = 1 - a;
+        for (int ff = 0; ff < F; ++ff) {
+          const double v = a * (f.y[size_t(k) * F + ff] + (a * a - 1) * f.d2[size_t(k) * F + ff]) +
+                           b * (f.y[size_t(k + 1) * F + ff] + (b * b - 1) * f.d2[size_t(k + 1) * F + ff]);
+          err += std::fabs(v - ref[q * F + ff]);
+        }
       }
+      err /= double(test.size()) * F;
+      if (err < c.hp.spline_accuracy || npts >= 100000) return f;
+      ++dof;
     }
-    err /= double(test.size()) * F;
-    if (err < c.hp.spline_accuracy || npts >= 100000) break;
-    ++dof;
+  };
+  // one spline per table, each refined on its own (as the reference keeps
+  // one interpolator per species), the tables stored one after another
+  std::vector<Fit> fits;
+  for (int t = 0; t < h.n_tabs; ++t) fits.push_back(fit_table(t, 0));
+  std::vector<int> tab_n, tab_cell0, tab_grid0;
+  std::vector<double> tab_h, tab_pu, grids;
+  int cells = 0;
+  for (const Fit& f : fits) {
+    tab_n.push_back(int(f.grid.size()));
+    tab_cell0.push_back(cells);
+    tab_grid0.push_back(int(grids.size()));
+    tab_h.push_back(f.h);
+    tab_pu.push_back(f.per_unit);
+    grids.insert(grids.end(), f.grid.begin(), f.grid.end());
+    cells += int(f.grid.size()) - 1;
   }
-  const int npts = int(grid.size());
   // packed per cell: {y_k, d_k, y_k+1, d_k+1}[f]
-  std::vector<double> tab(size_t(npts - 1) * F * 4);
-  std::vector<float> tab32(tab.size());
-  for (int k = 0; k + 1 < npts; ++k)
-    for (int f = 0; f < F; ++f) {
-      double* t = &tab[(size_t(k) * F + f) * 4];
-      t[0] = y[size_t(k) * F + f];
-      t[1] = d2[size_t(k) * F + f];
-      t[2] = y[size_t(k + 1) * F + f];
-      t[3] = d2[size_t(k + 1) * F + f];
-    }
-  for (size_t k = 0; k < tab.size(); ++k) tab32[k] = float(tab[k]);
+  std::vector<double> tab(size_t(cells) * F * 4);
+  for (int t = 0; t < h.n_tabs; ++t)
+    for (int k = 0; k + 1 < tab_n[t]; ++k)
+      for (int ff = 0; ff < F; ++ff) {
+        double* e = &tab[((size_t(tab_cell0[t]) + k) * F + ff) * 4];
+        e[0] = fits[t].y[size_t(k) * F + ff];
+        e[1] = fits[t].d2[size_t(k) * F + ff];
+        e[2] = fits[t].y[size_t(k + 1) * F + ff];
+        e[3] = fits[t].d2[size_t(k + 1) * F + ff];
+      }
+  std::vector<float> tab32(tab.begin(), tab.end());
+  const Fit& f0 = fits[0];
+  const int npts = tab_n[0];
   h.grid_n = npts;
   h.x1 = x1;
   h.x2 = x2;
-  h.h = h_step;
-  h.per_unit = per_unit;
+  h.h = f0.h;
+  h.per_unit = f0.per_unit;
   h.spl64 = tab;
   r.use_spline = 1;
+  // the scalar grid is table 0's (the only one without RadialDimReduction)
   r.grid_n = npts;
   r.search_size = npts - 2;
   r.x1 = x1;
-  r.h = h_step;
-  r.per_unit = per_unit;
-  r.grid = up(c.d_grid, grid);
+  r.h = f0.h;
+  r.per_unit = f0.per_unit;
+  r.grid = up(c.d_grid, grids);
   r.spl64 = up(c.d_spl64, tab);
   r.spl32 = up(c.d_spl32, tab32);
-  r.center = up(c.d_center, h.center_fin);
+  r.tab_n = up(c.d_tab_n, tab_n);
+  r.tab_cell0 = up(c.d_tab_cell0, tab_cell0);
+  r.tab_grid0 = up(c.d_tab_grid0, tab_grid0);
+  r.tab_h = up(c.d_tab_h, tab_h);
+  r.tab_pu = up(c.d_tab_pu, tab_pu);
+  r.tab_grid = r.grid;
 }
 
 // the batch is set; a neighbour list built with its size check deferred
@@ -1029,6 +1082,19 @@ int rgpu_set_structures(rgpu_context* ctx, int n_struct, const int64_t* atom_off
       if (it == sp.end() || *it != t) throw InvalidArgument("atom type missing from the species list");
       sof[a] = int(it - sp.begin());
     }
+    // RadialDimReduction: the spline table of each batch species (a species
+    // without projections is an error, calculator_spherical_expansion.hh
+    // :1339-1371)
+    std::vector<int> sp_tab;
+    if (!c.rh->tab_species.empty()) {
+      for (int z : sp) {
+        auto it = std::find(c.rh->tab_species.begin(), c.rh->tab_species.end(), z);
+        if (it == c.rh->tab_species.end())
+          throw RuntimeError("RadialDimReduction is missing projection matrices at least for species '" +
+                             std::to_string(z) + "'");
+        sp_tab.push_back(int(it - c.rh->tab_species.begin()));
+      }
+    }
     c.n_struct = n_struct;
     c.n_atoms = n_atoms;
     c.atom_off.assign(atom_offsets, atom_offsets + n_struct + 1);
@@ -1037,6 +1103,13 @@ int rgpu_set_structures(rgpu_context* ctx, int n_struct, const int64_t* atom_off
     c.types.assign(types, types + n_atoms);
     c.species = sp;
     c.species_of = sof;
+    c.rd.sp_tab = nullptr;
+    if (!sp_tab.empty()) {
+      int* dt = c.d_sp_tab.reserve(sp_tab.size());
+      RGPU_HIP(hipMemcpyAsync(dt, sp_tab.data(), sizeof(int) * sp_tab.size(), hipMemcpyHostToDevice,
+                              c.stream));
+      c.rd.sp_tab = dt;
+    }
     int* d = c.d_species_of.reserve(std::max<size_t>(sof.size(), 1));
     if (n_atoms) RGPU_HIP(hipMemcpyAsync(d, sof.data(), sizeof(int) * n_atoms, hipMemcpyHostToDevice, c.stream));
     int64_t* dof = c.d_atom_off.reserve(size_t(n_struct + 1));

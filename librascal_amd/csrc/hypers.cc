@@ -2,6 +2,7 @@
 
 #include <algorithm>
 #include <set>
+#include <string>
 
 #include "common.hh"
 
@@ -12,6 +13,44 @@ double unit_value(const json::Value& v) {
   // {"value": x, "unit": "AA"} or a bare number
   if (v.is_object()) return v.at("value").as_number();
   return v.as_number();
+}
+
+// RadialDimReduction projection matrices: {"Z": [L+1][N][N]}
+// (calculator_spherical_expansion.hh:1188-1323)
+void parse_projections(const json::Value& rdr, SoapHypers& o) {
+  if (!rdr.has("projection_matrices")) throw InvalidArgument("No projection matrices were given.");
+  const json::Value& pm = rdr.at("projection_matrices");
+  if (!pm.is_object() || pm.obj.empty())
+    throw InvalidArgument(
+        "Projection matrices should be of type std::map<std::string, "
+        "std::vector<std::vector<std::vector<double>>>>.");
+  const int L1 = o.max_angular + 1, N = o.max_radial;
+  const std::string shape = "projection_matrices should have the shape (n_species, max_angular+1, "
+                            "max_radial, max_radial)";
+  for (const auto& kv : pm.obj) {
+    int z = 0;
+    try {
+      z = std::stoi(kv.first);
+    } catch (const std::exception&) {
+      throw InvalidArgument("projection_matrices keys must be atomic numbers, got '" + kv.first + "'");
+    }
+    const json::Value& per_l = kv.second;
+    if (per_l.kind != json::Value::Array || int(per_l.arr.size()) != L1)
+      throw InvalidArgument("Projection matrices at species=" + kv.first +
+                            " should have max_angular+1 entries; " + shape);
+    o.rdr_species.push_back(z);
+    for (const json::Value& m : per_l.arr) {
+      if (m.kind != json::Value::Array || int(m.arr.size()) != N)
+        throw InvalidArgument("Projection matrices at species=" + kv.first +
+                              " should have max_radial rows; " + shape);
+      for (const json::Value& row : m.arr) {
+        if (row.kind != json::Value::Array || int(row.arr.size()) != N)
+          throw InvalidArgument("Projection matrices at species=" + kv.first +
+                                " should have max_radial columns; " + shape);
+        for (const json::Value& x : row.arr) o.rdr_proj.push_back(x.as_number());
+      }
+    }
+  }
 }
 }  // namespace
 
@@ -81,7 +120,7 @@ SoapHypers parse_soap_hypers(const std::string& text) {
       const json::Value& op = rcn.at("optimization");
       bool spline = op.has("Spline"), rdr = op.has("RadialDimReduction");
       if (rdr && !spline) throw InvalidArgument("RadialDimReduction requires' Spline to be set.");
-      if (rdr) throw InvalidArgument("RadialDimReduction is not provided by this library");
+      if (rdr) parse_projections(op.at("RadialDimReduction"), o);
       if (spline) {
         o.optimization = 1;
         const json::Value& sp = op.at("Spline");

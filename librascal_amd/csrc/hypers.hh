@@ -21,6 +21,10 @@ struct SoapHypers {
   double rs_rate = 0, rs_scale = 1, rs_exponent = 0;
   int radial_basis = 0;  // 0 GTO, 1 DVR
   int optimization = 0;  // 0 direct evaluation, 1 Spline
+  // RadialDimReduction (with Spline): per species (atomic number) the
+  // projections of the radial functions, [L+1][N components][N], flattened
+  std::vector<int> rdr_species;
+  std::vector<double> rdr_proj;
   double spline_accuracy = 0;
   bool normalize = true;
   bool compute_gradients = false;

@@ -51,6 +51,20 @@ struct RadialDev {
   // 1F1(o + 1, 2 o + 2, z) of the Bessel orders o = L, L + 1 at entries 0, 1
   const double* dvr_x;    // [N]
   const double* dvr_f;    // [N]
+  // RadialDimReduction: n_tabs spline tables, each on its own grid of
+  // tab_n[t] points (spacing tab_h[t], tab_pu[t] points per unit length,
+  // points at tab_grid + tab_grid0[t]), stored one after another from cell
+  // tab_cell0[t]; sp_tab [S] maps a batch species to its table (null with
+  // one table, which then uses the scalar grid above); center holds n_tabs
+  // centre terms of N values
+  int n_tabs;
+  const int* sp_tab;
+  const int* tab_n;
+  const int* tab_cell0;
+  const int* tab_grid0;
+  const double* tab_h;
+  const double* tab_pu;
+  const double* tab_grid;
   // spline on [x1, x2], grid_n points, packed per cell k and function
   // f = n*L1 + l as {y_k, d_k, y_k+1, d_k+1} with d = y'' h^2/6, so that one
   // vector load serves one evaluation
@@ -81,6 +95,12 @@ struct RadialHost {
   double fac_a = 0;
   std::vector<double> sigma, fac_b, norm, nfac, O, center_raw, center_fin;
   std::vector<double> dvr_x, dvr_f;  // DVR points and sqrt(w) x factors
+  // RadialDimReduction: spline tables, one per species (1 without it), the
+  // species' atomic numbers, projections [species][L+1][N][N] and centre
+  // terms [species][N]
+  int n_tabs = 1;
+  std::vector<int> tab_species;
+  std::vector<double> proj, tab_centers;
   std::vector<HypCoef> hyp;
   std::vector<double> hc, ha;
   // spline
@@ -89,6 +109,7 @@ struct RadialHost {
   std::vector<double> spl64;
 
   explicit RadialHost(const SoapHypers& hp);
+  void set_projections(const SoapHypers& hp);
 };
 
 // Natural cubic spline second derivatives (times h^2/6) of the columns of

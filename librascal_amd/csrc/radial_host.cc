@@ -259,6 +259,27 @@ static void gauss_legendre(double r0, double r1, int order_n, std::vector<double
   }
 }
 
+// RadialDimReduction: the centre term of species k is P_k[l=0] times the
+// finalised centre term (calculator_spherical_expansion.hh:1417-1428)
+void RadialHost::set_projections(const SoapHypers& hp) {
+  tab_species = hp.rdr_species;
+  proj = hp.rdr_proj;
+  n_tabs = tab_species.empty() ? 1 : int(tab_species.size());
+  if (tab_species.empty()) {
+    tab_centers = center_fin;
+    return;
+  }
+  const int L1 = L + 1;
+  tab_centers.assign(size_t(n_tabs) * N, 0.0);
+  for (int k = 0; k < n_tabs; ++k)
+    for (int c = 0; c < N; ++c) {
+      const double* P0 = &proj[((size_t(k) * L1 + 0) * N + c) * N];
+      double s = 0;
+      for (int n = 0; n < N; ++n) s += P0[n] * center_fin[n];
+      tab_centers[size_t(k) * N + c] = s;
+    }
+}
+
 RadialHost::RadialHost(const SoapHypers& hp) : N(hp.max_radial), L(hp.max_angular) {
   basis = hp.radial_basis;
   fac_a = 0.5 * pow_i(hp.gaussian_sigma, -2);
@@ -288,6 +309,7 @@ RadialHost::RadialHost(const SoapHypers& hp) : N(hp.max_radial), L(hp.max_angula
     hc.assign(size_t(2) * 2 * kHypTerms, 0.0);
     ha.assign(size_t(2) * 2 * kHypTerms, 0.0);
     for (int k = 0; k < 2; ++k) fill_hyp(*this, k, double(L + k + 1), double(2 * (L + k) + 2), tol);
+    set_projections(hp);
     return;
   }
   // calculator_spherical_expansion.hh:621-636
@@ -331,6 +353,7 @@ RadialHost::RadialHost(const SoapHypers& hp) : N(hp.max_radial), L(hp.max_angula
   center_fin.assign(N, 0.0);
   for (int a = 0; a < N; ++a)
     for (int b = 0; b < N; ++b) center_fin[a] += O[b * N + a] * center_raw[b];
+  set_projections(hp);
   // Hyp1f1 tables, tolerance 1e-13 and 200 terms (expansion.hh:697)
   const int L1 = L + 1;
   hyp.resize(N * L1);

@@ -240,6 +240,7 @@ struct PairBatch {
   T* dir;        // [PB][3]
   T* ca;         // [PB] spline weights of the pair's cell
   T* cb;         // [PB]
+  T* ih;         // [PB] inverse grid spacing of the pair's table
   T* res;        // [PB][4] per-pair contractions (adjoint)
   int* sp;       // [PB] neighbour species
   int* lst;      // [PB] batch positions grouped by species (forward)
@@ -252,7 +253,7 @@ __host__ __device__ inline size_t pair_batch_bytes(int PB, int LL, int F, int ts
   size_t b = 8 * size_t(PB) * (adjoint ? 6 : 2);
   if (direct) b += 8 * size_t(PB) * F * (adjoint ? 2 : 1);
   b += size_t(tsize) * PB *
-       (size_t(LL) * (adjoint ? 4 : 1) + size_t(F) * (adjoint ? 2 : 1) + 5 + (adjoint ? 4 : 0));
+       (size_t(LL) * (adjoint ? 4 : 1) + size_t(F) * (adjoint ? 2 : 1) + 6 + (adjoint ? 4 : 0));
   b = align_up(b, 8);
   b += 4 * size_t(PB) * (adjoint ? 4 : 3);
   return align_up(b, 64);
@@ -303,6 +304,8 @@ __device__ PairBatch<T> carve_batch(unsigned char* base, int PB, int LL, int F) 
   t += PB;
   pb.cb = t;
   t += PB;
+  pb.ih = t;
+  t += PB;
   pb.res = nullptr;
   if (ADJ) {
     pb.res = t;
@@ -320,7 +323,7 @@ __device__ PairBatch<T> carve_batch(unsigned char* base, int PB, int LL, int F) 
 // the batch), fetched a batch ahead so that their global latency overlaps
 // the work on the previous batch.
 struct PairPre {
-  double d, vx, vy, vz, fc, dfc, xhi;
+  double d, vx, vy, vz, fc, dfc, xhi, h;
   int j, sp, klo;
 };
 
@@ -338,11 +341,23 @@ __device__ __forceinline__ void pair_fetch(const RadialDev& rad, const PairView&
     f.j = pv.pair_j[p];
     f.sp = pv.species_of[f.j];
     if (!DIRECT) {
-      // spline cell as in spline_cell()
-      int k = int((f.d - rad.x1) * rad.per_unit) - 1;
-      k = k < 0 ? 0 : (k > rad.search_size ? rad.search_size : k);
-      f.klo = k;
-      f.xhi = rad.grid[k + 1];
+      // spline cell as in spline_cell(); with one table per species, on
+      // the neighbour species' own grid, its cells numbered on from the
+      // tables before it
+      if (rad.sp_tab) {
+        const int t = rad.sp_tab[f.sp], last = rad.tab_n[t] - 2;
+        int k = int((f.d - rad.x1) * rad.tab_pu[t]) - 1;
+        k = k < 0 ? 0 : (k > last ? last : k);
+        f.xhi = rad.tab_grid[rad.tab_grid0[t] + k + 1];
+        f.h = rad.tab_h[t];
+        f.klo = rad.tab_cell0[t] + k;
+      } else {
+        int k = int((f.d - rad.x1) * rad.per_unit) - 1;
+        k = k < 0 ? 0 : (k > rad.search_size ? rad.search_size : k);
+        f.xhi = rad.grid[k + 1];
+        f.h = rad.h;
+        f.klo = k;
+      }
     }
   }
 }
@@ -376,10 +391,11 @@ __device__ void stage_pairs(const RadialDev& rad, const T* yc, const PairPre& f,
     }
     b.sp[tid] = f.sp;
     if (!DIRECT) {
-      const double a = (f.xhi - d) / rad.h;
+      const double a = (f.xhi - d) / f.h;
       b.klo[tid] = f.klo;
       b.ca[tid] = T(a);
       b.cb[tid] = T(1 - a);
+      b.ih[tid] = T(1.0 / f.h);
     }
   }
   __syncthreads();
@@ -413,7 +429,6 @@ __device__ void stage_pairs(const RadialDev& rad, const T* yc, const PairPre& f,
   if (!DIRECT) {
     if (tid >= r0) {
       const FastDiv fF = fast_div(F), fL1 = fast_div(L1);
-      const T inv_h = T(1.0 / rad.h);
 #pragma unroll 4
       for (int it = tid - r0; it < nb * F; it += rstride) {
         const int q = it / fF, f = it - q * F;
@@ -421,7 +436,7 @@ __device__ void stage_pairs(const RadialDev& rad, const T* yc, const PairPre& f,
         cell.klo = b.klo[q];
         cell.a = b.ca[q];
         cell.b = b.cb[q];
-        cell.inv_h = inv_h;
+        cell.inv_h = b.ih[q];
         T v, dv;
         spline_eval<T>(rad, cell, f, &v, ADJ ? &dv : nullptr);
         if (ADJ) {
@@ -532,7 +547,7 @@ __global__ __launch_bounds__(kThreads, (expansion_min_waves<T, KPT, DIRECT>())) 
           if (q0 > 0)
             acc[k] = cs[e];
           else if (s == si && (ent[k] & 0xffff) == 0)
-            acc[k] = T(rad.center[e / fLL] * inv_s4pi);
+            acc[k] = T(rad.center[(rad.sp_tab ? rad.sp_tab[si] * rad.N : 0) + e / fLL] * inv_s4pi);
         }
       }
       for (int t = 0; t < cnt; ++t) {

@@ -92,8 +92,9 @@ def main():
 
     # spherical expansion features, GTO rows (test_calculator.cc:460-493)
     se = ubjson_read.load(os.path.join(tests_only, "spherical_expansion_reference.ubjson"))
-    # GTO rows in one pair of files, DVR rows (direct and Spline) in another
-    sel = {"GTO": ([], {}), "DVR": ([], {})}
+    # GTO rows in one pair of files, DVR rows (direct and Spline) in another,
+    # the RadialDimReduction rows (GTO and DVR) in a third
+    sel = {"GTO": ([], {}), "DVR": ([], {}), "RDR": ([], {})}
     idx = 0
     for fn in se["filenames"]:
         for rc in se["cutoffs"]:
@@ -101,7 +102,9 @@ def main():
                 hp = e["hypers"]
                 opt = hp["radial_contribution"].get("optimization", {})
                 basis = hp["radial_contribution"]["type"]
-                if basis not in sel or "RadialDimReduction" in opt:
+                if "RadialDimReduction" in opt:
+                    basis = "RDR"
+                if basis not in sel:
                     continue
                 rows, feats = sel[basis]
                 name = "f%d_rc%s_%d" % (idx // len(se["cutoffs"]), rc, k)
@@ -109,9 +112,18 @@ def main():
                 feats[name] = np.asarray(e["feature_matrix"])
             idx += 1
     for basis, stem in (("GTO", "spherical_expansion_reference"),
-                        ("DVR", "spherical_expansion_dvr_reference")):
+                        ("DVR", "spherical_expansion_dvr_reference"),
+                        ("RDR", "spherical_expansion_rdr_reference")):
         rows, feats = sel[basis]
         np.savez_compressed(os.path.join(HERE, stem + ".npz"), **feats)
+        if basis == "RDR":
+            # every row carries the same projection matrices: kept once
+            pms = set()
+            for r in rows:
+                red = r["hypers"]["radial_contribution"]["optimization"]["RadialDimReduction"]
+                pms.add(json.dumps(red.pop("projection_matrices"), sort_keys=True))
+            assert len(pms) == 1
+            rows = dict(projection_matrices=json.loads(pms.pop()), rows=rows)
         json.dump(rows, open(os.path.join(HERE, stem + ".json"), "w"), indent=1)
 
     # cosine kernels of the power spectrum (test_kernels.cc:62-101)

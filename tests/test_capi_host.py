@@ -67,7 +67,7 @@ def _create(hp):
      "integration points"),
     (lambda h: h["radial_contribution"].update(
         optimization=dict(Spline=dict(accuracy=1e-8),
-                          RadialDimReduction=dict(projection_matrices={}))), "RadialDimReduction"),
+                          RadialDimReduction=dict(projection_matrices={}))), "Projection matrices"),
     (lambda h: h["cutoff_function"].update(type="Step"), "Step"),
     (lambda h: h["gaussian_density"].update(type="PieceWise"), "PieceWise"),
     (lambda h: h.update(expansion_by_species_method="magic"), "magic"),

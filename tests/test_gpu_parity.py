@@ -263,3 +263,61 @@ def test_dvr_gradients_vs_finite_differences(spline):
             fd = (vals[0] - vals[1]) / (2 * h)
             assert abs(fd - g[atom, d]) < 1e-6 * np.abs(g).max() + 1e-6 * abs(fd)
     ctx.update_positions(pos)
+
+
+def _rdr_rows():
+    d = json.load(open(os.path.join(_GOLDEN_DIR, "spherical_expansion_rdr_reference.json")))
+    for r in d["rows"]:
+        r["hypers"]["radial_contribution"]["optimization"]["RadialDimReduction"][
+            "projection_matrices"] = d["projection_matrices"]
+    return d["rows"]
+
+
+@pytest.mark.parametrize("row", _rdr_rows(), ids=lambda r: r["name"])
+def test_radial_dim_reduction_expansion_reference(row):
+    """tests/test_calculator.cc:460-493, the RadialDimReduction rows of the
+    reference's spherical_expansion_reference.ubjson (GTO and DVR, Spline
+    1e-8, per-species projection matrices of every l): the FP64 GPU
+    expansion against the reference's own features, relative_error < 1e-8
+    (the reference's bar is 3e-6)."""
+    feats = np.load(os.path.join(_GOLDEN_DIR, "spherical_expansion_rdr_reference.npz"))[row["name"]]
+    sts = golden_structures()[_STRUCT_OF[row["structure"]]]
+    ctx, _ = make_ctx(row["hypers"], sts)
+    c, _ = ctx.expansion()
+    X = c.reshape(c.shape[0], -1)
+    assert X.shape == feats.shape
+    assert rel_err(feats, X).max() < 1e-8
+
+
+def test_radial_dim_reduction_missing_species_and_gradients():
+    """A batch species without projection matrices is an error (the
+    reference's runtime_error); with projections for every species, the
+    adjoint (spline derivatives of each species' table) matches central
+    differences of the GPU features."""
+    hp = soap_hypers(rc=3.5, sw=0.5, sigma=0.4, N=4, L=3, spline=1e-8)
+    rng = np.random.default_rng(7)
+    sts = golden_structures()["small_molecule"]
+    zs = sorted(set(pack(sts)[2].tolist()))
+    proj = {str(z): rng.uniform(size=(4, 4, 4)).tolist() for z in zs}
+    hp["radial_contribution"]["optimization"]["RadialDimReduction"] = dict(
+        projection_matrices={k: v for k, v in proj.items() if k != str(zs[0])})
+    with pytest.raises(_capi.RascalGPUError, match="missing projection matrices"):
+        make_ctx(hp, sts)
+    hp["radial_contribution"]["optimization"]["RadialDimReduction"]["projection_matrices"] = proj
+    ctx, _ = make_ctx(hp, sts)
+    X, _ = ctx.features()
+    G = rng.normal(size=X.shape)
+    g, _ = ctx.features_backward(G)
+    pos = pack(sts)[1]
+    h = 1e-5
+    for atom in range(min(4, len(pos))):
+        for d in range(3):
+            vals = []
+            for sgn in (1, -1):
+                p = pos.copy()
+                p[atom, d] += sgn * h
+                ctx.update_positions(p)
+                vals.append(np.sum(G * ctx.features()[0]))
+            fd = (vals[0] - vals[1]) / (2 * h)
+            assert abs(fd - g[atom, d]) < 1e-6 * np.abs(g).max() + 1e-6 * abs(fd)
+    ctx.update_positions(pos)
