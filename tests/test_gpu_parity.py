@@ -321,3 +321,24 @@ def test_radial_dim_reduction_missing_species_and_gradients():
             fd = (vals[0] - vals[1]) / (2 * h)
             assert abs(fd - g[atom, d]) < 1e-6 * np.abs(g).max() + 1e-6 * abs(fd)
     ctx.update_positions(pos)
+
+
+@pytest.mark.parametrize("kind", ["DVR", "DVR+RDR", "GTO+RDR"])
+def test_radial_options_mixed_precision(kind):
+    """The mixed path (FP32 expansion and invariants from the same FP32
+    spline tables) with the DVR basis and RadialDimReduction: normalised
+    invariants within 1e-5 of the FP64 path (|X| = 1)."""
+    hp = soap_hypers(rc=3.5, sw=0.5, sigma=0.4, N=6, L=4, spline=1e-8)
+    sts = golden_structures()["CaCrP2O7"]
+    if kind.startswith("DVR"):
+        hp["radial_contribution"]["type"] = "DVR"
+    if kind.endswith("RDR"):
+        rng = np.random.default_rng(2)
+        zs = sorted(set(pack(sts)[2].tolist()))
+        hp["radial_contribution"]["optimization"]["RadialDimReduction"] = dict(
+            projection_matrices={str(z): rng.uniform(size=(5, 6, 6)).tolist() for z in zs})
+    ctx, _ = make_ctx(hp, sts)
+    X64, _ = ctx.features()
+    ctx.set_precision(_capi.MIXED)
+    X32, _ = ctx.features()
+    assert np.abs(X32 - X64).max() < 1e-5
