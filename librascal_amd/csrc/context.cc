@@ -52,6 +52,8 @@ struct Context {
   DeviceBuffer<int> d_species_of;
   DeviceBuffer<int> d_sp_tab;  // batch species -> spline table (RadialDimReduction)
   DeviceBuffer<int> d_tab_n, d_tab_cell0, d_tab_grid0;
+  DeviceBuffer<YGradTerm<double>> d_ygrad64;
+  DeviceBuffer<YGradTerm<float>> d_ygrad32;
   DeviceBuffer<double> d_tab_h, d_tab_pu;
   DeviceBuffer<int64_t> d_atom_off;
   DeviceBuffer<int> d_centre_struct;
@@ -228,6 +230,77 @@ void build_tables(Context& c) {
     std::vector<float> yt32(yt.begin(), yt.end());
     r.ylm64 = up(c.d_ylm64, yt);
     r.ylm32 = up(c.d_ylm32, yt32);
+  }
+  {
+    // Gradient terms of Y_lm: coefficients of d_c (r^l Y_lm) = dY_c[lm] +
+    // l r_c Y_lm over the harmonics of degree l - 1, projected with a
+    // quadrature exact for these band-limited products (Gauss-Legendre in
+    // cos theta, uniform in phi), from the FP64 Y_lm and gradient kernel
+    const int L = h.L, LL = L1 * L1, nt = L + 2, np = 2 * L + 3;
+    std::vector<double> ct, wt;
+    gauss_legendre(-1.0, 1.0, nt, ct, wt);
+    const int nq = nt * np;
+    std::vector<double> dirs(size_t(3) * nq), wq(nq);
+    for (int it = 0; it < nt; ++it)
+      for (int ip = 0; ip < np; ++ip) {
+        const int q = it * np + ip;
+        const double ph = 2 * 3.14159265358979323846 * (ip + 0.5) / np, st = std::sqrt(1 - ct[it] * ct[it]);
+        dirs[3 * q] = st * std::cos(ph);
+        dirs[3 * q + 1] = st * std::sin(ph);
+        dirs[3 * q + 2] = ct[it];
+        wq[q] = wt[it] * 2 * 3.14159265358979323846 / np;
+      }
+    DeviceBuffer<double> d_dirs, d_Y, d_dY;
+    double* dd = d_dirs.reserve(dirs.size());
+    RGPU_HIP(hipMemcpy(dd, dirs.data(), sizeof(double) * dirs.size(), hipMemcpyHostToDevice));
+    double* dY0 = d_Y.reserve(size_t(nq) * LL);
+    double* dY1 = d_dY.reserve(size_t(nq) * 3 * LL);
+    launch_ylm_eval<double>(r, dd, nq, dY0, dY1, nullptr);
+    std::vector<double> Y(size_t(nq) * LL), dY(size_t(nq) * 3 * LL);
+    RGPU_HIP(hipMemcpy(Y.data(), dY0, sizeof(double) * Y.size(), hipMemcpyDeviceToHost));
+    RGPU_HIP(hipMemcpy(dY.data(), dY1, sizeof(double) * dY.size(), hipMemcpyDeviceToHost));
+    std::vector<YGradTerm<double>> g64(size_t(3) * LL, YGradTerm<double>{0.0, 0.0, 0, 0});
+    for (int l = 1; l <= L; ++l)
+      for (int lm = l * l; lm < (l + 1) * (l + 1); ++lm)
+        for (int cc = 0; cc < 3; ++cc) {
+          std::vector<std::pair<double, int>> terms;
+          double amax = 0;
+          std::vector<double> a(2 * l - 1);
+          for (int k = 0; k < 2 * l - 1; ++k) {
+            const int lp = (l - 1) * (l - 1) + k;
+            double num = 0, den = 0;
+            for (int q = 0; q < nq; ++q) {
+              const double f = dY[(size_t(q) * 3 + cc) * LL + lm] + l * dirs[3 * q + cc] * Y[size_t(q) * LL + lm];
+              num += wq[q] * f * Y[size_t(q) * LL + lp];
+              den += wq[q] * Y[size_t(q) * LL + lp] * Y[size_t(q) * LL + lp];
+            }
+            a[k] = num / den;
+            amax = std::max(amax, std::fabs(a[k]));
+          }
+          for (int k = 0; k < 2 * l - 1; ++k)
+            // the nonzero coefficients are O(1) and above (d_z (x +- iy)^l is
+            // exactly zero), so round-off is cut on an absolute scale
+            if (std::fabs(a[k]) > 1e-9 * std::max(amax, 1.0)) terms.push_back({a[k], (l - 1) * (l - 1) + k});
+          if (terms.size() > 2) throw RuntimeError("Y_lm gradient identity: more than two terms");
+          YGradTerm<double>& e = g64[size_t(lm) * 3 + cc];
+          if (terms.size() > 0) e.c0 = terms[0].first, e.i0 = terms[0].second;
+          if (terms.size() > 1) e.c1 = terms[1].first, e.i1 = terms[1].second;
+          // the identity reproduces the gradients at every quadrature point
+          double err = 0, ref = 0;
+          for (int q = 0; q < nq; ++q) {
+            const double* yq = &Y[size_t(q) * LL];
+            const double v = e.c0 * yq[e.i0] + e.c1 * yq[e.i1] - l * dirs[3 * q + cc] * yq[lm];
+            const double d = dY[(size_t(q) * 3 + cc) * LL + lm];
+            err = std::max(err, std::fabs(v - d));
+            ref = std::max(ref, std::fabs(d));
+          }
+          if (err > 1e-11 * std::max(ref, 1.0)) throw RuntimeError("Y_lm gradient identity does not hold");
+        }
+    std::vector<YGradTerm<float>> g32(g64.size());
+    for (size_t k = 0; k < g64.size(); ++k)
+      g32[k] = YGradTerm<float>{float(g64[k].c0), float(g64[k].c1), g64[k].i0, g64[k].i1};
+    r.ygrad64 = up(c.d_ygrad64, g64);
+    r.ygrad32 = up(c.d_ygrad32, g32);
   }
   r.use_spline = 0;
   r.n_tabs = h.n_tabs;

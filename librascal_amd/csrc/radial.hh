@@ -30,6 +30,16 @@ struct HypCoef {
   int is_n_and_l;   // 2(a-b) integer
 };
 
+// Gradient of Y_lm on the unit sphere (no 1/r factor) from the harmonics of
+// degree l - 1: the gradient of r^l Y_lm is a harmonic polynomial of degree
+// l - 1, so dY_c[lm] = c0 Y[i0] + c1 Y[i1] - l r_c Y[lm] with constant
+// coefficients (at most two terms per component for the real harmonics).
+template <class T>
+struct YGradTerm {
+  T c0, c1;
+  int i0, i1;
+};
+
 // Everything a kernel needs to evaluate the radial part, passed by value.
 struct RadialDev {
   int N, L, L1, LL;       // max_radial, max_angular, L+1, (L+1)^2
@@ -78,6 +88,9 @@ struct RadialDev {
   // sqrt(l(l+1)) ([L1] each); see ylm_coefficients()
   const double* ylm64;
   const float* ylm32;
+  // [LL][3] gradient terms (YGradTerm), FP64 and FP32
+  const YGradTerm<double>* ygrad64;
+  const YGradTerm<float>* ygrad32;
 };
 
 __host__ __device__ inline int ylm_table_size(int L1) { return 4 * L1 * L1 + 3 * L1; }
@@ -119,5 +132,9 @@ void spline_second_derivatives(int n, int F, double h, const std::vector<double>
 
 // Eigen 3.3 LinSpaced for floating point (used for the spline grids).
 std::vector<double> linspaced(int num, double lo, double hi);
+
+// Gauss-Legendre points (ascending) and weights on [r0, r1]
+void gauss_legendre(double r0, double r1, int order_n, std::vector<double>& x,
+                    std::vector<double>& w);
 
 }  // namespace rgpu

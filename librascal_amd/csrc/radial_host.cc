@@ -233,7 +233,7 @@ static void fill_hyp(RadialHost& h, int idx, double a, double b, double tol) {
 // (math/gauss_legendre.cc:30-73: Newton iteration on the Legendre
 // three-term recurrence from the starting guesses cos(pi (i + 3/4) / (n +
 // 1/2)), the symmetric half mirrored)
-static void gauss_legendre(double r0, double r1, int order_n, std::vector<double>& x,
+void gauss_legendre(double r0, double r1, int order_n, std::vector<double>& x,
                     std::vector<double>& w) {
   x.assign(order_n, 0.0);
   w.assign(order_n, 0.0);
@@ -241,7 +241,10 @@ static void gauss_legendre(double r0, double r1, int order_n, std::vector<double
   const int m = (order_n + 1) / 2;
   for (int i = 0; i < m; ++i) {
     double z = std::cos(3.14159265358979323846 * (i + 0.75) / (order_n + 0.5)), z1 = 0, pp = 0;
-    while (std::fabs(z - z1) > kDblFtol) {
+    // at least one Newton step: for odd orders the middle guess cos(pi/2)
+    // already sits within the tolerance of z1 = 0, and skipping the step
+    // would leave pp = 0 (an infinite weight)
+    do {
       double p1 = 1, p2 = 0;
       for (int j = 0; j < order_n; ++j) {
         const double p3 = p2;
@@ -251,7 +254,7 @@ static void gauss_legendre(double r0, double r1, int order_n, std::vector<double
       pp = order_n * (z * p1 - p2) / (z * z - 1.0);
       z1 = z;
       z = z1 - p1 / pp;
-    }
+    } while (std::fabs(z - z1) > kDblFtol);
     x[i] = mid - half * z;
     x[order_n - 1 - i] = mid + half * z;
     w[i] = (2.0 * half) / ((1.0 - z * z) * pp * pp);

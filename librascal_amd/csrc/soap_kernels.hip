@@ -84,6 +84,16 @@ __host__ __device__ inline size_t ylm_table_bytes(int L1) {
 }
 template <class T>
 __device__ __forceinline__ const T* ylm_table_src(const RadialDev& R);
+template <class T>
+__device__ __forceinline__ const YGradTerm<T>* ygrad_src(const RadialDev& R);
+template <>
+__device__ __forceinline__ const YGradTerm<double>* ygrad_src<double>(const RadialDev& R) {
+  return R.ygrad64;
+}
+template <>
+__device__ __forceinline__ const YGradTerm<float>* ygrad_src<float>(const RadialDev& R) {
+  return R.ygrad32;
+}
 template <>
 __device__ __forceinline__ const double* ylm_table_src<double>(const RadialDev& R) {
   return R.ylm64;
@@ -233,8 +243,9 @@ struct PairBatch {
   double* vec;   // [PB][3] pair vectors (adjoint)
   double* raw;   // [PB][F] raw radial integrals (direct Hyp1f1)
   double* draw;  // [PB][F] their derivatives (direct, adjoint)
+  YGradTerm<T>* gt;  // [LL][3] gradient terms of Y_lm (adjoint)
   T* Y;          // [PB][LL]
-  T* dY;         // [PB][3][LL] (adjoint)
+  T* dY;         // [PB][3][LL] (not staged: the adjoint uses gt)
   T* R;          // [PB][F] finalised radial functions (x fc in the forward)
   T* dR;         // [PB][F] (adjoint)
   T* dir;        // [PB][3]
@@ -250,10 +261,11 @@ struct PairBatch {
 
 __host__ __device__ inline size_t pair_batch_bytes(int PB, int LL, int F, int tsize, bool direct,
                                                    bool adjoint) {
-  size_t b = 8 * size_t(PB) * (adjoint ? 6 : 2);
+  // the adjoint's gradient-term table {T c0, c1; int i0, i1}[LL][3] first
+  size_t b = adjoint ? align_up(size_t(LL) * 3 * (2 * size_t(tsize) + 8), 64) : 0;
+  b += 8 * size_t(PB) * (adjoint ? 6 : 2);
   if (direct) b += 8 * size_t(PB) * F * (adjoint ? 2 : 1);
-  b += size_t(tsize) * PB *
-       (size_t(LL) * (adjoint ? 4 : 1) + size_t(F) * (adjoint ? 2 : 1) + 6 + (adjoint ? 4 : 0));
+  b += size_t(tsize) * PB * (size_t(LL) + size_t(F) * (adjoint ? 2 : 1) + 6 + (adjoint ? 4 : 0));
   b = align_up(b, 8);
   b += 4 * size_t(PB) * (adjoint ? 4 : 3);
   return align_up(b, 64);
@@ -262,6 +274,11 @@ __host__ __device__ inline size_t pair_batch_bytes(int PB, int LL, int F, int ts
 template <class T, bool DIRECT, bool ADJ>
 __device__ PairBatch<T> carve_batch(unsigned char* base, int PB, int LL, int F) {
   PairBatch<T> pb;
+  pb.gt = nullptr;
+  if (ADJ) {
+    pb.gt = reinterpret_cast<YGradTerm<T>*>(base);
+    base += align_up(size_t(LL) * 3 * sizeof(YGradTerm<T>), 64);
+  }
   double* d = reinterpret_cast<double*>(base);
   pb.dist = d;
   d += PB;
@@ -287,10 +304,6 @@ __device__ PairBatch<T> carve_batch(unsigned char* base, int PB, int LL, int F) 
   pb.Y = t;
   t += PB * LL;
   pb.dY = nullptr;
-  if (ADJ) {
-    pb.dY = t;
-    t += 3 * PB * LL;
-  }
   pb.R = t;
   t += PB * F;
   pb.dR = nullptr;
@@ -757,6 +770,12 @@ __device__ __forceinline__ void adjoint_pairs(const RadialDev& rad, int S, int P
   const bool vec4 = (N & 3) == 0;
   typedef typename Vec4<T>::type V4;
   double fi[3] = {0, 0, 0}, v6[6] = {0, 0, 0, 0, 0, 0};
+  // gradient terms of Y_lm (ordered before their first use by the barriers
+  // of the first batch's staging)
+  {
+    const YGradTerm<T>* src = ygrad_src<T>(rad);
+    for (int e = tid; e < 3 * LL; e += kThreads) pb.gt[e] = src[e];
+  }
   // the inputs of each batch are fetched while the previous one is
   // contracted
   PairPre pre;
@@ -768,11 +787,13 @@ __device__ __forceinline__ void adjoint_pairs(const RadialDev& rad, int S, int P
     for (int qb = gpw * wave; qb < nb; qb += gpw * kWaves) {
       const int q = qb + grp;
       const bool act = q < nb;
-      T A = T(0), Bx = T(0), By = T(0), Bz = T(0);
+      // B_c = sum_lm D[lm] dY_c[lm] with dY_c[lm] = sum of the gradient
+      // terms of Y_{l-1} - l r_c Y[lm]: the last part is summed once as
+      // Sl = sum_lm l D[lm] Y[lm]
+      T A = T(0), Bx = T(0), By = T(0), Bz = T(0), Sl = T(0);
       if (act) {
         const T* W = dc + pb.sp[q] * B;
         const T* Yq = pb.Y + q * LL;
-        const T* dYq = pb.dY + q * 3 * LL;
         const T* Rq = pb.R + q * F;
         const T* dRq = pb.dR + q * F;
         const T fc = T(pb.fc[q]), dfc = T(pb.dfc[q]);
@@ -796,10 +817,13 @@ __device__ __forceinline__ void adjoint_pairs(const RadialDev& rad, int S, int P
               Ddn += w[n] * dr[n];
             }
           }
-          A += (Ddn * fc + Dn * dfc) * Yq[lm];
-          Bx += Dn * dYq[lm];
-          By += Dn * dYq[LL + lm];
-          Bz += Dn * dYq[2 * LL + lm];
+          const T y = Yq[lm];
+          A += (Ddn * fc + Dn * dfc) * y;
+          Sl += T(l) * Dn * y;
+          const YGradTerm<T>* g = pb.gt + 3 * lm;
+          Bx += Dn * (g[0].c0 * Yq[g[0].i0] + g[0].c1 * Yq[g[0].i1]);
+          By += Dn * (g[1].c0 * Yq[g[1].i0] + g[1].c1 * Yq[g[1].i1]);
+          Bz += Dn * (g[2].c0 * Yq[g[2].i0] + g[2].c1 * Yq[g[2].i1]);
         }
       }
       for (int off = GS >> 1; off > 0; off >>= 1) {
@@ -807,12 +831,13 @@ __device__ __forceinline__ void adjoint_pairs(const RadialDev& rad, int S, int P
         Bx += __shfl_xor(Bx, off, GS);
         By += __shfl_xor(By, off, GS);
         Bz += __shfl_xor(Bz, off, GS);
+        Sl += __shfl_xor(Sl, off, GS);
       }
       if (act && gl == 0) {
         pb.res[4 * q] = A;
-        pb.res[4 * q + 1] = Bx;
-        pb.res[4 * q + 2] = By;
-        pb.res[4 * q + 3] = Bz;
+        pb.res[4 * q + 1] = Bx - pb.dir[3 * q] * Sl;
+        pb.res[4 * q + 2] = By - pb.dir[3 * q + 1] * Sl;
+        pb.res[4 * q + 3] = Bz - pb.dir[3 * q + 2] * Sl;
       }
     }
     __syncthreads();

@@ -238,16 +238,19 @@ def test_dvr_expansion_reference(row):
     assert rel_err(feats, X).max() < 1e-8
 
 
-@pytest.mark.parametrize("spline", [None, 1e-8])
-def test_dvr_gradients_vs_finite_differences(spline):
+@pytest.mark.parametrize("spline,N", [(None, 6), (1e-8, 6), (None, 5)])
+def test_dvr_gradients_vs_finite_differences(spline, N):
     """The adjoint with the DVR radial derivatives (Bessel-function gradient
     recursion): dL/dr for L = sum G . X of the normalised power spectrum
-    against central differences (h = 1e-5 A) of the GPU features."""
-    hp = soap_hypers(rc=3.5, sw=0.5, sigma=0.4, N=6, L=4, spline=spline)
+    against central differences (h = 1e-5 A) of the GPU features.  N = 5 has
+    a quadrature node at the middle of the interval, where a Newton iteration
+    that stops before its first step leaves an infinite weight."""
+    hp = soap_hypers(rc=3.5, sw=0.5, sigma=0.4, N=N, L=4, spline=spline)
     hp["radial_contribution"]["type"] = "DVR"
     sts = golden_structures()["small_molecule"]
     ctx, _ = make_ctx(hp, sts)
     X, _ = ctx.features()
+    assert np.isfinite(X).all()
     G = np.random.default_rng(3).normal(size=X.shape)
     g, _ = ctx.features_backward(G)
     pos = pack(sts)[1]
