@@ -930,7 +930,7 @@ void autotune_t(Context& c) {
     double* n2 = c.norm2.get();
     float best = 1e30f;
     int pick = 0;
-    for (int kc = 1; kc <= std::min(K, 8); ++kc) {
+    for (int kc = 1; kc <= std::min(K, 12); ++kc) {
       SoapTiles cand = t;
       cand.kc = kc;
       float ms;

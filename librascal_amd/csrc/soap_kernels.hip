@@ -585,7 +585,11 @@ __global__ __launch_bounds__(kThreads, (expansion_min_waves<T, KPT, DIRECT>())) 
 // sum_m c_a[n1,lm] c_b[n2,lm] (calculator_spherical_invariants.hh:747-788),
 // 4x4 register tiles over (n1, n2) per (key, l); KC keys at a time are
 // staged in LDS and written out contiguously.  The expansion rows sit in
-// LDS with an odd stride (no bank conflicts between the tile rows).
+// LDS with an odd stride (no bank conflicts between the tile rows), followed
+// by three zero rows, so that a tile reads its four rows unconditionally
+// (rows past N of a species read the next species' rows: those products
+// are never stored).  Work items run l-major, so the lanes of a wavefront
+// share one or two l and loop over the same number of m.
 // Un-normalised; |p|^2 is returned per centre.
 template <class T, bool SPLIT>
 __global__ __launch_bounds__(kThreads) void power_spectrum_kernel(int N, int L, int S, int KC, int64_t Dq,
@@ -600,8 +604,8 @@ __global__ __launch_bounds__(kThreads) void power_spectrum_kernel(int N, int L, 
   const int NT = (N + 3) / 4;
   const int KB = N * N * L1;  // one key block
   extern __shared__ __align__(64) unsigned char smem_raw[];
-  T* cs = reinterpret_cast<T*>(smem_raw);  // [S][N][LS]
-  T* stage = cs + S * N * LS;              // [KC][N][N][L1]
+  T* cs = reinterpret_cast<T*>(smem_raw);  // [S][N][LS], then 3 zero rows
+  T* stage = cs + (S * N + 3) * LS;        // [KC][N][N][L1]
   int* key_ab = reinterpret_cast<int*>(stage + KC * KB);  // [K] a << 8 | b
   int* tri = key_ab + K;  // [N(N+1)/2] n1 << 8 | n2 of the packed diagonal pairs
   __shared__ double red[kWaves];
@@ -611,6 +615,7 @@ __global__ __launch_bounds__(kThreads) void power_spectrum_kernel(int N, int L, 
     const int row = e / fLL;
     cs[row * LS + (e - row * LL)] = ci[e];
   }
+  for (int e = threadIdx.x; e < 3 * LS; e += kThreads) cs[S * N * LS + e] = T(0);
   for (int k = threadIdx.x; k < K; k += kThreads) {
     int a, b;
     key_species(k, S, &a, &b);
@@ -624,18 +629,20 @@ __global__ __launch_bounds__(kThreads) void power_spectrum_kernel(int N, int L, 
     }
   __syncthreads();
   const int64_t pbase = i * int64_t(K) * KB;
-  const FastDiv fNT = fast_div(NT), fL1 = fast_div(L1);
+  const FastDiv fNT = fast_div(NT), fL1 = fast_div(L1), fT = fast_div(NT * NT);
   double ss = 0;
   for (int k0 = 0; k0 < K; k0 += KC) {
     const int kc = min(KC, K - k0);
-    const int items = kc * L1 * NT * NT;
+    const int per_l = kc * NT * NT;
+    const FastDiv fPL = fast_div(per_l);
+    const int items = L1 * per_l;
     for (int it = threadIdx.x; it < items; it += kThreads) {
-      const int r = it / fNT;
-      const int t2 = it - r * NT;
-      const int r2 = r / fNT;
-      const int t1 = r - r2 * NT;
-      const int kk = r2 / fL1;
-      const int l = r2 - kk * L1;
+      const int l = it / fPL;
+      const int r = it - l * per_l;
+      const int kk = r / fT;
+      const int t = r - kk * (NT * NT);
+      const int t1 = t / fNT;
+      const int t2 = t - t1 * NT;
       const int ab = key_ab[k0 + kk];
       const int a = ab >> 8, b = ab & 255;
       // the packed output of a diagonal key needs n1 <= n2 only
@@ -652,8 +659,8 @@ __global__ __launch_bounds__(kThreads) void power_spectrum_kernel(int N, int L, 
         T va[4], vb[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
-          va[u] = (n1 + u < N) ? ca[(n1 + u) * LS + lm] : T(0);
-          vb[u] = (n2 + u < N) ? cb[(n2 + u) * LS + lm] : T(0);
+          va[u] = ca[(n1 + u) * LS + lm];
+          vb[u] = cb[(n2 + u) * LS + lm];
         }
 #pragma unroll
         for (int u = 0; u < 4; ++u)
@@ -1542,7 +1549,7 @@ void launch_power_spectrum_impl(const RadialDev& rad, int S, int64_t n, const T*
   if (n == 0) return;
   const int K = S * (S + 1) / 2;
   if (S > kMaxSpecies) throw InvalidArgument("at most 64 species");
-  const size_t cbytes = sizeof(T) * size_t(S) * rad.N * (rad.LL | 1);
+  const size_t cbytes = sizeof(T) * (size_t(S) * rad.N + 3) * (rad.LL | 1);
   const size_t kb = sizeof(T) * size_t(rad.N) * rad.N * rad.L1;
   int KC = cbytes + kb < kPsBudget ? int((kPsBudget - cbytes) / kb) : 1;
   if (tiles.kc > 0) KC = tiles.kc;
