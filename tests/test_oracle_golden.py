@@ -38,6 +38,40 @@ def test_spherical_harmonics_reference():
         assert np.linalg.norm(alp[:, :L + 1] - np.asarray(alps).reshape(L + 1, L + 1)) <= 10 * DBL_FTOL
 
 
+@pytest.mark.parametrize("L", [1, 4, 9])
+def test_ylm_gradient_two_term_identity(L):
+    """The adjoint's Y_lm gradients: r^l Y_lm is a harmonic polynomial, so its
+    gradient is one of degree l-1 and, on the unit sphere, dY_c[lm] =
+    sum_m' a^c_{lm,m'} Y_{l-1,m'} - l r_c Y_lm with at most two nonzero
+    a^c_{lm,m'} per (lm, c).  The coefficients are projected with Gauss-Legendre
+    (L + 2 points in cos theta) times 2L + 3 uniform points in phi, exact for
+    these products, from the oracle's harmonics and gradients; the identity
+    must then reproduce the gradients at random directions."""
+    nt, nph = L + 2, 2 * L + 3
+    ct, wt = np.polynomial.legendre.leggauss(nt)
+    ph = 2 * np.pi * (np.arange(nph) + 0.5) / nph
+    st = np.sqrt(1 - ct ** 2)
+    dirs = np.stack([np.outer(st, np.cos(ph)).ravel(), np.outer(st, np.sin(ph)).ravel(),
+                     np.repeat(ct, nph)], axis=1)
+    wq = np.repeat(wt, nph) * 2 * np.pi / nph
+    Y, dY = zip(*[O.ylm(L, d, derivatives=True)[:2] for d in dirs])
+    Y, dY = np.array(Y), np.array(dY)
+    test_dirs = np.random.default_rng(0).normal(size=(50, 3))
+    test_dirs /= np.linalg.norm(test_dirs, axis=1)[:, None]
+    Yt, dYt = zip(*[O.ylm(L, d, derivatives=True)[:2] for d in test_dirs])
+    Yt, dYt = np.array(Yt), np.array(dYt)
+    for l in range(1, L + 1):
+        lo = (l - 1) ** 2
+        for lm in range(l * l, (l + 1) ** 2):
+            for c in range(3):
+                f = dY[:, c, lm] + l * dirs[:, c] * Y[:, lm]
+                a = (wq * f) @ Y[:, lo:l * l] / (wq @ Y[:, lo:l * l] ** 2)
+                a[np.abs(a) < 1e-9] = 0.0
+                assert np.count_nonzero(a) <= 2
+                v = Yt[:, lo:l * l] @ a - l * test_dirs[:, c] * Yt[:, lm]
+                assert np.max(np.abs(v - dYt[:, c, lm])) < 1e-12 * max(1.0, np.abs(dYt[:, c, lm]).max())
+
+
 def _expansion_rows():
     rows = json.load(open(os.path.join(GOLDEN, "spherical_expansion_reference.json")))
     return [r for r in rows]
