@@ -55,33 +55,31 @@ __global__ void ghosts_kernel(const NLGeom* __restrict__ geom, const int* __rest
   const double x0 = pos[3 * a], x1 = pos[3 * a + 1], x2 = pos[3 * a + 2];
   int c = 0;
   int64_t w = WRITE ? off[a] + n_atoms : 0;
-  for (int idx = 0; idx < g.n_images; ++idx) {
-    int p[3];
-    int f = 1;
-    for (int i = 2; i >= 0; --i) {
-      p[i] = idx / f % g.rep[i] + g.pmin[i];
-      f *= g.rep[i];
-    }
-    if (p[0] == 0 && p[1] == 0 && p[2] == 0) continue;
-    double xg[3];
-    const double xs[3] = {x0, x1, x2};
-    bool inside = true;
-    for (int d = 0; d < 3; ++d) {
-      double t = g.cell[d * 3 + 0] * double(p[0]);
-      t = t + g.cell[d * 3 + 1] * double(p[1]);
-      t = t + g.cell[d * 3 + 2] * double(p[2]);
-      xg[d] = xs[d] + t;
-      if (!((xg[d] - g.ghost_min[d]) > g.tol && (xg[d] - g.ghost_max[d]) < -g.tol)) inside = false;
-    }
-    if (!inside) continue;
-    if (WRITE && w + c < cap) {
-      ext_pos[3 * (w + c)] = xg[0];
-      ext_pos[3 * (w + c) + 1] = xg[1];
-      ext_pos[3 * (w + c) + 2] = xg[2];
-      ext_orig[w + c] = int(a);
-    }
-    ++c;
-  }
+  // images in the order of the linear index ((p0 - pmin0) rep1 + p1 - pmin1)
+  // rep2 + p2 - pmin2, p2 fastest
+  const double xs[3] = {x0, x1, x2};
+  for (int p0 = g.pmin[0]; p0 < g.pmin[0] + g.rep[0]; ++p0)
+    for (int p1 = g.pmin[1]; p1 < g.pmin[1] + g.rep[1]; ++p1)
+      for (int p2 = g.pmin[2]; p2 < g.pmin[2] + g.rep[2]; ++p2) {
+        if (p0 == 0 && p1 == 0 && p2 == 0) continue;
+        double xg[3];
+        bool inside = true;
+        for (int d = 0; d < 3; ++d) {
+          double t = g.cell[d * 3 + 0] * double(p0);
+          t = t + g.cell[d * 3 + 1] * double(p1);
+          t = t + g.cell[d * 3 + 2] * double(p2);
+          xg[d] = xs[d] + t;
+          if (!((xg[d] - g.ghost_min[d]) > g.tol && (xg[d] - g.ghost_max[d]) < -g.tol)) inside = false;
+        }
+        if (!inside) continue;
+        if (WRITE && w + c < cap) {
+          ext_pos[3 * (w + c)] = xg[0];
+          ext_pos[3 * (w + c) + 1] = xg[1];
+          ext_pos[3 * (w + c) + 2] = xg[2];
+          ext_orig[w + c] = int(a);
+        }
+        ++c;
+      }
   if (!WRITE) count[a] = c;
 }
 
@@ -133,11 +131,30 @@ __global__ void bin_fill_kernel(const int64_t* __restrict__ ext_box, int64_t n_a
 }
 
 // ascending slot order within each box = the reference's tag order
-__global__ void bin_sort_kernel(const int64_t* __restrict__ box_start, int64_t n_boxes,
-                                int* __restrict__ box_items) {
+// boxes of up to kSortSlot items are sorted in LDS (element k of thread t at
+// k * kSortThreads + t), larger ones in place in global memory
+constexpr int kSortThreads = 64, kSortSlot = 32;
+__global__ __launch_bounds__(kSortThreads) void bin_sort_kernel(const int64_t* __restrict__ box_start,
+                                                                int64_t n_boxes, int* __restrict__ box_items) {
+  __shared__ int buf[kSortThreads * kSortSlot];
   int64_t bx = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (bx >= n_boxes) return;
   int64_t s = box_start[bx], e = box_start[bx + 1];
+  if (e - s <= kSortSlot) {
+    const int n = int(e - s), t = threadIdx.x;
+    for (int k = 0; k < n; ++k) buf[k * kSortThreads + t] = box_items[s + k];
+    for (int i = 1; i < n; ++i) {
+      const int v = buf[i * kSortThreads + t];
+      int j = i - 1;
+      while (j >= 0 && buf[j * kSortThreads + t] > v) {
+        buf[(j + 1) * kSortThreads + t] = buf[j * kSortThreads + t];
+        --j;
+      }
+      buf[(j + 1) * kSortThreads + t] = v;
+    }
+    for (int k = 0; k < n; ++k) box_items[s + k] = buf[k * kSortThreads + t];
+    return;
+  }
   for (int64_t i = s + 1; i < e; ++i) {
     int v = box_items[i];
     int64_t j = i - 1;
@@ -396,7 +413,9 @@ void launch_pass(PairList& pl, double rc, hipStream_t stream) {
   // ghosts
   int* gcount = pl.ghost_count.get();
   int64_t* goff = pl.ghost_off.get();
-  hipLaunchKernelGGL(ghosts_kernel<false>, blocks(n_atoms, TB), dim3(TB), 0, stream, dgeom, dstruct,
+  // one thread per atom: small workgroups spread the atoms over more CUs
+  const int TG = 64;
+  hipLaunchKernelGGL(ghosts_kernel<false>, blocks(n_atoms, TG), dim3(TG), 0, stream, dgeom, dstruct,
                      dpos, n_atoms, gcount, nullptr, nullptr, nullptr, int64_t(0));
   RGPU_LAUNCH_CHECK();
   exclusive_scan_device<int>(gcount, goff, n_atoms, stream, pl.scratch);
@@ -405,7 +424,7 @@ void launch_pass(PairList& pl, double rc, hipStream_t stream) {
   hipLaunchKernelGGL(originals_kernel, blocks(n_atoms, TB), dim3(TB), 0, stream, dpos, n_atoms, epos,
                      eorig);
   RGPU_LAUNCH_CHECK();
-  hipLaunchKernelGGL(ghosts_kernel<true>, blocks(n_atoms, TB), dim3(TB), 0, stream, dgeom, dstruct,
+  hipLaunchKernelGGL(ghosts_kernel<true>, blocks(n_atoms, TG), dim3(TG), 0, stream, dgeom, dstruct,
                      dpos, n_atoms, nullptr, goff, epos, eorig, cap_ext);
   RGPU_LAUNCH_CHECK();
   // binning
@@ -425,7 +444,8 @@ void launch_pass(PairList& pl, double rc, hipStream_t stream) {
   hipLaunchKernelGGL(bin_fill_kernel, blocks(cap_ext, TB), dim3(TB), 0, stream, ebox, n_atoms,
                      goff + n_atoms, cap_ext, bstart, bfill, bitems);
   RGPU_LAUNCH_CHECK();
-  hipLaunchKernelGGL(bin_sort_kernel, blocks(boxes, TB), dim3(TB), 0, stream, bstart, boxes, bitems);
+  hipLaunchKernelGGL(bin_sort_kernel, blocks(boxes, kSortThreads), dim3(kSortThreads), 0, stream, bstart, boxes,
+                     bitems);
   RGPU_LAUNCH_CHECK();
   // pairs
   int* pcount = pl.pair_count.get();

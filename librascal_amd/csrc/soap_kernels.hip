@@ -1305,12 +1305,19 @@ __global__ __launch_bounds__(kSegThreads) void segment_sum_kernel(const double* 
   if (s >= n_seg) return;
   __shared__ double red[kSegThreads / 64][8];
   double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  // unrolled over the largest width (uniform guards): v stays in registers
   for (int64_t a = seg_off[s] + threadIdx.x; a < seg_off[s + 1]; a += kSegThreads)
-    for (int k = 0; k < w; ++k) v[k] += in[int64_t(w) * a + k];
-  for (int k = 0; k < w; ++k)
-    for (int off = 32; off > 0; off >>= 1) v[k] += __shfl_down(v[k], off, 64);
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+      if (k < w) v[k] += in[int64_t(w) * a + k];
+#pragma unroll
+  for (int k = 0; k < 8; ++k)
+    if (k < w)
+      for (int off = 32; off > 0; off >>= 1) v[k] += __shfl_down(v[k], off, 64);
   if ((threadIdx.x & 63) == 0)
-    for (int k = 0; k < w; ++k) red[threadIdx.x >> 6][k] = v[k];
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+      if (k < w) red[threadIdx.x >> 6][k] = v[k];
   __syncthreads();
   if (threadIdx.x < w) {
     double t = 0;
@@ -1686,6 +1693,7 @@ void launch_pair_cutoff(const RadialDev& rad, const int64_t* n_pairs, int64_t ca
 void launch_segment_sum(const double* in, int w, const int64_t* seg_off, int n_seg, double* out,
                         hipStream_t stream, const double* scale) {
   if (n_seg == 0) return;
+  if (w < 1 || w > 8) throw InvalidArgument("segment sum: row width must be 1 to 8");
   hipLaunchKernelGGL(segment_sum_kernel, dim3(unsigned(n_seg)), dim3(kSegThreads), 0, stream, in, w, seg_off,
                      n_seg, scale, out);
   RGPU_LAUNCH_CHECK();
