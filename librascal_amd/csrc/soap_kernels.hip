@@ -408,7 +408,7 @@ __device__ void stage_pairs(const RadialDev& rad, const T* yc, const PairPre& f,
       b.klo[tid] = f.klo;
       b.ca[tid] = T(a);
       b.cb[tid] = T(1 - a);
-      b.ih[tid] = T(1.0 / f.h);
+      if (ADJ && rad.sp_tab) b.ih[tid] = T(1.0 / f.h);
     }
   }
   __syncthreads();
@@ -442,6 +442,9 @@ __device__ void stage_pairs(const RadialDev& rad, const T* yc, const PairPre& f,
   if (!DIRECT) {
     if (tid >= r0) {
       const FastDiv fF = fast_div(F), fL1 = fast_div(L1);
+      // one table: a single grid step (per-species tables keep theirs per pair)
+      const T inv_h = T(1.0 / rad.h);
+      const bool per_table = rad.sp_tab != nullptr;
 #pragma unroll 4
       for (int it = tid - r0; it < nb * F; it += rstride) {
         const int q = it / fF, f = it - q * F;
@@ -449,7 +452,7 @@ __device__ void stage_pairs(const RadialDev& rad, const T* yc, const PairPre& f,
         cell.klo = b.klo[q];
         cell.a = b.ca[q];
         cell.b = b.cb[q];
-        cell.inv_h = b.ih[q];
+        cell.inv_h = per_table ? b.ih[q] : inv_h;
         T v, dv;
         spline_eval<T>(rad, cell, f, &v, ADJ ? &dv : nullptr);
         if (ADJ) {
