@@ -113,6 +113,18 @@ def ptr(a):
     return C.c_void_p(a.data_ptr())
 
 
+def _check_out(a, shape):
+    """Output arrays and tensors: float64, contiguous, of the given shape."""
+    if tuple(a.shape) != tuple(shape):
+        raise ValueError("output of shape %s expected, got %s" % (tuple(shape), tuple(a.shape)))
+    if isinstance(a, np.ndarray):
+        ok = a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+    else:
+        ok = str(a.dtype) == "torch.float64" and a.is_contiguous()
+    if not ok:
+        raise ValueError("outputs must be contiguous float64")
+
+
 class Context:
     """Thin owner of an rgpu_context (one calculator + batch + GAP model)."""
 
@@ -197,17 +209,25 @@ class Context:
         check(lib().rgpu_get_expansion(self._p, ptr(c), ptr(km)))
         return c, km
 
-    def features(self):
+    def features(self, out=None):
+        """(X [n_centres][D], inverse norms [n_centres]).  With out (a float64
+        array or tensor of that shape, host or device) X is written there and
+        the norms are not returned."""
         n = self.info("n_centres")
         D = self.info("feature_size")
+        if out is not None:
+            _check_out(out, (n, D))
+            check(lib().rgpu_get_features(self._p, ptr(out), None))
+            return out, None
         X = np.zeros((n, D))
         inv = np.zeros(n)
         check(lib().rgpu_get_features(self._p, ptr(X), ptr(inv)))
         return X, inv
 
-    def features_backward(self, dL_dX, want_stress=True):
+    def features_backward(self, dL_dX, want_stress=True, out=None):
         """dL/dr [n_atoms][3] (and the negative virial) for dL/dX given as a
-        host array or a contiguous float64 torch tensor (host or device)."""
+        host array or a contiguous float64 torch tensor (host or device);
+        dL/dr goes into out when given (float64, host or device)."""
         n = self.info("n_centres")
         ns = self.info("n_struct")
         if hasattr(dL_dX, "data_ptr"):
@@ -216,7 +236,10 @@ class Context:
                 raise ValueError("dL/dX tensors must be contiguous float64")
         else:
             g = np.ascontiguousarray(dL_dX, np.float64)
-        out = np.zeros((n, 3))
+        if out is None:
+            out = np.zeros((n, 3))
+        else:
+            _check_out(out, (n, 3))
         st = np.zeros((ns, 6)) if want_stress else None
         check(lib().rgpu_features_backward(self._p, ptr(g), ptr(out), ptr(st)))
         return out, st
@@ -247,12 +270,21 @@ class Context:
         check(lib().rgpu_gap_kmm(self._p, ptr(k)))
         return k
 
-    def gap_predict(self, energy=True, gradient=True, neg_stress=True):
+    def gap_predict(self, energy=True, gradient=True, neg_stress=True, out=None):
+        """(energy [n_struct], gradient [n_atoms][3], neg_stress [n_struct][6]),
+        None for the outputs not requested; out = (e, g, s) gives the
+        destination arrays or tensors (float64, host or device, None to skip)."""
         ns = self.info("n_struct")
         n = self.info("n_centres")
-        e = np.zeros(ns) if energy else None
-        g = np.zeros((n, 3)) if gradient else None
-        s = np.zeros((ns, 6)) if neg_stress else None
+        if out is not None:
+            e, g, s = out
+            for a, shape in ((e, (ns,)), (g, (n, 3)), (s, (ns, 6))):
+                if a is not None:
+                    _check_out(a, shape)
+        else:
+            e = np.zeros(ns) if energy else None
+            g = np.zeros((n, 3)) if gradient else None
+            s = np.zeros((ns, 6)) if neg_stress else None
         check(lib().rgpu_gap_predict(self._p, ptr(e), ptr(g), ptr(s)))
         return e, g, s
 

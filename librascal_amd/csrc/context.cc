@@ -25,6 +25,9 @@ namespace rgpu {
 
 namespace {
 thread_local std::string g_last_error;
+const char* const kNoSparse =
+    "no sparse points set (sparse points are dropped when a batch with another species list is "
+    "set)";
 }
 
 struct Context {
@@ -1168,6 +1171,10 @@ int rgpu_set_structures(rgpu_context* ctx, int n_struct, const int64_t* atom_off
         sp_tab.push_back(int(it - c.rh->tab_species.begin()));
       }
     }
+    // sparse points are laid out over the species list they were set with
+    // (columns per key, rows grouped by species): any other list drops them
+    const bool species_changed = c.species != sp;
+    if (species_changed) c.have_sparse = false;
     c.n_struct = n_struct;
     c.n_atoms = n_atoms;
     c.atom_off.assign(atom_offsets, atom_offsets + n_struct + 1);
@@ -1197,8 +1204,6 @@ int rgpu_set_structures(rgpu_context* ctx, int n_struct, const int64_t* atom_off
     build_rows(c);
     RGPU_HIP(hipStreamSynchronize(c.stream));
     c.have_batch = true;
-    if (c.have_sparse && (int64_t(c.sp_off.size()) != int64_t(c.species.size()) + 1))
-      c.have_sparse = false;  // species layout changed: sparse points must be set again
   });
 }
 
@@ -1374,7 +1379,7 @@ int rgpu_gap_kmm(rgpu_context* ctx, double* kmm) {
   return guarded([&] {
     if (!ctx || !kmm) throw InvalidArgument("null argument");
     Context& c = ctx->c;
-    if (!c.have_sparse) throw InvalidArgument("no sparse points set");
+    if (!c.have_sparse) throw InvalidArgument(kNoSparse);
     const int64_t M = c.M, D = c.D();
     std::vector<int> iota(static_cast<size_t>(std::max<int64_t>(M, 1)));
     for (int64_t m = 0; m < M; ++m) iota[size_t(m)] = int(m);
@@ -1416,7 +1421,7 @@ int rgpu_gap_knm_gradient(rgpu_context* ctx, int neg_stress, double* out) {
     Context& c = ctx->c;
     require_batch(c);
     require_invariants(c);
-    if (!c.have_sparse) throw InvalidArgument("no sparse points set");
+    if (!c.have_sparse) throw InvalidArgument(kNoSparse);
     if (c.precision != RGPU_FP64)
       throw InvalidArgument("the K_NM gradient rows are computed in FP64 (set RGPU_FP64)");
     ensure_rep(c);
@@ -1466,7 +1471,7 @@ int rgpu_gap_knm(rgpu_context* ctx, int target, double* knm) {
     if (!ctx || !knm) throw InvalidArgument("null argument");
     Context& c = ctx->c;
     require_batch(c);
-    if (!c.have_sparse) throw InvalidArgument("no sparse points set");
+    if (!c.have_sparse) throw InvalidArgument(kNoSparse);
     if (target != 0 && target != 1) throw InvalidArgument("target must be 0 (Structure) or 1 (Atom)");
     ensure_rep(c);
     if (c.precision == RGPU_FP64)
@@ -1481,7 +1486,7 @@ int rgpu_gap_predict(rgpu_context* ctx, double* energy, double* gradient, double
     if (!ctx) throw InvalidArgument("null context");
     Context& c = ctx->c;
     require_batch_unsettled(c);
-    if (!c.have_sparse) throw InvalidArgument("no sparse points set");
+    if (!c.have_sparse) throw InvalidArgument(kNoSparse);
     if (!c.have_weights) throw InvalidArgument("sparse points were set without weights");
     // runs on an unchecked neighbour list, then checks it: a list whose
     // capacity was short held no pairs, so it is built again and the

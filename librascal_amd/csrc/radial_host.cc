@@ -232,19 +232,25 @@ static void fill_hyp(RadialHost& h, int idx, double a, double b, double tol) {
 // Gauss-Legendre points and weights on [r0, r1], order_n points, ascending
 // (math/gauss_legendre.cc:30-73: Newton iteration on the Legendre
 // three-term recurrence from the starting guesses cos(pi (i + 3/4) / (n +
-// 1/2)), the symmetric half mirrored)
+// 1/2)), the symmetric half mirrored).  The derivative pp is carried from
+// one root to the next, as in the reference: for an odd order the middle
+// guess cos(pi/2) ~ 6e-17 is already within the tolerance of z1 = 0, the
+// Newton loop does not run, and the middle weight is formed with the
+// previous root's derivative.  That weight is not the true quadrature
+// weight, but DVR models built by the reference use it, so the basis
+// (and every feature of such a model) is reproduced exactly.
 void gauss_legendre(double r0, double r1, int order_n, std::vector<double>& x,
                     std::vector<double>& w) {
+  if (order_n < 2)
+    throw RuntimeError("there should be at least 2 integration points but: order_n < 2 == true");
   x.assign(order_n, 0.0);
   w.assign(order_n, 0.0);
   const double mid = 0.5 * (r0 + r1), half = 0.5 * (r1 - r0);
   const int m = (order_n + 1) / 2;
+  double pp = 0;
   for (int i = 0; i < m; ++i) {
-    double z = std::cos(3.14159265358979323846 * (i + 0.75) / (order_n + 0.5)), z1 = 0, pp = 0;
-    // at least one Newton step: for odd orders the middle guess cos(pi/2)
-    // already sits within the tolerance of z1 = 0, and skipping the step
-    // would leave pp = 0 (an infinite weight)
-    do {
+    double z = std::cos(3.14159265358979323846 * (i + 0.75) / (order_n + 0.5)), z1 = 0;
+    while (std::fabs(z - z1) > kDblFtol) {
       double p1 = 1, p2 = 0;
       for (int j = 0; j < order_n; ++j) {
         const double p3 = p2;
@@ -254,7 +260,7 @@ void gauss_legendre(double r0, double r1, int order_n, std::vector<double>& x,
       pp = order_n * (z * p1 - p2) / (z * z - 1.0);
       z1 = z;
       z = z1 - p1 / pp;
-    } while (std::fabs(z - z1) > kDblFtol);
+    }
     x[i] = mid - half * z;
     x[order_n - 1 - i] = mid + half * z;
     w[i] = (2.0 * half) / ((1.0 - z * z) * pp * pp);

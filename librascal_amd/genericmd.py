@@ -6,7 +6,7 @@ evaluated on the GPU (one context reused across steps)."""
 import numpy as np
 
 from .io import load_obj
-from .neighbourlist import AtomsList
+from .neighbourlist import AtomsList, sanitize_non_periodic_structure
 from .synthetic import wrap_positions
 
 
@@ -39,9 +39,17 @@ class GenericMDCalculator:
     def calculate(self, positions, cell_matrix):
         """positions [N][3] and the cell with lattice vectors as rows (zero
         for non-periodic runs), in AA; returns (energy, forces [N][3],
-        stress [3][3]) in eV and AA, the stress the volume-normalised energy
-        gradient with respect to the cell (Voigt components as the model's
-        neg_stress, symmetrised)."""
+        stress [3][3]) in eV and AA.  The stress matrix is the reference's
+        (genericmd.py:123-129): the model's Voigt components (the
+        volume-normalised energy gradient with respect to the cell) placed in
+        the upper triangle, then the upper triangle including the diagonal
+        added transposed, so the off-diagonal entries are mirrored and the
+        diagonal entries doubled.
+
+        The first call sets the structure on the GPU; later calls move its
+        atoms (AtomsList.set_positions: the neighbour-list device pass
+        replayed from a HIP graph, its size check deferred to the end of the
+        prediction) and run one prediction for energy, forces and stress."""
         positions = np.asarray(positions, np.float64)
         cell_matrix = np.asarray(cell_matrix, np.float64)
         if positions.shape != (len(self.numbers), 3):
@@ -49,14 +57,23 @@ class GenericMDCalculator:
         if cell_matrix.shape != (3, 3):
             raise ValueError("Improper shape of cell info (expected 3x3 matrix)")
         cell = cell_matrix.T.copy()  # lattice vectors as columns
-        if np.any(self.pbc):
-            positions = wrap_positions(positions, cell, self.pbc.astype(np.int32))
-        frame = dict(positions=positions, numbers=self.numbers, cell=cell,
-                     pbc=self.pbc.astype(np.int32))
-        # a new batch per step: its neighbour list is rebuilt on the GPU, the
-        # context and the model's sparse points stay resident
-        self.manager = self.representation.transform(
-            AtomsList([frame], self.representation.nl_options))
+        if self.manager is None:
+            if np.any(self.pbc):
+                positions = wrap_positions(positions, cell, self.pbc.astype(np.int32))
+            frame = dict(positions=positions, numbers=self.numbers, cell=cell,
+                         pbc=self.pbc.astype(np.int32))
+            self.representation._context.enable_graphs(True)
+            self.manager = self.representation.transform(
+                AtomsList([frame], self.representation.nl_options))
+        elif np.any(self.pbc):
+            self.manager.set_positions(self.representation, positions, cell[None])
+        else:
+            # non-periodic: the atoms' bounding box (x1.05, atoms recentred)
+            # as each step's cell, as for the first step
+            st = sanitize_non_periodic_structure(
+                dict(positions=positions, numbers=self.numbers, cell=np.zeros((3, 3)),
+                     pbc=np.zeros(3, np.int32)))
+            self.manager.set_positions(self.representation, st["positions"], st["cell"][None])
         if self.model.kernel.kernel_type == "Sparse" and self.model.target_type == "Structure":
             energy, forces, stress_voigt = self.model.predict_energy_forces_stress(self.manager)
         else:
@@ -65,7 +82,7 @@ class GenericMDCalculator:
             stress_voigt = self.model.predict_stress(self.manager)
         stress = np.zeros((3, 3))
         stress[tuple(zip(*self.matrix_indices_in_voigt_notation))] = np.asarray(stress_voigt).ravel()
-        stress += np.triu(stress, 1).T
+        stress += np.triu(stress).T
         return energy, forces, stress
 
 

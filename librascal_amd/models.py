@@ -30,6 +30,9 @@ class SparsePoints:
         self.representation = representation
         self._chunks = []   # (species tuple, centre numbers [m], features [m][K(species)*inner])
         self._cache = {}
+        # bumped on every change: a context holding these points re-uploads
+        # them when the version it loaded is no longer current
+        self.version = 0
 
     def extend(self, atoms_list, selected_indices):
         """Add the environments selected_indices[s] (centre indices within
@@ -53,6 +56,7 @@ class SparsePoints:
         self._chunks.append((tuple(atoms_list.species), atoms_list.numbers[rows].copy(),
                              np.ascontiguousarray(X[rows])))
         self._cache.clear()
+        self.version += 1
         return self
 
     @classmethod
@@ -64,6 +68,7 @@ class SparsePoints:
         if values.shape[1] != representation.get_num_coefficients(len(species)):
             raise ValueError("values do not match the species list")
         sp._chunks.append((tuple(sorted(species)), np.asarray(numbers, np.int32).copy(), values))
+        sp.version += 1
         return sp
 
     def size(self):
@@ -115,9 +120,12 @@ def _set_model(rep, sparse_points, zeta, weights):
     ctx = rep._context
     loaded = rep._loaded
     species = loaded.species if isinstance(loaded, AtomsList) else list(loaded)
-    wkey = None if weights is None else hash(np.ascontiguousarray(weights, np.float64).tobytes())
-    token = (id(sparse_points), sparse_points.size(), tuple(species), int(zeta), wkey)
-    if rep._sparse_token == token:
+    wkey = None if weights is None else np.ascontiguousarray(weights, np.float64).tobytes()
+    # the token holds the points object itself (not its id, which Python
+    # reuses once an object is freed) and its version
+    token = (sparse_points, sparse_points.version, tuple(species), int(zeta), wkey)
+    old = rep._sparse_token
+    if (old is not None and old[0] is sparse_points and old[1:] == token[1:]):
         return
     ctx.set_sparse_points(zeta, sparse_points.numbers, sparse_points.values_for(species),
                           None if weights is None else np.asarray(weights, np.float64).ravel())

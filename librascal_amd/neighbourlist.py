@@ -62,6 +62,9 @@ class AtomsList:
             pack(self.structures)
         self.calculator = None
         self.species = None
+        # incremented whenever set_positions changes the positions, so that a
+        # result computed for earlier positions can tell that it is stale
+        self.positions_version = 0
 
     def __len__(self):
         return len(self.structures)
@@ -89,6 +92,36 @@ class AtomsList:
                              "calculator")
         calculator._load(self)
         return calculator
+
+    def set_positions(self, calculator, positions, cells=None):
+        """New positions [n_atoms][3] (and optionally cells [n_struct][3][3],
+        lattice vectors as columns) for the same atoms, e.g. one MD step:
+        periodic directions are wrapped into the cell, then the neighbour
+        list and the representation are rebuilt on the GPU with the batch's
+        buffers kept (the reference's manager.update, structure_manager.py
+        :231-258, followed by compute)."""
+        calculator = self._require(calculator)
+        from .synthetic import wrap_positions
+        pos = np.array(positions, np.float64).reshape(-1, 3)
+        if pos.shape[0] != self.n_centres:
+            raise ValueError("positions must have one row per atom of the batch")
+        if cells is not None:
+            self.cells = np.ascontiguousarray(np.asarray(cells, np.float64).reshape(-1, 3, 3))
+        off = self.atom_offsets
+        for s in range(len(off) - 1):
+            if np.any(self.pbc[s]):
+                pos[off[s]:off[s + 1]] = wrap_positions(pos[off[s]:off[s + 1]],
+                                                        self.cells[s].reshape(3, 3),
+                                                        self.pbc[s])
+        self.positions = np.ascontiguousarray(pos)
+        for k, st in enumerate(self.structures):
+            st["positions"] = self.positions[off[k]:off[k + 1]]
+            if cells is not None:
+                st["cell"] = self.cells[k].reshape(3, 3)
+        self.positions_version += 1
+        calculator._context.update_positions(self.positions,
+                                             None if cells is None else self.cells)
+        return self
 
     @property
     def managers(self):

@@ -284,3 +284,132 @@ def test_generic_md_calculator():
         calc.calculate(pos[:5], cell_rows)
     with pytest.raises(ValueError, match=r"Improper shape of cell info \(expected 3x3 matrix\)"):
         calc.calculate(pos, cell_rows[:2])
+
+
+def test_autograd_backward_after_a_later_forward():
+    """X1 = f(p1), X2 = f(p2) on the same AtomsList, then a backward through
+    X1: the adjoint must use the expansion at p1 (the context holds p2)."""
+    import torch
+    from librascal_amd.autograd import soap_features
+    rep = _rep()
+    frames = golden_structures()["CaCrP2O7"]
+    al = rep.transform(frames)
+    rng = np.random.default_rng(8)
+    p1 = torch.tensor(al.positions, dtype=torch.float64, device="cuda", requires_grad=True)
+    p2 = torch.tensor(al.positions + 0.05 * rng.normal(size=al.positions.shape),
+                      dtype=torch.float64, device="cuda", requires_grad=True)
+    X1 = soap_features(p1, rep, al)
+    X2 = soap_features(p2, rep, al)
+    assert X1.is_cuda and X2.is_cuda
+    R = torch.tensor(rng.normal(size=tuple(X1.shape)), dtype=torch.float64, device="cuda")
+    (X1 * R).sum().backward()
+    calc = O.Calculator(rep.hypers)
+    st1 = [dict(f, positions=p1.detach().cpu().numpy()[al.atom_offsets[k]:al.atom_offsets[k + 1]])
+           for k, f in enumerate(al.structures)]
+    b1 = O.BatchData(st1, species=al.species)
+    X1_ref, _ = calc.power_spectrum(b1)
+    assert np.abs(X1.detach().cpu().numpy() - X1_ref).max() < TOL * np.abs(X1_ref).max()
+    ref, _ = oracle_backward(calc, b1, R.cpu().numpy())
+    grad = p1.grad.cpu().numpy()
+    assert np.abs(grad - ref).max() < TOL * np.abs(ref).max()
+    # and X2's backward still sees p2
+    (X2 * R).sum().backward()
+    st2 = [dict(f, positions=al.positions[al.atom_offsets[k]:al.atom_offsets[k + 1]])
+           for k, f in enumerate(al.structures)]
+    ref2, _ = oracle_backward(calc, O.BatchData(st2, species=al.species), R.cpu().numpy())
+    assert np.abs(p2.grad.cpu().numpy() - ref2).max() < TOL * np.abs(ref2).max()
+
+
+def test_sparse_points_rebuilt_with_a_reused_identity():
+    """Sparse points made in a helper and freed, then others of the same
+    size and species: the context must upload the new points (Python may
+    give the new object the freed object's id)."""
+    rep = _rep()
+    frames, al, sp0, kernel, w = _model(rep, 2)
+
+    def points(step):
+        sp = SparsePoints(rep)
+        sel = [list(range(step, al.atom_offsets[s + 1] - al.atom_offsets[s], 3))[:4]
+               for s in range(len(al))]
+        sp.extend(al, sel)
+        return sp
+
+    sp_a = points(0)
+    n = sp_a.size()
+    k_a = kernel(al, sp_a).copy()
+    del sp_a
+    sp_b = points(1)
+    assert sp_b.size() == n
+    k_b = kernel(al, sp_b)
+    ref = O.gap_knm(O.Calculator(rep.hypers), O.BatchData(frames, species=al.species),
+                    O.SparseSet(sp_b.numbers, sp_b.values_for(al.species), 2), "Structure")
+    ss = O.SparseSet(sp_b.numbers, sp_b.values_for(al.species), 2)
+    kr = np.zeros_like(ref)
+    kr[:, ss.order] = ref
+    assert rel_err(kr, k_b).max() < TOL
+    assert np.abs(k_a - k_b).max() > 1e-6
+    # extending a set bumps its version: the next kernel call sees the new rows
+    sp_b.extend(al, [[0]] + [[] for _ in range(len(al) - 1)])
+    assert kernel(al, sp_b).shape[1] == n + 1
+
+
+def test_capi_sparse_points_dropped_with_another_species_list():
+    """Sparse points set for species {1, 6}, then a batch with {1, 8}: the
+    C ABI refuses GAP calls until sparse points are set again."""
+    from librascal_amd import _capi
+    import json
+    from conftest import soap_hypers
+    ctx = _capi.Context(json.dumps(soap_hypers()))
+    cell = (np.eye(3) * 8.0).ravel()
+    pos = np.array([[1.0, 1.0, 1.0], [2.0, 1.5, 1.0], [1.5, 2.5, 1.2]])
+    ctx.set_structures([0, 3], pos, [1, 6, 6], cell, [1, 1, 1], [1, 6])
+    X, _ = ctx.features()
+    ctx.set_sparse_points(2, np.array([1, 6], np.int32), X[:2], np.ones(2))
+    e, _, _ = ctx.gap_predict()
+    ctx.set_structures([0, 3], pos, [1, 8, 8], cell, [1, 1, 1], [1, 8])
+    for call in (lambda: ctx.gap_predict(), lambda: ctx.gap_knm(), lambda: ctx.gap_kmm()):
+        with pytest.raises(_capi.RascalGPUError, match="no sparse points set"):
+            call()
+    # the same species list keeps them
+    ctx.set_structures([0, 3], pos, [1, 6, 6], cell, [1, 1, 1], [1, 6])
+    ctx.set_sparse_points(2, np.array([1, 6], np.int32), X[:2], np.ones(2))
+    moved = pos.copy()
+    moved[1, 0] += 0.2
+    ctx.set_structures([0, 3], moved, [1, 6, 6], cell, [1, 1, 1], [1, 6])
+    e2, _, _ = ctx.gap_predict()
+    assert np.isfinite(e2).all() and abs(e2[0] - e[0]) > 0
+
+
+def test_generic_md_step_equals_update_and_predict():
+    """A GenericMDCalculator step after the first moves the atoms of its
+    batch (set_positions) and runs one prediction: energy, forces and the
+    Voigt stress equal a fresh transform + predict to 1e-12; the 3x3 matrix
+    is the reference's (genericmd.py:123-129): Voigt entries in the upper
+    triangle plus the transposed upper triangle, so the diagonal is doubled."""
+    from conftest import wrap_cubic_periodic
+    from librascal_amd.genericmd import GenericMDCalculator
+    from librascal_amd.io import load_obj
+    from librascal_amd.synthetic import wrap_positions
+    st = wrap_cubic_periodic(golden_structures()["methane_dimers"][0])
+    calc = GenericMDCalculator(_model_file(), True, structure_template=st)
+    pos = np.asarray(st["positions"])
+    cell_rows = np.asarray(st["cell"]).T
+    calc.calculate(pos, cell_rows)
+    al = calc.manager
+    rng = np.random.default_rng(4)
+    for step in range(3):
+        p = pos + 0.02 * rng.normal(size=pos.shape)
+        e, f, s = calc.calculate(p, cell_rows)
+        assert calc.manager is al  # the batch is reused
+        model = load_obj(_model_file())
+        rep = model.get_representation_calculator()
+        pw = wrap_positions(p, np.asarray(st["cell"]), np.asarray(st["pbc"]))
+        ref = rep.transform([dict(st, positions=pw)])
+        e_r, f_r, s_r = model.predict(ref), model.predict_forces(ref), model.predict_stress(ref)
+        assert abs(e[0] - e_r[0]) < 1e-12 * max(abs(e_r[0]), 1.0)
+        assert np.abs(f - f_r).max() < 1e-12 * np.abs(f_r).max()
+        v = s_r.ravel()
+        m = np.array([[2 * v[0], v[5], v[4]], [v[5], 2 * v[1], v[3]], [v[4], v[3], 2 * v[2]]])
+        assert np.abs(s - m).max() < 1e-12 * np.abs(m).max()
+    ctx = calc.representation._context
+    assert ctx.info("nl_graph_launches") >= 1
