@@ -19,7 +19,12 @@
  *       invariants [n_centres][K][N*N][L+1]     (K = S(S+1)/2 keys a<=b)
  *     which is the reference's dense export (property_block_sparse.hh
  *     :911-932) before dropping keys absent from the whole batch.
- *   - Every atom is a centre (no centre masks).
+ *   - Centres: every atom, or the atoms selected by a centre mask, in atom
+ *     order.  Per-centre outputs (features, expansion, key masks, Atom K_NM
+ *     rows) have one row per centre; per-atom outputs (gradients) one row per
+ *     atom.  Gradients of masked batches are refused, as in the reference.
+ *   - A sparsified power spectrum (hypers "coefficient_subselection") has
+ *     one feature per selected coefficient, in the order given (FP64 only).
  *   - Calls on one context are not re-entrant (as the reference calculators).
  */
 #ifndef RASCAL_MI355X_H_
@@ -43,7 +48,8 @@ extern "C" {
 #define RGPU_INFO_MAX_RADIAL 0
 #define RGPU_INFO_MAX_ANGULAR 1
 #define RGPU_INFO_N_SPECIES 2
-#define RGPU_INFO_FEATURE_SIZE 3 /* D = K N^2 (L+1) (PowerSpectrum) or S N (L+1)^2 */
+#define RGPU_INFO_FEATURE_SIZE 3 /* D = K N^2 (L+1) (PowerSpectrum; the number of selected
+                                    coefficients with coefficient_subselection) or S N (L+1)^2 */
 #define RGPU_INFO_N_CENTRES 4
 #define RGPU_INFO_N_PAIRS 5      /* strict pairs including the self pairs */
 #define RGPU_INFO_N_STRUCT 6
@@ -58,6 +64,7 @@ extern "C" {
 #define RGPU_INFO_TILE_PAIRS_ADJ 15 /* autotuned pairs per batch, adjoint */
 #define RGPU_INFO_TILE_KEYS_ADJ 16  /* key blocks per pass of the FP32 adjoint (0 = auto) */
 #define RGPU_INFO_NL_GRAPH_LAUNCHES 17 /* neighbour-list passes replayed from a graph */
+#define RGPU_INFO_N_ATOMS 18       /* atoms of the batch (N_CENTRES counts the centres) */
 
 /* stages timed by rgpu_enable_timing */
 #define RGPU_STAGE_NEIGHBOURS 0
@@ -106,7 +113,13 @@ int rgpu_set_precision(rgpu_context* ctx, int precision);
 int rgpu_get_info(rgpu_context* ctx, int key, int64_t* value);
 
 /* Batch of structures (host arrays).  Builds the neighbour list on the GPU.
- * species[n_species] sorted ascending must contain every atom type.
+ * species[n_species] sorted ascending must contain every atom type (with
+ * expansion_by_species_method "user defined" it must contain the
+ * global_species, and every centre's type must be one of them).
+ * centre_mask[n_atoms] (may be NULL: every atom) selects the centres, as
+ * the reference's center_atoms_mask (neighbourlist/structure_manager.py
+ * :337-340; structure_manager_centers.cc:37-100).  Sparse points set
+ * earlier are dropped when the species list changes.
  * Replaces AtomsList / ManagerCollection::add_structures with the stack
  * centers -> neighbourlist -> centercontribution -> strict
  * (structure_manager_collection.hh:177-236; adaptor_neighbour_list.hh
@@ -114,12 +127,13 @@ int rgpu_get_info(rgpu_context* ctx, int key, int64_t* value);
  * :373-455). */
 int rgpu_set_structures(rgpu_context* ctx, int n_struct, const int64_t* atom_offsets,
                         const double* positions, const int32_t* types, const double* cells,
-                        const int32_t* pbc, int n_species, const int32_t* species);
+                        const int32_t* pbc, int n_species, const int32_t* species,
+                        const uint8_t* centre_mask);
 /* New positions (and cells, may be NULL) for the same atoms and types, e.g.
  * one MD step; rebuilds the neighbour list. */
 int rgpu_update_positions(rgpu_context* ctx, const double* positions, const double* cells);
 
-/* Strict pair list, self pair first per centre: centre_offsets[n_centres+1],
+/* Strict pair list of every atom, self pair first: centre_offsets[n_atoms+1],
  * pair_j[n_pairs] (original atom index within the batch), pair_vec[n_pairs][3]
  * = r_j(image) - r_i.  pair_vec may be NULL. */
 int rgpu_get_neighbours(rgpu_context* ctx, int64_t* centre_offsets, int32_t* pair_j,
@@ -130,11 +144,23 @@ int rgpu_get_neighbours(rgpu_context* ctx, int64_t* centre_offsets, int32_t* pai
  * 693-931; calculator_spherical_expansion.hh:2139-2411). */
 int rgpu_compute(rgpu_context* ctx);
 /* expansion coefficients (dense layout) and per-centre key masks (bit s set
- * when species s is among the centre's expansion keys); either may be NULL */
+ * when species s is among the centre's expansion keys, by the hypers'
+ * expansion_by_species_method: the centre's and its neighbours' species,
+ * those of its structure, or the global species;
+ * calculator_spherical_expansion.hh:2414-2570); either may be NULL */
 int rgpu_get_expansion(rgpu_context* ctx, double* coeffs, uint64_t* key_mask);
 /* power spectrum [n_centres][D], normalised when the hypers say so, and the
  * per-centre inverse norms (1 when not normalising); either may be NULL */
 int rgpu_get_features(rgpu_context* ctx, double* features, double* inv_norms);
+
+/* Gradient of the features with respect to the positions, the reference's
+ * dense get_features_gradient matrix: out[3 n_pairs][D], rows 3 r + x for
+ * the pairs r of rgpu_get_neighbours (self pair first per centre), row
+ * 3 r + x = d X_i / d r_j,x for the pair (i, j); for the self pair the
+ * derivative with respect to r_i (bind_py_structure_manager.cc:622-744;
+ * calculator_spherical_invariants.hh:796-921).  FP64 precision, no centre
+ * mask.  out may be host or device memory. */
+int rgpu_get_features_gradient(rgpu_context* ctx, double* out);
 
 /* Adjoint of the features: given dL/dX [n_centres][D] (X as returned by
  * rgpu_get_features), writes dL/dr [n_atoms][3] and the negative virial

@@ -25,7 +25,7 @@ STAGES = dict(neighbours=0, expansion=1, power_spectrum=2, gap_kernel=3, gap_for
 INFO = dict(max_radial=0, max_angular=1, n_species=2, feature_size=3, n_centres=4,
             n_pairs=5, n_struct=6, spline_grid=7, n_sparse=8, is_invariants=9, precision=10,
             graph_captures=11, graph_launches=12, tile_pairs_fwd=13, tile_keys_ps=14,
-            tile_pairs_adj=15, tile_keys_adj=16, nl_graph_launches=17)
+            tile_pairs_adj=15, tile_keys_adj=16, nl_graph_launches=17, n_atoms=18)
 
 _lib = None
 
@@ -64,12 +64,13 @@ def lib():
     L.rgpu_autotune.argtypes = [p]
     L.rgpu_set_tiles.argtypes = [p, i, i, i, i]
     L.rgpu_get_info.argtypes = [p, i, C.POINTER(C.c_int64)]
-    L.rgpu_set_structures.argtypes = [p, i, p, p, p, p, p, i, p]
+    L.rgpu_set_structures.argtypes = [p, i, p, p, p, p, p, i, p, p]
     L.rgpu_update_positions.argtypes = [p, p, p]
     L.rgpu_get_neighbours.argtypes = [p, p, p, p]
     L.rgpu_compute.argtypes = [p]
     L.rgpu_get_expansion.argtypes = [p, p, p]
     L.rgpu_get_features.argtypes = [p, p, p]
+    L.rgpu_get_features_gradient.argtypes = [p, p]
     L.rgpu_features_backward.argtypes = [p, p, p, p]
     L.rgpu_set_sparse_points.argtypes = [p, i, i64, p, p, p]
     L.rgpu_gap_knm.argtypes = [p, i, p]
@@ -84,7 +85,7 @@ def lib():
     for name in ("rgpu_context_create", "rgpu_set_stream", "rgpu_set_precision", "rgpu_get_info",
                  "rgpu_enable_graphs", "rgpu_autotune", "rgpu_set_tiles",
                  "rgpu_set_structures", "rgpu_update_positions", "rgpu_get_neighbours",
-                 "rgpu_compute", "rgpu_get_expansion", "rgpu_get_features",
+                 "rgpu_compute", "rgpu_get_expansion", "rgpu_get_features", "rgpu_get_features_gradient",
                  "rgpu_features_backward", "rgpu_set_sparse_points", "rgpu_gap_knm", "rgpu_gap_kmm", "rgpu_gap_knm_gradient",
                  "rgpu_gap_predict", "rgpu_eval_radial", "rgpu_eval_ylm", "rgpu_enable_timing",
                  "rgpu_get_timing", "rgpu_reset_timing"):
@@ -172,15 +173,23 @@ class Context:
         check(lib().rgpu_set_tiles(self._p, int(pairs_fwd), int(keys_ps), int(pairs_adj),
                                    int(keys_adj)))
 
-    def set_structures(self, atom_offsets, positions, types, cells, pbc, species):
+    def set_structures(self, atom_offsets, positions, types, cells, pbc, species,
+                       centre_mask=None):
+        """cells [n_struct][3][3] with lattice vectors as columns;
+        centre_mask [n_atoms] (bool, None: every atom is a centre)."""
         ao = np.ascontiguousarray(atom_offsets, np.int64)
         pos = np.ascontiguousarray(positions, np.float64).reshape(-1, 3)
         ty = np.ascontiguousarray(types, np.int32)
         ce = np.ascontiguousarray(cells, np.float64).reshape(-1, 9)
         pb = np.ascontiguousarray(pbc, np.int32).reshape(-1, 3)
         sp = np.ascontiguousarray(sorted(species), np.int32)
+        cm = None
+        if centre_mask is not None:
+            cm = np.ascontiguousarray(np.asarray(centre_mask, bool).ravel(), np.uint8)
+            if cm.shape[0] != ty.shape[0]:
+                raise ValueError("centre_mask needs one entry per atom")
         check(lib().rgpu_set_structures(self._p, len(ao) - 1, ptr(ao), ptr(pos), ptr(ty), ptr(ce),
-                                        ptr(pb), len(sp), ptr(sp)))
+                                        ptr(pb), len(sp), ptr(sp), ptr(cm)))
 
     def update_positions(self, positions, cells=None):
         pos = np.ascontiguousarray(positions, np.float64).reshape(-1, 3)
@@ -188,7 +197,9 @@ class Context:
         check(lib().rgpu_update_positions(self._p, ptr(pos), ptr(ce)))
 
     def neighbours(self, with_vectors=True):
-        n = self.info("n_centres")
+        """Strict pairs of every atom (self pair first): offsets [n_atoms+1],
+        neighbour atom [n_pairs], pair vectors [n_pairs][3]."""
+        n = self.info("n_atoms")
         npairs = self.info("n_pairs")
         off = np.zeros(n + 1, np.int64)
         pj = np.zeros(npairs, np.int32)
@@ -209,6 +220,12 @@ class Context:
         check(lib().rgpu_get_expansion(self._p, ptr(c), ptr(km)))
         return c, km
 
+    def key_masks(self):
+        """Per-centre key masks (bit s: species s is an expansion key)."""
+        km = np.zeros(self.info("n_centres"), np.uint64)
+        check(lib().rgpu_get_expansion(self._p, None, ptr(km)))
+        return km
+
     def features(self, out=None):
         """(X [n_centres][D], inverse norms [n_centres]).  With out (a float64
         array or tensor of that shape, host or device) X is written there and
@@ -228,7 +245,7 @@ class Context:
         """dL/dr [n_atoms][3] (and the negative virial) for dL/dX given as a
         host array or a contiguous float64 torch tensor (host or device);
         dL/dr goes into out when given (float64, host or device)."""
-        n = self.info("n_centres")
+        n = self.info("n_atoms")
         ns = self.info("n_struct")
         if hasattr(dL_dX, "data_ptr"):
             g = dL_dX
@@ -243,6 +260,18 @@ class Context:
         st = np.zeros((ns, 6)) if want_stress else None
         check(lib().rgpu_features_backward(self._p, ptr(g), ptr(out), ptr(st)))
         return out, st
+
+    def features_gradient(self, out=None):
+        """dX/dr [3 n_pairs][D]: rows 3 r + x for the pairs r of neighbours()
+        (self pair first), FP64."""
+        rows = 3 * self.info("n_pairs")
+        D = self.info("feature_size")
+        if out is None:
+            out = np.zeros((rows, D))
+        else:
+            _check_out(out, (rows, D))
+        check(lib().rgpu_get_features_gradient(self._p, ptr(out)))
+        return out
 
     def set_sparse_points(self, zeta, species, values, weights=None):
         sp = np.ascontiguousarray(species, np.int32)
@@ -259,7 +288,7 @@ class Context:
 
     def gap_knm_gradient(self, neg_stress=False):
         """K_NM gradient rows [3 n_atoms (+ 6 n_struct)][n_sparse] (FP64)."""
-        n, ns, M = self.info("n_centres"), self.info("n_struct"), self.info("n_sparse")
+        n, ns, M = self.info("n_atoms"), self.info("n_struct"), self.info("n_sparse")
         out = np.zeros((3 * n + (6 * ns if neg_stress else 0), M))
         check(lib().rgpu_gap_knm_gradient(self._p, int(bool(neg_stress)), ptr(out)))
         return out
@@ -275,7 +304,7 @@ class Context:
         None for the outputs not requested; out = (e, g, s) gives the
         destination arrays or tensors (float64, host or device, None to skip)."""
         ns = self.info("n_struct")
-        n = self.info("n_centres")
+        n = self.info("n_atoms")
         if out is not None:
             e, g, s = out
             for a, shape in ((e, (ns,)), (g, (n, 3)), (s, (ns, 6))):

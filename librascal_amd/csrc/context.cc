@@ -61,10 +61,24 @@ struct Context {
   DeviceBuffer<int64_t> d_atom_off;
   DeviceBuffer<int> d_centre_struct;
   PairList pl;
+  // centres: every atom, or the atoms selected by a centre mask, in atom
+  // order (StructureManagerCenters puts them first,
+  // structure_manager_centers.cc:37-100)
+  bool masked = false;
+  int64_t n_centres = 0;
+  std::vector<int64_t> centre_idx;  // atom index of each centre
+  DeviceBuffer<int64_t> d_centre_idx;
+  // coefficient_subselection: full-layout column of each selected
+  // coefficient (-1 for a species pair outside the batch's species list)
+  std::vector<int64_t> sel_col;
+  DeviceBuffer<int64_t> d_sel_col;
 
   // representation
   bool have_rep = false;
   DeviceBuffer<double> c64, p64;
+  DeviceBuffer<double> ps64;         // selected invariants [n_atoms][n_sel] (un-normalised)
+  DeviceBuffer<double> gfull;        // dL/dX scattered into the full layout
+  DeviceBuffer<double> dcp, dcs;     // stored expansion derivatives (gradient export)
   DeviceBuffer<float> c32, p32;
   DeviceBuffer<double> norm2, tmp_norm2;
   DeviceBuffer<unsigned long long> keymask;
@@ -161,12 +175,19 @@ struct Context {
     if (own_stream) (void)hipStreamDestroy(own_stream);
   }
 
+  // feature size: the selected coefficients of a sparsified power
+  // spectrum, else the full layout
   int64_t D() const {
+    if (hp.kind == SoapHypers::kPowerSpectrum && hp.sparsified) return int64_t(hp.sel_a.size());
+    return D_full();
+  }
+  int64_t D_full() const {
     const int S = int(species.size());
     const int N = hp.max_radial, L1 = hp.max_angular + 1;
     if (hp.kind == SoapHypers::kPowerSpectrum) return int64_t(S) * (S + 1) / 2 * N * N * L1;
     return int64_t(S) * N * L1 * L1;
   }
+  bool sparsified() const { return hp.kind == SoapHypers::kPowerSpectrum && hp.sparsified; }
   // row stride of the packed invariants of the mixed path (PowerSpectrum):
   // the packed columns rounded up to 32, pad columns zero
   int64_t Dp() const {
@@ -528,21 +549,26 @@ void compute_rep(Context& c) {
   const int S = int(c.species.size());
   const int64_t n = c.n_atoms;
   const int64_t E = int64_t(S) * c.rd.N * c.rd.LL;
-  const int64_t D = c.D();
+  const int64_t D = c.D_full();
   const bool ps = c.hp.kind == SoapHypers::kPowerSpectrum;
   const int64_t cap = n > 0 ? c.pl.cap_pairs : 0;
+  constexpr bool split = std::is_same<T, float>::value;
+  if (split && c.sparsified())
+    throw InvalidArgument("coefficient_subselection is computed in FP64 only (set RGPU_FP64)");
   T* cc = cbuf<T>(c).reserve(size_t(std::max<int64_t>(n * E, 1)));
   unsigned long long* km = c.keymask.reserve(size_t(std::max<int64_t>(n, 1)));
   double* fc = c.pair_fc.reserve(size_t(std::max<int64_t>(cap, 1)));
   double* dfc = c.pair_dfc.reserve(size_t(std::max<int64_t>(cap, 1)));
   // FP64 keeps the invariants in FP64; the mixed path stores them as the
   // BF16 hi/lo operand planes of the matrix-core GEMMs
-  constexpr bool split = std::is_same<T, float>::value;
   T* pp = ps && !split ? pbuf<T>(c).reserve(size_t(std::max<int64_t>(n * D, 1))) : nullptr;
   const int64_t Dp = c.Dp();
   unsigned short* ph = ps && split ? c.p_hi.reserve(size_t(std::max<int64_t>(n * Dp, 1))) : nullptr;
   unsigned short* pl = ps && split ? c.p_lo.reserve(size_t(std::max<int64_t>(n * Dp, 1))) : nullptr;
   double* n2 = ps ? c.norm2.reserve(size_t(std::max<int64_t>(n, 1))) : nullptr;
+  // sparsified: the selected coefficients and their norms (p keeps S^T xs)
+  double* xs = split || !c.sparsified() ? nullptr
+                                        : c.ps64.reserve(size_t(std::max<int64_t>(n * c.D(), 1)));
   c.p32_valid = false;
   const int64_t* n_pairs_dev = c.pl.centre_off.get() + n;
   auto cutoff_and_expansion = [&] {
@@ -551,20 +577,23 @@ void compute_rep(Context& c) {
   };
   auto invariants = [&] {
     if (!ps) return;
-    if constexpr (split)
+    if constexpr (split) {
       launch_power_spectrum_split(c.rd, S, n, cc, ph, pl, n2, c.stream, c.tile());
-    else
+    } else {
       launch_power_spectrum<T>(c.rd, S, n, cc, pp, n2, c.stream, c.tile());
+      if (xs) launch_sparsify(n, D, c.D(), c.d_sel_col.get(), pp, xs, n2, c.stream);
+    }
   };
   if (c.use_graphs && !c.timing && n > 0) {
     // replay a captured graph of the stage while every buffer, size and
     // precision it was captured with is unchanged
     const std::vector<uintptr_t> sig = {
         uintptr_t(c.precision), uintptr_t(S), uintptr_t(n), uintptr_t(cap), uintptr_t(cc),
-        uintptr_t(km), uintptr_t(fc), uintptr_t(dfc), uintptr_t(pp), uintptr_t(ph),
+        uintptr_t(km), uintptr_t(fc), uintptr_t(dfc), uintptr_t(pp), uintptr_t(ph), uintptr_t(xs),
         uintptr_t(pl), uintptr_t(n2), uintptr_t(c.pl.centre_off.get()),
         uintptr_t(c.pl.pair_j.get()), uintptr_t(c.pl.pair_vec.get()), uintptr_t(c.pl.pair_dist.get()),
-        uintptr_t(c.d_species_of.get()), uintptr_t(c.tile().pb_fwd), uintptr_t(c.tile().kc)};
+        uintptr_t(c.d_species_of.get()), uintptr_t(c.tile().pb_fwd), uintptr_t(c.tile().kc),
+        uintptr_t(c.d_sel_col.get())};
     if (!c.rep_exec || sig != c.rep_sig) {
       c.drop_graph();
       hipGraph_t g = nullptr;
@@ -617,14 +646,20 @@ void require_invariants(const Context& c) {
     throw InvalidArgument("this operation needs SphericalInvariants (PowerSpectrum) hypers");
 }
 
-// GAP species blocks of the current batch
+// calculator_spherical_expansion.hh:2155-2158
+void require_unmasked(const Context& c) {
+  if (c.masked)
+    throw InvalidArgument("Can't compute spherical expansion gradients with masked center atoms");
+}
+
+// GAP species blocks of the current batch (the centres' rows)
 void build_rows(Context& c) {
   const int S = int(c.species.size());
   c.row_off.assign(S + 1, 0);
   std::vector<int> rows;
   rows.reserve(size_t(c.n_atoms));
   for (int s = 0; s < S; ++s) {
-    for (int64_t i = 0; i < c.n_atoms; ++i)
+    for (int64_t i : c.centre_idx)
       if (c.species_of[i] == s) rows.push_back(int(i));
     c.row_off[s + 1] = int64_t(rows.size());
   }
@@ -672,11 +707,20 @@ DeviceBuffer<float>& Gbuf<float>(Context& c) {
 }
 // Backward from G (device, [n][D] in precision T) to grad/virial (device).
 // xg (may be null): X_i . G_i per centre, times xg_scale.
+// (FP64, sparsified: G holds the selected columns and is scattered into the
+// full layout first)
 template <class T>
 void backward_device(Context& c, const T* G, double* grad, double* cvir, const double* xg = nullptr,
                      double xg_scale = 1.0) {
   const int S = int(c.species.size());
   StageTimer t(c, RGPU_STAGE_BACKWARD);
+  if constexpr (std::is_same<T, double>::value) {
+    if (c.sparsified()) {
+      double* gf = c.gfull.reserve(size_t(std::max<int64_t>(c.n_atoms * c.D_full(), 1)));
+      launch_scatter_columns(c.n_atoms, c.D(), c.d_sel_col.get(), G, c.D_full(), gf, c.stream);
+      G = gf;
+    }
+  }
   if constexpr (std::is_same<T, float>::value)
     launch_backward_split(c.rd, S, pair_view(c), c.c32.get(), c.p_hi.get(), c.p_lo.get(),
                           c.norm2.get(), c.hp.normalize, xg, xg_scale, G, grad, cvir, c.stream,
@@ -706,7 +750,7 @@ template <class T>
 const T* invariants_of(Context& c);
 template <>
 const double* invariants_of<double>(Context& c) {
-  return c.p64.get();
+  return c.sparsified() ? c.ps64.get() : c.p64.get();
 }
 template <>
 const float* invariants_of<float>(Context& c) {
@@ -765,9 +809,11 @@ void gap_predict_t(Context& c, double* energy, double* gradient, double* neg_str
   // the mixed path works on the packed invariants
   const int64_t n = c.n_atoms, D = std::is_same<T, float>::value ? c.Dp() : c.D();
   const bool want_grad = gradient || neg_stress;
+  if (want_grad) require_unmasked(c);
   double* ec = c.e_centre.reserve(size_t(std::max<int64_t>(n, 1)));
   RGPU_HIP(hipMemsetAsync(ec, 0, sizeof(double) * n, c.stream));
-  const T* X = std::is_same<T, double>::value ? pbuf<T>(c).get() : nullptr;
+  const T* X = nullptr;
+  if constexpr (std::is_same<T, double>::value) X = invariants_of<double>(c);
   std::vector<GapBlock<T>> blks;
   std::vector<int64_t> wt_off;
   int64_t wt_total = 0;
@@ -843,13 +889,18 @@ void gap_knm_t(Context& c, int target, double* knm) {
     launch_gap_kernel<T>(blk, c.M, nullptr, nullptr, target == 1 ? k : nullptr,
                          target == 0 ? k : nullptr, c.d_centre_struct.get(), c.stream);
   }
-  // columns back to the order the sparse points were given
-  std::vector<double> h(static_cast<size_t>(rows * c.M)), o(h.size());
+  // columns back to the order the sparse points were given; Atom rows of
+  // the centres only
+  std::vector<double> h(static_cast<size_t>(rows * c.M));
   RGPU_HIP(hipMemcpyAsync(h.data(), k, sizeof(double) * h.size(), hipMemcpyDeviceToHost, c.stream));
   RGPU_HIP(hipStreamSynchronize(c.stream));
-  for (int64_t r = 0; r < rows; ++r)
-    for (int64_t m = 0; m < c.M; ++m) o[size_t(r * c.M + c.sp_order[m])] = h[size_t(r * c.M + m)];
-  RGPU_HIP(hipMemcpy(knm, o.data(), sizeof(double) * o.size(), hipMemcpyDefault));
+  const int64_t out_rows = target == 0 ? rows : c.n_centres;
+  std::vector<double> o(static_cast<size_t>(out_rows * c.M));
+  for (int64_t r = 0; r < out_rows; ++r) {
+    const int64_t src = target == 0 ? r : c.centre_idx[size_t(r)];
+    for (int64_t m = 0; m < c.M; ++m) o[size_t(r * c.M + c.sp_order[m])] = h[size_t(src * c.M + m)];
+  }
+  if (!o.empty()) RGPU_HIP(hipMemcpy(knm, o.data(), sizeof(double) * o.size(), hipMemcpyDefault));
 }
 
 int fail(int code, const std::string& msg) {
@@ -1096,6 +1147,8 @@ int rgpu_set_precision(rgpu_context* ctx, int precision) {
     if (!ctx) throw InvalidArgument("null context");
     if (precision != RGPU_FP64 && precision != RGPU_MIXED) throw InvalidArgument("unknown precision");
     Context& c = ctx->c;
+    if (precision == RGPU_MIXED && c.sparsified())
+      throw InvalidArgument("coefficient_subselection is computed in FP64 only (set RGPU_FP64)");
     if (c.precision != precision) c.have_rep = false;
     c.precision = precision;
     if (c.have_sparse && precision == RGPU_MIXED && c.T32.get() == nullptr)
@@ -1112,7 +1165,8 @@ int rgpu_get_info(rgpu_context* ctx, int key, int64_t* value) {
       case RGPU_INFO_MAX_ANGULAR: *value = c.hp.max_angular; break;
       case RGPU_INFO_N_SPECIES: *value = int64_t(c.species.size()); break;
       case RGPU_INFO_FEATURE_SIZE: *value = c.D(); break;
-      case RGPU_INFO_N_CENTRES: *value = c.n_atoms; break;
+      case RGPU_INFO_N_CENTRES: *value = c.n_centres; break;
+      case RGPU_INFO_N_ATOMS: *value = c.n_atoms; break;
       case RGPU_INFO_N_PAIRS:
         if (c.have_batch) settle_nl(c);
         *value = c.have_batch ? c.pl.n_pairs : 0;
@@ -1136,7 +1190,8 @@ int rgpu_get_info(rgpu_context* ctx, int key, int64_t* value) {
 
 int rgpu_set_structures(rgpu_context* ctx, int n_struct, const int64_t* atom_offsets,
                         const double* positions, const int32_t* types, const double* cells,
-                        const int32_t* pbc, int n_species, const int32_t* species) {
+                        const int32_t* pbc, int n_species, const int32_t* species,
+                        const uint8_t* centre_mask) {
   return guarded([&] {
     if (!ctx || !atom_offsets || n_struct < 0 || n_species < 0) throw InvalidArgument("bad arguments");
     Context& c = ctx->c;
@@ -1157,6 +1212,50 @@ int rgpu_set_structures(rgpu_context* ctx, int n_struct, const int64_t* atom_off
       auto it = std::lower_bound(sp.begin(), sp.end(), t);
       if (it == sp.end() || *it != t) throw InvalidArgument("atom type missing from the species list");
       sof[a] = int(it - sp.begin());
+    }
+    // centres: the atoms of the mask (every atom without one)
+    std::vector<int64_t> cidx;
+    for (int64_t a = 0; a < n_atoms; ++a)
+      if (!centre_mask || centre_mask[a]) cidx.push_back(a);
+    const std::string& method = c.hp.expansion_by_species;
+    if (method == "user defined") {
+      // calculator_spherical_expansion.hh:2520-2545: every centre species is
+      // a global species; the layout is the global species list
+      std::vector<int> missing;
+      for (int64_t a : cidx) {
+        const int t = types[a];
+        if (std::find(c.hp.global_species.begin(), c.hp.global_species.end(), t) ==
+                c.hp.global_species.end() &&
+            std::find(missing.begin(), missing.end(), t) == missing.end())
+          missing.push_back(t);
+      }
+      if (!missing.empty()) {
+        std::string m = "global_species is missing at least these species: '";
+        for (int t : missing) m += std::to_string(t) + ", ";
+        throw RuntimeError(m + "'.");
+      }
+      for (int z : c.hp.global_species)
+        if (!std::binary_search(sp.begin(), sp.end(), z))
+          throw InvalidArgument("the species list must contain every global species (" +
+                                std::to_string(z) + " is missing)");
+    }
+    // coefficient_subselection: full-layout column of each selected
+    // coefficient over this species list
+    std::vector<int64_t> scol;
+    if (c.hp.kind == SoapHypers::kPowerSpectrum && c.hp.sparsified) {
+      const int N = c.hp.max_radial, L1 = c.hp.max_angular + 1, S = int(sp.size());
+      for (size_t k = 0; k < c.hp.sel_a.size(); ++k) {
+        auto ia = std::lower_bound(sp.begin(), sp.end(), c.hp.sel_a[k]);
+        auto ib = std::lower_bound(sp.begin(), sp.end(), c.hp.sel_b[k]);
+        if (ia == sp.end() || *ia != c.hp.sel_a[k] || ib == sp.end() || *ib != c.hp.sel_b[k]) {
+          scol.push_back(-1);
+          continue;
+        }
+        const int a = int(ia - sp.begin()), b = int(ib - sp.begin());
+        const int64_t key = int64_t(a) * S - int64_t(a) * (a - 1) / 2 + (b - a);
+        scol.push_back((key * N * N + int64_t(c.hp.sel_n1[k]) * N + c.hp.sel_n2[k]) * L1 +
+                       c.hp.sel_l[k]);
+      }
     }
     // RadialDimReduction: the spline table of each batch species (a species
     // without projections is an error, calculator_spherical_expansion.hh
@@ -1183,6 +1282,19 @@ int rgpu_set_structures(rgpu_context* ctx, int n_struct, const int64_t* atom_off
     c.types.assign(types, types + n_atoms);
     c.species = sp;
     c.species_of = sof;
+    c.masked = int64_t(cidx.size()) != n_atoms;
+    c.n_centres = int64_t(cidx.size());
+    c.centre_idx = cidx;
+    int64_t* dci = c.d_centre_idx.reserve(std::max<size_t>(cidx.size(), 1));
+    if (!cidx.empty())
+      RGPU_HIP(hipMemcpyAsync(dci, cidx.data(), sizeof(int64_t) * cidx.size(), hipMemcpyHostToDevice,
+                              c.stream));
+    c.sel_col = scol;
+    if (!scol.empty()) {
+      int64_t* dsc = c.d_sel_col.reserve(scol.size());
+      RGPU_HIP(hipMemcpyAsync(dsc, scol.data(), sizeof(int64_t) * scol.size(), hipMemcpyHostToDevice,
+                              c.stream));
+    }
     c.rd.sp_tab = nullptr;
     if (!sp_tab.empty()) {
       int* dt = c.d_sp_tab.reserve(sp_tab.size());
@@ -1252,20 +1364,55 @@ int rgpu_get_expansion(rgpu_context* ctx, double* coeffs, uint64_t* key_mask) {
     Context& c = ctx->c;
     require_batch(c);
     ensure_rep(c);
-    const int64_t E = int64_t(c.species.size()) * c.rd.N * c.rd.LL;
-    if (coeffs && c.n_atoms) {
-      if (c.precision == RGPU_FP64) {
-        RGPU_HIP(hipMemcpyAsync(coeffs, c.c64.get(), sizeof(double) * c.n_atoms * E, hipMemcpyDefault,
+    const int64_t E = int64_t(c.species.size()) * c.rd.N * c.rd.LL, nc = c.n_centres;
+    const int64_t* rows = c.masked ? c.d_centre_idx.get() : nullptr;
+    if (coeffs && nc) {
+      if (c.precision == RGPU_FP64 && !rows) {
+        RGPU_HIP(hipMemcpyAsync(coeffs, c.c64.get(), sizeof(double) * nc * E, hipMemcpyDefault,
                                 c.stream));
       } else {
-        double* o = c.out64.reserve(size_t(c.n_atoms * E));
-        launch_normalise<float>(c.n_atoms, E, c.c32.get(), nullptr, false, o, c.stream);
-        RGPU_HIP(hipMemcpyAsync(coeffs, o, sizeof(double) * c.n_atoms * E, hipMemcpyDefault, c.stream));
+        double* o = c.out64.reserve(size_t(nc * E));
+        if (c.precision == RGPU_FP64)
+          launch_normalise<double>(nc, E, c.c64.get(), nullptr, false, o, c.stream, rows);
+        else
+          launch_normalise<float>(nc, E, c.c32.get(), nullptr, false, o, c.stream, rows);
+        RGPU_HIP(hipMemcpyAsync(coeffs, o, sizeof(double) * nc * E, hipMemcpyDefault, c.stream));
       }
     }
-    if (key_mask && c.n_atoms)
-      RGPU_HIP(hipMemcpyAsync(key_mask, c.keymask.get(), sizeof(uint64_t) * c.n_atoms, hipMemcpyDefault,
-                              c.stream));
+    if (key_mask && nc) {
+      // per-atom masks of the expansion (neighbour species and the centre's
+      // own); the species-key method decides the keys of each centre
+      // (calculator_spherical_expansion.hh:2414-2570)
+      std::vector<uint64_t> km(static_cast<size_t>(c.n_atoms));
+      RGPU_HIP(hipMemcpyAsync(km.data(), c.keymask.get(), sizeof(uint64_t) * km.size(),
+                              hipMemcpyDeviceToHost, c.stream));
+      RGPU_HIP(hipStreamSynchronize(c.stream));
+      const std::string& method = c.hp.expansion_by_species;
+      std::vector<uint64_t> out(static_cast<size_t>(nc));
+      if (method == "user defined") {
+        uint64_t all = 0;
+        for (int z : c.hp.global_species) {
+          auto it = std::lower_bound(c.species.begin(), c.species.end(), z);
+          all |= 1ull << (it - c.species.begin());
+        }
+        std::fill(out.begin(), out.end(), all);
+      } else if (method == "structure wise") {
+        // every centre of a structure has the species of all of its centres
+        // and their neighbours
+        std::vector<uint64_t> per(static_cast<size_t>(c.n_struct), 0);
+        std::vector<int> st(static_cast<size_t>(nc));
+        for (int64_t r = 0, s = 0; r < nc; ++r) {
+          const int64_t a = c.centre_idx[size_t(r)];
+          while (c.atom_off[size_t(s) + 1] <= a) ++s;
+          st[size_t(r)] = int(s);
+          per[size_t(s)] |= km[size_t(a)];
+        }
+        for (int64_t r = 0; r < nc; ++r) out[size_t(r)] = per[size_t(st[size_t(r)])];
+      } else {
+        for (int64_t r = 0; r < nc; ++r) out[size_t(r)] = km[size_t(c.centre_idx[size_t(r)])];
+      }
+      RGPU_HIP(hipMemcpy(key_mask, out.data(), sizeof(uint64_t) * nc, hipMemcpyDefault));
+    }
     RGPU_HIP(hipStreamSynchronize(c.stream));
   });
 }
@@ -1277,22 +1424,56 @@ int rgpu_get_features(rgpu_context* ctx, double* features, double* inv_norms) {
     require_batch(c);
     require_invariants(c);
     ensure_rep(c);
-    const int64_t D = c.D(), n = c.n_atoms;
-    if (features && n) {
-      double* o = c.out64.reserve(size_t(n * D));
+    const int64_t D = c.D(), nc = c.n_centres;
+    const int64_t* rows = c.masked ? c.d_centre_idx.get() : nullptr;
+    if (features && nc) {
+      double* o = c.out64.reserve(size_t(nc * D));
       if (c.precision == RGPU_FP64)
-        launch_normalise<double>(n, D, c.p64.get(), c.norm2.get(), c.hp.normalize, o, c.stream);
+        launch_normalise<double>(nc, D, invariants_of<double>(c), c.norm2.get(), c.hp.normalize, o,
+                                 c.stream, rows);
       else
-        launch_normalise<float>(n, D, ensure_p32(c), c.norm2.get(), c.hp.normalize, o, c.stream);
-      RGPU_HIP(hipMemcpyAsync(features, o, sizeof(double) * n * D, hipMemcpyDefault, c.stream));
+        launch_normalise<float>(nc, D, ensure_p32(c), c.norm2.get(), c.hp.normalize, o, c.stream,
+                                rows);
+      RGPU_HIP(hipMemcpyAsync(features, o, sizeof(double) * nc * D, hipMemcpyDefault, c.stream));
     }
-    if (inv_norms && n) {
-      std::vector<double> h(static_cast<size_t>(n));
-      RGPU_HIP(hipMemcpyAsync(h.data(), c.norm2.get(), sizeof(double) * n, hipMemcpyDeviceToHost, c.stream));
+    if (inv_norms && nc) {
+      std::vector<double> h(static_cast<size_t>(c.n_atoms)), o(static_cast<size_t>(nc));
+      RGPU_HIP(hipMemcpyAsync(h.data(), c.norm2.get(), sizeof(double) * h.size(), hipMemcpyDeviceToHost,
+                              c.stream));
       RGPU_HIP(hipStreamSynchronize(c.stream));
-      for (auto& v : h) v = (c.hp.normalize && v > 0) ? 1.0 / std::sqrt(v) : 1.0;
-      RGPU_HIP(hipMemcpyAsync(inv_norms, h.data(), sizeof(double) * n, hipMemcpyDefault, c.stream));
+      for (int64_t r = 0; r < nc; ++r) {
+        const double v = h[size_t(c.centre_idx[size_t(r)])];
+        o[size_t(r)] = (c.hp.normalize && v > 0) ? 1.0 / std::sqrt(v) : 1.0;
+      }
+      RGPU_HIP(hipMemcpyAsync(inv_norms, o.data(), sizeof(double) * nc, hipMemcpyDefault, c.stream));
     }
+    RGPU_HIP(hipStreamSynchronize(c.stream));
+  });
+}
+
+int rgpu_get_features_gradient(rgpu_context* ctx, double* out) {
+  return guarded([&] {
+    if (!ctx || !out) throw InvalidArgument("null argument");
+    Context& c = ctx->c;
+    require_batch(c);
+    require_invariants(c);
+    require_unmasked(c);
+    if (c.precision != RGPU_FP64)
+      throw InvalidArgument("the feature gradients are computed in FP64 (set RGPU_FP64)");
+    ensure_rep(c);
+    const int S = int(c.species.size());
+    const int64_t n = c.n_atoms, np = c.pl.n_pairs, D = c.D();
+    const int64_t B = int64_t(c.rd.N) * c.rd.LL;
+    if (np == 0 || D == 0) return;
+    double* dcp = c.dcp.reserve(size_t(np * 3 * B));
+    double* dcs = c.dcs.reserve(size_t(std::max<int64_t>(n * 3 * S * B, 1)));
+    launch_pair_gradients(c.rd, S, pair_view(c), dcp, dcs, c.stream);
+    double* o = is_device_memory(out) ? out : c.tmp64.reserve(size_t(3 * np * D));
+    launch_feature_gradient_rows(c.rd.N, c.rd.L1, S, pair_view(c), np, c.c64.get(),
+                                 invariants_of<double>(c), c.norm2.get(), c.hp.normalize, dcp, dcs,
+                                 D, c.sparsified() ? c.d_sel_col.get() : nullptr, o, c.stream);
+    if (o != out)
+      RGPU_HIP(hipMemcpyAsync(out, o, sizeof(double) * 3 * np * D, hipMemcpyDeviceToHost, c.stream));
     RGPU_HIP(hipStreamSynchronize(c.stream));
   });
 }
@@ -1303,6 +1484,7 @@ int rgpu_features_backward(rgpu_context* ctx, const double* dL_dX, double* dL_dr
     Context& c = ctx->c;
     require_batch(c);
     require_invariants(c);
+    require_unmasked(c);
     ensure_rep(c);
     const int64_t D = c.D(), n = c.n_atoms;
     double* gd = c.grad.reserve(size_t(std::max<int64_t>(3 * n, 1)));
@@ -1370,7 +1552,8 @@ int rgpu_set_sparse_points(rgpu_context* ctx, int zeta, int64_t n_sparse, const 
     c.zeta = zeta;
     c.M = n_sparse;
     c.have_weights = weights != nullptr;
-    prepare_mixed_sparse(c.mixed_sparse, T, n_sparse, S, c.rd.N, c.rd.L1, c.stream);
+    // (the mixed path's packed operands; a sparsified layout is FP64 only)
+    if (!c.sparsified()) prepare_mixed_sparse(c.mixed_sparse, T, n_sparse, S, c.rd.N, c.rd.L1, c.stream);
     c.have_sparse = true;
   });
 }
@@ -1424,6 +1607,7 @@ int rgpu_gap_knm_gradient(rgpu_context* ctx, int neg_stress, double* out) {
     if (!c.have_sparse) throw InvalidArgument(kNoSparse);
     if (c.precision != RGPU_FP64)
       throw InvalidArgument("the K_NM gradient rows are computed in FP64 (set RGPU_FP64)");
+    require_unmasked(c);
     ensure_rep(c);
     const int S = int(c.species.size());
     const int64_t n = c.n_atoms, M = c.M, D = c.D(), ns = c.n_struct;
@@ -1434,7 +1618,7 @@ int rgpu_gap_knm_gradient(rgpu_context* ctx, int neg_stress, double* out) {
     double* d = c.tmp64.reserve(size_t(std::max<int64_t>(n * M, 1)));
     RGPU_HIP(hipMemsetAsync(d, 0, sizeof(double) * n * M, c.stream));
     for (int s = 0; s < S; ++s) {
-      GapBlock<double> blk = gap_block<double>(c, s, c.p64.get(), c.T64.get());
+      GapBlock<double> blk = gap_block<double>(c, s, invariants_of<double>(c), c.T64.get());
       blk.alpha = nullptr;
       blk.zeta = 1;
       if (blk.n_rows == 0 || blk.n_cols == 0) continue;
@@ -1444,6 +1628,8 @@ int rgpu_gap_knm_gradient(rgpu_context* ctx, int neg_stress, double* out) {
     double* gd = c.grad.reserve(size_t(std::max<int64_t>(3 * n, 1)));
     double* cv = c.cvir.reserve(size_t(std::max<int64_t>(6 * n, 1)));
     double* st = c.out64.reserve(size_t(6 * std::max<int64_t>(ns, 1)));
+    // a sparsified row enters the adjoint scattered into the full layout
+    double* grow = c.sparsified() ? c.gfull.reserve(size_t(c.D_full())) : nullptr;
     // column m (species-sorted): dL/dX_i = zeta d_im^(zeta-1) T_m for the
     // centres of m's species, through the adjoint; rows written in the
     // caller's sparse-point order
@@ -1451,8 +1637,13 @@ int rgpu_gap_knm_gradient(rgpu_context* ctx, int neg_stress, double* out) {
       for (int64_t m = c.sp_off[s]; m < c.sp_off[s + 1]; ++m) {
         launch_knm_gradient_scale(n, d + m, M, c.d_species_of.get(), s, c.zeta, gs, c.stream);
         RGPU_HIP(hipMemsetAsync(gd, 0, sizeof(double) * 3 * n, c.stream));
+        const double* row = c.T64.get() + m * D;
+        if (grow) {
+          launch_scatter_columns(1, D, c.d_sel_col.get(), row, c.D_full(), grow, c.stream);
+          row = grow;
+        }
         launch_backward_rank1(c.rd, S, pair_view(c), c.c64.get(), c.p64.get(), c.norm2.get(),
-                              c.hp.normalize, c.T64.get() + m * D, gs, gd, cv, c.stream, c.tile());
+                              c.hp.normalize, row, gs, gd, cv, c.stream, c.tile());
         const int64_t col = c.sp_order[size_t(m)];
         launch_strided_copy(gd, 3 * n, o + col, M, c.stream);
         if (neg_stress) {

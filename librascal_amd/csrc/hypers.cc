@@ -52,6 +52,49 @@ void parse_projections(const json::Value& rdr, SoapHypers& o) {
     }
   }
 }
+std::vector<int> int_list(const json::Value& v) {
+  if (v.kind != json::Value::Array) throw InvalidArgument("json: value is not a list");
+  std::vector<int> out;
+  for (const json::Value& x : v.arr) out.push_back(static_cast<int>(x.as_number()));
+  return out;
+}
+
+// coefficient_subselection {"a", "b", "n1", "n2", "l"} with the checks of
+// calculator_spherical_invariants.hh:315-393 (the same messages)
+void parse_subselection(const json::Value& cs, SoapHypers& o) {
+  o.sparsified = true;
+  o.sel_a = int_list(cs.at("a"));
+  o.sel_b = int_list(cs.at("b"));
+  o.sel_n1 = int_list(cs.at("n1"));
+  o.sel_n2 = int_list(cs.at("n2"));
+  o.sel_l = int_list(cs.at("l"));
+  const size_t n = o.sel_a.size();
+  const char* pre = "coefficient_subselection should have elements of the same size but: ";
+  if (o.sel_b.size() != n) throw InvalidArgument(std::string(pre) + "a.size() != b.size()");
+  if (o.sel_n1.size() != n) throw InvalidArgument(std::string(pre) + "a.size() != n1.size()");
+  if (o.sel_n2.size() != n) throw InvalidArgument(std::string(pre) + "a.size() != n2.size()");
+  if (o.sel_l.size() != n) throw InvalidArgument(std::string(pre) + "a.size() != l.size()");
+  if (n == 0) throw InvalidArgument("coefficient_subselection selects no coefficient");
+  auto max_of = [](const std::vector<int>& v) { return *std::max_element(v.begin(), v.end()); };
+  auto min_of = [](const std::vector<int>& v) { return *std::min_element(v.begin(), v.end()); };
+  if (min_of(o.sel_n1) < 0 || min_of(o.sel_n2) < 0 || min_of(o.sel_l) < 0)
+    throw InvalidArgument("coefficient_subselection indices must be non-negative");
+  const int lmax = max_of(o.sel_l), n1max = max_of(o.sel_n1), n2max = max_of(o.sel_n2);
+  if (lmax >= o.max_angular + 1)
+    throw RuntimeError("'max(l)' in coefficient_subselection is larger than 'max_angular+1'" +
+                       std::to_string(lmax) + " >= " + std::to_string(o.max_angular + 1));
+  if (n1max >= o.max_radial)
+    throw RuntimeError("'max(n1)' in coefficient_subselection is larger than 'max_radial'" +
+                       std::to_string(n1max) + " >= " + std::to_string(o.max_radial));
+  if (n2max >= o.max_radial)
+    throw RuntimeError("'max(n2)' in coefficient_subselection is larger than 'max_radial'" +
+                       std::to_string(n2max) + " >= " + std::to_string(o.max_radial));
+  for (size_t k = 0; k < n; ++k)
+    if (o.sel_a[k] > o.sel_b[k])
+      throw InvalidArgument(
+          "coefficient_subselection should have only lexicographically sorted pairs of atomic "
+          "species in the selection but: a > b");
+}
 }  // namespace
 
 SoapHypers parse_soap_hypers(const std::string& text) {
@@ -79,8 +122,7 @@ SoapHypers parse_soap_hypers(const std::string& text) {
                               "'PowerSpectrum', 'RadialSpectrum', or 'BiSpectrum'.");
       }
       o.normalize = h.at("normalize").as_bool();
-      if (h.has("coefficient_subselection"))
-        throw InvalidArgument("coefficient_subselection is not supported by this library");
+      if (h.has("coefficient_subselection")) parse_subselection(h.at("coefficient_subselection"), o);
     } else {
       o.kind = SoapHypers::kExpansion;
     }

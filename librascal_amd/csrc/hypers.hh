@@ -30,6 +30,10 @@ struct SoapHypers {
   bool compute_gradients = false;
   std::string expansion_by_species = "environment wise";
   std::vector<int> global_species;
+  // coefficient_subselection (sparsified power spectrum): feature k is
+  // p^{sel_a[k] sel_b[k]}[sel_n1[k], sel_n2[k], sel_l[k]] (atomic numbers a <= b)
+  bool sparsified = false;
+  std::vector<int> sel_a, sel_b, sel_n1, sel_n2, sel_l;
   std::string json;  // the hypers as given (calculator name, serialisation)
 };
 

@@ -55,10 +55,34 @@ void launch_power_spectrum_split(const RadialDev& rad, int S, int64_t n_centres,
                                  unsigned short* p_hi, unsigned short* p_lo, double* norm2,
                                  hipStream_t stream, const SoapTiles& tiles = SoapTiles());
 
-// Normalised copy X = p / |p| (or X = p), FP64 output for export.
+// Normalised copy X = p / |p| (or X = p), FP64 output for export: n rows,
+// row r from row rows[r] of p (rows null: row r).
 template <class T>
-void launch_normalise(int64_t n_centres, int64_t D, const T* p, const double* norm2, bool normalize,
-                      double* out, hipStream_t stream);
+void launch_normalise(int64_t n, int64_t D, const T* p, const double* norm2, bool normalize,
+                      double* out, hipStream_t stream, const int64_t* rows = nullptr);
+
+// Stored expansion derivatives (FP64) for the feature-gradient export:
+// dcp[p][3][N][(L+1)^2] per pair (the self slot unused) and the centre
+// blocks dcs[i][3][S][N][(L+1)^2] (calculator_spherical_expansion.hh
+// :2306-2401).
+void launch_pair_gradients(const RadialDev& rad, int S, const PairView& pv, double* dcp, double* dcs,
+                           hipStream_t stream);
+// Feature-gradient rows out[3 n_pairs][Dout] (row 3 r + x = dX_i / dr_x of
+// pair row r, self pair first per centre) from c (FP64, full layout), the
+// un-normalised p of the output columns p_rows[n_centres][Dout], norm2 and
+// the derivatives above; col (may be null) maps output columns to the full
+// layout (-1: zero column).
+void launch_feature_gradient_rows(int N, int L1, int S, const PairView& pv, int64_t n_pairs,
+                                  const double* c, const double* p_rows, const double* norm2,
+                                  bool normalize, const double* dcp, const double* dcs, int64_t Dout,
+                                  const int64_t* col, double* out, hipStream_t stream);
+// coefficient_subselection: xs[i][k] = p[i][col[k]], norm2[i] = |xs_i|^2,
+// then p[i] = S^T xs_i (the selected coefficients, zero elsewhere)
+void launch_sparsify(int64_t n, int64_t D, int64_t Dsel, const int64_t* col, double* p, double* xs,
+                     double* norm2, hipStream_t stream);
+// dst[i][:D] = S^T src[i][:Dsel] (repeated selections summed)
+void launch_scatter_columns(int64_t n, int64_t Dsel, const int64_t* col, const double* src, int64_t D,
+                            double* dst, hipStream_t stream);
 
 // Adjoint (FP64): given G = dL/dX per centre (X the exported features), accumulate
 // dL/dr on atoms (grad[n_atoms][3], +=) and write per-centre virials

@@ -736,15 +736,18 @@ __global__ __launch_bounds__(kThreads) void power_spectrum_kernel(int N, int L, 
   }
 }
 
+// out row r = row rows[r] of p (rows null: row r), divided by its norm when
+// normalising
 template <class T>
 __global__ void normalise_kernel(int64_t n, int64_t D, const T* __restrict__ p,
                                  const double* __restrict__ norm2, int normalize,
-                                 double* __restrict__ out) {
+                                 const int64_t* __restrict__ rows, double* __restrict__ out) {
   const int64_t total = n * D;
   for (int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; e < total;
        e += int64_t(gridDim.x) * blockDim.x) {
-    const int64_t i = e / D;
-    double v = double(p[e]);
+    const int64_t r = e / D;
+    const int64_t i = rows ? rows[r] : r;
+    double v = double(p[i * D + (e - r * D)]);
     if (normalize) {
       const double nr = sqrt(norm2[i]);
       if (nr > 0) v /= nr;
@@ -1399,6 +1402,191 @@ __global__ __launch_bounds__(kThreads) void ylm_eval_kernel(RadialDev rad, const
     for (int t = threadIdx.x; t < nb * 3 * LL; t += kThreads) dY[q0 * 3 * LL + t] = double(dYsh[t]);
 }
 
+// ---------------------------------------------------------------------------
+// Stored representation gradients (FP64), as the reference keeps them for
+// get_features_gradient.  Step 1, per-pair expansion derivatives
+// (calculator_spherical_expansion.hh:2306-2401): for the pair p = (i, j) that
+// is not the self pair,
+//   dcp[p][x][n][lm] = d c^{i, s_j}[n][lm] / d r_j,x
+//                    = (R' fc + R fc') Y r_hat_x + R fc dY_x / r
+// (dY_x from the two-term identity of the adjoint), and for the centre
+//   dcs[i][x][s][n][lm] = - sum of dcp over its pairs of species s that are
+// not periodic images of i itself.  One workgroup per centre; each thread
+// owns fixed (x, n, lm) entries, so it alone updates their centre sums.
+template <bool DIRECT>
+__global__ __launch_bounds__(kThreads) void pair_dc_kernel(RadialDev rad, int S, int PB, PairView pv,
+                                                           double* __restrict__ dcp,
+                                                           double* __restrict__ dcs) {
+  typedef double T;
+  const int64_t i = blockIdx.x;
+  const int N = rad.N, L1 = rad.L1, LL = rad.LL, F = N * L1, B = N * LL;
+  const int tid = threadIdx.x;
+  extern __shared__ __align__(64) unsigned char smem_raw[];
+  T* yc = reinterpret_cast<T*>(smem_raw);
+  PairBatch<T> pb = carve_batch<T, DIRECT, true>(smem_raw + ylm_table_bytes<T>(L1), PB, LL, F);
+  load_ylm_table<T>(rad, yc);
+  {
+    const YGradTerm<T>* src = ygrad_src<T>(rad);
+    for (int e = tid; e < 3 * LL; e += kThreads) pb.gt[e] = src[e];
+  }
+  double* ds = dcs + i * int64_t(3) * S * B;
+  for (int e = tid; e < 3 * S * B; e += kThreads) ds[e] = 0.0;
+  const int64_t p0 = pv.centre_off[i] + 1;  // the self pair has no derivative of its own
+  const int np = int(pv.centre_off[i + 1] - p0);
+  const FastDiv fB = fast_div(B), fLL = fast_div(LL);
+  PairPre pre;
+  pair_fetch<DIRECT, true>(rad, pv, p0, max(0, min(PB, np)), pre);
+  for (int q0 = 0; q0 < np; q0 += PB) {
+    const int nb = min(PB, np - q0);
+    stage_pairs<T, DIRECT, true>(rad, yc, pre, nb, pb, nullptr, nullptr, nullptr);
+    if (q0 + PB < np) pair_fetch<DIRECT, true>(rad, pv, p0 + q0 + PB, min(PB, np - q0 - PB), pre);
+    for (int e = tid; e < 3 * B; e += kThreads) {
+      const int x = e / fB, rr = e - x * B, n = rr / fLL, lm = rr - n * LL;
+      const int l = l_of_lm(lm);
+      const YGradTerm<T> g = pb.gt[3 * lm + x];
+      for (int q = 0; q < nb; ++q) {
+        const T* Yq = pb.Y + q * LL;
+        const double rx = pb.dir[3 * q + x];
+        const double fc = pb.fc[q], dfc = pb.dfc[q];
+        const double R = pb.R[q * F + l * N + n], dR = pb.dR[q * F + l * N + n];
+        const double dy = g.c0 * Yq[g.i0] + g.c1 * Yq[g.i1] - double(l) * rx * Yq[lm];
+        const double v = (dR * fc + R * dfc) * Yq[lm] * rx + R * fc * dy / pb.dist[q];
+        dcp[(p0 + q0 + q) * 3 * B + e] = v;
+        if (pb.pj[q] != int(i)) ds[(x * S + pb.sp[q]) * B + rr] -= v;
+      }
+    }
+    __syncthreads();  // the next batch overwrites this one
+  }
+}
+
+// Step 2, one workgroup per pair row r (centre i, self pair first): the
+// three rows dX/dr_x of the feature gradient matrix
+// (calculator_spherical_invariants.hh:796-921, normalisation :588-652):
+//   dp^{ab}[n1 n2 l] = f_l w_ab sum_m (dA[n1][lm] c_b[n2][lm] + c_a[n1][lm] dB[n2][lm])
+// with dA, dB the derivatives of c^{i,a}, c^{i,b}: the centre's own blocks
+// for the self row; for a neighbour row the pair derivative in the
+// neighbour species' block and zero in the others.  Then, when normalising,
+// dX = dp/|p| - p (p . dp)/|p|^3.  Output columns k: the full layout, or the
+// selected coefficients col[k] (full-layout index, -1 for a species pair
+// outside the batch); pv_rows[i][k] holds the un-normalised p of column k.
+__global__ __launch_bounds__(kThreads) void feature_gradient_rows_kernel(
+    int N, int L1, int S, PairView pv, const double* __restrict__ c,
+    const double* __restrict__ pv_rows, const double* __restrict__ norm2, int normalize,
+    const double* __restrict__ dcp, const double* __restrict__ dcs, int64_t Dout,
+    const int64_t* __restrict__ col, double* __restrict__ out) {
+  const int64_t r = blockIdx.x;
+  int64_t lo = 0, hi = pv.n_centres;  // the centre i with off[i] <= r < off[i + 1]
+  while (hi - lo > 1) {
+    const int64_t mid = (lo + hi) >> 1;
+    if (pv.centre_off[mid] <= r)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  const int64_t i = lo;
+  const bool self = r == pv.centre_off[i];
+  const int LL = L1 * L1, B = N * LL, KB = N * N * L1;
+  const int s = self ? -1 : pv.species_of[pv.pair_j[r]];
+  const double* ci = c + i * int64_t(S) * B;
+  const double* pi = pv_rows + i * Dout;
+  double dot[3] = {0, 0, 0};
+  for (int64_t k = threadIdx.x; k < Dout; k += kThreads) {
+    const int64_t o = col ? col[k] : k;
+    double g[3] = {0, 0, 0};
+    if (o >= 0) {
+      const int key = int(o / KB), rem = int(o - int64_t(key) * KB);
+      const int n1 = rem / (N * L1), rem2 = rem - n1 * N * L1, n2 = rem2 / L1, l = rem2 - n2 * L1;
+      int a, b;
+      key_pair(key, S, &a, &b);
+      const double f = (a < b ? 1.41421356237309504880 : 1.0) / sqrt(double(2 * l + 1));
+      const double* ca = ci + (a * N + n1) * LL;
+      const double* cb = ci + (b * N + n2) * LL;
+      for (int x = 0; x < 3; ++x) {
+        const double* dA = self ? dcs + ((i * 3 + x) * S + a) * B + n1 * LL
+                                : (a == s ? dcp + (r * 3 + x) * B + n1 * LL : nullptr);
+        const double* dB = self ? dcs + ((i * 3 + x) * S + b) * B + n2 * LL
+                                : (b == s ? dcp + (r * 3 + x) * B + n2 * LL : nullptr);
+        double acc = 0;
+        for (int lm = l * l; lm < (l + 1) * (l + 1); ++lm) {
+          if (dA) acc += dA[lm] * cb[lm];
+          if (dB) acc += ca[lm] * dB[lm];
+        }
+        g[x] = f * acc;
+        dot[x] += pi[k] * g[x];
+      }
+    }
+    for (int x = 0; x < 3; ++x) out[(r * 3 + x) * Dout + k] = g[x];
+  }
+  if (!normalize) return;
+  __shared__ double red[kWaves][3];
+  for (int x = 0; x < 3; ++x) {
+    double v = dot[x];
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6][x] = v;
+  }
+  __syncthreads();
+  double tot[3];
+  for (int x = 0; x < 3; ++x) {
+    tot[x] = 0;
+    for (int w = 0; w < kWaves; ++w) tot[x] += red[w][x];
+  }
+  const double n2 = norm2[i];
+  const double inv = n2 > 0 ? 1.0 / sqrt(n2) : 0.0;
+  for (int64_t k = threadIdx.x; k < Dout; k += kThreads)
+    for (int x = 0; x < 3; ++x) {
+      double& v = out[(r * 3 + x) * Dout + k];
+      v = v * inv - pi[k] * tot[x] * inv * inv * inv;
+    }
+}
+
+// Sparsified invariants (coefficient_subselection): xs[i][k] = p[i][col[k]]
+// (0 for col[k] < 0) and norm2[i] = |xs_i|^2; then row i of p becomes S^T xs_i
+// for the selection matrix S (the selected coefficients, summed over
+// repeated selections, zero elsewhere): with it and |xs_i| the adjoint's
+// normalisation term is the one of the selected features.
+__global__ __launch_bounds__(kThreads) void sparsify_kernel(int64_t D, int64_t Dsel,
+                                                            const int64_t* __restrict__ col,
+                                                            double* __restrict__ p,
+                                                            double* __restrict__ xs,
+                                                            double* __restrict__ norm2) {
+  const int64_t i = blockIdx.x;
+  double* pi = p + i * D;
+  double* xi = xs + i * Dsel;
+  double ss = 0;
+  for (int64_t k = threadIdx.x; k < Dsel; k += kThreads) {
+    const double v = col[k] >= 0 ? pi[col[k]] : 0.0;
+    xi[k] = v;
+    ss += v * v;
+  }
+  __shared__ double red[kWaves];
+  for (int off = 32; off > 0; off >>= 1) ss += __shfl_down(ss, off, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = ss;
+  __syncthreads();  // also: every selected value has been read
+  if (threadIdx.x == 0) {
+    double t = 0;
+    for (int w = 0; w < kWaves; ++w) t += red[w];
+    norm2[i] = t;
+  }
+  for (int64_t o = threadIdx.x; o < D; o += kThreads) pi[o] = 0.0;
+  __syncthreads();
+  for (int64_t k = threadIdx.x; k < Dsel; k += kThreads)
+    if (col[k] >= 0) unsafeAtomicAdd(pi + col[k], xi[k]);
+}
+
+// dst[i] = S^T src[i] (rows of Dsel selected columns into rows of D
+// columns, repeated selections summed), one workgroup per row
+__global__ __launch_bounds__(kThreads) void scatter_columns_kernel(int64_t Dsel,
+                                                                   const int64_t* __restrict__ col,
+                                                                   const double* __restrict__ src,
+                                                                   int64_t D, double* __restrict__ dst) {
+  const int64_t i = blockIdx.x;
+  double* di = dst + i * D;
+  for (int64_t o = threadIdx.x; o < D; o += kThreads) di[o] = 0.0;
+  __syncthreads();
+  for (int64_t k = threadIdx.x; k < Dsel; k += kThreads)
+    if (col[k] >= 0) unsafeAtomicAdd(di + col[k], src[i * Dsel + k]);
+}
+
 // LDS per workgroup: the forward and the power spectrum stay small enough
 // for several workgroups per CU; the adjoint may use more.
 constexpr size_t kFwdBudget = 32 * 1024;
@@ -1593,12 +1781,60 @@ void launch_power_spectrum_split(const RadialDev& rad, int S, int64_t n, const f
 
 template <class T>
 void launch_normalise(int64_t n, int64_t D, const T* p, const double* norm2, bool normalize, double* out,
-                      hipStream_t stream) {
-  if (n == 0) return;
+                      hipStream_t stream, const int64_t* rows) {
+  if (n == 0 || D == 0) return;
   int64_t total = n * D;
   unsigned blocks = unsigned(std::min<int64_t>((total + 255) / 256, 65536));
   hipLaunchKernelGGL(normalise_kernel<T>, dim3(blocks), dim3(256), 0, stream, n, D, p, norm2,
-                     int(normalize), out);
+                     int(normalize), rows, out);
+  RGPU_LAUNCH_CHECK();
+}
+
+void launch_pair_gradients(const RadialDev& rad, int S, const PairView& pv, double* dcp, double* dcs,
+                           hipStream_t stream) {
+  if (pv.n_centres == 0) return;
+  const int F = rad.N * rad.L1;
+  const size_t yt = ylm_table_bytes<double>(rad.L1);
+  const bool direct = !rad.use_spline;
+  int PB = 16;
+  while (PB > 1 && yt + pair_batch_bytes(PB, rad.LL, F, sizeof(double), direct, true) > kAdjBudget)
+    PB /= 2;
+  const size_t smem = yt + pair_batch_bytes(PB, rad.LL, F, sizeof(double), direct, true);
+  auto go = [&](auto kern) {
+    allow_lds(kern, smem);
+    hipLaunchKernelGGL(kern, dim3(unsigned(pv.n_centres)), dim3(kThreads), smem, stream, rad, S, PB,
+                       pv, dcp, dcs);
+  };
+  if (direct)
+    go(pair_dc_kernel<true>);
+  else
+    go(pair_dc_kernel<false>);
+  RGPU_LAUNCH_CHECK();
+}
+
+void launch_feature_gradient_rows(int N, int L1, int S, const PairView& pv, int64_t n_pairs,
+                                  const double* c, const double* p_rows, const double* norm2,
+                                  bool normalize, const double* dcp, const double* dcs, int64_t Dout,
+                                  const int64_t* col, double* out, hipStream_t stream) {
+  if (n_pairs == 0 || Dout == 0) return;
+  hipLaunchKernelGGL(feature_gradient_rows_kernel, dim3(unsigned(n_pairs)), dim3(kThreads), 0, stream,
+                     N, L1, S, pv, c, p_rows, norm2, int(normalize), dcp, dcs, Dout, col, out);
+  RGPU_LAUNCH_CHECK();
+}
+
+void launch_sparsify(int64_t n, int64_t D, int64_t Dsel, const int64_t* col, double* p, double* xs,
+                     double* norm2, hipStream_t stream) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(sparsify_kernel, dim3(unsigned(n)), dim3(kThreads), 0, stream, D, Dsel, col, p, xs,
+                     norm2);
+  RGPU_LAUNCH_CHECK();
+}
+
+void launch_scatter_columns(int64_t n, int64_t Dsel, const int64_t* col, const double* src, int64_t D,
+                            double* dst, hipStream_t stream) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(scatter_columns_kernel, dim3(unsigned(n)), dim3(kThreads), 0, stream, Dsel, col, src,
+                     D, dst);
   RGPU_LAUNCH_CHECK();
 }
 
@@ -1759,9 +1995,9 @@ template void launch_power_spectrum<double>(const RadialDev&, int, int64_t, cons
 template void launch_power_spectrum<float>(const RadialDev&, int, int64_t, const float*, float*,
                                            double*, hipStream_t, const SoapTiles&);
 template void launch_normalise<double>(int64_t, int64_t, const double*, const double*, bool, double*,
-                                       hipStream_t);
+                                       hipStream_t, const int64_t*);
 template void launch_normalise<float>(int64_t, int64_t, const float*, const double*, bool, double*,
-                                      hipStream_t);
+                                      hipStream_t, const int64_t*);
 template void launch_backward<double>(const RadialDev&, int, const PairView&, const double*,
                                       const double*, const double*, bool, const double*, double,
                                       const double*, double*, double*, hipStream_t, const SoapTiles&);

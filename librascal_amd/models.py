@@ -42,18 +42,23 @@ class SparsePoints:
             raise ValueError("atoms_list must be an AtomsList")
         if len(selected_indices) != len(atoms_list):
             raise ValueError("selected_indices needs one list per structure")
+        # centres of each structure (all atoms, or those of the centre mask)
+        cidx = np.arange(atoms_list.n_atoms)
+        if atoms_list.center_atoms_mask is not None:
+            cidx = cidx[atoms_list.center_atoms_mask]
+        coff = np.searchsorted(cidx, atoms_list.atom_offsets)
         rows = []
         for s, ids in enumerate(selected_indices):
-            n_s = atoms_list.atom_offsets[s + 1] - atoms_list.atom_offsets[s]
+            n_s = coff[s + 1] - coff[s]
             for i in ids:
                 if not 0 <= int(i) < n_s:
                     raise IndexError("centre index %d out of range for structure %d" % (i, s))
-                rows.append(int(atoms_list.atom_offsets[s] + int(i)))
+                rows.append(int(coff[s] + int(i)))
         if not rows:
             return self
         X = atoms_list.get_dense_features(self.representation)
         rows = np.asarray(rows)
-        self._chunks.append((tuple(atoms_list.species), atoms_list.numbers[rows].copy(),
+        self._chunks.append((tuple(atoms_list.species), atoms_list.numbers[cidx[rows]].copy(),
                              np.ascontiguousarray(X[rows])))
         self._cache.clear()
         self.version += 1
@@ -65,7 +70,7 @@ class SparsePoints:
         `species` (one row per point, centre atomic numbers `numbers`)."""
         sp = cls(representation)
         values = np.ascontiguousarray(values, np.float64)
-        if values.shape[1] != representation.get_num_coefficients(len(species)):
+        if values.shape[1] != _feature_size(representation, len(species)):
             raise ValueError("values do not match the species list")
         sp._chunks.append((tuple(sorted(species)), np.asarray(numbers, np.int32).copy(), values))
         sp.version += 1
@@ -94,6 +99,12 @@ class SparsePoints:
         if key in self._cache:
             return self._cache[key]
         rep = self.representation
+        if _sparsified(rep):
+            # one key holding the selected coefficients: no relayout
+            vals = (np.concatenate([X for _, _, X in self._chunks]) if self._chunks
+                    else np.zeros((0, _feature_size(rep, 0))))
+            self._cache[key] = vals
+            return vals
         inner = rep.get_num_coefficients(1)
         out_keys = rep.get_keys(key)
         parts = []
@@ -112,6 +123,18 @@ class SparsePoints:
     def get_features(self):
         """Dense features over the species of all the points."""
         return self.values_for(self.all_species)
+
+
+def _sparsified(rep):
+    return rep.hypers.get("coefficient_subselection") is not None
+
+
+def _feature_size(rep, n_species):
+    """Features per centre: the selected coefficients of a sparsified power
+    spectrum, else the dense layout over n_species species."""
+    if _sparsified(rep):
+        return len(rep.hypers["coefficient_subselection"]["a"])
+    return rep.get_num_coefficients(n_species)
 
 
 def _set_model(rep, sparse_points, zeta, weights):
@@ -235,13 +258,18 @@ class KRR:
         self.units = units
 
     def _get_property_baseline(self, managers):
+        """Self contributions of the centres (krr.py:68-90)."""
+        mask = managers.center_atoms_mask
         if self.target_type == "Structure":
             Y0 = np.zeros(len(managers))
             for s in range(len(managers)):
-                for z in managers.numbers[managers.atom_offsets[s]:managers.atom_offsets[s + 1]]:
-                    Y0[s] += self.self_contributions[int(z)]
+                a0, a1 = managers.atom_offsets[s], managers.atom_offsets[s + 1]
+                for k, z in enumerate(managers.numbers[a0:a1]):
+                    if mask is None or mask[a0 + k]:
+                        Y0[s] += self.self_contributions[int(z)]
             return Y0
-        return np.array([self.self_contributions[int(z)] for z in managers.numbers])
+        nums = managers.numbers if mask is None else managers.numbers[mask]
+        return np.array([self.self_contributions[int(z)] for z in nums])
 
     def predict(self, managers, KNM=None):
         rep = self.kernel._rep
@@ -268,8 +296,8 @@ class KRR:
             g = compute_sparse_kernel_gradients(self.kernel._rep, self.kernel, managers,
                                                 self.X_train, self.weights)
         else:
-            if 3 * managers.n_centres != KNM.shape[0]:
-                raise ValueError("KNM size mismatch {}!={}".format(3 * managers.n_centres,
+            if 3 * managers.n_atoms != KNM.shape[0]:
+                raise ValueError("KNM size mismatch {}!={}".format(3 * managers.n_atoms,
                                                                    KNM.shape[0]))
             g = (KNM @ self.weights).reshape(-1, 3)
         return -g
@@ -302,6 +330,23 @@ class KRR:
 
     def get_representation_calculator(self):
         return self.kernel._rep
+
+
+def compute_KNM(frames, X_sparse, kernel, soap):
+    """GAP kernel of structures against the sparse points, stacked with its
+    position gradients (krr.py:297-348): rows [n_frames] of summed
+    (Structure) kernels, then 3 n_atoms gradient rows, frame by frame.  All
+    frames are computed as one batch on the GPU (the reference loops over
+    them); a tqdm-wrapped list is accepted."""
+    if hasattr(frames, "iterable"):
+        frames = frames.iterable
+    frames = list(frames)
+    al = soap.transform(frames)
+    if kernel.target_type != "Structure":
+        raise ValueError("compute_KNM builds Structure kernel rows")
+    en = kernel(al, X_sparse)
+    grad = kernel(al, X_sparse, grad=(True, False))
+    return np.vstack([en, grad])
 
 
 def _gram_on_device(A):
@@ -354,4 +399,4 @@ def train_gap_model(kernel, frames, KNM_, X_sparse, y_train, self_contributions,
 
 
 __all__ = ["SparsePoints", "Kernel", "KRR", "compute_sparse_kernel_gradients",
-           "compute_sparse_kernel_neg_stress", "train_gap_model"]
+           "compute_sparse_kernel_neg_stress", "compute_KNM", "train_gap_model"]

@@ -161,7 +161,28 @@ def main():
     ski = json.load(open(os.path.join(tests_only, "sparse_kernel_inputs.json")))
     json.dump(ski, open(os.path.join(HERE, "sparse_kernel_inputs.json"), "w"), indent=1)
     print("golden fixtures written to", HERE)
+    sparsification()
+
+
+def sparsification():
+    """Sparsified power spectrum fixture (test_calculator.cc:622-748): the
+    full and coefficient_subselection hypers per case, the global ids of the
+    selected coefficients in the reference's per-centre full vector, and the
+    input structures they name."""
+    tests_only = os.path.join(DATA, "tests_only")
+    inputs = os.path.join(DATA, "inputs")
+    cases = json.load(open(os.path.join(tests_only, "sparsification_inputs.json")))
+    files = sorted(set(os.path.basename(c["filename"]) for c in cases))
+    out = dict(cases=[dict(structure=os.path.basename(c["filename"]), rep=c["hypers"]["rep"],
+                           rep_sparse=c["hypers"]["rep_sparse"],
+                           adaptors=c["hypers"]["adaptors"]) for c in cases],
+               structures={f: ase_json_structures(os.path.join(inputs, f)) for f in files})
+    json.dump(out, open(os.path.join(HERE, "sparsification_inputs.json"), "w"))
+    print("sparsification fixture written")
 
 
 if __name__ == "__main__":
-    main()
+    if len(sys.argv) > 1 and sys.argv[1] == "sparsification":
+        sparsification()
+    else:
+        main()

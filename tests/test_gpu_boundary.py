@@ -1,0 +1,409 @@
+"""The rest of the reference's calculator surface on the GPU: centre masks,
+the species-key methods, the sparsified power spectrum
+(coefficient_subselection), the stored feature gradients
+(get_features_gradient), compute_KNM, the neighbour-list pins of the
+reference's tests and its structure errors.  FP64 path, 1e-10 relative
+unless a test says otherwise."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+from conftest import GOLDEN, golden_structures, oracle_backward, rel_err, soap_hypers
+from librascal_amd import _capi
+from librascal_amd.models import KRR, Kernel, SparsePoints, compute_KNM
+from librascal_amd.neighbourlist import (AtomsList, mask_center_atoms_by_id,
+                                         mask_center_atoms_by_species)
+from librascal_amd.representations import SphericalInvariants, from_hypers
+from librascal_amd.synthetic import pack
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-10
+
+
+def _rel(ref, test):
+    ref = np.asarray(ref)
+    return float(np.abs(np.asarray(test) - ref).max() / max(np.abs(ref).max(), 1e-300))
+
+
+def _frames():
+    g = golden_structures()
+    return g["CaCrP2O7"] + g["small_molecule"] + g["SiC_moissanite_supercell"]
+
+
+# --------------------------------------------------------------------------
+# centre masks (test_calculator.cc:293-350: the rows of the selected
+# centres equal the rows of the full computation)
+
+def test_centre_mask_rows_match_the_full_batch():
+    hp = soap_hypers(rc=3.5, N=4, L=3, sigma=0.4)
+    rep = from_hypers(hp)
+    frames = [dict(f) for f in _frames()]
+    full = rep.transform(frames)
+    X_full = full.get_dense_features(rep).copy()
+    c_full, km_full = rep._context.expansion()
+    rng = np.random.default_rng(1)
+    masked = []
+    for f in frames:
+        f = dict(f)
+        n = len(f["numbers"])
+        mask_center_atoms_by_id(f, id_select=np.sort(rng.choice(n, size=max(1, n // 2),
+                                                                  replace=False)))
+        masked.append(f)
+    al = rep.transform(masked)
+    mask = al.center_atoms_mask
+    assert mask is not None and al.n_centres == int(mask.sum()) < al.n_atoms
+    X = al.get_dense_features(rep)
+    assert X.shape == (al.n_centres, X_full.shape[1])
+    assert np.array_equal(X, X_full[mask])
+    c, km = rep._context.expansion()
+    assert np.array_equal(c, c_full[mask]) and np.array_equal(km, km_full[mask])
+    info = al.get_representation_info()
+    assert np.array_equal(info[:, 2], al.numbers[mask])
+    # GAP: structure energies sum the selected centres only; Atom rows are
+    # the selected centres'
+    sp = SparsePoints(rep)
+    sp.extend(full, [[0, 1]] + [[] for _ in range(len(full) - 1)])
+    kern = Kernel(rep, name="GAP", zeta=2, target_type="Atom", kernel_type="Sparse")
+    k_atom_full = kern(full, sp).copy()
+    k_atom = kern(al, sp)
+    assert np.array_equal(k_atom, k_atom_full[mask])
+    kern.target_type = "Structure"
+    k_struct = kern(al, sp)
+    off = al.atom_offsets
+    ref = np.array([k_atom_full[off[s]:off[s + 1]][mask[off[s]:off[s + 1]]].sum(axis=0)
+                    for s in range(len(al))])
+    assert _rel(ref, k_struct) < TOL
+    w = np.random.default_rng(2).normal(size=sp.size())
+    model = KRR(w, kern, sp, {int(z): 0.25 for z in set(al.numbers.tolist())})
+    e = model.predict(al)
+    y0 = np.array([0.25 * mask[off[s]:off[s + 1]].sum() for s in range(len(al))])
+    assert _rel(ref @ w + y0, e) < TOL
+    # gradients of masked centres are refused, as in the reference
+    with pytest.raises(Exception, match="masked center atoms"):
+        model.predict_forces(al)
+    with pytest.raises(RuntimeError, match="masked center atoms"):
+        al.get_features_gradient(rep)
+
+
+def test_centre_mask_by_species():
+    st = dict(golden_structures()["CaCrP2O7"][0])
+    mask_center_atoms_by_species(st, species_select=[8])
+    assert np.array_equal(st["center_atoms_mask"], np.asarray(st["numbers"]) == 8)
+    mask_center_atoms_by_species(st, species_blacklist=[8])
+    assert not st["center_atoms_mask"].any()
+    rep = from_hypers(soap_hypers(rc=3.0, N=3, L=2))
+    st2 = dict(golden_structures()["CaCrP2O7"][0])
+    mask_center_atoms_by_species(st2, species_select=[20, 24])
+    X = rep.transform([st2]).get_dense_features(rep)
+    full = rep.transform([golden_structures()["CaCrP2O7"][0]]).get_dense_features(rep)
+    assert np.array_equal(X, full[np.isin(st2["numbers"], [20, 24])])
+
+
+# --------------------------------------------------------------------------
+# species keys (calculator_spherical_expansion.hh:2414-2570)
+
+def _method_hypers(method, global_species=()):
+    hp = soap_hypers(rc=3.0, N=3, L=2, sigma=0.4)
+    hp["expansion_by_species_method"] = method
+    hp["global_species"] = list(global_species)
+    return hp
+
+
+def _expected_keys(frames, species, method, global_species=()):
+    """Present (a <= b) keys of the dense export under each method, from the
+    oracle's neighbour lists."""
+    present = np.zeros(len(species), bool)
+    pairs = set()
+    idx = {z: k for k, z in enumerate(species)}
+    for f in frames:
+        nums = np.asarray(f["numbers"])
+        if method == "user defined":
+            sets = [set(idx[z] for z in global_species)] * len(nums)
+        else:
+            ro, rj, _, _ = O.neighbours(f["positions"], f["cell"], f["pbc"], 3.0)
+            env = [set([idx[nums[i]]] + [idx[nums[j]] for j in rj[ro[i] + 1:ro[i + 1]]])
+                   for i in range(len(nums))]
+            if method == "structure wise":
+                allsp = set().union(*env)
+                sets = [allsp] * len(nums)
+            else:
+                sets = env
+        for st in sets:
+            for a in st:
+                for b in st:
+                    if a <= b:
+                        pairs.add((a, b))
+    S = len(species)
+    return [k for k, (a, b) in enumerate((a, b) for a in range(S) for b in range(a, S))
+            if (a, b) in pairs]
+
+
+@pytest.mark.parametrize("method", ["environment wise", "structure wise", "user defined"])
+def test_species_key_methods(method):
+    g = golden_structures()
+    frames = g["small_molecule"] + g["SiC_moissanite_supercell"]
+    glob = (1, 6, 7, 8, 14) if method == "user defined" else ()
+    hp = _method_hypers(method, glob)
+    rep = from_hypers(hp)
+    al = rep.transform(frames)
+    species = al.species
+    if method == "user defined":
+        assert species == sorted(glob)
+    X = al.get_features(rep)
+    keys = _expected_keys(frames, species, method, glob)
+    inner = 3 * 3 * 3
+    assert X.shape[1] == len(keys) * inner
+    b = O.BatchData(frames, species=species)
+    calc = O.Calculator(_method_hypers("environment wise"))
+    Xr, _ = calc.power_spectrum(b)
+    Xr = Xr.reshape(len(Xr), -1, inner)[:, keys].reshape(len(Xr), -1)
+    assert _rel(Xr, X) < TOL
+
+
+def test_user_defined_species_must_cover_the_centres():
+    rep = from_hypers(_method_hypers("user defined", (1, 6)))
+    with pytest.raises(_capi.RascalGPUError, match="global_species is missing at least these species"):
+        rep.transform(golden_structures()["small_molecule"])
+
+
+# --------------------------------------------------------------------------
+# sparsified power spectrum (test_calculator.cc:622-748 with the reference's
+# sparsification_inputs.json: selected coefficients and their global ids)
+
+def _sparsification_cases():
+    d = json.load(open(os.path.join(GOLDEN, "sparsification_inputs.json")))
+    out = []
+    for k, c in enumerate(d["cases"]):
+        st = d["structures"][c["structure"]][0]
+        st = dict(positions=np.asarray(st["positions"]), numbers=np.asarray(st["numbers"]),
+                  cell=np.asarray(st["cell"]), pbc=np.asarray(st["pbc"]))
+        out.append(("%d-%s" % (k, c["structure"].split(".")[0]), st, c["rep"], c["rep_sparse"]))
+    return out
+
+
+@pytest.mark.parametrize("normalize", [False, True])
+@pytest.mark.parametrize("case", _sparsification_cases(), ids=lambda c: c[0])
+def test_sparsified_power_spectrum_reference_cases(case, normalize):
+    name, st, hp, hp_sparse = case
+    hp = dict(hp, normalize=normalize)
+    hp_sparse = json.loads(json.dumps(hp_sparse))
+    hp_sparse["normalize"] = normalize
+    ids = hp_sparse["coefficient_subselection"].pop("selected_features_global_ids")
+    rep = from_hypers(hp)
+    rep_s = from_hypers(hp_sparse)
+    al = rep.transform([st])
+    # "structure wise": every centre holds every key of the structure, so
+    # the reference's per-centre full vector is the dense row
+    X_full = al.get_dense_features(rep).copy()
+    G_full = al.get_features_gradient(rep).copy()
+    als = rep_s.transform([st])
+    Xs = als.get_features(rep_s)
+    assert Xs.shape == (len(st["numbers"]), len(ids))
+    if normalize:
+        # the selection is normalised on its own
+        raw = AtomsList([st])
+        rep_raw = from_hypers(dict(hp, normalize=False))
+        P = raw.compute(rep_raw).get_dense_features(rep_raw)[:, ids]
+        ref = P / np.linalg.norm(P, axis=1)[:, None]
+    else:
+        ref = X_full[:, ids]
+    assert _rel(ref, Xs) < TOL
+    Gs = als.get_features_gradient(rep_s)
+    assert Gs.shape == (G_full.shape[0], len(ids))
+    if not normalize:
+        assert _rel(G_full[:, ids], Gs) < TOL
+    # the adjoint of the sparsified features equals the stored gradient
+    # rows contracted with dL/dX
+    rng = np.random.default_rng(5)
+    dX = rng.normal(size=Xs.shape)
+    g_adj, _ = rep_s._context.features_backward(dX)
+    info = als.get_gradients_info(rep_s)
+    g_rows = np.zeros_like(g_adj)
+    Gr = Gs.reshape(-1, 3, len(ids))
+    for r, (_, i, j, _, _) in enumerate(info):
+        g_rows[j] += Gr[r] @ dX[i]
+    assert _rel(g_rows, g_adj) < TOL
+
+
+def test_sparsified_gap_forces_match_the_gradient_rows():
+    """A GAP model on sparsified features: its forces equal the stored
+    feature gradients contracted with dE/dX (dE/dX formed on the host from
+    the exported features and sparse points); its stress equals the
+    adjoint's."""
+    cases = _sparsification_cases()
+    name, st, hp, hp_sparse = cases[4]  # CaCrP2O7, mixed species selection
+    hp_sparse = json.loads(json.dumps(hp_sparse))
+    hp_sparse["coefficient_subselection"].pop("selected_features_global_ids")
+    hp_sparse["normalize"] = True
+    rep = from_hypers(hp_sparse)
+    al = rep.transform([st])
+    X = al.get_features(rep)
+    sp = SparsePoints(rep)
+    sp.extend(al, [[0, 3, 5, 9, 14, 20]])
+    zeta = 3
+    kern = Kernel(rep, name="GAP", zeta=zeta, target_type="Structure", kernel_type="Sparse")
+    w = np.random.default_rng(6).normal(size=sp.size())
+    model = KRR(w, kern, sp, {int(z): 0.0 for z in set(al.numbers.tolist())})
+    T = sp.get_features()
+    nums = al.numbers
+    K = X @ T.T
+    same = nums[:, None] == sp.numbers[None, :]
+    e_ref = np.sum(np.where(same, K ** zeta, 0.0) @ w)
+    assert _rel([e_ref], model.predict(al)) < TOL
+    dEdX = (np.where(same, zeta * K ** (zeta - 1), 0.0) * w[None, :]) @ T
+    G = al.get_features_gradient(rep).reshape(-1, 3, X.shape[1])
+    info = al.get_gradients_info(rep)
+    grad = np.zeros((al.n_atoms, 3))
+    for r, (_, i, j, _, _) in enumerate(info):
+        grad[j] += G[r] @ dEdX[i]
+    f = model.predict_forces(al)
+    assert _rel(grad, -f) < TOL
+    _, neg = rep._context.features_backward(dEdX)
+    assert _rel(neg, -model.predict_stress(al)) < TOL
+
+
+@pytest.mark.parametrize("normalize", [False, True])
+def test_features_gradient_vs_oracle(normalize):
+    """get_features_gradient rows (3 per pair, self pair first) against the
+    oracle's stored gradients, several structures, direct and splined
+    radial integrals."""
+    g = golden_structures()
+    frames = g["CaCrP2O7"] + g["small_molecule"]
+    for spline in (None, 1e-8):
+        hp = soap_hypers(rc=3.0, N=4, L=3, sigma=0.4, normalize=normalize, spline=spline)
+        rep = from_hypers(hp)
+        al = rep.transform(frames)
+        G = rep._context.features_gradient()
+        b = O.BatchData(frames, species=al.species)
+        grads, info = O.Calculator(hp).power_spectrum_gradients(b)
+        assert G.shape == (3 * len(grads), grads.shape[2])
+        assert _rel(grads.reshape(G.shape), G) < TOL
+        assert np.array_equal(al.get_gradients_info(rep)[:, 1:3], info[:, 1:3])
+
+
+def test_compute_KNM_stacks_energy_and_gradient_rows():
+    rep = from_hypers(soap_hypers(rc=3.5, N=4, L=3, sigma=0.5))
+    frames = golden_structures()["methane_dimers"][:4]
+    from conftest import wrap_cubic_periodic
+    frames = [wrap_cubic_periodic(f) for f in frames]
+    al = rep.transform(frames)
+    sp = SparsePoints(rep)
+    sp.extend(al, [[0, 2, 4]] + [[1, 5]] * (len(frames) - 1))
+    kern = Kernel(rep, name="GAP", zeta=2, target_type="Structure", kernel_type="Sparse")
+    knm = compute_KNM(frames, sp, kern, rep)
+    nat = sum(len(f["numbers"]) for f in frames)
+    assert knm.shape == (len(frames) + 3 * nat, sp.size())
+    b = O.BatchData(frames, species=al.species)
+    ss = O.SparseSet(sp.numbers, sp.values_for(al.species), 2)
+    ref = np.zeros_like(knm)
+    calc = O.Calculator(rep.hypers)
+    ref[:len(frames), ss.order] = O.gap_knm(calc, b, ss, "Structure")
+    ref[len(frames):, ss.order] = O.gap_knm_derivative(calc, b, ss)
+    assert _rel(ref, knm) < TOL
+
+
+# --------------------------------------------------------------------------
+# neighbour-list pins of the reference (test_adaptor_neighbour_list.cc)
+
+def _strict_counts(ctx):
+    off, _, vec = ctx.neighbours()
+    d = np.linalg.norm(vec, axis=1)
+    return off, d
+
+
+def _nl_ctx(st, rc):
+    ctx = _capi.Context(json.dumps(soap_hypers(rc=rc, N=2, L=1)))
+    off, pos, num, cells, pbc = pack([st])
+    ctx.set_structures(off, pos, num, cells, pbc, sorted(set(num.tolist())))
+    return ctx
+
+
+def test_neighbour_list_hcp_counts():
+    """Two unit cells of one hcp crystal (a = 1, c = sqrt(8/3)): 12 and 56
+    neighbours closer than 1.01 and 2.02 for every atom of both cells
+    (:233-330, fixture test_structure.hh:125-169)."""
+    a, c = 1.0, np.sqrt(8.0 / 3.0)
+    h, r = np.sqrt(3.0) / 2.0 * a, np.sqrt(3.0) / 6.0 * a
+    cell1 = np.array([[a, -0.5 * a, 0.0], [0.0, h, 0.0], [0.0, 0.0, c]])
+    cell2 = np.array([[a, 0.0, 0.5 * a], [0.0, c, 0.0], [0.0, 0.0, h]])
+    pos1 = np.array([[0.0, 0.0, 0.0], [0.0, h - r, c / 2.0]])
+    pos2 = np.array([[0.0, 0.0, 0.0], [a / 2.0, c / 2.0, r]])
+    for k, rc in enumerate((1.01, 2.02)):
+        for cell, pos in ((cell1, pos1), (cell2, pos2)):
+            st = dict(positions=pos, numbers=np.array([1, 1]), cell=cell, pbc=np.ones(3, int))
+            off, d = _strict_counts(_nl_ctx(st, rc))
+            for i in range(2):
+                dd = d[off[i] + 1:off[i + 1]]
+                assert np.count_nonzero(dd < rc) == (12, 56)[k]
+
+
+def test_neighbour_list_fcc_counts():
+    """Primitive (1 atom) and conventional (4 atoms) fcc cells give the atom
+    at the origin the same neighbours closer than 0.7 and 1.4 (:341-420)."""
+    a = 1.0
+    cell1 = np.array([[a, 0.5 * a, 0.5 * a], [0.0, 0.5 * a, 0.0], [0.0, 0.0, 0.5 * a]])
+    cell2 = np.eye(3) * a
+    pos2 = np.array([[0, 0, 0], [0.5, 0.5, 0], [0.5, 0, 0.5], [0, 0.5, 0.5]]) * a
+    for rc in (0.7, 1.4):
+        s1 = dict(positions=np.zeros((1, 3)), numbers=np.array([1]), cell=cell1, pbc=np.ones(3, int))
+        s2 = dict(positions=pos2, numbers=np.ones(4, int), cell=cell2, pbc=np.ones(3, int))
+        o1, d1 = _strict_counts(_nl_ctx(s1, rc))
+        o2, d2 = _strict_counts(_nl_ctx(s2, rc))
+        n1 = np.count_nonzero(d1[o1[0] + 1:o1[1]] < rc)
+        n2 = np.count_nonzero(d2[o2[0] + 1:o2[1]] < rc)
+        assert n1 == n2 == (0, 42)[int(rc > 1)]
+
+
+def test_neighbour_list_epsilon_at_the_cutoff():
+    """Triclinic cell (30 degree angles) with 12 atoms within epsilon of a
+    cell corner or rc - 2 epsilon from it: every atom has the 11 others as
+    neighbours and no periodic image (:555-590, fixture test_structure.hh
+    :482-512)."""
+    cell = np.array([[10.0, 0.0, 0.0], [9.396926207859085, 3.420201433256687, 0.0],
+                     [9.396926207859085, 1.6569316261720377, 2.992048701181514]]).T
+    pos = np.array([[0.001, 0.0, 0.0], [9.999, 0.0, 0.0], [10.0, 0.001, 0.0],
+                    [9.39692621, 3.41920143, 0.0], [19.39692621, 3.42020143, 0.001],
+                    [9.39692621, 1.65693163, 2.99104870], [0.498, 0.0, 0.0], [9.502, 0.0, 0.0],
+                    [10.0, 0.498, 0.0], [9.39692621, 2.92220143, 0.0],
+                    [19.39692621, 3.42020143, 0.498], [9.39692621, 1.65693163, 2.49404870]])
+    st = dict(positions=pos, numbers=np.ones(12, int), cell=cell, pbc=np.ones(3, int))
+    ctx = _nl_ctx(st, 0.997)
+    off, pj, vec = ctx.neighbours()
+    ro, rj, rv, _ = O.neighbours(pos, cell, st["pbc"], 0.997)
+    assert np.array_equal(off, ro) and np.array_equal(pj, rj) and np.array_equal(vec, rv)
+    for i in range(12):
+        nb = pj[off[i] + 1:off[i + 1]]
+        assert sorted(nb.tolist()) == [j for j in range(12) if j != i]
+
+
+# --------------------------------------------------------------------------
+# structure errors (atomic_structure.hh:180-262, structure_manager_centers.cc
+# :37-100), raised through the C ABI with the reference's messages
+
+def test_structure_errors():
+    ctx = _capi.Context(json.dumps(soap_hypers()))
+    cell = np.eye(3) * 5.0
+    pos = np.array([[1.0, 1.0, 1.0], [2.0, 2.0, 2.0]])
+
+    def setup(p=pos, c=cell, types=(1, 1), species=(1,)):
+        ctx.set_structures([0, 2], p, list(types), c, [1, 1, 1], list(species))
+
+    setup()
+    with pytest.raises(_capi.RascalGPUError, match="not inside the unit cell"):
+        setup(p=pos + [[0, 0, 0], [4.0, 0, 0]])
+    with pytest.raises(_capi.RascalGPUError, match="not inside the unit cell"):
+        setup(p=pos - [[0, 0, 0], [2.0 + 1e-6, 0, 0]])
+    with pytest.raises(_capi.RascalGPUError, match="should not be filled with zeros"):
+        setup(c=np.zeros((3, 3)))
+    with pytest.raises(_capi.RascalGPUError, match=r"should be in the range \[0, 130\]"):
+        setup(types=(1, 130), species=(1, 130))
+    with pytest.raises(_capi.RascalGPUError, match=r"should be in the range \[0, 130\]"):
+        setup(types=(-1, 1), species=(-1, 1))
+    # within the 1e-10 tolerance of the cell faces is accepted
+    setup(p=np.array([[0.0, 0.0, 0.0], [5.0 + 5e-11, 5.0, 5.0]]))
+    with pytest.raises(_capi.RascalGPUError, match="no structures set"):
+        _capi.Context(json.dumps(soap_hypers())).compute()
