@@ -65,6 +65,7 @@ extern "C" {
 #define RGPU_INFO_TILE_KEYS_ADJ 16  /* key blocks per pass of the FP32 adjoint (0 = auto) */
 #define RGPU_INFO_NL_GRAPH_LAUNCHES 17 /* neighbour-list passes replayed from a graph */
 #define RGPU_INFO_N_ATOMS 18       /* atoms of the batch (N_CENTRES counts the centres) */
+#define RGPU_INFO_TILE_ADJ_KIND 19 /* mixed-precision adjoint: 0 vector ALUs, 1 matrix cores */
 
 /* stages timed by rgpu_enable_timing */
 #define RGPU_STAGE_NEIGHBOURS 0
@@ -104,10 +105,12 @@ int rgpu_enable_graphs(rgpu_context* ctx, int enable);
  * (RGPU_INFO_TILE_*).  No counterpart in the reference. */
 int rgpu_autotune(rgpu_context* ctx);
 /* Sets the work partitions of the current precision directly (values <= 0:
- * built-in choice; keys_adj: key blocks of dL/dp staged in LDS per pass of
- * the mixed-precision adjoint), e.g. to reuse an autotuned set.  Results
- * agree between partitions to rounding.  No counterpart in the reference. */
-int rgpu_set_tiles(rgpu_context* ctx, int pairs_fwd, int keys_ps, int pairs_adj, int keys_adj);
+ * built-in choice; keys_adj: key blocks of dL/dX staged in LDS per pass of
+ * the mixed-precision adjoint; adj_kind: its vector-ALU (0) or matrix-core
+ * (1) kernel), e.g. to reuse an autotuned set.  Results agree between
+ * partitions to rounding.  No counterpart in the reference. */
+int rgpu_set_tiles(rgpu_context* ctx, int pairs_fwd, int keys_ps, int pairs_adj, int keys_adj,
+                   int adj_kind);
 /* RGPU_FP64 (default) or RGPU_MIXED; invalidates computed results. */
 int rgpu_set_precision(rgpu_context* ctx, int precision);
 int rgpu_get_info(rgpu_context* ctx, int key, int64_t* value);

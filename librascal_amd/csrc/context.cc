@@ -1041,7 +1041,17 @@ void autotune_t(Context& c) {
         const float ms = time_adjoint(cand);
         if (ms < best) best = ms, bestv = cand;
       }
+      // the matrix-core kernel (adj_kind 1) with its own key chunks
+      for (int div = 1; div <= 3; ++div) {
+        SoapTiles cand = t;
+        cand.adj_kind = 1;
+        cand.pb_adj = 0;
+        cand.keys_adj = div == 1 ? 0 : (K + div - 1) / div;
+        const float ms = time_adjoint(cand);
+        if (ms < best) best = ms, bestv = cand;
+      }
       t.keys_adj = bestv.keys_adj;
+      t.adj_kind = bestv.adj_kind;
     }
     best = 1e30f;
     pick = 0;
@@ -1109,7 +1119,8 @@ int rgpu_autotune(rgpu_context* ctx) {
   });
 }
 
-int rgpu_set_tiles(rgpu_context* ctx, int pairs_fwd, int keys_ps, int pairs_adj, int keys_adj) {
+int rgpu_set_tiles(rgpu_context* ctx, int pairs_fwd, int keys_ps, int pairs_adj, int keys_adj,
+                   int adj_kind) {
   return guarded([&] {
     if (!ctx) throw InvalidArgument("null context");
     if (pairs_fwd > 64 || pairs_adj > 64) throw InvalidArgument("at most 64 pairs per batch");
@@ -1119,6 +1130,8 @@ int rgpu_set_tiles(rgpu_context* ctx, int pairs_fwd, int keys_ps, int pairs_adj,
     t.kc = std::max(keys_ps, 0);
     t.pb_adj = std::max(pairs_adj, 0);
     t.keys_adj = std::max(keys_adj, 0);
+    if (adj_kind != 0 && adj_kind != 1) throw InvalidArgument("adj_kind must be 0 or 1");
+    t.adj_kind = adj_kind;
     c.have_rep = false;
     c.drop_graph();
   });
@@ -1183,6 +1196,7 @@ int rgpu_get_info(rgpu_context* ctx, int key, int64_t* value) {
       case RGPU_INFO_TILE_KEYS_PS: *value = c.tile().kc; break;
       case RGPU_INFO_TILE_PAIRS_ADJ: *value = c.tile().pb_adj; break;
       case RGPU_INFO_TILE_KEYS_ADJ: *value = c.tile().keys_adj; break;
+      case RGPU_INFO_TILE_ADJ_KIND: *value = c.tile().adj_kind; break;
       default: throw InvalidArgument("unknown info key");
     }
   });

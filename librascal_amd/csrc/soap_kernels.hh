@@ -34,6 +34,7 @@ struct SoapTiles {
   int kc = 0;      // key blocks staged per pass in the power spectrum
   int pb_adj = 0;  // pairs staged per batch in the adjoint (<= 64)
   int keys_adj = 0;  // key blocks staged per pass in the FP32 adjoint
+  int adj_kind = 0;  // FP32 adjoint: 0 vector ALUs, 1 matrix cores where they apply
 };
 
 // c[i][s][n][l*l+l+m] (finalised) and the per-centre key mask.

@@ -1261,6 +1261,332 @@ __global__ __launch_bounds__(kThreads) void adjoint_staged_kernel(
   adjoint_pairs<T, DIRECT>(rad, S, PB, pv, yc, cd, region, i, grad, centre_virial);
 }
 
+// ---------------------------------------------------------------------------
+// Adjoint of the mixed path on the matrix cores (v_mfma_f32_16x16x4_f32).
+//
+// Per angular channel l, with C_l[(a,n)][m] = c_a[n][l^2 + m] (S N rows,
+// 2l+1 columns) and f_l = (2l+1)^{-1/2}, the transpose of the power
+// spectrum p^{ab}[n1 n2 l] = f_l w_ab sum_m c_a[n1 lm] c_b[n2 lm]
+// (calculator_spherical_invariants.hh:747-788) applied to
+// dL/dp = alpha G - beta p is
+//   dL/dC_l = alpha f_l W_l C_l - 2 beta f_l^2 C_l (C_l^T C_l),
+// where W_l is the symmetric S N x S N matrix with blocks
+// W^{ab} = sqrt2 G^{ab} (a < b), its transpose (a > b) and G^{aa} + G^{aa T}
+// (the last term because |p|^2 = sum_l f_l^2 |C_l C_l^T|_F^2 =
+// sum_l f_l^2 |C_l^T C_l|_F^2, whose gradient is 4 f_l^2 C_l C_l^T C_l).
+// So the invariants themselves are never read: only G (packed layout, as
+// written by the force GEMM) and c.  X . G, when not given, follows from the
+// same products: p . G = 1/2 sum_l f_l <W_l C_l, C_l>.
+//
+// Tiles: one 16-row tile per species (N <= 16 rows used) and 16-column
+// tiles over m for each l; K runs over the 4-row steps of one species
+// block.  A key block (a, b) is staged in LDS as W^{ab} [l][n][y] (rows of
+// NP = 2 mod 4 floats, so that the 16 rows x 2 K columns of a half
+// wavefront's A fragment fall in distinct banks), the sqrt 2 of an
+// off-diagonal block folded in and a diagonal block unpacked symmetrised
+// (sqrt2 G' off the diagonal, 2 G' on it, G' packed).  It feeds the tiles
+// of rows a with K over block b, and (a < b) the tiles of rows b with K
+// over block a (read transposed).
+// Wavefront w owns the output tiles t = w, w + 4, ... (MAXT at most), its
+// accumulators in registers across all key chunks.  The result is written
+// over c in LDS as dL/dc [species][lm][n]; the pair phase follows.
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ f32x4 mfma4(float a, float b, f32x4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+
+template <bool DIRECT, int MAXT>
+__global__ __launch_bounds__(kThreads, 2) void adjoint_mfma_kernel(
+    RadialDev rad, int S, int PB, int KCH, PairView pv, const float* __restrict__ c,
+    const double* __restrict__ norm2, int normalize, const double* __restrict__ xg, double xg_scale,
+    const float* __restrict__ G, double* __restrict__ grad, double* __restrict__ centre_virial) {
+  const int64_t i = blockIdx.x;
+  const int N = rad.N, L = rad.L, L1 = rad.L1, LL = rad.LL, B = N * LL, E = S * B;
+  // row stride of c in LDS, 16 mod 32: the four 16-lane groups of a B
+  // fragment (rows y..y+3, 16 consecutive m) cover the banks twice
+  const int LS = (LL + 15) / 32 * 32 + 16;
+  const int NP = 4 * ((N + 1) / 4) + 2;  // row stride of a staged W block (>= N, 2 mod 4)
+  const int WB = L1 * N * NP;           // one staged key block
+  const int K = S * (S + 1) / 2;
+  const int64_t Dq = packed_row_stride(S, N, L1);
+  const int tid = threadIdx.x, lane = tid & 63;
+  // wavefront-uniform (scalar registers for the tile bookkeeping)
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int i16 = lane & 15, k4 = lane >> 4;
+  const int NKS = (N + 3) >> 2;  // K steps over one species block
+  extern __shared__ __align__(64) unsigned char smem_raw[];
+  float* yc = reinterpret_cast<float*>(smem_raw);
+  float* cs = reinterpret_cast<float*>(smem_raw + ylm_table_bytes<float>(L1));  // [S N][LS]
+  unsigned char* region = reinterpret_cast<unsigned char*>(cs) + align_up(sizeof(float) * S * N * LS, 64);
+  float* gl = reinterpret_cast<float*>(region);
+  __shared__ double red[kWaves];
+  __shared__ double sh_ab[2];
+  load_ylm_table<float>(rad, yc);
+  const float* ci = c + i * int64_t(E);
+  for (int e = tid; e < E; e += kThreads) {
+    const int row = e / LL;
+    cs[row * LS + (e - row * LL)] = ci[e];
+  }
+
+  // output tiles of this wavefront: (species a, l, column tile ct)
+  int T_col = 0;
+  for (int l = 0; l <= L; ++l) T_col += (2 * l + 16) >> 4;
+  const int n_tiles = S * T_col;
+  int t_a[MAXT], t_l[MAXT], t_ct[MAXT];
+#pragma unroll
+  for (int tt = 0; tt < MAXT; ++tt) {
+    const int t = wave + kWaves * tt;
+    t_a[tt] = -1;
+    t_l[tt] = 0;
+    t_ct[tt] = 0;
+    if (t < n_tiles) {
+      t_a[tt] = t / T_col;
+      int u = t - t_a[tt] * T_col, l = 0;
+      while (u >= ((2 * l + 16) >> 4)) {
+        u -= (2 * l + 16) >> 4;
+        ++l;
+      }
+      t_l[tt] = l;
+      t_ct[tt] = u;
+    }
+  }
+  // per-lane A offsets (l-independent) of the K steps: rows n = i16, K
+  // index y = 4 ks + k4 of a block, in the orientations ab / ba
+  int off_ab[4], off_ba[4], boff[4];
+  bool ok[4];  // K index inside the block (only tested when N is not a multiple of 4)
+  const bool npad = (N & 3) != 0;
+#pragma unroll
+  for (int ks = 0; ks < 4; ++ks) {
+    const int n = i16, y = 4 * ks + k4;
+    ok[ks] = y < N;
+    off_ab[ks] = n * NP + y;
+    off_ba[ks] = y * NP + n;
+    boff[ks] = y * LS;
+  }
+  f32x4 acc[MAXT];
+#pragma unroll
+  for (int tt = 0; tt < MAXT; ++tt) acc[tt] = f32x4{0.f, 0.f, 0.f, 0.f};
+  const int KB = N * N * L1, KD = packed_diag_block(N, L1);
+  const FastDiv fL1 = fast_div(L1), fN = fast_div(N);
+  __shared__ int tri[kMaxN * (kMaxN + 1) / 2];  // n1 << 8 | n2 of the packed diagonal pairs
+  for (int t = tid; t < N * (N + 1) / 2; t += kThreads) {
+    int n1, n2;
+    packed_pair(t, N, &n1, &n2);
+    tri[t] = (n1 << 8) | n2;
+  }
+  __syncthreads();  // tri and c in place
+
+  const float* Gi = G + i * Dq;
+  const float rt2 = 1.41421356237309504880f;
+  for (int k0 = 0; k0 < K; k0 += KCH) {
+    const int k1 = min(K, k0 + KCH);
+    if (k0 > 0) __syncthreads();  // every wavefront is done with the previous chunk
+    // stage W of the chunk's keys: the chunk's packed range read with
+    // independent 16-byte loads, each value scattered to its (l, n1, n2)
+    // slot (both slots for a diagonal block)
+    {
+      int a0, b0, a1, b1;
+      key_pair(k0, S, &a0, &b0);
+      const int g0 = int(packed_key_offset(a0, b0, S, N, L1));
+      int g1 = int(packed_feature_size(S, N, L1));
+      if (k1 < K) {
+        key_pair(k1, S, &a1, &b1);
+        g1 = int(packed_key_offset(a1, b1, S, N, L1));
+      }
+      const int cnt = g1 - g0;
+      auto put = [&](int o, float v) {
+        // key of packed column g0 + o (the chunk has at most K keys)
+        int key = k0, a = a0, b = b0;
+        int start = 0, len = a == b ? KD : KB;
+        while (o >= start + len) {
+          start += len;
+          ++key;
+          key_pair(key, S, &a, &b);
+          len = a == b ? KD : KB;
+        }
+        const int rel = o - start, t = rel / fL1, l = rel - t * L1;
+        float* w = gl + (key - k0) * WB + l * N * NP;
+        if (a < b) {
+          const int n1 = t / fN, n2 = t - n1 * N;
+          w[n1 * NP + n2] = rt2 * v;
+        } else {
+          const int n1 = tri[t] >> 8, n2 = tri[t] & 255;
+          const float x = (n1 < n2 ? rt2 : 2.0f) * v;
+          w[n1 * NP + n2] = x;
+          w[n2 * NP + n1] = x;
+        }
+      };
+      if (((g0 | cnt) & 3) == 0) {
+        const f32x4* src = reinterpret_cast<const f32x4*>(Gi + g0);
+#pragma unroll 4
+        for (int q = tid; q < (cnt >> 2); q += kThreads) {
+          const f32x4 v = src[q];
+          put(4 * q, v[0]);
+          put(4 * q + 1, v[1]);
+          put(4 * q + 2, v[2]);
+          put(4 * q + 3, v[3]);
+        }
+      } else {
+#pragma unroll 4
+        for (int q = tid; q < cnt; q += kThreads) put(q, Gi[g0 + q]);
+      }
+    }
+    __syncthreads();
+    // each tile of rows a takes K over the blocks b whose key is staged:
+    // W^{ab} = block (a, b) as stored for a <= b, block (b, a) read
+    // transposed for a > b.  Rows n >= N and columns m >= 2l+1 of a tile
+    // read neighbouring LDS values: they only reach outputs that are never
+    // used.  K indices y >= N (N not a multiple of 4) are zeroed.
+#pragma unroll
+    for (int tt = 0; tt < MAXT; ++tt) {
+      const int a = t_a[tt];
+      if (a < 0) continue;
+      const int l = t_l[tt];
+      const float* cl = cs + l * l + 16 * t_ct[tt] + i16;
+      for (int b = 0; b < S; ++b) {
+        const int lo = min(a, b), hi = max(a, b);
+        const int key = lo * S - lo * (lo - 1) / 2 + (hi - lo);
+        if (key < k0 || key >= k1) continue;
+        const float* gk = gl + (key - k0) * WB + l * N * NP;
+        const float* cb = cl + b * N * LS;
+        if (npad) {
+#pragma unroll
+          for (int ks = 0; ks < 4; ++ks) {
+            if (ks >= NKS) break;
+            const float av = ok[ks] ? gk[a <= b ? off_ab[ks] : off_ba[ks]] : 0.f;
+            const float bv = ok[ks] ? cb[boff[ks]] : 0.f;
+            acc[tt] = mfma4(av, bv, acc[tt]);
+          }
+        } else {
+#pragma unroll
+          for (int ks = 0; ks < 4; ++ks) {
+            if (ks >= NKS) break;
+            acc[tt] = mfma4(gk[a <= b ? off_ab[ks] : off_ba[ks]], cb[boff[ks]], acc[tt]);
+          }
+        }
+      }
+    }
+  }
+
+  // normalisation: alpha = 1/|p|, beta = (X . G)/|p|^2
+  double alpha = 1.0, beta = 0.0;
+  if (normalize) {
+    const double n2 = norm2[i];
+    double dotxg;
+    if (xg) {
+      dotxg = xg_scale * xg[i];
+    } else {
+      // p . G = 1/2 sum_l f_l <W_l C_l, C_l> from the accumulators
+      double d = 0;
+#pragma unroll
+      for (int tt = 0; tt < MAXT; ++tt) {
+        if (t_a[tt] < 0) continue;
+        const int l = t_l[tt], m = 16 * t_ct[tt] + i16;
+        if (m >= 2 * l + 1) continue;
+        const float fl = 1.0f / sqrtf(float(2 * l + 1));
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int n = 4 * k4 + v;
+          if (n < N) d += double(fl) * acc[tt][v] * cs[(t_a[tt] * N + n) * LS + l * l + m];
+        }
+      }
+      d *= 0.5;
+      for (int off = 32; off > 0; off >>= 1) d += __shfl_down(d, off, 64);
+      if (lane == 0) red[wave] = d;
+      __syncthreads();
+      if (tid == 0) {
+        double t = 0;
+        for (int w = 0; w < kWaves; ++w) t += red[w];
+        sh_ab[0] = t;
+      }
+      __syncthreads();
+      dotxg = n2 > 0 ? sh_ab[0] / sqrt(n2) : 0.0;
+    }
+    if (n2 > 0) {
+      alpha = 1.0 / sqrt(n2);
+      beta = dotxg / n2;
+    }
+  }
+#pragma unroll
+  for (int tt = 0; tt < MAXT; ++tt) {
+    const float s = float(alpha) / sqrtf(float(2 * t_l[tt] + 1));
+    acc[tt] *= s;
+  }
+  if (beta != 0.0) {
+    // M_l = -2 beta f_l^2 C_l^T C_l, [16 nct][16 nct] per l, in the G region
+    __shared__ int moff[kMaxL1];  // first entry of M_l
+    if (tid == 0) {
+      int o = 0;
+      for (int l = 0; l <= L; ++l) {
+        moff[l] = o;
+        const int w = 16 * ((2 * l + 16) >> 4);
+        o += w * w;
+      }
+    }
+    __syncthreads();  // the G chunk is no longer read
+    float* Ml = gl;
+    const int rows = S * N, RKS = (rows + 3) >> 2;
+    // tiles (l, ct, ct2) dealt to the wavefronts in turn
+    int tix = 0;
+    for (int l = 0; l <= L; ++l) {
+      const int nct = (2 * l + 16) >> 4, w = 16 * nct;
+      const float sc = float(-2.0 * beta) / float(2 * l + 1);
+      for (int ct = 0; ct < nct; ++ct)
+        for (int ct2 = 0; ct2 < nct; ++ct2, ++tix) {
+          if ((tix & (kWaves - 1)) != wave) continue;
+          const int m1 = 16 * ct + i16, m2 = 16 * ct2 + i16;
+          const bool ok1 = m1 < 2 * l + 1, ok2 = m2 < 2 * l + 1;
+          f32x4 mt = f32x4{0.f, 0.f, 0.f, 0.f};
+          for (int ks = 0; ks < RKS; ++ks) {
+            const int r = 4 * ks + k4;
+            float av = 0.f, bv = 0.f;
+            if (r < rows) {
+              if (ok1) av = cs[r * LS + l * l + m1];
+              if (ok2) bv = cs[r * LS + l * l + m2];
+            }
+            mt = mfma4(av, bv, mt);
+          }
+#pragma unroll
+          for (int v = 0; v < 4; ++v)
+            Ml[moff[l] + (16 * ct + 4 * k4 + v) * w + 16 * ct2 + i16] = sc * mt[v];
+        }
+    }
+    __syncthreads();
+    // dL/dC_l += C_l M_l
+#pragma unroll
+    for (int tt = 0; tt < MAXT; ++tt) {
+      if (t_a[tt] < 0) continue;
+      const int l = t_l[tt], w = 16 * ((2 * l + 16) >> 4);
+      const int m = 16 * t_ct[tt] + i16;
+      const int nks = (2 * l + 4) >> 2;  // ceil((2l+1)/4)
+      const float* ca = cs + (t_a[tt] * N + i16) * LS + l * l;
+      const float* mb = Ml + moff[l] + m;
+      for (int ks = 0; ks < nks; ++ks) {
+        const int mp = 4 * ks + k4;
+        const float av = (i16 < N && mp < 2 * l + 1) ? ca[mp] : 0.f;
+        const float bv = mp < 2 * l + 1 ? mb[mp * w] : 0.f;
+        acc[tt] = mfma4(av, bv, acc[tt]);
+      }
+    }
+  }
+  __syncthreads();  // every read of c is done: dL/dc goes over it
+  float* dc = cs;   // [S][LL][N]
+#pragma unroll
+  for (int tt = 0; tt < MAXT; ++tt) {
+    if (t_a[tt] < 0) continue;
+    const int l = t_l[tt], m = 16 * t_ct[tt] + i16;
+    if (m >= 2 * l + 1) continue;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int n = 4 * k4 + v;
+      if (n < N) dc[t_a[tt] * B + (l * l + m) * N + n] = acc[tt][v];
+    }
+  }
+  __syncthreads();
+  adjoint_pairs<float, DIRECT>(rad, S, PB, pv, yc, dc, region, i, grad, centre_virial);
+}
+
 // dL/dX of one centre per workgroup into the packed layout (packing.hh)
 __global__ void pack_gradient_kernel(int S, int N, int L1, const double* __restrict__ G,
                                      float* __restrict__ Gp) {
@@ -1658,10 +1984,56 @@ void launch_adjoint_t(const RadialDev& rad, int S, const PairView& pv, const T* 
   const size_t yt = ylm_table_bytes<T>(L1);
   const size_t blk = align_up(sizeof(T) * size_t(E), 64);
   if constexpr (sizeof(T) == 4) {
+    const int K = S * (S + 1) / 2;
+    // matrix-core adjoint: packed dL/dX staged KCH key blocks at a time,
+    // the invariants not read
+    int T_col = 0;
+    for (int l = 0; l <= rad.L; ++l) T_col += (2 * l + 16) >> 4;
+    const int maxt = (S * T_col + kWaves - 1) / kWaves;
+    if (tiles.adj_kind == 1 && N <= 16 && maxt <= 16 && Grow == nullptr) {
+      const size_t cbytes = align_up(sizeof(float) * size_t(S) * N * ((LL + 15) / 32 * 32 + 16), 64);
+      size_t mbytes = 0;
+      for (int l = 0; l <= rad.L; ++l) {
+        const size_t w = 16 * ((2 * l + 16) >> 4);
+        mbytes += sizeof(float) * w * w;
+      }
+      // packed G of a chunk of keys (the largest chunk: diagonal keys are
+      // the smaller blocks)
+      const size_t kbytes = sizeof(float) * size_t(L1) * N * (4 * ((N + 1) / 4) + 2);
+      int KCH = tiles.keys_adj > 0 ? std::min(tiles.keys_adj, K) : K;
+      // (A fragments of rows N..15 read past the last staged block)
+      const size_t slack = sizeof(float) * 16 * (4 * ((N + 1) / 4) + 2);
+      auto region_of = [&](int kch, int pb) {
+        return std::max({align_up(kch * kbytes + slack, 64), align_up(mbytes, 64),
+                         pair_batch_bytes(pb, LL, F, sizeof(T), DIRECT, true)});
+      };
+      int PB = tiles.pb_adj > 0 ? std::min(tiles.pb_adj, 64) : 16;
+      if (tiles.keys_adj <= 0)
+        while (KCH > 1 && yt + cbytes + region_of(KCH, 1) > kResidentBudget) --KCH;
+      if (tiles.pb_adj <= 0)
+        while (PB > 1 && pair_batch_bytes(PB, LL, F, sizeof(T), DIRECT, true) >
+                             std::max(align_up(KCH * kbytes, 64), align_up(mbytes, 64)))
+          --PB;
+      const size_t smem = yt + cbytes + region_of(KCH, PB);
+      auto go = [&](auto kern) {
+        allow_lds(kern, smem);
+        hipLaunchKernelGGL(kern, dim3(unsigned(pv.n_centres)), dim3(kThreads), smem, stream, rad, S, PB,
+                           KCH, pv, c, norm2, int(normalize), xg, xg_scale, G, grad, centre_virial);
+      };
+      if (maxt <= 4)
+        go(adjoint_mfma_kernel<DIRECT, 4>);
+      else if (maxt <= 8)
+        go(adjoint_mfma_kernel<DIRECT, 8>);
+      else if (maxt <= 12)
+        go(adjoint_mfma_kernel<DIRECT, 12>);
+      else
+        go(adjoint_mfma_kernel<DIRECT, 16>);
+      RGPU_LAUNCH_CHECK();
+      return;
+    }
     // mixed path: packed dL/dX and BF16 planes of p, staged in LDS KCH key
     // blocks at a time beside c
     if (!p_hi || !p_lo) throw InvalidArgument("the FP32 adjoint takes the split invariants");
-    const int K = S * (S + 1) / 2;
     const size_t kbytes = sizeof(T) * size_t(N) * N * L1;
     const int upt = (S * ((N + 3) / 4) * LL + kThreads - 1) / kThreads;
     if (upt > 8) throw InvalidArgument("max_radial x (max_angular+1)^2 too large for the FP32 adjoint");

@@ -86,7 +86,7 @@ def lib():
         L.orc_batch_pairs.argtypes = [p, C.POINTER(Batch)]
         L.orc_expansion.argtypes = [p, C.POINTER(Batch), p, p]
         L.orc_expansion_gradients.argtypes = [p, C.POINTER(Batch), p]
-        L.orc_power_spectrum.argtypes = [p, C.POINTER(Batch), p, p]
+        L.orc_power_spectrum.argtypes = [p, C.POINTER(Batch), p, p, i64]
         L.orc_power_spectrum_gradients.argtypes = [p, C.POINTER(Batch), p, p]
         L.orc_gap_knm.argtypes = [p, C.POINTER(Batch), C.POINTER(Sparse), i, p]
         L.orc_gap_kmm.argtypes = [C.POINTER(Sparse), i, p]
@@ -298,11 +298,15 @@ class Calculator:
         _check(lib().orc_expansion_gradients(self._p, batch.c, _ptr(g)))
         return g
 
-    def power_spectrum(self, batch):
+    def power_spectrum(self, batch, max_centres=-1):
+        """(features [n][D], inverse norms [n]) of the first max_centres
+        centres (all when negative)."""
         D = self.D(batch.n_species)
-        p = np.zeros((batch.n_atoms, D))
-        inv = np.zeros(batch.n_atoms)
-        _check(lib().orc_power_spectrum(self._p, batch.c, _ptr(p), _ptr(inv)))
+        n = batch.n_atoms if max_centres < 0 else min(max_centres, batch.n_atoms)
+        p = np.zeros((n, D))
+        inv = np.zeros(n)
+        _check(lib().orc_power_spectrum(self._p, batch.c, _ptr(p), _ptr(inv),
+                                        C.c_int64(int(max_centres))))
         return p, inv
 
     def power_spectrum_gradients(self, batch):

@@ -1443,11 +1443,11 @@ int orc_expansion_gradients(void* calc, const orc_batch* b, double* grads) {
 }
 
 int orc_power_spectrum(void* calc, const orc_batch* b, double* ps,
-                       double* inv_norms) {
+                       double* inv_norms, int64_t max_centres) {
   return guarded([&] {
     Calc& C = *static_cast<Calc*>(calc);
     const size_t D = size_t(n_keys(b->n_species)) * C.N * C.N * C.L1;
-    sweep(C, b, true, false, -1,
+    sweep(C, b, true, false, max_centres,
           [&](int, int64_t ig, int, const StructView&, CentreState& st) {
             if (ps) std::copy(st.ps.begin(), st.ps.end(), ps + ig * D);
             if (inv_norms)

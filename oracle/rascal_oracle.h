@@ -116,9 +116,10 @@ int orc_expansion(void* calc, const orc_batch* b, double* coeffs,
 /* expansion gradients per pair slot [n_pairs][3][S][N][(L+1)^2] */
 int orc_expansion_gradients(void* calc, const orc_batch* b, double* grads);
 /* power spectrum [n_atoms][K][N*N][L+1] (normalised if hypers.normalize),
- * inv_norms[n_atoms] (1/|p| or 1 when not normalising) */
+ * inv_norms[n_atoms] (1/|p| or 1 when not normalising); only the first
+ * max_centres centres of the batch when max_centres >= 0 */
 int orc_power_spectrum(void* calc, const orc_batch* b, double* ps,
-                       double* inv_norms);
+                       double* inv_norms, int64_t max_centres);
 /* power-spectrum gradients per pair slot [n_pairs][3][D] and the gradient
  * info rows [n_pairs][5] = (struct, i, j, type_i, type_j) with j the original
  * atom index within the structure */

@@ -209,7 +209,8 @@ def test_mixed_adjoint_partitions_agree(N, L):
     molecules with 4 species and a random dL/dX that is not symmetric in the
     diagonal-key blocks.  N = 5, L = 4 gives key blocks of 125 and 75
     elements (element-wise staging) and a partial group of radial channels;
-    N = 6 another partial group."""
+    N = 6 another partial group.  adj_kind 1 is the matrix-core kernel
+    (no invariants read; dL/dc = alpha f_l W_l C_l - 2 beta f_l^2 C_l C_l^T C_l)."""
     sts = molecules(n_mol=24, seed=5)
     hp = soap_hypers(rc=4.0, sw=0.5, sigma=0.4, N=N, L=L, spline=1e-8)
     ctx, sp = make_ctx(hp, sts)
@@ -219,16 +220,18 @@ def test_mixed_adjoint_partitions_agree(N, L):
     g64, s64 = ctx.features_backward(G)
     ctx.set_precision(_capi.MIXED)
     out = {}
-    for kch, pb in ((0, 0), (1, 4), (3, 16), (10, 6)):
-        ctx.set_tiles(pairs_adj=pb, keys_adj=kch)
-        assert ctx.tiles()["keys_adj"] == kch and ctx.tiles()["pairs_adj"] == pb
-        ctx.features()
-        out[kch, pb] = ctx.features_backward(G)
+    for kind in (0, 1):
+        for kch, pb in ((0, 0), (1, 4), (3, 16), (10, 6)):
+            ctx.set_tiles(pairs_adj=pb, keys_adj=kch, adj_kind=kind)
+            t = ctx.tiles()
+            assert t["keys_adj"] == kch and t["pairs_adj"] == pb and t["adj_kind"] == kind
+            ctx.features()
+            out[kind, kch, pb] = ctx.features_backward(G)
     scale = np.abs(g64).max()
     for key, (g, s) in out.items():
         assert np.abs(g - g64).max() < 2e-5 * scale, key
         assert np.abs(s - s64).max() < 2e-5 * np.abs(s64).max(), key
-        assert np.abs(g - out[0, 0][0]).max() < 1e-5 * scale, key
+        assert np.abs(g - out[0, 0, 0][0]).max() < 1e-5 * scale, key
 
 
 def test_large_gemm_tiles_mixed_vs_fp64():
