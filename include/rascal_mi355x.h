@@ -205,6 +205,20 @@ int rgpu_gap_kmm(rgpu_context* ctx, double* kmm);
  * sparse_kernel_predict.hh:206-377). */
 int rgpu_gap_predict(rgpu_context* ctx, double* energy, double* gradient, double* neg_stress);
 
+/* Allocation-free GAP energies: energy[n_struct] (device memory) = K_NM w,
+ * as rgpu_gap_predict's energies.  No counterpart in the reference: for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * serving loops, harnesses that forbid hipMalloc).  It calls no hipMalloc,
+ * hipFree or other allocating routine: the per-centre energies live in a
+ * static device array of 2^21 doubles (16 MiB per device), so batches of at
+ * most 2097152 centres are accepted (larger ones are rejected with an
+ * error), and every context buffer must already be sized for the batch
+ * (one rgpu_gap_predict on it first; otherwise an error, nothing allocated).
+ * The call is stream-ordered (no synchronisation): the energies are ready
+ * when the context's stream reaches them.  The scratch array is shared by
+ * every call on a device: calls on one device that overlap in time race. */
+int rgpu_gap_energy_noalloc(rgpu_context* ctx, double* energy);
+
 /* Stage timers: HIP events recorded on the context stream around each
  * stage's kernels.  rgpu_get_timing synchronises and returns the summed
  * milliseconds and the number of launches since the last reset.  No

@@ -78,6 +78,7 @@ def lib():
     L.rgpu_gap_kmm.argtypes = [p, p]
     L.rgpu_gap_knm_gradient.argtypes = [p, i, p]
     L.rgpu_gap_predict.argtypes = [p, p, p, p]
+    L.rgpu_gap_energy_noalloc.argtypes = [p, p]
     L.rgpu_eval_radial.argtypes = [p, i64, p, p, p]
     L.rgpu_eval_ylm.argtypes = [p, i64, p, p, p]
     L.rgpu_enable_timing.argtypes = [p, i]
@@ -88,7 +89,7 @@ def lib():
                  "rgpu_set_structures", "rgpu_update_positions", "rgpu_get_neighbours",
                  "rgpu_compute", "rgpu_get_expansion", "rgpu_get_features", "rgpu_get_features_gradient",
                  "rgpu_features_backward", "rgpu_set_sparse_points", "rgpu_gap_knm", "rgpu_gap_kmm", "rgpu_gap_knm_gradient",
-                 "rgpu_gap_predict", "rgpu_eval_radial", "rgpu_eval_ylm", "rgpu_enable_timing",
+                 "rgpu_gap_predict", "rgpu_gap_energy_noalloc", "rgpu_eval_radial", "rgpu_eval_ylm", "rgpu_enable_timing",
                  "rgpu_get_timing", "rgpu_reset_timing"):
         getattr(L, name).restype = C.c_int
     _lib = L

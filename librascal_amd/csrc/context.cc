@@ -2,9 +2,11 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <type_traits>
 #include <vector>
@@ -61,6 +63,9 @@ struct Context {
   DeviceBuffer<int64_t> d_atom_off;
   DeviceBuffer<int> d_centre_struct;
   PairList pl;
+  // a deferred size check found the list too small and no rebuild followed
+  // (the allocation-free path cannot rebuild): the next check rebuilds
+  bool nl_stale = false;
   // centres: every atom, or the atoms selected by a centre mask, in atom
   // order (StructureManagerCenters puts them first,
   // structure_manager_centers.cc:37-100)
@@ -122,12 +127,12 @@ struct Context {
   bool use_graphs = false;
   hipStream_t own_stream = nullptr;  // created for capture when no stream was given
   hipGraphExec_t rep_exec = nullptr;
-  std::vector<uintptr_t> rep_sig;
+  std::array<uintptr_t, 21> rep_sig{};
   int64_t graph_captures = 0, graph_launches = 0;
   void drop_graph() {
     if (rep_exec) (void)hipGraphExecDestroy(rep_exec);
     rep_exec = nullptr;
-    rep_sig.clear();
+    rep_sig.fill(0);
   }
 
   // stage timers (HIP events on `stream`)
@@ -471,7 +476,8 @@ void require_batch_unsettled(const Context& c) {
 // list is built again (synchronously) and the representation, computed from
 // a list without pairs, is invalidated.  Returns false in that case.
 bool settle_nl(Context& c) {
-  if (finish_pair_list(c.pl, c.stream)) return true;
+  if (!c.nl_stale && finish_pair_list(c.pl, c.stream)) return true;
+  c.nl_stale = false;
   rebuild_pair_list(c.pl, c.hp.cutoff, c.stream);
   c.have_rep = false;
   return false;
@@ -584,16 +590,23 @@ void compute_rep(Context& c) {
       if (xs) launch_sparsify(n, D, c.D(), c.d_sel_col.get(), pp, xs, n2, c.stream);
     }
   };
-  if (c.use_graphs && !c.timing && n > 0) {
-    // replay a captured graph of the stage while every buffer, size and
-    // precision it was captured with is unchanged
-    const std::vector<uintptr_t> sig = {
+  std::array<uintptr_t, 21> sig{};
+  bool replay = c.use_graphs && !c.timing && n > 0;
+  if (replay) {
+    sig = {
         uintptr_t(c.precision), uintptr_t(S), uintptr_t(n), uintptr_t(cap), uintptr_t(cc),
         uintptr_t(km), uintptr_t(fc), uintptr_t(dfc), uintptr_t(pp), uintptr_t(ph), uintptr_t(xs),
         uintptr_t(pl), uintptr_t(n2), uintptr_t(c.pl.centre_off.get()),
         uintptr_t(c.pl.pair_j.get()), uintptr_t(c.pl.pair_vec.get()), uintptr_t(c.pl.pair_dist.get()),
         uintptr_t(c.d_species_of.get()), uintptr_t(c.tile().pb_fwd), uintptr_t(c.tile().kc),
         uintptr_t(c.d_sel_col.get())};
+    // (instantiating a graph allocates: the allocation-free path launches
+    // the stage eagerly instead of capturing it)
+    if ((!c.rep_exec || sig != c.rep_sig) && g_no_alloc) replay = false;
+  }
+  if (replay) {
+    // replay a captured graph of the stage while every buffer, size and
+    // precision it was captured with is unchanged
     if (!c.rep_exec || sig != c.rep_sig) {
       c.drop_graph();
       hipGraph_t g = nullptr;
@@ -629,6 +642,31 @@ void compute_rep(Context& c) {
 }
 
 // (callers other than the prediction have settled the neighbour list)
+void compute(Context& c);
+
+// The allocation-free GAP energy keeps its per-centre energies in this
+// file-scope device array (each device holds its own copy until the process
+// ends).  Host code never uses the array's name as an address: the address
+// on each device comes from hipGetSymbolAddress, once per device, with that
+// device current.
+constexpr int64_t kScratchCentres = int64_t(1) << 21;  // 16 MiB of doubles
+static __device__ __align__(256) double scratch[kScratchCentres];
+
+double* scratch_on_current_device() {
+  static std::mutex mu;
+  static double* ptr[64] = {};
+  int dev = 0;
+  RGPU_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= 64) throw RuntimeError("device index out of range");
+  std::lock_guard<std::mutex> lock(mu);
+  if (!ptr[dev]) {
+    void* p = nullptr;
+    RGPU_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    ptr[dev] = static_cast<double*>(p);
+  }
+  return ptr[dev];
+}
+
 void compute(Context& c) {
   require_batch_unsettled(c);
   if (c.precision == RGPU_FP64)
@@ -872,6 +910,30 @@ void gap_predict_t(Context& c, double* energy, double* gradient, double* neg_str
     backward_device<T>(c, G, gd, cv, ec, double(c.zeta));
   }
   emit_outputs(c, ec, energy, gd, cv, gradient, neg_stress);
+}
+
+// Per-structure GAP energies into energy (device memory) with the
+// per-centre energies in ec (the scratch array): no buffer of the context
+// grows and nothing is allocated (the caller has set the scope).
+template <class T>
+void gap_energy_noalloc_t(Context& c, double* ec, double* energy) {
+  const int S = int(c.species.size());
+  const int64_t n = c.n_atoms, D = std::is_same<T, float>::value ? c.Dp() : c.D();
+  launch_fill(ec, n, 0.0, c.stream);
+  const T* X = nullptr;
+  if constexpr (std::is_same<T, double>::value) X = invariants_of<double>(c);
+  std::array<GapBlock<T>, kMaxSpecies> blks{};
+  for (int s = 0; s < S; ++s) {
+    blks[size_t(s)] = gap_block<T>(c, s, X, sparse_values<T>(c));
+    blks[size_t(s)].D = D;
+  }
+  if constexpr (std::is_same<T, float>::value)
+    run_gap_mixed_kernel(blks.data(), S, c.mixed_sparse, SplitPlanes{c.p_hi.get(), c.p_lo.get()}, ec,
+                         nullptr, c.stream);
+  else
+    run_gap_f64_kernel(blks.data(), S, ec, nullptr, c.stream);
+  launch_fill(energy, c.n_struct, 0.0, c.stream);
+  launch_segment_sum(ec, 1, c.d_atom_off.get(), c.n_struct, energy, c.stream);
 }
 
 template <class T>
@@ -1704,6 +1766,41 @@ int rgpu_gap_predict(rgpu_context* ctx, double* energy, double* gradient, double
         gap_predict_t<float>(c, energy, gradient, neg_stress);
       if (settle_nl(c)) break;
     }
+  });
+}
+
+int rgpu_gap_energy_noalloc(rgpu_context* ctx, double* energy) {
+  return guarded([&] {
+    if (!ctx || !energy) throw InvalidArgument("null argument");
+    Context& c = ctx->c;
+    require_batch_unsettled(c);
+    if (!c.have_sparse) throw InvalidArgument(kNoSparse);
+    if (!c.have_weights) throw InvalidArgument("sparse points were set without weights");
+    if (c.n_atoms > kScratchCentres)
+      throw InvalidArgument("rgpu_gap_energy_noalloc: the batch has " + std::to_string(c.n_atoms) +
+                            " centres, more than the " + std::to_string(kScratchCentres) +
+                            " its scratch holds (use rgpu_gap_predict)");
+    if (c.n_struct > 0 && !is_device_memory(energy))
+      throw InvalidArgument("rgpu_gap_energy_noalloc writes device memory: energy must be a "
+                            "device pointer");
+    if (c.timing)
+      throw InvalidArgument("stage timing creates events: disable it for rgpu_gap_energy_noalloc");
+    double* ec = scratch_on_current_device();
+    NoAllocScope no_alloc;
+    // a deferred size check that finds the list too small needs a rebuild,
+    // which allocates: leave it to the next allocating call
+    if (c.nl_stale || !finish_pair_list(c.pl, c.stream)) {
+      c.nl_stale = true;
+      c.have_rep = false;
+      throw RuntimeError(
+          "the neighbour list outgrew its buffers: run rgpu_gap_predict (it rebuilds the list) "
+          "before rgpu_gap_energy_noalloc");
+    }
+    ensure_rep(c);
+    if (c.precision == RGPU_FP64)
+      gap_energy_noalloc_t<double>(c, ec, energy);
+    else
+      gap_energy_noalloc_t<float>(c, ec, energy);
   });
 }
 

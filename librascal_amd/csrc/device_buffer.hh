@@ -9,6 +9,23 @@
 
 namespace rgpu {
 
+// Set while an allocation-free entry point runs on this thread: a buffer
+// that would have to grow throws instead of allocating.
+inline thread_local bool g_no_alloc = false;
+
+struct NoAllocScope {
+  NoAllocScope() { g_no_alloc = true; }
+  ~NoAllocScope() { g_no_alloc = false; }
+  NoAllocScope(const NoAllocScope&) = delete;
+  NoAllocScope& operator=(const NoAllocScope&) = delete;
+};
+
+[[noreturn]] inline void refuse_allocation() {
+  throw RuntimeError(
+      "this call allocates no device memory, and a buffer of the context is too small for the "
+      "current batch: run rgpu_gap_predict once for this batch first");
+}
+
 template <class T>
 class DeviceBuffer {
  public:
@@ -20,6 +37,7 @@ class DeviceBuffer {
   // Ensure capacity for n elements (contents are not preserved on growth).
   T* reserve(size_t n) {
     if (n > cap_) {
+      if (g_no_alloc) refuse_allocation();
       release();
       size_t bytes = (n == 0 ? 1 : n) * sizeof(T);
       RGPU_HIP(hipMalloc(reinterpret_cast<void**>(&ptr_), bytes));
@@ -54,6 +72,7 @@ class PinnedBuffer {
 
   T* reserve(size_t n) {
     if (n > cap_) {
+      if (g_no_alloc) refuse_allocation();
       release();
       RGPU_HIP(hipHostMalloc(reinterpret_cast<void**>(&ptr_), (n == 0 ? 1 : n) * sizeof(T),
                              hipHostMallocDefault));

@@ -123,6 +123,8 @@ void launch_backward_split(const RadialDev& rad, int S, const PairView& pv, cons
 // scale[i] = zeta d[i*ld]^(zeta-1) for the centres of species s, else 0
 void launch_knm_gradient_scale(int64_t n, const double* d, int64_t ld, const int* species_of, int s,
                                int zeta, double* scale, hipStream_t stream);
+// p[0, n) = v (a kernel: no memset node when the stream is captured)
+void launch_fill(double* p, int64_t n, double v, hipStream_t stream);
 // dst[r * ld] = src[r], r < n
 void launch_strided_copy(const double* src, int64_t n, double* dst, int64_t ld, hipStream_t stream);
 

@@ -1614,6 +1614,12 @@ __global__ void pack_gradient_kernel(int S, int N, int L1, const double* __restr
   }
 }
 
+__global__ void fill_f64_kernel(double* __restrict__ p, int64_t n, double v) {
+  for (int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; e < n;
+       e += int64_t(gridDim.x) * blockDim.x)
+    p[e] = v;
+}
+
 // n_pairs is read on the device (the last entry of the pair offsets), so one
 // launch shape (the pair capacity) serves every neighbour-list rebuild
 __global__ void pair_cutoff_kernel(RadialDev rad, const int64_t* __restrict__ n_pairs, int64_t cap,
@@ -2159,6 +2165,13 @@ void launch_normalise(int64_t n, int64_t D, const T* p, const double* norm2, boo
   unsigned blocks = unsigned(std::min<int64_t>((total + 255) / 256, 65536));
   hipLaunchKernelGGL(normalise_kernel<T>, dim3(blocks), dim3(256), 0, stream, n, D, p, norm2,
                      int(normalize), rows, out);
+  RGPU_LAUNCH_CHECK();
+}
+
+void launch_fill(double* p, int64_t n, double v, hipStream_t stream) {
+  if (n <= 0) return;
+  const unsigned blocks = unsigned(std::min<int64_t>((n + 255) / 256, 4096));
+  hipLaunchKernelGGL(fill_f64_kernel, dim3(blocks), dim3(256), 0, stream, p, n, v);
   RGPU_LAUNCH_CHECK();
 }
 
