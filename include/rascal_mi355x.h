@@ -66,6 +66,7 @@ extern "C" {
 #define RGPU_INFO_NL_GRAPH_LAUNCHES 17 /* neighbour-list passes replayed from a graph */
 #define RGPU_INFO_N_ATOMS 18       /* atoms of the batch (N_CENTRES counts the centres) */
 #define RGPU_INFO_TILE_ADJ_KIND 19 /* mixed-precision adjoint: 0 vector ALUs, 1 matrix cores */
+#define RGPU_INFO_HALF_LIST 20     /* 1: the expansion runs over the half neighbour list */
 
 /* stages timed by rgpu_enable_timing */
 #define RGPU_STAGE_NEIGHBOURS 0
@@ -111,6 +112,16 @@ int rgpu_autotune(rgpu_context* ctx);
  * partitions to rounding.  No counterpart in the reference. */
 int rgpu_set_tiles(rgpu_context* ctx, int pairs_fwd, int keys_ps, int pairs_adj, int keys_adj,
                    int adj_kind);
+/* Expansion over the half neighbour list (enable 1) or the full list (0,
+ * the default): each unordered pair's contribution is computed once and
+ * given to the other centre with the parity (-1)^l; the results equal the
+ * full list's.  The gradients (adjoint) always use the full list.  Refused,
+ * with the reference's messages, for masked centres and for periodic cell
+ * vectors shorter than twice the cutoff.  Replaces AdaptorHalfList in the
+ * manager stack (adaptor_half_neighbour_list.hh:319-380) with the
+ * expansion's half-list branch (calculator_spherical_expansion.hh
+ * :2159-2185, 2282-2299).  rgpu_autotune picks it where it is faster. */
+int rgpu_set_half_list(rgpu_context* ctx, int enable);
 /* RGPU_FP64 (default) or RGPU_MIXED; invalidates computed results. */
 int rgpu_set_precision(rgpu_context* ctx, int precision);
 int rgpu_get_info(rgpu_context* ctx, int key, int64_t* value);

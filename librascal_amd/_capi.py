@@ -26,7 +26,7 @@ INFO = dict(max_radial=0, max_angular=1, n_species=2, feature_size=3, n_centres=
             n_pairs=5, n_struct=6, spline_grid=7, n_sparse=8, is_invariants=9, precision=10,
             graph_captures=11, graph_launches=12, tile_pairs_fwd=13, tile_keys_ps=14,
             tile_pairs_adj=15, tile_keys_adj=16, nl_graph_launches=17, n_atoms=18,
-            tile_adj_kind=19)
+            tile_adj_kind=19, half_list=20)
 
 _lib = None
 
@@ -61,6 +61,7 @@ def lib():
     L.rgpu_context_destroy.restype = None
     L.rgpu_set_stream.argtypes = [p, p]
     L.rgpu_set_precision.argtypes = [p, i]
+    L.rgpu_set_half_list.argtypes = [p, i]
     L.rgpu_enable_graphs.argtypes = [p, i]
     L.rgpu_autotune.argtypes = [p]
     L.rgpu_set_tiles.argtypes = [p, i, i, i, i, i]
@@ -85,6 +86,7 @@ def lib():
     L.rgpu_get_timing.argtypes = [p, i, C.POINTER(C.c_double), C.POINTER(C.c_int64)]
     L.rgpu_reset_timing.argtypes = [p]
     for name in ("rgpu_context_create", "rgpu_set_stream", "rgpu_set_precision", "rgpu_get_info",
+                 "rgpu_set_half_list",
                  "rgpu_enable_graphs", "rgpu_autotune", "rgpu_set_tiles",
                  "rgpu_set_structures", "rgpu_update_positions", "rgpu_get_neighbours",
                  "rgpu_compute", "rgpu_get_expansion", "rgpu_get_features", "rgpu_get_features_gradient",
@@ -152,6 +154,11 @@ class Context:
     def set_stream(self, stream_handle):
         check(lib().rgpu_set_stream(self._p, C.c_void_p(stream_handle)))
 
+    def set_half_list(self, enable=True):
+        """Expansion over the half neighbour list (results equal the full
+        list's)."""
+        check(lib().rgpu_set_half_list(self._p, int(bool(enable))))
+
     def set_precision(self, precision):
         check(lib().rgpu_set_precision(self._p, int(precision)))
 
@@ -168,7 +175,8 @@ class Context:
         """Work partitions of the current precision (0 / -1: built-in)."""
         return dict(pairs_fwd=self.info("tile_pairs_fwd"), keys_ps=self.info("tile_keys_ps"),
                     pairs_adj=self.info("tile_pairs_adj"),
-                    keys_adj=self.info("tile_keys_adj"), adj_kind=self.info("tile_adj_kind"))
+                    keys_adj=self.info("tile_keys_adj"), adj_kind=self.info("tile_adj_kind"),
+                    half_list=self.info("half_list"))
 
     def set_tiles(self, pairs_fwd=0, keys_ps=0, pairs_adj=0, keys_adj=0, adj_kind=0):
         """Set the work partitions of the current precision (see tiles())."""

@@ -84,6 +84,11 @@ struct Context {
   DeviceBuffer<double> ps64;         // selected invariants [n_atoms][n_sel] (un-normalised)
   DeviceBuffer<double> gfull;        // dL/dX scattered into the full layout
   DeviceBuffer<double> dcp, dcs;     // stored expansion derivatives (gradient export)
+  // half neighbour list (rgpu_set_half_list) and its contribution rows
+  DeviceBuffer<int> half_count, half_j;
+  DeviceBuffer<int64_t> half_off, half_src, half_scan;
+  DeviceBuffer<double> half_vec, half_dist, half_fc, half_dfc, half_rows64;
+  DeviceBuffer<float> half_rows32;
   DeviceBuffer<float> c32, p32;
   DeviceBuffer<double> norm2, tmp_norm2;
   DeviceBuffer<unsigned long long> keymask;
@@ -127,7 +132,7 @@ struct Context {
   bool use_graphs = false;
   hipStream_t own_stream = nullptr;  // created for capture when no stream was given
   hipGraphExec_t rep_exec = nullptr;
-  std::array<uintptr_t, 21> rep_sig{};
+  std::array<uintptr_t, 24> rep_sig{};
   int64_t graph_captures = 0, graph_launches = 0;
   void drop_graph() {
     if (rep_exec) (void)hipGraphExecDestroy(rep_exec);
@@ -550,6 +555,8 @@ DeviceBuffer<float>& pbuf<float>(Context& c) {
   return c.p32;
 }
 
+void check_half_list(const Context& c);
+
 template <class T>
 void compute_rep(Context& c) {
   const int S = int(c.species.size());
@@ -577,9 +584,32 @@ void compute_rep(Context& c) {
                                         : c.ps64.reserve(size_t(std::max<int64_t>(n * c.D(), 1)));
   c.p32_valid = false;
   const int64_t* n_pairs_dev = c.pl.centre_off.get() + n;
+  // the half list's buffers are sized for the full list, before any capture
+  HalfList hl{};
+  if (c.tile().half_nl && n > 0) {
+    check_half_list(c);
+    const size_t cp = size_t(std::max<int64_t>(cap, 1)), B = size_t(c.rd.N) * c.rd.LL;
+    hl.count = c.half_count.reserve(size_t(n));
+    hl.off = c.half_off.reserve(size_t(n + 1));
+    hl.src = c.half_src.reserve(cp);
+    hl.j = c.half_j.reserve(cp);
+    hl.vec = c.half_vec.reserve(3 * cp);
+    hl.dist = c.half_dist.reserve(cp);
+    hl.fc = c.half_fc.reserve(cp);
+    hl.dfc = c.half_dfc.reserve(cp);
+    if constexpr (std::is_same<T, float>::value)
+      hl.rows = c.half_rows32.reserve(cp * B);
+    else
+      hl.rows = c.half_rows64.reserve(cp * B);
+    reserve_half_list_scan(n, c.half_scan);
+    hl.scan_scratch = &c.half_scan;
+  }
   auto cutoff_and_expansion = [&] {
     launch_pair_cutoff(c.rd, n_pairs_dev, cap, c.pl.pair_dist.get(), fc, dfc, c.stream);
-    launch_expansion<T>(c.rd, S, pair_view(c), cc, km, c.stream, c.tile());
+    if (hl.rows)
+      launch_expansion_half<T>(c.rd, S, pair_view(c), c.pl.pos.get(), hl, cc, km, c.stream, c.tile());
+    else
+      launch_expansion<T>(c.rd, S, pair_view(c), cc, km, c.stream, c.tile());
   };
   auto invariants = [&] {
     if (!ps) return;
@@ -590,7 +620,7 @@ void compute_rep(Context& c) {
       if (xs) launch_sparsify(n, D, c.D(), c.d_sel_col.get(), pp, xs, n2, c.stream);
     }
   };
-  std::array<uintptr_t, 21> sig{};
+  std::array<uintptr_t, 24> sig{};
   bool replay = c.use_graphs && !c.timing && n > 0;
   if (replay) {
     sig = {
@@ -599,7 +629,8 @@ void compute_rep(Context& c) {
         uintptr_t(pl), uintptr_t(n2), uintptr_t(c.pl.centre_off.get()),
         uintptr_t(c.pl.pair_j.get()), uintptr_t(c.pl.pair_vec.get()), uintptr_t(c.pl.pair_dist.get()),
         uintptr_t(c.d_species_of.get()), uintptr_t(c.tile().pb_fwd), uintptr_t(c.tile().kc),
-        uintptr_t(c.d_sel_col.get())};
+        uintptr_t(c.d_sel_col.get()), uintptr_t(c.tile().half_nl), uintptr_t(hl.src),
+        uintptr_t(hl.rows)};
     // (instantiating a graph allocates: the allocation-free path launches
     // the stage eagerly instead of capturing it)
     if ((!c.rep_exec || sig != c.rep_sig) && g_no_alloc) replay = false;
@@ -682,6 +713,31 @@ void ensure_rep(Context& c) {
 void require_invariants(const Context& c) {
   if (c.hp.kind != SoapHypers::kPowerSpectrum)
     throw InvalidArgument("this operation needs SphericalInvariants (PowerSpectrum) hypers");
+}
+
+// calculator_spherical_expansion.hh:2159-2185: the half list needs every
+// atom a centre and periodic cell vectors longer than the expansion's
+// diameter
+void check_half_list(const Context& c) {
+  if (c.masked)
+    throw RuntimeError(
+        "Half neighbor list should only be used when all the atoms inside the unit cell are "
+        "centers, i.e. center_atoms_mask should not mask atoms.");
+  for (int s = 0; s < c.n_struct; ++s) {
+    const double* C = &c.cells[9 * size_t(s)];
+    double len[3];
+    bool small = false;
+    for (int k = 0; k < 3; ++k) {
+      len[k] = std::sqrt(C[k] * C[k] + C[3 + k] * C[3 + k] + C[6 + k] * C[6 + k]);
+      if (c.pbc[3 * size_t(s) + k] && len[k] < 2. * c.hp.cutoff) small = true;
+    }
+    if (small)
+      throw RuntimeError(
+          "Half neighbor list should only be used when the diameter of the spherical expansion is "
+          "smaller than the unit cell in periodic directions: [" + std::to_string(len[0]) + " " +
+          std::to_string(len[1]) + " " + std::to_string(len[2]) + "] > " +
+          std::to_string(2 * c.hp.cutoff));
+  }
 }
 
 // calculator_spherical_expansion.hh:2155-2158
@@ -1036,6 +1092,42 @@ void autotune_t(Context& c) {
       if (ms < best) best = ms, pick = pb;
     }
     t.pb_fwd = pick;
+    // the expansion over the half neighbour list, where it applies
+    bool half_ok = true;
+    try {
+      check_half_list(c);
+    } catch (const RuntimeError&) {
+      half_ok = false;
+    }
+    if (half_ok) {
+      const double* pos = c.pl.pos.get();
+      const size_t cp = size_t(std::max<int64_t>(c.pl.cap_pairs, 1)), B = size_t(c.rd.N) * c.rd.LL;
+      HalfList hl{};
+      hl.count = c.half_count.reserve(size_t(n));
+      hl.off = c.half_off.reserve(size_t(n + 1));
+      hl.src = c.half_src.reserve(cp);
+      hl.j = c.half_j.reserve(cp);
+      hl.vec = c.half_vec.reserve(3 * cp);
+      hl.dist = c.half_dist.reserve(cp);
+      hl.fc = c.half_fc.reserve(cp);
+      hl.dfc = c.half_dfc.reserve(cp);
+      if constexpr (std::is_same<T, float>::value)
+        hl.rows = c.half_rows32.reserve(cp * B);
+      else
+        hl.rows = c.half_rows64.reserve(cp * B);
+      reserve_half_list_scan(n, c.half_scan);
+      hl.scan_scratch = &c.half_scan;
+      SoapTiles full = t, half = t;
+      full.half_nl = 0;
+      half.half_nl = 1;
+      const float ms_full =
+          time_ms([&] { launch_expansion<T>(c.rd, S, pv, cc, km, c.stream, full); });
+      const float ms_half = time_ms(
+          [&] { launch_expansion_half<T>(c.rd, S, pv, pos, hl, cc, km, c.stream, half); });
+      t.half_nl = ms_half < ms_full ? 1 : 0;
+    } else {
+      t.half_nl = 0;
+    }
   }
   if (ps) {
     const int K = S * (S + 1) / 2;
@@ -1217,6 +1309,17 @@ int rgpu_enable_graphs(rgpu_context* ctx, int enable) {
   });
 }
 
+int rgpu_set_half_list(rgpu_context* ctx, int enable) {
+  return guarded([&] {
+    if (!ctx) throw InvalidArgument("null context");
+    Context& c = ctx->c;
+    if (enable && c.have_batch) check_half_list(c);
+    c.tiles[0].half_nl = c.tiles[1].half_nl = enable ? 1 : 0;
+    c.have_rep = false;
+    c.drop_graph();
+  });
+}
+
 int rgpu_set_precision(rgpu_context* ctx, int precision) {
   return guarded([&] {
     if (!ctx) throw InvalidArgument("null context");
@@ -1259,6 +1362,7 @@ int rgpu_get_info(rgpu_context* ctx, int key, int64_t* value) {
       case RGPU_INFO_TILE_PAIRS_ADJ: *value = c.tile().pb_adj; break;
       case RGPU_INFO_TILE_KEYS_ADJ: *value = c.tile().keys_adj; break;
       case RGPU_INFO_TILE_ADJ_KIND: *value = c.tile().adj_kind; break;
+      case RGPU_INFO_HALF_LIST: *value = c.tile().half_nl; break;
       default: throw InvalidArgument("unknown info key");
     }
   });

@@ -5,6 +5,7 @@
 
 #include <cstdint>
 
+#include "device_buffer.hh"
 #include "radial.hh"
 
 namespace rgpu {
@@ -35,6 +36,7 @@ struct SoapTiles {
   int pb_adj = 0;  // pairs staged per batch in the adjoint (<= 64)
   int keys_adj = 0;  // key blocks staged per pass in the FP32 adjoint
   int adj_kind = 0;  // FP32 adjoint: 0 vector ALUs, 1 matrix cores where they apply
+  int half_nl = 0;   // expansion over the half neighbour list (1) or the full list (0)
 };
 
 // c[i][s][n][l*l+l+m] (finalised) and the per-centre key mask.
@@ -42,6 +44,31 @@ template <class T>
 void launch_expansion(const RadialDev& rad, int S, const PairView& pv, T* c,
                       unsigned long long* keymask, hipStream_t stream,
                       const SoapTiles& tiles = SoapTiles());
+
+// Half neighbour list of a full list (device buffers sized for the full
+// list; rows holds n_pairs x N (L+1)^2 values of the expansion precision)
+// and the expansion computed over it: the same c and key masks as
+// launch_expansion (adaptor_half_neighbour_list.hh:319-380;
+// calculator_spherical_expansion.hh:2282-2299).  pos: the atom positions
+// the list was built from.
+struct HalfList {
+  int* count;       // [n_centres] kept pairs
+  int64_t* off;     // [n_centres + 1]
+  int64_t* src;     // [n_pairs] row read by each full pair (< 0: mirror, -1 - row)
+  int* j;           // kept pairs: neighbour atom
+  double* vec;      //   [3] pair vector
+  double* dist;     //   distance
+  double* fc;       //   cutoff function
+  double* dfc;      //   its derivative
+  void* rows;       //   [N][(L+1)^2] contribution fc R Y
+  DeviceBuffer<int64_t>* scan_scratch;
+};
+// scratch of the half list's scan for n centres (reserved before a capture)
+void reserve_half_list_scan(int64_t n, DeviceBuffer<int64_t>& scratch);
+template <class T>
+void launch_expansion_half(const RadialDev& rad, int S, const PairView& pv, const double* pos,
+                           const HalfList& hl, T* c, unsigned long long* keymask, hipStream_t stream,
+                           const SoapTiles& tiles = SoapTiles());
 
 // p[i][key(a<=b)][n1*N+n2][l] UN-normalised, and norm2[i] = |p_i|^2 (FP64).
 template <class T>

@@ -28,6 +28,7 @@
 
 #include "soap_device.hh"
 #include "packing.hh"
+#include "scan.hh"
 #include "soap_kernels.hh"
 
 namespace rgpu {
@@ -1735,6 +1736,174 @@ __global__ __launch_bounds__(kThreads) void ylm_eval_kernel(RadialDev rad, const
 }
 
 // ---------------------------------------------------------------------------
+// Half neighbour list (adaptor_half_neighbour_list.hh:319-380: a pair i -> j
+// is kept when tag_j > tag_i, periodic images carrying the largest tags)
+// and the expansion over it (calculator_spherical_expansion.hh:2282-2299:
+// c^{j, s_i} += (-1)^l c^{ij}).  Built from the full list on the device:
+// the kept pairs are the self pair, the original neighbours j > i and every
+// periodic image; a dropped pair (original j < i) takes the row of its
+// mirror (j -> i, kept in j's list) with the parity (-1)^l, since
+// Y_lm(-r) = (-1)^l Y_lm(r) and R depends on |r| only.
+
+// pair p of centre i is the original neighbour j (no periodic shift): its
+// vector is exactly r_j - r_i, as the neighbour list forms it for originals
+__device__ __forceinline__ bool pair_original(const PairView& pv, const double* __restrict__ pos,
+                                              int64_t p, int64_t i, int j) {
+  return pv.pair_vec[3 * p] == pos[3 * j] - pos[3 * i] &&
+         pv.pair_vec[3 * p + 1] == pos[3 * j + 1] - pos[3 * i + 1] &&
+         pv.pair_vec[3 * p + 2] == pos[3 * j + 2] - pos[3 * i + 2];
+}
+
+__device__ __forceinline__ bool half_keep(const PairView& pv, const double* __restrict__ pos,
+                                          int64_t p, int64_t i) {
+  if (p == pv.centre_off[i]) return true;  // the self pair
+  const int j = pv.pair_j[p];
+  return j > int(i) || !pair_original(pv, pos, p, i, j);
+}
+
+__global__ void half_count_kernel(PairView pv, const double* __restrict__ pos, int* __restrict__ count) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= pv.n_centres) return;
+  int k = 0;
+  for (int64_t p = pv.centre_off[i]; p < pv.centre_off[i + 1]; ++p) k += half_keep(pv, pos, p, i);
+  count[i] = k;
+}
+
+// the kept pairs gathered in list order (a CSR by centre, self pair
+// first), and for every full pair the row it reads: src >= 0 its own kept
+// row, src < 0 the row -1 - src of its mirror
+__global__ void half_fill_kernel(PairView pv, const double* __restrict__ pos,
+                                 const int64_t* __restrict__ hoff, int64_t* __restrict__ src,
+                                 int* __restrict__ hj, double* __restrict__ hvec,
+                                 double* __restrict__ hdist, double* __restrict__ hfc,
+                                 double* __restrict__ hdfc) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= pv.n_centres) return;
+  int64_t h = hoff[i];
+  for (int64_t p = pv.centre_off[i]; p < pv.centre_off[i + 1]; ++p) {
+    if (!half_keep(pv, pos, p, i)) continue;
+    src[p] = h;
+    hj[h] = pv.pair_j[p];
+    hvec[3 * h] = pv.pair_vec[3 * p];
+    hvec[3 * h + 1] = pv.pair_vec[3 * p + 1];
+    hvec[3 * h + 2] = pv.pair_vec[3 * p + 2];
+    hdist[h] = pv.pair_dist[p];
+    hfc[h] = pv.pair_fc[p];
+    hdfc[h] = pv.pair_dfc[p];
+    ++h;
+  }
+}
+
+__global__ void half_mirror_kernel(PairView pv, const double* __restrict__ pos, int64_t* __restrict__ src) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= pv.n_centres) return;
+  for (int64_t p = pv.centre_off[i] + 1; p < pv.centre_off[i + 1]; ++p) {
+    if (half_keep(pv, pos, p, i)) continue;
+    const int j = pv.pair_j[p];
+    int64_t m = INT64_MIN;
+    for (int64_t q = pv.centre_off[j] + 1; q < pv.centre_off[j + 1]; ++q)
+      if (pv.pair_j[q] == int(i) && pair_original(pv, pos, q, j, int(i))) {
+        m = -1 - src[q];
+        break;
+      }
+    src[p] = m;
+  }
+}
+
+// rows[h][n][lm] = fc R_nl Y_lm of the kept pairs (not the self pairs), one
+// workgroup per centre over its kept pairs (hv: the half list as a
+// PairView)
+template <class T, bool DIRECT>
+__global__ __launch_bounds__(kThreads) void half_rows_kernel(RadialDev rad, int S, int PB, PairView hv,
+                                                             T* __restrict__ rows) {
+  const int64_t i = blockIdx.x;
+  const int N = rad.N, L1 = rad.L1, LL = rad.LL, F = N * L1, B = N * LL;
+  extern __shared__ __align__(64) unsigned char smem_raw[];
+  T* yc = reinterpret_cast<T*>(smem_raw);
+  PairBatch<T> pb = carve_batch<T, DIRECT, false>(smem_raw + ylm_table_bytes<T>(L1), PB, LL, F);
+  load_ylm_table<T>(rad, yc);
+  const int64_t p0 = hv.centre_off[i] + 1;
+  const int np = int(hv.centre_off[i + 1] - p0);
+  const FastDiv fB = fast_div(B), fLL = fast_div(LL);
+  PairPre pre;
+  pair_fetch<DIRECT, false>(rad, hv, p0, max(0, min(PB, np)), pre);
+  for (int q0 = 0; q0 < np; q0 += PB) {
+    const int nb = min(PB, np - q0);
+    stage_pairs<T, DIRECT, false>(rad, yc, pre, nb, pb, nullptr, nullptr, nullptr);
+    if (q0 + PB < np) pair_fetch<DIRECT, false>(rad, hv, p0 + q0 + PB, min(PB, np - q0 - PB), pre);
+    for (int it = threadIdx.x; it < nb * B; it += kThreads) {
+      const int q = it / fB, e = it - q * B, n = e / fLL, lm = e - n * LL;
+      rows[(p0 + q0 + q) * B + e] = pb.R[q * F + n * L1 + l_of_lm(lm)] * pb.Y[q * LL + lm];
+    }
+    __syncthreads();  // the next batch overwrites this one
+  }
+}
+
+// c[i][s][n][lm] = centre term + sum over the full list of centre i, in
+// list order, of the row of each pair (its own, or its mirror's times
+// (-1)^l) into the block of the neighbour species; and the key mask
+template <class T>
+__global__ __launch_bounds__(kThreads) void half_gather_kernel(RadialDev rad, int S, PairView pv,
+                                                               const int64_t* __restrict__ src,
+                                                               const T* __restrict__ rows,
+                                                               T* __restrict__ c,
+                                                               unsigned long long* __restrict__ keymask) {
+  const int64_t i = blockIdx.x;
+  const int N = rad.N, LL = rad.LL, B = N * LL;
+  constexpr int kChunk = kThreads;
+  __shared__ int sp[kChunk];
+  __shared__ int64_t row[kChunk];
+  __shared__ unsigned long long kmask;
+  const int si = pv.species_of[i];
+  const double inv_s4pi = 0.28209479177387814347;  // 1/sqrt(4 pi)
+  const int64_t p0 = pv.centre_off[i] + 1, p1 = pv.centre_off[i + 1];
+  if (threadIdx.x == 0) kmask = 1ull << si;
+  constexpr int kE = 4;  // entries per thread held across the pair chunks
+  for (int e0 = 0; e0 < S * B; e0 += kE * kThreads) {
+    T acc[kE];
+    int es[kE], er[kE], el[kE];
+#pragma unroll
+    for (int k = 0; k < kE; ++k) {
+      const int e = e0 + threadIdx.x + k * kThreads;
+      es[k] = e < S * B ? e / B : -1;
+      er[k] = e - es[k] * B;
+      const int lm = er[k] % LL;
+      el[k] = l_of_lm(lm);
+      acc[k] = T(0);
+      if (es[k] == si && lm == 0)
+        acc[k] = T(rad.center[(rad.sp_tab ? rad.sp_tab[si] * rad.N : 0) + er[k] / LL] * inv_s4pi);
+    }
+    for (int64_t c0 = p0; c0 < p1; c0 += kChunk) {
+      const int nq = int(min<int64_t>(kChunk, p1 - c0));
+      __syncthreads();  // the previous chunk is read
+      if (int(threadIdx.x) < nq) {
+        const int64_t p = c0 + threadIdx.x;
+        sp[threadIdx.x] = pv.species_of[pv.pair_j[p]];
+        row[threadIdx.x] = src[p];
+        if (e0 == 0) atomicOr(&kmask, 1ull << sp[threadIdx.x]);
+      }
+      __syncthreads();
+      for (int q = 0; q < nq; ++q) {
+        const int s = sp[q];
+        const int64_t r = row[q] >= 0 ? row[q] : -1 - row[q];
+        const bool mirror = row[q] < 0;
+#pragma unroll
+        for (int k = 0; k < kE; ++k)
+          if (es[k] == s) {
+            const T v = rows[r * B + er[k]];
+            acc[k] += (mirror && (el[k] & 1)) ? -v : v;
+          }
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < kE; ++k)
+      if (es[k] >= 0) c[i * int64_t(S) * B + es[k] * B + er[k]] = acc[k];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) keymask[i] = kmask;
+}
+
+// ---------------------------------------------------------------------------
 // Stored representation gradients (FP64), as the reference keeps them for
 // get_features_gradient.  Step 1, per-pair expansion derivatives
 // (calculator_spherical_expansion.hh:2306-2401): for the pair p = (i, j) that
@@ -2174,6 +2343,61 @@ void launch_fill(double* p, int64_t n, double v, hipStream_t stream) {
   hipLaunchKernelGGL(fill_f64_kernel, dim3(blocks), dim3(256), 0, stream, p, n, v);
   RGPU_LAUNCH_CHECK();
 }
+
+void reserve_half_list_scan(int64_t n, DeviceBuffer<int64_t>& scratch) {
+  exclusive_scan_reserve(n, scratch);
+}
+
+template <class T>
+void launch_expansion_half(const RadialDev& rad, int S, const PairView& pv, const double* pos,
+                           const HalfList& hl, T* c, unsigned long long* keymask, hipStream_t stream,
+                           const SoapTiles& tiles) {
+  const int64_t n = pv.n_centres;
+  if (n == 0) return;
+  const unsigned g1 = unsigned((n + 255) / 256);
+  hipLaunchKernelGGL(half_count_kernel, dim3(g1), dim3(256), 0, stream, pv, pos, hl.count);
+  RGPU_LAUNCH_CHECK();
+  exclusive_scan_device(hl.count, hl.off, n, stream, *hl.scan_scratch);
+  hipLaunchKernelGGL(half_fill_kernel, dim3(g1), dim3(256), 0, stream, pv, pos, hl.off, hl.src, hl.j,
+                     hl.vec, hl.dist, hl.fc, hl.dfc);
+  RGPU_LAUNCH_CHECK();
+  hipLaunchKernelGGL(half_mirror_kernel, dim3(g1), dim3(256), 0, stream, pv, pos, hl.src);
+  RGPU_LAUNCH_CHECK();
+  PairView hv = pv;
+  hv.centre_off = hl.off;
+  hv.pair_j = hl.j;
+  hv.pair_vec = hl.vec;
+  hv.pair_dist = hl.dist;
+  hv.pair_fc = hl.fc;
+  hv.pair_dfc = hl.dfc;
+  const int F = rad.N * rad.L1;
+  const bool direct = !rad.use_spline;
+  const size_t yt = ylm_table_bytes<T>(rad.L1);
+  int PB = tiles.pb_fwd > 0 ? std::min(tiles.pb_fwd, 64) : 64;
+  if (tiles.pb_fwd <= 0)
+    while (PB > 1 && yt + pair_batch_bytes(PB, rad.LL, F, sizeof(T), direct, false) > kFwdBudget) PB /= 2;
+  const size_t smem = yt + pair_batch_bytes(PB, rad.LL, F, sizeof(T), direct, false);
+  T* rows = static_cast<T*>(hl.rows);
+  auto go = [&](auto kern) {
+    allow_lds(kern, smem);
+    hipLaunchKernelGGL(kern, dim3(unsigned(n)), dim3(kThreads), smem, stream, rad, S, PB, hv, rows);
+  };
+  if (direct)
+    go(half_rows_kernel<T, true>);
+  else
+    go(half_rows_kernel<T, false>);
+  RGPU_LAUNCH_CHECK();
+  hipLaunchKernelGGL(half_gather_kernel<T>, dim3(unsigned(n)), dim3(kThreads), 0, stream, rad, S, pv,
+                     static_cast<const int64_t*>(hl.src), static_cast<const T*>(rows), c, keymask);
+  RGPU_LAUNCH_CHECK();
+}
+
+template void launch_expansion_half<double>(const RadialDev&, int, const PairView&, const double*,
+                                            const HalfList&, double*, unsigned long long*, hipStream_t,
+                                            const SoapTiles&);
+template void launch_expansion_half<float>(const RadialDev&, int, const PairView&, const double*,
+                                           const HalfList&, float*, unsigned long long*, hipStream_t,
+                                           const SoapTiles&);
 
 void launch_pair_gradients(const RadialDev& rad, int S, const PairView& pv, double* dcp, double* dcs,
                            hipStream_t stream) {

@@ -407,3 +407,63 @@ def test_structure_errors():
     setup(p=np.array([[0.0, 0.0, 0.0], [5.0 + 5e-11, 5.0, 5.0]]))
     with pytest.raises(_capi.RascalGPUError, match="no structures set"):
         _capi.Context(json.dumps(soap_hypers())).compute()
+
+
+# --------------------------------------------------------------------------
+# half neighbour list (test_calculator.cc:939-1083: the representation and
+# its gradients over a half list equal those over the full list)
+
+@pytest.mark.parametrize("spline", [None, 1e-8])
+def test_half_list_equals_full_list(spline):
+    from librascal_amd.synthetic import periodic_box
+    g = golden_structures()
+    sets = [g["CaCrP2O7"], g["small_molecule"] + g["SiC_moissanite_supercell"],
+            periodic_box(400, 0.06, (1, 6, 8), d_min=1.0, seed=41)]
+    for sts in sets:
+        # rc 2.5: every periodic cell vector is longer than 2 rc
+        hp = soap_hypers(rc=2.5, N=5, L=4, sigma=0.4, spline=spline)
+        full = _capi.Context(json.dumps(hp))
+        half = _capi.Context(json.dumps(hp))
+        off, pos, num, cells, pbc = pack(sts)
+        sp = sorted(set(num.tolist()))
+        for ctx in (full, half):
+            ctx.set_structures(off, pos, num, cells, pbc, sp)
+        half.set_half_list(True)
+        assert half.info("half_list") == 1
+        c_f, km_f = full.expansion()
+        c_h, km_h = half.expansion()
+        assert np.array_equal(km_f, km_h)
+        assert _rel(c_f, c_h) < 1e-13
+        X_f, _ = full.features()
+        X_h, _ = half.features()
+        assert _rel(X_f, X_h) < 1e-13
+        G = np.random.default_rng(1).normal(size=X_f.shape)
+        g_f, s_f = full.features_backward(G)
+        g_h, s_h = half.features_backward(G)
+        assert _rel(g_f, g_h) < 1e-12 and _rel(s_f, s_h) < 1e-12
+        # mixed precision over the half list: FP32 rounding of the full one
+        for ctx in (full, half):
+            ctx.set_precision(_capi.MIXED)
+        c32_f, _ = full.expansion()
+        c32_h, _ = half.expansion()
+        assert _rel(c32_f, c32_h) < 1e-5
+
+
+def test_half_list_refusals():
+    # the reference's conditions: no masked centres, periodic cells longer
+    # than twice the cutoff
+    hp = soap_hypers(rc=3.0, N=3, L=2)
+    ctx = _capi.Context(json.dumps(hp))
+    st = golden_structures()["diamond_cubic_distorted"][0]
+    off, pos, num, cells, pbc = pack([st])
+    ctx.set_structures(off, pos, num, cells, pbc, sorted(set(num.tolist())))
+    if np.linalg.norm(np.asarray(st["cell"]), axis=0).min() < 6.0:
+        with pytest.raises(_capi.RascalGPUError, match="diameter of the spherical expansion"):
+            ctx.set_half_list(True)
+    big = golden_structures()["CaCrP2O7"][0]
+    off, pos, num, cells, pbc = pack([big])
+    mask = np.zeros(len(num), bool)
+    mask[:3] = True
+    ctx.set_structures(off, pos, num, cells, pbc, sorted(set(num.tolist())), centre_mask=mask)
+    with pytest.raises(_capi.RascalGPUError, match="center_atoms_mask should not mask atoms"):
+        ctx.set_half_list(True)
