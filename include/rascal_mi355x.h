@@ -67,6 +67,8 @@ extern "C" {
 #define RGPU_INFO_N_ATOMS 18       /* atoms of the batch (N_CENTRES counts the centres) */
 #define RGPU_INFO_TILE_ADJ_KIND 19 /* mixed-precision adjoint: 0 vector ALUs, 1 matrix cores */
 #define RGPU_INFO_HALF_LIST 20     /* 1: the expansion runs over the half neighbour list */
+#define RGPU_INFO_FUSED 21         /* 1: expansion and invariants in one persistent launch */
+#define RGPU_INFO_FWD_LDS 22       /* 1: the expansion accumulates c in LDS and writes it once */
 
 /* stages timed by rgpu_enable_timing */
 #define RGPU_STAGE_NEIGHBOURS 0
@@ -108,10 +110,13 @@ int rgpu_autotune(rgpu_context* ctx);
 /* Sets the work partitions of the current precision directly (values <= 0:
  * built-in choice; keys_adj: key blocks of dL/dX staged in LDS per pass of
  * the mixed-precision adjoint; adj_kind: its vector-ALU (0) or matrix-core
- * (1) kernel), e.g. to reuse an autotuned set.  Results agree between
- * partitions to rounding.  No counterpart in the reference. */
+ * (1) kernel; fused: expansion and invariants in one persistent launch (1),
+ * two launches (0), two launches with the expansion writing c once from
+ * LDS (2), built-in (-1: two launches)), e.g. to reuse an autotuned
+ * set.  Results agree between partitions to rounding.  No counterpart in
+ * the reference. */
 int rgpu_set_tiles(rgpu_context* ctx, int pairs_fwd, int keys_ps, int pairs_adj, int keys_adj,
-                   int adj_kind);
+                   int adj_kind, int fused);
 /* Expansion over the half neighbour list (enable 1) or the full list (0,
  * the default): each unordered pair's contribution is computed once and
  * given to the other centre with the parity (-1)^l; the results equal the

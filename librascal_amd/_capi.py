@@ -26,7 +26,7 @@ INFO = dict(max_radial=0, max_angular=1, n_species=2, feature_size=3, n_centres=
             n_pairs=5, n_struct=6, spline_grid=7, n_sparse=8, is_invariants=9, precision=10,
             graph_captures=11, graph_launches=12, tile_pairs_fwd=13, tile_keys_ps=14,
             tile_pairs_adj=15, tile_keys_adj=16, nl_graph_launches=17, n_atoms=18,
-            tile_adj_kind=19, half_list=20)
+            tile_adj_kind=19, half_list=20, fused=21, fwd_lds=22)
 
 _lib = None
 
@@ -64,7 +64,7 @@ def lib():
     L.rgpu_set_half_list.argtypes = [p, i]
     L.rgpu_enable_graphs.argtypes = [p, i]
     L.rgpu_autotune.argtypes = [p]
-    L.rgpu_set_tiles.argtypes = [p, i, i, i, i, i]
+    L.rgpu_set_tiles.argtypes = [p, i, i, i, i, i, i]
     L.rgpu_get_info.argtypes = [p, i, C.POINTER(C.c_int64)]
     L.rgpu_set_structures.argtypes = [p, i, p, p, p, p, p, i, p, p]
     L.rgpu_update_positions.argtypes = [p, p, p]
@@ -176,12 +176,13 @@ class Context:
         return dict(pairs_fwd=self.info("tile_pairs_fwd"), keys_ps=self.info("tile_keys_ps"),
                     pairs_adj=self.info("tile_pairs_adj"),
                     keys_adj=self.info("tile_keys_adj"), adj_kind=self.info("tile_adj_kind"),
-                    half_list=self.info("half_list"))
+                    half_list=self.info("half_list"),
+                    fused=2 if self.info("fwd_lds") else self.info("fused"))
 
-    def set_tiles(self, pairs_fwd=0, keys_ps=0, pairs_adj=0, keys_adj=0, adj_kind=0):
+    def set_tiles(self, pairs_fwd=0, keys_ps=0, pairs_adj=0, keys_adj=0, adj_kind=0, fused=-1):
         """Set the work partitions of the current precision (see tiles())."""
         check(lib().rgpu_set_tiles(self._p, int(pairs_fwd), int(keys_ps), int(pairs_adj),
-                                   int(keys_adj), int(adj_kind)))
+                                   int(keys_adj), int(adj_kind), int(fused)))
 
     def set_structures(self, atom_offsets, positions, types, cells, pbc, species,
                        centre_mask=None):

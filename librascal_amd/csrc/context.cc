@@ -132,7 +132,7 @@ struct Context {
   bool use_graphs = false;
   hipStream_t own_stream = nullptr;  // created for capture when no stream was given
   hipGraphExec_t rep_exec = nullptr;
-  std::array<uintptr_t, 24> rep_sig{};
+  std::array<uintptr_t, 26> rep_sig{};
   int64_t graph_captures = 0, graph_launches = 0;
   void drop_graph() {
     if (rep_exec) (void)hipGraphExecDestroy(rep_exec);
@@ -604,15 +604,29 @@ void compute_rep(Context& c) {
     reserve_half_list_scan(n, c.half_scan);
     hl.scan_scratch = &c.half_scan;
   }
+  // expansion and invariants in one persistent launch (c kept in LDS
+  // between them) unless the half list or separate launches are selected
+  const bool fused = ps && !hl.rows && c.tile().fused == 1;
   auto cutoff_and_expansion = [&] {
     launch_pair_cutoff(c.rd, n_pairs_dev, cap, c.pl.pair_dist.get(), fc, dfc, c.stream);
-    if (hl.rows)
+    if (fused)
+      launch_expansion_ps<T>(c.rd, S, pair_view(c), cc, km, pp, ph, pl, n2, c.stream, c.tile());
+    else if (!hl.rows && c.tile().fwd_lds == 1)
+      launch_expansion_ps<T>(c.rd, S, pair_view(c), cc, km, nullptr, nullptr, nullptr, nullptr,
+                             c.stream, c.tile());
+    else if (hl.rows)
       launch_expansion_half<T>(c.rd, S, pair_view(c), c.pl.pos.get(), hl, cc, km, c.stream, c.tile());
     else
       launch_expansion<T>(c.rd, S, pair_view(c), cc, km, c.stream, c.tile());
   };
   auto invariants = [&] {
     if (!ps) return;
+    if (fused) {
+      // (the invariants came with the expansion: the selection only)
+      if constexpr (!split)
+        if (xs) launch_sparsify(n, D, c.D(), c.d_sel_col.get(), pp, xs, n2, c.stream);
+      return;
+    }
     if constexpr (split) {
       launch_power_spectrum_split(c.rd, S, n, cc, ph, pl, n2, c.stream, c.tile());
     } else {
@@ -620,7 +634,7 @@ void compute_rep(Context& c) {
       if (xs) launch_sparsify(n, D, c.D(), c.d_sel_col.get(), pp, xs, n2, c.stream);
     }
   };
-  std::array<uintptr_t, 24> sig{};
+  std::array<uintptr_t, 26> sig{};
   bool replay = c.use_graphs && !c.timing && n > 0;
   if (replay) {
     sig = {
@@ -630,7 +644,7 @@ void compute_rep(Context& c) {
         uintptr_t(c.pl.pair_j.get()), uintptr_t(c.pl.pair_vec.get()), uintptr_t(c.pl.pair_dist.get()),
         uintptr_t(c.d_species_of.get()), uintptr_t(c.tile().pb_fwd), uintptr_t(c.tile().kc),
         uintptr_t(c.d_sel_col.get()), uintptr_t(c.tile().half_nl), uintptr_t(hl.src),
-        uintptr_t(hl.rows)};
+        uintptr_t(hl.rows), uintptr_t(c.tile().fused), uintptr_t(c.tile().fwd_lds)};
     // (instantiating a graph allocates: the allocation-free path launches
     // the stage eagerly instead of capturing it)
     if ((!c.rep_exec || sig != c.rep_sig) && g_no_alloc) replay = false;
@@ -1092,6 +1106,27 @@ void autotune_t(Context& c) {
       if (ms < best) best = ms, pick = pb;
     }
     t.pb_fwd = pick;
+    // the expansion writing c once from LDS
+    {
+      SoapTiles reg = t, lds = t;
+      reg.fwd_lds = 0;
+      lds.fwd_lds = 1;
+      const float ms_reg = time_ms([&] { launch_expansion<T>(c.rd, S, pv, cc, km, c.stream, reg); });
+      float ms_lds = 1e30f;
+      for (int pb : {16, 32}) {
+        SoapTiles cand = lds;
+        cand.pb_fwd = pb;
+        try {
+          const float ms = time_ms([&] {
+            launch_expansion_ps<T>(c.rd, S, pv, cc, km, nullptr, nullptr, nullptr, nullptr, c.stream,
+                                   cand);
+          });
+          if (ms < ms_lds) ms_lds = ms, lds = cand;
+        } catch (const InvalidArgument&) {
+        }
+      }
+      if (ms_lds < ms_reg) t = lds;
+    }
     // the expansion over the half neighbour list, where it applies
     bool half_ok = true;
     try {
@@ -1155,6 +1190,27 @@ void autotune_t(Context& c) {
       if (ms < best) best = ms, pick = kc;
     }
     t.kc = pick;
+    // the fused expansion + invariants launch against the two launches
+    {
+      SoapTiles sep = t, fu = t;
+      sep.fused = 0;
+      fu.fused = 1;
+      const float ms_sep = time_ms([&] {
+        launch_expansion<T>(c.rd, S, pv, cc, km, c.stream, sep);
+        if constexpr (split)
+          launch_power_spectrum_split(c.rd, S, n, cc, ph, pl, n2, c.stream, sep);
+        else
+          launch_power_spectrum<T>(c.rd, S, n, cc, pp, n2, c.stream, sep);
+      });
+      float ms_fu = 1e30f;
+      try {
+        ms_fu = time_ms([&] {
+          launch_expansion_ps<T>(c.rd, S, pv, cc, km, pp, ph, pl, n2, c.stream, fu);
+        });
+      } catch (const InvalidArgument&) {
+      }
+      t.fused = ms_fu < ms_sep ? 1 : 0;
+    }
     // adjoint on a zero dL/dX (its time does not depend on the values)
     const int64_t D = std::max(c.D(), c.Dp());
     DeviceBuffer<T> G;
@@ -1274,7 +1330,7 @@ int rgpu_autotune(rgpu_context* ctx) {
 }
 
 int rgpu_set_tiles(rgpu_context* ctx, int pairs_fwd, int keys_ps, int pairs_adj, int keys_adj,
-                   int adj_kind) {
+                   int adj_kind, int fused) {
   return guarded([&] {
     if (!ctx) throw InvalidArgument("null context");
     if (pairs_fwd > 64 || pairs_adj > 64) throw InvalidArgument("at most 64 pairs per batch");
@@ -1286,6 +1342,9 @@ int rgpu_set_tiles(rgpu_context* ctx, int pairs_fwd, int keys_ps, int pairs_adj,
     t.keys_adj = std::max(keys_adj, 0);
     if (adj_kind != 0 && adj_kind != 1) throw InvalidArgument("adj_kind must be 0 or 1");
     t.adj_kind = adj_kind;
+    t.fused = fused < 0 ? -1 : (fused ? 1 : 0);
+    t.fwd_lds = fused == 2 ? 1 : 0;
+    if (fused == 2) t.fused = 0;
     c.have_rep = false;
     c.drop_graph();
   });
@@ -1363,6 +1422,8 @@ int rgpu_get_info(rgpu_context* ctx, int key, int64_t* value) {
       case RGPU_INFO_TILE_KEYS_ADJ: *value = c.tile().keys_adj; break;
       case RGPU_INFO_TILE_ADJ_KIND: *value = c.tile().adj_kind; break;
       case RGPU_INFO_HALF_LIST: *value = c.tile().half_nl; break;
+      case RGPU_INFO_FUSED: *value = c.tile().fused == 1; break;
+      case RGPU_INFO_FWD_LDS: *value = c.tile().fwd_lds; break;
       default: throw InvalidArgument("unknown info key");
     }
   });

@@ -37,6 +37,8 @@ struct SoapTiles {
   int keys_adj = 0;  // key blocks staged per pass in the FP32 adjoint
   int adj_kind = 0;  // FP32 adjoint: 0 vector ALUs, 1 matrix cores where they apply
   int half_nl = 0;   // expansion over the half neighbour list (1) or the full list (0)
+  int fused = -1;    // power spectrum in the expansion's launch (1), separate (0), built-in (-1)
+  int fwd_lds = 0;   // expansion accumulating c in LDS and writing it once (1)
 };
 
 // c[i][s][n][l*l+l+m] (finalised) and the per-centre key mask.
@@ -69,6 +71,16 @@ template <class T>
 void launch_expansion_half(const RadialDev& rad, int S, const PairView& pv, const double* pos,
                            const HalfList& hl, T* c, unsigned long long* keymask, hipStream_t stream,
                            const SoapTiles& tiles = SoapTiles());
+
+// Persistent fused expansion and power spectrum (one launch): c and the
+// key masks as launch_expansion, the invariants as launch_power_spectrum
+// (T double: p) or launch_power_spectrum_split (T float: p_hi / p_lo), and
+// norm2; c stays in LDS between the stages and is written to HBM once.
+// With p and p_hi null only the expansion runs, c written once.
+template <class T>
+void launch_expansion_ps(const RadialDev& rad, int S, const PairView& pv, T* c,
+                         unsigned long long* keymask, T* p, unsigned short* p_hi, unsigned short* p_lo,
+                         double* norm2, hipStream_t stream, const SoapTiles& tiles = SoapTiles());
 
 // p[i][key(a<=b)][n1*N+n2][l] UN-normalised, and norm2[i] = |p_i|^2 (FP64).
 template <class T>

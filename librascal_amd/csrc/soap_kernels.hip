@@ -25,6 +25,8 @@
 
 #include <algorithm>
 #include <string>
+#include <utility>
+#include <vector>
 
 #include "soap_device.hh"
 #include "packing.hh"
@@ -595,31 +597,11 @@ __global__ __launch_bounds__(kThreads, (expansion_min_waves<T, KPT, DIRECT>())) 
 // are never stored).  Work items run l-major, so the lanes of a wavefront
 // share one or two l and loop over the same number of m.
 // Un-normalised; |p|^2 is returned per centre.
-template <class T, bool SPLIT>
-__global__ __launch_bounds__(kThreads) void power_spectrum_kernel(int N, int L, int S, int KC, int64_t Dq,
-                                                                  const T* __restrict__ c,
-                                                                  T* __restrict__ p,
-                                                                  unsigned short* __restrict__ p_hi,
-                                                                  unsigned short* __restrict__ p_lo,
-                                                                  double* __restrict__ norm2) {
-  const int64_t i = blockIdx.x;
-  const int L1 = L + 1, LL = L1 * L1, B = N * LL, LS = LL | 1;
+// key_ab[k] = a << 8 | b of key k; tri[t] = n1 << 8 | n2 of the packed
+// diagonal pair t (packed planes only)
+template <bool SPLIT>
+__device__ void ps_tables(int N, int S, int* key_ab, int* tri) {
   const int K = S * (S + 1) / 2;
-  const int NT = (N + 3) / 4;
-  const int KB = N * N * L1;  // one key block
-  extern __shared__ __align__(64) unsigned char smem_raw[];
-  T* cs = reinterpret_cast<T*>(smem_raw);  // [S][N][LS], then 3 zero rows
-  T* stage = cs + (S * N + 3) * LS;        // [KC][N][N][L1]
-  int* key_ab = reinterpret_cast<int*>(stage + KC * KB);  // [K] a << 8 | b
-  int* tri = key_ab + K;  // [N(N+1)/2] n1 << 8 | n2 of the packed diagonal pairs
-  __shared__ double red[kWaves];
-  const T* ci = c + i * int64_t(S) * B;
-  const FastDiv fLL = fast_div(LL);
-  for (int e = threadIdx.x; e < S * B; e += kThreads) {
-    const int row = e / fLL;
-    cs[row * LS + (e - row * LL)] = ci[e];
-  }
-  for (int e = threadIdx.x; e < 3 * LS; e += kThreads) cs[S * N * LS + e] = T(0);
   for (int k = threadIdx.x; k < K; k += kThreads) {
     int a, b;
     key_species(k, S, &a, &b);
@@ -631,7 +613,23 @@ __global__ __launch_bounds__(kThreads) void power_spectrum_kernel(int N, int L, 
       packed_pair(t, N, &n1, &n2);
       tri[t] = (n1 << 8) | n2;
     }
-  __syncthreads();
+}
+
+// The power spectrum of centre i from its expansion rows in LDS (cs: S N
+// rows of LS = (L+1)^2 | 1 values, then three zero rows; key_ab and tri
+// filled), in KC-key passes over the stage buffer; writes the FP64/FP32
+// rows or the BF16 operand planes and norm2[i].  Every thread calls; ends
+// after a barrier.
+template <class T, bool SPLIT>
+__device__ void ps_body(int N, int L, int S, int KC, int64_t Dq, int64_t i, const T* cs, T* stage,
+                        const int* key_ab, const int* tri, T* __restrict__ p,
+                        unsigned short* __restrict__ p_hi, unsigned short* __restrict__ p_lo,
+                        double* __restrict__ norm2) {
+  const int L1 = L + 1, LS = (L1 * L1) | 1;
+  const int K = S * (S + 1) / 2;
+  const int NT = (N + 3) / 4;
+  const int KB = N * N * L1;  // one key block
+  __shared__ double red[kWaves];
   const int64_t pbase = i * int64_t(K) * KB;
   const FastDiv fNT = fast_div(NT), fL1 = fast_div(L1), fT = fast_div(NT * NT);
   double ss = 0;
@@ -735,10 +733,159 @@ __global__ __launch_bounds__(kThreads) void power_spectrum_kernel(int N, int L, 
     for (int w = 0; w < kWaves; ++w) t += red[w];
     norm2[i] = t;
   }
+  __syncthreads();  // red and the stage may be reused
 }
 
 // out row r = row rows[r] of p (rows null: row r), divided by its norm when
 // normalising
+
+template <class T, bool SPLIT>
+__global__ __launch_bounds__(kThreads) void power_spectrum_kernel(int N, int L, int S, int KC, int64_t Dq,
+                                                                  const T* __restrict__ c,
+                                                                  T* __restrict__ p,
+                                                                  unsigned short* __restrict__ p_hi,
+                                                                  unsigned short* __restrict__ p_lo,
+                                                                  double* __restrict__ norm2) {
+  const int64_t i = blockIdx.x;
+  const int L1 = L + 1, LL = L1 * L1, B = N * LL, LS = LL | 1;
+  const int K = S * (S + 1) / 2;
+  const int KB = N * N * L1;  // one key block
+  extern __shared__ __align__(64) unsigned char smem_raw[];
+  T* cs = reinterpret_cast<T*>(smem_raw);  // [S][N][LS], then 3 zero rows
+  T* stage = cs + (S * N + 3) * LS;        // [KC][N][N][L1]
+  int* key_ab = reinterpret_cast<int*>(stage + KC * KB);  // [K] a << 8 | b
+  int* tri = key_ab + K;  // [N(N+1)/2] n1 << 8 | n2 of the packed diagonal pairs
+  const T* ci = c + i * int64_t(S) * B;
+  const FastDiv fLL = fast_div(LL);
+  for (int e = threadIdx.x; e < S * B; e += kThreads) {
+    const int row = e / fLL;
+    cs[row * LS + (e - row * LL)] = ci[e];
+  }
+  for (int e = threadIdx.x; e < 3 * LS; e += kThreads) cs[S * N * LS + e] = T(0);
+  ps_tables<SPLIT>(N, S, key_ab, tri);
+  __syncthreads();
+  ps_body<T, SPLIT>(N, L, S, KC, Dq, i, cs, stage, key_ab, tri, p, p_hi, p_lo, norm2);
+}
+
+// ---------------------------------------------------------------------------
+// Persistent fused expansion -> power spectrum.  Each workgroup loops over
+// centres i = blockIdx.x, blockIdx.x + gridDim.x, ...  For a centre it
+// accumulates the expansion straight into LDS (the rows the power spectrum
+// reads: S N rows of LS values, the three zero rows after them set once),
+// writes c once to HBM for the adjoint (coalesced), then forms the power
+// spectrum from LDS as power_spectrum_kernel does (ps_body).  The pair
+// batches and the key stage share one LDS region; the first pair batch of
+// the next centre is fetched into registers during the power spectrum.
+// Same arithmetic as expansion_kernel + power_spectrum_kernel.  With PS
+// false only the expansion runs: c accumulated in LDS over all pair
+// batches and written once (expansion_kernel keeps one species block in
+// registers per batch and so stores and reloads c between batches).
+template <class T, int KPT, bool DIRECT, bool SPLIT, bool PS>
+__global__ __launch_bounds__(kThreads, 4) void expansion_ps_kernel(
+    RadialDev rad, int S, int PB, int KC, int64_t Dq, PairView pv, T* __restrict__ cout,
+    unsigned long long* __restrict__ keymask, T* __restrict__ p, unsigned short* __restrict__ p_hi,
+    unsigned short* __restrict__ p_lo, double* __restrict__ norm2) {
+  const int N = rad.N, L = rad.L, L1 = rad.L1, LL = rad.LL, F = N * L1, B = N * LL, LS = LL | 1;
+  const int K = S * (S + 1) / 2;
+  const int tid = threadIdx.x;
+  extern __shared__ __align__(64) unsigned char smem_raw[];
+  T* yc = reinterpret_cast<T*>(smem_raw);
+  T* cs = reinterpret_cast<T*>(smem_raw + ylm_table_bytes<T>(L1));
+  int* key_ab = reinterpret_cast<int*>(cs + (S * N + 3) * LS);
+  int* tri = key_ab + K;
+  unsigned char* region =
+      smem_raw + align_up(size_t(reinterpret_cast<unsigned char*>(tri + N * (N + 1) / 2) - smem_raw), 64);
+  PairBatch<T> pb = carve_batch<T, DIRECT, false>(region, PB, LL, F);
+  T* stage = reinterpret_cast<T*>(region);
+  __shared__ int s_start[kMaxSpecies], s_count[kMaxSpecies];
+  __shared__ unsigned long long kmask;
+  const double inv_s4pi = 0.28209479177387814347;  // 1/sqrt(4 pi)
+  load_ylm_table<T>(rad, yc);
+  if (PS) {
+    for (int e = tid; e < 3 * LS; e += kThreads) cs[S * N * LS + e] = T(0);
+    ps_tables<SPLIT>(N, S, key_ab, tri);
+  }
+
+  const FastDiv fLL = fast_div(LL);
+  int ent[KPT], off[KPT];  // (n*L1 + l) << 16 | lm, and n*LS + lm; -1 past the block
+#pragma unroll
+  for (int k = 0; k < KPT; ++k) {
+    const int e = tid + kThreads * k;
+    ent[k] = -1;
+    off[k] = 0;
+    if (e < B) {
+      const int n = e / fLL, lm = e - n * LL;
+      ent[k] = ((n * L1 + l_of_lm(lm)) << 16) | lm;
+      off[k] = n * LS + lm;
+    }
+  }
+  PairPre pre;
+  int64_t i = blockIdx.x;
+  if (i < pv.n_centres) {
+    const int64_t p0 = pv.centre_off[i] + 1;
+    pair_fetch<DIRECT, false>(rad, pv, p0, max(0, min(PB, int(pv.centre_off[i + 1] - p0))), pre);
+  }
+  for (; i < pv.n_centres; i += gridDim.x) {
+    const int64_t p0 = pv.centre_off[i] + 1;  // the self pair contributes nothing here
+    const int np = int(pv.centre_off[i + 1] - p0);
+    const int si = pv.species_of[i];
+    if (tid == 0) kmask = 1ull << si;
+    for (int q0 = 0; q0 == 0 || q0 < np; q0 += PB) {
+      const int nb = max(0, min(PB, np - q0));
+      if (tid < S) s_count[tid] = 0;
+      stage_pairs<T, DIRECT, false>(rad, yc, pre, nb, pb, s_start, s_count, &kmask);
+      if (q0 + PB < np) pair_fetch<DIRECT, false>(rad, pv, p0 + q0 + PB, min(PB, np - q0 - PB), pre);
+      for (int s = 0; s < S; ++s) {
+        const int cnt = s_count[s];
+        if (q0 > 0 && cnt == 0) continue;
+        const int st = s_start[s];
+        T* cb = cs + s * N * LS;
+        T acc[KPT];
+#pragma unroll
+        for (int k = 0; k < KPT; ++k) {
+          acc[k] = T(0);
+          if (ent[k] >= 0) {
+            if (q0 > 0)
+              acc[k] = cb[off[k]];
+            else if (s == si && (ent[k] & 0xffff) == 0)
+              acc[k] = T(rad.center[(rad.sp_tab ? rad.sp_tab[si] * rad.N : 0) +
+                                    (tid + kThreads * k) / fLL] * inv_s4pi);
+          }
+        }
+        for (int t = 0; t < cnt; ++t) {
+          const int q = pb.lst[st + t];
+          const T* Yq = pb.Y + q * LL;
+          const T* Rq = pb.R + q * F;
+#pragma unroll
+          for (int k = 0; k < KPT; ++k)
+            if (ent[k] >= 0) acc[k] += Rq[ent[k] >> 16] * Yq[ent[k] & 0xffff];
+        }
+#pragma unroll
+        for (int k = 0; k < KPT; ++k)
+          if (ent[k] >= 0) cb[off[k]] = acc[k];
+      }
+      __syncthreads();
+    }
+    // c for the adjoint, written once
+    T* ci = cout + i * int64_t(S) * B;
+    for (int e = tid; e < S * B; e += kThreads) {
+      const int row = e / fLL;
+      ci[e] = cs[row * LS + (e - row * LL)];
+    }
+    if (tid == 0) keymask[i] = kmask;
+    if (PS)
+      ps_body<T, SPLIT>(N, L, S, KC, Dq, i, cs, stage, key_ab, tri, p, p_hi, p_lo, norm2);
+    else
+      __syncthreads();  // c is read out before the next centre overwrites it
+    // the next centre's first pair batch
+    const int64_t in = i + gridDim.x;
+    if (in < pv.n_centres) {
+      const int64_t q0n = pv.centre_off[in] + 1;
+      pair_fetch<DIRECT, false>(rad, pv, q0n, max(0, min(PB, int(pv.centre_off[in + 1] - q0n))), pre);
+    }
+  }
+}
+
 template <class T>
 __global__ void normalise_kernel(int64_t n, int64_t D, const T* __restrict__ p,
                                  const double* __restrict__ norm2, int normalize,
@@ -2313,6 +2460,93 @@ void launch_power_spectrum_impl(const RadialDev& rad, int S, int64_t n, const T*
   }
   RGPU_LAUNCH_CHECK();
 }
+
+int device_cu_count() {
+  static int cached[64] = {0};
+  int dev = 0;
+  RGPU_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= 64) throw RuntimeError("device index out of range");
+  if (!cached[dev]) {
+    int v = 0;
+    RGPU_HIP(hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev));
+    cached[dev] = v;
+  }
+  return cached[dev];
+}
+
+template <class T, int KPT, bool DIRECT, bool SPLIT, bool PS>
+void launch_fused_t(const RadialDev& rad, int S, const PairView& pv, T* c, unsigned long long* keymask,
+                    T* p, unsigned short* p_hi, unsigned short* p_lo, double* norm2, hipStream_t stream,
+                    const SoapTiles& tiles) {
+  const int N = rad.N, L1 = rad.L1, LL = rad.LL, F = N * L1, K = S * (S + 1) / 2;
+  const size_t yt = ylm_table_bytes<T>(L1);
+  const size_t cbytes = sizeof(T) * (size_t(S) * N + 3) * (LL | 1);
+  const size_t fixed = align_up(yt + cbytes + sizeof(int) * size_t(K + N * (N + 1) / 2), 64);
+  const size_t kb = sizeof(T) * size_t(N) * N * L1;
+  int PB = tiles.pb_fwd > 0 ? std::min(tiles.pb_fwd, 64) : 16;
+  int KC = tiles.kc > 0 ? std::min(tiles.kc, K) : 2;
+  const size_t region = std::max(pair_batch_bytes(PB, LL, F, sizeof(T), DIRECT, false), PS ? KC * kb : 0);
+  const size_t smem = fixed + region;
+  auto kern = expansion_ps_kernel<T, KPT, DIRECT, SPLIT, PS>;
+  allow_lds(kern, smem);
+  // persistent: as many workgroups as fit at once (cached per launch shape)
+  static std::vector<std::pair<std::pair<const void*, size_t>, int>> occ_cache;
+  int per_cu = 0;
+  for (auto& e : occ_cache)
+    if (e.first.first == reinterpret_cast<const void*>(kern) && e.first.second == smem) per_cu = e.second;
+  if (!per_cu) {
+    RGPU_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kThreads, smem));
+    per_cu = std::max(per_cu, 1);
+    occ_cache.push_back({{reinterpret_cast<const void*>(kern), smem}, per_cu});
+  }
+  const int64_t grid = std::min<int64_t>(pv.n_centres, int64_t(per_cu) * device_cu_count());
+  hipLaunchKernelGGL(kern, dim3(unsigned(grid)), dim3(kThreads), smem, stream, rad, S, PB, KC,
+                     packed_row_stride(S, N, L1), pv, c, keymask, p, p_hi, p_lo, norm2);
+  RGPU_LAUNCH_CHECK();
+}
+
+template <class T, int KPT>
+void launch_fused_k(const RadialDev& rad, int S, const PairView& pv, T* c, unsigned long long* keymask,
+                    T* p, unsigned short* p_hi, unsigned short* p_lo, double* norm2, hipStream_t stream,
+                    const SoapTiles& tiles, bool ps) {
+  constexpr bool split = std::is_same<T, float>::value;
+  if (ps) {
+    if (rad.use_spline)
+      launch_fused_t<T, KPT, false, split, true>(rad, S, pv, c, keymask, p, p_hi, p_lo, norm2, stream, tiles);
+    else
+      launch_fused_t<T, KPT, true, split, true>(rad, S, pv, c, keymask, p, p_hi, p_lo, norm2, stream, tiles);
+  } else {
+    if (rad.use_spline)
+      launch_fused_t<T, KPT, false, split, false>(rad, S, pv, c, keymask, p, p_hi, p_lo, norm2, stream, tiles);
+    else
+      launch_fused_t<T, KPT, true, split, false>(rad, S, pv, c, keymask, p, p_hi, p_lo, norm2, stream, tiles);
+  }
+}
+
+template <class T>
+void launch_expansion_ps(const RadialDev& rad, int S, const PairView& pv, T* c,
+                         unsigned long long* keymask, T* p, unsigned short* p_hi, unsigned short* p_lo,
+                         double* norm2, hipStream_t stream, const SoapTiles& tiles) {
+  if (pv.n_centres == 0) return;
+  if (S > kMaxSpecies) throw InvalidArgument("at most 64 species");
+  const int kpt = (rad.N * rad.LL + kThreads - 1) / kThreads;
+  const bool ps = p || p_hi;
+  if (kpt <= 2)
+    launch_fused_k<T, 2>(rad, S, pv, c, keymask, p, p_hi, p_lo, norm2, stream, tiles, ps);
+  else if (kpt <= 5)
+    launch_fused_k<T, 5>(rad, S, pv, c, keymask, p, p_hi, p_lo, norm2, stream, tiles, ps);
+  else if (kpt <= 8)
+    launch_fused_k<T, 8>(rad, S, pv, c, keymask, p, p_hi, p_lo, norm2, stream, tiles, ps);
+  else
+    throw InvalidArgument("max_radial x (max_angular+1)^2 too large for the fused kernel");
+}
+
+template void launch_expansion_ps<double>(const RadialDev&, int, const PairView&, double*,
+                                          unsigned long long*, double*, unsigned short*,
+                                          unsigned short*, double*, hipStream_t, const SoapTiles&);
+template void launch_expansion_ps<float>(const RadialDev&, int, const PairView&, float*,
+                                         unsigned long long*, float*, unsigned short*, unsigned short*,
+                                         double*, hipStream_t, const SoapTiles&);
 
 template <class T>
 void launch_power_spectrum(const RadialDev& rad, int S, int64_t n, const T* c, T* p, double* norm2,

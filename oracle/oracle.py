@@ -91,7 +91,7 @@ def lib():
         L.orc_gap_knm.argtypes = [p, C.POINTER(Batch), C.POINTER(Sparse), i, p]
         L.orc_gap_kmm.argtypes = [C.POINTER(Sparse), i, p]
         L.orc_gap_knm_derivative.argtypes = [p, C.POINTER(Batch), C.POINTER(Sparse), i, p]
-        L.orc_gap_predict.argtypes = [p, C.POINTER(Batch), C.POINTER(Sparse), p, p, p, p, i64]
+        L.orc_gap_predict.argtypes = [p, C.POINTER(Batch), C.POINTER(Sparse), p, p, p, p, i64, i64]
         _lib = L
     return _lib
 
@@ -362,11 +362,13 @@ def gap_knm_derivative(calc, batch, sparse, neg_stress=False):
 
 
 def gap_predict(calc, batch, sparse, weights, energy=True, gradient=True,
-                neg_stress=True, max_centres=-1):
+                neg_stress=True, max_centres=-1, first_centre=0):
+    """Energies, dE/dr and the negative stress of the centres [first_centre,
+    first_centre + max_centres) of the batch (all centres by default)."""
     w = np.ascontiguousarray(np.asarray(weights, np.float64)[sparse.order])
     e = np.zeros(batch.n_struct) if energy else None
     g = np.zeros((batch.n_atoms, 3)) if gradient else None
     s = np.zeros((batch.n_struct, 6)) if neg_stress else None
     _check(lib().orc_gap_predict(calc._p, batch.c, sparse.c, _ptr(w), _ptr(e), _ptr(g),
-                                 _ptr(s), int(max_centres)))
+                                 _ptr(s), int(max_centres), int(first_centre)))
     return e, g, s

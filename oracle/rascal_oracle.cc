@@ -1247,12 +1247,12 @@ struct CentreState {
 
 template <class Fn>
 void sweep(Calc& C, const orc_batch* b, bool want_ps, bool want_grad,
-           int64_t max_centres, Fn fn) {
+           int64_t max_centres, Fn fn, int64_t first_centre = 0) {
   const int S = b->n_species;
   const size_t blk = size_t(C.N) * C.LL;
   const size_t D = size_t(n_keys(S)) * C.N * C.N * C.L1;
   if (S > 64) throw std::runtime_error("at most 64 species");
-  int64_t done = 0;
+  int64_t done = 0, seen = 0;
   for (int s = 0; s < b->n_struct; ++s) {
     StructView sv = view(b, s);
     check_structure(sv.n, sv.pos, sv.types, sv.cell);
@@ -1264,6 +1264,7 @@ void sweep(Calc& C, const orc_batch* b, bool want_ps, bool want_grad,
     if (want_ps) st.ps.resize(D);
     for (int i = 0; i < sv.n; ++i) {
       if (max_centres >= 0 && done >= max_centres) return;
+      if (seen++ < first_centre) continue;
       size_t np = size_t(P.off[i + 1] - P.off[i]);
       if (want_grad) st.dc.resize(np * 3 * S * blk);
       expand_centre(C, b, sv, i, P, st.c.data(), &st.mask,
@@ -1581,7 +1582,7 @@ int orc_gap_knm_derivative(void* calc, const orc_batch* b, const orc_sparse* sp,
 // models/sparse_kernel_predict.hh:70-377
 int orc_gap_predict(void* calc, const orc_batch* b, const orc_sparse* sp,
                     const double* w, double* energy, double* gradient,
-                    double* neg_stress, int64_t max_centres) {
+                    double* neg_stress, int64_t max_centres, int64_t first_centre) {
   return guarded([&] {
     Calc& C = *static_cast<Calc*>(calc);
     const size_t D = size_t(n_keys(b->n_species)) * C.N * C.N * C.L1;
@@ -1631,7 +1632,8 @@ int orc_gap_predict(void* calc, const orc_batch* b, const orc_sparse* sp,
                 }
               }
             }
-          });
+          },
+          first_centre);
     if (neg_stress)
       for (int s = 0; s < b->n_struct; ++s) {
         double vol = volume(b->cells + 9 * s);

@@ -145,11 +145,12 @@ int orc_gap_knm_derivative(void* calc, const orc_batch* b,
                            const orc_sparse* sp, int neg_stress, double* out);
 /* prediction: energy[n_struct] = K_NM . w (no baseline), gradient
  * [n_atoms][3] = dE/dr, neg_stress [n_struct][6] (Voigt, / volume).  Any
- * output may be NULL.  max_centres limits the work to the first centres of the
- * batch (timing subsets; -1 = all). */
+ * output may be NULL.  The work covers max_centres centres (-1 = all) from
+ * the centre with ordinal first_centre in the batch (timing subsets and
+ * parallel workers). */
 int orc_gap_predict(void* calc, const orc_batch* b, const orc_sparse* sp,
                     const double* weights, double* energy, double* gradient,
-                    double* neg_stress, int64_t max_centres);
+                    double* neg_stress, int64_t max_centres, int64_t first_centre);
 
 #ifdef __cplusplus
 }

@@ -380,3 +380,34 @@ def test_prediction_outputs_to_device_memory():
     _capi.check(_capi.lib().rgpu_gap_predict(ctx._p, _capi.ptr(eh), _capi.ptr(gd), _capi.ptr(sd)))
     torch.cuda.synchronize()
     assert close(eh, e) and close(gd.cpu().numpy(), g) and close(sd.cpu().numpy(), s)
+
+
+@pytest.mark.parametrize("spline", [None, 1e-8])
+@pytest.mark.parametrize("N,L", [(12, 9), (5, 4), (8, 6)])
+def test_fused_expansion_invariants_equal_two_launches(N, L, spline):
+    """The persistent fused expansion + power spectrum launch (c kept in LDS
+    between the stages), and the expansion writing c once from LDS (fused
+    2), against the separate kernels: expansion, key masks
+    and invariants equal in FP64 (to rounding) and in the mixed path, on
+    ragged molecules with 4 species (more centres than resident
+    workgroups for the larger shapes)."""
+    sts = molecules(n_mol=64, seed=9)
+    hp = soap_hypers(rc=4.0, sw=0.5, sigma=0.4, N=N, L=L, spline=spline)
+    for prec, tol in ((_capi.FP64, 1e-13), (_capi.MIXED, 1e-6)):
+        ctx, sp = make_ctx(hp, sts)
+        ctx.set_precision(prec)
+        out = {}
+        for fused in (0, 1, 2):
+            ctx.set_tiles(fused=fused)
+            assert ctx.tiles()["fused"] == fused
+            ctx.compute()
+            c, km = ctx.expansion()
+            X, inv = ctx.features()
+            out[fused] = (c, km, X, inv)
+        c0, km0, X0, inv0 = out[0]
+        for variant in (1, 2):
+            c1, km1, X1, inv1 = out[variant]
+            assert np.array_equal(km0, km1)
+            assert np.abs(c1 - c0).max() <= tol * np.abs(c0).max()
+            assert np.abs(X1 - X0).max() <= tol * np.abs(X0).max()
+            assert np.abs(inv1 - inv0).max() <= tol * np.abs(inv0).max()

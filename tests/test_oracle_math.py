@@ -401,3 +401,22 @@ def test_gap_knm_derivative_columns_finite_difference():
                     vals.append(O.gap_knm(calc, bb, ss, "Structure")[0, m])
                 fd[3 * k + c] = (vals[0] - vals[1]) / (2 * h)
         assert np.max(np.abs(fd - kd[:, m])) < 1e-6 * max(1e-12, np.abs(kd[:, m]).max())
+
+
+def test_gap_predict_centre_ranges_add_up():
+    """The oracle's prediction over consecutive centre ranges (the CPU
+    baseline's parallel workers) sums to the prediction over all centres."""
+    from librascal_amd.synthetic import periodic_box
+    sts = periodic_box(40, 0.05, (1, 8), d_min=1.0, seed=3)
+    hp = soap_hypers(rc=3.5, N=4, L=3, sigma=0.5)
+    calc = O.Calculator(hp)
+    b = O.BatchData(sts)
+    X, _ = calc.power_spectrum(b)
+    rng = np.random.default_rng(2)
+    sp = O.SparseSet(b.types[:6], X[:6] + 0.01 * rng.normal(size=X[:6].shape), 2)
+    w = rng.normal(size=6)
+    e, g, s = O.gap_predict(calc, b, sp, w)
+    parts = [O.gap_predict(calc, b, sp, w, first_centre=a, max_centres=15) for a in (0, 15, 30)]
+    assert np.allclose(sum(p[0] for p in parts), e, rtol=1e-12, atol=0)
+    assert np.allclose(sum(p[1] for p in parts), g, rtol=1e-12, atol=1e-14)
+    assert np.allclose(sum(p[2] for p in parts), s, rtol=1e-12, atol=1e-14)
