@@ -161,40 +161,66 @@ def main():
     value = n_centres * args.steps / elm
     value64 = n_centres * steps64 / el64
 
-    # roofline of the dominant kernel: the K = (X T^T)^zeta GEMM.  The mixed
-    # path multiplies the packed invariants (diagonal species blocks stored
-    # once, n1 <= n2): Dp columns instead of D, the same products.
-    M_s = w["n_sparse_per_species"]
-    rows = [int(np.sum(num == t)) for t in species]
+    # roofline of the dominant kernel, the adjoint (dL/dX -> dL/dc ->
+    # per-pair forces and virial): HBM bound in principle (it streams dL/dX,
+    # the invariant planes and c once), vector-ALU issue bound in practice
     S_, N_, L1_ = len(species), w["N"], w["L"] + 1
     Dp = S_ * N_ * (N_ + 1) // 2 * L1_ + (S_ * (S_ + 1) // 2 - S_) * N_ * N_ * L1_
-    # one grouped launch covers every species block: 2 * rows_s * M_s * Dp flop each
-    flop_gemm = sum(2.0 * r * M_s * Dp for r in rows)
-    k_ms, k_n = stm["gap_kernel"]
-    k_avg_ms = k_ms / max(k_n, 1)
-    achieved = flop_gemm / (k_avg_ms * 1e-3)
-    peak = PEAK_BF16 / 3.0  # three BF16 products per algorithmic multiply-add
-    traffic = profiled_traffic(GEMM_KERNEL)
-    roofline = dict(bound="mfma", achieved=achieved / 1e12, peak=peak / 1e12, unit="TFLOP/s",
-                    frac=achieved / peak, traffic=traffic,
-                    kernel="gap_kernel (%s: K = (X T^T)^zeta, all species blocks in one launch)"
-                    % GEMM_KERNEL,
-                    avg_launch_ms=k_avg_ms, flop_per_launch=flop_gemm, packed_columns=Dp,
-                    bf16_mfma_utilisation=3.0 * achieved / PEAK_BF16,
-                    traffic_source=TRAFFIC_FILE if traffic is not None else None)
-    # HBM-bound stages: algorithmic bytes per step / stage time (FP32 path)
+    Dq = (Dp + 31) // 32 * 32
     n_p = n_pairs
     c_bytes = 4.0 * n_centres * S_ * N_ * L1_ * L1_
-    p_bytes = 4.0 * n_centres * Dp  # packed invariants (hi + lo planes) or dL/dX
-    pair_bytes = 56.0 * n_p
-    stage_bytes = dict(expansion=c_bytes + pair_bytes, power_spectrum=c_bytes + p_bytes,
-                       backward=2 * p_bytes + c_bytes + pair_bytes)
-    hbm = {}
-    for k, b in stage_bytes.items():
-        ms = stm[k][0] / max(args.steps, 1)
-        hbm[k] = dict(bytes=b, ms=ms, achieved_GBps=b / (ms * 1e-3) / 1e9,
-                      frac=b / (ms * 1e-3) / PEAK_HBM)
-    roofline["hbm_stages"] = hbm
+    g_bytes = 4.0 * n_centres * Dq                     # dL/dX, FP32 packed rows
+    p_bytes = 2.0 * 2.0 * n_centres * Dq               # invariants, BF16 hi + lo planes
+    pair_bytes = 56.0 * n_p                            # vector, distance, fc, dfc, j, species
+    out_bytes = 24.0 * n_p + 48.0 * n_centres          # force updates, centre virials
+    adj_kind = tilesm.get("adj_kind", 0)
+    adj_kernel = ADJOINT_KERNELS[adj_kind]
+    adj_bytes = g_bytes + (0.0 if adj_kind == 1 else p_bytes) + c_bytes + pair_bytes + out_bytes
+    a_ms, a_n = stm["backward"]
+    a_avg_ms = a_ms / max(a_n, 1)
+    a_achieved = adj_bytes / (a_avg_ms * 1e-3)
+    valu = profiled_counter(SQ_FILE, adj_kernel, "SQ_INSTS_VALU")
+    valu_issue = None
+    if valu is not None:
+        # a wave64 VALU instruction occupies its SIMD 4 cycles; 1024 SIMDs
+        busy_s = valu * 4.0 / (1024 * CLOCK_HZ)
+        valu_issue = dict(insts_valu_per_launch=valu, issue_ms=busy_s * 1e3,
+                          frac_of_launch=busy_s / (a_avg_ms * 1e-3), clock_hz=CLOCK_HZ,
+                          source=SQ_FILE)
+    roofline = dict(bound="hbm", achieved=a_achieved / 1e9, peak=PEAK_HBM / 1e9, unit="GB/s",
+                    frac=a_achieved / PEAK_HBM, traffic=profiled_traffic(adj_kernel),
+                    kernel="backward (%s: dL/dX -> dL/dc -> pair forces and virials)" % adj_kernel,
+                    avg_launch_ms=a_avg_ms, bytes_per_launch=adj_bytes,
+                    bytes_per_centre=dict(dL_dX=g_bytes / n_centres,
+                                          invariant_planes=(0.0 if adj_kind == 1 else p_bytes)
+                                          / n_centres,
+                                          expansion=c_bytes / n_centres,
+                                          pair_records=pair_bytes / n_centres,
+                                          outputs=out_bytes / n_centres),
+                    valu_issue=valu_issue,
+                    traffic_source=TRAFFIC_FILE)
+    # the other stages against their own rooflines
+    M_s = w["n_sparse_per_species"]
+    rows = [int(np.sum(num == t)) for t in species]
+    flop_gemm = sum(2.0 * r * M_s * Dp for r in rows)  # packed columns, all species blocks
+    peak_mfma = PEAK_BF16 / 3.0  # three BF16 products per algorithmic multiply-add
+    secondary = {}
+    for stage, kern in (("gap_kernel", GEMM_KERNEL), ("gap_force", FORCE_GEMM_KERNEL)):
+        ms, nl = stm[stage]
+        avg = ms / max(nl, 1)
+        ach = flop_gemm / (avg * 1e-3)
+        secondary[stage] = dict(bound="mfma", kernel=kern, achieved=ach / 1e12,
+                                peak=peak_mfma / 1e12, unit="TFLOP/s", frac=ach / peak_mfma,
+                                avg_launch_ms=avg, flop_per_launch=flop_gemm,
+                                traffic=profiled_traffic(kern))
+    stage_bytes = dict(expansion=(c_bytes + pair_bytes, EXPANSION_KERNEL),
+                       power_spectrum=(c_bytes + p_bytes, PS_KERNEL))
+    for stage, (b, kern) in stage_bytes.items():
+        ms = stm[stage][0] / max(args.steps, 1)
+        secondary[stage] = dict(bound="hbm", kernel=kern, bytes=b, avg_launch_ms=ms,
+                                achieved=b / (ms * 1e-3) / 1e9, unit="GB/s",
+                                frac=b / (ms * 1e-3) / PEAK_HBM, traffic=profiled_traffic(kern))
+    roofline["secondary"] = secondary
 
     stage_ms = {k: round(v[0] / args.steps, 4) for k, v in stm.items()}
     stage_ms64 = {k: round(v[0] / steps64, 4) for k, v in st64.items()}
@@ -224,20 +250,40 @@ def main():
 
 
 TRAFFIC_FILE = "profiles/hbm_traffic_periodic_box_8192.md"
-# the energy GEMM's kernel (the packed layout always takes the direct-to-LDS
-# variant)
+SQ_FILE = "profiles/sq_counters_periodic_box_8192.md"
+# kernels of the headline step (the packed layout always takes the
+# direct-to-LDS GEMMs; the adjoint kernel is the autotuner's choice)
 GEMM_KERNEL = "bf16x3_gemm_dma<0, 128, 2>"
+FORCE_GEMM_KERNEL = "bf16x3_gemm_dma<1, 128, 2>"
+EXPANSION_KERNEL = "expansion_kernel<float, 5, false>"
+PS_KERNEL = "power_spectrum_kernel<float, true>"
+ADJOINT_KERNELS = {0: "adjoint_staged_kernel<false, 5>", 1: "adjoint_mfma_kernel<false, 12>"}
+# shader clock under this load (SQ_BUSY_CYCLES over the kernel durations)
+CLOCK_HZ = 2.0e9
+
+
+def _profile_rows(path):
+    path = os.path.join(ROOT, path)
+    if not os.path.exists(path):
+        return None, []
+    header, rows = None, []
+    for line in open(path):
+        if not line.startswith("|"):
+            continue
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if header is None:
+            header = cells
+        elif not set("".join(cells)) <= set("-: "):
+            rows.append(cells)
+    return header, rows
 
 
 def profiled_traffic(kernel):
     """HBM bytes per launch of `kernel` (FETCH_SIZE + WRITE_SIZE, scaled by the
     copy-kernel calibration) from the committed rocprofv3 --pmc summary, or
     None when it is absent."""
-    path = os.path.join(ROOT, TRAFFIC_FILE)
-    if not os.path.exists(path):
-        return None
-    for line in open(path):
-        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+    header, rows = _profile_rows(TRAFFIC_FILE)
+    for cells in rows:
         if len(cells) >= 5 and cells[0].strip("`") == kernel:
             try:
                 return float(cells[3]) + float(cells[4])
@@ -246,10 +292,64 @@ def profiled_traffic(kernel):
     return None
 
 
-def cpu_baseline(structures, hp, sp_species, sp_values, weights, zeta, n_centres):
+def profiled_counter(path, kernel, counter):
+    """Per-launch value of an SQ counter of `kernel` (summed over its
+    wavefronts) from a committed rocprofv3 --pmc summary, or None."""
+    header, rows = _profile_rows(path)
+    if not header or counter not in header:
+        return None
+    k = header.index(counter)
+    for cells in rows:
+        if cells[0].strip("`") == kernel and len(cells) > k:
+            try:
+                return float(cells[k].split()[0])
+            except ValueError:
+                return None
+    return None
+
+
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+# worker state of the all-cores baseline (one oracle calculator per process)
+_W = {}
+
+
+def _worker_init(structures, hp, sp_species, sp_path, weights, zeta):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    _W["O"] = O
+    _W["calc"] = O.Calculator(hp)
+    _W["batch"] = O.BatchData(structures)
+    _W["sparse"] = O.SparseSet(sp_species, np.load(sp_path, mmap_mode="r"), zeta)
+    _W["w"] = weights
+
+
+def _worker_run(rng):
+    first, count = rng
+    O = _W["O"]
+    t = time.perf_counter()
+    e, g, s = O.gap_predict(_W["calc"], _W["batch"], _W["sparse"], _W["w"], first_centre=first,
+                            max_centres=count)
+    return e, g, s, time.perf_counter() - t
+
+
+def cpu_baseline(structures, hp, sp_species, sp_values, weights, zeta, n_centres, per_worker=240):
     """The FP64 CPU oracle (serial restatement of the reference algorithm,
-    stored per-pair feature gradients) timed on one host core over the first
-    n_centres environments of the same box."""
+    stored per-pair feature gradients), energy + forces + virial of
+    environments of the same box: on one host core over the first
+    n_centres environments, and on all the cores given to this process
+    (one oracle process per core, consecutive ranges of per_worker
+    environments each)."""
+    import multiprocessing as mp
+    import tempfile
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     calc = O.Calculator(hp)
@@ -257,10 +357,45 @@ def cpu_baseline(structures, hp, sp_species, sp_values, weights, zeta, n_centres
     ss = O.SparseSet(sp_species, sp_values, zeta)
     t = time.perf_counter()
     O.gap_predict(calc, b, ss, weights, max_centres=n_centres)
-    el = time.perf_counter() - t
-    return dict(value=n_centres / el, unit="envs/s", cores=1, kind="port",
-                sample="first %d of %d environments of the benchmark box, energy + forces + "
-                       "virial, FP64" % (n_centres, b.n_atoms), seconds=el)
+    el1 = time.perf_counter() - t
+    single = dict(value=n_centres / el1, cores=1, centres=n_centres, seconds=el1)
+    nproc = os.cpu_count() or 1
+    try:
+        allowed = len(os.sched_getaffinity(0))
+    except AttributeError:
+        allowed = nproc
+    # this process's share of the host: the cores it may run on, at most 16
+    # (the CPU share of one GPU on the benchmark boxes)
+    workers = max(1, min(allowed, 16))
+    multi = None
+    tmp = tempfile.mkdtemp(prefix="rascal_cpu_")
+    try:
+        sp_path = os.path.join(tmp, "sparse.npy")
+        np.save(sp_path, np.ascontiguousarray(sp_values, np.float64))
+        ranges = [(k * per_worker, per_worker) for k in range(workers)]
+        ctx = mp.get_context("spawn")
+        with ctx.Pool(workers, initializer=_worker_init,
+                      initargs=(structures, hp, np.asarray(sp_species), sp_path,
+                                np.asarray(weights), zeta)) as pool:
+            pool.map(_worker_run, [(0, 1)] * workers)  # every worker set up
+            t = time.perf_counter()
+            res = pool.map(_worker_run, ranges, chunksize=1)
+            elw = time.perf_counter() - t
+        total = per_worker * workers
+        multi = dict(value=total / elw, cores=workers, centres=total, seconds=elw,
+                     max_worker_seconds=max(r[3] for r in res))
+    finally:
+        for f in os.listdir(tmp):
+            os.remove(os.path.join(tmp, f))
+        os.rmdir(tmp)
+    head = multi if multi is not None else single
+    return dict(value=head["value"], unit="envs/s", cores=head["cores"], kind="port",
+                sample="%d environments of the benchmark box (%d of %d per process, consecutive "
+                       "ranges) on %d cores, energy + forces + virial, FP64; one core: the first %d"
+                       % (head["centres"], per_worker if multi else n_centres, b.n_atoms,
+                          head["cores"], n_centres),
+                seconds=head["seconds"], single_core=single, all_cores=multi, nproc=nproc,
+                cpus_allowed=allowed, cpu_model=_cpu_model())
 
 
 if __name__ == "__main__":
