@@ -161,9 +161,8 @@ def main():
     value = n_centres * args.steps / elm
     value64 = n_centres * steps64 / el64
 
-    # roofline of the dominant kernel, the adjoint (dL/dX -> dL/dc ->
-    # per-pair forces and virial): HBM bound in principle (it streams dL/dX,
-    # the invariant planes and c once), vector-ALU issue bound in practice
+    # rooflines of the step's stages from their HIP-event times; the stage
+    # with the longest time per step leads, the others are "secondary"
     S_, N_, L1_ = len(species), w["N"], w["L"] + 1
     Dp = S_ * N_ * (N_ + 1) // 2 * L1_ + (S_ * (S_ + 1) // 2 - S_) * N_ * N_ * L1_
     Dq = (Dp + 31) // 32 * 32
@@ -174,53 +173,63 @@ def main():
     pair_bytes = 56.0 * n_p                            # vector, distance, fc, dfc, j, species
     out_bytes = 24.0 * n_p + 48.0 * n_centres          # force updates, centre virials
     adj_kind = tilesm.get("adj_kind", 0)
-    adj_kernel = ADJOINT_KERNELS[adj_kind]
-    adj_bytes = g_bytes + (0.0 if adj_kind == 1 else p_bytes) + c_bytes + pair_bytes + out_bytes
-    a_ms, a_n = stm["backward"]
-    a_avg_ms = a_ms / max(a_n, 1)
-    a_achieved = adj_bytes / (a_avg_ms * 1e-3)
-    valu = profiled_counter(SQ_FILE, adj_kernel, "SQ_INSTS_VALU")
-    valu_issue = None
-    if valu is not None:
+    adj_kernels = ADJOINT_KERNELS[adj_kind]
+    # the staged kernel (kind 0) reads the invariants; kind 3 passes dL/dc
+    # (the size of c) through HBM between its two kernels
+    p_read = p_bytes if adj_kind == 0 else 0.0
+    dc_trip = 2.0 * c_bytes if adj_kind == 3 else 0.0
+    adj_bytes = g_bytes + p_read + c_bytes + dc_trip + pair_bytes + out_bytes
+
+    def per_launch(stage):
+        ms, nl = stm[stage]
+        return ms / max(nl, 1)
+
+    def valu_issue(kernels, avg_ms):
+        vals = [profiled_counter(SQ_FILE, k, "SQ_INSTS_VALU") for k in kernels]
+        if any(v is None for v in vals):
+            return None
         # a wave64 VALU instruction occupies its SIMD 4 cycles; 1024 SIMDs
-        busy_s = valu * 4.0 / (1024 * CLOCK_HZ)
-        valu_issue = dict(insts_valu_per_launch=valu, issue_ms=busy_s * 1e3,
-                          frac_of_launch=busy_s / (a_avg_ms * 1e-3), clock_hz=CLOCK_HZ,
-                          source=SQ_FILE)
-    roofline = dict(bound="hbm", achieved=a_achieved / 1e9, peak=PEAK_HBM / 1e9, unit="GB/s",
-                    frac=a_achieved / PEAK_HBM, traffic=profiled_traffic(adj_kernel),
-                    kernel="backward (%s: dL/dX -> dL/dc -> pair forces and virials)" % adj_kernel,
-                    avg_launch_ms=a_avg_ms, bytes_per_launch=adj_bytes,
-                    bytes_per_centre=dict(dL_dX=g_bytes / n_centres,
-                                          invariant_planes=(0.0 if adj_kind == 1 else p_bytes)
-                                          / n_centres,
-                                          expansion=c_bytes / n_centres,
-                                          pair_records=pair_bytes / n_centres,
-                                          outputs=out_bytes / n_centres),
-                    valu_issue=valu_issue,
-                    traffic_source=TRAFFIC_FILE)
-    # the other stages against their own rooflines
+        busy_s = sum(vals) * 4.0 / (1024 * CLOCK_HZ)
+        return dict(insts_valu_per_launch=sum(vals), issue_ms=busy_s * 1e3,
+                    frac_of_launch=busy_s / (avg_ms * 1e-3), clock_hz=CLOCK_HZ, source=SQ_FILE)
+
+    def traffic_of(kernels):
+        t = [profiled_traffic(k) for k in kernels]
+        return None if any(v is None for v in t) else sum(t)
+
+    lines = {}
+    a_avg = per_launch("backward")
+    lines["backward"] = dict(
+        bound="hbm", achieved=adj_bytes / (a_avg * 1e-3) / 1e9, peak=PEAK_HBM / 1e9, unit="GB/s",
+        frac=adj_bytes / (a_avg * 1e-3) / PEAK_HBM, traffic=traffic_of(adj_kernels),
+        kernel="adjoint (%s: dL/dX -> dL/dc -> pair forces and virials)" % " + ".join(adj_kernels),
+        avg_launch_ms=a_avg, bytes_per_launch=adj_bytes,
+        bytes_per_centre=dict(dL_dX=g_bytes / n_centres, invariant_planes=p_read / n_centres,
+                              expansion=c_bytes / n_centres, dL_dc_round_trip=dc_trip / n_centres,
+                              pair_records=pair_bytes / n_centres, outputs=out_bytes / n_centres),
+        valu_issue=valu_issue(adj_kernels, a_avg), traffic_source=TRAFFIC_FILE)
     M_s = w["n_sparse_per_species"]
     rows = [int(np.sum(num == t)) for t in species]
     flop_gemm = sum(2.0 * r * M_s * Dp for r in rows)  # packed columns, all species blocks
     peak_mfma = PEAK_BF16 / 3.0  # three BF16 products per algorithmic multiply-add
-    secondary = {}
     for stage, kern in (("gap_kernel", GEMM_KERNEL), ("gap_force", FORCE_GEMM_KERNEL)):
-        ms, nl = stm[stage]
-        avg = ms / max(nl, 1)
+        avg = per_launch(stage)
         ach = flop_gemm / (avg * 1e-3)
-        secondary[stage] = dict(bound="mfma", kernel=kern, achieved=ach / 1e12,
-                                peak=peak_mfma / 1e12, unit="TFLOP/s", frac=ach / peak_mfma,
-                                avg_launch_ms=avg, flop_per_launch=flop_gemm,
-                                traffic=profiled_traffic(kern))
+        lines[stage] = dict(bound="mfma", kernel=kern, achieved=ach / 1e12, peak=peak_mfma / 1e12,
+                            unit="TFLOP/s", frac=ach / peak_mfma, avg_launch_ms=avg,
+                            flop_per_launch=flop_gemm, traffic=profiled_traffic(kern),
+                            traffic_source=TRAFFIC_FILE)
     stage_bytes = dict(expansion=(c_bytes + pair_bytes, EXPANSION_KERNEL),
                        power_spectrum=(c_bytes + p_bytes, PS_KERNEL))
     for stage, (b, kern) in stage_bytes.items():
-        ms = stm[stage][0] / max(args.steps, 1)
-        secondary[stage] = dict(bound="hbm", kernel=kern, bytes=b, avg_launch_ms=ms,
-                                achieved=b / (ms * 1e-3) / 1e9, unit="GB/s",
-                                frac=b / (ms * 1e-3) / PEAK_HBM, traffic=profiled_traffic(kern))
-    roofline["secondary"] = secondary
+        avg = per_launch(stage)
+        lines[stage] = dict(bound="hbm", kernel=kern, bytes_per_launch=b, avg_launch_ms=avg,
+                            achieved=b / (avg * 1e-3) / 1e9, peak=PEAK_HBM / 1e9, unit="GB/s",
+                            frac=b / (avg * 1e-3) / PEAK_HBM, traffic=profiled_traffic(kern),
+                            traffic_source=TRAFFIC_FILE)
+    lead = max(lines, key=lambda k: lines[k]["avg_launch_ms"])
+    roofline = dict(lines[lead], stage=lead)
+    roofline["secondary"] = {k: v for k, v in lines.items() if k != lead}
 
     stage_ms = {k: round(v[0] / args.steps, 4) for k, v in stm.items()}
     stage_ms64 = {k: round(v[0] / steps64, 4) for k, v in st64.items()}
@@ -257,7 +266,9 @@ GEMM_KERNEL = "bf16x3_gemm_dma<0, 128, 2>"
 FORCE_GEMM_KERNEL = "bf16x3_gemm_dma<1, 128, 2>"
 EXPANSION_KERNEL = "expansion_kernel<float, 5, false>"
 PS_KERNEL = "power_spectrum_kernel<float, true>"
-ADJOINT_KERNELS = {0: "adjoint_staged_kernel<false, 5>", 1: "adjoint_mfma_kernel<false, 12>"}
+ADJOINT_KERNELS = {0: ["adjoint_staged_kernel<false, 5>"], 1: ["adjoint_mfma_kernel<false, 12>"],
+                   2: ["adjoint_tile_kernel<12, 9, 6, 1, 10, false, true>"],
+                   3: ["adjoint_tile_kernel<12, 9, 6, 1, 6, false, false>", "adjoint_pair_kernel<12, 9>"]}
 # shader clock under this load (SQ_BUSY_CYCLES over the kernel durations)
 CLOCK_HZ = 2.0e9
 

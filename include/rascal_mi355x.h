@@ -109,8 +109,11 @@ int rgpu_enable_graphs(rgpu_context* ctx, int enable);
 int rgpu_autotune(rgpu_context* ctx);
 /* Sets the work partitions of the current precision directly (values <= 0:
  * built-in choice; keys_adj: key blocks of dL/dX staged in LDS per pass of
- * the mixed-precision adjoint; adj_kind: its vector-ALU (0) or matrix-core
- * (1) kernel; fused: expansion and invariants in one persistent launch (1),
+ * the mixed-precision adjoint; adj_kind: its staged vector-ALU kernel (0),
+ * matrix-core kernel (1), register-tile kernel (2), or register-tile key
+ * contraction followed by a pair kernel of its own (3, the default; 2 and 3
+ * are built for max_radial 12 / max_angular 9 and 8 / 6 and up to 8
+ * species, kind 0 runs elsewhere); fused: expansion and invariants in one persistent launch (1),
  * two launches (0), two launches with the expansion writing c once from
  * LDS (2), built-in (-1: two launches)), e.g. to reuse an autotuned
  * set.  Results agree between partitions to rounding.  No counterpart in

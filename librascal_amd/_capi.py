@@ -179,7 +179,7 @@ class Context:
                     half_list=self.info("half_list"),
                     fused=2 if self.info("fwd_lds") else self.info("fused"))
 
-    def set_tiles(self, pairs_fwd=0, keys_ps=0, pairs_adj=0, keys_adj=0, adj_kind=0, fused=-1):
+    def set_tiles(self, pairs_fwd=0, keys_ps=0, pairs_adj=0, keys_adj=0, adj_kind=3, fused=-1):
         """Set the work partitions of the current precision (see tiles())."""
         check(lib().rgpu_set_tiles(self._p, int(pairs_fwd), int(keys_ps), int(pairs_adj),
                                    int(keys_adj), int(adj_kind), int(fused)))

@@ -84,6 +84,7 @@ struct Context {
   DeviceBuffer<double> ps64;         // selected invariants [n_atoms][n_sel] (un-normalised)
   DeviceBuffer<double> gfull;        // dL/dX scattered into the full layout
   DeviceBuffer<double> dcp, dcs;     // stored expansion derivatives (gradient export)
+  DeviceBuffer<float> adj_work;      // dL/dc between the adjoint's two kernels (adj_kind 3)
   // half neighbour list (rgpu_set_half_list) and its contribution rows
   DeviceBuffer<int> half_count, half_j;
   DeviceBuffer<int64_t> half_off, half_src, half_scan;
@@ -829,10 +830,14 @@ void backward_device(Context& c, const T* G, double* grad, double* cvir, const d
       G = gf;
     }
   }
-  if constexpr (std::is_same<T, float>::value)
+  if constexpr (std::is_same<T, float>::value) {
+    float* work = nullptr;
+    if (c.tile().adj_kind == 3)
+      work = c.adj_work.reserve(size_t(std::max<int64_t>(c.n_atoms * S * c.rd.N * c.rd.LL, 1)));
     launch_backward_split(c.rd, S, pair_view(c), c.c32.get(), c.p_hi.get(), c.p_lo.get(),
                           c.norm2.get(), c.hp.normalize, xg, xg_scale, G, grad, cvir, c.stream,
-                          c.tile());
+                          c.tile(), work);
+  }
   else
     launch_backward<T>(c.rd, S, pair_view(c), cbuf<T>(c).get(), pbuf<T>(c).get(), c.norm2.get(),
                        c.hp.normalize, xg, xg_scale, G, grad, cvir, c.stream, c.tile());
@@ -1220,12 +1225,14 @@ void autotune_t(Context& c) {
     double* grad = gr.reserve(size_t(3 * n));
     double* cvir = cv.reserve(size_t(6 * n));
     RGPU_HIP(hipMemsetAsync(grad, 0, sizeof(double) * 3 * n, c.stream));
+    float* work = split ? c.adj_work.reserve(size_t(std::max<int64_t>(n * S * c.rd.N * c.rd.LL, 1)))
+                        : nullptr;
     auto time_adjoint = [&](const SoapTiles& cand) {
       try {
         return time_ms([&] {
           if constexpr (split)
             launch_backward_split(c.rd, S, pv, cc, ph, pl, n2, c.hp.normalize, nullptr, 1.0, g,
-                                  grad, cvir, c.stream, cand);
+                                  grad, cvir, c.stream, cand, cand.adj_kind == 3 ? work : nullptr);
           else
             launch_backward<T>(c.rd, S, pv, cc, pp, n2, c.hp.normalize, nullptr, 1.0, g, grad,
                                cvir, c.stream, cand);
@@ -1237,7 +1244,9 @@ void autotune_t(Context& c) {
     // key blocks per pass of the FP32 adjoint (the built-in choice fits an
     // LDS budget), then pairs per batch
     if (split) {
+      // the staged kernel (adj_kind 0) with its key chunks
       SoapTiles bestv = t;
+      bestv.adj_kind = 0;
       bestv.pb_adj = 0;
       bestv.keys_adj = 0;
       best = time_adjoint(bestv);
@@ -1260,17 +1269,31 @@ void autotune_t(Context& c) {
         const float ms = time_adjoint(cand);
         if (ms < best) best = ms, bestv = cand;
       }
+      // the register-tile kernels (adj_kind 2, and 3 with the pair phase
+      // as a kernel of its own): their chunks of key rounds
+      for (int kind : {2, 3})
+        for (int kch : {0, S + (S + 1) / 2, S}) {
+          SoapTiles cand = t;
+          cand.adj_kind = kind;
+          cand.pb_adj = 0;
+          cand.keys_adj = kch;
+          const float ms = time_adjoint(cand);
+          if (ms < best) best = ms, bestv = cand;
+        }
       t.keys_adj = bestv.keys_adj;
       t.adj_kind = bestv.adj_kind;
     }
+    // pairs per batch of the pair phase in the adjoint's workgroup (0: the
+    // most that fit; adj_kind 3 has a kernel of its own for it)
     best = 1e30f;
     pick = 0;
-    for (int pb : {2, 4, 6, 8, 12, 16, 24, 32}) {
-      SoapTiles cand = t;
-      cand.pb_adj = pb;
-      const float ms = time_adjoint(cand);
-      if (ms < best) best = ms, pick = pb;
-    }
+    if (t.adj_kind != 3)
+      for (int pb : {0, 2, 4, 6, 8, 12, 16, 24, 32}) {
+        SoapTiles cand = t;
+        cand.pb_adj = pb;
+        const float ms = time_adjoint(cand);
+        if (ms < best) best = ms, pick = pb;
+      }
     t.pb_adj = pick;
     RGPU_HIP(hipStreamSynchronize(c.stream));
   }
@@ -1340,7 +1363,7 @@ int rgpu_set_tiles(rgpu_context* ctx, int pairs_fwd, int keys_ps, int pairs_adj,
     t.kc = std::max(keys_ps, 0);
     t.pb_adj = std::max(pairs_adj, 0);
     t.keys_adj = std::max(keys_adj, 0);
-    if (adj_kind != 0 && adj_kind != 1) throw InvalidArgument("adj_kind must be 0 or 1");
+    if (adj_kind < 0 || adj_kind > 3) throw InvalidArgument("adj_kind must be 0, 1, 2 or 3");
     t.adj_kind = adj_kind;
     t.fused = fused < 0 ? -1 : (fused ? 1 : 0);
     t.fwd_lds = fused == 2 ? 1 : 0;

@@ -35,7 +35,11 @@ struct SoapTiles {
   int kc = 0;      // key blocks staged per pass in the power spectrum
   int pb_adj = 0;  // pairs staged per batch in the adjoint (<= 64)
   int keys_adj = 0;  // key blocks staged per pass in the FP32 adjoint
-  int adj_kind = 0;  // FP32 adjoint: 0 vector ALUs, 1 matrix cores where they apply
+  int adj_kind = 3;  // FP32 adjoint where it applies: 0 staged vector ALUs, 1 matrix cores,
+                     // 2 register tiles (keys_adj: key blocks per staged chunk of rounds),
+                     // 3 register tiles with the pair phase as a kernel of its own
+                     //   (spline radial functions; else as 2).  Kinds 2 and 3 fall back
+                     //   to 0 for sizes they are not built for.
   int half_nl = 0;   // expansion over the half neighbour list (1) or the full list (0)
   int fused = -1;    // power spectrum in the expansion's launch (1), separate (0), built-in (-1)
   int fwd_lds = 0;   // expansion accumulating c in LDS and writing it once (1)
@@ -152,12 +156,14 @@ void launch_pack_gradient(int S, int N, int L1, int64_t n, const double* G, floa
                           hipStream_t stream);
 
 // Adjoint of the mixed path: dL/dX (G) and p in the packed layout, p as its
-// BF16 operand planes (launch_power_spectrum_split)
+// BF16 operand planes (launch_power_spectrum_split).  work: room for
+// n_centres S (L+1)^2 N floats (dL/dc between the key contraction and the
+// pair kernel of adj_kind 3), or null.
 void launch_backward_split(const RadialDev& rad, int S, const PairView& pv, const float* c,
                            const unsigned short* p_hi, const unsigned short* p_lo,
                            const double* norm2, bool normalize, const double* xg, double xg_scale,
                            const float* G, double* grad, double* centre_virial, hipStream_t stream,
-                           const SoapTiles& tiles = SoapTiles());
+                           const SoapTiles& tiles = SoapTiles(), float* work = nullptr);
 
 // scale[i] = zeta d[i*ld]^(zeta-1) for the centres of species s, else 0
 void launch_knm_gradient_scale(int64_t n, const double* d, int64_t ld, const int* species_of, int s,

@@ -1735,6 +1735,663 @@ __global__ __launch_bounds__(kThreads, 2) void adjoint_mfma_kernel(
   adjoint_pairs<float, DIRECT>(rad, S, PB, pv, yc, dc, region, i, grad, centre_virial);
 }
 
+// ---------------------------------------------------------------------------
+// Adjoint of the mixed path on register tiles, for compile-time N and L.
+//
+// Same mathematics as adjoint_mfma_kernel: per angular channel l,
+//   dL/dC_l = alpha f_l W_l C_l - 2 beta f_l^2 C_l (C_l^T C_l),
+// with W_l assembled from the packed dL/dX blocks; the invariants are not
+// read.  Work split: wavefront w takes the rows of species a = w, w + 4, ...;
+// lane (tile, l, quad) owns an NR x 4 register tile of dL/dC_l (radial rows
+// tile*NR .. tile*NR+NR-1, columns m = 4 quad .. 4 quad + 3 of channel l), so
+// one (b, n2) step of W_l C_l is a 16-byte read of c_b[n2][l, m..m+3], NR
+// reads of W and 4 NR FMAs, every LDS offset of the step a compile-time
+// immediate.  The key blocks pass through LDS in chunks of whole rounds
+// (AdjRounds): each species has at most one block per round, so the
+// wavefronts get equal work in every chunk.  The packed values of the next
+// chunk are loaded into registers while the current chunk is contracted.
+//
+// LDS: Y_lm table | c [S][N][MT] (the m of each l padded to whole quads, pads
+// zero) | C_l^T C_l as [l][m'][4 ceil((2l+1)/4)] | staged blocks in the full
+// [n1][n2][l] layout (a diagonal block unpacked symmetric, its diagonal
+// entries times sqrt 2; the common sqrt 2 of all blocks applied to the
+// result), later reused by the pair batches.  dL/dc [S][LL][N] goes over c.
+
+// quads of a padded c row before channel l: sum_{l' < l} ceil((2l'+1)/4)
+__host__ __device__ constexpr int adj_quads_before(int l) { return l + (l - 1) * (l - 1) / 4; }
+// entries of the C_l^T C_l table before channel l
+__host__ __device__ constexpr int adj_gram_before(int l) {
+  int s = 0;
+  for (int t = 0; t < l; ++t) s += (2 * t + 1) * 4 * ((t >> 1) + 1);
+  return s;
+}
+
+
+// Schedule of the key blocks (a, b), a <= b, in rounds (circle method).
+// Round 0 holds the diagonal blocks, block k = (k, k).  With n = S rounded up
+// to even (species n-1 a bye when S is odd) and r' = r - 1, round r >= 1
+// pairs (r', n-1) as its block k = 0 and ((r'+k) mod (n-1), (r'-k) mod (n-1))
+// as block k for 1 <= k < n/2; a bye's block is dropped, so the blocks of a
+// round are k = k0 .. n/2 - 1, k0 = S mod 2.  Every unordered pair of species
+// meets in exactly one round.
+struct AdjRounds {
+  int S, n, k0, per;  // per: blocks of a round r >= 1
+  __host__ __device__ explicit AdjRounds(int s) : S(s), n(s + (s & 1)), k0(s & 1), per((s + (s & 1)) / 2 - (s & 1)) {}
+  __host__ __device__ int rounds() const { return n; }
+  __host__ __device__ int blocks(int r) const { return r == 0 ? S : per; }
+  __host__ __device__ void pair(int r, int k, int* a, int* b) const {
+    const int m = n - 1, rr = r - 1;
+    int x = rr, y = m;
+    if (k > 0) {
+      x = rr + k;
+      if (x >= m) x -= m;
+      y = rr - k;
+      if (y < 0) y += m;
+    }
+    *a = x < y ? x : y;
+    *b = x < y ? y : x;
+  }
+  // block index of species a in round r (-1: a bye)
+  __host__ __device__ int block_of(int r, int a) const {
+    if (r == 0) return a;
+    const int m = n - 1, rr = r - 1;
+    int k = 0;
+    if (a != m && a != rr) {
+      int d = a - rr;
+      if (d < 0) d += m;
+      k = d < m - d ? d : m - d;
+    }
+    return k < k0 ? -1 : k - k0;
+  }
+  __host__ __device__ int partner(int r, int a) const {
+    if (r == 0) return a;
+    const int m = n - 1, rr = r - 1;
+    int b;
+    if (a == m)
+      b = rr;
+    else if (a == rr)
+      b = m;
+    else {
+      b = (2 * rr - a) % m;
+      if (b < 0) b += m;
+    }
+    return b < S ? b : -1;
+  }
+  // end (exclusive) of the chunk of rounds that starts at r0: rounds are
+  // added while the chunk holds at most max_blocks blocks and at most
+  // pre * kThreads 16-byte values (one round at least)
+  __host__ __device__ int chunk_end(int r0, int max_blocks, int KD4, int KB4, int pre) const {
+    int r = r0, nb = 0, n4 = 0;
+    while (r < rounds()) {
+      const int rb = blocks(r), r4 = rb * (r == 0 ? KD4 : KB4);
+      if (r > r0 && (nb + rb > max_blocks || n4 + r4 > pre * kThreads)) break;
+      nb += rb;
+      n4 += r4;
+      ++r;
+    }
+    return r;
+  }
+};
+
+// One (b, n2) sweep of a register tile: acc[r] += W[row r][n2] c_b[n2][m..m+3]
+// for the NR rows of the tile (row stride RS, K stride KS in the staged block)
+template <int N, int NR, int MT, int RS, int KS>
+__device__ __forceinline__ void adj_tile_sweep(const float* __restrict__ g, const float* __restrict__ cb,
+                                               f32x4 (&acc)[NR]) {
+#pragma unroll
+  for (int n2 = 0; n2 < N; ++n2) {
+    const f32x4 cv = *reinterpret_cast<const f32x4*>(cb + n2 * MT);
+#pragma unroll
+    for (int r = 0; r < NR; ++r) acc[r] += g[r * RS + n2 * KS] * cv;
+  }
+}
+
+// PRE: 16-byte packed values per thread of one staged chunk.  PAIRS: the
+// pair phase follows in the same workgroup (dL/dc in LDS); otherwise dL/dc
+// [S][LL][N] of centre i is written to dcg + i S LL N for adjoint_pair_kernel
+// and the workgroup needs no Y_lm table or pair batch (three per CU).
+template <int N, int L, int NR, int ROLES, int PRE, bool DIRECT, bool PAIRS>
+__global__ __launch_bounds__(kThreads, (PAIRS || ROLES > 1) ? 2 : 3) void adjoint_tile_kernel(
+    RadialDev rad, int S, int PB, int max_blocks, PairView pv, const float* __restrict__ c,
+    const double* __restrict__ norm2, int normalize, const double* __restrict__ xg, double xg_scale,
+    const float* __restrict__ G, double* __restrict__ grad, double* __restrict__ centre_virial,
+    float* __restrict__ dcg) {
+  constexpr int L1 = L + 1, LL = L1 * L1;
+  constexpr int KB = N * N * L1, KD = N * (N + 1) / 2 * L1, KB4 = KB / 4, KD4 = KD / 4;
+  constexpr int Q = adj_quads_before(L1), MT = 4 * Q, TILES = N / NR;
+  constexpr int GRAM = adj_gram_before(L1);
+  constexpr int CPT = (kWaves * ROLES * N * Q + kThreads - 1) / kThreads;  // c quads per thread
+  static_assert(N % NR == 0 && TILES * Q <= 64, "one lane per register tile");
+  static_assert(KB % 4 == 0 && KD % 4 == 0, "key blocks in whole 16-byte values");
+  const float rt2 = 1.41421356237309504880f;
+  const int64_t i = blockIdx.x;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  extern __shared__ __align__(64) unsigned char smem_raw[];
+  float* yc = reinterpret_cast<float*>(smem_raw);
+  float* c4 = reinterpret_cast<float*>(smem_raw + (PAIRS ? ylm_table_bytes<float>(L1) : 0));
+  float* gram = reinterpret_cast<float*>(reinterpret_cast<unsigned char*>(c4) +
+                                         align_up(sizeof(float) * S * N * MT, 64));
+  unsigned char* region = reinterpret_cast<unsigned char*>(gram) + align_up(sizeof(float) * GRAM, 64);
+  float* gl = reinterpret_cast<float*>(region);
+  __shared__ int tri[N * (N + 1) / 2];  // n1 << 8 | n2 of the packed diagonal pairs
+  __shared__ double red[kWaves];
+  __shared__ double sh_dot;
+
+  // this lane's register tile
+  const bool lane_ok = lane < TILES * Q;
+  const int ln = lane_ok ? lane : 0;
+  const int tile = ln / Q, qq = ln - tile * Q;
+  int l = 0;
+#pragma unroll
+  for (int t = 1; t <= L; ++t) l += qq >= adj_quads_before(t) ? 1 : 0;
+  const int mq = qq - adj_quads_before(l);
+  const int cq = 4 * qq;                 // first column of the quad in a padded c row
+  const int gp = 4 * ((l >> 1) + 1);     // row length of C_l^T C_l
+  int go = 0;                            // its first entry
+#pragma unroll
+  for (int t = 0; t < L; ++t) go += t < l ? (2 * t + 1) * 4 * ((t >> 1) + 1) : 0;
+
+  const AdjRounds R(S);
+  const FastDiv fper = fast_div(R.per > 0 ? R.per : 1);
+  const int64_t Dq = packed_row_stride(S, N, L1);
+  const float* Gi = G + i * Dq;
+
+  // packed values of the chunk of rounds [r0, r1) held in registers.  The
+  // blocks are numbered in round order (block j < S: the diagonal block
+  // (j, j); then round r >= 1 from S + (r-1) per), and a chunk stages its
+  // blocks in that order, so block j of a chunk starting at block jb takes
+  // slot j - jb.  blk_src[j]: first packed column of block j.
+  __shared__ int blk_src[2 * kWaves * (2 * kWaves + 1) / 2];
+  auto first_block = [&](int r) { return r == 0 ? 0 : S + (r - 1) * R.per; };
+  f32x4 pre[PRE];
+  int info[PRE];  // slot * KB4 + q (bit 30: diagonal block), -1 none
+  auto fetch = [&](int r0, int r1) {
+    const int jb = first_block(r0);
+    const int nd = r0 == 0 ? S : 0, nd4 = nd * KD4;
+    const int tot = nd4 + (first_block(r1) - jb - nd) * KB4;
+#pragma unroll
+    for (int k = 0; k < PRE; ++k) {
+      const int u = tid + kThreads * k;
+      info[k] = -1;
+      if (u < tot) {
+        int j, q;
+        if (u < nd4) {
+          j = u / KD4;
+          q = u - j * KD4;
+        } else {
+          const int v = u - nd4;
+          j = v / KB4;
+          q = v - j * KB4;
+          j += nd;
+        }
+        info[k] = (u < nd4 ? (1 << 30) : 0) | (j * KB4 + q);
+        pre[k] = *reinterpret_cast<const f32x4*>(Gi + blk_src[jb + j] + 4 * q);
+      }
+    }
+  };
+  auto put = [&]() {
+#pragma unroll
+    for (int k = 0; k < PRE; ++k) {
+      const int d = info[k];
+      if (d < 0) continue;
+      if (!(d & (1 << 30))) {
+        *reinterpret_cast<f32x4*>(gl + 4 * d) = pre[k];
+      } else {
+        const int e = d & ((1 << 30) - 1), slot = e / KB4, q = e - slot * KB4;
+        float* w = gl + slot * KB;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int o = 4 * q + v, t = o / L1, lo = o - t * L1;
+          const int n1 = tri[t] >> 8, n2 = tri[t] & 255;
+          if (n1 == n2) {
+            w[(n1 * N + n1) * L1 + lo] = rt2 * pre[k][v];
+          } else {
+            w[(n1 * N + n2) * L1 + lo] = pre[k][v];
+            w[(n2 * N + n1) * L1 + lo] = pre[k][v];
+          }
+        }
+      }
+    }
+  };
+
+  // c into the padded rows, one quad (row, l, 4 m) per step; the loads of c
+  // and of the first chunk are in flight together
+  const int tot_q = S * N * Q;
+  const float* ci = c + i * int64_t(S) * N * LL;
+  f32x4 cv[CPT];
+#pragma unroll
+  for (int k = 0; k < CPT; ++k) {
+    const int u = tid + kThreads * k;
+    cv[k] = f32x4{0.f, 0.f, 0.f, 0.f};
+    if (u < tot_q) {
+      const int row = u / Q, qd = u - row * Q;
+      int lc = 0;
+#pragma unroll
+      for (int t = 1; t <= L; ++t) lc += qd >= adj_quads_before(t) ? 1 : 0;
+      const int m0 = 4 * (qd - adj_quads_before(lc));
+      const float* src = ci + row * LL + lc * lc + m0;
+#pragma unroll
+      for (int v = 0; v < 4; ++v)
+        if (m0 + v < 2 * lc + 1) cv[k][v] = src[v];
+    }
+  }
+  if (PAIRS) load_ylm_table<float>(rad, yc);
+  for (int t = tid; t < N * (N + 1) / 2; t += kThreads) {
+    int n1, n2;
+    packed_pair(t, N, &n1, &n2);
+    tri[t] = (n1 << 8) | n2;
+  }
+  for (int j = tid; j < first_block(R.rounds()); j += kThreads) {
+    int a = j, b = j;
+    if (j >= S) {
+      const int jj = j - S, rr = jj / fper;
+      R.pair(1 + rr, jj - rr * R.per + R.k0, &a, &b);
+    }
+    blk_src[j] = int(packed_key_offset(a, b, S, N, L1));
+  }
+  __syncthreads();  // the block table in place
+  int r0 = 0, r1 = R.chunk_end(0, max_blocks, KD4, KB4, PRE);
+  fetch(r0, r1);
+#pragma unroll
+  for (int k = 0; k < CPT; ++k) {
+    const int u = tid + kThreads * k;
+    if (u < tot_q) *reinterpret_cast<f32x4*>(c4 + 4 * u) = cv[k];
+  }
+  __syncthreads();  // c and tri in place
+
+  // C_l^T C_l: unit (l, m', quad) of the table
+  for (int u = tid; u < GRAM / 4; u += kThreads) {
+    int lg = 0;
+#pragma unroll
+    for (int t = 1; t <= L; ++t) lg += u >= adj_gram_before(t) / 4 ? 1 : 0;
+    int g0 = 0;
+#pragma unroll
+    for (int t = 0; t < L; ++t) g0 += t < lg ? (2 * t + 1) * ((t >> 1) + 1) : 0;
+    const int qc = (lg >> 1) + 1, rem = u - g0, m2 = rem / qc, q2 = rem - m2 * qc;
+    const float* cc = c4 + 4 * adj_quads_before(lg);
+    f32x4 s = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll 4
+    for (int row = 0; row < S * N; ++row) {
+      const float* cr = cc + row * MT;
+      s += cr[m2] * *reinterpret_cast<const f32x4*>(cr + 4 * q2);
+    }
+    *reinterpret_cast<f32x4*>(gram + 4 * u) = s;
+  }
+  put();
+  int r2 = r1 < R.rounds() ? R.chunk_end(r1, max_blocks, KD4, KB4, PRE) : r1;
+  if (r1 < r2) fetch(r1, r2);
+  __syncthreads();  // the table and the first chunk in place
+
+  f32x4 accG[ROLES][NR], accP[ROLES][NR];
+#pragma unroll
+  for (int ro = 0; ro < ROLES; ++ro)
+#pragma unroll
+    for (int r = 0; r < NR; ++r) accG[ro][r] = accP[ro][r] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+  // C_l (C_l^T C_l)
+#pragma unroll
+  for (int ro = 0; ro < ROLES; ++ro) {
+    const int a = wave + kWaves * ro;
+    if (a >= S) continue;
+    const float* ca = c4 + (a * N + tile * NR) * MT + 4 * adj_quads_before(l);
+    const float* mb = gram + go + 4 * mq;
+    for (int m2 = 0; m2 < 2 * l + 1; ++m2) {
+      const f32x4 mv = *reinterpret_cast<const f32x4*>(mb + m2 * gp);
+#pragma unroll
+      for (int r = 0; r < NR; ++r) accP[ro][r] += ca[r * MT + m2] * mv;
+    }
+  }
+
+  // W_l C_l over the chunks
+  while (true) {
+    const int rm0 = r0 == 0 ? 1 : r0, sd = r0 == 0 ? S : 0;
+    for (int r = r0; r < r1; ++r) {
+#pragma unroll
+      for (int ro = 0; ro < ROLES; ++ro) {
+        const int a = wave + kWaves * ro;
+        if (a >= S) continue;
+        const int k = R.block_of(r, a);
+        if (k < 0) continue;
+        const int b = R.partner(r, a);
+        const int slot = r == 0 ? k : sd + (r - rm0) * R.per + k;
+        const float* gb = gl + slot * KB + l;
+        const float* cb = c4 + b * N * MT + cq;
+        if (a <= b)
+          adj_tile_sweep<N, NR, MT, N * L1, L1>(gb + tile * NR * N * L1, cb, accG[ro]);
+        else
+          adj_tile_sweep<N, NR, MT, L1, N * L1>(gb + tile * NR * L1, cb, accG[ro]);
+      }
+    }
+    if (r1 >= R.rounds()) break;
+    __syncthreads();  // every wavefront is done with this chunk
+    put();
+    r0 = r1;
+    r1 = r2;
+    r2 = r1 < R.rounds() ? R.chunk_end(r1, max_blocks, KD4, KB4, PRE) : r1;
+    if (r1 < r2) fetch(r1, r2);
+    __syncthreads();
+  }
+
+  // normalisation: alpha = 1/|p|, beta = (X . G)/|p|^2
+  double alpha = 1.0, beta = 0.0;
+  if (normalize) {
+    const double n2 = norm2[i];
+    double dotxg;
+    if (xg) {
+      dotxg = xg_scale * xg[i];
+    } else {
+      // p . G = 1/2 sum_l f_l <W_l C_l, C_l>, W_l C_l = sqrt 2 accG
+      double d = 0;
+      if (lane_ok) {
+        const double fl = 1.0 / sqrt(double(2 * l + 1));
+#pragma unroll
+        for (int ro = 0; ro < ROLES; ++ro) {
+          const int a = wave + kWaves * ro;
+          if (a >= S) continue;
+#pragma unroll
+          for (int r = 0; r < NR; ++r) {
+            const f32x4 cc = *reinterpret_cast<const f32x4*>(c4 + (a * N + tile * NR + r) * MT + cq);
+#pragma unroll
+            for (int v = 0; v < 4; ++v) d += fl * double(accG[ro][r][v]) * double(cc[v]);
+          }
+        }
+      }
+      d *= 0.5 * 1.41421356237309504880;
+      for (int off = 32; off > 0; off >>= 1) d += __shfl_down(d, off, 64);
+      if (lane == 0) red[wave] = d;
+      __syncthreads();
+      if (tid == 0) {
+        double t = 0;
+        for (int w = 0; w < kWaves; ++w) t += red[w];
+        sh_dot = t;
+      }
+      __syncthreads();
+      dotxg = n2 > 0 ? sh_dot / sqrt(n2) : 0.0;
+    }
+    if (n2 > 0) {
+      alpha = 1.0 / sqrt(n2);
+      beta = dotxg / n2;
+    }
+  }
+  __syncthreads();  // every read of c and of the staged blocks is done
+  float* dc = c4;   // [S][LL][N]
+  if (lane_ok) {
+    const float fl = 1.0f / sqrtf(float(2 * l + 1));
+    const float sg = float(alpha) * rt2 * fl, sp = float(-2.0 * beta) * fl * fl;
+#pragma unroll
+    for (int ro = 0; ro < ROLES; ++ro) {
+      const int a = wave + kWaves * ro;
+      if (a >= S) continue;
+#pragma unroll
+      for (int r = 0; r < NR; ++r)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int m = 4 * mq + v;
+          if (m < 2 * l + 1)
+            dc[(a * LL + l * l + m) * N + tile * NR + r] = sg * accG[ro][r][v] + sp * accP[ro][r][v];
+        }
+    }
+  }
+  __syncthreads();
+  if constexpr (PAIRS) {
+    adjoint_pairs<float, DIRECT>(rad, S, PB, pv, yc, dc, region, i, grad, centre_virial);
+  } else {
+    (void)PB;
+    (void)grad;
+    (void)centre_virial;
+    static_assert((LL * N) % 4 == 0, "dL/dc rows in whole 16-byte values");
+    const int E4 = S * LL * N / 4;
+    f32x4* dst = reinterpret_cast<f32x4*>(dcg + i * int64_t(S) * LL * N);
+    for (int e = tid; e < E4; e += kThreads) dst[e] = reinterpret_cast<const f32x4*>(dc)[e];
+  }
+}
+
+// Real spherical harmonics Y_l^{+-m} (l = m..L) of one unit direction on the
+// lane of order m (0 <= m <= L), written to Y[(l*l + l +- m) * ys]: the
+// recursion of ylm_wave without the gradient (no cross-lane operations).
+__device__ __forceinline__ void ylm_lane(int L, const float* __restrict__ yc, float x, float y, float z, int m,
+                                         float* Y, int ys) {
+  const int L1 = L + 1;
+  const float* dfac = yc + 4 * L1 * L1;
+  const float* s2l1 = dfac + L1;
+  float n2 = x * x + y * y + z * z;
+  if (fabs(double(n2) - 1.0) > 1e-6) {
+    const float inv = 1.0f / sqrtf(n2);
+    x *= inv;
+    y *= inv;
+    z *= inv;
+  }
+  const float cos_t = z;
+  const float sxy = sqrtf(x * x + y * y);
+  float cos_p = 1.0f, sin_p = 0.0f;
+  if (double(sxy) >= kDblFtol) {
+    cos_p = x / sxy;
+    sin_p = y / sxy;
+  }
+  const float sin_t = sqrtf(1.0f - cos_t * cos_t);
+  float c_m = 1.0f, s_m = 0.0f, c_m1 = cos_p, s_m1 = sin_p;
+  for (int k = 1; k <= m; ++k) {
+    const float cn = 2.0f * cos_p * c_m1 - c_m;
+    const float sn = 2.0f * cos_p * s_m1 - s_m;
+    c_m = c_m1;
+    s_m = s_m1;
+    c_m1 = cn;
+    s_m1 = sn;
+  }
+  const float s2pi = 0.39894228040143267794f;
+  float diag = s2pi;
+  for (int k = 1; k <= m; ++k) diag = diag * sin_t * dfac[k];
+  const float isq2 = 0.70710678118654752440f;
+  float P1 = 0.0f, P2 = 0.0f;
+  for (int l = m; l <= L; ++l) {
+    float P;
+    if (m == l) {
+      P = diag;
+    } else if (m == l - 1) {
+      P = (l == 1) ? cos_t * 1.73205080756887729353f * s2pi : diag * cos_t * s2l1[l];
+    } else {
+      const Vec4<float>::type c4 = reinterpret_cast<const Vec4<float>::type*>(yc)[l * L1 + m];
+      float t = cos_t * P1;
+      t += c4.y * P2;
+      t *= c4.x;
+      P = t;
+    }
+    const int base = l * l + l;
+    if (m == 0) {
+      Y[base * ys] = P * isq2;
+    } else {
+      Y[(base + m) * ys] = P * c_m;
+      Y[(base - m) * ys] = P * s_m;
+    }
+    P2 = P1;
+    P1 = P;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Pair phase of the mixed adjoint as a kernel of its own, after
+// adjoint_tile_kernel wrote dL/dc [S][LL][N] per centre: one workgroup per
+// centre, a group of 16 lanes per pair (16 pairs in flight).  With D[lm] =
+// sum_n dL/dc[s_j][lm][n] R_nl, the pair gradient of adjoint_pairs splits
+// over the radial channels: lane n of the group evaluates R_nl and R'_nl of
+// its channel for every l (spline) and accumulates
+//   A_n  = sum_l (R'_nl fc + R_nl fc') sum_m dc[lm][n] Y_lm,
+//   Sl_n = sum_l l R_nl sum_m dc[lm][n] Y_lm,
+//   B_n  = sum_l R_nl sum_m dc[lm][n] gY_lm   (gY: the gradient terms of Y),
+// with {Y, gY}[lm] of the pair staged once per group in LDS; the group sums
+// over n and its lane 0 forms the force in FP64 as adjoint_pairs does.  No
+// barrier between pairs (a group lives in one wavefront) and a small LDS
+// footprint keep several workgroups per CU in flight.
+constexpr int kPairGroup = 16;
+
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__host__ __device__ inline size_t pair_kernel_lds(int S, int N, int L1) {
+  const int LL = L1 * L1;
+  return ylm_table_bytes<float>(L1) + align_up(sizeof(YGradTerm<float>) * 3 * LL, 64) +
+         align_up(sizeof(float) * S * LL * N, 64) + sizeof(float) * 4 * LL * (kThreads / kPairGroup);
+}
+
+template <int N, int L>
+__global__ __launch_bounds__(kThreads) void adjoint_pair_kernel(RadialDev rad, int S, PairView pv,
+                                                                const float* __restrict__ dcg,
+                                                                double* __restrict__ grad,
+                                                                double* __restrict__ centre_virial) {
+  constexpr int L1 = L + 1, LL = L1 * L1, NG = kThreads / kPairGroup;
+  static_assert(N <= kPairGroup && L1 <= kPairGroup, "one lane per radial channel and per order m");
+  const int64_t i = blockIdx.x;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int gl = lane & (kPairGroup - 1), grp = tid / kPairGroup;
+  extern __shared__ __align__(64) unsigned char smem_raw[];
+  float* yc = reinterpret_cast<float*>(smem_raw);
+  YGradTerm<float>* gt = reinterpret_cast<YGradTerm<float>*>(smem_raw + ylm_table_bytes<float>(L1));
+  float* dc = reinterpret_cast<float*>(reinterpret_cast<unsigned char*>(gt) +
+                                       align_up(sizeof(YGradTerm<float>) * 3 * LL, 64));
+  f32x4* yg = reinterpret_cast<f32x4*>(reinterpret_cast<unsigned char*>(dc) +
+                                       align_up(sizeof(float) * S * LL * N, 64));
+  __shared__ double red[kWaves][9];
+  load_ylm_table<float>(rad, yc);
+  {
+    const YGradTerm<float>* src = ygrad_src<float>(rad);
+    for (int e = tid; e < 3 * LL; e += kThreads) gt[e] = src[e];
+    const int E4 = S * LL * N / 4;
+    const f32x4* d4 = reinterpret_cast<const f32x4*>(dcg + i * int64_t(S) * LL * N);
+    for (int e = tid; e < E4; e += kThreads) reinterpret_cast<f32x4*>(dc)[e] = d4[e];
+  }
+  __syncthreads();
+  const int64_t p0 = pv.centre_off[i] + 1;
+  const int np = int(pv.centre_off[i + 1] - p0);
+  f32x4* Yg = yg + grp * LL;
+  double fi[3] = {0, 0, 0}, v6[6] = {0, 0, 0, 0, 0, 0};
+  for (int q0 = 0; q0 < np; q0 += NG) {
+    const int q = q0 + grp;
+    if (q < np) {  // uniform within the group
+      const int64_t p = p0 + q;
+      const double d = pv.pair_dist[p];
+      const double vx = pv.pair_vec[3 * p], vy = pv.pair_vec[3 * p + 1], vz = pv.pair_vec[3 * p + 2];
+      const double fcd = pv.pair_fc[p], dfcd = pv.pair_dfc[p];
+      const int j = pv.pair_j[p];
+      const int sj = pv.species_of[j];
+      // spline cell of the distance (as pair_fetch / stage_pairs)
+      int klo;
+      double xhi, h;
+      if (rad.sp_tab) {
+        const int t = rad.sp_tab[sj], last = rad.tab_n[t] - 2;
+        int k = int((d - rad.x1) * rad.tab_pu[t]) - 1;
+        k = k < 0 ? 0 : (k > last ? last : k);
+        xhi = rad.tab_grid[rad.tab_grid0[t] + k + 1];
+        h = rad.tab_h[t];
+        klo = rad.tab_cell0[t] + k;
+      } else {
+        int k = int((d - rad.x1) * rad.per_unit) - 1;
+        k = k < 0 ? 0 : (k > rad.search_size ? rad.search_size : k);
+        xhi = rad.grid[k + 1];
+        h = rad.h;
+        klo = k;
+      }
+      const double ca = (xhi - d) / h;
+      SplineCell<float> cell;
+      cell.klo = klo;
+      cell.a = float(ca);
+      cell.b = float(1 - ca);
+      cell.inv_h = rad.sp_tab ? float(1.0 / h) : float(1.0 / rad.h);
+      float R[L1], dR[L1];
+      const int n = gl < N ? gl : N - 1;
+#pragma unroll
+      for (int l = 0; l <= L; ++l) spline_eval<float>(rad, cell, n * L1 + l, &R[l], &dR[l]);
+      const float dx = float(vx / d), dy = float(vy / d), dz = float(vz / d);
+      if (gl <= L) ylm_lane(L, yc, dx, dy, dz, gl, reinterpret_cast<float*>(Yg), 4);
+      wave_lds_sync();
+      for (int lm = gl; lm < LL; lm += kPairGroup) {
+        const YGradTerm<float>* g = gt + 3 * lm;
+        const float gx = g[0].c0 * Yg[g[0].i0].x + g[0].c1 * Yg[g[0].i1].x;
+        const float gy = g[1].c0 * Yg[g[1].i0].x + g[1].c1 * Yg[g[1].i1].x;
+        const float gz = g[2].c0 * Yg[g[2].i0].x + g[2].c1 * Yg[g[2].i1].x;
+        float* o = reinterpret_cast<float*>(Yg + lm);
+        o[1] = gx;
+        o[2] = gy;
+        o[3] = gz;
+      }
+      wave_lds_sync();
+      float A = 0.f, Sl = 0.f, Bx = 0.f, By = 0.f, Bz = 0.f;
+      if (gl < N) {
+        const float ffc = float(fcd), fdfc = float(dfcd);
+        const float* dcs = dc + sj * LL * N + gl;
+#pragma unroll
+        for (int l = 0; l <= L; ++l) {
+          float s1 = 0.f, sx = 0.f, sy = 0.f, sz = 0.f;
+#pragma unroll
+          for (int m = 0; m < 2 * l + 1; ++m) {
+            const int lm = l * l + m;
+            const float t = dcs[lm * N];
+            const f32x4 y4 = Yg[lm];
+            s1 += t * y4.x;
+            sx += t * y4.y;
+            sy += t * y4.z;
+            sz += t * y4.w;
+          }
+          A += s1 * (dR[l] * ffc + R[l] * fdfc);
+          Sl += s1 * (float(l) * R[l]);
+          Bx += sx * R[l];
+          By += sy * R[l];
+          Bz += sz * R[l];
+        }
+      }
+#pragma unroll
+      for (int off = kPairGroup / 2; off > 0; off >>= 1) {
+        A += __shfl_xor(A, off, kPairGroup);
+        Sl += __shfl_xor(Sl, off, kPairGroup);
+        Bx += __shfl_xor(Bx, off, kPairGroup);
+        By += __shfl_xor(By, off, kPairGroup);
+        Bz += __shfl_xor(Bz, off, kPairGroup);
+      }
+      if (gl == 0) {
+        const double sc = fcd / d;
+        const double gv[3] = {double(A) * vx / d + sc * double(Bx - dx * Sl),
+                              double(A) * vy / d + sc * double(By - dy * Sl),
+                              double(A) * vz / d + sc * double(Bz - dz * Sl)};
+        unsafeAtomicAdd(grad + 3 * j, gv[0]);
+        unsafeAtomicAdd(grad + 3 * j + 1, gv[1]);
+        unsafeAtomicAdd(grad + 3 * j + 2, gv[2]);
+        if (j != int(i)) {
+          fi[0] -= gv[0];
+          fi[1] -= gv[1];
+          fi[2] -= gv[2];
+        }
+        // r_ji = r_i - r_j(image) = -pair vector
+        v6[0] -= vx * gv[0];
+        v6[1] -= vy * gv[1];
+        v6[2] -= vz * gv[2];
+        v6[3] -= vy * gv[2];
+        v6[4] -= vz * gv[0];
+        v6[5] -= vx * gv[1];
+      }
+      wave_lds_sync();  // the group's {Y, gY} are read before the next pair overwrites them
+    }
+  }
+  double r[9] = {fi[0], fi[1], fi[2], v6[0], v6[1], v6[2], v6[3], v6[4], v6[5]};
+#pragma unroll
+  for (int k = 0; k < 9; ++k) {
+    for (int off = 32; off > 0; off >>= 1) r[k] += __shfl_down(r[k], off, 64);
+    if (lane == 0) red[wave][k] = r[k];
+  }
+  __syncthreads();
+  if (tid < 9) {
+    double t = 0;
+    for (int w = 0; w < kWaves; ++w) t += red[w][tid];
+    if (tid < 3) {
+      if (t != 0.0) unsafeAtomicAdd(grad + 3 * i + tid, t);
+    } else {
+      centre_virial[6 * i + tid - 3] = t;
+    }
+  }
+}
+
 // dL/dX of one centre per workgroup into the packed layout (packing.hh)
 __global__ void pack_gradient_kernel(int S, int N, int L1, const double* __restrict__ G,
                                      float* __restrict__ Gp) {
@@ -2295,18 +2952,113 @@ void launch_expansion_k(const RadialDev& rad, int S, const PairView& pv, T* c,
     launch_expansion_t<T, KPT, true>(rad, S, pv, c, keymask, stream, tiles);
 }
 
+// dynamic LDS of the register-tile adjoint: two workgroups per CU with the
+// pair phase in the same workgroup, three without it
+constexpr size_t kTileAdjBudget = 77 * 1024;
+constexpr size_t kTileSplitBudget = 52 * 1024;
+
+template <int N, int L, int NR, int ROLES, int PRE, bool DIRECT, bool PAIRS>
+bool go_adjoint_tile(const RadialDev& rad, int S, const PairView& pv, const float* c, const double* norm2,
+                     bool normalize, const double* xg, double xg_scale, const float* G, double* grad,
+                     double* centre_virial, float* work, hipStream_t stream, const SoapTiles& tiles) {
+  constexpr int L1 = L + 1, LL = L1 * L1, F = N * L1;
+  constexpr int KB = N * N * L1, KD = N * (N + 1) / 2 * L1;
+  constexpr int MT = 4 * adj_quads_before(L1), GRAM = adj_gram_before(L1);
+  const AdjRounds R(S);
+  // one round must fit the prefetch registers
+  if (S * (KD / 4) > PRE * kThreads || R.per * (KB / 4) > PRE * kThreads) return false;
+  const size_t budget = PAIRS ? kTileAdjBudget : kTileSplitBudget;
+  const size_t fixed = (PAIRS ? ylm_table_bytes<float>(L1) : 0) + align_up(sizeof(float) * S * N * MT, 64) +
+                       align_up(sizeof(float) * GRAM, 64);
+  int maxb = tiles.keys_adj;
+  if (maxb <= 0) maxb = fixed < budget ? int((budget - fixed) / (sizeof(float) * KB)) : 0;
+  maxb = std::max(maxb, std::max(S, R.per));
+  int big = 0;  // blocks of the largest chunk
+  for (int r0 = 0; r0 < R.rounds();) {
+    const int r1 = R.chunk_end(r0, maxb, KD / 4, KB / 4, PRE);
+    int nb = 0;
+    for (int r = r0; r < r1; ++r) nb += R.blocks(r);
+    big = std::max(big, nb);
+    r0 = r1;
+  }
+  const size_t region = align_up(sizeof(float) * size_t(big) * KB, 64);
+  int PB = 0;
+  size_t smem = fixed + region;
+  if (PAIRS) {
+    PB = tiles.pb_adj > 0 ? std::min(tiles.pb_adj, 64) : 64;
+    if (tiles.pb_adj <= 0)
+      while (PB > 1 && pair_batch_bytes(PB, LL, F, sizeof(float), DIRECT, true) > region) --PB;
+    smem = fixed + std::max(region, pair_batch_bytes(PB, LL, F, sizeof(float), DIRECT, true));
+  }
+  auto kern = adjoint_tile_kernel<N, L, NR, ROLES, PRE, DIRECT, PAIRS>;
+  allow_lds(kern, smem);
+  hipLaunchKernelGGL(kern, dim3(unsigned(pv.n_centres)), dim3(kThreads), smem, stream, rad, S, PB, maxb, pv,
+                     c, norm2, int(normalize), xg, xg_scale, G, grad, centre_virial, work);
+  RGPU_LAUNCH_CHECK();
+  if (!PAIRS) {
+    const size_t ps = pair_kernel_lds(S, N, L1);
+    auto pk = adjoint_pair_kernel<N, L>;
+    allow_lds(pk, ps);
+    hipLaunchKernelGGL(pk, dim3(unsigned(pv.n_centres)), dim3(kThreads), ps, stream, rad, S, pv,
+                       static_cast<const float*>(work), grad, centre_virial);
+    RGPU_LAUNCH_CHECK();
+  }
+  return true;
+}
+
+// The register-tile adjoint for the (max_radial, max_angular) it is built
+// for (up to 8 species); false where it does not apply.  With a workspace
+// of n_centres S (L+1)^2 N floats and spline radial functions, the pair
+// phase runs as adjoint_pair_kernel (split); otherwise in the same
+// workgroup.
+template <int N, int L, int NR, bool DIRECT>
+bool launch_adjoint_tile_nl(const RadialDev& rad, int S, const PairView& pv, const float* c, const double* norm2,
+                            bool normalize, const double* xg, double xg_scale, const float* G, double* grad,
+                            double* centre_virial, float* work, hipStream_t stream, const SoapTiles& tiles) {
+  if (work && !DIRECT) {
+    if (S <= kWaves)
+      return go_adjoint_tile<N, L, NR, 1, 6, false, false>(rad, S, pv, c, norm2, normalize, xg, xg_scale, G, grad,
+                                                           centre_virial, work, stream, tiles);
+    return go_adjoint_tile<N, L, NR, 2, 8, false, false>(rad, S, pv, c, norm2, normalize, xg, xg_scale, G, grad,
+                                                         centre_virial, work, stream, tiles);
+  }
+  if (S <= kWaves)
+    return go_adjoint_tile<N, L, NR, 1, 10, DIRECT, true>(rad, S, pv, c, norm2, normalize, xg, xg_scale, G, grad,
+                                                          centre_virial, nullptr, stream, tiles);
+  return go_adjoint_tile<N, L, NR, 2, 10, DIRECT, true>(rad, S, pv, c, norm2, normalize, xg, xg_scale, G, grad,
+                                                        centre_virial, nullptr, stream, tiles);
+}
+
+template <bool DIRECT>
+bool launch_adjoint_tile(const RadialDev& rad, int S, const PairView& pv, const float* c, const double* norm2,
+                         bool normalize, const double* xg, double xg_scale, const float* G, double* grad,
+                         double* centre_virial, float* work, hipStream_t stream, const SoapTiles& tiles) {
+  if (S > 2 * kWaves) return false;
+  if (rad.N == 12 && rad.L == 9)
+    return launch_adjoint_tile_nl<12, 9, 6, DIRECT>(rad, S, pv, c, norm2, normalize, xg, xg_scale, G, grad,
+                                                    centre_virial, work, stream, tiles);
+  if (rad.N == 8 && rad.L == 6)
+    return launch_adjoint_tile_nl<8, 6, 2, DIRECT>(rad, S, pv, c, norm2, normalize, xg, xg_scale, G, grad,
+                                                   centre_virial, work, stream, tiles);
+  return false;
+}
+
 template <class T, bool DIRECT>
 void launch_adjoint_t(const RadialDev& rad, int S, const PairView& pv, const T* c, const T* p,
                       const unsigned short* p_hi, const unsigned short* p_lo,
                       const double* norm2, bool normalize, const double* xg, double xg_scale,
                       const T* G, double* grad, double* centre_virial, hipStream_t stream,
                       const SoapTiles& tiles, const T* Grow = nullptr,
-                      const double* Gscale = nullptr) {
+                      const double* Gscale = nullptr, float* work = nullptr) {
   const int N = rad.N, L1 = rad.L1, LL = rad.LL, F = N * L1, E = S * N * LL;
   const size_t yt = ylm_table_bytes<T>(L1);
   const size_t blk = align_up(sizeof(T) * size_t(E), 64);
   if constexpr (sizeof(T) == 4) {
     const int K = S * (S + 1) / 2;
+    if ((tiles.adj_kind == 2 || tiles.adj_kind == 3) && Grow == nullptr &&
+        launch_adjoint_tile<DIRECT>(rad, S, pv, c, norm2, normalize, xg, xg_scale, G, grad, centre_virial,
+                                    tiles.adj_kind == 3 ? work : nullptr, stream, tiles))
+      return;
     // matrix-core adjoint: packed dL/dX staged KCH key blocks at a time,
     // the invariants not read
     int T_col = 0;
@@ -2712,14 +3464,14 @@ void launch_backward_split(const RadialDev& rad, int S, const PairView& pv, cons
                            const unsigned short* p_hi, const unsigned short* p_lo,
                            const double* norm2, bool normalize, const double* xg, double xg_scale,
                            const float* G, double* grad, double* centre_virial, hipStream_t stream,
-                           const SoapTiles& tiles) {
+                           const SoapTiles& tiles, float* work) {
   if (pv.n_centres == 0) return;
   if (rad.use_spline)
     launch_adjoint_t<float, false>(rad, S, pv, c, nullptr, p_hi, p_lo, norm2, normalize, xg, xg_scale,
-                                   G, grad, centre_virial, stream, tiles);
+                                   G, grad, centre_virial, stream, tiles, nullptr, nullptr, work);
   else
     launch_adjoint_t<float, true>(rad, S, pv, c, nullptr, p_hi, p_lo, norm2, normalize, xg, xg_scale,
-                                  G, grad, centre_virial, stream, tiles);
+                                  G, grad, centre_virial, stream, tiles, nullptr, nullptr, work);
 }
 
 void launch_pack_gradient(int S, int N, int L1, int64_t n, const double* G, float* Gp,

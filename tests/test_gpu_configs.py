@@ -192,7 +192,10 @@ def test_autotuned_tiles_keep_results():
         G = rng.normal(size=X0.shape)
         g0, s0 = ctx.features_backward(G)
         tiles = ctx.autotune()
-        assert tiles["pairs_fwd"] > 0 and tiles["keys_ps"] > 0 and tiles["pairs_adj"] > 0
+        # pairs_adj 0: as many pairs per batch as fit (or, adj_kind 3, the
+        # pair phase in a kernel of its own)
+        assert tiles["pairs_fwd"] > 0 and tiles["keys_ps"] > 0 and tiles["pairs_adj"] >= 0
+        assert tiles["adj_kind"] in (0, 1, 2, 3)
         X1, _ = ctx.features()
         tol = 1e-12 if precision == _capi.FP64 else 1e-5
         assert np.abs(X1 - X0).max() < tol * np.abs(X0).max()
@@ -201,7 +204,7 @@ def test_autotuned_tiles_keep_results():
         assert np.abs(s1 - s0).max() < tol * np.abs(s0).max()
 
 
-@pytest.mark.parametrize("N,L", [(12, 9), (5, 4), (6, 3)])
+@pytest.mark.parametrize("N,L", [(12, 9), (8, 6), (5, 4), (6, 3)])
 def test_mixed_adjoint_partitions_agree(N, L):
     """Mixed-precision adjoint (packed dL/dX and invariants, dL/dp staged in
     LDS one, three or all key blocks per pass, several pair batch sizes) vs
@@ -210,7 +213,11 @@ def test_mixed_adjoint_partitions_agree(N, L):
     diagonal-key blocks.  N = 5, L = 4 gives key blocks of 125 and 75
     elements (element-wise staging) and a partial group of radial channels;
     N = 6 another partial group.  adj_kind 1 is the matrix-core kernel
-    (no invariants read; dL/dc = alpha f_l W_l C_l - 2 beta f_l^2 C_l C_l^T C_l)."""
+    (no invariants read; dL/dc = alpha f_l W_l C_l - 2 beta f_l^2 C_l C_l^T C_l),
+    adj_kind 2 the register-tile kernel with the same formula (built for
+    N, L = 12, 9 and 8, 6; the staged kernel elsewhere), keys_adj its key
+    blocks per staged chunk of rounds; adj_kind 3 the same with the pair
+    phase as a kernel of its own."""
     sts = molecules(n_mol=24, seed=5)
     hp = soap_hypers(rc=4.0, sw=0.5, sigma=0.4, N=N, L=L, spline=1e-8)
     ctx, sp = make_ctx(hp, sts)
@@ -220,7 +227,7 @@ def test_mixed_adjoint_partitions_agree(N, L):
     g64, s64 = ctx.features_backward(G)
     ctx.set_precision(_capi.MIXED)
     out = {}
-    for kind in (0, 1):
+    for kind in (0, 1, 2, 3):
         for kch, pb in ((0, 0), (1, 4), (3, 16), (10, 6)):
             ctx.set_tiles(pairs_adj=pb, keys_adj=kch, adj_kind=kind)
             t = ctx.tiles()
@@ -232,6 +239,37 @@ def test_mixed_adjoint_partitions_agree(N, L):
         assert np.abs(g - g64).max() < 2e-5 * scale, key
         assert np.abs(s - s64).max() < 2e-5 * np.abs(s64).max(), key
         assert np.abs(g - out[0, 0, 0][0]).max() < 1e-5 * scale, key
+
+
+@pytest.mark.parametrize("n_species", [1, 2, 3, 5, 8])
+@pytest.mark.parametrize("N,L", [(12, 9), (8, 6)])
+def test_tile_adjoint_species_counts(n_species, N, L):
+    """The register-tile adjoint (adj_kind 2; 3 with the pair phase as a
+    kernel of its own) over 1-8 species: the circle-
+    method rounds with a bye (odd counts), one or two species per wavefront
+    (more than 4 species), chunks of one round (keys_adj 1) and of all
+    rounds; mixed forces and stress vs the FP64 adjoint on a random dL/dX,
+    with and without normalisation (beta = 0)."""
+    species = (1, 6, 7, 8, 9, 14, 15, 16)[:n_species]
+    probs = np.full(n_species, 1.0 / n_species)
+    sts = molecules(n_mol=16, n_min=12, n_max=40, species=species, probs=probs, seed=40 + n_species)
+    for normalize in (True, False):
+        hp = soap_hypers(rc=4.0, sw=0.5, sigma=0.4, N=N, L=L, spline=1e-8)
+        hp["normalize"] = normalize
+        ctx, sp = make_ctx(hp, sts)
+        X64, _ = ctx.features()
+        rng = np.random.default_rng(n_species)
+        G = rng.normal(size=X64.shape)
+        g64, s64 = ctx.features_backward(G)
+        ctx.set_precision(_capi.MIXED)
+        scale = np.abs(g64).max()
+        for kind in (2, 3):
+            for kch in (0, 1, 2 * n_species):
+                ctx.set_tiles(pairs_adj=0, keys_adj=kch, adj_kind=kind)
+                ctx.features()
+                g, s = ctx.features_backward(G)
+                assert np.abs(g - g64).max() < 2e-5 * scale, (normalize, kind, kch)
+                assert np.abs(s - s64).max() < 2e-5 * np.abs(s64).max(), (normalize, kind, kch)
 
 
 def test_large_gemm_tiles_mixed_vs_fp64():
