@@ -1,14 +1,18 @@
 """The reference's Python model API (SparsePoints, Kernel, KRR, module
 gradient functions) and the autograd bindings, on the GPU, against the FP64
 oracle (tolerance 1e-10 relative, FP64 path)."""
+import json
+import os
+
 import numpy as np
 import pytest
 
 import oracle as O
-from conftest import golden_structures, oracle_backward, rel_err
+from conftest import GOLDEN, golden_structures, oracle_backward, rel_err
 from librascal_amd.models import (KRR, Kernel, SparsePoints, compute_sparse_kernel_gradients,
                                   compute_sparse_kernel_neg_stress, train_gap_model)
-from librascal_amd.representations import SphericalInvariants
+from librascal_amd.representations import SphericalInvariants, from_hypers
+from librascal_amd.synthetic import wrap_positions
 
 pytestmark = pytest.mark.gpu
 
@@ -79,6 +83,82 @@ def test_kernel_gradient_rows(zeta):
     ns = compute_sparse_kernel_neg_stress(rep, kernel, al, sp, w)
     assert np.abs(kd[:3 * b.n_atoms] @ w - g.ravel()).max() < TOL * np.abs(g).max()
     assert np.abs(kd[3 * b.n_atoms:] @ w - ns.ravel()).max() < TOL * np.abs(ns).max()
+
+
+def _reference_relative_error(ref, test, epsilon):
+    """math::relative_error (math/utils.hh:152-194): |ref - test| / |ref|,
+    0 where both are below epsilon, the plain difference where only ref is."""
+    diff = np.abs(ref - test)
+    zr, zt = np.abs(ref) < epsilon, np.abs(test) < epsilon
+    out = np.where(zr, diff, diff / np.where(zr, 1.0, np.abs(ref)))
+    return np.where(zr & zt, 0.0, out)
+
+
+_SKI_STRUCTURES = {"diamond_cubic_distorted.json": "diamond_cubic_distorted",
+                   "CaCrP2O7_mvc-11955_symmetrized.json": "CaCrP2O7",
+                   "small_molecules-20.json": "small_molecules_20"}
+
+
+@pytest.mark.parametrize("case", range(6))
+def test_sparse_kernel_reference_cases(case):
+    """The reference's own sparse-kernel gradient cases
+    (sparse_kernel_inputs.json, test_sparse_kernels.cc:126-330): its hypers
+    (GTO, direct radial integrals, n 3, l 3, structure- or environment-wise
+    keys, normalised or not), structures and selected sparse points, GAP
+    zeta 2 on Structure targets.  K_NM derivative rows vs central finite
+    differences of K_NM with the case's h (atoms displaced and wrapped as
+    compute_displaced_kernel does; cells strained as displace_strain_tensor
+    does, divided by -volume), with the reference's thresholds: relative
+    error 1e-3 for the position rows (zero below 1e-14) and 6e-3 for the
+    negative-stress rows (zero below 1e-11); and the prediction routines
+    equal to the rows contracted with unit weights."""
+    c = json.load(open(os.path.join(GOLDEN, "sparse_kernel_inputs.json")))[case]
+    frames = golden_structures()[_SKI_STRUCTURES[os.path.basename(c["filename"])]]
+    frames = [dict(f) for f in frames[:c["n_structures"]]]
+    rep = from_hypers(c["calculator"])
+    al = rep.transform(frames)
+    sp = SparsePoints(rep)
+    sp.extend(al, c["selected_ids"])
+    kernel = Kernel(rep, name="GAP", zeta=c["kernel"]["zeta"], target_type="Structure",
+                    kernel_type="Sparse")
+    kd = kernel(al, sp, grad=(True, False), compute_neg_stress=True)
+    n_at = sum(len(f["numbers"]) for f in frames)
+    assert kd.shape == (3 * n_at + 6 * len(frames), sp.size())
+    h = c["h"]
+    copies, rows = [], []
+    row = 0
+    for f in frames:
+        for a in range(len(f["numbers"])):
+            for d in range(3):
+                for sgn in (1.0, -1.0):
+                    pos = f["positions"].copy()
+                    pos[a, d] += sgn * h
+                    copies.append(dict(f, positions=wrap_positions(pos, f["cell"], f["pbc"])))
+                rows.append((row, 1.0 / (2 * h)))
+                row += 1
+    for f in frames:
+        vol = abs(np.linalg.det(f["cell"]))
+        for (x, y) in ((0, 0), (1, 1), (2, 2), (1, 2), (0, 2), (0, 1)):
+            for sgn in (1.0, -1.0):
+                shift = np.eye(3)
+                shift[x, y] += sgn * h
+                cell = shift.T @ f["cell"]
+                pos = (cell @ np.linalg.inv(f["cell"]) @ f["positions"].T).T
+                copies.append(dict(f, positions=pos, cell=cell))
+            rows.append((row, -1.0 / (2 * h * vol)))
+            row += 1
+    k = kernel(rep.transform(copies), sp)
+    fd = np.stack([(k[2 * q] - k[2 * q + 1]) * sc for q, (_, sc) in enumerate(rows)])
+    npos = 3 * n_at
+    ef = _reference_relative_error(kd[:npos], fd[:npos], 1e-14)
+    es = _reference_relative_error(kd[npos:], fd[npos:], 1e-11)
+    assert ef.max() < 1e-3, (np.unravel_index(ef.argmax(), ef.shape), ef.max())
+    assert es.max() < 6e-3, (np.unravel_index(es.argmax(), es.shape), es.max())
+    w = np.ones(sp.size())
+    g = compute_sparse_kernel_gradients(rep, kernel, al, sp, w).ravel()
+    ns = compute_sparse_kernel_neg_stress(rep, kernel, al, sp, w).ravel()
+    assert _reference_relative_error(kd[:npos] @ w, g, 1e-14).max() < 1e-3
+    assert _reference_relative_error(kd[npos:] @ w, ns, 1e-11).max() < 6e-3
 
 
 def test_krr_predictions_and_gradient_functions():
