@@ -243,8 +243,8 @@ def test_dvr_gradients_vs_finite_differences(spline, N):
     """The adjoint with the DVR radial derivatives (Bessel-function gradient
     recursion): dL/dr for L = sum G . X of the normalised power spectrum
     against central differences (h = 1e-5 A) of the GPU features.  N = 5 has
-    a quadrature node at the middle of the interval, where a Newton iteration
-    that stops before its first step leaves an infinite weight."""
+    a quadrature node at the middle of the interval, whose weight the
+    reference's routine forms with the previous root's derivative."""
     hp = soap_hypers(rc=3.5, sw=0.5, sigma=0.4, N=N, L=4, spline=spline)
     hp["radial_contribution"]["type"] = "DVR"
     sts = golden_structures()["small_molecule"]
@@ -266,6 +266,55 @@ def test_dvr_gradients_vs_finite_differences(spline, N):
             fd = (vals[0] - vals[1]) / (2 * h)
             assert abs(fd - g[atom, d]) < 1e-6 * np.abs(g).max() + 1e-6 * abs(fd)
     ctx.update_positions(pos)
+
+
+def _reference_gauss_legendre(r0, r1, n):
+    """The reference's Gauss-Legendre loop (math/gauss_legendre.cc:30-73)
+    as plain Python: Newton's iteration per root from z1 = 0 while
+    |z - z1| > DBL_FTOL, the Legendre derivative pp carried over from the
+    previous root when the loop does not run."""
+    x, w = np.zeros(n), np.zeros(n)
+    mid, half = 0.5 * (r0 + r1), 0.5 * (r1 - r0)
+    pp = 0.0
+    for i in range((n + 1) // 2):
+        z, z1 = np.cos(np.pi * (i + 0.75) / (n + 0.5)), 0.0
+        while abs(z - z1) > 100 * 2.220446049250313e-16:
+            p1, p2 = 1.0, 0.0
+            for j in range(n):
+                p1, p2 = ((2 * j + 1) * z * p1 - j * p2) / (j + 1), p1
+            pp = n * (z * p1 - p2) / (z * z - 1.0)
+            z1, z = z, z - p1 / pp
+        x[i], x[n - 1 - i] = mid - half * z, mid + half * z
+        w[i] = w[n - 1 - i] = 2.0 * half / ((1.0 - z * z) * pp * pp)
+    return x, w
+
+
+@pytest.mark.parametrize("N", [4, 5, 7])
+def test_dvr_quadrature_follows_the_reference(N):
+    """DVR centre term of an isolated atom: sqrt(w_n) x_n exp(-x_n^2 / 2
+    sigma^2) times one common factor (calculator_spherical_expansion.hh
+    :809-848), with the points and weights of the reference's Gauss-Legendre
+    loop on [0, rc + 3 sigma].  For odd N that loop skips Newton's iteration
+    at the middle root and keeps the previous root's derivative; the middle
+    weight then differs from the textbook quadrature's, and the GPU basis
+    follows the reference there too."""
+    rc, sigma = 3.5, 0.4
+    hp = soap_hypers(rc=rc, sw=0.5, sigma=sigma, N=N, L=2, spline=None, normalize=False)
+    hp["radial_contribution"]["type"] = "DVR"
+    st = dict(positions=np.array([[10.0, 10.0, 10.0]]), numbers=np.array([1]),
+              cell=np.eye(3) * 20.0, pbc=np.zeros(3, int))
+    ctx, _ = make_ctx(hp, [st])
+    c, _ = ctx.expansion()
+    c0 = c.reshape(N, -1)[:, 0]
+    x, w = _reference_gauss_legendre(0.0, rc + 3 * sigma, N)
+    ratio = c0 / (np.sqrt(w) * x * np.exp(-0.5 * x ** 2 / sigma ** 2))
+    assert np.abs(ratio / ratio[0] - 1.0).max() < 1e-12
+    zt, wt = np.polynomial.legendre.leggauss(N)
+    wt = wt * 0.5 * (rc + 3 * sigma)
+    if N % 2:
+        assert abs(wt[N // 2] / w[N // 2] - 1.0) > 1e-2
+    else:
+        assert np.abs(wt / w - 1.0).max() < 1e-12
 
 
 def _rdr_rows():
