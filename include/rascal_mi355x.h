@@ -217,6 +217,10 @@ int rgpu_gap_knm_gradient(rgpu_context* ctx, int neg_stress, double* out);
  * sparse points were given (SparseKernelImpl<GAP>::compute(sparse_points),
  * models/sparse_kernels.hh:136-150). */
 int rgpu_gap_kmm(rgpu_context* ctx, double* kmm);
+/* out [cols][cols] = A^T A for A [rows][cols], row-major FP64 host arrays,
+ * on the FP64 matrix cores (no context needed).  Replaces the normal-matrix
+ * product np.dot(KNM.T, KNM) of train_gap_model (models/krr.py:468). */
+int rgpu_gram_f64(int64_t rows, int64_t cols, const double* A, double* out);
 /* Prediction without K_NM rows: energy[n_struct] = K_NM w, gradient
  * [n_atoms][3] = dE/dr, neg_stress [n_struct][6] (/ volume).  Any output may
  * be NULL; gradient/neg_stress trigger the adjoint

@@ -77,6 +77,7 @@ def lib():
     L.rgpu_set_sparse_points.argtypes = [p, i, i64, p, p, p]
     L.rgpu_gap_knm.argtypes = [p, i, p]
     L.rgpu_gap_kmm.argtypes = [p, p]
+    L.rgpu_gram_f64.argtypes = [i64, i64, p, p]
     L.rgpu_gap_knm_gradient.argtypes = [p, i, p]
     L.rgpu_gap_predict.argtypes = [p, p, p, p]
     L.rgpu_gap_energy_noalloc.argtypes = [p, p]
@@ -90,12 +91,22 @@ def lib():
                  "rgpu_enable_graphs", "rgpu_autotune", "rgpu_set_tiles",
                  "rgpu_set_structures", "rgpu_update_positions", "rgpu_get_neighbours",
                  "rgpu_compute", "rgpu_get_expansion", "rgpu_get_features", "rgpu_get_features_gradient",
-                 "rgpu_features_backward", "rgpu_set_sparse_points", "rgpu_gap_knm", "rgpu_gap_kmm", "rgpu_gap_knm_gradient",
+                 "rgpu_features_backward", "rgpu_set_sparse_points", "rgpu_gap_knm", "rgpu_gap_kmm", "rgpu_gram_f64", "rgpu_gap_knm_gradient",
                  "rgpu_gap_predict", "rgpu_gap_energy_noalloc", "rgpu_eval_radial", "rgpu_eval_ylm", "rgpu_enable_timing",
                  "rgpu_get_timing", "rgpu_reset_timing"):
         getattr(L, name).restype = C.c_int
     _lib = L
     return L
+
+
+def gram_f64(A):
+    """A^T A (FP64) on the GPU's FP64 matrix cores, for a host array A."""
+    A = np.ascontiguousarray(A, np.float64)
+    if A.ndim != 2:
+        raise ValueError("gram_f64 takes a 2-d array")
+    out = np.empty((A.shape[1], A.shape[1]), np.float64)
+    check(lib().rgpu_gram_f64(int(A.shape[0]), int(A.shape[1]), ptr(A), ptr(out)))
+    return out
 
 
 def check(rc):

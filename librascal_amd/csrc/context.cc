@@ -1822,6 +1822,33 @@ int rgpu_set_sparse_points(rgpu_context* ctx, int zeta, int64_t n_sparse, const 
   });
 }
 
+int rgpu_gram_f64(int64_t rows, int64_t cols, const double* A, double* out) {
+  return guarded([&] {
+    if (rows < 0 || cols < 0) throw InvalidArgument("negative matrix size");
+    if ((rows > 0 && cols > 0 && !A) || (cols > 0 && !out)) throw InvalidArgument("null argument");
+    if ((rows + 31) / 32 > 65535 || (cols + 31) / 32 > 65535)
+      throw InvalidArgument("matrix too large for rgpu_gram_f64 (at most 2097120 rows and columns)");
+    const size_t nc = size_t(cols) * size_t(cols);
+    if (rows == 0 || cols == 0) {
+      if (cols > 0) std::fill(out, out + nc, 0.0);
+      return;
+    }
+    DeviceBuffer<double> dA, dAt, dC;
+    DeviceBuffer<int> dI;
+    std::vector<int> iota(static_cast<size_t>(cols));
+    for (int64_t k = 0; k < cols; ++k) iota[size_t(k)] = int(k);
+    const size_t na = size_t(rows) * size_t(cols);
+    double* a = dA.reserve(na);
+    double* at = dAt.reserve(na);
+    double* cm = dC.reserve(nc);
+    int* io = dI.reserve(iota.size());
+    RGPU_HIP(hipMemcpy(a, A, sizeof(double) * na, hipMemcpyHostToDevice));
+    RGPU_HIP(hipMemcpy(io, iota.data(), sizeof(int) * iota.size(), hipMemcpyHostToDevice));
+    run_gram_f64(rows, cols, a, at, io, cm, nullptr);
+    RGPU_HIP(hipMemcpy(out, cm, sizeof(double) * nc, hipMemcpyDeviceToHost));
+  });
+}
+
 int rgpu_gap_kmm(rgpu_context* ctx, double* kmm) {
   return guarded([&] {
     if (!ctx || !kmm) throw InvalidArgument("null argument");

@@ -15,4 +15,9 @@ void run_gap_f64_kernel(const GapBlock<double>* blks, int n_blocks, double* ener
 void run_gap_f64_force(const GapBlock<double>* blks, int n_blocks, double* const* wts, double* G,
                        hipStream_t stream);
 
+// C (n x n) = A^T A for A [rows][n] (device pointers; At: room for n x rows
+// doubles, iota: 0 .. n-1 as ints)
+void run_gram_f64(int64_t rows, int64_t n, const double* A, double* At, const int* iota, double* C,
+                  hipStream_t stream);
+
 }  // namespace rgpu

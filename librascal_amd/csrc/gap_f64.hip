@@ -226,6 +226,23 @@ void launch(const ProblemSet& ps, int64_t tiles, int epi, hipStream_t stream) {
   RGPU_LAUNCH_CHECK();
 }
 
+// out[c][r] = in[r][c] for in [rows][cols], 32 x 32 tiles through LDS
+__global__ void transpose_f64_kernel(int64_t rows, int64_t cols, const double* __restrict__ in,
+                                     double* __restrict__ out) {
+  __shared__ double t[32][33];
+  const int64_t r0 = int64_t(blockIdx.y) * 32, c0 = int64_t(blockIdx.x) * 32;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8 threads
+  for (int k = ty; k < 32; k += 8) {
+    const int64_t r = r0 + k, c = c0 + tx;
+    if (r < rows && c < cols) t[k][tx] = in[r * cols + c];
+  }
+  __syncthreads();
+  for (int k = ty; k < 32; k += 8) {
+    const int64_t c = c0 + k, r = r0 + tx;
+    if (r < rows && c < cols) out[c * rows + r] = t[tx][k];
+  }
+}
+
 }  // namespace
 
 void run_gap_f64_kernel(const GapBlock<double>* blks, int n_blocks, double* energy, double* const* wts,
@@ -259,6 +276,24 @@ void run_gap_f64_kernel(const GapBlock<double>* blks, int n_blocks, double* ener
     }
     launch(ps, tiles, 0, stream);
   }
+}
+
+void run_gram_f64(int64_t rows, int64_t n, const double* A, double* At, const int* iota, double* C,
+                  hipStream_t stream) {
+  if (rows == 0 || n == 0) return;
+  const dim3 grid(unsigned((n + 31) / 32), unsigned((rows + 31) / 32));
+  hipLaunchKernelGGL(transpose_f64_kernel, grid, dim3(256), 0, stream, rows, n, A, At);
+  RGPU_LAUNCH_CHECK();
+  // C[i][:] = sum_r At[i][r] A[r][:], the force product with W~ = A^T
+  GapBlock<double> blk{};
+  blk.n_rows = n;
+  blk.row_idx = iota;
+  blk.n_cols = rows;
+  blk.col0 = 0;
+  blk.Tm = A;
+  blk.D = n;
+  double* const wts[1] = {At};
+  run_gap_f64_force(&blk, 1, wts, C, stream);
 }
 
 void run_gap_f64_force(const GapBlock<double>* blks, int n_blocks, double* const* wts, double* G,

@@ -350,13 +350,9 @@ def compute_KNM(frames, X_sparse, kernel, soap):
 
 
 def _gram_on_device(A):
-    """A^T A in FP64 on the GPU (a plain library GEMM; torch is the device
-    plumbing here)."""
-    import torch
-    if not torch.cuda.is_available():
-        raise RuntimeError("train_gap_model needs the GPU (K_NM^T K_NM is formed there)")
-    t = torch.from_numpy(np.ascontiguousarray(A, np.float64)).to("cuda")
-    return (t.T.contiguous() @ t).cpu().numpy()
+    """A^T A in FP64 on the GPU (the FP64 matrix-core GEMM of the GAP path,
+    rgpu_gram_f64)."""
+    return _capi.gram_f64(A)
 
 
 def train_gap_model(kernel, frames, KNM_, X_sparse, y_train, self_contributions, grad_train=None,

@@ -161,6 +161,19 @@ def test_sparse_kernel_reference_cases(case):
     assert _reference_relative_error(kd[npos:] @ w, ns, 1e-11).max() < 6e-3
 
 
+@pytest.mark.parametrize("shape", [(300, 70), (1, 5), (129, 64), (1000, 257), (0, 4)])
+def test_gram_f64_matches_numpy(shape):
+    """rgpu_gram_f64 (A^T A on the FP64 matrix cores, the normal matrix of
+    train_gap_model, krr.py:468) against numpy, for sizes off the 64 x 64
+    tile and an empty A."""
+    from librascal_amd import _capi
+    A = np.random.default_rng(shape[0]).normal(size=shape)
+    G = _capi.gram_f64(A)
+    ref = A.T @ A
+    assert G.shape == ref.shape
+    assert np.abs(G - ref).max() <= 1e-12 * max(np.abs(ref).max(), 1.0)
+
+
 def test_krr_predictions_and_gradient_functions():
     rep = _rep()
     frames, al, sp, kernel, w = _model(rep, 2)
