@@ -260,16 +260,21 @@ class KRR:
     def _get_property_baseline(self, managers):
         """Self contributions of the centres (krr.py:68-90)."""
         mask = managers.center_atoms_mask
+        nums = np.asarray(managers.numbers).ravel()
+        uniq = np.unique(nums)
+        table = np.array([self.self_contributions[int(z)] for z in uniq], np.float64)
+        per_atom = table[np.searchsorted(uniq, nums)] if nums.size else np.zeros(0)
         if self.target_type == "Structure":
-            Y0 = np.zeros(len(managers))
-            for s in range(len(managers)):
-                a0, a1 = managers.atom_offsets[s], managers.atom_offsets[s + 1]
-                for k, z in enumerate(managers.numbers[a0:a1]):
-                    if mask is None or mask[a0 + k]:
-                        Y0[s] += self.self_contributions[int(z)]
+            if mask is not None:
+                per_atom = np.where(np.asarray(mask, bool), per_atom, 0.0)
+            off = np.asarray(managers.atom_offsets, np.int64)
+            Y0 = np.zeros(len(off) - 1)
+            filled = off[1:] > off[:-1]
+            if per_atom.size and filled.any():
+                sums = np.add.reduceat(per_atom, np.minimum(off[:-1], per_atom.size - 1))
+                Y0[filled] = sums[filled]
             return Y0
-        nums = managers.numbers if mask is None else managers.numbers[mask]
-        return np.array([self.self_contributions[int(z)] for z in nums])
+        return per_atom if mask is None else per_atom[np.asarray(mask, bool)]
 
     def predict(self, managers, KNM=None):
         rep = self.kernel._rep

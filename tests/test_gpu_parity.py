@@ -221,6 +221,32 @@ def _dvr_rows():
     return json.load(open(os.path.join(_GOLDEN_DIR, "spherical_expansion_dvr_reference.json")))
 
 
+def _gto_rows():
+    return json.load(open(os.path.join(_GOLDEN_DIR, "spherical_expansion_reference.json")))
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("row", _gto_rows(), ids=lambda r: r["name"])
+def test_gto_expansion_reference_on_gpu(row, precision):
+    """tests/test_calculator.cc:460-493, the 32 GTO rows of the reference's
+    spherical_expansion_reference.ubjson (CaCrP2O7 and a small molecule, rc
+    2 and 3, sigma 0.2 and 0.5, ShiftedCosine and RadialScaling, direct and
+    Spline 1e-8) straight against the GPU expansion: FP64 at 1e-8
+    (relative_error as the reference's, which holds itself to 3e-6), FP32
+    at 1e-5 of each row's largest coefficient."""
+    feats = np.load(os.path.join(_GOLDEN_DIR, "spherical_expansion_reference.npz"))[row["name"]]
+    sts = golden_structures()[_STRUCT_OF[row["structure"]]]
+    ctx, _ = make_ctx(row["hypers"], sts, precision=_capi.FP64 if precision == "fp64" else _capi.MIXED)
+    c, _ = ctx.expansion()
+    X = c.reshape(c.shape[0], -1)
+    assert X.shape == feats.shape
+    if precision == "fp64":
+        assert rel_err(feats, X).max() < 1e-8
+    else:
+        row_max = np.abs(feats).max(axis=1, keepdims=True)
+        assert np.all(np.abs(X - feats) <= 1e-5 * row_max)
+
+
 @pytest.mark.parametrize("row", _dvr_rows(), ids=lambda r: r["name"])
 def test_dvr_expansion_reference(row):
     """tests/test_calculator.cc:460-493, the DVR rows of the reference's
