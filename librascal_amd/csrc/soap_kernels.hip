@@ -621,7 +621,7 @@ __device__ void ps_tables(int N, int S, int* key_ab, int* tri) {
 // rows or the BF16 operand planes and norm2[i].  Every thread calls; ends
 // after a barrier.
 template <class T, bool SPLIT>
-__device__ void ps_body(int N, int L, int S, int KC, int64_t Dq, int64_t i, const T* cs, T* stage,
+__device__ __forceinline__ void ps_body(int N, int L, int S, int KC, int64_t Dq, int64_t i, const T* cs, T* stage,
                         const int* key_ab, const int* tri, T* __restrict__ p,
                         unsigned short* __restrict__ p_hi, unsigned short* __restrict__ p_lo,
                         double* __restrict__ norm2) {
@@ -684,7 +684,48 @@ __device__ void ps_body(int N, int L, int S, int KC, int64_t Dq, int64_t i, cons
       // n1 < n2 entries times sqrt 2) as the matrix-core operand planes
       // v = hi + lo (RNE BF16 each); the norm from the full layout
       const int KD = packed_diag_block(N, L1);
-      for (int kk = 0; kk < kc; ++kk) {
+      // four values per step where every key block keeps 16-byte stage
+      // reads and 8-byte plane stores aligned
+      const bool vec4 = ((KB | KD) & 3) == 0;
+      for (int kk = 0; kk < kc && vec4; ++kk) {
+        const int ab = key_ab[k0 + kk];
+        const int a = ab >> 8, b = ab & 255;
+        const T* sk = stage + kk * KB;
+        const int64_t ob = i * Dq + packed_key_offset(a, b, S, N, L1);
+        uint2* h2 = reinterpret_cast<uint2*>(p_hi + ob);
+        uint2* l2 = reinterpret_cast<uint2*>(p_lo + ob);
+        const int nq = (a < b ? KB : KD) >> 2;
+        for (int q = threadIdx.x; q < nq; q += kThreads) {
+          float v[4];
+          if (a < b) {
+            const typename Vec4<T>::type s4 = reinterpret_cast<const typename Vec4<T>::type*>(sk)[q];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+              v[u] = float(s4[u]);
+              ss += double(v[u]) * double(v[u]);
+            }
+          } else {
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+              const int o = 4 * q + u, t = o / fL1, l = o - t * L1;
+              const int n1 = tri[t] >> 8, n2 = tri[t] & 255;
+              v[u] = float(sk[(n1 * N + n2) * L1 + l]);
+              ss += (n1 < n2 ? 2.0 : 1.0) * double(v[u]) * double(v[u]);
+              if (n1 < n2) v[u] *= 1.41421356237309504880f;
+            }
+          }
+          unsigned hb[4], lb[4];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            const __bf16 h = (__bf16)v[u];
+            hb[u] = __builtin_bit_cast(unsigned short, h);
+            lb[u] = __builtin_bit_cast(unsigned short, (__bf16)(v[u] - (float)h));
+          }
+          h2[q] = make_uint2(hb[0] | (hb[1] << 16), hb[2] | (hb[3] << 16));
+          l2[q] = make_uint2(lb[0] | (lb[1] << 16), lb[2] | (lb[3] << 16));
+        }
+      }
+      for (int kk = 0; kk < kc && !vec4; ++kk) {
         const int ab = key_ab[k0 + kk];
         const int a = ab >> 8, b = ab & 255;
         const T* sk = stage + kk * KB;
@@ -752,7 +793,7 @@ __global__ __launch_bounds__(kThreads) void power_spectrum_kernel(int N, int L, 
   const int KB = N * N * L1;  // one key block
   extern __shared__ __align__(64) unsigned char smem_raw[];
   T* cs = reinterpret_cast<T*>(smem_raw);  // [S][N][LS], then 3 zero rows
-  T* stage = cs + (S * N + 3) * LS;        // [KC][N][N][L1]
+  T* stage = cs + align_up(size_t(S * N + 3) * LS, 4);  // [KC][N][N][L1], 16-byte aligned
   int* key_ab = reinterpret_cast<int*>(stage + KC * KB);  // [K] a << 8 | b
   int* tri = key_ab + K;  // [N(N+1)/2] n1 << 8 | n2 of the packed diagonal pairs
   const T* ci = c + i * int64_t(S) * B;
@@ -3193,7 +3234,7 @@ void launch_power_spectrum_impl(const RadialDev& rad, int S, int64_t n, const T*
   if (n == 0) return;
   const int K = S * (S + 1) / 2;
   if (S > kMaxSpecies) throw InvalidArgument("at most 64 species");
-  const size_t cbytes = sizeof(T) * (size_t(S) * rad.N + 3) * (rad.LL | 1);
+  const size_t cbytes = sizeof(T) * align_up((size_t(S) * rad.N + 3) * (rad.LL | 1), 4);
   const size_t kb = sizeof(T) * size_t(rad.N) * rad.N * rad.L1;
   int KC = cbytes + kb < kPsBudget ? int((kPsBudget - cbytes) / kb) : 1;
   if (tiles.kc > 0) KC = tiles.kc;
