@@ -69,6 +69,7 @@ extern "C" {
 #define RGPU_INFO_HALF_LIST 20     /* 1: the expansion runs over the half neighbour list */
 #define RGPU_INFO_FUSED 21         /* 1: expansion and invariants in one persistent launch */
 #define RGPU_INFO_FWD_LDS 22       /* 1: the expansion accumulates c in LDS and writes it once */
+#define RGPU_INFO_FWD_GROUP 23     /* 1: the FP32 expansion runs on pair groups where it applies */
 
 /* stages timed by rgpu_enable_timing */
 #define RGPU_STAGE_NEIGHBOURS 0
@@ -115,7 +116,8 @@ int rgpu_autotune(rgpu_context* ctx);
  * are built for max_radial 12 / max_angular 9 and 8 / 6 and up to 8
  * species, kind 0 runs elsewhere); fused: expansion and invariants in one persistent launch (1),
  * two launches (0), two launches with the expansion writing c once from
- * LDS (2), built-in (-1: two launches)), e.g. to reuse an autotuned
+ * LDS (2), two launches with the FP32 expansion on pair groups (3),
+ * built-in (-1: two launches)), e.g. to reuse an autotuned
  * set.  Results agree between partitions to rounding.  No counterpart in
  * the reference. */
 int rgpu_set_tiles(rgpu_context* ctx, int pairs_fwd, int keys_ps, int pairs_adj, int keys_adj,

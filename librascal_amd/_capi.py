@@ -26,7 +26,7 @@ INFO = dict(max_radial=0, max_angular=1, n_species=2, feature_size=3, n_centres=
             n_pairs=5, n_struct=6, spline_grid=7, n_sparse=8, is_invariants=9, precision=10,
             graph_captures=11, graph_launches=12, tile_pairs_fwd=13, tile_keys_ps=14,
             tile_pairs_adj=15, tile_keys_adj=16, nl_graph_launches=17, n_atoms=18,
-            tile_adj_kind=19, half_list=20, fused=21, fwd_lds=22)
+            tile_adj_kind=19, half_list=20, fused=21, fwd_lds=22, fwd_group=23)
 
 _lib = None
 
@@ -188,7 +188,8 @@ class Context:
                     pairs_adj=self.info("tile_pairs_adj"),
                     keys_adj=self.info("tile_keys_adj"), adj_kind=self.info("tile_adj_kind"),
                     half_list=self.info("half_list"),
-                    fused=2 if self.info("fwd_lds") else self.info("fused"))
+                    fused=2 if self.info("fwd_lds") else (3 if self.info("fwd_group")
+                                                          else self.info("fused")))
 
     def set_tiles(self, pairs_fwd=0, keys_ps=0, pairs_adj=0, keys_adj=0, adj_kind=3, fused=-1):
         """Set the work partitions of the current precision (see tiles())."""

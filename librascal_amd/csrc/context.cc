@@ -61,7 +61,6 @@ struct Context {
   DeviceBuffer<YGradTerm<float>> d_ygrad32;
   DeviceBuffer<double> d_tab_h, d_tab_pu;
   DeviceBuffer<int64_t> d_atom_off;
-  DeviceBuffer<int> d_centre_struct;
   PairList pl;
   // a deferred size check found the list too small and no rebuild followed
   // (the allocation-free path cannot rebuild): the next check rebuilds
@@ -85,6 +84,12 @@ struct Context {
   DeviceBuffer<double> gfull;        // dL/dX scattered into the full layout
   DeviceBuffer<double> dcp, dcs;     // stored expansion derivatives (gradient export)
   DeviceBuffer<float> adj_work;      // dL/dc between the adjoint's two kernels (adj_kind 3)
+  // deterministic force sum: the adjoint's pair and centre forces, and the
+  // reverse pair lists of the current neighbour list (rev_valid)
+  DeviceBuffer<double> pair_force, centre_force;
+  DeviceBuffer<int> rev_count;
+  DeviceBuffer<int64_t> rev_off, rev_list, rev_scan;
+  bool rev_valid = false;
   // half neighbour list (rgpu_set_half_list) and its contribution rows
   DeviceBuffer<int> half_count, half_j;
   DeviceBuffer<int64_t> half_off, half_src, half_scan;
@@ -112,6 +117,7 @@ struct Context {
 
   // scratch
   DeviceBuffer<double> e_centre, grad, cvir, tmp64, out64, knm_grad, knm_scale;
+  DeviceBuffer<double> e_part;  // energy GEMM partials per column tile and centre
   PinnedBuffer<double> out_stage;  // page-locked staging of a prediction's outputs
   DeviceBuffer<double> d_inv_vol;  // [n_struct] 1 / cell volume
   std::vector<double> inv_vol;     // host copy of the uploaded values
@@ -133,7 +139,7 @@ struct Context {
   bool use_graphs = false;
   hipStream_t own_stream = nullptr;  // created for capture when no stream was given
   hipGraphExec_t rep_exec = nullptr;
-  std::array<uintptr_t, 26> rep_sig{};
+  std::array<uintptr_t, 27> rep_sig{};
   int64_t graph_captures = 0, graph_launches = 0;
   void drop_graph() {
     if (rep_exec) (void)hipGraphExecDestroy(rep_exec);
@@ -486,6 +492,7 @@ bool settle_nl(Context& c) {
   c.nl_stale = false;
   rebuild_pair_list(c.pl, c.hp.cutoff, c.stream);
   c.have_rep = false;
+  c.rev_valid = false;
   return false;
 }
 
@@ -502,6 +509,7 @@ double cell_volume(const double* C) {
 void build_batch_nl(Context& c, const double* positions, bool defer = false) {
   StageTimer t(c, RGPU_STAGE_NEIGHBOURS);
   c.have_rep = false;
+  c.rev_valid = false;
   if (c.n_atoms == 0) {
     // empty batch: only the species list (feature layout) is defined
     c.pl.n_atoms = c.pl.n_pairs = 0;
@@ -532,6 +540,23 @@ PairView pair_view(Context& c) {
   pv.species_of = c.d_species_of.get();
   pv.pair_fc = c.pair_fc.get();
   pv.pair_dfc = c.pair_dfc.get();
+  return pv;
+}
+
+// The pair view of an adjoint launch: room for its pair and centre forces,
+// and the reverse pair lists of the current list (built once per list)
+PairView backward_view(Context& c, ReverseLists* rl) {
+  PairView pv = pair_view(c);
+  const int64_t n = c.n_atoms, cap = std::max<int64_t>(n > 0 ? c.pl.cap_pairs : 0, 1);
+  pv.pair_force = c.pair_force.reserve(size_t(3 * cap));
+  pv.centre_force = c.centre_force.reserve(size_t(std::max<int64_t>(3 * n, 1)));
+  rl->count = c.rev_count.reserve(size_t(std::max<int64_t>(n, 1)));
+  rl->roff = c.rev_off.reserve(size_t(n + 1));
+  rl->rlist = c.rev_list.reserve(size_t(cap));
+  if (!c.rev_valid) {
+    launch_reverse_pairs(pv, *rl, c.rev_scan, c.stream);
+    c.rev_valid = true;
+  }
   return pv;
 }
 
@@ -635,7 +660,7 @@ void compute_rep(Context& c) {
       if (xs) launch_sparsify(n, D, c.D(), c.d_sel_col.get(), pp, xs, n2, c.stream);
     }
   };
-  std::array<uintptr_t, 26> sig{};
+  std::array<uintptr_t, 27> sig{};
   bool replay = c.use_graphs && !c.timing && n > 0;
   if (replay) {
     sig = {
@@ -645,7 +670,8 @@ void compute_rep(Context& c) {
         uintptr_t(c.pl.pair_j.get()), uintptr_t(c.pl.pair_vec.get()), uintptr_t(c.pl.pair_dist.get()),
         uintptr_t(c.d_species_of.get()), uintptr_t(c.tile().pb_fwd), uintptr_t(c.tile().kc),
         uintptr_t(c.d_sel_col.get()), uintptr_t(c.tile().half_nl), uintptr_t(hl.src),
-        uintptr_t(hl.rows), uintptr_t(c.tile().fused), uintptr_t(c.tile().fwd_lds)};
+        uintptr_t(hl.rows), uintptr_t(c.tile().fused), uintptr_t(c.tile().fwd_lds),
+        uintptr_t(c.tile().fwd_group)};
     // (instantiating a graph allocates: the allocation-free path launches
     // the stage eagerly instead of capturing it)
     if ((!c.rep_exec || sig != c.rep_sig) && g_no_alloc) replay = false;
@@ -690,13 +716,13 @@ void compute_rep(Context& c) {
 // (callers other than the prediction have settled the neighbour list)
 void compute(Context& c);
 
-// The allocation-free GAP energy keeps its per-centre energies in this
-// file-scope device array (each device holds its own copy until the process
-// ends).  Host code never uses the array's name as an address: the address
+// The allocation-free GAP energy keeps its per-centre energies and the
+// energy GEMM's partial sums in this file-scope device array (each device
+// holds its own copy until the process ends).  Host code never uses the array's name as an address: the address
 // on each device comes from hipGetSymbolAddress, once per device, with that
 // device current.
-constexpr int64_t kScratchCentres = int64_t(1) << 21;  // 16 MiB of doubles
-static __device__ __align__(256) double scratch[kScratchCentres];
+constexpr int64_t kScratchDoubles = int64_t(1) << 23;  // 64 MiB
+static __device__ __align__(256) double scratch[kScratchDoubles];
 
 double* scratch_on_current_device() {
   static std::mutex mu;
@@ -823,6 +849,8 @@ void backward_device(Context& c, const T* G, double* grad, double* cvir, const d
                      double xg_scale = 1.0) {
   const int S = int(c.species.size());
   StageTimer t(c, RGPU_STAGE_BACKWARD);
+  ReverseLists rl;
+  const PairView pv = backward_view(c, &rl);
   if constexpr (std::is_same<T, double>::value) {
     if (c.sparsified()) {
       double* gf = c.gfull.reserve(size_t(std::max<int64_t>(c.n_atoms * c.D_full(), 1)));
@@ -834,13 +862,13 @@ void backward_device(Context& c, const T* G, double* grad, double* cvir, const d
     float* work = nullptr;
     if (c.tile().adj_kind == 3)
       work = c.adj_work.reserve(size_t(std::max<int64_t>(c.n_atoms * S * c.rd.N * c.rd.LL, 1)));
-    launch_backward_split(c.rd, S, pair_view(c), c.c32.get(), c.p_hi.get(), c.p_lo.get(),
-                          c.norm2.get(), c.hp.normalize, xg, xg_scale, G, grad, cvir, c.stream,
-                          c.tile(), work);
+    launch_backward_split(c.rd, S, pv, c.c32.get(), c.p_hi.get(), c.p_lo.get(), c.norm2.get(),
+                          c.hp.normalize, xg, xg_scale, G, cvir, c.stream, c.tile(), work);
+  } else {
+    launch_backward<T>(c.rd, S, pv, cbuf<T>(c).get(), pbuf<T>(c).get(), c.norm2.get(),
+                       c.hp.normalize, xg, xg_scale, G, cvir, c.stream, c.tile());
   }
-  else
-    launch_backward<T>(c.rd, S, pair_view(c), cbuf<T>(c).get(), pbuf<T>(c).get(), c.norm2.get(),
-                       c.hp.normalize, xg, xg_scale, G, grad, cvir, c.stream, c.tile());
+  launch_force_gather(pv, rl, grad, c.stream);
 }
 
 // FP32 invariants of the mixed path in the full layout (for export and
@@ -916,6 +944,18 @@ void emit_outputs(Context& c, const double* e_centre, double* energy, double* gr
     if (o.n && !o.device) std::memcpy(o.dst, h + o.off, sizeof(double) * o.n);
 }
 
+// column tiles of the widest species block in the energy GEMM of precision
+// T (the partial row sums it keeps per centre)
+template <class T>
+int energy_tiles(const Context& c) {
+  int t = 0;
+  for (size_t s = 0; s + 1 < c.sp_off.size(); ++s) {
+    const int64_t m = c.sp_off[s + 1] - c.sp_off[s];
+    t = std::max(t, std::is_same<T, float>::value ? gap_mixed_energy_tiles(m) : gap_f64_energy_tiles(m));
+  }
+  return t;
+}
+
 template <class T>
 void gap_predict_t(Context& c, double* energy, double* gradient, double* neg_stress) {
   const int S = int(c.species.size());
@@ -924,6 +964,7 @@ void gap_predict_t(Context& c, double* energy, double* gradient, double* neg_str
   const bool want_grad = gradient || neg_stress;
   if (want_grad) require_unmasked(c);
   double* ec = c.e_centre.reserve(size_t(std::max<int64_t>(n, 1)));
+  double* part = c.e_part.reserve(size_t(std::max<int64_t>(n * energy_tiles<T>(c), 1)));
   RGPU_HIP(hipMemsetAsync(ec, 0, sizeof(double) * n, c.stream));
   const T* X = nullptr;
   if constexpr (std::is_same<T, double>::value) X = invariants_of<double>(c);
@@ -955,7 +996,7 @@ void gap_predict_t(Context& c, double* energy, double* gradient, double* neg_str
     const SplitPlanes xs{c.p_hi.get(), c.p_lo.get()};
     {
       StageTimer t(c, RGPU_STAGE_GAP_KERNEL);
-      run_gap_mixed_kernel(blks.data(), S, c.mixed_sparse, xs, ec, want_grad ? wts.data() : nullptr,
+      run_gap_mixed_kernel(blks.data(), S, c.mixed_sparse, xs, ec, part, n, want_grad ? wts.data() : nullptr,
                            c.stream);
     }
     if (want_grad) {
@@ -969,7 +1010,7 @@ void gap_predict_t(Context& c, double* energy, double* gradient, double* neg_str
       for (int s = 0; s < S; ++s) wts[s] = wt_all + wt_off[s];
     {
       StageTimer t(c, RGPU_STAGE_GAP_KERNEL);
-      run_gap_f64_kernel(blks.data(), S, ec, want_grad ? wts.data() : nullptr, c.stream);
+      run_gap_f64_kernel(blks.data(), S, ec, part, n, want_grad ? wts.data() : nullptr, c.stream);
     }
     if (want_grad) {
       StageTimer t(c, RGPU_STAGE_GAP_FORCE);
@@ -988,10 +1029,11 @@ void gap_predict_t(Context& c, double* energy, double* gradient, double* neg_str
 }
 
 // Per-structure GAP energies into energy (device memory) with the
-// per-centre energies in ec (the scratch array): no buffer of the context
-// grows and nothing is allocated (the caller has set the scope).
+// per-centre energies in ec and the GEMM partials in part (both in the
+// scratch array): no buffer of the context grows and nothing is allocated
+// (the caller has set the scope).
 template <class T>
-void gap_energy_noalloc_t(Context& c, double* ec, double* energy) {
+void gap_energy_noalloc_t(Context& c, double* ec, double* part, double* energy) {
   const int S = int(c.species.size());
   const int64_t n = c.n_atoms, D = std::is_same<T, float>::value ? c.Dp() : c.D();
   launch_fill(ec, n, 0.0, c.stream);
@@ -1004,9 +1046,9 @@ void gap_energy_noalloc_t(Context& c, double* ec, double* energy) {
   }
   if constexpr (std::is_same<T, float>::value)
     run_gap_mixed_kernel(blks.data(), S, c.mixed_sparse, SplitPlanes{c.p_hi.get(), c.p_lo.get()}, ec,
-                         nullptr, c.stream);
+                         part, n, nullptr, c.stream);
   else
-    run_gap_f64_kernel(blks.data(), S, ec, nullptr, c.stream);
+    run_gap_f64_kernel(blks.data(), S, ec, part, n, nullptr, c.stream);
   launch_fill(energy, c.n_struct, 0.0, c.stream);
   launch_segment_sum(ec, 1, c.d_atom_off.get(), c.n_struct, energy, c.stream);
 }
@@ -1016,16 +1058,19 @@ void gap_knm_t(Context& c, int target, double* knm) {
   const int S = int(c.species.size());
   const int64_t n = c.n_atoms;
   const int64_t rows = target == 0 ? c.n_struct : n;
-  double* k = c.tmp64.reserve(size_t(std::max<int64_t>(rows * c.M, 1)));
-  RGPU_HIP(hipMemsetAsync(k, 0, sizeof(double) * rows * c.M, c.stream));
+  // rows per centre; the Structure rows are their sums over each
+  // structure's centres, in centre order (no atomics: the same on every run)
+  double* ka = c.tmp64.reserve(size_t(std::max<int64_t>(n * c.M, 1)));
+  double* k = target == 0 ? c.out64.reserve(size_t(std::max<int64_t>(rows * c.M, 1))) : ka;
+  RGPU_HIP(hipMemsetAsync(ka, 0, sizeof(double) * n * c.M, c.stream));
   const T* X = invariants_of<T>(c);
   for (int s = 0; s < S; ++s) {
     GapBlock<T> blk = gap_block<T>(c, s, X, sparse_values<T>(c));
     blk.alpha = nullptr;
     if (blk.n_rows == 0 || blk.n_cols == 0) continue;
-    launch_gap_kernel<T>(blk, c.M, nullptr, nullptr, target == 1 ? k : nullptr,
-                         target == 0 ? k : nullptr, c.d_centre_struct.get(), c.stream);
+    launch_gap_kernel<T>(blk, c.M, nullptr, nullptr, ka, c.stream);
   }
+  if (target == 0) launch_rows_segment_sum(ka, c.M, c.d_atom_off.get(), c.n_struct, k, c.stream);
   // columns back to the order the sparse points were given; Atom rows of
   // the centres only
   std::vector<double> h(static_cast<size_t>(rows * c.M));
@@ -1102,6 +1147,8 @@ void autotune_t(Context& c) {
     for (int pb : {8, 16, 32, 48, 64}) {
       SoapTiles cand = t;
       cand.pb_fwd = pb;
+      cand.fwd_group = 0;  // pairs per batch of the per-entry expansion
+      cand.fwd_lds = 0;
       float ms;
       try {
         ms = time_ms([&] { launch_expansion<T>(c.rd, S, pv, cc, km, c.stream, cand); });
@@ -1111,12 +1158,19 @@ void autotune_t(Context& c) {
       if (ms < best) best = ms, pick = pb;
     }
     t.pb_fwd = pick;
-    // the expansion writing c once from LDS
+    // the pair-group expansion against the per-entry one, then the
+    // expansion writing c once from LDS
     {
-      SoapTiles reg = t, lds = t;
+      SoapTiles reg = t, lds = t, ent = t;
       reg.fwd_lds = 0;
+      reg.fwd_group = 1;
+      ent.fwd_lds = 0;
+      ent.fwd_group = 0;
       lds.fwd_lds = 1;
-      const float ms_reg = time_ms([&] { launch_expansion<T>(c.rd, S, pv, cc, km, c.stream, reg); });
+      float ms_reg = time_ms([&] { launch_expansion<T>(c.rd, S, pv, cc, km, c.stream, reg); });
+      const float ms_ent = time_ms([&] { launch_expansion<T>(c.rd, S, pv, cc, km, c.stream, ent); });
+      if (ms_ent < ms_reg) reg = ent, ms_reg = ms_ent;
+      t = reg;
       float ms_lds = 1e30f;
       for (int pb : {16, 32}) {
         SoapTiles cand = lds;
@@ -1219,23 +1273,23 @@ void autotune_t(Context& c) {
     // adjoint on a zero dL/dX (its time does not depend on the values)
     const int64_t D = std::max(c.D(), c.Dp());
     DeviceBuffer<T> G;
-    DeviceBuffer<double> gr, cv;
+    DeviceBuffer<double> cv;
     T* g = G.reserve(size_t(n * D));
     RGPU_HIP(hipMemsetAsync(g, 0, sizeof(T) * n * D, c.stream));
-    double* grad = gr.reserve(size_t(3 * n));
     double* cvir = cv.reserve(size_t(6 * n));
-    RGPU_HIP(hipMemsetAsync(grad, 0, sizeof(double) * 3 * n, c.stream));
+    ReverseLists rl;
+    const PairView bpv = backward_view(c, &rl);
     float* work = split ? c.adj_work.reserve(size_t(std::max<int64_t>(n * S * c.rd.N * c.rd.LL, 1)))
                         : nullptr;
     auto time_adjoint = [&](const SoapTiles& cand) {
       try {
         return time_ms([&] {
           if constexpr (split)
-            launch_backward_split(c.rd, S, pv, cc, ph, pl, n2, c.hp.normalize, nullptr, 1.0, g,
-                                  grad, cvir, c.stream, cand, cand.adj_kind == 3 ? work : nullptr);
+            launch_backward_split(c.rd, S, bpv, cc, ph, pl, n2, c.hp.normalize, nullptr, 1.0, g,
+                                  cvir, c.stream, cand, cand.adj_kind == 3 ? work : nullptr);
           else
-            launch_backward<T>(c.rd, S, pv, cc, pp, n2, c.hp.normalize, nullptr, 1.0, g, grad,
-                               cvir, c.stream, cand);
+            launch_backward<T>(c.rd, S, bpv, cc, pp, n2, c.hp.normalize, nullptr, 1.0, g, cvir,
+                               c.stream, cand);
         });
       } catch (const InvalidArgument&) {
         return 1e30f;  // needs more LDS than the device provides
@@ -1365,9 +1419,10 @@ int rgpu_set_tiles(rgpu_context* ctx, int pairs_fwd, int keys_ps, int pairs_adj,
     t.keys_adj = std::max(keys_adj, 0);
     if (adj_kind < 0 || adj_kind > 3) throw InvalidArgument("adj_kind must be 0, 1, 2 or 3");
     t.adj_kind = adj_kind;
-    t.fused = fused < 0 ? -1 : (fused ? 1 : 0);
+    if (fused > 3) throw InvalidArgument("fused must be -1, 0, 1, 2 or 3");
+    t.fused = fused < 0 ? -1 : (fused == 1 ? 1 : 0);
     t.fwd_lds = fused == 2 ? 1 : 0;
-    if (fused == 2) t.fused = 0;
+    t.fwd_group = fused == 3 ? 1 : 0;
     c.have_rep = false;
     c.drop_graph();
   });
@@ -1447,6 +1502,7 @@ int rgpu_get_info(rgpu_context* ctx, int key, int64_t* value) {
       case RGPU_INFO_HALF_LIST: *value = c.tile().half_nl; break;
       case RGPU_INFO_FUSED: *value = c.tile().fused == 1; break;
       case RGPU_INFO_FWD_LDS: *value = c.tile().fwd_lds; break;
+      case RGPU_INFO_FWD_GROUP: *value = c.tile().fwd_group; break;
       default: throw InvalidArgument("unknown info key");
     }
   });
@@ -1571,11 +1627,6 @@ int rgpu_set_structures(rgpu_context* ctx, int n_struct, const int64_t* atom_off
     int64_t* dof = c.d_atom_off.reserve(size_t(n_struct + 1));
     RGPU_HIP(hipMemcpyAsync(dof, c.atom_off.data(), sizeof(int64_t) * (n_struct + 1), hipMemcpyHostToDevice,
                             c.stream));
-    std::vector<int> cs(size_t(std::max<int64_t>(n_atoms, 1)));
-    for (int s = 0; s < n_struct; ++s)
-      for (int64_t a = atom_offsets[s]; a < atom_offsets[s + 1]; ++a) cs[a] = s;
-    int* dcs = c.d_centre_struct.reserve(cs.size());
-    RGPU_HIP(hipMemcpyAsync(dcs, cs.data(), sizeof(int) * cs.size(), hipMemcpyHostToDevice, c.stream));
     build_batch_nl(c, positions);
     build_rows(c);
     RGPU_HIP(hipStreamSynchronize(c.stream));
@@ -1876,7 +1927,7 @@ int rgpu_gap_kmm(rgpu_context* ctx, double* kmm) {
       b.alpha = nullptr;
       b.zeta = c.zeta;
       b.D = D;
-      launch_gap_kernel<double>(b, M, nullptr, nullptr, k, nullptr, nullptr, c.stream);
+      launch_gap_kernel<double>(b, M, nullptr, nullptr, k, c.stream);
       m0 = m1;
     }
     std::vector<double> h(static_cast<size_t>(M * M)), o(h.size());
@@ -1913,7 +1964,7 @@ int rgpu_gap_knm_gradient(rgpu_context* ctx, int neg_stress, double* out) {
       blk.alpha = nullptr;
       blk.zeta = 1;
       if (blk.n_rows == 0 || blk.n_cols == 0) continue;
-      launch_gap_kernel<double>(blk, M, nullptr, nullptr, d, nullptr, c.d_centre_struct.get(), c.stream);
+      launch_gap_kernel<double>(blk, M, nullptr, nullptr, d, c.stream);
     }
     double* gs = c.knm_scale.reserve(size_t(std::max<int64_t>(n, 1)));
     double* gd = c.grad.reserve(size_t(std::max<int64_t>(3 * n, 1)));
@@ -1921,20 +1972,22 @@ int rgpu_gap_knm_gradient(rgpu_context* ctx, int neg_stress, double* out) {
     double* st = c.out64.reserve(size_t(6 * std::max<int64_t>(ns, 1)));
     // a sparsified row enters the adjoint scattered into the full layout
     double* grow = c.sparsified() ? c.gfull.reserve(size_t(c.D_full())) : nullptr;
+    ReverseLists rl;
+    const PairView bpv = backward_view(c, &rl);
     // column m (species-sorted): dL/dX_i = zeta d_im^(zeta-1) T_m for the
     // centres of m's species, through the adjoint; rows written in the
     // caller's sparse-point order
     for (int s = 0; s < S; ++s)
       for (int64_t m = c.sp_off[s]; m < c.sp_off[s + 1]; ++m) {
         launch_knm_gradient_scale(n, d + m, M, c.d_species_of.get(), s, c.zeta, gs, c.stream);
-        RGPU_HIP(hipMemsetAsync(gd, 0, sizeof(double) * 3 * n, c.stream));
         const double* row = c.T64.get() + m * D;
         if (grow) {
           launch_scatter_columns(1, D, c.d_sel_col.get(), row, c.D_full(), grow, c.stream);
           row = grow;
         }
-        launch_backward_rank1(c.rd, S, pair_view(c), c.c64.get(), c.p64.get(), c.norm2.get(),
-                              c.hp.normalize, row, gs, gd, cv, c.stream, c.tile());
+        launch_backward_rank1(c.rd, S, bpv, c.c64.get(), c.p64.get(), c.norm2.get(),
+                              c.hp.normalize, row, gs, cv, c.stream, c.tile());
+        launch_force_gather(bpv, rl, gd, c.stream);
         const int64_t col = c.sp_order[size_t(m)];
         launch_strided_copy(gd, 3 * n, o + col, M, c.stream);
         if (neg_stress) {
@@ -1991,10 +2044,15 @@ int rgpu_gap_energy_noalloc(rgpu_context* ctx, double* energy) {
     require_batch_unsettled(c);
     if (!c.have_sparse) throw InvalidArgument(kNoSparse);
     if (!c.have_weights) throw InvalidArgument("sparse points were set without weights");
-    if (c.n_atoms > kScratchCentres)
-      throw InvalidArgument("rgpu_gap_energy_noalloc: the batch has " + std::to_string(c.n_atoms) +
-                            " centres, more than the " + std::to_string(kScratchCentres) +
-                            " its scratch holds (use rgpu_gap_predict)");
+    // scratch: the per-centre energies, then the energy GEMM's partial sums
+    // (one per centre and column tile of the widest species block)
+    const int64_t n = c.n_atoms, lead = (n + 31) / 32 * 32;
+    const int tiles = c.precision == RGPU_FP64 ? energy_tiles<double>(c) : energy_tiles<float>(c);
+    if (lead + n * tiles > kScratchDoubles)
+      throw InvalidArgument("rgpu_gap_energy_noalloc: the batch has " + std::to_string(n) + " centres x (1 + " +
+                            std::to_string(tiles) + " column tiles) = " + std::to_string(lead + n * tiles) +
+                            " doubles of scratch, more than the " + std::to_string(kScratchDoubles) +
+                            " it holds (use rgpu_gap_predict)");
     if (c.n_struct > 0 && !is_device_memory(energy))
       throw InvalidArgument("rgpu_gap_energy_noalloc writes device memory: energy must be a "
                             "device pointer");
@@ -2013,9 +2071,9 @@ int rgpu_gap_energy_noalloc(rgpu_context* ctx, double* energy) {
     }
     ensure_rep(c);
     if (c.precision == RGPU_FP64)
-      gap_energy_noalloc_t<double>(c, ec, energy);
+      gap_energy_noalloc_t<double>(c, ec, ec + lead, energy);
     else
-      gap_energy_noalloc_t<float>(c, ec, energy);
+      gap_energy_noalloc_t<float>(c, ec, ec + lead, energy);
   });
 }
 

@@ -36,7 +36,8 @@ struct Problem {
   int tiles_n;
   const double* norm2;
   const double* alpha;
-  double* energy;
+  double* epart;  // energy partials [tiles_n][ldp], by centre
+  int64_t ldp;
   double* wt;
   double* G;
   int64_t ldg;
@@ -196,9 +197,9 @@ __global__ __launch_bounds__(NT, 2) void f64_gemm(ProblemSet ps) {
         for (int e = 0; e < 4; ++e) esum[wn][wm * 32 + i * 16 + fq + 4 * e] = rowsum[i][e];
     }
     __syncthreads();
-    if (P.energy && tid < BM) {
+    if (P.epart && tid < BM) {
       const int64_t r = r0 + tid;
-      if (r < P.M) atomicAdd(P.energy + P.c_rows[r], esum[0][tid] + esum[1][tid]);
+      if (r < P.M) P.epart[tn * P.ldp + P.c_rows[r]] = esum[0][tid] + esum[1][tid];
     }
   } else {
 #pragma unroll
@@ -245,10 +246,14 @@ __global__ void transpose_f64_kernel(int64_t rows, int64_t cols, const double* _
 
 }  // namespace
 
-void run_gap_f64_kernel(const GapBlock<double>* blks, int n_blocks, double* energy, double* const* wts,
-                        hipStream_t stream) {
+int gap_f64_energy_tiles(int64_t n_cols) { return int((n_cols + BN - 1) / BN); }
+
+void run_gap_f64_kernel(const GapBlock<double>* blks, int n_blocks, double* energy, double* part, int64_t ld,
+                        double* const* wts, hipStream_t stream) {
+  static_assert(kMaxProblems <= EnergyTiles::kMax, "one energy reduction per grouped launch");
   for (int b0 = 0; b0 < n_blocks; b0 += kMaxProblems) {
     ProblemSet ps{};
+    EnergyTiles et;
     int64_t tiles = 0;
     for (int b = b0; b < std::min(n_blocks, b0 + kMaxProblems); ++b) {
       const GapBlock<double>& blk = blks[b];
@@ -268,13 +273,18 @@ void run_gap_f64_kernel(const GapBlock<double>* blks, int n_blocks, double* ener
       P.tiles_n = int((P.N + BN - 1) / BN);
       P.norm2 = blk.norm2;
       P.alpha = blk.alpha + blk.col0;
-      P.energy = energy;
+      P.epart = energy ? part : nullptr;
+      P.ldp = ld;
       P.wt = wts ? wts[b] : nullptr;
       tiles += ((P.M + BM - 1) / BM) * P.tiles_n;
       ps.normalize = blk.normalize;
       ps.zeta = blk.zeta;
+      et.rows[et.n] = blk.row_idx;
+      et.m[et.n] = blk.n_rows;
+      et.tiles[et.n++] = P.tiles_n;
     }
     launch(ps, tiles, 0, stream);
+    if (energy) launch_energy_tiles(et, part, ld, energy, stream);
   }
 }
 

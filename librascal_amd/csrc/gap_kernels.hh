@@ -28,11 +28,30 @@ struct GapBlock {
 };
 
 // GEMM1 + epilogue.  Any of energy (per centre, +=), wt ([n_rows][n_cols]),
-// knm_atom ([n_centres][M], written), knm_struct ([n_struct][M], +=, with
-// centre_struct) may be null.
+// knm_atom ([n_centres][M], written) may be null.
 template <class T>
 void launch_gap_kernel(const GapBlock<T>& blk, int64_t M, double* energy, T* wt, double* knm_atom,
-                       double* knm_struct, const int* centre_struct, hipStream_t stream);
+                       hipStream_t stream);
+
+// out[s][:] = sum over rows a in [seg_off[s], seg_off[s+1]) of in[a][:]
+// (rows of w doubles, summed in row order; out written)
+void launch_rows_segment_sum(const double* in, int64_t w, const int64_t* seg_off, int n_seg, double* out,
+                             hipStream_t stream);
+
+// Per-centre energies without atomics: a GEMM's energy epilogue stores the
+// partial row sum of its column tile t at part[t * ld + centre], and
+// launch_energy_tiles adds the sum over the tiles, taken in tile order, to
+// energy[centre] (the same value on every run).  One entry per species
+// block: its centres and its number of column tiles.
+struct EnergyTiles {
+  static constexpr int kMax = 16;
+  int n = 0;
+  const int* rows[kMax] = {};
+  int64_t m[kMax] = {};
+  int tiles[kMax] = {};
+};
+void launch_energy_tiles(const EnergyTiles& et, const double* part, int64_t ld, double* energy,
+                         hipStream_t stream);
 
 // GEMM2: G[row_idx[r]][:] = sum_m wt[r][m] Tm[col0+m][:]
 template <class T>

@@ -29,8 +29,7 @@ __device__ __forceinline__ T pow_zeta_dev(T v, int zeta) {
 
 template <class T>
 __global__ __launch_bounds__(kGT) void gap_kernel_tile(GapBlock<T> blk, int64_t M, double* energy,
-                                                       T* wt, double* knm_atom, double* knm_struct,
-                                                       const int* centre_struct) {
+                                                       T* wt, double* knm_atom) {
   __shared__ T As[kTK][kTM + 1];
   __shared__ T Bs[kTK][kTN + 1];
   __shared__ double esum[kTM][kTN / 4 + 1];
@@ -96,7 +95,6 @@ __global__ __launch_bounds__(kGT) void gap_kernel_tile(GapBlock<T> blk, int64_t 
           }
         }
         if (knm_atom) knm_atom[int64_t(c) * M + mg] = double(kz);
-        if (knm_struct) atomicAdd(knm_struct + int64_t(centre_struct[c]) * M + mg, double(kz));
       }
     }
     esum[ty * 4 + u][tx] = erow;
@@ -170,11 +168,10 @@ __global__ __launch_bounds__(kGT) void gap_force_tile(GapBlock<T> blk, const T* 
 
 template <class T>
 void launch_gap_kernel(const GapBlock<T>& blk, int64_t M, double* energy, T* wt, double* knm_atom,
-                       double* knm_struct, const int* centre_struct, hipStream_t stream) {
+                       hipStream_t stream) {
   if (blk.n_rows == 0 || blk.n_cols == 0) return;
   dim3 grid(unsigned((blk.n_cols + kTN - 1) / kTN), unsigned((blk.n_rows + kTM - 1) / kTM));
-  hipLaunchKernelGGL(gap_kernel_tile<T>, grid, dim3(kGT), 0, stream, blk, M, energy, wt, knm_atom,
-                     knm_struct, centre_struct);
+  hipLaunchKernelGGL(gap_kernel_tile<T>, grid, dim3(kGT), 0, stream, blk, M, energy, wt, knm_atom);
   RGPU_LAUNCH_CHECK();
 }
 
@@ -186,10 +183,55 @@ void launch_gap_force_contraction(const GapBlock<T>& blk, const T* wt, T* G, hip
   RGPU_LAUNCH_CHECK();
 }
 
+namespace {
+
+__global__ void energy_tiles_kernel(EnergyTiles et, const double* __restrict__ part, int64_t ld,
+                                    double* __restrict__ energy) {
+  int64_t r = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  int pi = 0;
+  while (pi < et.n && r >= et.m[pi]) r -= et.m[pi++];
+  if (pi >= et.n) return;
+  const int c = et.rows[pi][r];
+  double s = 0.0;
+  for (int t = 0; t < et.tiles[pi]; ++t) s += part[int64_t(t) * ld + c];
+  energy[c] += s;
+}
+
+// out[s][m] = sum of in[a][m] over the rows a of segment s, in row order
+__global__ void rows_segment_sum_kernel(const double* __restrict__ in, int64_t w,
+                                        const int64_t* __restrict__ seg_off, double* __restrict__ out) {
+  const int64_t m = int64_t(blockIdx.y) * blockDim.x + threadIdx.x, s = blockIdx.x;
+  if (m >= w) return;
+  double t = 0.0;
+  for (int64_t a = seg_off[s]; a < seg_off[s + 1]; ++a) t += in[a * w + m];
+  out[s * w + m] = t;
+}
+
+}  // namespace
+
+void launch_rows_segment_sum(const double* in, int64_t w, const int64_t* seg_off, int n_seg, double* out,
+                             hipStream_t stream) {
+  if (w == 0 || n_seg == 0) return;
+  // segments along x (no 65535 limit), column chunks along y
+  if ((w + 255) / 256 > 65535) throw InvalidArgument("rows of more than 16776960 columns");
+  hipLaunchKernelGGL(rows_segment_sum_kernel, dim3(unsigned(n_seg), unsigned((w + 255) / 256)), dim3(256), 0,
+                     stream, in, w, seg_off, out);
+  RGPU_LAUNCH_CHECK();
+}
+
+void launch_energy_tiles(const EnergyTiles& et, const double* part, int64_t ld, double* energy,
+                         hipStream_t stream) {
+  int64_t rows = 0;
+  for (int i = 0; i < et.n; ++i) rows += et.m[i];
+  if (rows == 0) return;
+  hipLaunchKernelGGL(energy_tiles_kernel, dim3(unsigned((rows + 255) / 256)), dim3(256), 0, stream, et, part,
+                     ld, energy);
+  RGPU_LAUNCH_CHECK();
+}
+
 template void launch_gap_kernel<double>(const GapBlock<double>&, int64_t, double*, double*, double*,
-                                        double*, const int*, hipStream_t);
-template void launch_gap_kernel<float>(const GapBlock<float>&, int64_t, double*, float*, double*, double*,
-                                       const int*, hipStream_t);
+                                        hipStream_t);
+template void launch_gap_kernel<float>(const GapBlock<float>&, int64_t, double*, float*, double*, hipStream_t);
 template void launch_gap_force_contraction<double>(const GapBlock<double>&, const double*, double*,
                                                    hipStream_t);
 template void launch_gap_force_contraction<float>(const GapBlock<float>&, const float*, float*,

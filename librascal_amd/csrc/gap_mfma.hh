@@ -36,10 +36,14 @@ void prepare_mixed_sparse(MixedSparse& ms, const std::vector<double>& T, int64_t
 // K tile GEMM + epilogue for all species blocks in grouped launches.  The
 // features are given split: X = xs.hi + xs.lo, [n_centres][D] (rows gathered
 // through the blocks' row_idx, un-normalised, scaled by 1/|p| in the
-// epilogue).  Outputs: energy (per centre, +=) and the weighted kernel
+// epilogue).  Outputs: energy (per centre, +=; summed over the column
+// tiles through part, [gap_mixed_energy_tiles(n_cols) of the widest block]
+// x [ld >= n_centres] doubles, see EnergyTiles) and the weighted kernel
 // gradient W~ of each block, split, [n_rows][n_cols] (wts may be null).
+int gap_mixed_energy_tiles(int64_t n_cols);
 void run_gap_mixed_kernel(const GapBlock<float>* blks, int n_blocks, const MixedSparse& ms,
-                          SplitPlanes xs, double* energy, const SplitPlanes* wts, hipStream_t stream);
+                          SplitPlanes xs, double* energy, double* part, int64_t ld, const SplitPlanes* wts,
+                          hipStream_t stream);
 // T~ rows: G[row_idx[r]][:] = sum_m W~_b[r][m] T[col0+m][:]
 void run_gap_mixed_force(const GapBlock<float>* blks, int n_blocks, const MixedSparse& ms,
                          const SplitPlanes* wts, float* G, hipStream_t stream);

@@ -60,7 +60,8 @@ struct Problem {
   // kernel epilogue
   const double* norm2;
   const double* alpha;         // offset to the block's first sparse point
-  double* energy;
+  double* epart;               // energy partials [tiles_n][ldp], by centre
+  int64_t ldp;
   unsigned short* wth;         // [M][N] W~ split (may be null)
   unsigned short* wtl;
   // force epilogue
@@ -174,9 +175,9 @@ __device__ __forceinline__ void gemm_epilogue(const ProblemSet& ps, const Proble
           esum[wn][wm * 64 + i * 32 + 8 * (e >> 2) + 4 * half + (e & 3)] = rowsum[i][e];
     }
     __syncthreads();
-    if (P.energy && tid < TBM) {
+    if (P.epart && tid < TBM) {
       const int64_t r = r0 + tid;
-      if (r < P.M) atomicAdd(P.energy + P.c_rows[r], esum[0][tid] + esum[1][tid]);
+      if (r < P.M) P.epart[(c0 / BN) * P.ldp + P.c_rows[r]] = esum[0][tid] + esum[1][tid];
     }
   } else {
 #pragma unroll
@@ -617,10 +618,15 @@ void prepare_mixed_sparse(MixedSparse& ms, const std::vector<double>& T, int64_t
   RGPU_HIP(hipStreamSynchronize(stream));
 }
 
+int gap_mixed_energy_tiles(int64_t n_cols) { return int((n_cols + BN - 1) / BN); }
+
 void run_gap_mixed_kernel(const GapBlock<float>* blks, int n_blocks, const MixedSparse& ms,
-                          SplitPlanes xs, double* energy, const SplitPlanes* wts, hipStream_t stream) {
+                          SplitPlanes xs, double* energy, double* part, int64_t ld, const SplitPlanes* wts,
+                          hipStream_t stream) {
+  static_assert(kMaxProblems <= EnergyTiles::kMax, "one energy reduction per grouped launch");
   for (int b0 = 0; b0 < n_blocks; b0 += kMaxProblems) {
     ProblemSet ps{};
+    EnergyTiles et;
     for (int b = b0; b < std::min(n_blocks, b0 + kMaxProblems); ++b) {
       const GapBlock<float>& blk = blks[b];
       if (blk.n_rows == 0 || blk.n_cols == 0) continue;
@@ -639,14 +645,19 @@ void run_gap_mixed_kernel(const GapBlock<float>* blks, int n_blocks, const Mixed
       P.norm2 = blk.norm2;
       if (!blk.alpha) throw InvalidArgument("the GAP energy kernel needs weights");
       P.alpha = blk.alpha + blk.col0;
-      P.energy = energy;
+      P.epart = energy ? part : nullptr;
+      P.ldp = ld;
       P.wth = wts ? wts[b].hi : nullptr;
       P.wtl = wts ? wts[b].lo : nullptr;
       finish_problem(P);
       ps.normalize = blk.normalize;
       ps.zeta = blk.zeta;
+      et.rows[et.n] = blk.row_idx;
+      et.m[et.n] = blk.n_rows;
+      et.tiles[et.n++] = gap_mixed_energy_tiles(blk.n_cols);
     }
     launch_set<0>(ps, stream);
+    if (energy) launch_energy_tiles(et, part, ld, energy, stream);
   }
 }
 

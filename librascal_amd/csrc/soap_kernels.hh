@@ -21,7 +21,27 @@ struct PairView {
   const int* species_of;      // [n_atoms] species index in the dense layout
   const double* pair_fc;      // [n_pairs] cutoff function at the pair distance
   const double* pair_dfc;     // [n_pairs] its derivative
+  // adjoint outputs of the deterministic force sum: the force of pair p on
+  // its neighbour [n_pairs][3], and minus the sum over a centre's pairs
+  // (periodic images of the centre itself left out) [n_centres][3]
+  double* pair_force = nullptr;
+  double* centre_force = nullptr;
 };
+
+// Reverse pair lists for the deterministic force sum: for atom j, the pairs
+// p (self pairs left out) whose neighbour is j, in no particular order:
+// rlist[roff[j] .. roff[j+1]).  count: [n] ints of scratch; roff: n + 1;
+// rlist has room for every pair of the list's capacity.
+struct ReverseLists {
+  int* count = nullptr;
+  int64_t* roff = nullptr;
+  int64_t* rlist = nullptr;
+};
+void launch_reverse_pairs(const PairView& pv, const ReverseLists& rl, DeviceBuffer<int64_t>& scan_scratch,
+                          hipStream_t stream);
+// grad[j] = centre_force[j] + sum over rlist of j of pair_force[p], in
+// ascending pair order (the forces of the adjoint, summed in a fixed order)
+void launch_force_gather(const PairView& pv, const ReverseLists& rl, double* grad, hipStream_t stream);
 
 // pair_fc / pair_dfc of the first *n_pairs (device value) of cap pairs
 // (cutoff_functions.hh:69-283)
@@ -43,6 +63,7 @@ struct SoapTiles {
   int half_nl = 0;   // expansion over the half neighbour list (1) or the full list (0)
   int fused = -1;    // power spectrum in the expansion's launch (1), separate (0), built-in (-1)
   int fwd_lds = 0;   // expansion accumulating c in LDS and writing it once (1)
+  int fwd_group = 0; // FP32 expansion on pair groups where it applies (1) or per entry (0)
 };
 
 // c[i][s][n][l*l+l+m] (finalised) and the per-centre key mask.
@@ -128,8 +149,9 @@ void launch_sparsify(int64_t n, int64_t D, int64_t Dsel, const int64_t* col, dou
 void launch_scatter_columns(int64_t n, int64_t Dsel, const int64_t* col, const double* src, int64_t D,
                             double* dst, hipStream_t stream);
 
-// Adjoint (FP64): given G = dL/dX per centre (X the exported features), accumulate
-// dL/dr on atoms (grad[n_atoms][3], +=) and write per-centre virials
+// Adjoint (FP64): given G = dL/dX per centre (X the exported features), write
+// the pair and centre forces of dL/dr (pv.pair_force, pv.centre_force;
+// launch_force_gather sums them per atom) and per-centre virials
 // centre_virial[n_centres][6] (sum over the centre's pairs of r_ji (x)
 // dL/dr_j, Voigt xx,yy,zz,yz,xz,xy, not divided by the volume).  xg, when
 // not null, holds X_i . G_i per centre (times xg_scale); otherwise it is
@@ -137,7 +159,7 @@ void launch_scatter_columns(int64_t n, int64_t Dsel, const int64_t* col, const d
 template <class T>
 void launch_backward(const RadialDev& rad, int S, const PairView& pv, const T* c, const T* p,
                      const double* norm2, bool normalize, const double* xg, double xg_scale,
-                     const T* G, double* grad, double* centre_virial, hipStream_t stream,
+                     const T* G, double* centre_virial, hipStream_t stream,
                      const SoapTiles& tiles = SoapTiles());
 
 // launch_backward<double> for a rank-one dL/dX: G_i = Gscale[i] * Grow
@@ -145,7 +167,7 @@ void launch_backward(const RadialDev& rad, int S, const PairView& pv, const T* c
 // skipped and get a zero virial).  Used for the K_NM gradient rows.
 void launch_backward_rank1(const RadialDev& rad, int S, const PairView& pv, const double* c,
                            const double* p, const double* norm2, bool normalize, const double* Grow,
-                           const double* Gscale, double* grad, double* centre_virial,
+                           const double* Gscale, double* centre_virial,
                            hipStream_t stream, const SoapTiles& tiles = SoapTiles());
 
 // Pack dL/dX [n][D] (full layout, FP64) into the packed layout of the mixed
@@ -162,7 +184,7 @@ void launch_pack_gradient(int S, int N, int L1, int64_t n, const double* G, floa
 void launch_backward_split(const RadialDev& rad, int S, const PairView& pv, const float* c,
                            const unsigned short* p_hi, const unsigned short* p_lo,
                            const double* norm2, bool normalize, const double* xg, double xg_scale,
-                           const float* G, double* grad, double* centre_virial, hipStream_t stream,
+                           const float* G, double* centre_virial, hipStream_t stream,
                            const SoapTiles& tiles = SoapTiles(), float* work = nullptr);
 
 // scale[i] = zeta d[i*ld]^(zeta-1) for the centres of species s, else 0

@@ -24,6 +24,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <string>
 #include <utility>
 #include <vector>
@@ -955,8 +956,7 @@ __global__ void normalise_kernel(int64_t n, int64_t D, const T* __restrict__ p,
 template <class T, bool DIRECT>
 __device__ __forceinline__ void adjoint_pairs(const RadialDev& rad, int S, int PB, const PairView& pv,
                                               const T* yc, const T* dc, unsigned char* region,
-                                              int64_t i, double* __restrict__ grad,
-                                              double* __restrict__ centre_virial) {
+                                              int64_t i, double* __restrict__ centre_virial) {
   const int N = rad.N, L1 = rad.L1, LL = rad.LL, F = N * L1, B = N * LL;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   // lanes per pair: the largest power of two in [16, 64] that still gives
@@ -1053,9 +1053,10 @@ __device__ __forceinline__ void adjoint_pairs(const RadialDev& rad, int S, int P
                            A * vy / d + sc * double(pb.res[4 * q + 2]),
                            A * vz / d + sc * double(pb.res[4 * q + 3])};
       const int j = pb.pj[q];
-      unsafeAtomicAdd(grad + 3 * j, g[0]);
-      unsafeAtomicAdd(grad + 3 * j + 1, g[1]);
-      unsafeAtomicAdd(grad + 3 * j + 2, g[2]);
+      double* pf = pv.pair_force + 3 * (p0 + q0 + q);
+      pf[0] = g[0];
+      pf[1] = g[1];
+      pf[2] = g[2];
       if (j != int(i)) {
         fi[0] -= g[0];
         fi[1] -= g[1];
@@ -1078,8 +1079,7 @@ __device__ __forceinline__ void adjoint_pairs(const RadialDev& rad, int S, int P
     for (int k = 0; k < 9; ++k)
       for (int off = 32; off > 0; off >>= 1) r[k] += __shfl_down(r[k], off, 64);
     if (lane == 0) {
-      for (int k = 0; k < 3; ++k)
-        if (r[k] != 0.0) unsafeAtomicAdd(grad + 3 * i + k, r[k]);
+      for (int k = 0; k < 3; ++k) pv.centre_force[3 * i + k] = r[k];
       for (int k = 0; k < 6; ++k) centre_virial[6 * i + k] = r[3 + k];
     }
   }
@@ -1097,9 +1097,12 @@ __device__ __forceinline__ void adjoint_pairs(const RadialDev& rad, int S, int P
 //  3. per pair (i, j): g = sum_{n,lm} dL/dc[s_j][n][lm]
 //       [ (R' fc + R fc') Y r_hat + R fc grad_Y / r ]
 //     (the pair-gradient formula of calculator_spherical_expansion.hh
-//     :2306-2366 contracted with dL/dc), grad[j] += g, and the centre gets
-//     -sum g unless j is a periodic image of i; the virial sum r_ji (x) g is
-//     reduced per centre (sparse_kernel_predict.hh:354-364).
+//     :2306-2366 contracted with dL/dc) is stored per pair, and the centre
+//     keeps -sum g over the pairs whose j is not a periodic image of i;
+//     force_gather_kernel then sums, per atom, its centre term and the g of
+//     the pairs naming it, in ascending pair order (no atomics: the forces
+//     are the same on every run); the virial sum r_ji (x) g is reduced per
+//     centre (sparse_kernel_predict.hh:354-364).
 constexpr int kAdjPre = 8;  // key-block elements per thread held for the prefetch
 
 template <class T, bool DIRECT>
@@ -1108,13 +1111,16 @@ __global__ __launch_bounds__(kThreads) void adjoint_kernel(
     const T* __restrict__ p, const double* __restrict__ norm2, int normalize,
     const double* __restrict__ xg, double xg_scale, const T* __restrict__ G,
     const T* __restrict__ Grow, const double* __restrict__ Gscale,
-    double* __restrict__ grad, double* __restrict__ centre_virial) {
+    double* __restrict__ centre_virial) {
   typedef typename Vec4<T>::type V4;
   const int64_t i = blockIdx.x;
   // rank-one dL/dX (Grow != null): G_i = Gscale[i] Grow; a centre with a
   // zero scale contributes nothing
   if (Grow && Gscale[i] == 0.0) {
     if (threadIdx.x < 6) centre_virial[6 * i + threadIdx.x] = 0.0;
+    if (threadIdx.x < 3) pv.centre_force[3 * i + threadIdx.x] = 0.0;
+    const int64_t q0 = 3 * (pv.centre_off[i] + 1), q1 = 3 * pv.centre_off[i + 1];
+    for (int64_t q = q0 + threadIdx.x; q < q1; q += kThreads) pv.pair_force[q] = 0.0;
     return;
   }
   const int N = rad.N, L1 = rad.L1, LL = rad.LL, B = N * LL, E = S * B;
@@ -1248,7 +1254,7 @@ __global__ __launch_bounds__(kThreads) void adjoint_kernel(
   }
 
   // per-pair contraction; the batch reuses the region after dL/dc
-  adjoint_pairs<T, DIRECT>(rad, S, PB, pv, yc, dc, region, i, grad, centre_virial);
+  adjoint_pairs<T, DIRECT>(rad, S, PB, pv, yc, dc, region, i, centre_virial);
 }
 
 // Adjoint of the mixed path: dL/dX (G) and p in the packed layout (p as its
@@ -1272,7 +1278,7 @@ __global__ __launch_bounds__(kThreads) void adjoint_staged_kernel(
     RadialDev rad, int S, int PB, int KCH, PairView pv, const float* __restrict__ c,
     const unsigned short* __restrict__ p_hi, const unsigned short* __restrict__ p_lo,
     const double* __restrict__ norm2, int normalize, const double* __restrict__ xg, double xg_scale,
-    const float* __restrict__ G, double* __restrict__ grad, double* __restrict__ centre_virial) {
+    const float* __restrict__ G, double* __restrict__ centre_virial) {
   typedef float T;
   typedef typename Vec4<T>::type V4;
   const int64_t i = blockIdx.x;
@@ -1447,7 +1453,7 @@ __global__ __launch_bounds__(kThreads) void adjoint_staged_kernel(
   __syncthreads();
 
   // 3. pairs; the batches reuse the g region
-  adjoint_pairs<T, DIRECT>(rad, S, PB, pv, yc, cd, region, i, grad, centre_virial);
+  adjoint_pairs<T, DIRECT>(rad, S, PB, pv, yc, cd, region, i, centre_virial);
 }
 
 // ---------------------------------------------------------------------------
@@ -1489,7 +1495,7 @@ template <bool DIRECT, int MAXT>
 __global__ __launch_bounds__(kThreads, 2) void adjoint_mfma_kernel(
     RadialDev rad, int S, int PB, int KCH, PairView pv, const float* __restrict__ c,
     const double* __restrict__ norm2, int normalize, const double* __restrict__ xg, double xg_scale,
-    const float* __restrict__ G, double* __restrict__ grad, double* __restrict__ centre_virial) {
+    const float* __restrict__ G, double* __restrict__ centre_virial) {
   const int64_t i = blockIdx.x;
   const int N = rad.N, L = rad.L, L1 = rad.L1, LL = rad.LL, B = N * LL, E = S * B;
   // row stride of c in LDS, 16 mod 32: the four 16-lane groups of a B
@@ -1773,7 +1779,7 @@ __global__ __launch_bounds__(kThreads, 2) void adjoint_mfma_kernel(
     }
   }
   __syncthreads();
-  adjoint_pairs<float, DIRECT>(rad, S, PB, pv, yc, dc, region, i, grad, centre_virial);
+  adjoint_pairs<float, DIRECT>(rad, S, PB, pv, yc, dc, region, i, centre_virial);
 }
 
 // ---------------------------------------------------------------------------
@@ -1895,8 +1901,7 @@ template <int N, int L, int NR, int ROLES, int PRE, bool DIRECT, bool PAIRS>
 __global__ __launch_bounds__(kThreads, (PAIRS || ROLES > 1) ? 2 : 3) void adjoint_tile_kernel(
     RadialDev rad, int S, int PB, int max_blocks, PairView pv, const float* __restrict__ c,
     const double* __restrict__ norm2, int normalize, const double* __restrict__ xg, double xg_scale,
-    const float* __restrict__ G, double* __restrict__ grad, double* __restrict__ centre_virial,
-    float* __restrict__ dcg) {
+    const float* __restrict__ G, double* __restrict__ centre_virial, float* __restrict__ dcg) {
   constexpr int L1 = L + 1, LL = L1 * L1;
   constexpr int KB = N * N * L1, KD = N * (N + 1) / 2 * L1, KB4 = KB / 4, KD4 = KD / 4;
   constexpr int Q = adj_quads_before(L1), MT = 4 * Q, TILES = N / NR;
@@ -2176,10 +2181,9 @@ __global__ __launch_bounds__(kThreads, (PAIRS || ROLES > 1) ? 2 : 3) void adjoin
   }
   __syncthreads();
   if constexpr (PAIRS) {
-    adjoint_pairs<float, DIRECT>(rad, S, PB, pv, yc, dc, region, i, grad, centre_virial);
+    adjoint_pairs<float, DIRECT>(rad, S, PB, pv, yc, dc, region, i, centre_virial);
   } else {
     (void)PB;
-    (void)grad;
     (void)centre_virial;
     static_assert((LL * N) % 4 == 0, "dL/dc rows in whole 16-byte values");
     const int E4 = S * LL * N / 4;
@@ -2278,10 +2282,162 @@ __host__ __device__ inline size_t pair_kernel_lds(int S, int N, int L1) {
          align_up(sizeof(float) * S * LL * N, 64) + sizeof(float) * 4 * LL * (kThreads / kPairGroup);
 }
 
+// FP32 spline cell of a pair at distance d whose neighbour has species sj
+// (the cell search of pair_fetch, per-species tables included)
+__device__ __forceinline__ SplineCell<float> pair_cell(const RadialDev& rad, double d, int sj) {
+  int klo;
+  double xhi, h;
+  if (rad.sp_tab) {
+    const int t = rad.sp_tab[sj], last = rad.tab_n[t] - 2;
+    int k = int((d - rad.x1) * rad.tab_pu[t]) - 1;
+    k = k < 0 ? 0 : (k > last ? last : k);
+    xhi = rad.tab_grid[rad.tab_grid0[t] + k + 1];
+    h = rad.tab_h[t];
+    klo = rad.tab_cell0[t] + k;
+  } else {
+    int k = int((d - rad.x1) * rad.per_unit) - 1;
+    k = k < 0 ? 0 : (k > rad.search_size ? rad.search_size : k);
+    xhi = rad.grid[k + 1];
+    h = rad.h;
+    klo = k;
+  }
+  const double ca = (xhi - d) / h;
+  SplineCell<float> cell;
+  cell.klo = klo;
+  cell.a = float(ca);
+  cell.b = float(1 - ca);
+  cell.inv_h = rad.sp_tab ? float(1.0 / h) : float(1.0 / rad.h);
+  return cell;
+}
+
+// l of the packed index lm = l*l + l + m, for compile-time lm
+__host__ __device__ constexpr int l_of_lm_c(int lm) {
+  int l = 0;
+  while ((l + 1) * (l + 1) <= lm) ++l;
+  return l;
+}
+
+// ---------------------------------------------------------------------------
+// Expansion of the mixed path on pair groups (FP32 spline radial functions,
+// compile-time L).  One workgroup per centre; wavefront w takes the
+// neighbours of species s = w, w + 4, ... in list order, four at a time, one
+// per group of 16 lanes.  Lane n of a group evaluates f_c R_nl of its radial
+// channel for every l; the group's lanes m <= L form Y_lm of the pair
+// (ylm_lane) in LDS; every lane then adds f_c R_nl Y_lm to its registers
+// c[s][n][lm] for all lm.  The four groups' sums are added with shuffles in
+// a fixed order (so the result does not depend on timing), the species rows
+// pass through LDS and are written once, coalesced, with the centre term.
+// The same sums as expansion_kernel (calculator_spherical_expansion.hh
+// :2139-2300), accumulated in another order.
+template <int L>
+__global__ __launch_bounds__(kThreads) void expansion_group_kernel(RadialDev rad, int S, PairView pv,
+                                                                   float* __restrict__ cout,
+                                                                   unsigned long long* __restrict__ keymask) {
+  constexpr int L1 = L + 1, LL = L1 * L1, LLP = (LL + 3) & ~3;
+  constexpr int G = 64 / kPairGroup;  // groups per wavefront
+  const int N = rad.N;
+  const int64_t i = blockIdx.x;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int gl = lane & (kPairGroup - 1), grp = lane / kPairGroup;
+  extern __shared__ __align__(64) unsigned char smem_raw[];
+  float* yc = reinterpret_cast<float*>(smem_raw);
+  float* yg = reinterpret_cast<float*>(smem_raw + ylm_table_bytes<float>(L1));  // [waves][G][LLP]
+  float* cl = yg + kWaves * G * LLP;                                           // [S][N][LL]
+  __shared__ unsigned long long kmask;
+  const int si = pv.species_of[i];
+  load_ylm_table<float>(rad, yc);
+  if (tid == 0) kmask = 1ull << si;
+  __syncthreads();
+  const int64_t p0 = pv.centre_off[i] + 1;  // the self pair contributes the centre term only
+  const int np = int(pv.centre_off[i + 1] - p0);
+  float* Yg = yg + (wave * G + grp) * LLP;
+  const int n = gl < N ? gl : N - 1;
+  for (int s = wave; s < S; s += kWaves) {
+    float acc[LL];
+#pragma unroll
+    for (int lm = 0; lm < LL; ++lm) acc[lm] = 0.f;
+    bool any = false;
+    for (int b = 0; b < np; b += 64) {
+      const int t = b + lane;
+      const bool mine = t < np && pv.species_of[pv.pair_j[p0 + t]] == s;
+      unsigned long long m = __ballot(mine);
+      any = any || m != 0;
+      while (m) {
+        // the next G pairs of this species, one per group
+        int my = -1;
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+          if (m) {
+            const int bit = __ffsll((long long)m) - 1;
+            if (g == grp) my = bit;
+            m &= m - 1;
+          }
+        }
+        if (my >= 0) {
+          const int64_t p = p0 + b + my;
+          const double d = pv.pair_dist[p];
+          const double fcd = pv.pair_fc[p];
+          const SplineCell<float> cell = pair_cell(rad, d, s);
+          float R[L1];
+          const float fc = float(fcd);
+#pragma unroll
+          for (int l = 0; l <= L; ++l) {
+            float v;
+            spline_eval<float>(rad, cell, n * L1 + l, &v, nullptr);
+            R[l] = v * fc;
+          }
+          if (gl <= L)
+            ylm_lane(L, yc, float(pv.pair_vec[3 * p] / d), float(pv.pair_vec[3 * p + 1] / d),
+                     float(pv.pair_vec[3 * p + 2] / d), gl, Yg, 1);
+          wave_lds_sync();
+#pragma unroll
+          for (int q = 0; q < LLP / 4; ++q) {
+            const f32x4 y4 = *reinterpret_cast<const f32x4*>(Yg + 4 * q);
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+              if (4 * q + u < LL) acc[4 * q + u] += R[l_of_lm_c(4 * q + u)] * y4[u];
+          }
+          wave_lds_sync();  // Y is read before the group's next pair overwrites it
+        }
+      }
+    }
+    // the groups' partial sums, (g0 + g1) + (g2 + g3)
+#pragma unroll
+    for (int lm = 0; lm < LL; ++lm) {
+      acc[lm] += __shfl_xor(acc[lm], kPairGroup, 64);
+      acc[lm] += __shfl_xor(acc[lm], 2 * kPairGroup, 64);
+    }
+    if (grp == 0 && gl < N) {
+      float* row = cl + (s * N + gl) * LL;
+#pragma unroll
+      for (int lm = 0; lm < LL; ++lm) row[lm] = acc[lm];
+    }
+    if (lane == 0 && any) atomicOr(&kmask, 1ull << s);
+  }
+  __syncthreads();
+  const int E = S * N * LL;
+  const double inv_s4pi = 0.28209479177387814347;  // 1/sqrt(4 pi)
+  const double* centre = rad.center + (rad.sp_tab ? rad.sp_tab[si] * N : 0);
+  float* ci = cout + i * int64_t(E);
+  for (int e = tid; e < E; e += kThreads) {
+    float v = cl[e];
+    const int row = e / LL;
+    if (row / N == si && e - row * LL == 0) v += float(centre[row - si * N] * inv_s4pi);
+    ci[e] = v;
+  }
+  if (tid == 0) keymask[i] = kmask;
+}
+
+__host__ __device__ inline size_t expansion_group_lds(int S, int N, int L1) {
+  const int LL = L1 * L1, LLP = (LL + 3) & ~3;
+  return ylm_table_bytes<float>(L1) + sizeof(float) * (size_t(kWaves) * (64 / kPairGroup) * LLP +
+                                                       size_t(S) * N * LL);
+}
+
 template <int N, int L>
 __global__ __launch_bounds__(kThreads) void adjoint_pair_kernel(RadialDev rad, int S, PairView pv,
                                                                 const float* __restrict__ dcg,
-                                                                double* __restrict__ grad,
                                                                 double* __restrict__ centre_virial) {
   constexpr int L1 = L + 1, LL = L1 * L1, NG = kThreads / kPairGroup;
   static_assert(N <= kPairGroup && L1 <= kPairGroup, "one lane per radial channel and per order m");
@@ -2318,29 +2474,7 @@ __global__ __launch_bounds__(kThreads) void adjoint_pair_kernel(RadialDev rad, i
       const double fcd = pv.pair_fc[p], dfcd = pv.pair_dfc[p];
       const int j = pv.pair_j[p];
       const int sj = pv.species_of[j];
-      // spline cell of the distance (as pair_fetch / stage_pairs)
-      int klo;
-      double xhi, h;
-      if (rad.sp_tab) {
-        const int t = rad.sp_tab[sj], last = rad.tab_n[t] - 2;
-        int k = int((d - rad.x1) * rad.tab_pu[t]) - 1;
-        k = k < 0 ? 0 : (k > last ? last : k);
-        xhi = rad.tab_grid[rad.tab_grid0[t] + k + 1];
-        h = rad.tab_h[t];
-        klo = rad.tab_cell0[t] + k;
-      } else {
-        int k = int((d - rad.x1) * rad.per_unit) - 1;
-        k = k < 0 ? 0 : (k > rad.search_size ? rad.search_size : k);
-        xhi = rad.grid[k + 1];
-        h = rad.h;
-        klo = k;
-      }
-      const double ca = (xhi - d) / h;
-      SplineCell<float> cell;
-      cell.klo = klo;
-      cell.a = float(ca);
-      cell.b = float(1 - ca);
-      cell.inv_h = rad.sp_tab ? float(1.0 / h) : float(1.0 / rad.h);
+      const SplineCell<float> cell = pair_cell(rad, d, sj);
       float R[L1], dR[L1];
       const int n = gl < N ? gl : N - 1;
 #pragma unroll
@@ -2396,9 +2530,10 @@ __global__ __launch_bounds__(kThreads) void adjoint_pair_kernel(RadialDev rad, i
         const double gv[3] = {double(A) * vx / d + sc * double(Bx - dx * Sl),
                               double(A) * vy / d + sc * double(By - dy * Sl),
                               double(A) * vz / d + sc * double(Bz - dz * Sl)};
-        unsafeAtomicAdd(grad + 3 * j, gv[0]);
-        unsafeAtomicAdd(grad + 3 * j + 1, gv[1]);
-        unsafeAtomicAdd(grad + 3 * j + 2, gv[2]);
+        double* pf = pv.pair_force + 3 * p;
+        pf[0] = gv[0];
+        pf[1] = gv[1];
+        pf[2] = gv[2];
         if (j != int(i)) {
           fi[0] -= gv[0];
           fi[1] -= gv[1];
@@ -2426,7 +2561,7 @@ __global__ __launch_bounds__(kThreads) void adjoint_pair_kernel(RadialDev rad, i
     double t = 0;
     for (int w = 0; w < kWaves; ++w) t += red[w][tid];
     if (tid < 3) {
-      if (t != 0.0) unsafeAtomicAdd(grad + 3 * i + tid, t);
+      pv.centre_force[3 * i + tid] = t;
     } else {
       centre_virial[6 * i + tid - 3] = t;
     }
@@ -3000,7 +3135,7 @@ constexpr size_t kTileSplitBudget = 52 * 1024;
 
 template <int N, int L, int NR, int ROLES, int PRE, bool DIRECT, bool PAIRS>
 bool go_adjoint_tile(const RadialDev& rad, int S, const PairView& pv, const float* c, const double* norm2,
-                     bool normalize, const double* xg, double xg_scale, const float* G, double* grad,
+                     bool normalize, const double* xg, double xg_scale, const float* G,
                      double* centre_virial, float* work, hipStream_t stream, const SoapTiles& tiles) {
   constexpr int L1 = L + 1, LL = L1 * L1, F = N * L1;
   constexpr int KB = N * N * L1, KD = N * (N + 1) / 2 * L1;
@@ -3034,14 +3169,14 @@ bool go_adjoint_tile(const RadialDev& rad, int S, const PairView& pv, const floa
   auto kern = adjoint_tile_kernel<N, L, NR, ROLES, PRE, DIRECT, PAIRS>;
   allow_lds(kern, smem);
   hipLaunchKernelGGL(kern, dim3(unsigned(pv.n_centres)), dim3(kThreads), smem, stream, rad, S, PB, maxb, pv,
-                     c, norm2, int(normalize), xg, xg_scale, G, grad, centre_virial, work);
+                     c, norm2, int(normalize), xg, xg_scale, G, centre_virial, work);
   RGPU_LAUNCH_CHECK();
   if (!PAIRS) {
     const size_t ps = pair_kernel_lds(S, N, L1);
     auto pk = adjoint_pair_kernel<N, L>;
     allow_lds(pk, ps);
     hipLaunchKernelGGL(pk, dim3(unsigned(pv.n_centres)), dim3(kThreads), ps, stream, rad, S, pv,
-                       static_cast<const float*>(work), grad, centre_virial);
+                       static_cast<const float*>(work), centre_virial);
     RGPU_LAUNCH_CHECK();
   }
   return true;
@@ -3054,32 +3189,32 @@ bool go_adjoint_tile(const RadialDev& rad, int S, const PairView& pv, const floa
 // workgroup.
 template <int N, int L, int NR, bool DIRECT>
 bool launch_adjoint_tile_nl(const RadialDev& rad, int S, const PairView& pv, const float* c, const double* norm2,
-                            bool normalize, const double* xg, double xg_scale, const float* G, double* grad,
+                            bool normalize, const double* xg, double xg_scale, const float* G,
                             double* centre_virial, float* work, hipStream_t stream, const SoapTiles& tiles) {
   if (work && !DIRECT) {
     if (S <= kWaves)
-      return go_adjoint_tile<N, L, NR, 1, 6, false, false>(rad, S, pv, c, norm2, normalize, xg, xg_scale, G, grad,
+      return go_adjoint_tile<N, L, NR, 1, 6, false, false>(rad, S, pv, c, norm2, normalize, xg, xg_scale, G,
                                                            centre_virial, work, stream, tiles);
-    return go_adjoint_tile<N, L, NR, 2, 8, false, false>(rad, S, pv, c, norm2, normalize, xg, xg_scale, G, grad,
+    return go_adjoint_tile<N, L, NR, 2, 8, false, false>(rad, S, pv, c, norm2, normalize, xg, xg_scale, G,
                                                          centre_virial, work, stream, tiles);
   }
   if (S <= kWaves)
-    return go_adjoint_tile<N, L, NR, 1, 10, DIRECT, true>(rad, S, pv, c, norm2, normalize, xg, xg_scale, G, grad,
+    return go_adjoint_tile<N, L, NR, 1, 10, DIRECT, true>(rad, S, pv, c, norm2, normalize, xg, xg_scale, G,
                                                           centre_virial, nullptr, stream, tiles);
-  return go_adjoint_tile<N, L, NR, 2, 10, DIRECT, true>(rad, S, pv, c, norm2, normalize, xg, xg_scale, G, grad,
+  return go_adjoint_tile<N, L, NR, 2, 10, DIRECT, true>(rad, S, pv, c, norm2, normalize, xg, xg_scale, G,
                                                         centre_virial, nullptr, stream, tiles);
 }
 
 template <bool DIRECT>
 bool launch_adjoint_tile(const RadialDev& rad, int S, const PairView& pv, const float* c, const double* norm2,
-                         bool normalize, const double* xg, double xg_scale, const float* G, double* grad,
+                         bool normalize, const double* xg, double xg_scale, const float* G,
                          double* centre_virial, float* work, hipStream_t stream, const SoapTiles& tiles) {
   if (S > 2 * kWaves) return false;
   if (rad.N == 12 && rad.L == 9)
-    return launch_adjoint_tile_nl<12, 9, 6, DIRECT>(rad, S, pv, c, norm2, normalize, xg, xg_scale, G, grad,
+    return launch_adjoint_tile_nl<12, 9, 6, DIRECT>(rad, S, pv, c, norm2, normalize, xg, xg_scale, G,
                                                     centre_virial, work, stream, tiles);
   if (rad.N == 8 && rad.L == 6)
-    return launch_adjoint_tile_nl<8, 6, 2, DIRECT>(rad, S, pv, c, norm2, normalize, xg, xg_scale, G, grad,
+    return launch_adjoint_tile_nl<8, 6, 2, DIRECT>(rad, S, pv, c, norm2, normalize, xg, xg_scale, G,
                                                    centre_virial, work, stream, tiles);
   return false;
 }
@@ -3088,7 +3223,7 @@ template <class T, bool DIRECT>
 void launch_adjoint_t(const RadialDev& rad, int S, const PairView& pv, const T* c, const T* p,
                       const unsigned short* p_hi, const unsigned short* p_lo,
                       const double* norm2, bool normalize, const double* xg, double xg_scale,
-                      const T* G, double* grad, double* centre_virial, hipStream_t stream,
+                      const T* G, double* centre_virial, hipStream_t stream,
                       const SoapTiles& tiles, const T* Grow = nullptr,
                       const double* Gscale = nullptr, float* work = nullptr) {
   const int N = rad.N, L1 = rad.L1, LL = rad.LL, F = N * L1, E = S * N * LL;
@@ -3097,7 +3232,7 @@ void launch_adjoint_t(const RadialDev& rad, int S, const PairView& pv, const T* 
   if constexpr (sizeof(T) == 4) {
     const int K = S * (S + 1) / 2;
     if ((tiles.adj_kind == 2 || tiles.adj_kind == 3) && Grow == nullptr &&
-        launch_adjoint_tile<DIRECT>(rad, S, pv, c, norm2, normalize, xg, xg_scale, G, grad, centre_virial,
+        launch_adjoint_tile<DIRECT>(rad, S, pv, c, norm2, normalize, xg, xg_scale, G, centre_virial,
                                     tiles.adj_kind == 3 ? work : nullptr, stream, tiles))
       return;
     // matrix-core adjoint: packed dL/dX staged KCH key blocks at a time,
@@ -3133,7 +3268,7 @@ void launch_adjoint_t(const RadialDev& rad, int S, const PairView& pv, const T* 
       auto go = [&](auto kern) {
         allow_lds(kern, smem);
         hipLaunchKernelGGL(kern, dim3(unsigned(pv.n_centres)), dim3(kThreads), smem, stream, rad, S, PB,
-                           KCH, pv, c, norm2, int(normalize), xg, xg_scale, G, grad, centre_virial);
+                           KCH, pv, c, norm2, int(normalize), xg, xg_scale, G, centre_virial);
       };
       if (maxt <= 4)
         go(adjoint_mfma_kernel<DIRECT, 4>);
@@ -3163,7 +3298,7 @@ void launch_adjoint_t(const RadialDev& rad, int S, const PairView& pv, const T* 
     auto go = [&](auto kern) {
       allow_lds(kern, smem);
       hipLaunchKernelGGL(kern, dim3(unsigned(pv.n_centres)), dim3(kThreads), smem, stream, rad, S, PB, KCH,
-                         pv, c, p_hi, p_lo, norm2, int(normalize), xg, xg_scale, G, grad, centre_virial);
+                         pv, c, p_hi, p_lo, norm2, int(normalize), xg, xg_scale, G, centre_virial);
     };
     if (upt <= 2)
       go(adjoint_staged_kernel<DIRECT, 2>);
@@ -3194,7 +3329,7 @@ void launch_adjoint_t(const RadialDev& rad, int S, const PairView& pv, const T* 
     auto kern = adjoint_kernel<T, DIRECT>;
     allow_lds(kern, smem);
     hipLaunchKernelGGL(kern, dim3(unsigned(pv.n_centres)), dim3(kThreads), smem, stream, rad, S, PB,
-                       c_in_lds, pv, c, p, norm2, int(normalize), xg, xg_scale, G, Grow, Gscale, grad,
+                       c_in_lds, pv, c, p, norm2, int(normalize), xg, xg_scale, G, Grow, Gscale,
                        centre_virial);
     RGPU_LAUNCH_CHECK();
   }
@@ -3202,11 +3337,36 @@ void launch_adjoint_t(const RadialDev& rad, int S, const PairView& pv, const T* 
 
 }  // namespace
 
+// The pair-group expansion for the max_angular it is built for (FP32 spline
+// radial functions, max_radial <= 16, at least 3 species: with fewer, most
+// of its wavefronts would idle); false where it does not apply.
+template <class T>
+bool launch_expansion_group(const RadialDev& rad, int S, const PairView& pv, T* c,
+                            unsigned long long* keymask, hipStream_t stream, const SoapTiles& tiles) {
+  if constexpr (sizeof(T) != 4) {
+    return false;
+  } else {
+    if (!tiles.fwd_group || !rad.use_spline || rad.N > kPairGroup || S < 3) return false;
+    auto go = [&](auto kern) {
+      const size_t smem = expansion_group_lds(S, rad.N, rad.L1);
+      allow_lds(kern, smem);
+      hipLaunchKernelGGL(kern, dim3(unsigned(pv.n_centres)), dim3(kThreads), smem, stream, rad, S, pv, c,
+                         keymask);
+      RGPU_LAUNCH_CHECK();
+      return true;
+    };
+    if (rad.L == 9) return go(expansion_group_kernel<9>);
+    if (rad.L == 6) return go(expansion_group_kernel<6>);
+    return false;
+  }
+}
+
 template <class T>
 void launch_expansion(const RadialDev& rad, int S, const PairView& pv, T* c,
                       unsigned long long* keymask, hipStream_t stream, const SoapTiles& tiles) {
   if (pv.n_centres == 0) return;
   if (S > kMaxSpecies) throw InvalidArgument("at most 64 species");
+  if (launch_expansion_group<T>(rad, S, pv, c, keymask, stream, tiles)) return;
   const int B = rad.N * rad.LL;
   const int kpt = (B + kThreads - 1) / kThreads;
   if (kpt <= 2)
@@ -3477,42 +3637,42 @@ void launch_scatter_columns(int64_t n, int64_t Dsel, const int64_t* col, const d
 template <class T>
 void launch_backward(const RadialDev& rad, int S, const PairView& pv, const T* c, const T* p,
                      const double* norm2, bool normalize, const double* xg, double xg_scale,
-                     const T* G, double* grad, double* centre_virial, hipStream_t stream,
+                     const T* G, double* centre_virial, hipStream_t stream,
                      const SoapTiles& tiles) {
   if (pv.n_centres == 0) return;
   if (rad.use_spline)
     launch_adjoint_t<T, false>(rad, S, pv, c, p, nullptr, nullptr, norm2, normalize, xg, xg_scale, G,
-                               grad, centre_virial, stream, tiles);
+                               centre_virial, stream, tiles);
   else
     launch_adjoint_t<T, true>(rad, S, pv, c, p, nullptr, nullptr, norm2, normalize, xg, xg_scale, G,
-                              grad, centre_virial, stream, tiles);
+                              centre_virial, stream, tiles);
 }
 
 void launch_backward_rank1(const RadialDev& rad, int S, const PairView& pv, const double* c,
                            const double* p, const double* norm2, bool normalize, const double* Grow,
-                           const double* Gscale, double* grad, double* centre_virial,
+                           const double* Gscale, double* centre_virial,
                            hipStream_t stream, const SoapTiles& tiles) {
   if (pv.n_centres == 0) return;
   if (rad.use_spline)
     launch_adjoint_t<double, false>(rad, S, pv, c, p, nullptr, nullptr, norm2, normalize, nullptr, 1.0,
-                                    nullptr, grad, centre_virial, stream, tiles, Grow, Gscale);
+                                    nullptr, centre_virial, stream, tiles, Grow, Gscale);
   else
     launch_adjoint_t<double, true>(rad, S, pv, c, p, nullptr, nullptr, norm2, normalize, nullptr, 1.0,
-                                   nullptr, grad, centre_virial, stream, tiles, Grow, Gscale);
+                                   nullptr, centre_virial, stream, tiles, Grow, Gscale);
 }
 
 void launch_backward_split(const RadialDev& rad, int S, const PairView& pv, const float* c,
                            const unsigned short* p_hi, const unsigned short* p_lo,
                            const double* norm2, bool normalize, const double* xg, double xg_scale,
-                           const float* G, double* grad, double* centre_virial, hipStream_t stream,
+                           const float* G, double* centre_virial, hipStream_t stream,
                            const SoapTiles& tiles, float* work) {
   if (pv.n_centres == 0) return;
   if (rad.use_spline)
     launch_adjoint_t<float, false>(rad, S, pv, c, nullptr, p_hi, p_lo, norm2, normalize, xg, xg_scale,
-                                   G, grad, centre_virial, stream, tiles, nullptr, nullptr, work);
+                                   G, centre_virial, stream, tiles, nullptr, nullptr, work);
   else
     launch_adjoint_t<float, true>(rad, S, pv, c, nullptr, p_hi, p_lo, norm2, normalize, xg, xg_scale,
-                                  G, grad, centre_virial, stream, tiles, nullptr, nullptr, work);
+                                  G, centre_virial, stream, tiles, nullptr, nullptr, work);
 }
 
 void launch_pack_gradient(int S, int N, int L1, int64_t n, const double* G, float* Gp,
@@ -3634,8 +3794,132 @@ template void launch_normalise<double>(int64_t, int64_t, const double*, const do
                                        hipStream_t, const int64_t*);
 template void launch_normalise<float>(int64_t, int64_t, const float*, const double*, bool, double*,
                                       hipStream_t, const int64_t*);
+namespace {
+
+// ---------------------------------------------------------------------------
+// Deterministic force sum.  The adjoint writes each pair's force and each
+// centre's own share; the force on atom j is its centre share plus the
+// forces of the pairs whose neighbour is j, added in ascending pair index
+// over reverse lists (counted, scanned, filled and sorted per atom), so the
+// result does not depend on the order in which workgroups ran.
+// one wavefront per centre, its lanes over the centre's pairs (the self
+// pair skipped)
+__global__ __launch_bounds__(256) void rev_count_kernel(int64_t n, const int64_t* __restrict__ centre_off,
+                                                        const int* __restrict__ pair_j, int* __restrict__ count) {
+  const int64_t i = int64_t(blockIdx.x) * 4 + (threadIdx.x >> 6);
+  if (i >= n) return;
+  for (int64_t p = centre_off[i] + 1 + (threadIdx.x & 63); p < centre_off[i + 1]; p += 64)
+    atomicAdd(count + pair_j[p], 1);
+}
+
+// slots of atom j's list taken from the top down (count[j] returns to 0);
+// the order inside a list is arbitrary: force_gather_kernel ranks it
+__global__ __launch_bounds__(256) void rev_fill_kernel(int64_t n, const int64_t* __restrict__ centre_off,
+                                                       const int* __restrict__ pair_j,
+                                                       const int64_t* __restrict__ roff, int* __restrict__ count,
+                                                       int64_t* __restrict__ rlist) {
+  const int64_t i = int64_t(blockIdx.x) * 4 + (threadIdx.x >> 6);
+  if (i >= n) return;
+  for (int64_t p = centre_off[i] + 1 + (threadIdx.x & 63); p < centre_off[i + 1]; p += 64) {
+    const int j = pair_j[p];
+    rlist[roff[j] + atomicSub(count + j, 1) - 1] = p;
+  }
+}
+
+// One wavefront per atom j: grad[j] = centre_force[j] + the pair forces of
+// the pairs naming j, lane l summing the pairs of rank l, l + 64, ... in
+// ascending pair index, then a fixed butterfly over the lanes.  Lists of at
+// most kGatherCap pairs are ranked in LDS; longer ones take their pairs in
+// ascending order one per step (the same sums, more slowly).
+constexpr int kGatherCap = 256;
+
+__global__ __launch_bounds__(256) void force_gather_kernel(int64_t n, const int64_t* __restrict__ roff,
+                                                           const int64_t* __restrict__ rlist,
+                                                           const double* __restrict__ pf,
+                                                           const double* __restrict__ cf, double* __restrict__ grad) {
+  __shared__ int64_t lst[4][kGatherCap], ord[4][kGatherCap];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int64_t j = int64_t(blockIdx.x) * 4 + w;
+  if (j >= n) return;  // whole wavefronts; no workgroup barrier follows
+  const int64_t b = roff[j];
+  const int m = int(roff[j + 1] - b);
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+  if (m <= kGatherCap) {
+    for (int k = lane; k < m; k += 64) lst[w][k] = rlist[b + k];
+    wave_lds_sync();
+    for (int k = lane; k < m; k += 64) {
+      const int64_t v = lst[w][k];
+      int r = 0;
+      for (int q = 0; q < m; ++q) r += lst[w][q] < v ? 1 : 0;
+      ord[w][r] = v;
+    }
+    wave_lds_sync();
+    for (int k = lane; k < m; k += 64) {
+      const double* f = pf + 3 * ord[w][k];
+      s0 += f[0];
+      s1 += f[1];
+      s2 += f[2];
+    }
+  } else {
+    int64_t prev = -1;
+    for (int k = 0; k < m; ++k) {
+      long long best = LLONG_MAX;
+      for (int q = lane; q < m; q += 64) {
+        const long long v = rlist[b + q];
+        if (v > prev && v < best) best = v;
+      }
+      for (int off = 32; off > 0; off >>= 1) {
+        const long long o = __shfl_xor(best, off, 64);
+        best = o < best ? o : best;
+      }
+      prev = best;
+      if (lane == (k & 63)) {
+        const double* f = pf + 3 * best;
+        s0 += f[0];
+        s1 += f[1];
+        s2 += f[2];
+      }
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    s0 += __shfl_xor(s0, off, 64);
+    s1 += __shfl_xor(s1, off, 64);
+    s2 += __shfl_xor(s2, off, 64);
+  }
+  if (lane == 0) {
+    grad[3 * j] = cf[3 * j] + s0;
+    grad[3 * j + 1] = cf[3 * j + 1] + s1;
+    grad[3 * j + 2] = cf[3 * j + 2] + s2;
+  }
+}
+
+}  // namespace
+
+void launch_reverse_pairs(const PairView& pv, const ReverseLists& rl, DeviceBuffer<int64_t>& scan_scratch,
+                          hipStream_t stream) {
+  const int64_t n = pv.n_centres;
+  if (n == 0) return;
+  const unsigned grid = unsigned((n + 3) / 4);  // a wavefront per centre
+  RGPU_HIP(hipMemsetAsync(rl.count, 0, sizeof(int) * n, stream));
+  hipLaunchKernelGGL(rev_count_kernel, dim3(grid), dim3(256), 0, stream, n, pv.centre_off, pv.pair_j, rl.count);
+  RGPU_LAUNCH_CHECK();
+  exclusive_scan_reserve(n, scan_scratch);
+  exclusive_scan_device(rl.count, rl.roff, n, stream, scan_scratch);
+  hipLaunchKernelGGL(rev_fill_kernel, dim3(grid), dim3(256), 0, stream, n, pv.centre_off, pv.pair_j, rl.roff,
+                     rl.count, rl.rlist);
+  RGPU_LAUNCH_CHECK();
+}
+
+void launch_force_gather(const PairView& pv, const ReverseLists& rl, double* grad, hipStream_t stream) {
+  const int64_t n = pv.n_centres;
+  if (n == 0) return;
+  hipLaunchKernelGGL(force_gather_kernel, dim3(unsigned((n + 3) / 4)), dim3(256), 0, stream, n, rl.roff,
+                     rl.rlist, pv.pair_force, pv.centre_force, grad);
+  RGPU_LAUNCH_CHECK();
+}
+
 template void launch_backward<double>(const RadialDev&, int, const PairView&, const double*,
                                       const double*, const double*, bool, const double*, double,
-                                      const double*, double*, double*, hipStream_t, const SoapTiles&);
+                                      const double*, double*, hipStream_t, const SoapTiles&);
 
 }  // namespace rgpu

@@ -392,16 +392,82 @@ def test_deferred_neighbour_check_rebuilds_a_short_list(graphs):
         assert ctx.info("nl_graph_launches") == 0
 
 
+@pytest.mark.parametrize("precision", [_capi.FP64, _capi.MIXED])
+def test_reductions_are_repeatable(precision):
+    """Every sum of the prediction runs in a fixed order (the energy GEMM's
+    per-tile partials added in tile order, per-pair forces gathered per
+    atom in pair order, virials and structure sums per segment), so two
+    predictions, and one after the neighbour list and its reverse lists are
+    rebuilt, agree bit for bit; the K_NM Structure rows equal the Atom rows
+    summed in centre order."""
+    sts = triclinic_cells(n_cells=4, n_min=40, n_max=90, seed=41)
+    hp = soap_hypers(rc=4.0, sw=0.5, sigma=0.5, N=8, L=6, spline=1e-8)
+    ctx, sp = make_ctx(hp, sts)
+    ctx.set_precision(precision)
+    X, _ = ctx.features()
+    num = pack(sts)[2]
+    rng = np.random.default_rng(42)
+    # 300 points per species: several column tiles of both energy GEMMs
+    spec = np.concatenate([np.full(300, t, np.int32) for t in sp])
+    vals = np.concatenate([X[rng.choice(np.where(num == t)[0], 300)] for t in sp])
+    ctx.set_sparse_points(2, spec, vals, rng.normal(size=len(spec)))
+    first = ctx.gap_predict()
+    again = ctx.gap_predict()
+    ctx.update_positions(pack(sts)[1])
+    rebuilt = ctx.gap_predict()
+    for a, b, c in zip(first, again, rebuilt):
+        assert np.array_equal(a, b) and np.array_equal(a, c)
+    k_struct, k_atom = ctx.gap_knm("Structure"), ctx.gap_knm("Atom")
+    assert np.array_equal(ctx.gap_knm("Structure"), k_struct)
+    off = pack(sts)[0]
+    for s in range(len(sts)):
+        t = np.zeros(k_atom.shape[1])
+        for a in range(off[s], off[s + 1]):
+            t += k_atom[a]
+        assert np.array_equal(k_struct[s], t)
+
+
+@pytest.mark.parametrize("precision", [_capi.FP64, _capi.MIXED])
+def test_forces_with_long_reverse_lists(precision):
+    """Atoms named by more than 256 pairs (dense jittered diamond, a 3.57,
+    rc 7.5: about 300 neighbours) have their pair forces summed on the
+    long-list path of the force gather: forces (and the FP64 stress)
+    against the oracle, and bit-for-bit repeatable."""
+    sts = diamond_cells(n_cells=1, reps=2, a=3.57, jitter=0.05, species=6, seed=3)
+    hp = soap_hypers(rc=7.5, sw=0.5, sigma=0.5, N=6, L=4, spline=1e-8)
+    ctx, sp = make_ctx(hp, sts)
+    off, pj, _ = ctx.neighbours(with_vectors=False)
+    strict = np.concatenate([pj[off[i] + 1:off[i + 1]] for i in range(len(off) - 1)])
+    assert np.bincount(strict).min() > 256
+    X, _ = ctx.features()
+    rng = np.random.default_rng(5)
+    spec, vals, w = np.full(40, 6, np.int32), X[rng.choice(len(X), 40)], rng.normal(size=40)
+    ctx.set_sparse_points(2, spec, vals, w)
+    ctx.set_precision(precision)
+    e, g, s = ctx.gap_predict()
+    calc = O.Calculator(hp)
+    e_ref, g_ref, s_ref = O.gap_predict(calc, O.BatchData(sts, species=sp), O.SparseSet(spec, vals, 2), w)
+    # mixed: FP32 pair sums over ~300 neighbours with strong cancellation
+    # (1.3e-4 of the largest component measured)
+    tol = FP64_TOL if precision == _capi.FP64 else 1e-3
+    assert np.abs(g - g_ref).max() < tol * np.abs(g_ref).max()
+    if precision == _capi.FP64:  # the stress takes no part in the gather
+        assert np.abs(s - s_ref).max() < tol * np.abs(s_ref).max()
+    again = ctx.gap_predict()
+    for a, b in zip((e, g, s), again):
+        assert np.array_equal(a, b)
+
+
 def test_prediction_outputs_to_device_memory():
     """Outputs given as device pointers (contiguous torch tensors on the GPU)
-    receive the same values as host arrays (to the rounding of the FP64
-    force atomics, whose order varies between predictions)."""
+    receive exactly the values written to host arrays (the prediction's
+    sums run in a fixed order)."""
     import torch
     sts, hp, ctx, sp, spec, vals, w = _gap_box()
     e, g, s = ctx.gap_predict()
 
     def close(a, b):
-        return np.abs(a - b).max() <= 1e-12 * np.abs(b).max()
+        return np.array_equal(a, b)
 
     ed = torch.zeros(e.shape, dtype=torch.float64, device="cuda")
     gd = torch.zeros(g.shape, dtype=torch.float64, device="cuda")
@@ -424,8 +490,9 @@ def test_prediction_outputs_to_device_memory():
 @pytest.mark.parametrize("N,L", [(12, 9), (5, 4), (8, 6)])
 def test_fused_expansion_invariants_equal_two_launches(N, L, spline):
     """The persistent fused expansion + power spectrum launch (c kept in LDS
-    between the stages), and the expansion writing c once from LDS (fused
-    2), against the separate kernels: expansion, key masks
+    between the stages), the expansion writing c once from LDS (fused 2) and
+    the FP32 expansion on pair groups (fused 3; per-entry in FP64 and for
+    N, L it is not built for), against the separate kernels: expansion, key masks
     and invariants equal in FP64 (to rounding) and in the mixed path, on
     ragged molecules with 4 species (more centres than resident
     workgroups for the larger shapes)."""
@@ -435,7 +502,7 @@ def test_fused_expansion_invariants_equal_two_launches(N, L, spline):
         ctx, sp = make_ctx(hp, sts)
         ctx.set_precision(prec)
         out = {}
-        for fused in (0, 1, 2):
+        for fused in (0, 1, 2, 3):
             ctx.set_tiles(fused=fused)
             assert ctx.tiles()["fused"] == fused
             ctx.compute()
@@ -443,7 +510,7 @@ def test_fused_expansion_invariants_equal_two_launches(N, L, spline):
             X, inv = ctx.features()
             out[fused] = (c, km, X, inv)
         c0, km0, X0, inv0 = out[0]
-        for variant in (1, 2):
+        for variant in (1, 2, 3):
             c1, km1, X1, inv1 = out[variant]
             assert np.array_equal(km0, km1)
             assert np.abs(c1 - c0).max() <= tol * np.abs(c0).max()

@@ -338,7 +338,7 @@ def test_reference_model_file_predicts_like_oracle():
     assert np.all(np.abs(e - e_ref) < TOL * (np.abs(K) @ np.abs(model.weights)))
     assert np.abs(f + g_ref).max() < TOL * np.abs(g_ref).max()
     assert np.abs(s + s_ref).max() < TOL * np.abs(s_ref).max()
-    # one GPU pass for all three (force atomics sum in any order)
+    # one GPU pass for all three
     e1, f1, s1 = model.predict_energy_forces_stress(al)
     assert np.abs(e1 - e).max() < 1e-12 * np.abs(e).max()
     assert np.abs(f1 - f).max() < 1e-12 * np.abs(f).max()
