@@ -227,17 +227,22 @@ int rgpu_gram_f64(int64_t rows, int64_t cols, const double* A, double* out);
  * [n_atoms][3] = dE/dr, neg_stress [n_struct][6] (/ volume).  Any output may
  * be NULL; gradient/neg_stress trigger the adjoint
  * (compute_sparse_kernel_gradients / _neg_stress,
- * sparse_kernel_predict.hh:206-377). */
+ * sparse_kernel_predict.hh:206-377).  Every floating-point sum runs in a
+ * fixed order (no floating-point atomics): repeated calls on the same
+ * inputs return identical bits. */
 int rgpu_gap_predict(rgpu_context* ctx, double* energy, double* gradient, double* neg_stress);
 
 /* Allocation-free GAP energies: energy[n_struct] (device memory) = K_NM w,
  * as rgpu_gap_predict's energies.  No counterpart in the reference: for
  * callers that cannot allocate device memory (code under HIP graph capture,
  * serving loops, harnesses that forbid hipMalloc).  It calls no hipMalloc,
- * hipFree or other allocating routine: the per-centre energies live in a
- * static device array of 2^21 doubles (16 MiB per device), so batches of at
- * most 2097152 centres are accepted (larger ones are rejected with an
- * error), and every context buffer must already be sized for the batch
+ * hipFree or other allocating routine: the per-centre energies and the
+ * energy GEMM's per-column-tile partial sums live in a static device array
+ * of 2^23 doubles (64 MiB per device), so a batch of n centres is accepted
+ * when ceil(n/32)*32 + n*T <= 2^23, T being the column tiles of the widest
+ * species block (ceil(M_s/128) mixed, ceil(M_s/64) FP64; larger batches
+ * are rejected with an error), and every context buffer must already be
+ * sized for the batch
  * (one rgpu_gap_predict on it first; otherwise an error, nothing allocated).
  * The call is stream-ordered (no synchronisation): the energies are ready
  * when the context's stream reaches them.  The scratch array is shared by

@@ -926,9 +926,14 @@ void emit_outputs(Context& c, const double* e_centre, double* energy, double* gr
   double* h = c.out_stage.reserve(std::max<size_t>(ne + ng + nst, 1));
   double* hd = c.out_stage.device();
   std::memset(h, 0, sizeof(double) * (ne + ng + nst));
-  if (ne) launch_segment_sum(e_centre, 1, c.d_atom_off.get(), ns, hd, c.stream);
+  if (ne && nst)
+    launch_segment_sum2(e_centre, 1, hd, nullptr, cvir_d, 6, hd + ne + ng, c.d_inv_vol.get(), c.d_atom_off.get(), ns,
+                        c.stream);
+  else if (ne)
+    launch_segment_sum(e_centre, 1, c.d_atom_off.get(), ns, hd, c.stream);
+  else if (nst)
+    launch_segment_sum(cvir_d, 6, c.d_atom_off.get(), ns, hd + ne + ng, c.stream, c.d_inv_vol.get());
   if (ng) launch_strided_copy(grad_d, int64_t(ng), hd + ne, 1, c.stream);
-  if (nst) launch_segment_sum(cvir_d, 6, c.d_atom_off.get(), ns, hd + ne + ng, c.stream, c.d_inv_vol.get());
   struct Out {
     double* dst;
     size_t off, n;

@@ -7,7 +7,10 @@
 //
 // Tiling: 256 threads (4 wavefronts, 2 x 2), block tile 64 x 64, K step 16;
 // each wavefront owns 32 x 32 = 2 x 2 MFMA tiles of 16 x 16.  Grouped launch
-// over species blocks as the mixed-precision kernel.
+// over species blocks, tiles dealt to the XCDs as the mixed-precision
+// kernel's (groups of kGroupM row panels walked column by column, a
+// contiguous range of them per XCD, so the tiles resident on one XCD share
+// operand panels through its L2).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -23,6 +26,7 @@ typedef __attribute__((__vector_size__(4 * sizeof(double)))) double f64x4;
 
 constexpr int BM = 64, BN = 64, BK = 16, LDK = BK + 1, NT = 256;
 constexpr int kMaxProblems = 16;
+constexpr int kGroupM = 8;  // row panels per tile group
 
 struct Problem {
   int64_t M, N, K;
@@ -33,7 +37,7 @@ struct Problem {
   int64_t ldb;
   const int* c_rows;
   int64_t tile0;
-  int tiles_n;
+  int tiles_m, tiles_n;
   const double* norm2;
   const double* alpha;
   double* epart;  // energy partials [tiles_n][ldp], by centre
@@ -45,6 +49,7 @@ struct Problem {
 
 struct ProblemSet {
   Problem p[kMaxProblems];
+  int64_t total_tiles;
   int n;
   int normalize;
   int zeta;
@@ -71,12 +76,19 @@ __global__ __launch_bounds__(NT, 2) void f64_gemm(ProblemSet ps) {
   __shared__ double sB[BN * LDK];
   __shared__ double esum[2][BM];
 
-  const int64_t bid = blockIdx.x;
+  // the workgroups go to the 8 XCDs round robin: XCD x takes linear tiles
+  // [x per_xcd, (x + 1) per_xcd)
+  const int64_t per_xcd = (int64_t(gridDim.x) + 7) / 8;
+  const int64_t bid = int64_t(blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
+  if (bid >= ps.total_tiles) return;  // whole workgroup, before any barrier
   int pi = 0;
   while (pi + 1 < ps.n && bid >= ps.p[pi + 1].tile0) ++pi;
   const Problem& P = ps.p[pi];
   const int64_t local = bid - P.tile0;
-  const int64_t tm = local / P.tiles_n, tn = local - tm * P.tiles_n;
+  const int64_t per_group = int64_t(kGroupM) * P.tiles_n;
+  const int64_t grp = local / per_group, in = local - grp * per_group;
+  const int64_t gm = std::min<int64_t>(kGroupM, P.tiles_m - grp * kGroupM);
+  const int64_t tn = in / gm, tm = grp * kGroupM + (in - tn * gm);
   const int64_t r0 = tm * BM, c0 = tn * BN;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave >> 1, wn = wave & 1;
@@ -218,12 +230,14 @@ __global__ __launch_bounds__(NT, 2) void f64_gemm(ProblemSet ps) {
   }
 }
 
-void launch(const ProblemSet& ps, int64_t tiles, int epi, hipStream_t stream) {
+void launch(ProblemSet& ps, int64_t tiles, int epi, hipStream_t stream) {
   if (tiles == 0) return;
+  ps.total_tiles = tiles;
+  const unsigned grid = unsigned((tiles + 7) / 8 * 8);  // whole rounds over the 8 XCDs
   if (epi == 0)
-    hipLaunchKernelGGL(f64_gemm<0>, dim3(unsigned(tiles)), dim3(NT), 0, stream, ps);
+    hipLaunchKernelGGL(f64_gemm<0>, dim3(grid), dim3(NT), 0, stream, ps);
   else
-    hipLaunchKernelGGL(f64_gemm<1>, dim3(unsigned(tiles)), dim3(NT), 0, stream, ps);
+    hipLaunchKernelGGL(f64_gemm<1>, dim3(grid), dim3(NT), 0, stream, ps);
   RGPU_LAUNCH_CHECK();
 }
 
@@ -270,6 +284,7 @@ void run_gap_f64_kernel(const GapBlock<double>* blks, int n_blocks, double* ener
       P.ldb = blk.D;
       P.c_rows = blk.row_idx;
       P.tile0 = tiles;
+      P.tiles_m = int((P.M + BM - 1) / BM);
       P.tiles_n = int((P.N + BN - 1) / BN);
       P.norm2 = blk.norm2;
       P.alpha = blk.alpha + blk.col0;
@@ -325,6 +340,7 @@ void run_gap_f64_force(const GapBlock<double>* blks, int n_blocks, double* const
       P.ldb = blk.D;
       P.c_rows = blk.row_idx;
       P.tile0 = tiles;
+      P.tiles_m = int((P.M + BM - 1) / BM);
       P.tiles_n = int((P.N + BN - 1) / BN);
       P.G = G;
       P.ldg = blk.D;

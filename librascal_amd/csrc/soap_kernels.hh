@@ -199,6 +199,11 @@ void launch_strided_copy(const double* src, int64_t n, double* dst, int64_t ld, 
 // [seg_off[s], seg_off[s+1])
 void launch_segment_sum(const double* in, int w, const int64_t* seg_off, int n_seg, double* out,
                         hipStream_t stream, const double* scale = nullptr);
+// two segment sums over the same segments in one launch (each as
+// launch_segment_sum)
+void launch_segment_sum2(const double* in0, int w0, double* out0, const double* scale0, const double* in1, int w1,
+                         double* out1, const double* scale1, const int64_t* seg_off, int n_seg,
+                         hipStream_t stream);
 // raw radial integrals (FP64 Hyp1f1) raw[pt][n][l] (+ derivatives, may be null)
 void launch_radial_direct(const RadialDev& rad, const double* r, int64_t n_pts, double* raw,
                           double* draw, hipStream_t stream);

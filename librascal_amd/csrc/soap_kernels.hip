@@ -1145,7 +1145,11 @@ __global__ __launch_bounds__(kThreads) void adjoint_kernel(
     cs = c + i * int64_t(E);
     gA = reinterpret_cast<T*>(region);
   }
-  T* gB = gA + L1 * N * N4;  // gA[l][x][y] = g[y][x][l], gB[l][x][y] = g[x][y][l]
+  // gA[l][x][y] = g[y][x][l], gB[l][x][y] = g[x][y][l]; the l planes LS
+  // apart, two values more than a plane, so that the transposing stores of
+  // consecutive l (one per thread) fall in different LDS banks
+  const int LS = N * N4 + 2;
+  T* gB = gA + L1 * LS;
   __shared__ double red[kWaves][9];
   __shared__ double sh_ab[2];
 
@@ -1180,7 +1184,7 @@ __global__ __launch_bounds__(kThreads) void adjoint_kernel(
   }
   const T ta = T(alpha), tb = T(beta);
   for (int e = tid; e < E; e += kThreads) dc[e] = T(0);
-  for (int e = tid; e < 2 * L1 * N * N4; e += kThreads) gA[e] = T(0);  // zero pads
+  for (int e = tid; e < 2 * L1 * LS; e += kThreads) gA[e] = T(0);  // zero pads
 
   // key blocks: element o = (x*N + y)*L1 + l; thread holds o = tid + 256 r
   const FastDiv fNL1 = fast_div(N * L1), fL1 = fast_div(L1), fLL = fast_div(LL);
@@ -1207,8 +1211,8 @@ __global__ __launch_bounds__(kThreads) void adjoint_kernel(
         const int o = tid + kThreads * r;
         if (o < KB) {
           const int x = o / fNL1, rr = o - x * N * L1, y = rr / fL1, l = rr - y * L1;
-          gA[(l * N + y) * N4 + x] = pre[r];
-          gB[(l * N + x) * N4 + y] = pre[r];
+          gA[l * LS + y * N4 + x] = pre[r];
+          gB[l * LS + x * N4 + y] = pre[r];
         }
       }
     } else {
@@ -1216,14 +1220,14 @@ __global__ __launch_bounds__(kThreads) void adjoint_kernel(
       for (int o = tid; o < KB; o += kThreads) {
         const T g = ta * gval(pk + o) - tb * pval(pk + o);
         const int x = o / fNL1, rr = o - x * N * L1, y = rr / fL1, l = rr - y * L1;
-        gA[(l * N + y) * N4 + x] = g;
-        gB[(l * N + x) * N4 + y] = g;
+        gA[l * LS + y * N4 + x] = g;
+        gB[l * LS + x * N4 + y] = g;
       }
     }
     __syncthreads();
     if (a == b) {
       // symmetrise: gA[l][x][y] = g[y][x] + g[x][y]
-      for (int e = tid; e < L1 * N * N4; e += kThreads) gA[e] += gB[e];
+      for (int e = tid; e < L1 * LS; e += kThreads) gA[e] += gB[e];
       __syncthreads();
     }
     if (key + 1 < K) fetch(key + 1);
@@ -1234,7 +1238,7 @@ __global__ __launch_bounds__(kThreads) void adjoint_kernel(
       const int uu = side_b ? u - units_a : u;
       const int nt = uu / fLL, lm = uu - nt * LL;
       const int l = l_of_lm(lm);
-      const T* gt = (side_b ? gB : gA) + l * N * N4 + 4 * nt;
+      const T* gt = (side_b ? gB : gA) + l * LS + 4 * nt;
       const T* cx = cs + (side_b ? a : b) * B + lm;
       V4 s4 = {T(0), T(0), T(0), T(0)};
       for (int x = 0; x < N; ++x) {
@@ -2617,18 +2621,42 @@ __global__ void pair_cutoff_kernel(RadialDev rad, const int64_t* __restrict__ n_
 // per-structure (segment) sums of per-centre rows of width w <= 8, times
 // scale[s] when given (+=)
 constexpr int kSegThreads = 1024;
-__global__ __launch_bounds__(kSegThreads) void segment_sum_kernel(const double* __restrict__ in, int w,
-                                   const int64_t* __restrict__ seg_off, int n_seg,
-                                   const double* __restrict__ scale, double* __restrict__ out) {
-  const int s = blockIdx.x;
-  if (s >= n_seg) return;
+struct SegJob {
+  const double* in;
+  double* out;
+  const double* scale;
+  int w;
+};
+
+// One workgroup per (job, segment): blocks [0, n_seg) run job 0, blocks
+// [n_seg, 2 n_seg) job 1.  Each thread takes rows a, a + T, a + 2T, ... in
+// that order, four rows' loads issued before their adds; a fixed tree sums
+// the threads (the same bits on every run).
+__global__ __launch_bounds__(kSegThreads) void segment_sum_kernel(SegJob j0, SegJob j1,
+                                                                  const int64_t* __restrict__ seg_off, int n_seg) {
+  const bool second = int(blockIdx.x) >= n_seg;
+  const int s = second ? int(blockIdx.x) - n_seg : int(blockIdx.x);
+  const double* __restrict__ in = second ? j1.in : j0.in;
+  double* __restrict__ out = second ? j1.out : j0.out;
+  const double* __restrict__ scale = second ? j1.scale : j0.scale;
+  const int w = second ? j1.w : j0.w;
   __shared__ double red[kSegThreads / 64][8];
   double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  const int64_t a1 = seg_off[s + 1];
   // unrolled over the largest width (uniform guards): v stays in registers
-  for (int64_t a = seg_off[s] + threadIdx.x; a < seg_off[s + 1]; a += kSegThreads)
+  for (int64_t a = seg_off[s] + threadIdx.x; a < a1; a += 4 * kSegThreads) {
+    double x[4][8];
 #pragma unroll
-    for (int k = 0; k < 8; ++k)
-      if (k < w) v[k] += in[int64_t(w) * a + k];
+    for (int u = 0; u < 4; ++u) {
+      const int64_t r = a + int64_t(u) * kSegThreads;
+#pragma unroll
+      for (int k = 0; k < 8; ++k) x[u][k] = (k < w && r < a1) ? in[int64_t(w) * r + k] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int k = 0; k < 8; ++k) v[k] += x[u][k];
+  }
 #pragma unroll
   for (int k = 0; k < 8; ++k)
     if (k < w)
@@ -3315,7 +3343,7 @@ void launch_adjoint_t(const RadialDev& rad, int S, const PairView& pv, const T* 
     (void)p_hi;
     (void)p_lo;
     const int N4 = (N + 3) & ~3;
-    const size_t gbytes = sizeof(T) * 2 * size_t(L1) * N * N4;
+    const size_t gbytes = sizeof(T) * 2 * size_t(L1) * (N * N4 + 2);  // two l-plane sets, LS apart
     const int c_in_lds = yt + blk + blk + gbytes <= kAdjBudget ? 1 : 0;
     const size_t region = (c_in_lds ? blk : 0) + gbytes;
     const size_t fixed = yt + blk;
@@ -3729,8 +3757,19 @@ void launch_segment_sum(const double* in, int w, const int64_t* seg_off, int n_s
                         hipStream_t stream, const double* scale) {
   if (n_seg == 0) return;
   if (w < 1 || w > 8) throw InvalidArgument("segment sum: row width must be 1 to 8");
-  hipLaunchKernelGGL(segment_sum_kernel, dim3(unsigned(n_seg)), dim3(kSegThreads), 0, stream, in, w, seg_off,
-                     n_seg, scale, out);
+  const SegJob j{in, out, scale, w};
+  hipLaunchKernelGGL(segment_sum_kernel, dim3(unsigned(n_seg)), dim3(kSegThreads), 0, stream, j, j, seg_off,
+                     n_seg);
+  RGPU_LAUNCH_CHECK();
+}
+
+void launch_segment_sum2(const double* in0, int w0, double* out0, const double* scale0, const double* in1, int w1,
+                         double* out1, const double* scale1, const int64_t* seg_off, int n_seg,
+                         hipStream_t stream) {
+  if (n_seg == 0) return;
+  if (w0 < 1 || w0 > 8 || w1 < 1 || w1 > 8) throw InvalidArgument("segment sum: row width must be 1 to 8");
+  hipLaunchKernelGGL(segment_sum_kernel, dim3(unsigned(2 * n_seg)), dim3(kSegThreads), 0, stream,
+                     SegJob{in0, out0, scale0, w0}, SegJob{in1, out1, scale1, w1}, seg_off, n_seg);
   RGPU_LAUNCH_CHECK();
 }
 
