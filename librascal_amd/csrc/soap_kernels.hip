@@ -532,14 +532,19 @@ __global__ __launch_bounds__(kThreads, (expansion_min_waves<T, KPT, DIRECT>())) 
   load_ylm_table<T>(rad, yc);
 
   const FastDiv fLL = fast_div(LL);
-  int ent[KPT];  // (n*L1 + l) << 16 | lm, or -1
+  // entry k of this thread: e = tid + 256 k < B, (n, lm) = (e / LL, e % LL);
+  // its R and Y offsets within a pair's rows (entries past B read the
+  // first ones, so that the pair loop has no branch, and are never stored)
+  int offR[KPT], offY[KPT];
 #pragma unroll
   for (int k = 0; k < KPT; ++k) {
     const int e = tid + kThreads * k;
-    ent[k] = -1;
+    offR[k] = 0;
+    offY[k] = 0;
     if (e < B) {
       const int n = e / fLL, lm = e - n * LL;
-      ent[k] = ((n * L1 + l_of_lm(lm)) << 16) | lm;
+      offR[k] = n * L1 + l_of_lm(lm);
+      offY[k] = lm;
     }
   }
   if (tid == 0) kmask = 1ull << si;
@@ -563,10 +568,10 @@ __global__ __launch_bounds__(kThreads, (expansion_min_waves<T, KPT, DIRECT>())) 
       for (int k = 0; k < KPT; ++k) {
         const int e = tid + kThreads * k;
         acc[k] = T(0);
-        if (ent[k] >= 0) {
+        if (e < B) {
           if (q0 > 0)
             acc[k] = cs[e];
-          else if (s == si && (ent[k] & 0xffff) == 0)
+          else if (s == si && offY[k] == 0)
             acc[k] = T(rad.center[(rad.sp_tab ? rad.sp_tab[si] * rad.N : 0) + e / fLL] * inv_s4pi);
         }
       }
@@ -575,12 +580,11 @@ __global__ __launch_bounds__(kThreads, (expansion_min_waves<T, KPT, DIRECT>())) 
         const T* Yq = pb.Y + q * LL;
         const T* Rq = pb.R + q * F;
 #pragma unroll
-        for (int k = 0; k < KPT; ++k)
-          if (ent[k] >= 0) acc[k] += Rq[ent[k] >> 16] * Yq[ent[k] & 0xffff];
+        for (int k = 0; k < KPT; ++k) acc[k] += Rq[offR[k]] * Yq[offY[k]];
       }
 #pragma unroll
       for (int k = 0; k < KPT; ++k)
-        if (ent[k] >= 0) cs[tid + kThreads * k] = acc[k];
+        if (tid + kThreads * k < B) cs[tid + kThreads * k] = acc[k];
     }
     __syncthreads();
   }
