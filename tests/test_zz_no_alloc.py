@@ -45,7 +45,9 @@ def test_noalloc_energy_matches_gap_predict(precision):
     big = triclinic_cells(n_cells=5, n_min=60, n_max=120, seed=31)
     small = periodic_box(96, 0.05, (1, 8), d_min=1.0, seed=32)
     for sts in (big, small):
-        ctx = _batch(sts, precision)
+        # 150 points per species: several column tiles of the energy GEMM
+        # (partial sums kept in the scratch array)
+        ctx = _batch(sts, precision, per_species=150)
         e_ref, _, _ = ctx.gap_predict()  # sizes every buffer of the batch
         for _ in range(2):
             e = _noalloc(ctx, len(sts))
