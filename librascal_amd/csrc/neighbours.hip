@@ -169,7 +169,11 @@ __global__ __launch_bounds__(kSortThreads) void bin_sort_kernel(const int64_t* _
 // One wavefront per centre: 27-box stencil (last dimension fastest,
 // adaptor_neighbour_list.hh:92-158), strict inclusive cutoff on the squared
 // distance (adaptor_strict.hh:414-427).  Pass 1 counts, pass 2 writes in
-// stencil/box order, preceded by the self pair.
+// stencil/box order, preceded by the self pair.  The candidates of the 27
+// boxes are taken as one run (box s's items at [first[s], first[s] + size
+// of s)), 64 per step, so that the lanes stay busy although a box holds only
+// a few atoms; candidate k is item k - first[s] of the last box s whose
+// first[s] <= k, and the ballot keeps the stencil/box order.
 template <bool WRITE>
 __global__ __launch_bounds__(256) void pairs_kernel(
     const NLGeom* __restrict__ geom, const int* __restrict__ atom_struct,
@@ -178,9 +182,10 @@ __global__ __launch_bounds__(256) void pairs_kernel(
     double rc, int* __restrict__ pair_count, const int64_t* __restrict__ centre_off,
     int* __restrict__ pair_j, double* __restrict__ pair_vec, double* __restrict__ pair_dist,
     int64_t cap) {
-  const int lane = threadIdx.x & 63;
-  const int64_t i = int64_t(blockIdx.x) * (blockDim.x / 64) + (threadIdx.x >> 6);
-  if (i >= n_atoms) return;
+  __shared__ int64_t first[4][32], base[4][32];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int64_t i = int64_t(blockIdx.x) * (blockDim.x / 64) + wave;
+  if (i >= n_atoms) return;  // whole wavefronts; no workgroup barrier follows
   const NLGeom& g = geom[atom_struct[i]];
   const double xi0 = ext_pos[3 * i], xi1 = ext_pos[3 * i + 1], xi2 = ext_pos[3 * i + 2];
   const double xi[3] = {xi0, xi1, xi2};
@@ -195,48 +200,74 @@ __global__ __launch_bounds__(256) void pairs_kernel(
     pair_vec[3 * w + 2] = 0.0;
     pair_dist[w] = 0.0;
   }
-  int64_t n = 1;
-  for (int s = 0; s < 27; ++s) {
+  // lane s < 27: box s of the stencil; inclusive scan of the box sizes
+  int64_t size = 0, start = 0;
+  if (lane < 27) {
     int c[3];
     int f = 1;
     for (int d = 2; d >= 0; --d) {
-      c[d] = s / f % 3 + b[d] - 1;
+      c[d] = lane / f % 3 + b[d] - 1;
       f *= 3;
     }
-    int64_t bl = box_lin(g, c);
-    int64_t s0 = box_start[bl], s1 = box_start[bl + 1];
-    for (int64_t q0 = s0; q0 < s1; q0 += 64) {
-      int64_t q = q0 + lane;
-      bool keep = false;
-      int t = -1;
-      double v0 = 0, v1 = 0, v2 = 0, d2 = 0;
-      if (q < s1) {
-        t = box_items[q];
-        if (t != int(i)) {
-          v0 = ext_pos[3 * t] - xi0;
-          v1 = ext_pos[3 * t + 1] - xi1;
-          v2 = ext_pos[3 * t + 2] - xi2;
-          double a0 = v0 * v0;
-          double a1 = v1 * v1;
-          double a2 = v2 * v2;
-          d2 = (a0 + a1) + a2;
-          keep = d2 <= rc2;
-        }
+    const int64_t bl = box_lin(g, c);
+    start = box_start[bl];
+    size = box_start[bl + 1] - start;
+  }
+  int64_t incl = size;
+  for (int off = 1; off < 32; off <<= 1) {
+    const int64_t o = __shfl_up(incl, off, 64);
+    if (lane >= off) incl += o;
+  }
+  const int64_t total = __shfl(incl, 26, 64);
+  if (lane < 27) {
+    first[wave][lane] = incl - size;
+    base[wave][lane] = start;
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  int64_t n = 1;
+  for (int64_t k0 = 0; k0 < total; k0 += 64) {
+    const int64_t k = k0 + lane;
+    bool keep = false;
+    int t = -1;
+    double v0 = 0, v1 = 0, v2 = 0, d2 = 0;
+    if (k < total) {
+      // last box whose run starts at or before k (empty boxes share their
+      // start with the next box and are passed over)
+      int lo = 0, hi = 26;
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (first[wave][mid] <= k)
+          lo = mid;
+        else
+          hi = mid - 1;
       }
-      unsigned long long mask = __ballot(keep);
-      if (WRITE && keep) {
-        unsigned long long below = mask & ((1ull << lane) - 1ull);
-        int64_t o = w + n + __popcll(below);
-        if (o < cap) {
-          pair_j[o] = ext_orig[t];
-          pair_vec[3 * o] = v0;
-          pair_vec[3 * o + 1] = v1;
-          pair_vec[3 * o + 2] = v2;
-          pair_dist[o] = sqrt(d2);
-        }
+      t = box_items[base[wave][lo] + (k - first[wave][lo])];
+      if (t != int(i)) {
+        v0 = ext_pos[3 * t] - xi0;
+        v1 = ext_pos[3 * t + 1] - xi1;
+        v2 = ext_pos[3 * t + 2] - xi2;
+        double a0 = v0 * v0;
+        double a1 = v1 * v1;
+        double a2 = v2 * v2;
+        d2 = (a0 + a1) + a2;
+        keep = d2 <= rc2;
       }
-      n += __popcll(mask);
     }
+    unsigned long long mask = __ballot(keep);
+    if (WRITE && keep) {
+      unsigned long long below = mask & ((1ull << lane) - 1ull);
+      int64_t o = w + n + __popcll(below);
+      if (o < cap) {
+        pair_j[o] = ext_orig[t];
+        pair_vec[3 * o] = v0;
+        pair_vec[3 * o + 1] = v1;
+        pair_vec[3 * o + 2] = v2;
+        pair_dist[o] = sqrt(d2);
+      }
+    }
+    n += __popcll(mask);
   }
   if (!WRITE && lane == 0) pair_count[i] = int(n);
 }
