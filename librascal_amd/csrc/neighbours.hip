@@ -42,28 +42,34 @@ __device__ __forceinline__ int64_t box_lin(const NLGeom& g, const int* b) {
 }
 
 // ghost images of atom a that fall inside the ghost bounds
-// (adaptor_neighbour_list.hh:1044-1072); count or write
+// (adaptor_neighbour_list.hh:1044-1072); count or write.  One wavefront per
+// atom, its lanes over the images in the order of the linear index
+// ((p0 - pmin0) rep1 + p1 - pmin1) rep2 + p2 - pmin2 (p2 fastest), 64 per
+// step; the ballot keeps that order.
 template <bool WRITE>
-__global__ void ghosts_kernel(const NLGeom* __restrict__ geom, const int* __restrict__ atom_struct,
-                              const double* __restrict__ pos, int64_t n_atoms,
-                              int* __restrict__ count, const int64_t* __restrict__ off,
-                              double* __restrict__ ext_pos, int* __restrict__ ext_orig,
-                              int64_t cap) {
-  int64_t a = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(256) void ghosts_kernel(const NLGeom* __restrict__ geom,
+                                                     const int* __restrict__ atom_struct,
+                                                     const double* __restrict__ pos, int64_t n_atoms,
+                                                     int* __restrict__ count, const int64_t* __restrict__ off,
+                                                     double* __restrict__ ext_pos, int* __restrict__ ext_orig,
+                                                     int64_t cap) {
+  const int lane = threadIdx.x & 63;
+  const int64_t a = int64_t(blockIdx.x) * (blockDim.x / 64) + (threadIdx.x >> 6);
   if (a >= n_atoms) return;
   const NLGeom& g = geom[atom_struct[a]];
-  const double x0 = pos[3 * a], x1 = pos[3 * a + 1], x2 = pos[3 * a + 2];
+  const double xs[3] = {pos[3 * a], pos[3 * a + 1], pos[3 * a + 2]};
+  const int r12 = g.rep[1] * g.rep[2], n_img = g.rep[0] * r12;
   int c = 0;
-  int64_t w = WRITE ? off[a] + n_atoms : 0;
-  // images in the order of the linear index ((p0 - pmin0) rep1 + p1 - pmin1)
-  // rep2 + p2 - pmin2, p2 fastest
-  const double xs[3] = {x0, x1, x2};
-  for (int p0 = g.pmin[0]; p0 < g.pmin[0] + g.rep[0]; ++p0)
-    for (int p1 = g.pmin[1]; p1 < g.pmin[1] + g.rep[1]; ++p1)
-      for (int p2 = g.pmin[2]; p2 < g.pmin[2] + g.rep[2]; ++p2) {
-        if (p0 == 0 && p1 == 0 && p2 == 0) continue;
-        double xg[3];
-        bool inside = true;
+  const int64_t w = WRITE ? off[a] + n_atoms : 0;
+  for (int k0 = 0; k0 < n_img; k0 += 64) {
+    const int k = k0 + lane;
+    bool inside = false;
+    double xg[3] = {0, 0, 0};
+    if (k < n_img) {
+      const int q0 = k / r12, rem = k - q0 * r12, q1 = rem / g.rep[2], q2 = rem - q1 * g.rep[2];
+      const int p0 = g.pmin[0] + q0, p1 = g.pmin[1] + q1, p2 = g.pmin[2] + q2;
+      if (!(p0 == 0 && p1 == 0 && p2 == 0)) {
+        inside = true;
         for (int d = 0; d < 3; ++d) {
           double t = g.cell[d * 3 + 0] * double(p0);
           t = t + g.cell[d * 3 + 1] * double(p1);
@@ -71,16 +77,21 @@ __global__ void ghosts_kernel(const NLGeom* __restrict__ geom, const int* __rest
           xg[d] = xs[d] + t;
           if (!((xg[d] - g.ghost_min[d]) > g.tol && (xg[d] - g.ghost_max[d]) < -g.tol)) inside = false;
         }
-        if (!inside) continue;
-        if (WRITE && w + c < cap) {
-          ext_pos[3 * (w + c)] = xg[0];
-          ext_pos[3 * (w + c) + 1] = xg[1];
-          ext_pos[3 * (w + c) + 2] = xg[2];
-          ext_orig[w + c] = int(a);
-        }
-        ++c;
       }
-  if (!WRITE) count[a] = c;
+    }
+    const unsigned long long mask = __ballot(inside);
+    if (WRITE && inside) {
+      const int64_t o = w + c + __popcll(mask & ((1ull << lane) - 1ull));
+      if (o < cap) {
+        ext_pos[3 * o] = xg[0];
+        ext_pos[3 * o + 1] = xg[1];
+        ext_pos[3 * o + 2] = xg[2];
+        ext_orig[o] = int(a);
+      }
+    }
+    c += __popcll(mask);
+  }
+  if (!WRITE && lane == 0) count[a] = c;
 }
 
 __global__ void originals_kernel(const double* __restrict__ pos, int64_t n_atoms,
@@ -444,9 +455,9 @@ void launch_pass(PairList& pl, double rc, hipStream_t stream) {
   // ghosts
   int* gcount = pl.ghost_count.get();
   int64_t* goff = pl.ghost_off.get();
-  // one thread per atom: small workgroups spread the atoms over more CUs
-  const int TG = 64;
-  hipLaunchKernelGGL(ghosts_kernel<false>, blocks(n_atoms, TG), dim3(TG), 0, stream, dgeom, dstruct,
+  // one wavefront per atom
+  const dim3 ggrid(unsigned((n_atoms + 3) / 4));
+  hipLaunchKernelGGL(ghosts_kernel<false>, ggrid, dim3(256), 0, stream, dgeom, dstruct,
                      dpos, n_atoms, gcount, nullptr, nullptr, nullptr, int64_t(0));
   RGPU_LAUNCH_CHECK();
   exclusive_scan_device<int>(gcount, goff, n_atoms, stream, pl.scratch);
@@ -455,7 +466,7 @@ void launch_pass(PairList& pl, double rc, hipStream_t stream) {
   hipLaunchKernelGGL(originals_kernel, blocks(n_atoms, TB), dim3(TB), 0, stream, dpos, n_atoms, epos,
                      eorig);
   RGPU_LAUNCH_CHECK();
-  hipLaunchKernelGGL(ghosts_kernel<true>, blocks(n_atoms, TG), dim3(TG), 0, stream, dgeom, dstruct,
+  hipLaunchKernelGGL(ghosts_kernel<true>, ggrid, dim3(256), 0, stream, dgeom, dstruct,
                      dpos, n_atoms, nullptr, goff, epos, eorig, cap_ext);
   RGPU_LAUNCH_CHECK();
   // binning
