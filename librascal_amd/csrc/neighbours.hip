@@ -141,31 +141,25 @@ __global__ void bin_fill_kernel(const int64_t* __restrict__ ext_box, int64_t n_a
   box_items[box_start[bl] + k] = int(t);
 }
 
-// ascending slot order within each box = the reference's tag order
-// boxes of up to kSortSlot items are sorted in LDS (element k of thread t at
-// k * kSortThreads + t), larger ones in place in global memory
-constexpr int kSortThreads = 64, kSortSlot = 32;
-__global__ __launch_bounds__(kSortThreads) void bin_sort_kernel(const int64_t* __restrict__ box_start,
-                                                                int64_t n_boxes, int* __restrict__ box_items) {
-  __shared__ int buf[kSortThreads * kSortSlot];
-  int64_t bx = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+// ascending slot order within each box = the reference's tag order; one
+// wavefront per box: boxes of up to 64 items are ranked in registers (an
+// item's rank is the number of smaller items; slots are distinct), larger
+// ones are sorted in place by the wavefront's first lane
+__global__ __launch_bounds__(256) void bin_sort_kernel(const int64_t* __restrict__ box_start, int64_t n_boxes,
+                                                       int* __restrict__ box_items) {
+  const int lane = threadIdx.x & 63;
+  const int64_t bx = int64_t(blockIdx.x) * (blockDim.x / 64) + (threadIdx.x >> 6);
   if (bx >= n_boxes) return;
-  int64_t s = box_start[bx], e = box_start[bx + 1];
-  if (e - s <= kSortSlot) {
-    const int n = int(e - s), t = threadIdx.x;
-    for (int k = 0; k < n; ++k) buf[k * kSortThreads + t] = box_items[s + k];
-    for (int i = 1; i < n; ++i) {
-      const int v = buf[i * kSortThreads + t];
-      int j = i - 1;
-      while (j >= 0 && buf[j * kSortThreads + t] > v) {
-        buf[(j + 1) * kSortThreads + t] = buf[j * kSortThreads + t];
-        --j;
-      }
-      buf[(j + 1) * kSortThreads + t] = v;
-    }
-    for (int k = 0; k < n; ++k) box_items[s + k] = buf[k * kSortThreads + t];
+  const int64_t s = box_start[bx], e = box_start[bx + 1];
+  const int m = int(e - s);
+  if (e - s <= 64) {
+    const int v = lane < m ? box_items[s + lane] : 0;
+    int r = 0;
+    for (int q = 0; q < m; ++q) r += __shfl(v, q, 64) < v ? 1 : 0;
+    if (lane < m) box_items[s + r] = v;  // every lane read its item above
     return;
   }
+  if (lane != 0) return;
   for (int64_t i = s + 1; i < e; ++i) {
     int v = box_items[i];
     int64_t j = i - 1;
@@ -486,7 +480,7 @@ void launch_pass(PairList& pl, double rc, hipStream_t stream) {
   hipLaunchKernelGGL(bin_fill_kernel, blocks(cap_ext, TB), dim3(TB), 0, stream, ebox, n_atoms,
                      goff + n_atoms, cap_ext, bstart, bfill, bitems);
   RGPU_LAUNCH_CHECK();
-  hipLaunchKernelGGL(bin_sort_kernel, blocks(boxes, kSortThreads), dim3(kSortThreads), 0, stream, bstart, boxes,
+  hipLaunchKernelGGL(bin_sort_kernel, dim3(unsigned((boxes + 3) / 4)), dim3(256), 0, stream, bstart, boxes,
                      bitems);
   RGPU_LAUNCH_CHECK();
   // pairs
