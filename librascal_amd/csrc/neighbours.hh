@@ -34,7 +34,6 @@ struct PairList {
   DeviceBuffer<int64_t> ext_box;     // [n_ext] global box index
   DeviceBuffer<int> box_count;       // [n_boxes]
   DeviceBuffer<int64_t> box_start;   // [n_boxes+1]
-  DeviceBuffer<int> box_fill;        // [n_boxes]
   DeviceBuffer<int> box_items;       // [n_ext]
   DeviceBuffer<int> pair_count;      // [n_atoms]
   DeviceBuffer<int64_t> centre_off;  // [n_atoms+1]

@@ -45,22 +45,35 @@ __device__ __forceinline__ int64_t box_lin(const NLGeom& g, const int* b) {
 // (adaptor_neighbour_list.hh:1044-1072); count or write.  One wavefront per
 // atom, its lanes over the images in the order of the linear index
 // ((p0 - pmin0) rep1 + p1 - pmin1) rep2 + p2 - pmin2 (p2 fastest), 64 per
-// step; the ballot keeps that order.
+// step; the ballot keeps that order.  The write pass also puts the atom
+// itself in the first n_atoms slots; the count pass also clears the error
+// flag and the box counters of the binning that follows.
 template <bool WRITE>
 __global__ __launch_bounds__(256) void ghosts_kernel(const NLGeom* __restrict__ geom,
                                                      const int* __restrict__ atom_struct,
                                                      const double* __restrict__ pos, int64_t n_atoms,
                                                      int* __restrict__ count, const int64_t* __restrict__ off,
                                                      double* __restrict__ ext_pos, int* __restrict__ ext_orig,
-                                                     int64_t cap) {
+                                                     int64_t cap, int* __restrict__ err,
+                                                     int* __restrict__ box_count, int64_t n_boxes) {
   const int lane = threadIdx.x & 63;
   const int64_t a = int64_t(blockIdx.x) * (blockDim.x / 64) + (threadIdx.x >> 6);
   if (a >= n_atoms) return;
+  if (!WRITE && lane == 0) {
+    if (a == 0) *err = 0;
+    for (int64_t b = a; b < n_boxes; b += n_atoms) box_count[b] = 0;
+  }
   const NLGeom& g = geom[atom_struct[a]];
   const double xs[3] = {pos[3 * a], pos[3 * a + 1], pos[3 * a + 2]};
   const int r12 = g.rep[1] * g.rep[2], n_img = g.rep[0] * r12;
   int c = 0;
   const int64_t w = WRITE ? off[a] + n_atoms : 0;
+  if (WRITE && lane == 0) {
+    ext_pos[3 * a] = xs[0];
+    ext_pos[3 * a + 1] = xs[1];
+    ext_pos[3 * a + 2] = xs[2];
+    ext_orig[a] = int(a);
+  }
   for (int k0 = 0; k0 < n_img; k0 += 64) {
     const int k = k0 + lane;
     bool inside = false;
@@ -94,15 +107,6 @@ __global__ __launch_bounds__(256) void ghosts_kernel(const NLGeom* __restrict__ 
   if (!WRITE && lane == 0) count[a] = c;
 }
 
-__global__ void originals_kernel(const double* __restrict__ pos, int64_t n_atoms,
-                                 double* __restrict__ ext_pos, int* __restrict__ ext_orig) {
-  int64_t a = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (a >= n_atoms) return;
-  ext_pos[3 * a] = pos[3 * a];
-  ext_pos[3 * a + 1] = pos[3 * a + 1];
-  ext_pos[3 * a + 2] = pos[3 * a + 2];
-  ext_orig[a] = int(a);
-}
 
 // box of every extended slot (adaptor_neighbour_list.hh:1076-1083,
 // :395-422 range check)
@@ -128,16 +132,18 @@ __global__ void bin_count_kernel(const NLGeom* __restrict__ geom, const int* __r
   atomicAdd(box_count + bl, 1);
 }
 
+// slots of a box taken from the top down (box_count returns to 0); the
+// order inside a box is set by bin_sort_kernel
 __global__ void bin_fill_kernel(const int64_t* __restrict__ ext_box, int64_t n_atoms,
                                 const int64_t* __restrict__ n_ghosts, int64_t cap,
-                                const int64_t* __restrict__ box_start, int* __restrict__ box_fill,
+                                const int64_t* __restrict__ box_start, int* __restrict__ box_count,
                                 int* __restrict__ box_items) {
   int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   const int64_t n_ext = n_atoms + *n_ghosts;
   if (t >= n_ext || t >= cap) return;
   int64_t bl = ext_box[t];
   if (bl < 0) return;
-  int k = atomicAdd(box_fill + bl, 1);
+  int k = atomicSub(box_count + bl, 1) - 1;
   box_items[box_start[bl] + k] = int(t);
 }
 
@@ -399,11 +405,6 @@ bool settle(PairList& pl, int64_t cap_ext, int64_t cap_pairs) {
   return false;
 }
 
-__global__ void zero_i32_kernel(int* __restrict__ p, int64_t n) {
-  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = 0;
-}
-
 // every buffer of one device pass at the current capacities (reserved
 // before a capture, so that nothing is allocated inside it)
 void reserve_pass(PairList& pl) {
@@ -416,7 +417,6 @@ void reserve_pass(PairList& pl) {
   pl.ext_orig.reserve(pl.cap_ext);
   pl.box_count.reserve(boxes);
   pl.box_start.reserve(boxes + 1);
-  pl.box_fill.reserve(boxes);
   pl.box_items.reserve(pl.cap_ext);
   pl.ext_box.reserve(pl.cap_ext);
   pl.pair_count.reserve(n_atoms);
@@ -444,41 +444,30 @@ void launch_pass(PairList& pl, double rc, hipStream_t stream) {
                      3 * n_atoms, dpos);
   RGPU_LAUNCH_CHECK();
   int* err = pl.err.get();
-  hipLaunchKernelGGL(zero_i32_kernel, dim3(1), dim3(64), 0, stream, err, int64_t(1));
-  RGPU_LAUNCH_CHECK();
-  // ghosts
+  int* bcount = pl.box_count.get();
+  // ghosts (one wavefront per atom); the count pass clears err and bcount
   int* gcount = pl.ghost_count.get();
   int64_t* goff = pl.ghost_off.get();
-  // one wavefront per atom
   const dim3 ggrid(unsigned((n_atoms + 3) / 4));
   hipLaunchKernelGGL(ghosts_kernel<false>, ggrid, dim3(256), 0, stream, dgeom, dstruct,
-                     dpos, n_atoms, gcount, nullptr, nullptr, nullptr, int64_t(0));
+                     dpos, n_atoms, gcount, nullptr, nullptr, nullptr, int64_t(0), err, bcount, boxes);
   RGPU_LAUNCH_CHECK();
   exclusive_scan_device<int>(gcount, goff, n_atoms, stream, pl.scratch);
   double* epos = pl.ext_pos.get();
   int* eorig = pl.ext_orig.get();
-  hipLaunchKernelGGL(originals_kernel, blocks(n_atoms, TB), dim3(TB), 0, stream, dpos, n_atoms, epos,
-                     eorig);
-  RGPU_LAUNCH_CHECK();
   hipLaunchKernelGGL(ghosts_kernel<true>, ggrid, dim3(256), 0, stream, dgeom, dstruct,
-                     dpos, n_atoms, nullptr, goff, epos, eorig, cap_ext);
+                     dpos, n_atoms, nullptr, goff, epos, eorig, cap_ext, nullptr, nullptr, int64_t(0));
   RGPU_LAUNCH_CHECK();
   // binning
-  int* bcount = pl.box_count.get();
   int64_t* bstart = pl.box_start.get();
-  int* bfill = pl.box_fill.get();
   int* bitems = pl.box_items.get();
   int64_t* ebox = pl.ext_box.get();
-  hipLaunchKernelGGL(zero_i32_kernel, blocks(boxes, TB), dim3(TB), 0, stream, bcount, boxes);
-  RGPU_LAUNCH_CHECK();
-  hipLaunchKernelGGL(zero_i32_kernel, blocks(boxes, TB), dim3(TB), 0, stream, bfill, boxes);
-  RGPU_LAUNCH_CHECK();
   hipLaunchKernelGGL(bin_count_kernel, blocks(cap_ext, TB), dim3(TB), 0, stream, dgeom, dstruct, epos,
                      eorig, n_atoms, goff + n_atoms, cap_ext, rc, ebox, bcount, err);
   RGPU_LAUNCH_CHECK();
   exclusive_scan_device<int>(bcount, bstart, boxes, stream, pl.scratch2);
   hipLaunchKernelGGL(bin_fill_kernel, blocks(cap_ext, TB), dim3(TB), 0, stream, ebox, n_atoms,
-                     goff + n_atoms, cap_ext, bstart, bfill, bitems);
+                     goff + n_atoms, cap_ext, bstart, bcount, bitems);
   RGPU_LAUNCH_CHECK();
   hipLaunchKernelGGL(bin_sort_kernel, dim3(unsigned((boxes + 3) / 4)), dim3(256), 0, stream, bstart, boxes,
                      bitems);
@@ -511,7 +500,7 @@ void launch_pass_graph(PairList& pl, double rc, hipStream_t stream) {
       uintptr_t(pl.atom_struct.get()), uintptr_t(pl.pos.get()), uintptr_t(pl.pos_stage.device()),
       uintptr_t(pl.err.get()), uintptr_t(pl.ghost_count.get()), uintptr_t(pl.ghost_off.get()),
       uintptr_t(pl.ext_pos.get()), uintptr_t(pl.ext_orig.get()), uintptr_t(pl.box_count.get()),
-      uintptr_t(pl.box_start.get()), uintptr_t(pl.box_fill.get()), uintptr_t(pl.box_items.get()),
+      uintptr_t(pl.box_start.get()), uintptr_t(pl.box_items.get()),
       uintptr_t(pl.ext_box.get()), uintptr_t(pl.pair_count.get()), uintptr_t(pl.centre_off.get()),
       uintptr_t(pl.pair_j.get()), uintptr_t(pl.pair_vec.get()), uintptr_t(pl.pair_dist.get()),
       uintptr_t(pl.tot_stage.device()), uintptr_t(pl.scratch.get()), uintptr_t(pl.scratch2.get()),
