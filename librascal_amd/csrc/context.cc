@@ -88,6 +88,7 @@ struct Context {
   // reverse pair lists of the current neighbour list (rev_valid)
   DeviceBuffer<double> pair_force, centre_force;
   DeviceBuffer<int> rev_count;
+  int64_t rev_count_zero = 0;  // rev_count[0, rev_count_zero) holds zeros
   DeviceBuffer<int64_t> rev_off, rev_list, rev_scan;
   bool rev_valid = false;
   // half neighbour list (rgpu_set_half_list) and its contribution rows
@@ -550,11 +551,18 @@ PairView backward_view(Context& c, ReverseLists* rl) {
   const int64_t n = c.n_atoms, cap = std::max<int64_t>(n > 0 ? c.pl.cap_pairs : 0, 1);
   pv.pair_force = c.pair_force.reserve(size_t(3 * cap));
   pv.centre_force = c.centre_force.reserve(size_t(std::max<int64_t>(3 * n, 1)));
+  const size_t count_cap = c.rev_count.capacity();
   rl->count = c.rev_count.reserve(size_t(std::max<int64_t>(n, 1)));
+  if (c.rev_count.capacity() != count_cap) c.rev_count_zero = 0;  // fresh memory
   rl->roff = c.rev_off.reserve(size_t(n + 1));
   rl->rlist = c.rev_list.reserve(size_t(cap));
   if (!c.rev_valid) {
-    launch_reverse_pairs(pv, *rl, c.rev_scan, c.stream);
+    // the fill takes every count it raised back to zero (a launch that
+    // throws leaves the counts unknown)
+    const int64_t zero = c.rev_count_zero;
+    c.rev_count_zero = 0;
+    launch_reverse_pairs(pv, *rl, c.rev_scan, c.stream, n > zero);
+    c.rev_count_zero = std::max(zero, n);
     c.rev_valid = true;
   }
   return pv;
@@ -1026,7 +1034,7 @@ void gap_predict_t(Context& c, double* energy, double* gradient, double* neg_str
   if (want_grad) {
     gd = c.grad.reserve(size_t(std::max<int64_t>(3 * n, 1)));
     cv = c.cvir.reserve(size_t(std::max<int64_t>(6 * n, 1)));
-    RGPU_HIP(hipMemsetAsync(gd, 0, sizeof(double) * 3 * n, c.stream));
+    // (the force gather writes every atom's gradient)
     // the GAP energy is homogeneous of degree zeta in X, so X_i . G_i = zeta E_i
     backward_device<T>(c, G, gd, cv, ec, double(c.zeta));
   }
@@ -1809,7 +1817,6 @@ int rgpu_features_backward(rgpu_context* ctx, const double* dL_dX, double* dL_dr
     const int64_t D = c.D(), n = c.n_atoms;
     double* gd = c.grad.reserve(size_t(std::max<int64_t>(3 * n, 1)));
     double* cv = c.cvir.reserve(size_t(std::max<int64_t>(6 * n, 1)));
-    RGPU_HIP(hipMemsetAsync(gd, 0, sizeof(double) * 3 * n, c.stream));
     double* g64 = c.tmp64.reserve(size_t(std::max<int64_t>(n * D, 1)));
     RGPU_HIP(hipMemcpyAsync(g64, dL_dX, sizeof(double) * n * D, hipMemcpyDefault, c.stream));
     if (c.precision == RGPU_FP64) {

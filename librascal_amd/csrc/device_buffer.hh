@@ -54,6 +54,7 @@ class DeviceBuffer {
   T* get() const { return ptr_; }
   size_t size() const { return size_; }
   size_t bytes() const { return size_ * sizeof(T); }
+  size_t capacity() const { return cap_; }
 
  private:
   T* ptr_ = nullptr;

@@ -37,8 +37,10 @@ struct ReverseLists {
   int64_t* roff = nullptr;
   int64_t* rlist = nullptr;
 };
+// (clear_count: count is not known to be zero; the fill returns every
+// count to zero)
 void launch_reverse_pairs(const PairView& pv, const ReverseLists& rl, DeviceBuffer<int64_t>& scan_scratch,
-                          hipStream_t stream);
+                          hipStream_t stream, bool clear_count);
 // grad[j] = centre_force[j] + sum over rlist of j of pair_force[p], in
 // ascending pair order (the forces of the adjoint, summed in a fixed order)
 void launch_force_gather(const PairView& pv, const ReverseLists& rl, double* grad, hipStream_t stream);

@@ -3939,11 +3939,11 @@ __global__ __launch_bounds__(256) void force_gather_kernel(int64_t n, const int6
 }  // namespace
 
 void launch_reverse_pairs(const PairView& pv, const ReverseLists& rl, DeviceBuffer<int64_t>& scan_scratch,
-                          hipStream_t stream) {
+                          hipStream_t stream, bool clear_count) {
   const int64_t n = pv.n_centres;
   if (n == 0) return;
   const unsigned grid = unsigned((n + 3) / 4);  // a wavefront per centre
-  RGPU_HIP(hipMemsetAsync(rl.count, 0, sizeof(int) * n, stream));
+  if (clear_count) RGPU_HIP(hipMemsetAsync(rl.count, 0, sizeof(int) * n, stream));
   hipLaunchKernelGGL(rev_count_kernel, dim3(grid), dim3(256), 0, stream, n, pv.centre_off, pv.pair_j, rl.count);
   RGPU_LAUNCH_CHECK();
   exclusive_scan_reserve(n, scan_scratch);
