@@ -625,11 +625,14 @@ __device__ void ps_tables(int N, int S, int* key_ab, int* tri) {
 // filled), in KC-key passes over the stage buffer; writes the FP64/FP32
 // rows or the BF16 operand planes and norm2[i].  Every thread calls; ends
 // after a barrier.
-template <class T, bool SPLIT>
-__device__ __forceinline__ void ps_body(int N, int L, int S, int KC, int64_t Dq, int64_t i, const T* cs, T* stage,
+template <class T, bool SPLIT, int NC = -1, int LC = -1>
+__device__ __forceinline__ void ps_body(int N_, int L_, int S, int KC, int64_t Dq, int64_t i, const T* cs, T* stage,
                         const int* key_ab, const int* tri, T* __restrict__ p,
                         unsigned short* __restrict__ p_hi, unsigned short* __restrict__ p_lo,
                         double* __restrict__ norm2) {
+  // NC, LC: N and L fixed at compile time (the LDS offsets of a tile's rows
+  // and stage entries become immediates), or -1 for the runtime values
+  const int N = NC >= 0 ? NC : N_, L = LC >= 0 ? LC : L_;
   const int L1 = L + 1, LS = (L1 * L1) | 1;
   const int K = S * (S + 1) / 2;
   const int NT = (N + 3) / 4;
@@ -785,13 +788,14 @@ __device__ __forceinline__ void ps_body(int N, int L, int S, int KC, int64_t Dq,
 // out row r = row rows[r] of p (rows null: row r), divided by its norm when
 // normalising
 
-template <class T, bool SPLIT>
-__global__ __launch_bounds__(kThreads) void power_spectrum_kernel(int N, int L, int S, int KC, int64_t Dq,
+template <class T, bool SPLIT, int NC = -1, int LC = -1>
+__global__ __launch_bounds__(kThreads) void power_spectrum_kernel(int N_, int L_, int S, int KC, int64_t Dq,
                                                                   const T* __restrict__ c,
                                                                   T* __restrict__ p,
                                                                   unsigned short* __restrict__ p_hi,
                                                                   unsigned short* __restrict__ p_lo,
                                                                   double* __restrict__ norm2) {
+  const int N = NC >= 0 ? NC : N_, L = LC >= 0 ? LC : L_;
   const int64_t i = blockIdx.x;
   const int L1 = L + 1, LL = L1 * L1, B = N * LL, LS = LL | 1;
   const int K = S * (S + 1) / 2;
@@ -810,7 +814,7 @@ __global__ __launch_bounds__(kThreads) void power_spectrum_kernel(int N, int L, 
   for (int e = threadIdx.x; e < 3 * LS; e += kThreads) cs[S * N * LS + e] = T(0);
   ps_tables<SPLIT>(N, S, key_ab, tri);
   __syncthreads();
-  ps_body<T, SPLIT>(N, L, S, KC, Dq, i, cs, stage, key_ab, tri, p, p_hi, p_lo, norm2);
+  ps_body<T, SPLIT, NC, LC>(N, L, S, KC, Dq, i, cs, stage, key_ab, tri, p, p_hi, p_lo, norm2);
 }
 
 // ---------------------------------------------------------------------------
@@ -3432,7 +3436,14 @@ void launch_power_spectrum_impl(const RadialDev& rad, int S, int64_t n, const T*
   if (tiles.kc > 0) KC = tiles.kc;
   KC = std::max(1, std::min(KC, K));
   const size_t smem = cbytes + size_t(KC) * kb + sizeof(int) * size_t(K + rad.N * (rad.N + 1) / 2);
-  if (p_hi) {
+  // the benchmark shape (n_max 12, l_max 9) with its sizes fixed at compile time
+  constexpr bool kFixedShape = true;
+  if (p_hi && kFixedShape && rad.N == 12 && rad.L == 9) {
+    auto kern = power_spectrum_kernel<T, true, 12, 9>;
+    allow_lds(kern, smem);
+    hipLaunchKernelGGL(kern, dim3(unsigned(n)), dim3(kThreads), smem, stream, rad.N, rad.L, S, KC,
+                       packed_row_stride(S, rad.N, rad.L1), c, p, p_hi, p_lo, norm2);
+  } else if (p_hi) {
     auto kern = power_spectrum_kernel<T, true>;
     allow_lds(kern, smem);
     hipLaunchKernelGGL(kern, dim3(unsigned(n)), dim3(kThreads), smem, stream, rad.N, rad.L, S, KC,
