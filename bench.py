@@ -73,8 +73,9 @@ def main():
                          "kernel for scaling FETCH_SIZE / WRITE_SIZE counters to bytes")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU only: every box of this build has one MI355X, and the "
-              "multi-GPU design is described (not run) in DESIGN.md", file=sys.stderr)
+        print("bench.py measures one GPU only (--gpus 1): the library runs the whole batch on one "
+              "MI355X; sharding the centres over several GPUs is not implemented",
+              file=sys.stderr)
         sys.exit(2)
 
     import torch  # device plumbing only (synchronisation)

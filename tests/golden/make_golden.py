@@ -3,7 +3,7 @@
 Reads only DATA files of the reference tree (UBJSON/JSON/XYZ fixtures and
 input structures) and stores compact subsets under tests/golden/.  Run it in
 a checkout that has the reference at REFERENCE_ROOT (default
-/root/reference); the GPU box never needs it.
+/root/reference); the tests read only the fixtures it writes.
 
     python tests/golden/make_golden.py
 """
