@@ -428,6 +428,36 @@ def test_reductions_are_repeatable(precision):
 
 
 @pytest.mark.parametrize("precision", [_capi.FP64, _capi.MIXED])
+def test_species_without_sparse_points(precision):
+    """A species of the batch with no sparse points adds nothing to the
+    energy (its kernel rows are empty) but its atoms still move the other
+    centres' environments: energies, forces and stress against the oracle on
+    a 3-species box whose sparse points cover two species, 200 per species
+    (several column tiles), and bit-for-bit repeatable."""
+    sts = periodic_box(300, 0.06, (1, 6, 8), d_min=1.0, seed=43)
+    hp = soap_hypers(rc=4.0, sw=0.5, sigma=0.5, N=6, L=4, spline=1e-8)
+    ctx, sp = make_ctx(hp, sts)
+    X, _ = ctx.features()
+    num = pack(sts)[2]
+    rng = np.random.default_rng(44)
+    spec = np.concatenate([np.full(200, t, np.int32) for t in (1, 8)])
+    vals = np.concatenate([X[rng.choice(np.where(num == t)[0], 200)] for t in (1, 8)])
+    w = rng.normal(size=len(spec))
+    ctx.set_sparse_points(2, spec, vals, w)
+    ctx.set_precision(precision)
+    e, g, s = ctx.gap_predict()
+    e_ref, g_ref, s_ref = O.gap_predict(O.Calculator(hp), O.BatchData(sts, species=sp),
+                                        O.SparseSet(spec, vals, 2), w)
+    tol = FP64_TOL if precision == _capi.FP64 else 1e-4
+    assert rel_err(e_ref, e).max() < tol
+    assert np.abs(g - g_ref).max() < tol * np.abs(g_ref).max()
+    if precision == _capi.FP64:
+        assert np.abs(s - s_ref).max() < tol * np.abs(s_ref).max()
+    for a, b in zip((e, g, s), ctx.gap_predict()):
+        assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("precision", [_capi.FP64, _capi.MIXED])
 def test_forces_with_long_reverse_lists(precision):
     """Atoms named by more than 256 pairs (dense jittered diamond, a 3.57,
     rc 7.5: about 300 neighbours) have their pair forces summed on the
