@@ -21,6 +21,7 @@
 #include "neighbours.hh"
 #include "packing.hh"
 #include "radial.hh"
+#include "scan.hh"
 #include "soap_kernels.hh"
 
 namespace rgpu {
@@ -89,6 +90,17 @@ struct Context {
   DeviceBuffer<double> pair_force, centre_force;
   DeviceBuffer<int> rev_count;
   int64_t rev_count_zero = 0;  // rev_count[0, rev_count_zero) holds zeros
+  // the reverse lists of a prediction are built on a second stream while
+  // its GEMMs run (rev_pending: the context stream has not waited yet)
+  hipStream_t side = nullptr;
+  hipEvent_t ev_list = nullptr, ev_rev = nullptr;
+  bool rev_pending = false;
+  // the context stream waits for the side stream's reverse lists
+  void join_side() {
+    if (!rev_pending) return;
+    RGPU_HIP(hipStreamWaitEvent(stream, ev_rev, 0));
+    rev_pending = false;
+  }
   DeviceBuffer<int64_t> rev_off, rev_list, rev_scan;
   bool rev_valid = false;
   // half neighbour list (rgpu_set_half_list) and its contribution rows
@@ -190,6 +202,12 @@ struct Context {
       (void)hipEventDestroy(p.b);
     }
     for (auto e : ev_pool) (void)hipEventDestroy(e);
+    if (side) {
+      (void)hipStreamSynchronize(side);
+      (void)hipEventDestroy(ev_list);
+      (void)hipEventDestroy(ev_rev);
+      (void)hipStreamDestroy(side);
+    }
     if (own_stream) (void)hipStreamDestroy(own_stream);
   }
 
@@ -491,6 +509,7 @@ void require_batch_unsettled(const Context& c) {
 bool settle_nl(Context& c) {
   if (!c.nl_stale && finish_pair_list(c.pl, c.stream)) return true;
   c.nl_stale = false;
+  c.join_side();  // the side stream may still read the list
   rebuild_pair_list(c.pl, c.hp.cutoff, c.stream);
   c.have_rep = false;
   c.rev_valid = false;
@@ -508,6 +527,7 @@ double cell_volume(const double* C) {
 }
 
 void build_batch_nl(Context& c, const double* positions, bool defer = false) {
+  c.join_side();  // the side stream may still read the list
   StageTimer t(c, RGPU_STAGE_NEIGHBOURS);
   c.have_rep = false;
   c.rev_valid = false;
@@ -544,6 +564,49 @@ PairView pair_view(Context& c) {
   return pv;
 }
 
+// Room for the reverse pair lists of the current list (the same buffers on
+// every call for one list)
+void reverse_buffers(Context& c, ReverseLists* rl) {
+  const int64_t n = c.n_atoms, cap = std::max<int64_t>(n > 0 ? c.pl.cap_pairs : 0, 1);
+  const size_t count_cap = c.rev_count.capacity();
+  rl->count = c.rev_count.reserve(size_t(std::max<int64_t>(n, 1)));
+  if (c.rev_count.capacity() != count_cap) c.rev_count_zero = 0;  // fresh memory
+  rl->roff = c.rev_off.reserve(size_t(n + 1));
+  rl->rlist = c.rev_list.reserve(size_t(cap));
+}
+
+// Builds the reverse pair lists of the current list on stream s (after the
+// list itself on that stream).
+void build_reverse_lists(Context& c, const PairView& pv, const ReverseLists& rl, hipStream_t s) {
+  // the fill takes every count it raised back to zero (a launch that throws
+  // leaves the counts unknown)
+  const int64_t n = c.n_atoms, zero = c.rev_count_zero;
+  c.rev_count_zero = 0;
+  launch_reverse_pairs(pv, rl, c.rev_scan, s, n > zero);
+  c.rev_count_zero = std::max(zero, n);
+  c.rev_valid = true;
+}
+
+// Starts the reverse pair lists of the current list on the side stream,
+// after the work already on the context stream, so that they are built
+// while the GEMMs run; backward_view makes the context stream wait for them.
+void prefetch_reverse_lists(Context& c) {
+  if (c.rev_valid || c.n_atoms == 0) return;
+  if (!c.side) {
+    RGPU_HIP(hipStreamCreateWithFlags(&c.side, hipStreamNonBlocking));
+    RGPU_HIP(hipEventCreateWithFlags(&c.ev_list, hipEventDisableTiming));
+    RGPU_HIP(hipEventCreateWithFlags(&c.ev_rev, hipEventDisableTiming));
+  }
+  ReverseLists rl;
+  reverse_buffers(c, &rl);
+  exclusive_scan_reserve(c.n_atoms, c.rev_scan);  // no allocation on the side stream's way
+  RGPU_HIP(hipEventRecord(c.ev_list, c.stream));
+  RGPU_HIP(hipStreamWaitEvent(c.side, c.ev_list, 0));
+  build_reverse_lists(c, pair_view(c), rl, c.side);
+  RGPU_HIP(hipEventRecord(c.ev_rev, c.side));
+  c.rev_pending = true;
+}
+
 // The pair view of an adjoint launch: room for its pair and centre forces,
 // and the reverse pair lists of the current list (built once per list)
 PairView backward_view(Context& c, ReverseLists* rl) {
@@ -551,20 +614,9 @@ PairView backward_view(Context& c, ReverseLists* rl) {
   const int64_t n = c.n_atoms, cap = std::max<int64_t>(n > 0 ? c.pl.cap_pairs : 0, 1);
   pv.pair_force = c.pair_force.reserve(size_t(3 * cap));
   pv.centre_force = c.centre_force.reserve(size_t(std::max<int64_t>(3 * n, 1)));
-  const size_t count_cap = c.rev_count.capacity();
-  rl->count = c.rev_count.reserve(size_t(std::max<int64_t>(n, 1)));
-  if (c.rev_count.capacity() != count_cap) c.rev_count_zero = 0;  // fresh memory
-  rl->roff = c.rev_off.reserve(size_t(n + 1));
-  rl->rlist = c.rev_list.reserve(size_t(cap));
-  if (!c.rev_valid) {
-    // the fill takes every count it raised back to zero (a launch that
-    // throws leaves the counts unknown)
-    const int64_t zero = c.rev_count_zero;
-    c.rev_count_zero = 0;
-    launch_reverse_pairs(pv, *rl, c.rev_scan, c.stream, n > zero);
-    c.rev_count_zero = std::max(zero, n);
-    c.rev_valid = true;
-  }
+  reverse_buffers(c, rl);
+  if (!c.rev_valid) build_reverse_lists(c, pv, *rl, c.stream);
+  c.join_side();
   return pv;
 }
 
@@ -2039,6 +2091,9 @@ int rgpu_gap_predict(rgpu_context* ctx, double* energy, double* gradient, double
     // capacity was short held no pairs, so it is built again and the
     // prediction repeated
     for (int attempt = 0; attempt < 2; ++attempt) {
+      // the reverse pair lists of the forces are built beside the
+      // representation stage
+      if ((gradient || neg_stress) && !c.masked) prefetch_reverse_lists(c);
       ensure_rep(c);
       if (c.precision == RGPU_FP64)
         gap_predict_t<double>(c, energy, gradient, neg_stress);
