@@ -84,60 +84,47 @@ __global__ static __launch_bounds__(kScanThreads) void scan_small_kernel(
   if (threadIdx.x == 0) *total = carry_sh;
 }
 
-// Whole scan in one workgroup of 1024 threads, for short arrays, where one
-// launch beats three: chunks of 8192 items with a running carry; wavefront
-// w of a chunk takes items [512 w, 512 w + 512) as eight coalesced rounds of
-// 64, scans each round with shuffles, chains the rounds, and adds the totals
-// of the wavefronts before it (one barrier per chunk).  out[n] = total.
+// Whole scan in one workgroup of 1024 threads (8 items each per chunk,
+// running carry between chunks), out[n] = total; for short arrays, where
+// one launch beats three.
 constexpr int kScan1Threads = 1024;
 template <class In>
 __global__ __launch_bounds__(kScan1Threads) void scan_single_kernel(const In* __restrict__ in,
                                                                     int64_t* __restrict__ out,
                                                                     int64_t n) {
-  constexpr int W = kScan1Threads / 64, R = kScanItems;
-  __shared__ int64_t wsum[2][W];
+  __shared__ int64_t wsum[kScan1Threads / 64];
+  __shared__ int64_t carry_sh;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  int64_t carry = 0;
-  int buf = 0;
-  for (int64_t c0 = 0; c0 < n; c0 += int64_t(kScan1Threads) * R, buf ^= 1) {
-    const int64_t wbase = c0 + int64_t(wave) * 64 * R;
-    int64_t v[R], incl[R];
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-      const int64_t idx = wbase + 64 * r + lane;
-      v[r] = idx < n ? int64_t(in[idx]) : 0;
-      incl[r] = v[r];
+  if (threadIdx.x == 0) carry_sh = 0;
+  __syncthreads();
+  for (int64_t c0 = 0; c0 < n; c0 += int64_t(kScan1Threads) * kScanItems) {
+    const int64_t base = c0 + int64_t(threadIdx.x) * kScanItems;
+    int64_t v[kScanItems];
+    int64_t tsum = 0;
+    for (int k = 0; k < kScanItems; ++k) {
+      v[k] = base + k < n ? int64_t(in[base + k]) : 0;
+      tsum += v[k];
     }
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1)
-#pragma unroll
-      for (int r = 0; r < R; ++r) {
-        const int64_t o = __shfl_up(incl[r], off, 64);
-        if (lane >= off) incl[r] += o;
-      }
-    // chain the rounds: round r starts after the totals of rounds < r
-    int64_t run = 0;
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-      const int64_t tot = __shfl(incl[r], 63, 64);
-      incl[r] += run;
-      run += tot;
+    int64_t incl = tsum;
+    for (int off = 1; off < 64; off <<= 1) {
+      const int64_t o = __shfl_up(incl, off, 64);
+      if (lane >= off) incl += o;
     }
-    if (lane == 0) wsum[buf][wave] = run;
+    if (lane == 63) wsum[wave] = incl;
     __syncthreads();
-    int64_t before = carry;
-    for (int w = 0; w < W; ++w) {
-      const int64_t t = wsum[buf][w];
-      if (w < wave) before += t;
-      carry += t;
+    int64_t run = carry_sh;
+    for (int w = 0; w < wave; ++w) run += wsum[w];
+    const int64_t last = run + incl;  // inclusive end of this thread
+    run += incl - tsum;
+    for (int k = 0; k < kScanItems; ++k) {
+      if (base + k < n) out[base + k] = run;
+      run += v[k];
     }
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-      const int64_t idx = wbase + 64 * r + lane;
-      if (idx < n) out[idx] = before + incl[r] - v[r];
-    }
+    __syncthreads();
+    if (threadIdx.x == kScan1Threads - 1) carry_sh = last;
+    __syncthreads();
   }
-  if (threadIdx.x == 0) out[n] = carry;
+  if (threadIdx.x == 0) out[n] = carry_sh;
 }
 
 // scratch that exclusive_scan_device(n) uses (reserved ahead of a graph
