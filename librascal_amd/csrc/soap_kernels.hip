@@ -122,10 +122,11 @@ __device__ void load_ylm_table(const RadialDev& R, T* dst) {
 // Legendre recursion with the sqrt(1/(2 pi)) normalisation folded in, m = 0
 // scaled by 1/sqrt(2), cos/sin(m phi) by Chebyshev recurrence, phi = 0 on
 // the z axis, and the pole/equator branch of the gradient at sin(theta) 0.1.
-template <class T>
-__device__ void ylm_wave(int L, const T* __restrict__ yc, const T* dirs, int q0, int nq, T* Y, T* dY,
+template <class T, int LC = -1>
+__device__ void ylm_wave(int L_, const T* __restrict__ yc, const T* dirs, int q0, int nq, T* Y, T* dY,
                          int ystride, int dystride) {
   typedef typename Vec4<T>::type V4;
+  const int L = LC >= 0 ? LC : L_;  // LC: l_max fixed at compile time
   const int L1 = L + 1;
   const int lane = threadIdx.x & 63;
   const int seg = lane / L1;
@@ -386,11 +387,12 @@ __device__ __forceinline__ void pair_fetch(const RadialDev& rad, const PairView&
 // species: s_count[s] pairs listed from lst[s_start[s]].  Wavefronts with
 // no Y_lm work take the radial functions.  All threads call; returns after
 // a barrier.
-template <class T, bool DIRECT, bool ADJ>
+template <class T, bool DIRECT, bool ADJ, int NC = -1, int LC = -1>
 __device__ void stage_pairs(const RadialDev& rad, const T* yc, const PairPre& f, int nb,
                             PairBatch<T>& b, int* s_start, int* s_count,
                             unsigned long long* kmask) {
-  const int N = rad.N, L = rad.L, L1 = rad.L1, LL = rad.LL, F = N * L1;
+  // NC, LC: n_max and l_max fixed at compile time (or -1)
+  const int N = NC >= 0 ? NC : rad.N, L = LC >= 0 ? LC : rad.L, L1 = L + 1, LL = L1 * L1, F = N * L1;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   if (tid < nb) {
     const double d = f.d;
@@ -438,7 +440,7 @@ __device__ void stage_pairs(const RadialDev& rad, const T* yc, const PairPre& f,
   }
   const int per_wave = 64 / L1;
   for (int qb = wave * per_wave; qb < nb; qb += kWaves * per_wave)
-    ylm_wave<T>(L, yc, b.dir, qb, nb, b.Y, ADJ ? b.dY : nullptr, LL, 3 * LL);
+    ylm_wave<T, LC>(L, yc, b.dir, qb, nb, b.Y, ADJ ? b.dY : nullptr, LL, 3 * LL);
   // radial functions on the wavefronts left idle by the Y_lm pass
   const int ywaves = (nb + per_wave - 1) / per_wave;
   const int r0 = ywaves >= kWaves ? 0 : ywaves * 64;
@@ -512,13 +514,14 @@ constexpr int expansion_min_waves() {
 // ---------------------------------------------------------------------------
 // Expansion forward: c[i][s][n][lm] = [s == s_i][lm == 0] R0[n] / sqrt(4 pi)
 //   + sum over pairs (i, j), s_j = s, of fc(r) R_nl(r) Y_lm(r_hat)
-// KPT = entries of one species block per thread.
-template <class T, int KPT, bool DIRECT>
+// KPT = entries of one species block per thread; NC, LC: n_max and l_max
+// fixed at compile time (or -1).
+template <class T, int KPT, bool DIRECT, int NC = -1, int LC = -1>
 __global__ __launch_bounds__(kThreads, (expansion_min_waves<T, KPT, DIRECT>())) void expansion_kernel(
     RadialDev rad, int S, int PB, PairView pv, T* __restrict__ cout,
     unsigned long long* __restrict__ keymask) {
   const int64_t i = blockIdx.x;
-  const int N = rad.N, L1 = rad.L1, LL = rad.LL, F = N * L1, B = N * LL;
+  const int N = NC >= 0 ? NC : rad.N, L1 = LC >= 0 ? LC + 1 : rad.L1, LL = L1 * L1, F = N * L1, B = N * LL;
   extern __shared__ __align__(64) unsigned char smem_raw[];
   T* yc = reinterpret_cast<T*>(smem_raw);
   PairBatch<T> pb = carve_batch<T, DIRECT, false>(smem_raw + ylm_table_bytes<T>(L1), PB, LL, F);
@@ -555,7 +558,7 @@ __global__ __launch_bounds__(kThreads, (expansion_min_waves<T, KPT, DIRECT>())) 
   for (int q0 = 0; q0 == 0 || q0 < np; q0 += PB) {
     const int nb = max(0, min(PB, np - q0));
     if (tid < S) s_count[tid] = 0;
-    stage_pairs<T, DIRECT, false>(rad, yc, pre, nb, pb, s_start, s_count, &kmask);
+    stage_pairs<T, DIRECT, false, NC, LC>(rad, yc, pre, nb, pb, s_start, s_count, &kmask);
     // next batch's inputs in flight during this batch's accumulation
     if (q0 + PB < np) pair_fetch<DIRECT, false>(rad, pv, p0 + q0 + PB, min(PB, np - q0 - PB), pre);
     for (int s = 0; s < S; ++s) {
@@ -3136,7 +3139,7 @@ void allow_lds(K kernel, size_t smem) {
                                  hipFuncAttributeMaxDynamicSharedMemorySize, int(smem)));
 }
 
-template <class T, int KPT, bool DIRECT>
+template <class T, int KPT, bool DIRECT, int NC = -1, int LC = -1>
 void launch_expansion_t(const RadialDev& rad, int S, const PairView& pv, T* c,
                         unsigned long long* keymask, hipStream_t stream, const SoapTiles& tiles) {
   const int F = rad.N * rad.L1;
@@ -3148,7 +3151,7 @@ void launch_expansion_t(const RadialDev& rad, int S, const PairView& pv, T* c,
     while (PB > 1 && yt + pair_batch_bytes(PB, rad.LL, F, sizeof(T), DIRECT, false) > kFwdBudget)
       PB /= 2;
   const size_t smem = yt + pair_batch_bytes(PB, rad.LL, F, sizeof(T), DIRECT, false);
-  auto kern = expansion_kernel<T, KPT, DIRECT>;
+  auto kern = expansion_kernel<T, KPT, DIRECT, NC, LC>;
   allow_lds(kern, smem);
   hipLaunchKernelGGL(kern, dim3(unsigned(pv.n_centres)), dim3(kThreads), smem, stream, rad, S, PB, pv,
                      c, keymask);
@@ -3405,6 +3408,13 @@ void launch_expansion(const RadialDev& rad, int S, const PairView& pv, T* c,
   if (launch_expansion_group<T>(rad, S, pv, c, keymask, stream, tiles)) return;
   const int B = rad.N * rad.LL;
   const int kpt = (B + kThreads - 1) / kThreads;
+  // the benchmark shape (n_max 12, l_max 9, spline radial functions) with
+  // its sizes fixed at compile time
+  constexpr bool kFixedShape = true;
+  if (kFixedShape && rad.use_spline && rad.N == 12 && rad.L == 9) {
+    launch_expansion_t<T, 5, false, 12, 9>(rad, S, pv, c, keymask, stream, tiles);
+    return;
+  }
   if (kpt <= 2)
     launch_expansion_k<T, 2>(rad, S, pv, c, keymask, stream, tiles);
   else if (kpt <= 4)
