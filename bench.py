@@ -265,8 +265,8 @@ SQ_FILE = "profiles/sq_counters_periodic_box_8192.md"
 # direct-to-LDS GEMMs; the adjoint kernel is the autotuner's choice)
 GEMM_KERNEL = "bf16x3_gemm_dma<0, 128, 2>"
 FORCE_GEMM_KERNEL = "bf16x3_gemm_dma<1, 128, 2>"
-EXPANSION_KERNEL = "expansion_kernel<float, 5, false>"
-PS_KERNEL = "power_spectrum_kernel<float, true>"
+EXPANSION_KERNEL = "expansion_kernel<float, 5, false, 12, 9>"
+PS_KERNEL = "power_spectrum_kernel<float, true, 12, 9>"
 ADJOINT_KERNELS = {0: ["adjoint_staged_kernel<false, 5>"], 1: ["adjoint_mfma_kernel<false, 12>"],
                    2: ["adjoint_tile_kernel<12, 9, 6, 1, 10, false, true>"],
                    3: ["adjoint_tile_kernel<12, 9, 6, 1, 6, false, false>", "adjoint_pair_kernel<12, 9>"]}
