@@ -589,7 +589,8 @@ void build_reverse_lists(Context& c, const PairView& pv, const ReverseLists& rl,
 
 // Starts the reverse pair lists of the current list on the side stream,
 // after the work already on the context stream, so that they are built
-// while the GEMMs run; backward_view makes the context stream wait for them.
+// while the representation stage runs; backward_view makes the context
+// stream wait for them.
 void prefetch_reverse_lists(Context& c) {
   if (c.rev_valid || c.n_atoms == 0) return;
   if (!c.side) {
