@@ -3094,7 +3094,9 @@ __global__ __launch_bounds__(kThreads) void sparsify_kernel(int64_t D, int64_t D
 }
 
 // dst[i] = S^T src[i] (rows of Dsel selected columns into rows of D
-// columns, repeated selections summed), one workgroup per row
+// columns, repeated selections summed), one workgroup per row (a slot takes
+// one value onto zero, or two, which sum the same in either order; three or
+// more copies of one column sum in arrival order)
 __global__ __launch_bounds__(kThreads) void scatter_columns_kernel(int64_t Dsel,
                                                                    const int64_t* __restrict__ col,
                                                                    const double* __restrict__ src,
