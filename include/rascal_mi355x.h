@@ -228,9 +228,8 @@ int rgpu_gram_f64(int64_t rows, int64_t cols, const double* A, double* out);
  * be NULL; gradient/neg_stress trigger the adjoint
  * (compute_sparse_kernel_gradients / _neg_stress,
  * sparse_kernel_predict.hh:206-377).  Every floating-point sum runs in a
- * fixed order: repeated calls on the same inputs return identical bits
- * (unless a coefficient_subselection names one coefficient three or more
- * times). */
+ * fixed order (no floating-point atomics): repeated calls on the same
+ * inputs return identical bits. */
 int rgpu_gap_predict(rgpu_context* ctx, double* energy, double* gradient, double* neg_stress);
 
 /* Allocation-free GAP energies: energy[n_struct] (device memory) = K_NM w,

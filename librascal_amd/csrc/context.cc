@@ -76,7 +76,14 @@ struct Context {
   // coefficient_subselection: full-layout column of each selected
   // coefficient (-1 for a species pair outside the batch's species list)
   std::vector<int64_t> sel_col;
-  DeviceBuffer<int64_t> d_sel_col;
+  DeviceBuffer<int64_t> d_sel_col, d_sel_inv_off, d_sel_inv_k;  // selection and its inverse
+  SelectionMap selection() const {
+    SelectionMap m;
+    m.col = d_sel_col.get();
+    m.inv_off = d_sel_inv_off.get();
+    m.inv_k = d_sel_inv_k.get();
+    return m;
+  }
 
   // representation
   bool have_rep = false;
@@ -152,7 +159,7 @@ struct Context {
   bool use_graphs = false;
   hipStream_t own_stream = nullptr;  // created for capture when no stream was given
   hipGraphExec_t rep_exec = nullptr;
-  std::array<uintptr_t, 27> rep_sig{};
+  std::array<uintptr_t, 29> rep_sig{};
   int64_t graph_captures = 0, graph_launches = 0;
   void drop_graph() {
     if (rep_exec) (void)hipGraphExecDestroy(rep_exec);
@@ -711,17 +718,17 @@ void compute_rep(Context& c) {
     if (fused) {
       // (the invariants came with the expansion: the selection only)
       if constexpr (!split)
-        if (xs) launch_sparsify(n, D, c.D(), c.d_sel_col.get(), pp, xs, n2, c.stream);
+        if (xs) launch_sparsify(n, D, c.D(), c.selection(), pp, xs, n2, c.stream);
       return;
     }
     if constexpr (split) {
       launch_power_spectrum_split(c.rd, S, n, cc, ph, pl, n2, c.stream, c.tile());
     } else {
       launch_power_spectrum<T>(c.rd, S, n, cc, pp, n2, c.stream, c.tile());
-      if (xs) launch_sparsify(n, D, c.D(), c.d_sel_col.get(), pp, xs, n2, c.stream);
+      if (xs) launch_sparsify(n, D, c.D(), c.selection(), pp, xs, n2, c.stream);
     }
   };
-  std::array<uintptr_t, 27> sig{};
+  std::array<uintptr_t, 29> sig{};
   bool replay = c.use_graphs && !c.timing && n > 0;
   if (replay) {
     sig = {
@@ -732,7 +739,7 @@ void compute_rep(Context& c) {
         uintptr_t(c.d_species_of.get()), uintptr_t(c.tile().pb_fwd), uintptr_t(c.tile().kc),
         uintptr_t(c.d_sel_col.get()), uintptr_t(c.tile().half_nl), uintptr_t(hl.src),
         uintptr_t(hl.rows), uintptr_t(c.tile().fused), uintptr_t(c.tile().fwd_lds),
-        uintptr_t(c.tile().fwd_group)};
+        uintptr_t(c.tile().fwd_group), uintptr_t(c.d_sel_inv_off.get()), uintptr_t(c.d_sel_inv_k.get())};
     // (instantiating a graph allocates: the allocation-free path launches
     // the stage eagerly instead of capturing it)
     if ((!c.rep_exec || sig != c.rep_sig) && g_no_alloc) replay = false;
@@ -915,7 +922,7 @@ void backward_device(Context& c, const T* G, double* grad, double* cvir, const d
   if constexpr (std::is_same<T, double>::value) {
     if (c.sparsified()) {
       double* gf = c.gfull.reserve(size_t(std::max<int64_t>(c.n_atoms * c.D_full(), 1)));
-      launch_scatter_columns(c.n_atoms, c.D(), c.d_sel_col.get(), G, c.D_full(), gf, c.stream);
+      launch_scatter_columns(c.n_atoms, c.D(), c.selection(), G, c.D_full(), gf, c.stream);
       G = gf;
     }
   }
@@ -1680,6 +1687,21 @@ int rgpu_set_structures(rgpu_context* ctx, int n_struct, const int64_t* atom_off
       int64_t* dsc = c.d_sel_col.reserve(scol.size());
       RGPU_HIP(hipMemcpyAsync(dsc, scol.data(), sizeof(int64_t) * scol.size(), hipMemcpyHostToDevice,
                               c.stream));
+      // inverse: the selections naming each full-layout column, ascending
+      const int64_t Df = c.D_full();
+      std::vector<int64_t> inv_off(size_t(Df + 1), 0), inv_k;
+      for (int64_t k : scol)
+        if (k >= 0) ++inv_off[size_t(k + 1)];
+      for (int64_t o = 0; o < Df; ++o) inv_off[size_t(o + 1)] += inv_off[size_t(o)];
+      inv_k.resize(size_t(std::max<int64_t>(inv_off[size_t(Df)], 1)));
+      std::vector<int64_t> fill(inv_off.begin(), inv_off.end() - 1);
+      for (size_t k = 0; k < scol.size(); ++k)
+        if (scol[k] >= 0) inv_k[size_t(fill[size_t(scol[k])]++)] = int64_t(k);
+      int64_t* dio = c.d_sel_inv_off.reserve(inv_off.size());
+      int64_t* dik = c.d_sel_inv_k.reserve(inv_k.size());
+      RGPU_HIP(hipStreamSynchronize(c.stream));  // no kernel still reads the old map
+      RGPU_HIP(hipMemcpy(dio, inv_off.data(), sizeof(int64_t) * inv_off.size(), hipMemcpyHostToDevice));
+      RGPU_HIP(hipMemcpy(dik, inv_k.data(), sizeof(int64_t) * inv_k.size(), hipMemcpyHostToDevice));
     }
     c.rd.sp_tab = nullptr;
     if (!sp_tab.empty()) {
@@ -2047,7 +2069,7 @@ int rgpu_gap_knm_gradient(rgpu_context* ctx, int neg_stress, double* out) {
         launch_knm_gradient_scale(n, d + m, M, c.d_species_of.get(), s, c.zeta, gs, c.stream);
         const double* row = c.T64.get() + m * D;
         if (grow) {
-          launch_scatter_columns(1, D, c.d_sel_col.get(), row, c.D_full(), grow, c.stream);
+          launch_scatter_columns(1, D, c.selection(), row, c.D_full(), grow, c.stream);
           row = grow;
         }
         launch_backward_rank1(c.rd, S, bpv, c.c64.get(), c.p64.get(), c.norm2.get(),

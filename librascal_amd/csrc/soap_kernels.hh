@@ -143,12 +143,22 @@ void launch_feature_gradient_rows(int N, int L1, int S, const PairView& pv, int6
                                   const double* c, const double* p_rows, const double* norm2,
                                   bool normalize, const double* dcp, const double* dcs, int64_t Dout,
                                   const int64_t* col, double* out, hipStream_t stream);
-// coefficient_subselection: xs[i][k] = p[i][col[k]], norm2[i] = |xs_i|^2,
-// then p[i] = S^T xs_i (the selected coefficients, zero elsewhere)
-void launch_sparsify(int64_t n, int64_t D, int64_t Dsel, const int64_t* col, double* p, double* xs,
+// coefficient_subselection as device arrays: col[k] (Dsel; -1 for a
+// coefficient outside the layout) and its inverse, the selections k naming
+// full-layout column o in ascending order at inv_k[inv_off[o] ..
+// inv_off[o+1])
+struct SelectionMap {
+  const int64_t* col = nullptr;
+  const int64_t* inv_off = nullptr;
+  const int64_t* inv_k = nullptr;
+};
+// xs[i][k] = p[i][col[k]], norm2[i] = |xs_i|^2, then p[i] = S^T xs_i (the
+// selected coefficients, zero elsewhere)
+void launch_sparsify(int64_t n, int64_t D, int64_t Dsel, const SelectionMap& sel, double* p, double* xs,
                      double* norm2, hipStream_t stream);
-// dst[i][:D] = S^T src[i][:Dsel] (repeated selections summed)
-void launch_scatter_columns(int64_t n, int64_t Dsel, const int64_t* col, const double* src, int64_t D,
+// dst[i][:D] = S^T src[i][:Dsel] (repeated selections summed in ascending
+// selection order)
+void launch_scatter_columns(int64_t n, int64_t Dsel, const SelectionMap& sel, const double* src, int64_t D,
                             double* dst, hipStream_t stream);
 
 // Adjoint (FP64): given G = dL/dX per centre (X the exported features), write

@@ -3066,6 +3066,8 @@ __global__ __launch_bounds__(kThreads) void feature_gradient_rows_kernel(
 // normalisation term is the one of the selected features.
 __global__ __launch_bounds__(kThreads) void sparsify_kernel(int64_t D, int64_t Dsel,
                                                             const int64_t* __restrict__ col,
+                                                            const int64_t* __restrict__ inv_off,
+                                                            const int64_t* __restrict__ inv_k,
                                                             double* __restrict__ p,
                                                             double* __restrict__ xs,
                                                             double* __restrict__ norm2) {
@@ -3081,32 +3083,36 @@ __global__ __launch_bounds__(kThreads) void sparsify_kernel(int64_t D, int64_t D
   __shared__ double red[kWaves];
   for (int off = 32; off > 0; off >>= 1) ss += __shfl_down(ss, off, 64);
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = ss;
-  __syncthreads();  // also: every selected value has been read
+  __syncthreads();  // also: every selected value has been read and xi written
   if (threadIdx.x == 0) {
     double t = 0;
     for (int w = 0; w < kWaves; ++w) t += red[w];
     norm2[i] = t;
   }
-  for (int64_t o = threadIdx.x; o < D; o += kThreads) pi[o] = 0.0;
-  __syncthreads();
-  for (int64_t k = threadIdx.x; k < Dsel; k += kThreads)
-    if (col[k] >= 0) unsafeAtomicAdd(pi + col[k], xi[k]);
+  // column o takes the selections naming it, in ascending selection order
+  for (int64_t o = threadIdx.x; o < D; o += kThreads) {
+    double t = 0.0;
+    for (int64_t q = inv_off[o]; q < inv_off[o + 1]; ++q) t += xi[inv_k[q]];
+    pi[o] = t;
+  }
 }
 
 // dst[i] = S^T src[i] (rows of Dsel selected columns into rows of D
-// columns, repeated selections summed), one workgroup per row (a slot takes
-// one value onto zero, or two, which sum the same in either order; three or
-// more copies of one column sum in arrival order)
+// columns, repeated selections summed in ascending selection order through
+// the inverse map inv_off/inv_k), one workgroup per row
 __global__ __launch_bounds__(kThreads) void scatter_columns_kernel(int64_t Dsel,
-                                                                   const int64_t* __restrict__ col,
+                                                                   const int64_t* __restrict__ inv_off,
+                                                                   const int64_t* __restrict__ inv_k,
                                                                    const double* __restrict__ src,
                                                                    int64_t D, double* __restrict__ dst) {
   const int64_t i = blockIdx.x;
   double* di = dst + i * D;
-  for (int64_t o = threadIdx.x; o < D; o += kThreads) di[o] = 0.0;
-  __syncthreads();
-  for (int64_t k = threadIdx.x; k < Dsel; k += kThreads)
-    if (col[k] >= 0) unsafeAtomicAdd(di + col[k], src[i * Dsel + k]);
+  const double* si = src + i * Dsel;
+  for (int64_t o = threadIdx.x; o < D; o += kThreads) {
+    double t = 0.0;
+    for (int64_t q = inv_off[o]; q < inv_off[o + 1]; ++q) t += si[inv_k[q]];
+    di[o] = t;
+  }
 }
 
 // LDS per workgroup: the forward and the power spectrum stay small enough
@@ -3673,19 +3679,19 @@ void launch_feature_gradient_rows(int N, int L1, int S, const PairView& pv, int6
   RGPU_LAUNCH_CHECK();
 }
 
-void launch_sparsify(int64_t n, int64_t D, int64_t Dsel, const int64_t* col, double* p, double* xs,
+void launch_sparsify(int64_t n, int64_t D, int64_t Dsel, const SelectionMap& sel, double* p, double* xs,
                      double* norm2, hipStream_t stream) {
   if (n == 0) return;
-  hipLaunchKernelGGL(sparsify_kernel, dim3(unsigned(n)), dim3(kThreads), 0, stream, D, Dsel, col, p, xs,
-                     norm2);
+  hipLaunchKernelGGL(sparsify_kernel, dim3(unsigned(n)), dim3(kThreads), 0, stream, D, Dsel, sel.col,
+                     sel.inv_off, sel.inv_k, p, xs, norm2);
   RGPU_LAUNCH_CHECK();
 }
 
-void launch_scatter_columns(int64_t n, int64_t Dsel, const int64_t* col, const double* src, int64_t D,
+void launch_scatter_columns(int64_t n, int64_t Dsel, const SelectionMap& sel, const double* src, int64_t D,
                             double* dst, hipStream_t stream) {
   if (n == 0) return;
-  hipLaunchKernelGGL(scatter_columns_kernel, dim3(unsigned(n)), dim3(kThreads), 0, stream, Dsel, col, src,
-                     D, dst);
+  hipLaunchKernelGGL(scatter_columns_kernel, dim3(unsigned(n)), dim3(kThreads), 0, stream, Dsel, sel.inv_off,
+                     sel.inv_k, src, D, dst);
   RGPU_LAUNCH_CHECK();
 }
 
