@@ -644,6 +644,21 @@ __device__ __forceinline__ void ps_body(int N_, int L_, int S, int KC, int64_t D
   const int64_t pbase = i * int64_t(K) * KB;
   const FastDiv fNT = fast_div(NT), fL1 = fast_div(L1), fT = fast_div(NT * NT);
   double ss = 0;
+  // packed diagonal value o of a key block: its offset in the full block,
+  // bit 30 set for an n1 < n2 entry (the same for every diagonal key; the
+  // four of this thread's first quad are kept in registers)
+  auto diag_src = [&](int o) {
+    const int t = o / fL1, l = o - t * L1;
+    const int n1 = tri[t] >> 8, n2 = tri[t] & 255;
+    return ((n1 * N + n2) * L1 + l) | (n1 < n2 ? 1 << 30 : 0);
+  };
+  int dsrc[4] = {0, 0, 0, 0};
+  if (SPLIT) {
+    const int KDp = packed_diag_block(N, L1);
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+      if (4 * int(threadIdx.x) + u < KDp) dsrc[u] = diag_src(4 * int(threadIdx.x) + u);
+  }
   for (int k0 = 0; k0 < K; k0 += KC) {
     const int kc = min(KC, K - k0);
     const int per_l = kc * NT * NT;
@@ -718,11 +733,13 @@ __device__ __forceinline__ void ps_body(int N_, int L_, int S, int KC, int64_t D
           } else {
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
-              const int o = 4 * q + u, t = o / fL1, l = o - t * L1;
-              const int n1 = tri[t] >> 8, n2 = tri[t] & 255;
-              v[u] = float(sk[(n1 * N + n2) * L1 + l]);
-              ss += (n1 < n2 ? 2.0 : 1.0) * double(v[u]) * double(v[u]);
-              if (n1 < n2) v[u] *= 1.41421356237309504880f;
+              // source of packed value 4 q + u in the key block (bit 30:
+              // an n1 < n2 entry); this thread's first quad from registers
+              int d = q == int(threadIdx.x) ? dsrc[u] : diag_src(4 * q + u);
+              const bool off = d >> 30;
+              v[u] = float(sk[d & ((1 << 30) - 1)]);
+              ss += (off ? 2.0 : 1.0) * double(v[u]) * double(v[u]);
+              if (off) v[u] *= 1.41421356237309504880f;
             }
           }
           unsigned hb[4], lb[4];
