@@ -3478,6 +3478,11 @@ void launch_power_spectrum_impl(const RadialDev& rad, int S, int64_t n, const T*
     allow_lds(kern, smem);
     hipLaunchKernelGGL(kern, dim3(unsigned(n)), dim3(kThreads), smem, stream, rad.N, rad.L, S, KC,
                        packed_row_stride(S, rad.N, rad.L1), c, p, p_hi, p_lo, norm2);
+  } else if (!p_hi && kFixedShape && rad.N == 12 && rad.L == 9) {
+    auto kern = power_spectrum_kernel<T, false, 12, 9>;
+    allow_lds(kern, smem);
+    hipLaunchKernelGGL(kern, dim3(unsigned(n)), dim3(kThreads), smem, stream, rad.N, rad.L, S, KC,
+                       packed_row_stride(S, rad.N, rad.L1), c, p, p_hi, p_lo, norm2);
   } else if (p_hi) {
     auto kern = power_spectrum_kernel<T, true>;
     allow_lds(kern, smem);
