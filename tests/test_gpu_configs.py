@@ -22,7 +22,8 @@ import pytest
 import oracle as O
 from conftest import expansion_rel, ps_rel, rel_err, soap_hypers
 from librascal_amd import _capi
-from librascal_amd.synthetic import diamond_cells, molecules, pack, periodic_box, triclinic_cells
+from librascal_amd.synthetic import (diamond_cells, molecules, pack, periodic_box, triclinic_cells,
+                                     wrap_positions)
 
 pytestmark = pytest.mark.gpu
 
@@ -425,6 +426,40 @@ def test_reductions_are_repeatable(precision):
         for a in range(off[s], off[s + 1]):
             t += k_atom[a]
         assert np.array_equal(k_struct[s], t)
+
+
+def test_caller_stream_with_graphs_matches_default_stream():
+    """A context on a caller's stream (a torch stream), with graph replay,
+    over two position updates: the reverse pair lists built on the context's
+    second stream are joined into the caller's stream, and every prediction
+    equals that of a context on the default stream, bit for bit."""
+    import torch
+    sts = triclinic_cells(n_cells=3, n_min=40, n_max=90, seed=45)
+    hp = soap_hypers(rc=4.0, sw=0.5, sigma=0.5, N=8, L=6, spline=1e-8)
+    a, sp = make_ctx(hp, sts, precision=_capi.MIXED)
+    b, _ = make_ctx(hp, sts, precision=_capi.MIXED)
+    stream = torch.cuda.Stream()
+    a.set_stream(stream.cuda_stream)
+    a.enable_graphs(True)
+    X, _ = b.features()
+    num = pack(sts)[2]
+    rng = np.random.default_rng(46)
+    spec = np.concatenate([np.full(40, t, np.int32) for t in sp])
+    vals = np.concatenate([X[rng.choice(np.where(num == t)[0], 40)] for t in sp])
+    w = rng.normal(size=len(spec))
+    a.set_sparse_points(2, spec, vals, w)
+    b.set_sparse_points(2, spec, vals, w)
+    off, pos0, _, cells, pbc = pack(sts)
+    for it in range(3):
+        pos = pos0 + (0.0 if it == 0 else 0.01) * rng.normal(size=pos0.shape)
+        for k in range(len(sts)):  # moved atoms wrapped back into their cells
+            seg = slice(off[k], off[k + 1])
+            pos[seg] = wrap_positions(pos[seg], cells[k], pbc[k])
+        a.update_positions(pos)
+        b.update_positions(pos)
+        for x, y in zip(a.gap_predict(), b.gap_predict()):
+            assert np.array_equal(x, y)
+    torch.cuda.synchronize()
 
 
 @pytest.mark.parametrize("precision", [_capi.FP64, _capi.MIXED])
