@@ -93,7 +93,10 @@ const char* rgpu_last_error(void);
 int rgpu_context_create(const char* hypers_json, rgpu_context** ctx);
 void rgpu_context_destroy(rgpu_context* ctx);
 
-/* HIP stream used by every later call (default: the null stream). */
+/* HIP stream used by every later call (default: the null stream).  A
+ * prediction with forces also runs the reverse pair lists of its force sum
+ * on a second stream owned by the context (default priority), joined into
+ * this stream by an event before the forces. */
 int rgpu_set_stream(rgpu_context* ctx, void* hip_stream);
 /* Replay the representation stage (cutoff, expansion, invariants) of
  * rgpu_compute as a captured HIP graph while the batch's buffers and sizes
